@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Throughput of the batched IR-drop solve on one MI355X.
+
+Workload (one "step"): one batched solve of 4096 independent 128x128
+crossbars -- 4 cell states {2e6, 1.6e6, 1.3e6, 1e6} ohm drawn uniformly, wire
+resistance ~ U[1, 5] ohm per crossbar, wordline inputs in {0, 0.2} V with
+p = 0.5, bitlines grounded -- to <= 1e-6 relative bitline-current error
+(FP32 red-black SOR sweeps, FP64 residual test / refinement / readout).
+Inputs are synthetic, seeded, and resident in HBM before the timed region.
+
+Reported: whole-job solves/s of the plain (zero-drop initial guess) solver
+and, when a trained warm-start network is present
+(fadesim_amd/warmstart/weights), of the NN-warm-started solver with the
+network's inference inside the timed step; mean sweeps of both; the roofline
+of the solve kernel from HIP events on its launch stream; the CPU baseline
+(the oracle's restatement of the reference's lexicographic solver, OpenMP
+over instances) on a bounded sample, and the measured error of a GPU sample
+against the exact FP64 oracle (checker of the cpu_baseline leg).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+STATES4 = np.array([2e6, 1.6e6, 1.3e6, 1e6])
+M = N = 128
+
+
+def make_batch(B, seed):
+    rng = np.random.default_rng(seed)
+    st = rng.integers(0, 4, (B, M, N))
+    g = (1.0 / STATES4[st]).astype(np.float32)
+    r = rng.uniform(1.0, 5.0, B)
+    vin = np.where(rng.random((B, M)) < 0.5, 0.2, 0.0)
+    return g, r, vin
+
+
+def algorithmic_bytes(sweeps_sum, B):
+    """Streaming-schedule bytes: 4 B of conductance per cell per sweep, plus
+    the inputs and the FP64 readout (4*M + 8*N per crossbar)."""
+    return 4.0 * M * N * sweeps_sum + B * (4.0 * M + 8.0 * N)
+
+
+def load_traffic():
+    p = os.path.join(REPO, "profiles", "rbsor_128x128_b4096_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(sample_B, seed):
+    """The oracle's restatement of the reference's lexicographic solver
+    (rule of the reference: break at max|dv| < 1e-9, w = 1.97 -- the first
+    decade that meets 1e-6) on a bounded sample, OpenMP over instances; and
+    the exact FP64 oracle answers used to check the GPU's error."""
+    import oracle
+    g, r, vin = make_batch(sample_B, seed)
+    t0 = time.perf_counter()
+    iout, sweeps, threads = oracle.nodegs_batch(g, r, vin, 100000, 1.97, 1e-9)
+    dt = time.perf_counter() - t0
+    iex = oracle.exact_batch(g, r, vin)
+    err = float(np.max(np.abs(iout - iex) / np.abs(iex)))
+    return {"value": sample_B / dt, "unit": "solves/s", "cores": int(threads), "kind": "port",
+            "sample": f"{sample_B} instances of the same 128x128 distribution (seed {seed}), "
+                      f"lexicographic SOR w=1.97 break max|dv|<1e-9, mean {sweeps.mean():.0f} sweeps, "
+                      f"{dt:.2f} s wall, max rel I_out err vs exact FP64 {err:.2e}"}, (g, r, vin, iex)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the solves shard "
+              "without exchange across GPUs; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fadesim_amd.solver import last_kernel_ms, solve_batch
+    from fadesim_amd import warmstart
+
+    dev = torch.device("cuda:0")
+    B = args.batch
+    g, r, vin = make_batch(B, seed=2024)
+    tg = torch.tensor(g, device=dev)
+    tr = torch.tensor(r, device=dev)
+    tv = torch.tensor(vin, device=dev)
+    net = warmstart.load_default(dev)
+
+    def step_plain(timing=False):
+        return solve_batch(tg, tr, tv, tol=args.tol, timing=timing)
+
+    def step_warm(timing=False):
+        w = net.predict(tg, tr, tv)
+        return solve_batch(tg, tr, tv, warm=w, tol=args.tol, timing=timing)
+
+    def run(step):
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        kms = []
+        sweeps = []
+        res = None
+        for _ in range(args.steps):
+            res = step(timing=True)
+            kms.append(last_kernel_ms())
+            sweeps.append(res.sweeps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sw = torch.stack(sweeps).to(torch.float64)
+        return dt, kms, sw, res
+
+    dt_p, kms_p, sw_p, res_p = run(step_plain)
+    assert int((res_p.status != 0).sum()) == 0, "plain solve did not converge"
+    plain_sps = B * args.steps / dt_p
+    out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU"}
+    warm = None
+    if net is not None:
+        dt_w, kms_w, sw_w, res_w = run(step_warm)
+        assert int((res_w.status != 0).sum()) == 0, "warm-started solve did not converge"
+        warm = {"solves_per_s": B * args.steps / dt_w, "ms_per_step": 1e3 * dt_w / args.steps,
+                "mean_sweeps": float(sw_w.mean()), "kernel_ms": float(np.mean(kms_w)),
+                "network": net.describe()}
+    value = warm["solves_per_s"] if warm else plain_sps
+    dt_main = dt_w if warm else dt_p
+    kms_main = kms_w if warm else kms_p
+    sw_main = sw_w if warm else sw_p
+    kernel_ms = float(np.mean(kms_main))
+    ach = algorithmic_bytes(float(sw_main.sum()) / args.steps, B) / (kernel_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    out.update({
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt_main / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{B} x 128x128 crossbars, 4 states {{2e6,1.6e6,1.3e6,1e6}} ohm, "
+                               "r ~ U[1,5] ohm, vin in {0,0.2} V, to 1e-6 rel I_out err",
+                   "batch": B, "rows": M, "cols": N, "tol": args.tol,
+                   "initial_guess": "nn-warm-start" if warm else "zero-drop"},
+        "plain": {"solves_per_s": plain_sps, "ms_per_step": 1e3 * dt_p / args.steps,
+                  "mean_sweeps": float(sw_p.mean()), "kernel_ms": float(np.mean(kms_p))},
+        "warm_start": warm,
+        "sweep_reduction": (float(sw_p.mean()) / warm["mean_sweeps"]) if warm else None,
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS,
+                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                     "kernel": "rbsor_kernel<Geo<4,8,32,16>>", "kernel_ms": kernel_ms,
+                     "bytes_model": "sweeps*4*M*N + 4*M + 8*N per crossbar (conductance streamed per sweep)"},
+    })
+    if not args.no_cpu:
+        cpu, (gs, rs, vs, iex) = cpu_baseline(args.cpu_sample, seed=77)
+        out["cpu_baseline"] = cpu
+        # error of the GPU on the same sample, against the exact FP64 oracle
+        rp = solve_batch(torch.tensor(gs, device=dev), torch.tensor(rs, device=dev),
+                         torch.tensor(vs, device=dev), tol=args.tol)
+        out["max_rel_err_plain"] = float(np.max(np.abs(rp.iout.cpu().numpy() - iex) / np.abs(iex)))
+        if net is not None:
+            tgs, trs, tvs = (torch.tensor(a, device=dev) for a in (gs, rs, vs))
+            rw = solve_batch(tgs, trs, tvs, warm=net.predict(tgs, trs, tvs), tol=args.tol)
+            out["max_rel_err_warm"] = float(np.max(np.abs(rw.iout.cpu().numpy() - iex) / np.abs(iex)))
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,104 @@
+"""ctypes binding of the C ABI in include/fadesim.h (libfadesim.so).
+
+The library is built in-tree by ``fadesim_amd.build``; nothing here falls
+back to another implementation: if the library is missing or a call fails,
+a RuntimeError says so.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libfadesim.so")
+
+FADESIM_OK = 0
+ST_CONVERGED, ST_MAXIT, ST_NONFINITE = 0, 1, 2
+F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING = 1, 2, 4, 8
+
+_vp = ctypes.c_void_p
+
+
+class BatchDesc(ctypes.Structure):
+    """Mirror of ``fadesim_batch_desc``."""
+    _fields_ = [
+        ("batch", ctypes.c_int), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
+        ("d_g", _vp), ("d_r_wire", _vp), ("d_vin", _vp), ("d_vout", _vp), ("d_warm", _vp),
+        ("omega", ctypes.c_double), ("tol", ctypes.c_double), ("safety", ctypes.c_double),
+        ("switch_tol", ctypes.c_double), ("max_sweeps", ctypes.c_int),
+        ("flags", ctypes.c_uint),
+        ("d_iout", _vp), ("d_sweeps", _vp), ("d_status", _vp), ("d_est", _vp), ("d_field", _vp),
+    ]
+
+
+class Gs64Desc(ctypes.Structure):
+    """Mirror of ``fadesim_gs64_desc``."""
+    _fields_ = [
+        ("rows", ctypes.c_int), ("cols", ctypes.c_int),
+        ("d_g", _vp), ("d_ru", _vp), ("ru_ld", ctypes.c_int), ("d_rd", _vp), ("rd_ld", ctypes.c_int),
+        ("d_vin", _vp), ("d_vout", _vp),
+        ("iter_number", ctypes.c_int), ("omega", ctypes.c_double),
+        ("enable_break", ctypes.c_int), ("tol", ctypes.c_double),
+        ("d_vup", _vp), ("d_vdown", _vp), ("d_iout", _vp), ("d_record", _vp), ("d_sweeps", _vp),
+    ]
+
+
+class HookDesc(ctypes.Structure):
+    """Mirror of ``fadesim_hook_desc``."""
+    _fields_ = [
+        ("dtype_bytes", ctypes.c_int),
+        ("d_input", _vp), ("d_G", _vp),
+        ("batch_size", ctypes.c_int), ("row_size", ctypes.c_int), ("col_size", ctypes.c_int),
+        ("numX", ctypes.c_int), ("numY", ctypes.c_int), ("iter_times", ctypes.c_int),
+        ("g_wire", ctypes.c_double), ("beta", ctypes.c_double),
+        ("enable_break", ctypes.c_int), ("break_threshold", ctypes.c_double),
+        ("nonlinear", ctypes.c_int), ("nonlinear_params", ctypes.c_double * 6),
+        ("ret_v", ctypes.c_int), ("d_out", _vp), ("d_sweeps", _vp),
+    ]
+
+
+# Every symbol include/fadesim.h declares.
+EXPORTED = [
+    "fadesim_last_error", "fadesim_version", "fadesim_solve_batch", "fadesim_default_omega",
+    "fadesim_last_kernel_ms",
+    "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs",
+]
+
+_LIB = None
+
+
+def lib():
+    """Load libfadesim.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m fadesim_amd.build` "
+                "(there is no fallback implementation)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.fadesim_last_error.restype = ctypes.c_char_p
+        L.fadesim_version.restype = ctypes.c_int
+        L.fadesim_solve_batch.argtypes = [ctypes.POINTER(BatchDesc), _vp]
+        L.fadesim_solve_batch.restype = ctypes.c_int
+        L.fadesim_last_kernel_ms.restype = ctypes.c_double
+        L.fadesim_default_omega.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.fadesim_default_omega.restype = ctypes.c_double
+        if hasattr(L, "fadesim_gs64_solve"):
+            L.fadesim_gs64_solve.argtypes = [ctypes.POINTER(Gs64Desc), _vp]
+            L.fadesim_gs64_solve.restype = ctypes.c_int
+        if hasattr(L, "fadesim_hook_ir_drop_gs"):
+            L.fadesim_hook_ir_drop_gs.argtypes = [ctypes.POINTER(HookDesc), _vp]
+            L.fadesim_hook_ir_drop_gs.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int, what: str) -> None:
+    if rc != FADESIM_OK:
+        msg = lib().fadesim_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def stream_handle(device=None) -> int:
+    """hipStream_t of torch's current stream on `device`."""
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,174 @@
+// FP64 red-black SOR for one crossbar with per-segment wire resistances and
+// the reference stopping rule -- the solver behind the `sim` CLI's
+// nodebasedGSMethod command.
+//
+// Same network and update as NodebasedGSMethod::nodeBaseGS
+// (crossbarHspice/V2/concrete_command.h:1218-1273) in absolute node voltages,
+// with wordline segments read from RU (array2D::nodeBasedGS,
+// crossbarHspice/2dArray.cc:1562-1588 -- V2 reads them from RDLine, which is
+// the same for square arrays of one wire resistance).  Red-black order:
+// colour of u_ij is (i+j)&1, of d_ij (i+j+1)&1, colour 0 first.  Node update
+// without fused multiply-adds (built with -ffp-contract=off):
+//   u: num = (aL*L + aR*R) + g*d,  u' = wD*num + (1-w)*u,  wD = w/((aL+aR)+g)
+//   d: num = (bU*U + bD*Dn) + g*u, d' = wD*num + (1-w)*d,  wD = w/((bU+bD)+g)
+// Stop after the first sweep whose max |x' - x| < tol (concrete_command.h:1256-1272).
+// The readout sums (u-d)*g over ascending rows per column, like the reference.
+//
+// One workgroup; fields and coefficients live in the workspace (L2 resident
+// for the array sizes the CLI handles).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "common.h"
+#include "fadesim.h"
+
+namespace fadesim {
+namespace {
+
+constexpr int kGsThreads = 1024;
+
+struct GsArgs {
+    int n, m;
+    const double *g, *ru, *rd;
+    int ru_ld, rd_ld;
+    const double *vin, *vout;
+    int iter_number;
+    double w;
+    int enable_break;
+    double tol;
+    double *u, *d, *coef; // workspace: u, d [n*m]; coef [6][n*m]
+    double *vup, *vdown, *iout;
+    int *record, *sweeps;
+};
+
+__global__ void __launch_bounds__(kGsThreads) gs64_kernel(GsArgs A)
+{
+    const int n = A.n, m = A.m, t = threadIdx.x;
+    const size_t nm = (size_t)n * m;
+    double *aL = A.coef, *aR = A.coef + nm, *bU = A.coef + 2 * nm, *bD = A.coef + 3 * nm;
+    double *wDu = A.coef + 4 * nm, *wDd = A.coef + 5 * nm;
+    double *u = A.u, *d = A.d;
+    __shared__ double wred[kGsThreads / 64];
+
+    for (size_t k = t; k < nm; k += kGsThreads) {
+        const int i = (int)(k / m), j = (int)(k % m);
+        aL[k] = 1.0 / A.ru[(size_t)i * A.ru_ld + j];
+        aR[k] = (j < m - 1) ? 1.0 / A.ru[(size_t)i * A.ru_ld + j + 1] : 0.0;
+        bU[k] = (i > 0) ? 1.0 / A.rd[(size_t)j * A.rd_ld + i] : 0.0;
+        bD[k] = 1.0 / A.rd[(size_t)j * A.rd_ld + i + 1];
+        wDu[k] = A.w / ((aL[k] + aR[k]) + A.g[k]);
+        wDd[k] = A.w / ((bU[k] + bD[k]) + A.g[k]);
+        u[k] = A.vin[i];
+        d[k] = A.vout[j];
+    }
+    __syncthreads();
+    const double omw = 1.0 - A.w;
+    int record = -1, it;
+    for (it = 0; it < A.iter_number; ++it) {
+        double mx = 0.0;
+        for (int c = 0; c < 2; ++c) {
+            for (size_t k = t; k < nm; k += kGsThreads) {
+                const int i = (int)(k / m), j = (int)(k % m);
+                if (((i + j) & 1) == c) {
+                    const double L = j ? u[k - 1] : A.vin[i];
+                    const double R = (j < m - 1) ? u[k + 1] : 0.0;
+                    const double num = (aL[k] * L + aR[k] * R) + A.g[k] * d[k];
+                    const double nv = wDu[k] * num + omw * u[k];
+                    mx = fmax(mx, fabs(nv - u[k]));
+                    u[k] = nv;
+                } else {
+                    const double U = i ? d[k - m] : 0.0;
+                    const double Dn = (i < n - 1) ? d[k + m] : A.vout[j];
+                    const double num = (bU[k] * U + bD[k] * Dn) + A.g[k] * u[k];
+                    const double nv = wDd[k] * num + omw * d[k];
+                    mx = fmax(mx, fabs(nv - d[k]));
+                    d[k] = nv;
+                }
+            }
+            __syncthreads();
+        }
+        if (A.enable_break) {
+            for (int off = 32; off > 0; off >>= 1)
+                mx = fmax(mx, __shfl_xor(mx, off, 64));
+            if ((t & 63) == 0)
+                wred[t >> 6] = mx;
+            __syncthreads();
+            double all = 0.0;
+            for (int w = 0; w < kGsThreads / 64; ++w)
+                all = fmax(all, wred[w]);
+            __syncthreads(); // wred is rewritten next sweep
+            if (all < A.tol) {
+                record = it;
+                ++it;
+                break;
+            }
+        }
+    }
+    for (size_t k = t; k < nm; k += kGsThreads) {
+        A.vup[k] = u[k];
+        A.vdown[k] = d[k];
+    }
+    for (int j = t; j < m; j += kGsThreads) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const size_t k = (size_t)i * m + j;
+            s += (u[k] - d[k]) * A.g[k];
+        }
+        A.iout[j] = s;
+    }
+    if (t == 0) {
+        *A.record = record;
+        *A.sweeps = it;
+    }
+}
+
+} // namespace
+} // namespace fadesim
+
+using namespace fadesim;
+
+extern "C" int fadesim_gs64_solve(const fadesim_gs64_desc *d, void *stream)
+{
+    if (!d)
+        return set_error(FADESIM_E_INVALID, "null descriptor");
+    if (!(d->rows >= 2 && d->cols >= 2))
+        return set_error(FADESIM_E_INVALID, "m and n should greater than 2");
+    if (!d->d_g || !d->d_ru || !d->d_rd || !d->d_vin || !d->d_vout || !d->d_vup || !d->d_vdown ||
+        !d->d_iout || !d->d_record || !d->d_sweeps)
+        return set_error(FADESIM_E_INVALID, "all device pointers are required");
+    if (d->ru_ld < d->cols + 1 || d->rd_ld < d->rows + 1)
+        return set_error(FADESIM_E_INVALID, "ru_ld must be >= cols+1 and rd_ld >= rows+1");
+    if (d->iter_number < 0)
+        return set_error(FADESIM_E_INVALID, "iter_number must be >= 0");
+    const size_t nm = (size_t)d->rows * d->cols;
+    double *ws = (double *)workspace(8 * nm * sizeof(double));
+    if (!ws)
+        return FADESIM_E_NOMEM;
+    GsArgs a;
+    a.n = d->rows;
+    a.m = d->cols;
+    a.g = d->d_g;
+    a.ru = d->d_ru;
+    a.rd = d->d_rd;
+    a.ru_ld = d->ru_ld;
+    a.rd_ld = d->rd_ld;
+    a.vin = d->d_vin;
+    a.vout = d->d_vout;
+    a.iter_number = d->iter_number;
+    a.w = d->omega;
+    a.enable_break = d->enable_break;
+    a.tol = d->tol;
+    a.u = ws;
+    a.d = ws + nm;
+    a.coef = ws + 2 * nm;
+    a.vup = d->d_vup;
+    a.vdown = d->d_vdown;
+    a.iout = d->d_iout;
+    a.record = d->d_record;
+    a.sweeps = d->d_sweeps;
+    hipLaunchKernelGGL(gs64_kernel, dim3(1), dim3(kGsThreads), 0, (hipStream_t)stream, a);
+    FADESIM_HIP_CHECK(hipGetLastError());
+    return FADESIM_OK;
+}

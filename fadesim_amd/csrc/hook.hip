@@ -1,0 +1,309 @@
+// The PimTorch ir_drop_gs hook with the reference's own iteration, for
+// callers that need its exact (often unconverged) results: line
+// Gauss-Seidel/SOR, all wordlines left->right then all bitlines bottom->top
+// each sweep (pimtorch/.../SplitArrCuda/split_arr_matmul.cu:27-119), readout
+// sum_i (u - d) G (:121-140), sinh-nonlinear cells by the lagged Picard
+// update (:142-196, 241-262).
+//
+// One workgroup per crossbar (b, x, y).  The fields are updated in place:
+// walking a wordline left->right, the left neighbour is already new and the
+// right one still old -- exactly the reference's v_up / v_up_prev pairing --
+// and likewise bottom->top on a bitline.  Fields and the nonlinear
+// conductances live in the workspace (L2); the wordline phase keeps its
+// running left value and the bitline phase its running lower value in
+// registers.  Arithmetic follows the reference expression by expression
+// (-ffp-contract=off); only sinh differs in the last bits between devices.
+//
+// Nonlinear: sweep k uses G computed from the state two sweeps back (the
+// reference computes it from the buffer it is about to overwrite), so each
+// sweep first computes the next sweep's G from the current state.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "common.h"
+#include "fadesim.h"
+
+namespace fadesim {
+namespace {
+
+constexpr int kHookThreads = 256;
+
+template <typename T>
+struct HookArgs {
+    const T *input;
+    const T *G;
+    int bsize, row, col, numX, numY;
+    int sweeps_this_launch;
+    int first_sweep; // 0 on the first launch: initialise the fields
+    T gw, be, omb, thr;
+    int nonlinear;
+    double nl[6];
+    T *u, *d;         // [grid][row][col]
+    T *Gc, *Gn;       // nonlinear conductances of this and the next sweep
+    unsigned *moved;  // break mode: set when any node moved by more than thr
+    int want_moved;
+};
+
+template <typename T> __device__ __forceinline__ T dev_sinh(T x);
+template <> __device__ __forceinline__ float dev_sinh<float>(float x) { return sinhf(x); }
+template <> __device__ __forceinline__ double dev_sinh<double>(double x) { return sinh(x); }
+template <typename T> __device__ __forceinline__ T dev_abs(T x) { return x < (T)0 ? -x : x; }
+
+// Nonlinear cell conductance of split_arr_matmul.cu:142-196 for voltage V.
+template <typename T>
+__device__ __forceinline__ T nonlinear_g(T V, T z, const double *nl)
+{
+    if (V < (T)1e-5 && V > -(T)1e-5)
+        V = V > (T)0 ? (T)1e-5 : -(T)1e-5;
+    V = dev_abs(V);
+    const T Ion = (T)nl[3] * dev_sinh<T>(V * (T)nl[2]);
+    const T Ioff = (T)nl[5] * dev_sinh<T>(V * (T)nl[4]);
+    const T R = ((T)1 - z) * V / Ion + z * V / Ioff;
+    return (T)1 / R;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
+{
+    const int xb = blockIdx.x; // (b, x, y) flattened
+    const int b = xb / (A.numX * A.numY);
+    const int x = (xb / A.numY) % A.numX;
+    const int y = xb % A.numY;
+    const int row = A.row, col = A.col, t = threadIdx.x;
+    const size_t cells = (size_t)row * col;
+    T *u = A.u + (size_t)xb * cells;
+    T *d = A.d + (size_t)xb * cells;
+    const T *in = A.input + ((size_t)b * A.numX + x) * row;
+    // linear G of cell (i, c): G[x][i][y*col + c]
+    auto Glin = [&](int i, int c) -> T { return A.G[((size_t)x * row + i) * ((size_t)A.numY * col) + (size_t)y * col + c]; };
+    T *Gc = A.nonlinear ? A.Gc + (size_t)xb * cells : nullptr;
+    T *Gn = A.nonlinear ? A.Gn + (size_t)xb * cells : nullptr;
+    const T gw = A.gw, be = A.be, omb = A.omb;
+
+    if (A.first_sweep) {
+        for (size_t k = t; k < cells; k += kHookThreads) {
+            u[k] = in[k / col];
+            d[k] = (T)0;
+        }
+        __syncthreads();
+        if (A.nonlinear) {
+            // G of sweeps 0 and 1 both come from the initial state
+            for (size_t k = t; k < cells; k += kHookThreads) {
+                const int i = (int)(k / col), c = (int)(k % col);
+                const T gl = Glin(i, c);
+                T z = ((T)1 / gl - (T)A.nl[0]) / (T)(A.nl[1] - A.nl[0]);
+                if (z < (T)0)
+                    z = (T)0;
+                Gc[k] = Gn[k] = nonlinear_g<T>(u[k] - d[k], z, A.nl);
+            }
+            __syncthreads();
+        }
+    }
+    bool moved = false;
+    for (int s = 0; s < A.sweeps_this_launch; ++s) {
+        if (A.nonlinear && !(A.first_sweep && s == 0)) {
+            // Gc <- Gn (computed last sweep); Gn <- G of the current state
+            for (size_t k = t; k < cells; k += kHookThreads) {
+                const int i = (int)(k / col), c = (int)(k % col);
+                const T gl = Glin(i, c);
+                T z = ((T)1 / gl - (T)A.nl[0]) / (T)(A.nl[1] - A.nl[0]);
+                if (z < (T)0)
+                    z = (T)0;
+                Gc[k] = Gn[k];
+                Gn[k] = nonlinear_g<T>(u[k] - d[k], z, A.nl);
+            }
+            __syncthreads();
+        }
+        auto Gof = [&](int i, int c) -> T { return A.nonlinear ? Gc[(size_t)i * col + c] : Glin(i, c); };
+        // wordlines, left -> right (split_arr_matmul.cu:75-119)
+        for (int i = t; i < row; i += kHookThreads) {
+            T *ur = u + (size_t)i * col;
+            const T *dr = d + (size_t)i * col;
+            T left = in[i];
+            T old = ur[0];
+            T G0 = Gof(i, 0);
+            T sumg = gw + gw + G0;
+            T nv = be * (gw * left + gw * ur[1] + G0 * dr[0]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > A.thr;
+            ur[0] = nv;
+            left = nv;
+            int j;
+            for (j = 1; j < col - 1; ++j) {
+                old = ur[j];
+                const T Gj = Gof(i, j);
+                sumg = gw * (T)2 + Gj;
+                nv = be * (gw * (left + ur[j + 1]) + Gj * dr[j]) / sumg + omb * old;
+                moved |= dev_abs(nv - old) > A.thr;
+                ur[j] = nv;
+                left = nv;
+            }
+            old = ur[j];
+            const T Gj = Gof(i, j);
+            sumg = gw + Gj;
+            nv = be * (gw * left + Gj * dr[j]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > A.thr;
+            ur[j] = nv;
+        }
+        __syncthreads();
+        // bitlines, bottom -> top (split_arr_matmul.cu:27-73)
+        for (int c = t; c < col; c += kHookThreads) {
+            int i = row - 1;
+            T old = d[(size_t)i * col + c];
+            T Gi = Gof(i, c);
+            T sumg = gw + gw + Gi;
+            T nv = be * (gw * d[(size_t)(i - 1) * col + c] + gw * (T)0 + Gi * u[(size_t)i * col + c]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > A.thr;
+            d[(size_t)i * col + c] = nv;
+            T down = nv;
+            for (i = row - 2; i > 0; --i) {
+                old = d[(size_t)i * col + c];
+                Gi = Gof(i, c);
+                sumg = gw * (T)2 + Gi;
+                nv = be * (gw * (d[(size_t)(i - 1) * col + c] + down) + Gi * u[(size_t)i * col + c]) / sumg + omb * old;
+                moved |= dev_abs(nv - old) > A.thr;
+                d[(size_t)i * col + c] = nv;
+                down = nv;
+            }
+            old = d[c];
+            Gi = Gof(0, c);
+            sumg = gw + Gi;
+            nv = be * (gw * down + Gi * u[c]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > A.thr;
+            d[c] = nv;
+        }
+        __syncthreads();
+    }
+    if (A.want_moved && moved)
+        atomicOr(A.moved, 1u);
+}
+
+// Readout: out[b][x][y*col + c] = sum_i (u - d) G, or the voltage differences.
+template <typename T>
+__global__ void hook_readout(HookArgs<T> A, T *out, int ret_v)
+{
+    const int xb = blockIdx.x;
+    const int b = xb / (A.numX * A.numY);
+    const int x = (xb / A.numY) % A.numX;
+    const int y = xb % A.numY;
+    const int row = A.row, col = A.col;
+    const size_t cells = (size_t)row * col;
+    const T *u = A.u + (size_t)xb * cells;
+    const T *d = A.d + (size_t)xb * cells;
+    if (ret_v) {
+        for (size_t k = threadIdx.x; k < cells; k += blockDim.x)
+            out[(size_t)xb * cells + k] = u[k] - d[k];
+        return;
+    }
+    for (int c = threadIdx.x; c < col; c += blockDim.x) {
+        T s = (T)0;
+        for (int i = 0; i < row; ++i) {
+            const size_t k = (size_t)i * col + c;
+            const T G = A.nonlinear ? A.Gc[(size_t)xb * cells + k]
+                                    : A.G[((size_t)x * row + i) * ((size_t)A.numY * col) + (size_t)y * col + c];
+            s += (u[k] - d[k]) * G;
+        }
+        out[(((size_t)b * A.numX + x) * A.numY + y) * col + c] = s;
+    }
+}
+
+__global__ void set_u32(unsigned *p, unsigned v) { *p = v; }
+__global__ void set_i32(int *p, int v) { *p = v; }
+
+template <typename T>
+int run_hook(const fadesim_hook_desc *D, hipStream_t s)
+{
+    const int grid = D->batch_size * D->numX * D->numY;
+    const size_t cells = (size_t)D->row_size * D->col_size;
+    const size_t nfield = (size_t)grid * cells;
+    const size_t planes = D->nonlinear ? 4 : 2;
+    char *ws = (char *)workspace(planes * nfield * sizeof(T) + 256);
+    if (!ws)
+        return FADESIM_E_NOMEM;
+    HookArgs<T> A;
+    A.input = (const T *)D->d_input;
+    A.G = (const T *)D->d_G;
+    A.bsize = D->batch_size;
+    A.row = D->row_size;
+    A.col = D->col_size;
+    A.numX = D->numX;
+    A.numY = D->numY;
+    A.gw = (T)D->g_wire;
+    A.be = (T)D->beta;
+    A.omb = (T)1 - A.be;
+    A.thr = (T)D->break_threshold;
+    A.nonlinear = D->nonlinear;
+    for (int k = 0; k < 6; ++k)
+        A.nl[k] = D->nonlinear_params[k];
+    A.u = (T *)ws;
+    A.d = A.u + nfield;
+    A.Gc = D->nonlinear ? A.d + nfield : nullptr;
+    A.Gn = D->nonlinear ? A.Gc + nfield : nullptr;
+    A.moved = (unsigned *)(ws + planes * nfield * sizeof(T));
+    int done = 0;
+    if (!D->enable_break) {
+        A.sweeps_this_launch = D->iter_times;
+        A.first_sweep = 1;
+        A.want_moved = 0;
+        hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+        FADESIM_HIP_CHECK(hipGetLastError());
+        done = D->iter_times;
+    } else {
+        // one sweep per launch and a host check of the "moved" flag, as the
+        // reference's host loop does (split_arr_matmul.cu:295-306)
+        A.sweeps_this_launch = 1;
+        A.want_moved = 1;
+        for (int it = 0; it < D->iter_times; ++it) {
+            A.first_sweep = (it == 0);
+            hipLaunchKernelGGL(set_u32, dim3(1), dim3(1), 0, s, A.moved, 0u);
+            hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+            FADESIM_HIP_CHECK(hipGetLastError());
+            unsigned h = 0;
+            FADESIM_HIP_CHECK(hipMemcpyAsync(&h, A.moved, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            FADESIM_HIP_CHECK(hipStreamSynchronize(s));
+            ++done;
+            if (!h)
+                break;
+        }
+    }
+    if (done == 0) {
+        // no sweep: the fields still need their initial state for the readout
+        A.sweeps_this_launch = 0;
+        A.first_sweep = 1;
+        A.want_moved = 0;
+        hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+        FADESIM_HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL((hook_readout<T>), dim3(grid), dim3(128), 0, s, A, (T *)D->d_out, D->ret_v);
+    FADESIM_HIP_CHECK(hipGetLastError());
+    if (D->d_sweeps) {
+        hipLaunchKernelGGL(set_i32, dim3(1), dim3(1), 0, s, D->d_sweeps, done);
+        FADESIM_HIP_CHECK(hipGetLastError());
+    }
+    return FADESIM_OK;
+}
+
+} // namespace
+} // namespace fadesim
+
+using namespace fadesim;
+
+extern "C" int fadesim_hook_ir_drop_gs(const fadesim_hook_desc *d, void *stream)
+{
+    if (!d)
+        return set_error(FADESIM_E_INVALID, "null descriptor");
+    if (d->row_size < 2)
+        return set_error(FADESIM_E_INVALID, "row_size should be >=2");
+    if (d->col_size < 2)
+        return set_error(FADESIM_E_INVALID, "col_size should be >=2");
+    if (d->batch_size < 1 || d->numX < 1 || d->numY < 1 || d->iter_times < 0)
+        return set_error(FADESIM_E_INVALID, "batch_size, numX, numY must be >= 1 and iter_times >= 0");
+    if (!d->d_input || !d->d_G || !d->d_out)
+        return set_error(FADESIM_E_INVALID, "input, G and out are required");
+    if (d->dtype_bytes == 4)
+        return run_hook<float>(d, (hipStream_t)stream);
+    if (d->dtype_bytes == 8)
+        return run_hook<double>(d, (hipStream_t)stream);
+    return set_error(FADESIM_E_INVALID, "dtype_bytes must be 4 (float) or 8 (double)");
+}

@@ -1,0 +1,518 @@
+// `sim <config>` -- the crossbar command language of the FADESim CLI, with
+// nodebasedGSMethod solved on the GPU through fadesim_gs64_solve.
+//
+// Command semantics follow the reference CLI (V2 registry
+// crossbarHspice/V2/main.cpp:68-117, command bodies
+// crossbarHspice/V2/concrete_command.h:51-1127; V1 spellings of
+// crossbarHspice/2dArray.cc:24-873 are accepted too): one command per line,
+// whitespace-separated arguments, `//`, `%` and `#` comment lines.  Output
+// files keep the reference formats: fastmode_nodebasedGS.out
+// (concrete_command.h:1321-1353) and the operation log default.log
+// (crossbarHspice/V2/main.cpp:61-66, abstract_array.h:17-29).  Like the
+// reference, an error prints "Error(s) occured in line N: ..." and the log,
+// and the process exits 0 (FADESIM_STRICT_EXIT=1 makes it exit 1).
+//
+// Extension: `seed <n>` makes randSetR / randSetInputV reproducible (the
+// reference seeds them from the clock or std::random_device).
+// Commands that only drive SPICE netlist generation or other solvers are
+// accepted and logged as not handled by this build.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fadesim.h"
+
+namespace {
+
+struct CmdError : std::runtime_error {
+    explicit CmdError(const std::string &s) : std::runtime_error(s) {}
+};
+
+struct Array2D {
+    int n = 0, m = 0;
+    std::vector<std::vector<int>> arr;           // cell state index
+    std::vector<std::vector<double>> RU, RD;     // RU[n][m+1], RD[m][n+1]
+    std::vector<double> v[4];                    // up, left, down, right
+    std::vector<int> bv[4];
+    double r_load[4] = {-1, -1, -1, -1};
+    std::vector<double> cellR_states, inputV_states;
+    std::vector<std::string> history;
+    bool seeded = false;
+    std::mt19937_64 rng;
+
+    void setSize(int nn, int mm)
+    {
+        n = nn;
+        m = mm;
+        arr.assign(n, std::vector<int>(m, 0));
+        RU.assign(n, std::vector<double>(m + 1, 0.0));
+        RD.assign(m, std::vector<double>(n + 1, 0.0));
+        v[0] = v[2] = std::vector<double>(m, 0.0);
+        v[1] = v[3] = std::vector<double>(n, 0.0);
+        bv[0] = bv[2] = std::vector<int>(m, 0);
+        bv[1] = bv[3] = std::vector<int>(n, 0);
+    }
+    void log(const std::string &s) { history.push_back(s); }
+    void printLog(std::ostream &os) const
+    {
+        os << std::endl << "--------Operation Log---------" << std::endl;
+        for (auto &h : history)
+            os << h << std::endl;
+        os << "--------Log end--------" << std::endl;
+    }
+};
+
+template <typename T>
+T parse(const std::string &s)
+{
+    std::istringstream iss(s);
+    T v;
+    iss >> v;
+    if (iss.fail() || !iss.eof())
+        throw CmdError("Failed to parse argument: " + s);
+    return v;
+}
+
+int getDir(const std::string &d)
+{
+    if (d == "up") return 0;
+    if (d == "left") return 1;
+    if (d == "down") return 2;
+    if (d == "right") return 3;
+    throw CmdError("Arg str not matched Direction should be up|left|down|right.");
+}
+
+bool checkYes(std::string s)
+{
+    for (auto &c : s)
+        c = (char)::tolower(c);
+    return s == "yes" || s == "1" || s == "true" || s == "y" || s == "t";
+}
+
+void need_size(const Array2D &a)
+{
+    if (a.n == 0 || a.m == 0)
+        throw CmdError("Please set arraySize before this command");
+}
+
+std::vector<int> sample(Array2D &a, const std::vector<double> &p, size_t len)
+{
+    double sum = 0;
+    for (double x : p)
+        sum += x;
+    if (std::abs(sum - 1.0) > 1e-6)
+        throw CmdError("Error: p sum is not 1");
+    if (!a.seeded) {
+        std::random_device rd;
+        a.rng.seed(((uint64_t)rd() << 32) ^ rd());
+        a.seeded = true;
+    }
+    std::discrete_distribution<int> dist(p.begin(), p.end());
+    std::vector<int> out(len);
+    for (auto &x : out)
+        x = dist(a.rng);
+    return out;
+}
+
+#define HIPCHK(e)                                                                  \
+    do {                                                                           \
+        hipError_t r_ = (e);                                                       \
+        if (r_ != hipSuccess)                                                      \
+            throw CmdError(std::string("HIP error: ") + hipGetErrorString(r_));    \
+    } while (0)
+
+void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
+{
+    if (args.size() < 3)
+        throw CmdError("args number not matched (<3)");
+    const int iter_time = parse<int>(args[0]);
+    const double w = std::stod(args[1]);
+    const int mode = parse<int>(args[2]);
+    bool enable_break = false;
+    double tol = 0;
+    if (args.size() > 3) {
+        if (args.size() != 5)
+            throw CmdError("args number not matched (!=5)");
+        enable_break = checkYes(args[3]);
+        if (enable_break)
+            tol = std::stod(args[4]);
+    }
+    if (mode & 1)
+        throw CmdError("mode bit 0 is not supported now");
+    if (!(a.m >= 2 && a.n >= 2))
+        throw CmdError("m and n should greater than 2");
+    if (mode & 4)
+        throw CmdError("mode bit 2 is not supported now");
+    const int n = a.n, m = a.m;
+    const size_t nm = (size_t)n * m;
+    std::vector<double> g(nm), ru((size_t)n * (m + 1)), rd((size_t)m * (n + 1));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            const int s = a.arr[i][j];
+            if (s < 0 || s >= (int)a.cellR_states.size())
+                throw CmdError("cell state " + std::to_string(s) + " has no resistance (cellRStates)");
+            g[(size_t)i * m + j] = 1.0 / a.cellR_states[s];
+        }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= m; ++j)
+            ru[(size_t)i * (m + 1) + j] = a.RU[i][j];
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i <= n; ++i)
+            rd[(size_t)j * (n + 1) + i] = a.RD[j][i];
+    std::ostringstream os;
+    std::vector<double> vup(nm), vdown(nm), iout(m);
+    int record = -1, sweeps = 0;
+    double seconds = 0;
+    if ((mode & 2) == 0) {
+        double *d_g, *d_ru, *d_rd, *d_vin, *d_vout, *d_vup, *d_vdn, *d_iout;
+        int *d_int;
+        HIPCHK(hipMalloc(&d_g, nm * 8));
+        HIPCHK(hipMalloc(&d_ru, ru.size() * 8));
+        HIPCHK(hipMalloc(&d_rd, rd.size() * 8));
+        HIPCHK(hipMalloc(&d_vin, n * 8));
+        HIPCHK(hipMalloc(&d_vout, m * 8));
+        HIPCHK(hipMalloc(&d_vup, nm * 8));
+        HIPCHK(hipMalloc(&d_vdn, nm * 8));
+        HIPCHK(hipMalloc(&d_iout, m * 8));
+        HIPCHK(hipMalloc(&d_int, 2 * sizeof(int)));
+        HIPCHK(hipMemcpy(d_g, g.data(), nm * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_ru, ru.data(), ru.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_rd, rd.data(), rd.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_vin, a.v[1].data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_vout, a.v[2].data(), m * 8, hipMemcpyHostToDevice));
+        fadesim_gs64_desc d{};
+        d.rows = n;
+        d.cols = m;
+        d.d_g = d_g;
+        d.d_ru = d_ru;
+        d.ru_ld = m + 1;
+        d.d_rd = d_rd;
+        d.rd_ld = n + 1;
+        d.d_vin = d_vin;
+        d.d_vout = d_vout;
+        d.iter_number = iter_time;
+        d.omega = w;
+        d.enable_break = enable_break;
+        d.tol = tol;
+        d.d_vup = d_vup;
+        d.d_vdown = d_vdn;
+        d.d_iout = d_iout;
+        d.d_record = d_int;
+        d.d_sweeps = d_int + 1;
+        HIPCHK(hipDeviceSynchronize());
+        auto t0 = std::chrono::steady_clock::now();
+        int rc = fadesim_gs64_solve(&d, nullptr);
+        if (rc != FADESIM_OK)
+            throw CmdError(std::string("solver: ") + fadesim_last_error());
+        HIPCHK(hipDeviceSynchronize());
+        seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        int hi[2];
+        HIPCHK(hipMemcpy(vup.data(), d_vup, nm * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(vdown.data(), d_vdn, nm * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(iout.data(), d_iout, m * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hi, d_int, 2 * sizeof(int), hipMemcpyDeviceToHost));
+        record = hi[0];
+        sweeps = hi[1];
+        for (void *p : {(void *)d_g, (void *)d_ru, (void *)d_rd, (void *)d_vin, (void *)d_vout,
+                        (void *)d_vup, (void *)d_vdn, (void *)d_iout, (void *)d_int})
+            HIPCHK(hipFree(p));
+        os << "Duration time = " << seconds << " s" << std::endl;
+    } else {
+        // mode bit 1: no sweeps, the initial guess is reported
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < m; ++j) {
+                vup[(size_t)i * m + j] = a.v[1][i];
+                vdown[(size_t)i * m + j] = a.v[2][j];
+            }
+        for (int j = 0; j < m; ++j) {
+            double s = 0;
+            for (int i = 0; i < n; ++i)
+                s += (vup[(size_t)i * m + j] - vdown[(size_t)i * m + j]) * g[(size_t)i * m + j];
+            iout[j] = s;
+        }
+    }
+    os << "GS method mem usage  = "
+       << (sizeof(int) * 5 + sizeof(double) * 3 + nm * sizeof(double) * 5 + m * sizeof(double) * 2 +
+           n * sizeof(double))
+       << " bytes (Compuated by simply multiplying the number"
+          " of variables by the variable space)"
+       << std::endl;
+    if (record != -1)
+        os << "threshould break, iter times =  " << record << std::endl;
+    else
+        os << "iter times = " << iter_time << std::endl;
+    // partial-KCL residual sum, as concrete_command.h:1293-1319
+    const double gw = 1.0 / a.RD[0][0];
+    auto U = [&](int i, int j) { return vup[(size_t)i * m + j]; };
+    auto D = [&](int i, int j) { return vdown[(size_t)i * m + j]; };
+    auto G = [&](int i, int j) { return g[(size_t)i * m + j]; };
+    double err = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 1; j < m - 1; ++j)
+            err += std::fabs((U(i, j - 1) - U(i, j) + U(i, j + 1) - U(i, j)) * gw + (D(i, j) - U(i, j)) * G(i, j));
+        if (m > 1)
+            err += std::fabs((U(i, m - 2) - U(i, m - 1)) * gw + (D(i, m - 1) - U(i, m - 1)) * G(i, m - 1));
+    }
+    for (int j = 0; j < m; ++j)
+        err += std::fabs((U(0, j) - D(0, j)) * G(0, j) + (n > 1 ? D(1, j) - D(0, j) : 0) * gw);
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 0; j < m; ++j)
+            err += std::fabs((U(i, j) - D(i, j)) * G(i, j) + (D(i - 1, j) - D(i, j) + D(i + 1, j) - D(i, j)) * gw);
+    std::ofstream of("fastmode_nodebasedGS.out");
+    of << "error I sum = " << err << std::endl;
+    os << "error I sum = " << err << std::endl;
+    of << "vup = " << std::endl;
+    of << std::scientific << std::setprecision(12);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < m; ++j)
+            of << std::setw(20) << U(i, j);
+        of << std::endl;
+    }
+    of << std::endl << "vdown = " << std::endl;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < m; ++j)
+            of << std::setw(20) << D(i, j);
+        of << std::endl;
+    }
+    of << std::endl << "iout = " << std::endl;
+    for (int j = 0; j < m; ++j)
+        of << std::setw(20) << iout[j];
+    of << std::endl;
+    std::cout << "evaluate ir drop using GS-method (GPU, red-black), w = " << w << ", sweeps = " << sweeps
+              << ", print to file fastmode_nodebasedGS.out" << std::endl;
+    a.log(os.str());
+}
+
+void run_line(Array2D &a, const std::string &line)
+{
+    std::istringstream iss(line);
+    std::vector<std::string> tok;
+    for (std::string s; iss >> s;)
+        tok.push_back(s);
+    if (tok.empty())
+        return;
+    const std::string cmd = tok[0];
+    std::vector<std::string> args(tok.begin() + 1, tok.end());
+    if (cmd == "%" || cmd == "#" || cmd.rfind("//", 0) == 0 || cmd.rfind("%", 0) == 0)
+        return;
+    std::ostringstream os;
+    auto nargs = [&](size_t k) {
+        if (args.size() != k)
+            throw CmdError("Failed to parse argument due to the number of args not match.");
+    };
+    if (cmd == "arraySize") {
+        nargs(2);
+        const int n = parse<int>(args[0]), m = parse<int>(args[1]);
+        if (n < 1 || m < 1)
+            throw CmdError("array size must be positive");
+        a.setSize(n, m);
+        os << "Set array size to ( " << n << ", " << m << " )";
+    } else if (cmd == "line_resistance" || (cmd == "setLineR" && args.size() == 1)) {
+        nargs(1);
+        need_size(a);
+        const double r = parse<double>(args[0]);
+        for (auto &row : a.RU)
+            for (auto &x : row)
+                x = r;
+        for (auto &row : a.RD)
+            for (auto &x : row)
+                x = r;
+        os << "Set all line resistance to " << r;
+    } else if (cmd == "setLineR") {
+        nargs(4);
+        need_size(a);
+        const int x = parse<int>(args[1]), y = parse<int>(args[2]);
+        const double r = parse<double>(args[3]);
+        if (args[0] == "up") {
+            if (x < 0 || x >= a.n || y < 0 || y > a.m || r < 0)
+                throw CmdError("Arg Direction not matched:setLineR (up|down) x:int<n y:int<m r:double>0");
+            a.RU[x][y] = r;
+            os << "RULine (" << x << ", " << y << ") to " << r;
+        } else if (args[0] == "down") {
+            if (x < 0 || x >= a.m || y < 0 || y > a.n || r < 0)
+                throw CmdError("Arg Direction not matched:setLineR (up|down) x:int<n y:int<m r:double>0");
+            a.RD[x][y] = r;
+            os << "RDLine (" << x << ", " << y << ") to " << r;
+        } else {
+            throw CmdError("Arg Direction not matched:setLineR up|down x:int y:int r:double");
+        }
+    } else if (cmd == "rload") {
+        nargs(2);
+        need_size(a);
+        const std::string md = args[0];
+        const double r = parse<double>(args[1]);
+        if (md == "all") {
+            for (auto &row : a.RU)
+                row.front() = row.back() = r;
+            for (auto &row : a.RD)
+                row.front() = row.back() = r;
+            for (double &x : a.r_load)
+                x = r;
+        } else if (md == "left" || md == "right") {
+            for (auto &row : a.RU)
+                (md == "left" ? row.front() : row.back()) = r;
+            a.r_load[md == "left" ? 1 : 3] = r;
+        } else if (md == "up" || md == "down") {
+            for (auto &row : a.RD)
+                (md == "up" ? row.front() : row.back()) = r;
+            a.r_load[md == "up" ? 0 : 2] = r;
+        } else {
+            throw CmdError("Arg str not matched");
+        }
+        os << "set " << md << " line r_load to " << r << " omega";
+    } else if (cmd == "setUseLine" || cmd == "setNotUseLine" || cmd == "setUseLine++") {
+        need_size(a);
+        if (args.empty())
+            throw CmdError("Failed to parse argument with 0 args.");
+        const int use = cmd == "setNotUseLine" ? 0 : (cmd == "setUseLine" ? 1 : 2);
+        const int d = getDir(args[0]);
+        for (size_t k = 1; k < args.size(); ++k) {
+            const int x = parse<int>(args[k]);
+            if (x == -1)
+                for (auto &b : a.bv[d])
+                    b = use;
+            else
+                a.bv[d].at(x) = use;
+        }
+        os << (use ? "set use Direction " : "set not use Direction ") << d;
+    } else if (cmd == "setLineV") {
+        need_size(a);
+        if (args.size() != 3)
+            throw CmdError("Args size not matched!");
+        const int d = getDir(args[0]);
+        const int x = parse<int>(args[1]);
+        const double val = std::stod(args[2]);
+        if (x == -1)
+            for (auto &vv : a.v[d])
+                vv = val;
+        else {
+            if (x < 0 || x >= (int)a.v[d].size())
+                throw CmdError("line index out of range");
+            a.v[d][x] = val;
+        }
+        os << "set direction " << d << (x == -1 ? " all line" : " line No." + std::to_string(x))
+           << " to voltage " << val << " [d=(0, 1, 2, 3)->(up, left, down, right)]";
+    } else if (cmd == "setCellR") {
+        nargs(3);
+        need_size(a);
+        const int x = parse<int>(args[0]), y = parse<int>(args[1]), z = parse<int>(args[2]);
+        os << "setCellR " << x << ' ' << y << ' ' << z;
+        for (int i = 0; i < a.n; ++i) {
+            if (x != -1 && i != x)
+                continue;
+            for (int j = 0; j < a.m; ++j)
+                if (y == -1 || j == y)
+                    a.arr[i][j] = z;
+        }
+        if ((x != -1 && (x < 0 || x >= a.n)) || (y != -1 && (y < 0 || y >= a.m)))
+            throw CmdError("cell index out of range");
+    } else if (cmd == "cellRStates" || cmd == "cellRstates") {
+        try {
+            const int k = std::stoi(args.at(0));
+            a.cellR_states.assign(k, 0.0);
+            for (int i = 0; i < k; ++i)
+                a.cellR_states.at(i) = std::stod(args.at(i + 1));
+        } catch (const std::exception &) {
+            throw CmdError("cell r set fail!");
+        }
+    } else if (cmd == "inputVStates" || cmd == "inputVstates") {
+        try {
+            const int k = std::stoi(args.at(0));
+            a.inputV_states.assign(k, 0.0);
+            for (int i = 0; i < k; ++i)
+                a.inputV_states.at(i) = std::stod(args.at(i + 1));
+        } catch (const std::exception &) {
+            throw CmdError("input set fail!");
+        }
+        os << "set cell input v states, note that, this is useful when calling randSetInputV";
+    } else if (cmd == "seed") {
+        nargs(1);
+        a.rng.seed(parse<unsigned long long>(args[0]));
+        a.seeded = true;
+        os << "random seed " << args[0];
+    } else if (cmd == "randSetR") {
+        need_size(a);
+        std::vector<double> p;
+        for (auto &s : args)
+            p.push_back(parse<double>(s));
+        if (p.size() != a.cellR_states.size())
+            throw CmdError("p size should equal to item size");
+        auto seq = sample(a, p, (size_t)a.n * a.m);
+        for (int i = 0; i < a.n; ++i)
+            for (int j = 0; j < a.m; ++j)
+                a.arr[i][j] = seq[(size_t)i * a.m + j];
+        os << "randSetR finish";
+    } else if (cmd == "randSetInputV") {
+        need_size(a);
+        if (args.empty())
+            throw CmdError("Failed to parse argument with 0 args.");
+        const int d = getDir(args[0]);
+        std::vector<double> p;
+        for (size_t k = 1; k < args.size(); ++k)
+            p.push_back(parse<double>(args[k]));
+        if (p.size() != a.inputV_states.size())
+            throw CmdError("p size should equal to item size");
+        auto seq = sample(a, p, a.v[d].size());
+        for (size_t i = 0; i < seq.size(); ++i)
+            a.v[d][i] = a.inputV_states[seq[i]];
+        os << "randSetInputV " << args[0] << " finish";
+    } else if (cmd == "senseBitlineI" || cmd == "senseCellV" || cmd == "senseWordlineI" ||
+               cmd == "useRtypeReRAMCell" || cmd == "dc" || cmd == "ac") {
+        os << cmd << " noted";
+    } else if (cmd == "selector") {
+        if (!args.empty() && checkYes(args[0]))
+            throw CmdError("selector cells are not supported by nodebasedGSMethod");
+        os << "selector no";
+    } else if (cmd == "topString" || cmd == "bottomString" || cmd == "build" || cmd == "capacity" ||
+               cmd == "capacitance" || cmd == "setLineV++" || cmd == "simpleWriteOne" ||
+               cmd == "simpleWriteZero" || cmd == "simpleReadOut" || cmd == "fastmode" ||
+               cmd == "fastsolve" || cmd.rfind("ir_", 0) == 0) {
+        os << cmd << ": not handled by this build (SPICE generation and other solvers); ignored";
+        std::cout << os.str() << std::endl;
+    } else if (cmd == "nodebasedGSMethod") {
+        nodebasedGS(a, args);
+        return;
+    } else {
+        throw CmdError("Unknown command: " + cmd);
+    }
+    a.log(os.str());
+}
+
+} // namespace
+
+int main(int argc, char *argv[])
+{
+    const std::string input = argc >= 2 ? argv[1] : "config";
+    Array2D arr;
+    int line = 0;
+    const bool strict = std::getenv("FADESIM_STRICT_EXIT") && std::string(std::getenv("FADESIM_STRICT_EXIT")) == "1";
+    try {
+        std::ifstream fin(input);
+        if (!fin)
+            throw CmdError("cannot open config file " + input);
+        for (std::string s; std::getline(fin, s);) {
+            ++line;
+            run_line(arr, s);
+        }
+        std::cout << "Log print to default.log" << std::endl;
+        std::ofstream os("default.log");
+        arr.printLog(os);
+        std::cout << "--------Run end && Succeed---------" << std::endl;
+    } catch (const std::exception &e) {
+        std::cout << "Error(s) occured in line " << line << ": " << e.what() << std::endl;
+        arr.printLog(std::cout);
+        return strict ? 1 : 0;
+    }
+    return 0;
+}

@@ -1,0 +1,110 @@
+"""Batched IR-drop solve on the GPU (torch tensors in, torch tensors out).
+
+Thin host layer over ``fadesim_solve_batch`` (include/fadesim.h): validates
+shapes, allocates outputs on the inputs' device and launches on torch's
+current stream.  The solve itself -- FP32 red-black SOR on drop variables,
+FP64 adjoint-weighted residual test, one FP64 refinement step, FP64 readout
+-- is the HIP kernel in csrc/rbsor.hip.
+
+It replaces the per-crossbar sweep loops of
+``NodebasedGSMethod::nodeBaseGS`` (crossbarHspice/V2/concrete_command.h:1136-1361)
+and ``ir_drop_gs`` (pimtorch/.../SplitArrCuda/split_arr_matmul.cu:204-333) for
+uniform wire resistance per instance, converged to a relative bitline-current
+tolerance.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import torch
+
+from . import _native as nat
+
+SUPPORTED_SHAPES = {(32, 32), (64, 64), (128, 128), (128, 64), (64, 128)}
+
+
+@dataclass
+class SolveResult:
+    iout: torch.Tensor            # [B, N] float64 bitline currents (A)
+    sweeps: torch.Tensor          # [B] int32
+    status: torch.Tensor          # [B] int32 (0 converged, 1 max sweeps, 2 non-finite)
+    est: torch.Tensor             # [B] float64 final relative error estimate
+    field: Optional[torch.Tensor]  # [B, 2, M, N] float32 drops (vin-u, d-vout) or None
+
+
+def last_kernel_ms() -> float:
+    """Duration of the solve kernel of the last ``solve_batch(..., timing=True)``
+    (HIP events on the launch stream)."""
+    return float(nat.lib().fadesim_last_kernel_ms())
+
+
+def default_omega(rows: int, cols: int) -> float:
+    return float(nat.lib().fadesim_default_omega(int(rows), int(cols)))
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
+                warm: Optional[torch.Tensor] = None, *, omega: Optional[float] = None,
+                tol: float = 1e-6, safety: float = 2.0, switch_tol: float = 2e-5,
+                max_sweeps: int = 20000, refine: bool = True, dwr: bool = False,
+                fixed_sweeps: Optional[int] = None, want_field: bool = False,
+                out: Optional[torch.Tensor] = None, timing: bool = False) -> SolveResult:
+    """Solve B crossbars of one shape.
+
+    g [B, M, N] float32 (S), r_wire [B] or scalar (ohm), vin [B, M] (V),
+    vout [B, N] or None (0 V), warm [B, 2, M, N] float32 initial drops or None.
+    ``fixed_sweeps=k`` runs exactly k sweeps with no test and no refinement.
+    """
+    if g.dim() != 3:
+        raise ValueError("g must be [B, M, N]")
+    B, M, N = g.shape
+    if (M, N) not in SUPPORTED_SHAPES:
+        raise ValueError(f"shape {M}x{N} not supported by the batched kernel: {sorted(SUPPORTED_SHAPES)}")
+    if not g.is_cuda:
+        raise ValueError("g must be a GPU tensor")
+    dev = g.device
+    g = g.to(torch.float32).contiguous()
+    if not torch.is_tensor(r_wire):
+        r_wire = torch.full((B,), float(r_wire), dtype=torch.float64, device=dev)
+    r_wire = r_wire.to(device=dev, dtype=torch.float64).reshape(-1).expand(B).contiguous()
+    vin = vin.to(device=dev, dtype=torch.float64).reshape(B, M).contiguous()
+    if vout is not None:
+        vout = vout.to(device=dev, dtype=torch.float64).reshape(B, N).contiguous()
+    if warm is not None:
+        warm = warm.to(device=dev, dtype=torch.float32).reshape(B, 2, M, N).contiguous()
+    iout = out if out is not None else torch.empty((B, N), dtype=torch.float64, device=dev)
+    if iout.shape != (B, N) or iout.dtype != torch.float64 or not iout.is_contiguous():
+        raise ValueError("out must be a contiguous float64 [B, N] tensor")
+    sweeps = torch.empty(B, dtype=torch.int32, device=dev)
+    status = torch.empty(B, dtype=torch.int32, device=dev)
+    est = torch.empty(B, dtype=torch.float64, device=dev)
+    field = torch.empty((B, 2, M, N), dtype=torch.float32, device=dev) if want_field else None
+
+    flags = 0
+    if fixed_sweeps is not None:
+        flags |= nat.F_FIXED_SWEEPS
+        max_sweeps = int(fixed_sweeps)
+    if not refine:
+        flags |= nat.F_NO_REFINE
+    if dwr:
+        flags |= nat.F_DWR
+    if timing:
+        flags |= nat.F_TIMING
+    d = nat.BatchDesc()
+    d.batch, d.rows, d.cols = B, M, N
+    d.d_g, d.d_r_wire, d.d_vin = g.data_ptr(), r_wire.data_ptr(), vin.data_ptr()
+    d.d_vout, d.d_warm = _ptr(vout), _ptr(warm)
+    d.omega = float(omega) if omega else 0.0
+    d.tol, d.safety, d.switch_tol = float(tol), float(safety), float(switch_tol)
+    d.max_sweeps, d.flags = int(max_sweeps), int(flags)
+    d.d_iout, d.d_sweeps, d.d_status, d.d_est = iout.data_ptr(), sweeps.data_ptr(), status.data_ptr(), est.data_ptr()
+    d.d_field = _ptr(field)
+    with torch.cuda.device(dev):
+        rc = nat.lib().fadesim_solve_batch(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
+    nat.check(rc, "fadesim_solve_batch")
+    return SolveResult(iout, sweeps, status, est, field)

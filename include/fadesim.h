@@ -1,0 +1,156 @@
+/*
+ * fadesim.h -- C ABI of the MI355X-native FADESim IR-drop solver.
+ *
+ * Every pointer argument named d_* is a device pointer (HBM) owned by the
+ * caller; every size is in elements.  `stream` is a hipStream_t passed as
+ * void* (NULL = the default stream).  Calls are stream-ordered and do not
+ * synchronise the host unless stated.  Functions return 0 on success or a
+ * negative FADESIM_E_* code; fadesim_last_error() returns the message of the
+ * last failure on the calling thread.
+ *
+ * Network (both entry points): an n x m crossbar; wordline ("up") nodes u_ij
+ * are driven from the left through segment 0 by vin_i, bitline ("down")
+ * nodes d_ij sink at the bottom through segment n into vout_j; cell (i,j)
+ * joins u_ij and d_ij with conductance g_ij.  The bitline current read out
+ * is I_j = sum_i g_ij (u_ij - d_ij).
+ */
+#ifndef FADESIM_H
+#define FADESIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FADESIM_OK 0
+#define FADESIM_E_INVALID -1   /* bad argument (size, pointer, omega ...) */
+#define FADESIM_E_UNSUPPORTED -2 /* shape not handled by any kernel */
+#define FADESIM_E_HIP -3       /* HIP runtime error */
+#define FADESIM_E_NOMEM -4     /* workspace allocation failed */
+
+/* per-instance status written by fadesim_solve_batch */
+#define FADESIM_ST_CONVERGED 0
+#define FADESIM_ST_MAXIT 1
+#define FADESIM_ST_NONFINITE 2
+
+/* flags of fadesim_batch_desc */
+#define FADESIM_F_FIXED_SWEEPS 1u /* run exactly max_sweeps sweeps, no test, no refinement */
+#define FADESIM_F_NO_REFINE 2u    /* skip the FP64 iterative-refinement step */
+#define FADESIM_F_DWR 4u          /* return I - (adjoint-weighted residual) and stop on it */
+#define FADESIM_F_TIMING 8u       /* record HIP events around the solve kernel */
+
+const char *fadesim_last_error(void);
+int fadesim_version(void);
+
+/*
+ * Batched IR-drop solve, FP32 red-black SOR on drop variables with an FP64
+ * residual-based convergence test, one FP64 iterative-refinement step and an
+ * FP64 readout.  All instances of one call share rows x cols.
+ *
+ * Replaces the per-crossbar solve of
+ *   NodebasedGSMethod::nodeBaseGS   crossbarHspice/V2/concrete_command.h:1136-1361
+ *   array2D::nodeBasedGS            crossbarHspice/2dArray.cc:1377-1687
+ * and the sweep loop of ir_drop_gs  pimtorch/pimfixedpoint/fixedPoint/SplitArrCuda/
+ *                                   split_arr_matmul.cu:204-333
+ * for uniform wire resistance per instance, converged to a relative I_out
+ * tolerance instead of a fixed or max|dv| sweep rule.
+ */
+typedef struct fadesim_batch_desc {
+    int batch;              /* B instances */
+    int rows, cols;         /* n, m: multiples of 4, 8..128 */
+    const float *d_g;       /* [B][n][m] cell conductance (S) */
+    const double *d_r_wire; /* [B] wire segment resistance (ohm) */
+    const double *d_vin;    /* [B][n] wordline source voltage (V) */
+    const double *d_vout;   /* [B][m] bitline sink voltage (V), or NULL = 0 V */
+    const float *d_warm;    /* [B][2][n][m] initial drops (vin-u, d-vout), or NULL = 0 */
+    double omega;           /* SOR factor in (0,2); <= 0 selects the built-in rule */
+    double tol;             /* target relative I_out error (e.g. 1e-6) */
+    double safety;          /* stop when estimate * safety <= tol (>= 1; 0 -> 2) */
+    double switch_tol;      /* estimate at which the refinement step is taken (0 -> 2e-5) */
+    int max_sweeps;         /* sweep budget (FIXED_SWEEPS: exact count) */
+    unsigned flags;         /* FADESIM_F_* */
+    double *d_iout;         /* [B][m] bitline currents (A), FP64 */
+    int *d_sweeps;          /* [B] sweeps performed, or NULL */
+    int *d_status;          /* [B] FADESIM_ST_*, or NULL */
+    double *d_est;          /* [B] final relative error estimate, or NULL */
+    float *d_field;         /* [B][2][n][m] final drops, or NULL */
+} fadesim_batch_desc;
+
+int fadesim_solve_batch(const fadesim_batch_desc *desc, void *stream);
+
+/* Duration (ms) of the solve kernel of the last fadesim_solve_batch call made
+ * with FADESIM_F_TIMING on this thread, from HIP events on the call's stream;
+ * waits for that kernel.  Negative when no timed call was made. */
+double fadesim_last_kernel_ms(void);
+
+/* The built-in relaxation factor for an n x m array (used when omega <= 0). */
+double fadesim_default_omega(int rows, int cols);
+
+/*
+ * One crossbar, FP64 red-black SOR in node voltages with per-segment wire
+ * resistances and the reference's stopping rule (stop after the first sweep
+ * whose max |dv| < tol; fixed iter_number sweeps without a break).  The
+ * solver behind the `sim` CLI command
+ *   nodebasedGSMethod <iter> <omega> <mode> [yes <tol>]
+ * replacing NodebasedGSMethod::nodeBaseGS
+ *   crossbarHspice/V2/concrete_command.h:1136-1361 (command :1097-1127)
+ * and array2D::nodeBasedGS  crossbarHspice/2dArray.cc:1377-1687.
+ * The sweep order is red-black, not lexicographic: the fixed point is the
+ * same, the iterates are not.
+ */
+typedef struct fadesim_gs64_desc {
+    int rows, cols;        /* n, m >= 2 */
+    const double *d_g;     /* [n][m] cell conductance (S) */
+    const double *d_ru;    /* [n][ru_ld] wordline segments (ohm), j in [0, m] */
+    int ru_ld;             /* >= m + 1 */
+    const double *d_rd;    /* [m][rd_ld] bitline segments (ohm), i in [0, n] */
+    int rd_ld;             /* >= n + 1 */
+    const double *d_vin;   /* [n] left (wordline) drive (V) */
+    const double *d_vout;  /* [m] bottom (bitline) sink voltage (V) */
+    int iter_number;       /* sweep budget */
+    double omega;          /* SOR factor */
+    int enable_break;      /* reference break rule on/off */
+    double tol;            /* break threshold on max |dv| (V) */
+    double *d_vup;         /* [n][m] out: wordline node voltages */
+    double *d_vdown;       /* [n][m] out: bitline node voltages */
+    double *d_iout;        /* [m] out: sum_i (vup - vdown) g */
+    int *d_record;         /* out: 0-based sweep of the break, or -1 */
+    int *d_sweeps;         /* out: sweeps run */
+} fadesim_gs64_desc;
+
+int fadesim_gs64_solve(const fadesim_gs64_desc *desc, void *stream);
+
+/*
+ * The PimTorch hook, with the reference's iteration: line Gauss-Seidel/SOR,
+ * all wordlines left->right then all bitlines bottom->top per sweep, source
+ * and sink conductance g_wire, initial state u = input, d = 0; a fixed
+ * iter_times, or a break when no node of the whole batch moved by more than
+ * break_threshold; sinh-nonlinear cells by the lagged-conductance Picard
+ * update.  Replaces ir_drop_gs and its kernels
+ *   pimtorch/pimfixedpoint/fixedPoint/SplitArrCuda/split_arr_matmul.cu:27-140, 204-333
+ * (pybind11 entry :1431).  dtype_bytes selects float (4) or double (8) for
+ * input, G and out.  Shapes: input [batch][numX][row], linear G
+ * [numX][row][numY*col]; out [batch][numX][numY*col] currents, or
+ * [batch][numX][numY][row][col] voltage differences with ret_v.
+ */
+typedef struct fadesim_hook_desc {
+    int dtype_bytes;
+    const void *d_input;
+    const void *d_G;
+    int batch_size, row_size, col_size, numX, numY;
+    int iter_times;
+    double g_wire, beta;
+    int enable_break;
+    double break_threshold;
+    int nonlinear;
+    double nonlinear_params[6]; /* Ron, Roff, a_on, b_on, a_off, b_off */
+    int ret_v;
+    void *d_out;
+    int *d_sweeps; /* out (device): sweeps run, or NULL */
+} fadesim_hook_desc;
+
+int fadesim_hook_ir_drop_gs(const fadesim_hook_desc *desc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FADESIM_H */

@@ -1,0 +1,215 @@
+"""CPU oracle for the FADESim IR-drop solver path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this package, and only as the checker.  The product
+(``fadesim_amd``) never imports it.
+
+numpy wrappers over ``liboracle.so`` (oracle/fadesim_oracle.c):
+
+* :func:`nodegs`       lexicographic SOR restating
+  ``NodebasedGSMethod::nodeBaseGS`` (crossbarHspice/V2/concrete_command.h:1136-1361)
+  and ``array2D::nodeBasedGS`` (crossbarHspice/2dArray.cc:1377-1687).
+* :func:`exact`        exact FP64 banded-Cholesky solve of the same network.
+* :func:`exact_batch`  the same, batched over instances with uniform wire r.
+* :func:`rb_f64`       FP64 red-black replay of the GPU drop-in kernel.
+* :func:`rb_f32`       FP32 red-black replay of the GPU batched kernel.
+* :func:`hook_gs`      the PimTorch hook's line-GS ordering
+  (pimtorch/pimfixedpoint/fixedPoint/SplitArrCuda/split_arr_matmul.cu:27-333).
+* :func:`errisum`      the partial-KCL "error I sum" (concrete_command.h:1293-1319).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the recipe in oracle/Makefile."""
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        L = ctypes.CDLL(so)
+        i, d = ctypes.c_int, ctypes.c_double
+        L.oracle_nodegs.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, i, d, i, d,
+                                    _dp, _dp, _dp, _ip]
+        L.oracle_nodegs.restype = i
+        L.oracle_errisum.argtypes = [i, i, d, _dp, _dp, _dp]
+        L.oracle_errisum.restype = d
+        L.oracle_exact.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_exact.restype = i
+        L.oracle_exact_batch_f32g.argtypes = [i, i, i, _fp, _dp, _dp, _dp, _dp]
+        L.oracle_exact_batch_f32g.restype = i
+        L.oracle_nodegs_batch_f32g.argtypes = [i, i, i, _fp, _dp, _dp, i, d, d, _dp, _ip]
+        L.oracle_nodegs_batch_f32g.restype = i
+        L.oracle_rb_f64.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, i, d, i, d,
+                                    _dp, _dp, _dp, _ip]
+        L.oracle_rb_f64.restype = i
+        L.oracle_rb_f32.argtypes = [i, i, _fp, d, _dp, _dp, d, i, _fp, _fp]
+        L.oracle_rb_f32.restype = None
+        for name, fp in (("oracle_hook_gs_f64", _dp), ("oracle_hook_gs_f32", _fp)):
+            f = getattr(L, name)
+            f.argtypes = [fp, fp, i, i, i, i, i, i, d, d, i, d, i, _dp, i, fp]
+            f.restype = i
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pd(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pf(a):
+    return a.ctypes.data_as(_fp)
+
+
+def segments_uniform(n: int, m: int, r: float):
+    """RU [n, m+1] and RD [m, n+1] filled with one wire resistance."""
+    return np.full((n, m + 1), float(r)), np.full((m, n + 1), float(r))
+
+
+def nodegs(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0):
+    """Lexicographic SOR.  Returns (vup, vdown, iout, record)."""
+    g = _d(g)
+    n, m = g.shape
+    RU, RD = _d(RU), _d(RD)
+    vup = np.empty((n, m))
+    vdn = np.empty((n, m))
+    iout = np.empty(m)
+    rec = ctypes.c_int(0)
+    rc = lib().oracle_nodegs(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                             _pd(_d(vin)), _pd(_d(vout)), int(iters), float(w),
+                             int(bool(enable_break)), float(tol), _pd(vup), _pd(vdn),
+                             _pd(iout), ctypes.byref(rec))
+    if rc != 0:
+        raise ValueError("m and n should greater than 2")
+    return vup, vdn, iout, rec.value
+
+
+def nodegs_batch(g32, r, vin, iters, w, tol):
+    """Lexicographic SOR (reference rule, break at max|dv| < tol) over a batch
+    [B, n, m] of FP32 conductances with one wire resistance each and grounded
+    bitlines, OpenMP over instances.  Returns (iout [B, m], sweeps [B], threads)."""
+    g32 = _f(g32)
+    B, n, m = g32.shape
+    r = _d(np.broadcast_to(np.asarray(r, dtype=np.float64), (B,)))
+    vin = _d(vin).reshape(B, n)
+    iout = np.empty((B, m))
+    sweeps = np.empty(B, dtype=np.int32)
+    nt = lib().oracle_nodegs_batch_f32g(B, n, m, _pf(g32), _pd(r), _pd(vin), int(iters), float(w),
+                                        float(tol), _pd(iout), sweeps.ctypes.data_as(_ip))
+    return iout, sweeps, nt
+
+
+def errisum(g, vup, vdown, g_wire):
+    g = _d(g)
+    n, m = g.shape
+    return lib().oracle_errisum(n, m, float(g_wire), _pd(g), _pd(_d(vup)), _pd(_d(vdown)))
+
+
+def exact(g, vin, vout, RU, RD):
+    """Exact FP64 solve.  Returns (u, d, iout)."""
+    g = _d(g)
+    n, m = g.shape
+    RU, RD = _d(RU), _d(RD)
+    u = np.empty((n, m))
+    d = np.empty((n, m))
+    iout = np.empty(m)
+    rc = lib().oracle_exact(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                            _pd(_d(vin)), _pd(_d(vout)), _pd(u), _pd(d), _pd(iout))
+    if rc != 0:
+        raise RuntimeError(f"oracle_exact failed ({rc})")
+    return u, d, iout
+
+
+def exact_batch(g32, r, vin, vout=None):
+    """Exact FP64 readout for a batch [B, n, m] of FP32 conductances with one
+    wire resistance per instance.  Returns iout [B, m] (FP64)."""
+    g32 = _f(g32)
+    B, n, m = g32.shape
+    r = _d(np.broadcast_to(np.asarray(r, dtype=np.float64), (B,)))
+    vin = _d(vin).reshape(B, n)
+    vo = None if vout is None else _d(vout).reshape(B, m)
+    iout = np.empty((B, m))
+    rc = lib().oracle_exact_batch_f32g(B, n, m, _pf(g32), _pd(r), _pd(vin),
+                                       None if vo is None else _pd(vo), _pd(iout))
+    if rc != 0:
+        raise RuntimeError("oracle_exact_batch_f32g failed")
+    return iout
+
+
+def rb_f64(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0):
+    """FP64 red-black SOR.  Returns (u, d, iout, record, sweeps)."""
+    g = _d(g)
+    n, m = g.shape
+    RU, RD = _d(RU), _d(RD)
+    u = np.empty((n, m))
+    d = np.empty((n, m))
+    iout = np.empty(m)
+    rec = ctypes.c_int(0)
+    sweeps = lib().oracle_rb_f64(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                                 _pd(_d(vin)), _pd(_d(vout)), int(iters), float(w),
+                                 int(bool(enable_break)), float(tol), _pd(u), _pd(d),
+                                 _pd(iout), ctypes.byref(rec))
+    return u, d, iout, rec.value, sweeps
+
+
+def rb_f32(g32, r, vin, vout, w, sweeps, p=None, q=None):
+    """FP32 red-black SOR on drop variables for exactly `sweeps` sweeps.
+    Returns (p, q) (FP32)."""
+    g32 = _f(g32)
+    n, m = g32.shape
+    p = np.zeros((n, m), np.float32) if p is None else _f(p).copy()
+    q = np.zeros((n, m), np.float32) if q is None else _f(q).copy()
+    vo = None if vout is None else _d(vout)
+    lib().oracle_rb_f32(n, m, _pf(g32), float(r), _pd(_d(vin)),
+                        None if vo is None else _pd(vo), float(w), int(sweeps), _pf(p), _pf(q))
+    return p, q
+
+
+def hook_gs(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, g_wire, beta,
+            enable_break, break_threshold, nonlinear_cell_enable, nonlinear_params,
+            ret_V=False):
+    """Line-GS of the PimTorch hook.  dtype follows G (float32 or float64).
+    Returns (out, sweeps) with out [bsize, numX, numY*col] or
+    [bsize, numX, numY, row, col] when ret_V."""
+    f64 = np.asarray(G).dtype == np.float64
+    cv = _d if f64 else _f
+    pp = _pd if f64 else _pf
+    inp, G = cv(inp), cv(G)
+    nl = _d(list(nonlinear_params) + [0.0] * (6 - len(nonlinear_params)))
+    if ret_V:
+        out = np.empty((batch_size, numX, numY, row_size, col_size), G.dtype)
+    else:
+        out = np.empty((batch_size, numX, numY * col_size), G.dtype)
+    f = lib().oracle_hook_gs_f64 if f64 else lib().oracle_hook_gs_f32
+    sweeps = f(pp(inp), pp(G), batch_size, row_size, col_size, numX, numY, int(iter_times),
+               float(g_wire), float(beta), int(bool(enable_break)), float(break_threshold),
+               int(bool(nonlinear_cell_enable)), _pd(nl), int(bool(ret_V)), pp(out))
+    return out, sweeps

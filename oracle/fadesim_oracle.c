@@ -1,0 +1,663 @@
+/*
+ * CPU oracle for the FADESim IR-drop solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (fadesim_amd/, include/)
+ * links, loads or calls this file; only tests/, the smoke check and the CPU
+ * baseline leg of bench.py do, and there only as the checker.
+ *
+ * Contents
+ *   oracle_nodegs      lexicographic point SOR, a statement-by-statement
+ *                      restatement of NodebasedGSMethod::nodeBaseGS
+ *                      (crossbarHspice/V2/concrete_command.h:1136-1361) and
+ *                      array2D::nodeBasedGS (crossbarHspice/2dArray.cc:1377-1687).
+ *                      Pinned bitwise against golden vectors written by the
+ *                      reference `sim` binaries (tests/golden/).
+ *   oracle_exact       exact FP64 solve of the same linear network by banded
+ *                      Cholesky (the matrix is SPD).  Pinned against
+ *                      scipy.sparse.linalg.spsolve, 2x2 closed forms and the
+ *                      KCL/monotonicity invariants (tests/test_oracle.py).
+ *   oracle_errisum     the partial-KCL "error I sum" of concrete_command.h:1293-1319.
+ *   oracle_rb_f64      FP64 red-black SOR replay of the GPU drop-in kernel
+ *                      (reference stopping rule max|dv| < tol).
+ *   oracle_rb_f32      FP32 red-black SOR replay of the GPU batched kernel on
+ *                      drop variables (fixed sweep count, for bitwise checks).
+ *   oracle_hook_gs_*   the PimTorch hook's line-GS ordering
+ *                      (pimtorch/.../SplitArrCuda/split_arr_matmul.cu:27-140,204-333),
+ *                      linear and sinh-nonlinear cells, FP32 and FP64.
+ *
+ * Build: oracle/Makefile (gcc, -ffp-contract=off so that no multiply-add is
+ * fused: the reference binary is built for baseline x86-64 which has no FMA).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX(i, j, m) ((size_t)(i) * (size_t)(m) + (size_t)(j))
+
+/* ------------------------------------------------------------------------ */
+/* Lexicographic SOR (reference CLI solver)                                  */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * RU: wordline segment resistances, RU[i*ru_ld + j], j in [0, m]; segment 0
+ *     joins the left source to node (i,0), segment j joins (i,j-1)-(i,j).
+ * RD: bitline segment resistances, RD[j*rd_ld + i], i in [0, n]; segment i
+ *     joins (i-1,j)-(i,j), segment n joins (n-1,j) to the bottom sink.
+ * V2 passes RD for both (concrete_command.h:1214 aliases RULine to RDLine).
+ * g:  cell conductances n*m (the reference computes 1.0/cellR_states[state],
+ *     concrete_command.h:1163-1166; callers pass that same quotient).
+ * Returns 0, or -1 if n<2 or m<2 (concrete_command.h:1199-1202).
+ * vup/vdown/iout are the final iterate and the readout sum_i (vup-vdown)*g
+ * (concrete_command.h:1344-1348); *record is the 0-based break sweep or -1.
+ */
+int oracle_nodegs(int n, int m, const double *RU, int ru_ld, const double *RD,
+                  int rd_ld, const double *g, const double *vin,
+                  const double *vout, int iter_number, double w,
+                  int enable_break, double tol, double *vup_out,
+                  double *vdown_out, double *iout, int *record_out)
+{
+    if (!(m >= 2 && n >= 2))
+        return -1;
+    size_t nm = (size_t)n * m;
+    double *vup[2], *vdown[2];
+    vup[0] = (double *)malloc(nm * sizeof(double));
+    vup[1] = (double *)malloc(nm * sizeof(double));
+    vdown[0] = (double *)malloc(nm * sizeof(double));
+    vdown[1] = (double *)malloc(nm * sizeof(double));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            vup[1][IDX(i, j, m)] = vup[0][IDX(i, j, m)] = vin[i];
+            vdown[1][IDX(i, j, m)] = vdown[0][IDX(i, j, m)] = vout[j];
+        }
+#define RUL(i, j) RU[(size_t)(i) * ru_ld + (j)]
+#define RDL(j, i) RD[(size_t)(j) * rd_ld + (i)]
+    int record = -1;
+    for (int iter = 0; iter < iter_number; ++iter) {
+        int x = iter & 1, y = 1 - x;
+        double *ux = vup[x], *uy = vup[y], *dx = vdown[x], *dy = vdown[y];
+        for (int i = 0; i < n; ++i) {
+            for (int j = 0; j < m; ++j) {
+                double sumg;
+                double l = (j == 0 ? vin[i] : ux[IDX(i, j - 1, m)]), r,
+                       o = dy[IDX(i, j, m)];
+                double gij = g[IDX(i, j, m)];
+                if (j == m - 1) {
+                    sumg = 1.0 / RUL(i, j) + gij;
+                    ux[IDX(i, j, m)] = w * (l / RUL(i, j) + gij * o) / sumg +
+                                       (1 - w) * uy[IDX(i, j, m)];
+                } else {
+                    r = uy[IDX(i, j + 1, m)];
+                    sumg = 1.0 / RUL(i, j) + 1.0 / RUL(i, j + 1) + gij;
+                    ux[IDX(i, j, m)] =
+                        w * (l / RUL(i, j) + r / RUL(i, j + 1) + gij * o) / sumg +
+                        (1 - w) * uy[IDX(i, j, m)];
+                }
+                r = (i == n - 1 ? vout[j] : dy[IDX(i + 1, j, m)]);
+                o = ux[IDX(i, j, m)];
+                if (i == 0) {
+                    sumg = 1.0 / RDL(j, 1) + gij;
+                    dx[IDX(i, j, m)] = w * (r / RDL(j, 1) + gij * o) / sumg +
+                                       (1 - w) * dy[IDX(i, j, m)];
+                } else {
+                    l = dx[IDX(i - 1, j, m)];
+                    sumg = 1.0 / RDL(j, i) + 1.0 / RDL(j, i + 1) + gij;
+                    dx[IDX(i, j, m)] =
+                        w * (l / RDL(j, i) + r / RDL(j, i + 1) + gij * o) / sumg +
+                        (1 - w) * dy[IDX(i, j, m)];
+                }
+            }
+        }
+        if (enable_break) {
+            double mx = 0;
+            for (size_t k = 0; k < nm; ++k) {
+                mx = fmax(mx, fabs(dx[k] - dy[k]));
+                mx = fmax(mx, fabs(ux[k] - uy[k]));
+            }
+            if (mx < tol) {
+                record = iter;
+                break;
+            }
+        }
+    }
+#undef RUL
+#undef RDL
+    int final_i = 1 - (iter_number & 1);
+    if (record != -1)
+        final_i = record & 1;
+    memcpy(vup_out, vup[final_i], nm * sizeof(double));
+    memcpy(vdown_out, vdown[final_i], nm * sizeof(double));
+    for (int j = 0; j < m; ++j)
+        iout[j] = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j)
+            iout[j] += (vup_out[IDX(i, j, m)] - vdown_out[IDX(i, j, m)]) *
+                       g[IDX(i, j, m)];
+    *record_out = record;
+    free(vup[0]);
+    free(vup[1]);
+    free(vdown[0]);
+    free(vdown[1]);
+    return 0;
+}
+
+/* Partial-KCL residual sum printed as "error I sum"
+ * (concrete_command.h:1293-1319; 2dArray.cc:1620-1645). */
+double oracle_errisum(int n, int m, double g_wire, const double *g,
+                      const double *vup, const double *vdown)
+{
+    double s = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 1; j < m - 1; ++j)
+            s += fabs((vup[IDX(i, j - 1, m)] - vup[IDX(i, j, m)] +
+                       vup[IDX(i, j + 1, m)] - vup[IDX(i, j, m)]) * g_wire +
+                      (vdown[IDX(i, j, m)] - vup[IDX(i, j, m)]) * g[IDX(i, j, m)]);
+        if (m > 1)
+            s += fabs((vup[IDX(i, m - 2, m)] - vup[IDX(i, m - 1, m)]) * g_wire +
+                      (vdown[IDX(i, m - 1, m)] - vup[IDX(i, m - 1, m)]) *
+                          g[IDX(i, m - 1, m)]);
+    }
+    for (int j = 0; j < m; ++j)
+        s += fabs((vup[IDX(0, j, m)] - vdown[IDX(0, j, m)]) * g[IDX(0, j, m)] +
+                  (n > 1 ? vdown[IDX(1, j, m)] - vdown[IDX(0, j, m)] : 0) * g_wire);
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 0; j < m; ++j)
+            s += fabs((vup[IDX(i, j, m)] - vdown[IDX(i, j, m)]) * g[IDX(i, j, m)] +
+                      (vdown[IDX(i - 1, j, m)] - vdown[IDX(i, j, m)] +
+                       vdown[IDX(i + 1, j, m)] - vdown[IDX(i, j, m)]) * g_wire);
+    return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exact FP64 solve: banded Cholesky                                         */
+/* ------------------------------------------------------------------------ */
+/*
+ * Unknown k = 2*(i*m+j)+t, t=0 wordline node u_ij, t=1 bitline node d_ij.
+ * KCL (the equations nodeBaseGS relaxes):
+ *   (g + a_j + [j<m-1]a_{j+1}) u_ij - a_j u_i,j-1 - [j<m-1]a_{j+1} u_i,j+1 - g d_ij
+ *        = [j=0] a_0 vin_i                       (a_j = 1/RU[i][j], u_i,-1 = vin_i)
+ *   (g + [i>0]b_i + b_{i+1}) d_ij - [i>0]b_i d_i-1,j - [i<n-1]b_{i+1} d_i+1,j - g u_ij
+ *        = [i=n-1] b_n vout_j                    (b_i = 1/RD[j][i])
+ * vout may be NULL (all bitlines grounded).  Half bandwidth kd = 2m.  Row-major lower band: L[k*(kd+1) + (kd - (k-c))]
+ * holds entry (k, c) for c in [k-kd, k].
+ */
+int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
+                 int rd_ld, const double *g, const double *vin,
+                 const double *vout, double *u_out, double *d_out, double *iout)
+{
+    if (n < 1 || m < 1)
+        return -1;
+    const long N = 2L * n * m;
+    const long kd = 2L * m;
+    const long W = kd + 1;
+    double *L = (double *)calloc((size_t)N * W, sizeof(double));
+    double *b = (double *)calloc((size_t)N, sizeof(double));
+    if (!L || !b) {
+        free(L);
+        free(b);
+        return -2;
+    }
+#define BAND(r, c) L[(size_t)(r) * W + (kd - ((r) - (c)))]
+#define RUL(i, j) RU[(size_t)(i) * ru_ld + (j)]
+#define RDL(j, i) RD[(size_t)(j) * rd_ld + (i)]
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            double gij = g[IDX(i, j, m)];
+            long ku = 2L * (i * (long)m + j), kdn = ku + 1;
+            double aj = 1.0 / RUL(i, j);
+            double aj1 = (j < m - 1) ? 1.0 / RUL(i, j + 1) : 0.0;
+            BAND(ku, ku) = gij + aj + aj1;
+            if (j > 0)
+                BAND(ku, ku - 2) = -aj;
+            if (j == 0)
+                b[ku] += aj * vin[i];
+            double bi = (i > 0) ? 1.0 / RDL(j, i) : 0.0;
+            double bi1 = 1.0 / RDL(j, i + 1);
+            BAND(kdn, kdn) = gij + bi + bi1;
+            BAND(kdn, ku) = -gij;
+            if (i > 0)
+                BAND(kdn, kdn - 2L * m) = -bi;
+            if (i == n - 1 && vout)
+                b[kdn] += bi1 * vout[j];
+        }
+#undef RUL
+#undef RDL
+    /* in-place Cholesky A = L L^T */
+    for (long k = 0; k < N; ++k) {
+        long c0 = k - kd < 0 ? 0 : k - kd;
+        /* off-diagonal entries of row k */
+        for (long c = c0; c < k; ++c) {
+            long lo = (c - kd < 0 ? 0 : c - kd);
+            if (lo < c0)
+                lo = c0;
+            double s = BAND(k, c);
+            const double *rk = &BAND(k, lo);
+            const double *rc = &BAND(c, lo);
+            long len = c - lo;
+#pragma omp simd reduction(- : s)
+            for (long t = 0; t < len; ++t)
+                s -= rk[t] * rc[t];
+            BAND(k, c) = s / BAND(c, c);
+        }
+        double s = BAND(k, k);
+        const double *rk = &BAND(k, c0);
+        long len = k - c0;
+#pragma omp simd reduction(- : s)
+        for (long t = 0; t < len; ++t)
+            s -= rk[t] * rk[t];
+        if (!(s > 0)) {
+            free(L);
+            free(b);
+            return -3;
+        }
+        BAND(k, k) = sqrt(s);
+    }
+    /* forward L y = b */
+    for (long k = 0; k < N; ++k) {
+        long c0 = k - kd < 0 ? 0 : k - kd;
+        double s = b[k];
+        for (long c = c0; c < k; ++c)
+            s -= BAND(k, c) * b[c];
+        b[k] = s / BAND(k, k);
+    }
+    /* backward L^T x = y */
+    for (long k = N - 1; k >= 0; --k) {
+        b[k] /= BAND(k, k);
+        double xk = b[k];
+        long c0 = k - kd < 0 ? 0 : k - kd;
+        for (long c = c0; c < k; ++c)
+            b[c] -= BAND(k, c) * xk;
+    }
+#undef BAND
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            u_out[IDX(i, j, m)] = b[2 * IDX(i, j, m)];
+            d_out[IDX(i, j, m)] = b[2 * IDX(i, j, m) + 1];
+        }
+    for (int j = 0; j < m; ++j)
+        iout[j] = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j)
+            iout[j] += (u_out[IDX(i, j, m)] - d_out[IDX(i, j, m)]) * g[IDX(i, j, m)];
+    free(L);
+    free(b);
+    return 0;
+}
+
+/* Batched exact solve for uniform wire resistance r[k] per instance, FP32
+ * conductances (the batched kernel's input format), OpenMP over instances.
+ * Output: iout[k*m + j]. */
+int oracle_exact_batch_f32g(int B, int n, int m, const float *g32,
+                            const double *r, const double *vin,
+                            const double *vout, double *iout)
+{
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+    for (int k = 0; k < B; ++k) {
+        size_t nm = (size_t)n * m;
+        double *RU = (double *)malloc((size_t)n * (m + 1) * sizeof(double));
+        double *RD = (double *)malloc((size_t)m * (n + 1) * sizeof(double));
+        double *g = (double *)malloc(nm * sizeof(double));
+        double *u = (double *)malloc(nm * sizeof(double));
+        double *d = (double *)malloc(nm * sizeof(double));
+        for (size_t t = 0; t < (size_t)n * (m + 1); ++t)
+            RU[t] = r[k];
+        for (size_t t = 0; t < (size_t)m * (n + 1); ++t)
+            RD[t] = r[k];
+        for (size_t t = 0; t < nm; ++t)
+            g[t] = (double)g32[(size_t)k * nm + t];
+        int e = oracle_exact(n, m, RU, m + 1, RD, n + 1, g, vin + (size_t)k * n,
+                             vout ? vout + (size_t)k * m : NULL, u, d,
+                             iout + (size_t)k * m);
+        if (e)
+            err |= 1;
+        free(RU);
+        free(RD);
+        free(g);
+        free(u);
+        free(d);
+    }
+    return err ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 red-black SOR: replay of the GPU drop-in kernel (rb_f64)             */
+/* ------------------------------------------------------------------------ */
+/*
+ * Colours: c(u_ij) = (i+j)&1, c(d_ij) = (i+j+1)&1; half-sweep 0 then 1.
+ * Node update (no fused multiply-add; same order on the GPU):
+ *   u: num = (aL*L + aR*R) + g*d ; u' = wD*num + (1-w)*u
+ *      aL = 1/RU[i][j], aR = [j<m-1] 1/RU[i][j+1], L = j? u_i,j-1 : vin_i,
+ *      R = j<m-1 ? u_i,j+1 : 0, wD = w/((aL+aR)+g)
+ *   d: num = (bU*U + bD*Dn) + g*u ; d' = wD*num + (1-w)*d
+ *      bU = [i>0] 1/RD[j][i], bD = 1/RD[j][i+1], U = i? d_i-1,j : 0,
+ *      Dn = i<n-1 ? d_i+1,j : vout_j, wD = w/((bU+bD)+g)
+ * Stop after the first sweep whose max |x' - x| < tol (reference rule,
+ * concrete_command.h:1256-1272).  iout_j = sum_{i ascending} (u-d)*g.
+ * Returns sweeps performed; *record = 0-based break sweep or -1.
+ */
+int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
+                  int rd_ld, const double *g, const double *vin,
+                  const double *vout, int iter_number, double w,
+                  int enable_break, double tol, double *u, double *d,
+                  double *iout, int *record_out)
+{
+    size_t nm = (size_t)n * m;
+    double *wDu = (double *)malloc(nm * sizeof(double));
+    double *wDd = (double *)malloc(nm * sizeof(double));
+    double *aL = (double *)malloc(nm * sizeof(double));
+    double *aR = (double *)malloc(nm * sizeof(double));
+    double *bU = (double *)malloc(nm * sizeof(double));
+    double *bD = (double *)malloc(nm * sizeof(double));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            size_t k = IDX(i, j, m);
+            aL[k] = 1.0 / RU[(size_t)i * ru_ld + j];
+            aR[k] = (j < m - 1) ? 1.0 / RU[(size_t)i * ru_ld + j + 1] : 0.0;
+            bU[k] = (i > 0) ? 1.0 / RD[(size_t)j * rd_ld + i] : 0.0;
+            bD[k] = 1.0 / RD[(size_t)j * rd_ld + i + 1];
+            wDu[k] = w / ((aL[k] + aR[k]) + g[k]);
+            wDd[k] = w / ((bU[k] + bD[k]) + g[k]);
+            u[k] = vin[i];
+            d[k] = vout[j];
+        }
+    double omw = 1.0 - w;
+    int record = -1, it;
+    for (it = 0; it < iter_number; ++it) {
+        double mx = 0;
+        for (int c = 0; c < 2; ++c)
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < m; ++j) {
+                    size_t k = IDX(i, j, m);
+                    if (((i + j) & 1) == c) {
+                        double L = j ? u[k - 1] : vin[i];
+                        double R = (j < m - 1) ? u[k + 1] : 0.0;
+                        double num = (aL[k] * L + aR[k] * R) + g[k] * d[k];
+                        double nv = wDu[k] * num + omw * u[k];
+                        mx = fmax(mx, fabs(nv - u[k]));
+                        u[k] = nv;
+                    } else {
+                        double U = i ? d[k - m] : 0.0;
+                        double Dn = (i < n - 1) ? d[k + m] : vout[j];
+                        double num = (bU[k] * U + bD[k] * Dn) + g[k] * u[k];
+                        double nv = wDd[k] * num + omw * d[k];
+                        mx = fmax(mx, fabs(nv - d[k]));
+                        d[k] = nv;
+                    }
+                }
+        if (enable_break && mx < tol) {
+            record = it;
+            ++it;
+            break;
+        }
+    }
+    for (int j = 0; j < m; ++j) {
+        double s = 0;
+        for (int i = 0; i < n; ++i)
+            s += (u[IDX(i, j, m)] - d[IDX(i, j, m)]) * g[IDX(i, j, m)];
+        iout[j] = s;
+    }
+    *record_out = record;
+    free(wDu);
+    free(wDd);
+    free(aL);
+    free(aR);
+    free(bU);
+    free(bD);
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 red-black SOR on drop variables: replay of the GPU batched kernel    */
+/* ------------------------------------------------------------------------ */
+/*
+ * Drop variables p = vin_i - u_ij, q = d_ij - vout_j, c = vin_i - vout_j,
+ * uniform wire conductance a = 1/r, hcoef = w/(2a) with w the FP32 omega:
+ *   hn  = (float)(-(g * hcoef)),  rho = (float)((g * c) * hcoef)
+ *   tt  = fmaf(hn, p + q, rho)
+ *   x'  = (cA*x  -> fmaf(cB, N1 + N2, .)) + tt,  cA = 1-w, cB = w/2
+ *   p: N1 = j ? p_i,j-1 : 0 (source),  N2 = j<m-1 ? p_i,j+1 : fmaf(2/w, tt, N1)
+ *   q: N2 = i<n-1 ? q_i+1,j : 0 (sink), N1 = i ? q_i-1,j : fmaf(2/w, tt, N2)
+ * (the ghost at an open end makes the update the exact SOR step there).
+ * Cell (i,j) updates p in half-sweep (i+j)&1 and q in the other one.
+ * Runs exactly `sweeps` sweeps from the given p, q (in/out).
+ */
+void oracle_rb_f32(int n, int m, const float *g, double r, const double *vin,
+                   const double *vout, double w, int sweeps, float *p, float *q)
+{
+    size_t nm = (size_t)n * m;
+    float *hn = (float *)malloc(nm * sizeof(float));
+    float *rho = (float *)malloc(nm * sizeof(float));
+    const float wf = (float)w;
+    const double hcoef = 0.5 * (double)wf * r;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            const double gd = (double)g[IDX(i, j, m)];
+            const double c = vin[i] - (vout ? vout[j] : 0.0);
+            hn[IDX(i, j, m)] = (float)(-(gd * hcoef));
+            rho[IDX(i, j, m)] = (float)((gd * c) * hcoef);
+        }
+    const float cA = 1.0f - wf;
+    const float cB = 0.5f * wf;
+    const float c2 = 2.0f / wf;
+    for (int s = 0; s < sweeps; ++s)
+        for (int col = 0; col < 2; ++col)
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < m; ++j) {
+                    size_t k = IDX(i, j, m);
+                    const float sm = p[k] + q[k];
+                    const float tt = fmaf(hn[k], sm, rho[k]);
+                    if (((i + j) & 1) == col) {
+                        const float n1 = j ? p[k - 1] : 0.0f;
+                        const float n2 = (j < m - 1) ? p[k + 1] : fmaf(c2, tt, n1);
+                        float t2 = cA * p[k];
+                        t2 = fmaf(cB, n1 + n2, t2);
+                        p[k] = t2 + tt;
+                    } else {
+                        const float n2 = (i < n - 1) ? q[k + m] : 0.0f;
+                        const float n1 = i ? q[k - m] : fmaf(c2, tt, n2);
+                        float t2 = cA * q[k];
+                        t2 = fmaf(cB, n1 + n2, t2);
+                        q[k] = t2 + tt;
+                    }
+                }
+    free(hn);
+    free(rho);
+}
+
+/* ------------------------------------------------------------------------ */
+/* PimTorch hook line-GS (split_arr_matmul.cu:27-140, 204-333)               */
+/* ------------------------------------------------------------------------ */
+/*
+ * Crossbar (b, x, y) of a [bsize, numX, numY] grid; row_size x col_size
+ * cells; input[b][x][row]; linear G[x][row][y*col + c] shared over the batch.
+ * Sweep: wordlines left->right (new left, old right, old bitline), then
+ * bitlines bottom->top (new lower, old upper, new wordline); source and sink
+ * conductances equal g_wire (split_arr_matmul.cu:274-275, 290-291).
+ * Nonlinear cells (split_arr_matmul.cu:246-262): before each sweep the cell
+ * conductance becomes 1/R(V) of the voltage difference held in the buffer the
+ * sweep is about to overwrite (two sweeps old; the initial state for the
+ * first two sweeps), with |V| < 1e-5 clamped to +-1e-5 (deal_v :142-166),
+ *   R(V) = (1-z)|V|/(b_on sinh(a_on|V|)) + z|V|/(b_off sinh(a_off|V|)),
+ *   z = max(0, (1/G - Ron)/(Roff - Ron))      (:176-196).
+ * Break: after a sweep, stop when no node moved by more than thr (:295-306).
+ * Output: ret_v ? (u-d)[b][x][y][row][col] : I[b][x][y*col + c].
+ * Returns the number of sweeps run.
+ */
+#define HOOK_GS_BODY(T, SINH, FABS)                                                    \
+    const size_t cells = (size_t)row * col;                                            \
+    const size_t nxb = (size_t)bsize * numX * numY;                                    \
+    T *u0 = (T *)malloc(nxb * cells * sizeof(T)), *u1 = (T *)malloc(nxb * cells * sizeof(T)); \
+    T *d0 = (T *)calloc(nxb * cells, sizeof(T)), *d1 = (T *)calloc(nxb * cells, sizeof(T));  \
+    T *Gc = (T *)malloc(nxb * cells * sizeof(T));                                      \
+    T *z = nonlinear ? (T *)malloc((size_t)numX * numY * cells * sizeof(T)) : NULL;    \
+    const T gw = (T)g_wire, be = (T)beta, omb = (T)1 - be;                             \
+    for (int b = 0; b < bsize; ++b)                                                    \
+        for (int x = 0; x < numX; ++x)                                                 \
+            for (int y = 0; y < numY; ++y)                                             \
+                for (int i = 0; i < row; ++i)                                          \
+                    for (int c = 0; c < col; ++c) {                                    \
+                        size_t k = ((((size_t)b * numX + x) * numY + y) * row + i) * col + c; \
+                        u0[k] = u1[k] = input[((size_t)b * numX + x) * row + i];       \
+                        Gc[k] = G[((size_t)x * row + i) * (size_t)(numY * col) + (size_t)y * col + c]; \
+                    }                                                                  \
+    if (nonlinear)                                                                     \
+        for (int x = 0; x < numX; ++x)                                                 \
+            for (int y = 0; y < numY; ++y)                                             \
+                for (int i = 0; i < row; ++i)                                          \
+                    for (int c = 0; c < col; ++c) {                                    \
+                        T gl = G[((size_t)x * row + i) * (size_t)(numY * col) + (size_t)y * col + c]; \
+                        T zz = ((T)1 / gl - (T)nl[0]) / (T)(nl[1] - nl[0]);            \
+                        if (zz < 0) zz = 0;                                            \
+                        z[(((size_t)x * numY + y) * row + i) * col + c] = zz;          \
+                    }                                                                  \
+    /* cur/prev buffers: the sweep writes cur, reads prev; then swap */                \
+    T *ucur = u0, *uprev = u1, *dcur = d0, *dprev = d1;                                \
+    int it;                                                                            \
+    for (it = 0; it < iter_times; ++it) {                                              \
+        if (nonlinear) {                                                               \
+            for (int b = 0; b < bsize; ++b)                                            \
+                for (int x = 0; x < numX; ++x)                                         \
+                    for (int y = 0; y < numY; ++y)                                     \
+                        for (size_t e = 0; e < cells; ++e) {                           \
+                            size_t k = (((size_t)b * numX + x) * numY + y) * cells + e; \
+                            T V = ucur[k] - dcur[k];                                   \
+                            if (V < (T)1e-5 && V > -(T)1e-5)                           \
+                                V = V > 0 ? (T)1e-5 : -(T)1e-5;                        \
+                            V = FABS(V);                                               \
+                            T zz = z[((size_t)x * numY + y) * cells + e];              \
+                            T Ion = (T)nl[3] * SINH(V * (T)nl[2]);                     \
+                            T Ioff = (T)nl[5] * SINH(V * (T)nl[4]);                    \
+                            T R = ((T)1 - zz) * V / Ion + zz * V / Ioff;               \
+                            Gc[k] = (T)1 / R;                                          \
+                        }                                                              \
+        }                                                                              \
+        for (size_t xb = 0; xb < nxb; ++xb) {                                          \
+            int b = (int)(xb / ((size_t)numX * numY));                                 \
+            int x = (int)((xb / numY) % numX);                                         \
+            T *uc = ucur + xb * cells, *up = uprev + xb * cells;                       \
+            T *dc = dcur + xb * cells, *dp = dprev + xb * cells;                       \
+            const T *Gk = Gc + xb * cells;                                             \
+            for (int i = 0; i < row; ++i) {                                            \
+                T *ucr = uc + (size_t)i * col, *upr = up + (size_t)i * col;            \
+                const T *dpr = dp + (size_t)i * col, *Gr = Gk + (size_t)i * col;       \
+                T left = input[((size_t)b * numX + x) * row + i];                      \
+                T sumg = gw + gw + Gr[0];                                              \
+                ucr[0] = be * (gw * left + gw * upr[1] + Gr[0] * dpr[0]) / sumg + omb * upr[0]; \
+                int j;                                                                 \
+                for (j = 1; j < col - 1; ++j) {                                        \
+                    sumg = gw * 2 + Gr[j];                                             \
+                    ucr[j] = be * (gw * (ucr[j - 1] + upr[j + 1]) + Gr[j] * dpr[j]) / sumg + omb * upr[j]; \
+                }                                                                      \
+                sumg = gw + Gr[j];                                                     \
+                ucr[j] = be * (gw * ucr[j - 1] + Gr[j] * dpr[j]) / sumg + omb * upr[j];\
+            }                                                                          \
+            for (int c = 0; c < col; ++c) {                                            \
+                int i = row - 1;                                                       \
+                T sumg = gw + gw + Gk[(size_t)i * col + c];                            \
+                dc[(size_t)i * col + c] = be * (gw * dp[(size_t)(i - 1) * col + c] + gw * (T)0 + \
+                    Gk[(size_t)i * col + c] * uc[(size_t)i * col + c]) / sumg + omb * dp[(size_t)i * col + c]; \
+                for (i = row - 2; i > 0; --i) {                                        \
+                    sumg = gw * 2 + Gk[(size_t)i * col + c];                           \
+                    dc[(size_t)i * col + c] = be * (gw * (dp[(size_t)(i - 1) * col + c] + dc[(size_t)(i + 1) * col + c]) + \
+                        Gk[(size_t)i * col + c] * uc[(size_t)i * col + c]) / sumg + omb * dp[(size_t)i * col + c]; \
+                }                                                                      \
+                sumg = gw + Gk[c];                                                     \
+                dc[c] = be * (gw * dc[(size_t)col + c] + Gk[c] * uc[c]) / sumg + omb * dp[c]; \
+            }                                                                          \
+        }                                                                              \
+        T *t = ucur; ucur = uprev; uprev = t;                                          \
+        t = dcur; dcur = dprev; dprev = t;                                             \
+        if (enable_break) {                                                            \
+            long cnt = 0;                                                              \
+            for (size_t k = 0; k < nxb * cells; ++k) {                                 \
+                cnt += FABS(ucur[k] - uprev[k]) > (T)thr;                              \
+                cnt += FABS(dcur[k] - dprev[k]) > (T)thr;                              \
+            }                                                                          \
+            if (cnt == 0) { ++it; break; }                                             \
+        }                                                                              \
+    }                                                                                  \
+    if (ret_v) {                                                                       \
+        for (size_t k = 0; k < nxb * cells; ++k)                                       \
+            out[k] = uprev[k] - dprev[k];                                              \
+    } else {                                                                           \
+        for (size_t xb = 0; xb < nxb; ++xb) {                                          \
+            int b = (int)(xb / ((size_t)numX * numY));                                 \
+            int x = (int)((xb / numY) % numX);                                         \
+            int y = (int)(xb % numY);                                                  \
+            for (int c = 0; c < col; ++c) {                                            \
+                T s = 0;                                                               \
+                for (int i = 0; i < row; ++i) {                                       \
+                    size_t k = xb * cells + (size_t)i * col + c;                       \
+                    s += (uprev[k] - dprev[k]) * Gc[k];                                \
+                }                                                                      \
+                out[(((size_t)b * numX + x) * numY + y) * col + c] = s;                \
+            }                                                                          \
+        }                                                                              \
+    }                                                                                  \
+    free(u0); free(u1); free(d0); free(d1); free(Gc); free(z);                         \
+    return it;
+
+int oracle_hook_gs_f64(const double *input, const double *G, int bsize, int row,
+                       int col, int numX, int numY, int iter_times,
+                       double g_wire, double beta, int enable_break, double thr,
+                       int nonlinear, const double *nl, int ret_v, double *out)
+{
+    HOOK_GS_BODY(double, sinh, fabs)
+}
+
+int oracle_hook_gs_f32(const float *input, const float *G, int bsize, int row,
+                       int col, int numX, int numY, int iter_times,
+                       double g_wire, double beta, int enable_break, double thr,
+                       int nonlinear, const double *nl, int ret_v, float *out)
+{
+    HOOK_GS_BODY(float, sinhf, fabsf)
+}
+
+/* CPU baseline: the lexicographic restatement (oracle_nodegs) over a batch of
+ * instances with FP32 conductances, one wire resistance each and grounded
+ * bitlines, OpenMP over instances.  Returns the number of threads used. */
+int oracle_nodegs_batch_f32g(int B, int n, int m, const float *g32, const double *r,
+                             const double *vin, int iters, double w, double tol,
+                             double *iout, int *sweeps)
+{
+    int nthreads = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        {
+#ifdef _OPENMP
+            nthreads = omp_get_num_threads();
+#endif
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < B; ++k) {
+        size_t nm = (size_t)n * m;
+        double *RU = (double *)malloc((size_t)n * (m + 1) * sizeof(double));
+        double *RD = (double *)malloc((size_t)m * (n + 1) * sizeof(double));
+        double *g = (double *)malloc(nm * sizeof(double));
+        double *vo = (double *)calloc((size_t)m, sizeof(double));
+        double *u = (double *)malloc(nm * sizeof(double));
+        double *d = (double *)malloc(nm * sizeof(double));
+        for (size_t t = 0; t < (size_t)n * (m + 1); ++t)
+            RU[t] = r[k];
+        for (size_t t = 0; t < (size_t)m * (n + 1); ++t)
+            RD[t] = r[k];
+        for (size_t t = 0; t < nm; ++t)
+            g[t] = (double)g32[(size_t)k * nm + t];
+        int rec = -1;
+        oracle_nodegs(n, m, RU, m + 1, RD, n + 1, g, vin + (size_t)k * n, vo, iters, w, 1, tol, u, d,
+                      iout + (size_t)k * m, &rec);
+        sweeps[k] = rec >= 0 ? rec + 1 : iters;
+        free(RU);
+        free(RD);
+        free(g);
+        free(vo);
+        free(u);
+        free(d);
+    }
+    return nthreads;
+}

@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Write the golden vectors of tests/golden/ by running the reference CLI.
+
+The reference `sim` binaries (V1 and V2) are compiled from the reference's own
+source files by ``make -C oracle ref`` into oracle/_ref/ (never committed).
+This script writes seeded configs -- explicit ``setCellR`` / ``setLineV`` lines,
+since the reference has no seeded random generator -- runs each binary in a
+scratch directory and stores what it printed (``fastmode_nodebasedGS.out``,
+``default.log`` or stdout) as small .npz fixtures.  Only data leaves this
+script: the inputs (states, voltages, resistances) and the outputs.
+
+Usage: python tests/golden/make_golden.py   (needs /root/reference)
+"""
+from __future__ import annotations
+
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF_BIN = os.path.join(REPO, "oracle", "_ref")
+
+STATES4 = [2e6, 1.6e6, 1.3e6, 1e6]
+
+
+def parse_out(path, n, m):
+    """Parse fastmode_nodebasedGS.out (format of concrete_command.h:1321-1353)."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    errsum = float(lines[0].split("=")[1])
+    k = lines.index("vup = ")
+    vup = [[float(x) for x in lines[k + 1 + i].split()] for i in range(n)]
+    k = lines.index("vdown = ")
+    vdn = [[float(x) for x in lines[k + 1 + i].split()] for i in range(n)]
+    k = lines.index("iout = ")
+    iout = [float(x) for x in lines[k + 1].split()]
+    # keep the printed text of iout too: the strings are the exact digits
+    iout_txt = lines[k + 1].split()
+    return errsum, np.array(vup), np.array(vdn), np.array(iout), iout_txt
+
+
+def write_config(path, version, n, m, states, cell_R, r, vin, vout, gs_args,
+                 seg_up=(), seg_down=()):
+    vin = [float(v) for v in vin]
+    vout = [float(v) for v in vout]
+    r = float(r)
+    L = []
+    L.append("// generated by tests/golden/make_golden.py")
+    L.append(f"arraySize {n} {m}")
+    if version == 1:
+        L.append("selector no")
+    L.append(f"line_resistance {r!r}")
+    for (x, y, z) in seg_up:
+        L.append(f"setLineR up {x} {y} {z!r}")
+    for (x, y, z) in seg_down:
+        L.append(f"setLineR down {x} {y} {z!r}")
+    L.append("setUseLine left -1")
+    L.append("setUseLine down -1")
+    L.append("senseBitlineI down -1")
+    L.append(f"setLineV left -1 {vin[0]!r}")
+    for i, v in enumerate(vin):
+        if v != vin[0]:
+            L.append(f"setLineV left {i} {v!r}")
+    L.append(f"setLineV down -1 {vout[0]!r}")
+    for j, v in enumerate(vout):
+        if v != vout[0]:
+            L.append(f"setLineV down {j} {v!r}")
+    kw = "cellRStates" if version == 2 else "cellRstates"
+    L.append(f"{kw} {len(cell_R)} " + " ".join(repr(float(x)) for x in cell_R))
+    base = int(np.bincount(states.ravel()).argmax())
+    L.append(f"setCellR -1 -1 {base}")
+    for i in range(n):
+        for j in range(m):
+            if states[i, j] != base:
+                L.append(f"setCellR {i} {j} {int(states[i, j])}")
+    L.append("nodebasedGSMethod " + " ".join(str(a) for a in gs_args))
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=(),
+             seg_down=(), keep_fields=True):
+    exe = os.path.join(REF_BIN, f"sim_v{version}")
+    with tempfile.TemporaryDirectory() as td:
+        cfg = os.path.join(td, "cfg")
+        write_config(cfg, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up, seg_down)
+        res = subprocess.run([exe, cfg], cwd=td, capture_output=True, text=True, timeout=3600)
+        text = res.stdout
+        if "Error" in text:
+            raise RuntimeError(f"{name}: reference reported an error:\n{text[:300]}")
+        if version == 2:
+            with open(os.path.join(td, "default.log")) as f:
+                text = f.read()
+        outp = os.path.join(td, "fastmode_nodebasedGS.out")
+        errsum, vup, vdn, iout, iout_txt = parse_out(outp, n, m)
+    mrec = re.findall(r"threshould break, iter times =\s+(\d+)", text)
+    record = int(mrec[-1]) if mrec else -1
+    dur = [float(x) for x in re.findall(r"(?:Duration )?time = ([0-9.eE+-]+) s", text)]
+    RU = np.full((n, m + 1), float(r))
+    RD = np.full((m, n + 1), float(r))
+    for (x, y, z) in seg_up:
+        RU[x, y] = z
+    for (x, y, z) in seg_down:
+        RD[x, y] = z
+    data = dict(version=version, n=n, m=m, states=states.astype(np.uint8),
+                cell_R=np.array(cell_R, float), RU=RU, RD=RD, vin=np.array(vin, float),
+                vout=np.array(vout, float), iters=int(gs_args[0]), omega_text=str(gs_args[1]),
+                enable_break=int(len(gs_args) > 3 and gs_args[3] == "yes"),
+                tol=float(gs_args[4]) if len(gs_args) > 4 else 0.0,
+                record=record, errsum=errsum, iout=iout, iout_txt=np.array(iout_txt),
+                solve_seconds=np.array(dur))
+    if keep_fields:
+        data["vup"] = vup
+        data["vdown"] = vdn
+    else:
+        rows = np.array([0, n // 2, n - 1])
+        data["rows"] = rows
+        data["vup_rows"] = vup[rows]
+        data["vdown_rows"] = vdn[rows]
+        data["vup_rowsum"] = vup.sum(1)
+        data["vdown_rowsum"] = vdn.sum(1)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **data)
+    print(f"{name}: v{version} {n}x{m} record={record} iout[0]={iout[0]:.6e} "
+          f"t={dur}", flush=True)
+
+
+def main():
+    if not (os.path.exists(os.path.join(REF_BIN, "sim_v2"))
+            and os.path.exists(os.path.join(REF_BIN, "sim_v1"))):
+        subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
+
+    # Config 1: 128x128, 4 states, r = 2.93, all wordlines 1 V from the left.
+    rng = np.random.default_rng(20240601)
+    st = rng.integers(0, 4, (128, 128))
+    ones, zeros = [1.0] * 128, [0.0] * 128
+    for tol in ("1e-8", "1e-12"):
+        run_case(f"c1_v2_tol{tol}", 2, 128, 128, st, STATES4, 2.93, ones, zeros,
+                 [100000, 1.95, 0, "yes", tol], keep_fields=False)
+    run_case("c1_v1_tol1e-8", 1, 128, 128, st, STATES4, 2.93, ones, zeros,
+             [100000, 1.95, 0, "yes", "1e-8"], keep_fields=False)
+
+    # Config-2 distribution: vin in {0, 0.2} V, r ~ U[1, 5] ohm, omega 1.97.
+    for k in range(8):
+        rng = np.random.default_rng(1000 + k)
+        st = rng.integers(0, 4, (128, 128))
+        r = float(np.round(rng.uniform(1.0, 5.0), 6))
+        vin = list(np.where(rng.random(128) < 0.5, 0.2, 0.0))
+        run_case(f"c2_v2_{k}", 2, 128, 128, st, STATES4, r, vin, zeros,
+                 [100000, 1.97, 0, "yes", "1e-10"], keep_fields=False)
+
+    # Other sizes (V2 handles square arrays only: RULine aliases RDLine).
+    for n, w, args in ((64, 1.95, [100000, 1.95, 0, "yes", "1e-10"]),
+                       (256, 1.98, [100000, 1.98, 0, "yes", "1e-9"]),
+                       (512, 1.99, [150, 1.99, 0, "no", "0"])):
+        rng = np.random.default_rng(7 + n)
+        st = rng.integers(0, 4, (n, n))
+        vin = list(np.where(rng.random(n) < 0.5, 0.2, 0.0))
+        run_case(f"size{n}_v2", 2, n, n, st, STATES4, 2.93, vin, [0.0] * n, args,
+                 keep_fields=False)
+
+    # Small square cases through V2 with full fields; fixed counts and breaks.
+    smalls = [(2, 2, [100000, 1.95, 0, "yes", "1e-12"]), (3, 3, [17, 1.5, 0]),
+              (4, 4, [100000, 1.8, 0, "yes", "1e-11"]), (8, 8, [37, 1.9, 0]),
+              (16, 16, [100000, 1.9, 0, "yes", "1e-10"]), (8, 8, [0, 1.9, 0])]
+    for idx, (n, m, args) in enumerate(smalls):
+        rng = np.random.default_rng(300 + idx)
+        st = rng.integers(0, 4, (n, m))
+        vin = list(np.round(rng.uniform(0, 1, n), 3))
+        vout = list(np.round(rng.uniform(0, 0.05, m), 3))
+        run_case(f"small{idx}_v2_{n}x{m}", 2, n, m, st, STATES4, 2.93, vin, vout, args)
+
+    # Non-square and per-segment resistances through V1 (ω parsed with stof).
+    v1cases = [(3, 5, [100000, 1.7, 0, "yes", "1e-12"], (), ()),
+               (8, 4, [100000, 1.8, 0, "yes", "1e-12"], ((1, 2, 7.5), (5, 0, 1.25)),
+                ((0, 3, 9.0), (2, 8, 0.5))),
+               (5, 3, [23, 1.6, 0], ((4, 3, 2.0),), ((1, 1, 4.0),)),
+               (2, 7, [100000, 1.9, 0, "yes", "1e-12"], (), ((6, 2, 11.0),))]
+    for idx, (n, m, args, su, sd) in enumerate(v1cases):
+        rng = np.random.default_rng(500 + idx)
+        st = rng.integers(0, 4, (n, m))
+        vin = list(np.round(rng.uniform(0, 1, n), 3))
+        vout = list(np.round(rng.uniform(0, 0.05, m), 3))
+        run_case(f"small{idx}_v1_{n}x{m}", 1, n, m, st, STATES4, 2.93, vin, vout, args,
+                 seg_up=su, seg_down=sd)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,335 @@
+"""Pins of the CPU oracle (test infrastructure) -- no GPU.
+
+* the lexicographic restatement against the golden vectors written by the
+  reference `sim` binaries (printed digits identical, identical sweep count);
+* the exact banded-Cholesky solve against scipy's sparse LU, a 2x2 closed
+  form, KCL/monotonicity invariants and the r -> 0 ideal-MVM limit;
+* the FP32/FP64 red-black replays against the exact solve (fixed point);
+* the hook line-GS restatement against a literal per-node Python replay of
+  split_arr_matmul.cu:27-140 on tiny arrays, and its nonlinear fixed point
+  against a brute-force Newton solve.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from conftest import GOLDEN, STATES4, load_golden
+
+GOLDEN_FILES = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def _case_inputs(z):
+    g = 1.0 / z["cell_R"][z["states"]]
+    w = float(z["omega_text"])
+    if int(z["version"]) == 1:
+        w = float(np.float32(w))  # V1 parses omega with stof (2dArray.cc:65)
+        RU = z["RU"]
+    else:
+        RU = z["RD"]  # V2 reads wordline segments from RDLine (concrete_command.h:1214)
+    return g, w, RU, z["RD"]
+
+
+@pytest.mark.parametrize("path", GOLDEN_FILES, ids=[os.path.basename(p)[:-4] for p in GOLDEN_FILES])
+def test_nodegs_matches_reference_binary(path):
+    z = np.load(path)
+    g, w, RU, RD = _case_inputs(z)
+    vup, vdn, iout, rec = oracle.nodegs(g, z["vin"], z["vout"], RU, RD, int(z["iters"]), w,
+                                        bool(z["enable_break"]), float(z["tol"]))
+    assert rec == int(z["record"])
+    txt = np.array(["%.12e" % x for x in iout])
+    assert (txt == z["iout_txt"]).all()
+    assert oracle.errisum(g, vup, vdn, 1.0 / RD[0, 0]) == pytest.approx(float(z["errsum"]), rel=1e-5)
+    if "vup" in z:
+        np.testing.assert_allclose(vup, z["vup"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(vdn, z["vdown"], rtol=1e-12, atol=1e-300)
+    else:
+        rows = z["rows"]
+        np.testing.assert_allclose(vup[rows], z["vup_rows"], rtol=1e-12)
+        np.testing.assert_allclose(vdn.sum(1), z["vdown_rowsum"], rtol=1e-10, atol=1e-14)
+
+
+def test_nodegs_rejects_degenerate():
+    with pytest.raises(ValueError):
+        oracle.nodegs(np.ones((1, 4)) * 1e-6, [1.0], np.zeros(4), *oracle.segments_uniform(1, 4, 1.0), 10, 1.5)
+
+
+def _scipy_exact(g, vin, vout, RU, RD):
+    n, m = g.shape
+    idx = lambda i, j, t: 2 * (i * m + j) + t
+    rows, cols, vals = [], [], []
+    b = np.zeros(2 * n * m)
+
+    def add(r, c, v):
+        rows.append(r), cols.append(c), vals.append(v)
+    for i in range(n):
+        for j in range(m):
+            u, d = idx(i, j, 0), idx(i, j, 1)
+            aj = 1 / RU[i, j]
+            aj1 = 1 / RU[i, j + 1] if j < m - 1 else 0.0
+            add(u, u, g[i, j] + aj + aj1)
+            add(u, d, -g[i, j])
+            if j > 0:
+                add(u, idx(i, j - 1, 0), -aj)
+            else:
+                b[u] += aj * vin[i]
+            if j < m - 1:
+                add(u, idx(i, j + 1, 0), -aj1)
+            bi = 1 / RD[j, i] if i > 0 else 0.0
+            bi1 = 1 / RD[j, i + 1]
+            add(d, d, g[i, j] + bi + bi1)
+            add(d, u, -g[i, j])
+            if i > 0:
+                add(d, idx(i - 1, j, 1), -bi)
+            if i < n - 1:
+                add(d, idx(i + 1, j, 1), -bi1)
+            else:
+                b[d] += bi1 * vout[j]
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(2 * n * m, 2 * n * m))
+    x = spla.spsolve(A.tocsc(), b)
+    u = x[0::2].reshape(n, m)
+    d = x[1::2].reshape(n, m)
+    return u, d, ((u - d) * g).sum(0)
+
+
+@pytest.mark.parametrize("n,m", [(2, 2), (3, 5), (8, 4), (17, 9), (32, 32)])
+def test_exact_matches_scipy(n, m):
+    rng = np.random.default_rng(n * 100 + m)
+    g = 1.0 / STATES4[rng.integers(0, 4, (n, m))]
+    RU = rng.uniform(1, 5, (n, m + 1))
+    RD = rng.uniform(1, 5, (m, n + 1))
+    vin = rng.uniform(0, 1, n)
+    vout = rng.uniform(0, 0.1, m)
+    u, d, i1 = oracle.exact(g, vin, vout, RU, RD)
+    u2, d2, i2 = _scipy_exact(g, vin, vout, RU, RD)
+    np.testing.assert_allclose(u, u2, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(d, d2, rtol=1e-10, atol=1e-15)
+    np.testing.assert_allclose(i1, i2, rtol=1e-10)
+
+
+def test_exact_2x2_closed_form():
+    """2x2 with one wire resistance: compare with a hand-eliminated dense solve."""
+    g = np.array([[1e-3, 2e-3], [3e-3, 4e-3]])
+    r = 10.0
+    a = 1 / r
+    vin = np.array([1.0, 0.5])
+    # unknown order u00 u01 u10 u11 d00 d01 d10 d11
+    A = np.zeros((8, 8))
+    bvec = np.zeros(8)
+    U = lambda i, j: 2 * i + j
+    D = lambda i, j: 4 + 2 * i + j
+    for i in range(2):
+        A[U(i, 0), U(i, 0)] = g[i, 0] + 2 * a
+        A[U(i, 0), U(i, 1)] = -a
+        A[U(i, 1), U(i, 1)] = g[i, 1] + a
+        A[U(i, 1), U(i, 0)] = -a
+        bvec[U(i, 0)] = a * vin[i]
+        for j in range(2):
+            A[U(i, j), D(i, j)] = -g[i, j]
+            A[D(i, j), U(i, j)] = -g[i, j]
+    for j in range(2):
+        A[D(0, j), D(0, j)] = g[0, j] + a
+        A[D(0, j), D(1, j)] = -a
+        A[D(1, j), D(1, j)] = g[1, j] + 2 * a
+        A[D(1, j), D(0, j)] = -a
+    x = np.linalg.solve(A, bvec)
+    iref = np.array([sum(g[i, j] * (x[U(i, j)] - x[D(i, j)]) for i in range(2)) for j in range(2)])
+    _, _, iout = oracle.exact(g, vin, np.zeros(2), *oracle.segments_uniform(2, 2, r))
+    np.testing.assert_allclose(iout, iref, rtol=1e-13)
+    # KCL readout equality: the current through the sink segment equals I_j
+    np.testing.assert_allclose(iout, x[[D(1, 0), D(1, 1)]] * a, rtol=1e-12)
+
+
+def test_exact_invariants_and_limit():
+    rng = np.random.default_rng(5)
+    n = m = 24
+    g = 1.0 / STATES4[rng.integers(0, 4, (n, m))]
+    vin = rng.uniform(0, 1, n)
+    for r in (2.93, 1e-9):
+        u, d, iout = oracle.exact(g, vin, np.zeros(m), *oracle.segments_uniform(n, m, r))
+        # monotone: 0 <= d <= u <= vin (M-matrix, nonnegative drive)
+        assert (d >= -1e-15).all() and (u - d >= -1e-15).all() and (u <= vin[:, None] + 1e-15).all()
+        # KCL readout equality at the sink segment
+        np.testing.assert_allclose(iout, d[-1] / r, rtol=1e-8 if r > 1 else 1e-3)
+        ideal = vin @ g
+        assert (iout <= ideal * (1 + 1e-12)).all()
+        if r < 1e-6:
+            np.testing.assert_allclose(iout, ideal, rtol=1e-8)  # r -> 0: ideal MVM
+
+
+def test_exact_linearity():
+    rng = np.random.default_rng(6)
+    n, m = 12, 10
+    g = 1.0 / STATES4[rng.integers(0, 4, (n, m))]
+    RU, RD = oracle.segments_uniform(n, m, 3.0)
+    v1, v2 = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    i1 = oracle.exact(g, v1, np.zeros(m), RU, RD)[2]
+    i2 = oracle.exact(g, v2, np.zeros(m), RU, RD)[2]
+    i12 = oracle.exact(g, 2 * v1 - 0.5 * v2, np.zeros(m), RU, RD)[2]
+    np.testing.assert_allclose(i12, 2 * i1 - 0.5 * i2, rtol=1e-10, atol=1e-20)
+
+
+def test_reference_converges_to_exact():
+    """The reference at tol 1e-12 sits 1.6e-10 from the exact solve; at 1e-8, 1.9e-6."""
+    z = load_golden("c1_v2_tol1e-12")
+    g = 1.0 / z["cell_R"][z["states"]]
+    _, _, iex = oracle.exact(g, z["vin"], z["vout"], z["RD"], z["RD"])
+    assert np.max(np.abs(z["iout"] - iex) / np.abs(iex)) < 1e-9
+    z8 = load_golden("c1_v2_tol1e-8")
+    e8 = np.max(np.abs(z8["iout"] - iex) / np.abs(iex))
+    assert 1e-7 < e8 < 5e-6
+
+
+def test_rb_f64_fixed_point():
+    rng = np.random.default_rng(7)
+    n, m = 16, 12
+    g = 1.0 / STATES4[rng.integers(0, 4, (n, m))]
+    RU = rng.uniform(1, 5, (n, m + 1))
+    RD = rng.uniform(1, 5, (m, n + 1))
+    vin = rng.uniform(0, 1, n)
+    vout = rng.uniform(0, 0.05, m)
+    _, _, iex = oracle.exact(g, vin, vout, RU, RD)
+    u, d, iout, rec, sweeps = oracle.rb_f64(g, vin, vout, RU, RD, 100000, 1.8, True, 1e-15)
+    assert rec >= 0 and sweeps == rec + 1
+    np.testing.assert_allclose(iout, iex, rtol=1e-9)
+
+
+def test_rb_f32_fixed_point():
+    """FP32 drop-variable sweeps approach the exact answer to FP32 accuracy."""
+    rng = np.random.default_rng(8)
+    n = m = 32
+    g32 = (1.0 / STATES4[rng.integers(0, 4, (n, m))]).astype(np.float32)
+    vin = np.where(rng.random(n) < 0.5, 0.2, 0.0)
+    r = 3.0
+    iex = oracle.exact_batch(g32[None], r, vin[None])[0]
+    p, q = oracle.rb_f32(g32, r, vin, None, 1.9, 600)
+    gd = g32.astype(np.float64)
+    iout = (gd * (vin[:, None] - p.astype(np.float64) - q.astype(np.float64))).sum(0)
+    assert np.max(np.abs(iout - iex) / np.abs(iex)) < 2e-6
+
+
+# ---------------------------------------------------------------- hook line-GS
+def _hook_literal(inp, G, bsize, row, col, numX, numY, iters, gw, beta, brk, thr):
+    """Literal per-node transcription of the hook's two kernels
+    (split_arr_matmul.cu:27-73 vdown, :75-119 vup) and host loop (:246-306), FP64, linear."""
+    up = np.repeat(np.repeat(inp[:, :, None, :, None], numY, 2), col, 4).astype(np.float64)
+    dn = np.zeros((bsize, numX, numY, row, col))
+    upp, dnp = up.copy(), dn.copy()
+    Gv = G.reshape(numX, row, numY, col)
+    n_done = 0
+    for it in range(iters):
+        for b in range(bsize):
+            for x in range(numX):
+                for y in range(numY):
+                    Gc = lambda i, j: Gv[x, i, y, j]
+                    for rr in range(row):
+                        j = 0
+                        left = inp[b, x, rr]
+                        sumg = gw + gw + Gc(rr, j)
+                        up[b, x, y, rr, j] = beta * (gw * left + gw * upp[b, x, y, rr, j + 1] + Gc(rr, j) * dnp[b, x, y, rr, j]) / sumg + (1 - beta) * upp[b, x, y, rr, j]
+                        for j in range(1, col - 1):
+                            sumg = gw * 2 + Gc(rr, j)
+                            up[b, x, y, rr, j] = beta * (gw * (up[b, x, y, rr, j - 1] + upp[b, x, y, rr, j + 1]) + Gc(rr, j) * dnp[b, x, y, rr, j]) / sumg + (1 - beta) * upp[b, x, y, rr, j]
+                        j = col - 1
+                        sumg = gw + Gc(rr, j)
+                        up[b, x, y, rr, j] = beta * (gw * up[b, x, y, rr, j - 1] + Gc(rr, j) * dnp[b, x, y, rr, j]) / sumg + (1 - beta) * upp[b, x, y, rr, j]
+                    for c in range(col):
+                        i = row - 1
+                        sumg = gw + gw + Gc(i, c)
+                        dn[b, x, y, i, c] = beta * (gw * dnp[b, x, y, i - 1, c] + gw * 0 + Gc(i, c) * up[b, x, y, i, c]) / sumg + (1 - beta) * dnp[b, x, y, i, c]
+                        for i in range(row - 2, 0, -1):
+                            sumg = gw * 2 + Gc(i, c)
+                            dn[b, x, y, i, c] = beta * (gw * (dnp[b, x, y, i - 1, c] + dn[b, x, y, i + 1, c]) + Gc(i, c) * up[b, x, y, i, c]) / sumg + (1 - beta) * dnp[b, x, y, i, c]
+                        sumg = gw + Gc(0, c)
+                        dn[b, x, y, 0, c] = beta * (gw * dn[b, x, y, 1, c] + Gc(0, c) * up[b, x, y, 0, c]) / sumg + (1 - beta) * dnp[b, x, y, 0, c]
+        up, upp = upp, up
+        dn, dnp = dnp, dn
+        n_done += 1
+        if brk and (np.abs(up - upp) > thr).sum() == 0 and (np.abs(dn - dnp) > thr).sum() == 0:
+            break
+    cur = np.zeros((bsize, numX, numY, col))
+    for i in range(row):
+        cur += (upp[:, :, :, i, :] - dnp[:, :, :, i, :]) * np.transpose(Gv[:, i, :, :], (0, 1, 2))[None]
+    return cur.reshape(bsize, numX, numY * col), n_done
+
+
+@pytest.mark.parametrize("row,col,iters,brk", [(2, 2, 7, False), (3, 4, 9, False), (4, 3, 200, True)])
+def test_hook_restatement_matches_literal(row, col, iters, brk):
+    rng = np.random.default_rng(row * 10 + col)
+    bsize, numX, numY = 2, 2, 3
+    inp = rng.uniform(0, 1, (bsize, numX, row))
+    G = rng.uniform(1e-3, 5e-3, (numX, row, numY * col))
+    out, n = oracle.hook_gs(inp, G, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.5, brk, 1e-9, False, [])
+    ref, n2 = _hook_literal(inp, G, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.5, brk, 1e-9)
+    assert n == n2
+    np.testing.assert_allclose(out, ref, rtol=1e-14, atol=1e-18)
+
+
+def test_hook_converged_equals_exact_with_grounded_sink():
+    """With break at 1e-15 the hook's fixed point is the exact network solve
+    (source and sink conductance g_wire, vout = 0)."""
+    rng = np.random.default_rng(11)
+    row, col = 8, 6
+    G = rng.uniform(1e-4, 1e-3, (1, row, col))
+    inp = rng.uniform(0, 1, (1, 1, row))
+    out, n = oracle.hook_gs(inp, G, 1, row, col, 1, 1, 100000, 1 / 2.93, 1.8, True, 1e-15, False, [])
+    _, _, iex = oracle.exact(G[0], inp[0, 0], np.zeros(col), *oracle.segments_uniform(row, col, 2.93))
+    np.testing.assert_allclose(out[0, 0], iex, rtol=1e-10)
+
+
+def _nonlinear_G(V, z, nl):
+    V = np.where((V < 1e-5) & (V > -1e-5), np.where(V > 0, 1e-5, -1e-5), V)
+    V = np.abs(V)
+    Ion = nl[3] * np.sinh(V * nl[2])
+    Ioff = nl[5] * np.sinh(V * nl[4])
+    return 1.0 / ((1 - z) * V / Ion + z * V / Ioff)
+
+
+def test_hook_nonlinear_fixed_point_is_newton_solution():
+    """The Picard fixed point of the nonlinear hook satisfies the nonlinear KCL:
+    checked against a brute-force Newton solve of the 2x3 network."""
+    from scipy.optimize import fsolve
+    row, col = 2, 3
+    Ron, Roff = 1e3, 1e6
+    a_on = a_off = 2.5
+    b_on = 1.0 / (Ron * np.sinh(a_on * 1.0))
+    b_off = 1.0 / (Roff * np.sinh(a_off * 1.0))
+    nl = [Ron, Roff, a_on, b_on, a_off, b_off]
+    Rlin = np.array([[2e3, 5e3, 1e4], [3e3, 8e3, 2e4]])
+    G = (1 / Rlin)[None]
+    inp = np.array([[[0.3, 0.2]]])
+    gw = 1 / 2.93
+    out, n = oracle.hook_gs(inp, G, 1, row, col, 1, 1, 20000, gw, 1.0, True, 1e-14, True, nl)
+    z = np.clip((Rlin - Ron) / (Roff - Ron), 0, None)
+
+    def kcl(x):
+        u = x[:row * col].reshape(row, col)
+        d = x[row * col:].reshape(row, col)
+        V = u - d
+        Icell = _nonlinear_G(V, z, nl) * V
+        res = []
+        for i in range(row):
+            for j in range(col):
+                left = inp[0, 0, i] if j == 0 else u[i, j - 1]
+                f = gw * (left - u[i, j]) - Icell[i, j]
+                if j < col - 1:
+                    f += gw * (u[i, j + 1] - u[i, j])
+                res.append(f)
+        for i in range(row):
+            for j in range(col):
+                below = 0.0 if i == row - 1 else d[i + 1, j]
+                f = gw * (below - d[i, j]) + Icell[i, j]
+                if i > 0:
+                    f += gw * (d[i - 1, j] - d[i, j])
+                res.append(f)
+        return np.array(res) * 1e3
+    x0 = np.concatenate([np.repeat(inp[0, 0][:, None], col, 1).ravel(), np.zeros(row * col)])
+    x = fsolve(kcl, x0, xtol=1e-14)
+    u = x[:row * col].reshape(row, col)
+    d = x[row * col:].reshape(row, col)
+    I = (_nonlinear_G(u - d, z, nl) * (u - d)).sum(0)
+    np.testing.assert_allclose(out[0, 0], I, rtol=1e-6)

@@ -1,0 +1,141 @@
+"""GPU parity of the batched red-black solver (fadesim_solve_batch).
+
+* bit-exact: a fixed number of sweeps against the CPU replay of the same
+  arithmetic (oracle_rb_f32), from a zero and from a random warm start;
+* converged: relative bitline-current error <= 1e-6 against the exact FP64
+  solve, on config-2 instances (4 states, r ~ U[1,5] ohm, vin in {0, 0.2} V)
+  and other sizes, and against the reference CLI's golden vectors;
+* edge cases: exact warm start, nonzero vout, all-zero input, NaN input,
+  max-sweep exhaustion, unsupported shape.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import STATES4, config2_instances, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(cuda, g, r, vin, vout=None, warm=None, **kw):
+    import torch
+    from fadesim_amd.solver import solve_batch
+    t = lambda a, dt: None if a is None else torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=cuda)
+    res = solve_batch(t(g, torch.float32), t(np.asarray(r, dtype=np.float64).reshape(-1), torch.float64),
+                      t(vin, torch.float64), t(vout, torch.float64), t(warm, torch.float32), **kw)
+    torch.cuda.synchronize()
+    return res
+
+
+def _relerr(a, b):
+    return np.max(np.abs(a - b) / np.abs(b))
+
+
+@pytest.mark.parametrize("n,m", [(32, 32), (64, 64), (128, 128), (128, 64)])
+@pytest.mark.parametrize("warm", [False, True])
+def test_fixed_sweeps_bitwise(cuda, n, m, warm):
+    B = 3
+    g, r, vin = config2_instances(B, n, m, seed=n + m + warm)
+    rng = np.random.default_rng(1)
+    vout = rng.uniform(0, 0.01, (B, m))
+    w0 = rng.uniform(0, 1e-3, (B, 2, n, m)).astype(np.float32) if warm else None
+    res = _solve(cuda, g, r, vin, vout, w0, omega=1.97, fixed_sweeps=37, want_field=True)
+    field = res.field.cpu().numpy()
+    iout = res.iout.cpu().numpy()
+    assert (res.sweeps.cpu().numpy() == 37).all()
+    for b in range(B):
+        p0 = None if w0 is None else w0[b, 0]
+        q0 = None if w0 is None else w0[b, 1]
+        p, q = oracle.rb_f32(g[b], r[b], vin[b], vout[b], 1.97, 37, p0, q0)
+        assert np.array_equal(field[b, 0], p), "p field differs from the CPU replay"
+        assert np.array_equal(field[b, 1], q), "q field differs from the CPU replay"
+        gd = g[b].astype(np.float64)
+        ref = (gd * (vin[b][:, None] - vout[b][None, :] - p.astype(np.float64) - q.astype(np.float64))).sum(0)
+        np.testing.assert_allclose(iout[b], ref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_converged_config2_vs_exact(cuda, n):
+    B = 16 if n == 128 else 32
+    g, r, vin = config2_instances(B, n, seed=100 + n)
+    iex = oracle.exact_batch(g, r, vin)
+    res = _solve(cuda, g, r, vin, tol=1e-6)
+    st = res.status.cpu().numpy()
+    assert (st == 0).all(), st
+    err = _relerr(res.iout.cpu().numpy(), iex)
+    assert err <= 1e-6, err
+    est = res.est.cpu().numpy()
+    assert (est <= 0.5e-6).all()
+
+
+def test_matches_reference_golden_config2(cuda):
+    """Against the reference CLI (tol 1e-10, ~1e-8 from exact) on its own inputs."""
+    for k in range(8):
+        z = load_golden(f"c2_v2_{k}")
+        g = (1.0 / z["cell_R"][z["states"]]).astype(np.float32)
+        r = float(z["RD"][0, 0])
+        res = _solve(cuda, g[None], [r], z["vin"][None], tol=1e-6)
+        assert _relerr(res.iout.cpu().numpy()[0], z["iout"]) <= 1.1e-6
+
+
+def test_matches_reference_golden_config1(cuda):
+    """Config 1 (128x128, 4 states, r = 2.93, 1 V): reference at tol 1e-12."""
+    z = load_golden("c1_v2_tol1e-12")
+    g = (1.0 / z["cell_R"][z["states"]]).astype(np.float32)
+    res = _solve(cuda, g[None], [2.93], z["vin"][None], tol=1e-7)
+    # the reference's conductances are FP64 1/R; the kernel's are their FP32
+    # roundings, which moves I by < 1e-7 relative
+    assert _relerr(res.iout.cpu().numpy()[0], z["iout"]) <= 2e-7
+
+
+def test_vout_and_exact_warm_start(cuda):
+    B, n = 4, 64
+    g, r, vin = config2_instances(B, n, seed=7)
+    rng = np.random.default_rng(3)
+    vout = rng.uniform(0, 0.02, (B, n))
+    iex = np.stack([oracle.exact(g[b].astype(np.float64), vin[b], vout[b], *oracle.segments_uniform(n, n, r[b]))[2]
+                    for b in range(B)])
+    res = _solve(cuda, g, r, vin, vout, tol=1e-6)
+    assert _relerr(res.iout.cpu().numpy(), iex) <= 1e-6
+    # warm start = the exact drops: converged at the first test
+    warm = np.zeros((B, 2, n, n), np.float32)
+    for b in range(B):
+        u, d, _ = oracle.exact(g[b].astype(np.float64), vin[b], vout[b], *oracle.segments_uniform(n, n, r[b]))
+        warm[b, 0] = vin[b][:, None] - u
+        warm[b, 1] = d - vout[b][None, :]
+    res2 = _solve(cuda, g, r, vin, vout, warm, tol=1e-6)
+    assert (res2.sweeps.cpu().numpy() <= 32).all()
+    assert _relerr(res2.iout.cpu().numpy(), iex) <= 1e-6
+
+
+def test_dwr_correction(cuda):
+    B, n = 8, 128
+    g, r, vin = config2_instances(B, n, seed=21)
+    iex = oracle.exact_batch(g, r, vin)
+    plain = _solve(cuda, g, r, vin, tol=1e-6)
+    corr = _solve(cuda, g, r, vin, tol=1e-6, dwr=True)
+    assert _relerr(corr.iout.cpu().numpy(), iex) <= 1e-6
+    assert corr.sweeps.cpu().numpy().mean() < plain.sweeps.cpu().numpy().mean()
+
+
+def test_zero_input_and_nonfinite_and_maxit(cuda):
+    B, n = 3, 32
+    g, r, vin = config2_instances(B, n, seed=5)
+    vin[0] = 0.0
+    g[1, 3, 4] = np.nan
+    res = _solve(cuda, g, r, vin, tol=1e-6, max_sweeps=20000)
+    st = res.status.cpu().numpy()
+    iout = res.iout.cpu().numpy()
+    assert st[0] == 0 and (iout[0] == 0).all()
+    assert st[1] == 2
+    assert st[2] == 0
+    res = _solve(cuda, g[2:], r[2:], vin[2:], tol=1e-12, max_sweeps=40)
+    assert res.status.cpu().numpy()[0] == 1 and res.sweeps.cpu().numpy()[0] == 40
+
+
+def test_unsupported_shape_raises(cuda):
+    import torch
+    from fadesim_amd.solver import solve_batch
+    g = torch.ones((1, 48, 48), device=cuda)
+    with pytest.raises(ValueError):
+        solve_batch(g, 1.0, torch.zeros((1, 48), device=cuda))
