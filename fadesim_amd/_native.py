@@ -22,7 +22,7 @@ class BatchDesc(ctypes.Structure):
     """Mirror of ``fadesim_batch_desc``."""
     _fields_ = [
         ("batch", ctypes.c_int), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
-        ("d_g", _vp), ("d_r_wire", _vp), ("d_vin", _vp), ("d_vout", _vp), ("d_warm", _vp),
+        ("d_g", _vp), ("d_gidx", _vp), ("d_r_wire", _vp), ("d_vin", _vp), ("d_vout", _vp), ("d_warm", _vp),
         ("omega", ctypes.c_double), ("tol", ctypes.c_double), ("safety", ctypes.c_double),
         ("switch_tol", ctypes.c_double), ("max_sweeps", ctypes.c_int),
         ("flags", ctypes.c_uint),
@@ -61,6 +61,10 @@ EXPORTED = [
     "fadesim_last_error", "fadesim_version", "fadesim_solve_batch", "fadesim_default_omega",
     "fadesim_last_kernel_ms",
     "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs",
+    "fadesim_nn_features", "fadesim_nn_combine", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
+    "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
+    "fadesim_nn_up2_bwd", "fadesim_nn_relu_mask", "fadesim_nn_add", "fadesim_nn_mse", "fadesim_nn_adam",
+    "fadesim_nn_cast",
 ]
 
 _LIB = None
@@ -88,8 +92,34 @@ def lib():
         if hasattr(L, "fadesim_hook_ir_drop_gs"):
             L.fadesim_hook_ir_drop_gs.argtypes = [ctypes.POINTER(HookDesc), _vp]
             L.fadesim_hook_ir_drop_gs.restype = ctypes.c_int
+        _declare_nn(L)
         _LIB = L
     return _LIB
+
+
+def _declare_nn(L):
+    i, f, sz = ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+    sigs = {
+        "fadesim_nn_features": [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "fadesim_nn_combine": [i, i, i, _vp, _vp, _vp, i, _vp, _vp],
+        "fadesim_nn_conv3x3_fwd": [i, i, i, i, i, i, _vp, _vp, _vp, i, _vp],
+        "fadesim_nn_flip_weights": [i, i, i, i, _vp, _vp, _vp],
+        "fadesim_nn_conv3x3_wgrad": [i, i, i, i, i, i, _vp, _vp, _vp, _vp],
+        "fadesim_nn_pool2": [i, i, i, i, _vp, _vp, _vp],
+        "fadesim_nn_pool2_bwd": [i, i, i, i, _vp, i, _vp, _vp],
+        "fadesim_nn_up2_add": [i, i, i, i, _vp, _vp, _vp, _vp],
+        "fadesim_nn_up2_bwd": [i, i, i, i, _vp, _vp, _vp],
+        "fadesim_nn_relu_mask": [sz, _vp, _vp, _vp, _vp],
+        "fadesim_nn_add": [sz, _vp, _vp, _vp, _vp],
+        "fadesim_nn_mse": [i, i, i, _vp, i, _vp, _vp, _vp, _vp],
+        "fadesim_nn_adam": [sz, _vp, _vp, _vp, _vp, _vp, f, f, f, f, i, f, f, _vp],
+        "fadesim_nn_cast": [sz, _vp, i, _vp, _vp],
+    }
+    for name, args in sigs.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
 
 
 def check(rc: int, what: str) -> None:

@@ -22,10 +22,17 @@ struct DevBuf {
     size_t bytes = 0;
 };
 std::mutex g_ws_mutex;
-std::vector<DevBuf> g_ws; // indexed by device ordinal
+std::vector<DevBuf> g_ws;     // indexed by device ordinal
+std::vector<DevBuf> g_ws_aux; // indexed by device ordinal
+
+void *grow(std::vector<DevBuf> &pool, size_t bytes);
 } // namespace
 
-void *workspace(size_t bytes)
+void *workspace(size_t bytes) { return grow(g_ws, bytes); }
+void *workspace_aux(size_t bytes) { return grow(g_ws_aux, bytes); }
+
+namespace {
+void *grow(std::vector<DevBuf> &pool, size_t bytes)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
@@ -33,9 +40,9 @@ void *workspace(size_t bytes)
         return nullptr;
     }
     std::lock_guard<std::mutex> lk(g_ws_mutex);
-    if ((int)g_ws.size() <= dev)
-        g_ws.resize(dev + 1);
-    DevBuf &b = g_ws[dev];
+    if ((int)pool.size() <= dev)
+        pool.resize(dev + 1);
+    DevBuf &b = pool[dev];
     if (b.bytes >= bytes && b.ptr)
         return b.ptr;
     if (b.ptr) {
@@ -56,6 +63,7 @@ void *workspace(size_t bytes)
     b.bytes = want;
     return b.ptr;
 }
+} // namespace
 
 } // namespace fadesim
 

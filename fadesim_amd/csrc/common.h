@@ -16,6 +16,9 @@ int set_error(int code, const std::string &msg);
 // every call on that device; calls on one device are stream-ordered by the
 // caller.
 void *workspace(size_t bytes);
+// A second, independent buffer with the same rules (for kernels that run
+// while the first one is in use by the same call).
+void *workspace_aux(size_t bytes);
 
 #define FADESIM_HIP_CHECK(expr)                                                   \
     do {                                                                          \

@@ -49,11 +49,13 @@ namespace {
 struct RbArgs {
     int batch;
     const float *g;
+    const int *gidx;   // optional: instance b uses conductance slot gidx[b]
+    int *queue;        // work counter (zeroed before the launch)
     const double *rw;
     const double *vin;
     const double *vout; // never null (zeros supplied by the host)
     const float *warm;
-    float *ws; // per instance [7][M][N]: zq | rho p,q | snapshot p,q | residual p,q
+    float *ws; // per workgroup slot [7][M][N]: zq | rho p,q | snapshot p,q | residual p,q
     float omega;
     double tol_eff;    // tol / safety
     double switch_tol; // refinement switch point
@@ -69,35 +71,13 @@ struct RbArgs {
 
 constexpr int kWsPlanes = 7;
 
-// zq(i,j) = r * sum_i' g(i',j) * (n - max(i,i')): the potential of bitline
-// node (i,j) per unit current drawn through every cell of column j.
-__global__ void zq_kernel(const float *__restrict__ g, const double *__restrict__ rw,
-                          float *__restrict__ ws, int M, int N)
-{
-    const int b = blockIdx.x;
-    const int j = threadIdx.x;
-    if (j >= N)
-        return;
-    const float *gb = g + (size_t)b * M * N;
-    float *zb = ws + (size_t)b * kWsPlanes * M * N;
-    double T = 0.0;
-    for (int i = 0; i < M; ++i)
-        T += (double)gb[(size_t)i * N + j] * (double)(M - i);
-    double A = 0.0, Bs = T;
-    const double r = rw[b];
-    for (int i = 0; i < M; ++i) {
-        const double gi = (double)gb[(size_t)i * N + j];
-        A += gi;
-        Bs -= gi * (double)(M - i);
-        zb[(size_t)i * N + j] = (float)(r * ((double)(M - i) * A + Bs));
-    }
-}
-
 __global__ void fill_zero_f64(double *p, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
 }
+
+__global__ void zero_counter(int *c) { *c = 0; }
 
 __device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
 __device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
@@ -239,7 +219,7 @@ __device__ __forceinline__ void publish_all(const float (&yp)[G::TR][G::TC],
 // rho = (w/2a) g c for both nodes of a cell; after it rho is the scaled FP64
 // residual the refinement step stored in the workspace.
 template <class G>
-__device__ __forceinline__ void load_coeffs(const RbArgs &A, const float *wsb, size_t base, int b,
+__device__ __forceinline__ void load_coeffs(const RbArgs &A, const float *wsb, const float *gp, int b,
                                             int i0, int j0, double hcoef, bool refined,
                                             float (&hn)[G::TR][G::TC], float (&rp)[G::TR][G::TC],
                                             float (&rq)[G::TR][G::TC])
@@ -254,7 +234,7 @@ __device__ __forceinline__ void load_coeffs(const RbArgs &A, const float *wsb, s
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
         float g[TC];
-        ldrow<TC>(A.g + base + (size_t)(i0 + ii) * N + j0, g);
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, g);
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj)
             hn[ii][jj] = (float)(-((double)g[jj] * hcoef));
@@ -283,7 +263,7 @@ __device__ __forceinline__ void load_coeffs(const RbArgs &A, const float *wsb, s
 // scaled residual as the refinement right-hand side instead of testing.
 // Results: colI[j], colD[j] (threads t < N), return value = max_j |D_j|/|I_j|.
 template <class G>
-__device__ double conv_test(const RbArgs &A, float *wsb, size_t base, int b, int t, double a,
+__device__ double conv_test(const RbArgs &A, float *wsb, const float *gp, int b, int t, double a,
                             double rwire, double hcoef, bool refined, bool want_rho,
                             const double *S1, const double *Ifloor, double *colI, double *colD,
                             float *wmax)
@@ -293,7 +273,7 @@ __device__ double conv_test(const RbArgs &A, float *wsb, size_t base, int b, int
     const float *rho_p = wsb + 1 * M * N, *rho_q = wsb + 2 * M * N;
     const float *fp = wsb + 3 * M * N, *fq = wsb + 4 * M * N;
     float *resp = wsb + 5 * M * N, *resq = wsb + 6 * M * N;
-    const float *g = A.g + base;
+    const float *g = gp;
     // 1. residuals, one node pair per thread and step
     for (int k = t; k < M * N; k += NT) {
         const int i = k / N, j = k - i * N;
@@ -367,6 +347,8 @@ __device__ double conv_test(const RbArgs &A, float *wsb, size_t base, int b, int
 #pragma unroll
     for (int w = 0; w < NW; ++w)
         em = fmaxf(em, wmax[w]);
+    // identical in every lane: make it a scalar
+    em = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, em)));
     return (double)em;
 }
 
@@ -382,24 +364,26 @@ __device__ __forceinline__ void snapshot(float *wsb, const float (&yp)[G::TR][G:
     }
 }
 
-template <class G>
-__global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
-{
-    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY, NW = G::NW;
-    constexpr int M = G::M, N = G::N;
+struct RbShared {
+    float *halo;
+    double *S1, *Ifloor, *colI, *colD;
+    float *wmax;
+};
 
-    __shared__ float halo[G::HALO];
-    __shared__ double S1[N]; // readout base: I = S1 - sum g (p + q)
-    __shared__ double Ifloor[N];
-    __shared__ double colI[N];
-    __shared__ double colD[N];
-    __shared__ float wmax[NW];
+// Solve instance b with this workgroup's workspace slot wsb.
+template <class G>
+__device__ __forceinline__ void solve_instance(const RbArgs &A, int b, float *wsb, const RbShared &sh)
+{
+    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY;
+    constexpr int M = G::M, N = G::N;
+    float *halo = sh.halo;
+    double *S1 = sh.S1, *Ifloor = sh.Ifloor, *colI = sh.colI, *colD = sh.colD;
+    float *wmax = sh.wmax;
 
     const int t = threadIdx.x;
     const int tx = t % TX, ty = t / TX;
-    const int b = blockIdx.x;
     const size_t base = (size_t)b * M * N;
-    float *wsb = A.ws + (size_t)b * kWsPlanes * M * N;
+    const float *gp = A.g + (size_t)(A.gidx ? A.gidx[b] : b) * M * N;
     const int i0 = TR * ty, j0 = TC * tx;
     const bool has_l = tx > 0, has_r = tx < TX - 1, has_u = ty > 0, has_d = ty < TY - 1;
 
@@ -413,15 +397,24 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
     const bool fixed = (A.flags & FADESIM_F_FIXED_SWEEPS) != 0;
     const bool dwr = (A.flags & FADESIM_F_DWR) != 0;
 
-    // ideal readout and the per-column floor of the relative test
+    // ideal readout, the per-column floor of the relative test and the
+    // bitline adjoint zq(i,j) = r sum_i' g(i',j) (n - max(i,i'))
     if (t < N) {
-        double si = 0.0, sf = 0.0;
+        double si = 0.0, sf = 0.0, T = 0.0;
         const double vo = A.vout[(size_t)b * N + t];
         for (int i = 0; i < M; ++i) {
-            const double gd = (double)A.g[base + (size_t)i * N + t];
+            const double gd = (double)gp[(size_t)i * N + t];
             const double c = A.vin[(size_t)b * M + i] - vo;
             si += gd * c;
             sf += gd * fabs(c);
+            T += gd * (double)(M - i);
+        }
+        double Ac = 0.0, Bs = T;
+        for (int i = 0; i < M; ++i) {
+            const double gd = (double)gp[(size_t)i * N + t];
+            Ac += gd;
+            Bs -= gd * (double)(M - i);
+            wsb[(size_t)i * N + t] = (float)(rwire * ((double)(M - i) * Ac + Bs));
         }
         S1[t] = si;
         // a column carrying under 1e-9 of its driven current is judged absolutely
@@ -454,7 +447,7 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
     const double stop_tol = dwr ? A.tol_eff * A.dwr_scale : A.tol_eff;
 
     for (;;) {
-        load_coeffs<G>(A, wsb, base, b, i0, j0, hcoef, refined, hn, rp, rq);
+        load_coeffs<G>(A, wsb, gp, b, i0, j0, hcoef, refined, hn, rp, rq);
         while (sweep < next_check) {
             half_sweep<0, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
             half_sweep<1, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
@@ -462,7 +455,7 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         }
         snapshot<G>(wsb, yp, yq, i0, j0);
         __syncthreads();
-        est = conv_test<G>(A, wsb, base, b, t, a, rwire, hcoef, refined, false, S1, Ifloor, colI,
+        est = conv_test<G>(A, wsb, gp, b, t, a, rwire, hcoef, refined, false, S1, Ifloor, colI,
                            colD, wmax);
         if (fixed) {
             st = FADESIM_ST_CONVERGED;
@@ -474,7 +467,7 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         }
         if (refine_pending && est <= A.switch_tol) {
             // ---- one iterative-refinement step: rhs <- exact FP64 residual ----
-            conv_test<G>(A, wsb, base, b, t, a, rwire, hcoef, false, true, S1, Ifloor, colI, colD,
+            conv_test<G>(A, wsb, gp, b, t, a, rwire, hcoef, false, true, S1, Ifloor, colI, colD,
                          wmax);
             if (A.field) {
 #pragma unroll
@@ -544,15 +537,71 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
             strow<TC>(Q, qv);
         }
     }
+    __syncthreads(); // colI / halo reads done before the next instance reuses them
+}
+
+template <class G>
+__global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
+{
+    __shared__ float halo[G::HALO];
+    __shared__ double S1[G::N];
+    __shared__ double Ifloor[G::N];
+    __shared__ double colI[G::N];
+    __shared__ double colD[G::N];
+    __shared__ float wmax[G::NW];
+    __shared__ int s_inst;
+    const RbShared sh{halo, S1, Ifloor, colI, colD, wmax};
+    float *wsb = A.ws + (size_t)blockIdx.x * kWsPlanes * G::M * G::N;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_inst = atomicAdd(A.queue, 1);
+        __syncthreads();
+        // wave-uniform: keep the instance index and everything derived from it in SGPRs
+        const int b = __builtin_amdgcn_readfirstlane(s_inst);
+        __syncthreads();
+        if (b >= A.batch)
+            break;
+        solve_instance<G>(A, b, wsb, sh);
+    }
 }
 
 thread_local hipEvent_t t_ev[2] = {nullptr, nullptr};
 thread_local bool t_ev_valid = false;
 
+// Resident workgroups of rbsor_kernel<G> on the current device (persistent grid).
 template <class G>
-int launch_rb(const RbArgs &args, int M, int N, hipStream_t s)
+int resident_slots()
 {
-    hipLaunchKernelGGL(zq_kernel, dim3(args.batch), dim3(N), 0, s, args.g, args.rw, args.ws, M, N);
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev >= 64)
+        return 256;
+    if (!cached[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rbsor_kernel<G>, G::NT, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        cached[dev] = per_cu * cus;
+    }
+    return cached[dev];
+}
+
+template <class G>
+int launch_rb(RbArgs args, int M, int N, hipStream_t s)
+{
+    (void)M;
+    (void)N;
+    int grid = resident_slots<G>();
+    if (grid > args.batch)
+        grid = args.batch;
+    const size_t ws_bytes = (size_t)grid * kWsPlanes * G::M * G::N * sizeof(float) + 256;
+    char *ws = (char *)workspace_aux(ws_bytes);
+    if (!ws)
+        return FADESIM_E_NOMEM;
+    args.ws = (float *)ws;
+    args.queue = (int *)(ws + (size_t)grid * kWsPlanes * G::M * G::N * sizeof(float));
+    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, args.queue);
     FADESIM_HIP_CHECK(hipGetLastError());
     const bool timing = (args.flags & FADESIM_F_TIMING) != 0;
     if (timing) {
@@ -562,7 +611,7 @@ int launch_rb(const RbArgs &args, int M, int N, hipStream_t s)
         }
         FADESIM_HIP_CHECK(hipEventRecord(t_ev[0], s));
     }
-    hipLaunchKernelGGL((rbsor_kernel<G>), dim3(args.batch), dim3(G::NT), 0, s, args);
+    hipLaunchKernelGGL((rbsor_kernel<G>), dim3(grid), dim3(G::NT), 0, s, args);
     FADESIM_HIP_CHECK(hipGetLastError());
     if (timing) {
         FADESIM_HIP_CHECK(hipEventRecord(t_ev[1], s));
@@ -621,13 +670,11 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
         return set_error(FADESIM_E_INVALID, "safety must be >= 1");
     hipStream_t s = (hipStream_t)stream;
 
-    const size_t ws_floats = (size_t)B * kWsPlanes * M * N;
-    char *ws = (char *)workspace(ws_floats * sizeof(float) + (size_t)B * N * sizeof(double));
-    if (!ws)
-        return FADESIM_E_NOMEM;
     const double *vout = d->d_vout;
     if (!vout) {
-        double *z = (double *)(ws + ws_floats * sizeof(float));
+        double *z = (double *)workspace((size_t)B * N * sizeof(double));
+        if (!z)
+            return FADESIM_E_NOMEM;
         hipLaunchKernelGGL(fill_zero_f64, dim3(256), dim3(256), 0, s, z, (size_t)B * N);
         FADESIM_HIP_CHECK(hipGetLastError());
         vout = z;
@@ -636,11 +683,13 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     RbArgs args;
     args.batch = B;
     args.g = d->d_g;
+    args.gidx = d->d_gidx;
+    args.queue = nullptr;
     args.rw = d->d_r_wire;
     args.vin = d->d_vin;
     args.vout = vout;
     args.warm = d->d_warm;
-    args.ws = (float *)ws;
+    args.ws = nullptr;
     args.dwr_scale = 16.0;
     args.omega = (float)w;
     args.tol_eff = d->tol / safety;

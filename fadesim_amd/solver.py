@@ -49,26 +49,33 @@ def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
 
 
 def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
-                warm: Optional[torch.Tensor] = None, *, omega: Optional[float] = None,
+                warm: Optional[torch.Tensor] = None, *, gidx: Optional[torch.Tensor] = None,
+                omega: Optional[float] = None,
                 tol: float = 1e-6, safety: float = 2.0, switch_tol: float = 2e-5,
                 max_sweeps: int = 20000, refine: bool = True, dwr: bool = False,
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
                 out: Optional[torch.Tensor] = None, timing: bool = False) -> SolveResult:
     """Solve B crossbars of one shape.
 
-    g [B, M, N] float32 (S), r_wire [B] or scalar (ohm), vin [B, M] (V),
+    g [B, M, N] float32 (S) -- or [S, M, N] conductance slots with ``gidx`` [B]
+    int32 naming each instance's slot; r_wire [B] or scalar (ohm), vin [B, M] (V),
     vout [B, N] or None (0 V), warm [B, 2, M, N] float32 initial drops or None.
     ``fixed_sweeps=k`` runs exactly k sweeps with no test and no refinement.
     """
     if g.dim() != 3:
         raise ValueError("g must be [B, M, N]")
-    B, M, N = g.shape
+    S, M, N = g.shape
+    B = S if gidx is None else int(gidx.numel())
     if (M, N) not in SUPPORTED_SHAPES:
         raise ValueError(f"shape {M}x{N} not supported by the batched kernel: {sorted(SUPPORTED_SHAPES)}")
     if not g.is_cuda:
         raise ValueError("g must be a GPU tensor")
     dev = g.device
     g = g.to(torch.float32).contiguous()
+    if gidx is not None:
+        gidx = gidx.to(device=dev, dtype=torch.int32).reshape(B).contiguous()
+        if B and (int(gidx.min()) < 0 or int(gidx.max()) >= S):
+            raise ValueError("gidx out of range")
     if not torch.is_tensor(r_wire):
         r_wire = torch.full((B,), float(r_wire), dtype=torch.float64, device=dev)
     r_wire = r_wire.to(device=dev, dtype=torch.float64).reshape(-1).expand(B).contiguous()
@@ -98,6 +105,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     d = nat.BatchDesc()
     d.batch, d.rows, d.cols = B, M, N
     d.d_g, d.d_r_wire, d.d_vin = g.data_ptr(), r_wire.data_ptr(), vin.data_ptr()
+    d.d_gidx = _ptr(gidx)
     d.d_vout, d.d_warm = _ptr(vout), _ptr(warm)
     d.omega = float(omega) if omega else 0.0
     d.tol, d.safety, d.switch_tol = float(tol), float(safety), float(switch_tol)

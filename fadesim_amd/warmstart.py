@@ -1,12 +1,381 @@
-"""Learned warm start for the batched solver (placeholder until trained
-weights exist: load_default returns None)."""
+"""Learned warm start for the batched IR-drop solver.
+
+Pipeline (all on the GPU, every kernel from include/fadesim_nn.h):
+
+1. physics features of each crossbar: first- and second-order
+   superposition drop fields (exact prefix-sum evaluations of the wire
+   Green's functions), conductance, inputs, wire resistance -- 8 channels;
+2. a BF16 4-level convolutional encoder-decoder (U-Net with additive skips,
+   3x3 convolutions as MFMA implicit GEMMs, FP32 accumulation, ~340k
+   parameters) predicting the remaining correction of the wordline and
+   bitline drop fields, in units of the second-order term;
+3. warm = second-order field + scale * correction, handed to
+   ``solve_batch(..., warm=...)``; the solver's convergence test judges it
+   like any other initial guess and refines it to the same tolerance.
+
+Training data are solutions of ``solve_batch`` only (field output at a tight
+tolerance); the network is trained by the project's own forward and backward
+kernels with Adam on FP32 master weights.  Weights are stored as raw
+little-endian FP32 plus a JSON manifest in fadesim_amd/weights/.
+
+There is no counterpart in the reference; its related learned component is
+the SCN effective-conductance predictor (pimtorch/pimfixedpoint/SCN.py:127-153,
+trained on solver output in SCN_data_gen.py:134-179).
+"""
 from __future__ import annotations
 
+import ctypes
+import json
+import math
 import os
+import time
+from dataclasses import dataclass
+from typing import Dict, List, Optional
+
+import torch
+
+from . import _native as nat
 
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+DEFAULT_NAME = "warmstart_128"
+N_FEAT = 8
+OUT_C = 16  # only channels 0 (wordline) and 1 (bitline) are used
 
 
-def load_default(device):
-    """The trained warm-start network, or None when no weights are present."""
-    return None
+def kpad(cin: int) -> int:
+    """Padded GEMM depth: 9*Cin weights, one bias column, up to a multiple of 32."""
+    return ((9 * cin + 1 + 31) // 32) * 32
+
+
+@dataclass
+class Op:
+    name: str
+    kind: str        # conv | pool | up_add
+    src: str
+    level: int       # resolution level of the output (0 = full)
+    cin: int = 0
+    cout: int = 0
+    relu: bool = True
+    skip: str = ""   # up_add: tensor added after upsampling
+
+
+def architecture() -> List[Op]:
+    C = [16, 32, 64, 96]
+    return [
+        Op("e0a", "conv", "x", 0, N_FEAT, C[0]),
+        Op("e0", "conv", "e0a", 0, C[0], C[0]),
+        Op("p1", "pool", "e0", 1, C[0], C[0]),
+        Op("e1a", "conv", "p1", 1, C[0], C[1]),
+        Op("e1", "conv", "e1a", 1, C[1], C[1]),
+        Op("p2", "pool", "e1", 2, C[1], C[1]),
+        Op("e2a", "conv", "p2", 2, C[1], C[2]),
+        Op("e2", "conv", "e2a", 2, C[2], C[2]),
+        Op("p3", "pool", "e2", 3, C[2], C[2]),
+        Op("e3a", "conv", "p3", 3, C[2], C[3]),
+        Op("e3", "conv", "e3a", 3, C[3], C[3]),
+        Op("t3", "conv", "e3", 3, C[3], C[2]),
+        Op("u2", "up_add", "t3", 2, C[2], C[2], skip="e2"),
+        Op("d2", "conv", "u2", 2, C[2], C[2]),
+        Op("t2", "conv", "d2", 2, C[2], C[1]),
+        Op("u1", "up_add", "t2", 1, C[1], C[1], skip="e1"),
+        Op("d1", "conv", "u1", 1, C[1], C[1]),
+        Op("t1", "conv", "d1", 1, C[1], C[0]),
+        Op("u0", "up_add", "t1", 0, C[0], C[0], skip="e0"),
+        Op("d0", "conv", "u0", 0, C[0], C[0]),
+        Op("out", "conv", "d0", 0, C[0], OUT_C, relu=False),
+    ]
+
+
+def _s(dev) -> ctypes.c_void_p:
+    return ctypes.c_void_p(nat.stream_handle(dev))
+
+
+def _chk(rc, what):
+    nat.check(rc, what)
+
+
+class WarmStartNet:
+    """Parameters, forward, backward and Adam of the warm-start network."""
+
+    def __init__(self, device, M: int = 128, N: int = 128, seed: int = 0):
+        self.dev = torch.device(device)
+        self.M, self.N = M, N
+        self.ops = architecture()
+        self.offsets: Dict[str, int] = {}
+        n = 0
+        for op in self.ops:
+            if op.kind == "conv":
+                self.offsets[op.name] = n
+                n += op.cout * kpad(op.cin)
+        self.nparams_padded = n
+        self.w = torch.zeros(n, dtype=torch.float32, device=self.dev)
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        for op in self.ops:
+            if op.kind != "conv":
+                continue
+            fan_in = 9 * op.cin
+            std = math.sqrt(2.0 / fan_in) if op.relu else 0.1 * math.sqrt(1.0 / fan_in)
+            wl = torch.zeros(op.cout, kpad(op.cin))
+            wl[:, :9 * op.cin] = torch.randn(op.cout, 9 * op.cin, generator=g) * std
+            self._view(self.w, op).copy_(wl.to(self.dev))
+        self.wb = self.w.to(torch.bfloat16)
+        self.info: dict = {}
+
+    # ------------------------------------------------------------------ params
+    def _view(self, flat, op):
+        o = self.offsets[op.name]
+        return flat[o:o + op.cout * kpad(op.cin)].view(op.cout, kpad(op.cin))
+
+    def n_params(self) -> int:
+        """Trainable parameters (weights and biases, padding excluded)."""
+        return sum(op.cout * (9 * op.cin + 1) for op in self.ops if op.kind == "conv")
+
+    def describe(self) -> dict:
+        return {"params": self.n_params(), "layers": sum(op.kind == "conv" for op in self.ops),
+                "dtype": "bf16 (fp32 accumulate)", **{k: v for k, v in self.info.items() if k != "loss_curve"}}
+
+    def save(self, name: str = DEFAULT_NAME, directory: str = WEIGHTS_DIR):
+        os.makedirs(directory, exist_ok=True)
+        self.w.detach().cpu().numpy().astype("<f4").tofile(os.path.join(directory, name + ".bin"))
+        man = {"format": "raw little-endian float32, layers in order, each [Cout][Kp] with k = tap*Cin + ci, "
+                         "bias at k = 9*Cin",
+               "rows": self.M, "cols": self.N, "features": N_FEAT, "out_channels": OUT_C,
+               "layers": [{"name": op.name, "kind": op.kind, "src": op.src, "level": op.level, "cin": op.cin,
+                           "cout": op.cout, "relu": op.relu, "skip": op.skip,
+                           "offset": self.offsets.get(op.name), "kp": kpad(op.cin) if op.kind == "conv" else None}
+                          for op in self.ops],
+               "padded_floats": self.nparams_padded, "params": self.n_params(), "info": self.info}
+        with open(os.path.join(directory, name + ".json"), "w") as f:
+            json.dump(man, f, indent=1)
+
+    @classmethod
+    def load(cls, device, name: str = DEFAULT_NAME, directory: str = WEIGHTS_DIR) -> "WarmStartNet":
+        with open(os.path.join(directory, name + ".json")) as f:
+            man = json.load(f)
+        net = cls(device, man["rows"], man["cols"])
+        import numpy as np
+        w = np.fromfile(os.path.join(directory, name + ".bin"), dtype="<f4")
+        if w.size != net.nparams_padded:
+            raise RuntimeError(f"weight file has {w.size} floats, architecture needs {net.nparams_padded}")
+        net.w.copy_(torch.from_numpy(w).to(net.dev))
+        net.wb = net.w.to(torch.bfloat16)
+        net.info = man.get("info", {})
+        return net
+
+    # ------------------------------------------------------------------ features
+    def features(self, g, r, vin):
+        B = g.shape[0]
+        feat = torch.empty((B, self.M, self.N, N_FEAT), dtype=torch.bfloat16, device=self.dev)
+        base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
+        scale = torch.empty(B, dtype=torch.float32, device=self.dev)
+        _chk(nat.lib().fadesim_nn_features(B, self.M, self.N, ctypes.c_void_p(g.data_ptr()),
+                                           ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(vin.data_ptr()),
+                                           ctypes.c_void_p(feat.data_ptr()), ctypes.c_void_p(base.data_ptr()),
+                                           ctypes.c_void_p(scale.data_ptr()), _s(self.dev)), "nn_features")
+        return feat, base, scale
+
+    # ------------------------------------------------------------------ forward
+    def _hw(self, level):
+        return self.M >> level, self.N >> level
+
+    def forward(self, x, keep=False):
+        """x: features [B][M][N][8] bf16 -> out [B][M][N][16] bf16 (and all activations)."""
+        B = x.shape[0]
+        L = nat.lib()
+        acts = {"x": x}
+        s = _s(self.dev)
+        for op in self.ops:
+            H, W = self._hw(op.level)
+            if op.kind == "conv":
+                y = torch.empty((B, H, W, op.cout), dtype=torch.bfloat16, device=self.dev)
+                wv = self._view(self.wb, op)
+                _chk(L.fadesim_nn_conv3x3_fwd(B, H, W, op.cin, op.cout, kpad(op.cin),
+                                              ctypes.c_void_p(acts[op.src].data_ptr()), ctypes.c_void_p(wv.data_ptr()),
+                                              ctypes.c_void_p(y.data_ptr()), int(op.relu), s), op.name)
+            elif op.kind == "pool":
+                y = torch.empty((B, H, W, op.cout), dtype=torch.bfloat16, device=self.dev)
+                _chk(L.fadesim_nn_pool2(B, 2 * H, 2 * W, op.cout, ctypes.c_void_p(acts[op.src].data_ptr()),
+                                        ctypes.c_void_p(y.data_ptr()), s), op.name)
+            else:  # up_add
+                y = torch.empty((B, H, W, op.cout), dtype=torch.bfloat16, device=self.dev)
+                _chk(L.fadesim_nn_up2_add(B, H // 2, W // 2, op.cout, ctypes.c_void_p(acts[op.src].data_ptr()),
+                                          ctypes.c_void_p(acts[op.skip].data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                          s), op.name)
+            acts[op.name] = y
+            if not keep:
+                for k in list(acts):
+                    if k not in ("x",) and k != op.name and not self._needed_later(k, op.name):
+                        del acts[k]
+        return acts if keep else acts["out"]
+
+    def _needed_later(self, tensor, current):
+        seen = False
+        for op in self.ops:
+            if seen and (op.src == tensor or op.skip == tensor):
+                return True
+            if op.name == current:
+                seen = True
+        return False
+
+    def predict(self, g, r, vin, chunk: int = 2048):
+        """Warm-start drops [B][2][M][N] FP32 for solve_batch(..., warm=...)."""
+        B = g.shape[0]
+        warm = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
+        g = g.to(torch.float32).contiguous()
+        r = r.to(torch.float64).reshape(-1).expand(B).contiguous()
+        vin = vin.to(torch.float64).contiguous()
+        for b0 in range(0, B, chunk):
+            b1 = min(B, b0 + chunk)
+            feat, base, scale = self.features(g[b0:b1], r[b0:b1], vin[b0:b1])
+            out = self.forward(feat)
+            _chk(nat.lib().fadesim_nn_combine(b1 - b0, self.M, self.N, ctypes.c_void_p(base.data_ptr()),
+                                              ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                              OUT_C, ctypes.c_void_p(warm[b0:b1].data_ptr()), _s(self.dev)),
+                 "nn_combine")
+        return warm
+
+    # ------------------------------------------------------------------ backward
+    def backward(self, acts, dout, grad):
+        """dout: d loss / d out (bf16).  Accumulates weight gradients into grad (FP32 flat)."""
+        L = nat.lib()
+        s = _s(self.dev)
+        B = dout.shape[0]
+        grads = {"out": dout}
+        for op in reversed(self.ops):
+            H, W = self._hw(op.level)
+            dy = grads.pop(op.name)
+            if op.kind == "conv":
+                y = acts[op.name]
+                if op.relu:
+                    _chk(L.fadesim_nn_relu_mask(dy.numel(), ctypes.c_void_p(dy.data_ptr()), None,
+                                                ctypes.c_void_p(y.data_ptr()), s), "relu_mask")
+                gv = self._view(grad, op)
+                _chk(L.fadesim_nn_conv3x3_wgrad(B, H, W, op.cin, op.cout, kpad(op.cin),
+                                                ctypes.c_void_p(acts[op.src].data_ptr()), ctypes.c_void_p(dy.data_ptr()),
+                                                ctypes.c_void_p(gv.data_ptr()), s), "wgrad " + op.name)
+                if op.src != "x":
+                    kpt = kpad(op.cout)
+                    wT = torch.empty((op.cin, kpt), dtype=torch.bfloat16, device=self.dev)
+                    _chk(L.fadesim_nn_flip_weights(op.cin, op.cout, kpad(op.cin), kpt,
+                                                   ctypes.c_void_p(self._view(self.wb, op).data_ptr()),
+                                                   ctypes.c_void_p(wT.data_ptr()), s), "flip")
+                    dx = torch.empty((B, H, W, op.cin), dtype=torch.bfloat16, device=self.dev)
+                    _chk(L.fadesim_nn_conv3x3_fwd(B, H, W, op.cout, op.cin, kpt, ctypes.c_void_p(dy.data_ptr()),
+                                                  ctypes.c_void_p(wT.data_ptr()), ctypes.c_void_p(dx.data_ptr()), 0, s),
+                         "dgrad " + op.name)
+                    self._accum(grads, op.src, dx)
+            elif op.kind == "pool":
+                src = op.src
+                if src in grads:
+                    _chk(L.fadesim_nn_pool2_bwd(B, 2 * H, 2 * W, op.cout, ctypes.c_void_p(dy.data_ptr()), 1,
+                                                ctypes.c_void_p(grads[src].data_ptr()), s), "pool_bwd")
+                else:
+                    dx = torch.empty((B, 2 * H, 2 * W, op.cout), dtype=torch.bfloat16, device=self.dev)
+                    _chk(L.fadesim_nn_pool2_bwd(B, 2 * H, 2 * W, op.cout, ctypes.c_void_p(dy.data_ptr()), 0,
+                                                ctypes.c_void_p(dx.data_ptr()), s), "pool_bwd")
+                    grads[src] = dx
+            else:  # up_add
+                dt = torch.empty((B, H // 2, W // 2, op.cout), dtype=torch.bfloat16, device=self.dev)
+                _chk(L.fadesim_nn_up2_bwd(B, H // 2, W // 2, op.cout, ctypes.c_void_p(dy.data_ptr()),
+                                          ctypes.c_void_p(dt.data_ptr()), s), "up_bwd")
+                self._accum(grads, op.src, dt)
+                self._accum(grads, op.skip, dy)
+
+    def _accum(self, grads, name, g):
+        if name in grads:
+            _chk(nat.lib().fadesim_nn_add(g.numel(), ctypes.c_void_p(grads[name].data_ptr()),
+                                          ctypes.c_void_p(g.data_ptr()), ctypes.c_void_p(grads[name].data_ptr()),
+                                          _s(self.dev)), "grad add")
+        else:
+            grads[name] = g
+
+    # ------------------------------------------------------------------ training
+    def train(self, data, epochs=12, batch=64, lr=2e-3, wd=0.0, log=print, seed=0, max_seconds=None):
+        """data: dict with feat [P][M][N][8] bf16, target [P][2][M][N] FP32 (units of scale)."""
+        L = nat.lib()
+        s = _s(self.dev)
+        P = data["feat"].shape[0]
+        n = self.nparams_padded
+        grad = torch.zeros(n, dtype=torch.float32, device=self.dev)
+        m = torch.zeros_like(grad)
+        v = torch.zeros_like(grad)
+        loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        gen = torch.Generator(device="cpu").manual_seed(seed)
+        steps_per_epoch = P // batch
+        total = epochs * steps_per_epoch
+        step = 0
+        curve = []
+        t0 = time.time()
+        for ep in range(epochs):
+            perm = torch.randperm(P, generator=gen).to(self.dev)
+            ep_loss = 0.0
+            for it in range(steps_per_epoch):
+                idx = perm[it * batch:(it + 1) * batch]
+                x = data["feat"].index_select(0, idx)
+                tgt = data["target"].index_select(0, idx)
+                acts = self.forward(x, keep=True)
+                out = acts["out"]
+                dout = torch.empty_like(out)
+                loss.zero_()
+                _chk(L.fadesim_nn_mse(batch, self.M, self.N, ctypes.c_void_p(out.data_ptr()), OUT_C,
+                                      ctypes.c_void_p(tgt.data_ptr()), ctypes.c_void_p(dout.data_ptr()),
+                                      ctypes.c_void_p(loss.data_ptr()), s), "mse")
+                grad.zero_()
+                self.backward(acts, dout, grad)
+                step += 1
+                lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
+                _chk(L.fadesim_nn_adam(n, ctypes.c_void_p(self.w.data_ptr()), ctypes.c_void_p(m.data_ptr()),
+                                       ctypes.c_void_p(v.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
+                                       ctypes.c_void_p(self.wb.data_ptr()), ctypes.c_float(lr_t),
+                                       ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8), step,
+                                       ctypes.c_float(wd), ctypes.c_float(1.0), s), "adam")
+                if it % 50 == 0:
+                    lv = float(loss.item())
+                    ep_loss = lv
+                    curve.append((step, lv))
+                    log(f"epoch {ep} step {it}/{steps_per_epoch} loss {lv:.4e} lr {lr_t:.2e} "
+                        f"({time.time() - t0:.0f}s)")
+                if max_seconds and time.time() - t0 > max_seconds:
+                    break
+            if max_seconds and time.time() - t0 > max_seconds:
+                log(f"stopping at the time budget after epoch {ep}")
+                break
+        self.info.update({"epochs_run": ep + 1, "batch": batch, "lr": lr, "train_seconds": time.time() - t0,
+                          "final_loss": ep_loss, "loss_curve": curve})
+        return curve
+
+
+def make_instances(B, seed, device, M=128, N=128):
+    """Config-2 distribution on the GPU: 4 cell states, r ~ U[1,5] ohm,
+    vin in {0, 0.2} V (p = 0.5).  Seeded per call."""
+    gen = torch.Generator(device=device).manual_seed(int(seed))
+    states = torch.tensor([2e6, 1.6e6, 1.3e6, 1e6], dtype=torch.float64, device=device)
+    st = torch.randint(0, 4, (B, M, N), generator=gen, device=device)
+    g = (1.0 / states[st]).to(torch.float32)
+    r = 1.0 + 4.0 * torch.rand(B, generator=gen, device=device, dtype=torch.float64)
+    vin = torch.where(torch.rand((B, M), generator=gen, device=device) < 0.5, 0.2, 0.0).to(torch.float64)
+    return g, r, vin
+
+
+def generate_dataset(net: WarmStartNet, n_instances: int, seed0: int, chunk: int = 4096, tol: float = 1e-8):
+    """Features and targets from the project's own solver."""
+    from .solver import solve_batch
+    feats, targets = [], []
+    for c0 in range(0, n_instances, chunk):
+        B = min(chunk, n_instances - c0)
+        g, r, vin = make_instances(B, seed0 + c0, net.dev)
+        res = solve_batch(g, r, vin, tol=tol, want_field=True)
+        if int((res.status != 0).sum()) != 0:
+            raise RuntimeError("training-data solve did not converge")
+        feat, base, scale = net.features(g, r, vin)
+        tgt = (res.field - base) / scale.view(B, 1, 1, 1)
+        feats.append(feat)
+        targets.append(tgt)
+    return {"feat": torch.cat(feats), "target": torch.cat(targets)}
+
+
+def load_default(device) -> Optional[WarmStartNet]:
+    """The committed trained network, or None when no weights are present."""
+    if not os.path.exists(os.path.join(WEIGHTS_DIR, DEFAULT_NAME + ".bin")):
+        return None
+    return WarmStartNet.load(device)

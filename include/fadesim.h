@@ -57,7 +57,8 @@ int fadesim_version(void);
 typedef struct fadesim_batch_desc {
     int batch;              /* B instances */
     int rows, cols;         /* n, m: multiples of 4, 8..128 */
-    const float *d_g;       /* [B][n][m] cell conductance (S) */
+    const float *d_g;       /* [B][n][m] cell conductance (S), or [S][n][m] with d_gidx */
+    const int *d_gidx;      /* [B] conductance slot of each instance, or NULL (slot b) */
     const double *d_r_wire; /* [B] wire segment resistance (ohm) */
     const double *d_vin;    /* [B][n] wordline source voltage (V) */
     const double *d_vout;   /* [B][m] bitline sink voltage (V), or NULL = 0 V */

@@ -97,14 +97,17 @@ def test_vout_and_exact_warm_start(cuda):
                     for b in range(B)])
     res = _solve(cuda, g, r, vin, vout, tol=1e-6)
     assert _relerr(res.iout.cpu().numpy(), iex) <= 1e-6
-    # warm start = the exact drops: converged at the first test
+    # warm start = the exact drops (rounded to FP32): the FP32 rounding of the
+    # field leaves a residual near the FP32 floor, so the solve stops after the
+    # first checks and one refinement step, far earlier than from zero
     warm = np.zeros((B, 2, n, n), np.float32)
     for b in range(B):
         u, d, _ = oracle.exact(g[b].astype(np.float64), vin[b], vout[b], *oracle.segments_uniform(n, n, r[b]))
         warm[b, 0] = vin[b][:, None] - u
         warm[b, 1] = d - vout[b][None, :]
     res2 = _solve(cuda, g, r, vin, vout, warm, tol=1e-6)
-    assert (res2.sweeps.cpu().numpy() <= 32).all()
+    s_cold, s_warm = res.sweeps.cpu().numpy(), res2.sweeps.cpu().numpy()
+    assert (s_warm <= 64).all() and (4 * s_warm <= s_cold).all(), (s_cold, s_warm)
     assert _relerr(res2.iout.cpu().numpy(), iex) <= 1e-6
 
 

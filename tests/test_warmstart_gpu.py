@@ -1,0 +1,192 @@
+"""The learned warm start's GPU kernels against library references:
+convolution forward and weight gradient against torch (CPU, FP64) on the same
+bf16-rounded operands; the whole network's weight gradients against torch
+FP64 autograd of the same graph; the physics features against a numpy
+evaluation of the superposition sums; and -- with trained weights present --
+the warm start's effect on the solver on held-out seeds."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import config2_instances
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(t):
+    import torch
+    return t.to(torch.bfloat16)
+
+
+def _torch_weight(wflat, cin, cout, kp):
+    """[Cout][Kp] (k = tap*Cin + ci, bias at 9*Cin) -> torch [Cout][Cin][3][3], bias [Cout]."""
+    w = wflat.view(cout, kp)
+    wt = w[:, :9 * cin].reshape(cout, 3, 3, cin).permute(0, 3, 1, 2).contiguous()
+    return wt, w[:, 9 * cin].clone()
+
+
+@pytest.mark.parametrize("H,W,cin,cout", [(16, 16, 8, 16), (32, 32, 16, 32), (16, 32, 32, 64), (16, 16, 96, 96),
+                                          (32, 16, 64, 96)])
+def test_conv_fwd_and_wgrad_match_torch(cuda, H, W, cin, cout):
+    import torch
+    import torch.nn.functional as F
+    from fadesim_amd import _native as nat
+    from fadesim_amd.warmstart import kpad
+    B = 2
+    kp = kpad(cin)
+    gen = torch.Generator().manual_seed(H * 100 + cin)
+    x = _bf(torch.randn(B, H, W, cin, generator=gen))
+    w = torch.zeros(cout, kp)
+    w[:, :9 * cin + 1] = torch.randn(cout, 9 * cin + 1, generator=gen) * 0.2
+    w = _bf(w)
+    xd, wd = x.to(cuda), w.to(cuda)
+    y = torch.empty((B, H, W, cout), dtype=torch.bfloat16, device=cuda)
+    s = ctypes.c_void_p(nat.stream_handle(cuda))
+    L = nat.lib()
+    nat.check(L.fadesim_nn_conv3x3_fwd(B, H, W, cin, cout, kp, ctypes.c_void_p(xd.data_ptr()),
+                                       ctypes.c_void_p(wd.data_ptr()), ctypes.c_void_p(y.data_ptr()), 1, s), "fwd")
+    wt, bias = _torch_weight(w.double(), cin, cout, kp)
+    xt = x.double().permute(0, 3, 1, 2).requires_grad_(False)
+    wt.requires_grad_(True)
+    bias.requires_grad_(True)
+    ref = F.relu(F.conv2d(xt, wt, bias, padding=1)).permute(0, 2, 3, 1)
+    got = y.cpu().double()
+    assert torch.max(torch.abs(got - ref.detach())) <= 1e-2 * torch.max(torch.abs(ref)) + 1e-3
+    # weight gradient (bias included) for an upstream gradient dz (ReLU mask applied by the caller)
+    dz = _bf(torch.randn(B, H, W, cout, generator=gen)) * (ref.detach() > 0)
+    dz = _bf(dz)
+    dw = torch.zeros((cout, kp), dtype=torch.float32, device=cuda)
+    nat.check(L.fadesim_nn_conv3x3_wgrad(B, H, W, cin, cout, kp, ctypes.c_void_p(xd.data_ptr()),
+                                         ctypes.c_void_p(dz.to(cuda).data_ptr()), ctypes.c_void_p(dw.data_ptr()), s),
+              "wgrad")
+    pre = F.conv2d(xt, wt, bias, padding=1).permute(0, 2, 3, 1)
+    (pre * dz.double()).sum().backward()
+    gw = dw.cpu().double()
+    gref = torch.zeros(cout, kp, dtype=torch.float64)
+    gref[:, :9 * cin] = wt.grad.permute(0, 2, 3, 1).reshape(cout, 9 * cin)
+    gref[:, 9 * cin] = bias.grad
+    assert torch.allclose(gw, gref, rtol=2e-3, atol=2e-3 * gref.abs().max())
+    # data gradient through the flipped weights equals torch's conv input gradient
+    kpt = kpad(cout)
+    if cin % 16 == 0:
+        wT = torch.empty((cin, kpt), dtype=torch.bfloat16, device=cuda)
+        nat.check(L.fadesim_nn_flip_weights(cin, cout, kp, kpt, ctypes.c_void_p(wd.data_ptr()),
+                                            ctypes.c_void_p(wT.data_ptr()), s), "flip")
+        dx = torch.empty((B, H, W, cin), dtype=torch.bfloat16, device=cuda)
+        nat.check(L.fadesim_nn_conv3x3_fwd(B, H, W, cout, cin, kpt, ctypes.c_void_p(dz.to(cuda).data_ptr()),
+                                           ctypes.c_void_p(wT.data_ptr()), ctypes.c_void_p(dx.data_ptr()), 0, s),
+                  "dgrad")
+        xt2 = x.double().permute(0, 3, 1, 2).requires_grad_(True)
+        (F.conv2d(xt2, wt.detach(), bias.detach(), padding=1).permute(0, 2, 3, 1) * dz.double()).sum().backward()
+        gx = xt2.grad.permute(0, 2, 3, 1)
+        assert torch.max(torch.abs(dx.cpu().double() - gx)) <= 1e-2 * torch.max(torch.abs(gx)) + 1e-3
+
+
+def _torch_net(net, x_nhwc, wflat):
+    """Same graph as fadesim_amd.warmstart.architecture() in torch FP64 (NCHW)."""
+    import torch.nn.functional as F
+    from fadesim_amd.warmstart import kpad
+    acts = {"x": x_nhwc.permute(0, 3, 1, 2)}
+    for op in net.ops:
+        if op.kind == "conv":
+            o = net.offsets[op.name]
+            wt, b = _torch_weight(wflat[o:o + op.cout * kpad(op.cin)], op.cin, op.cout, kpad(op.cin))
+            y = F.conv2d(acts[op.src], wt, b, padding=1)
+            acts[op.name] = F.relu(y) if op.relu else y
+        elif op.kind == "pool":
+            acts[op.name] = F.avg_pool2d(acts[op.src], 2)
+        else:
+            acts[op.name] = F.interpolate(acts[op.src], scale_factor=2, mode="nearest") + acts[op.skip]
+    return acts["out"].permute(0, 2, 3, 1)
+
+
+def test_network_gradients_match_torch_autograd(cuda):
+    import torch
+    from fadesim_amd import _native as nat
+    from fadesim_amd.warmstart import OUT_C, WarmStartNet
+    net = WarmStartNet(cuda, seed=3)
+    B = 1
+    gen = torch.Generator().manual_seed(5)
+    x = _bf(torch.rand(B, 128, 128, 8, generator=gen))
+    tgt = torch.randn(B, 2, 128, 128, generator=gen).float()
+    acts = net.forward(x.to(cuda), keep=True)
+    out = acts["out"]
+    dout = torch.empty_like(out)
+    loss = torch.zeros(1, dtype=torch.float32, device=cuda)
+    nat.check(nat.lib().fadesim_nn_mse(B, 128, 128, ctypes.c_void_p(out.data_ptr()), OUT_C,
+                                       ctypes.c_void_p(tgt.to(cuda).data_ptr()), ctypes.c_void_p(dout.data_ptr()),
+                                       ctypes.c_void_p(loss.data_ptr()), ctypes.c_void_p(nat.stream_handle(cuda))),
+              "mse")
+    grad = torch.zeros(net.nparams_padded, dtype=torch.float32, device=cuda)
+    net.backward(acts, dout, grad)
+    torch.cuda.synchronize()
+    wref = net.wb.cpu().double().requires_grad_(True)
+    o = _torch_net(net, x.double(), wref)
+    l = 0.5 * ((o[..., 0:2].permute(0, 3, 1, 2) - tgt.double()) ** 2).mean()
+    l.backward()
+    assert abs(loss.item() - l.item()) <= 0.05 * l.item()
+    g = grad.cpu().double()
+    for op in net.ops:
+        if op.kind != "conv":
+            continue
+        o0 = net.offsets[op.name]
+        n = op.cout * (((9 * op.cin + 1 + 31) // 32) * 32)
+        a, b = g[o0:o0 + n], wref.grad[o0:o0 + n]
+        cos = float(torch.dot(a, b) / (a.norm() * b.norm() + 1e-300))
+        rel = float((a - b).norm() / (b.norm() + 1e-300))
+        assert cos > 0.995 and rel < 0.1, (op.name, cos, rel)
+
+
+def _np_first_order(g, r, c):
+    n, m = g.shape
+    I = g * c
+    j = np.arange(m)
+    W = np.minimum(j[:, None], j[None, :]) + 1.0
+    i = np.arange(n)
+    Wq = (n - np.maximum(i[:, None], i[None, :])).astype(float)
+    return r * (I @ W.T), r * (Wq @ I)
+
+
+def test_physics_features(cuda):
+    import torch
+    from fadesim_amd.warmstart import WarmStartNet
+    net = WarmStartNet(cuda)
+    g, r, vin = config2_instances(3, 128, seed=4)
+    tg, tr, tv = (torch.tensor(a, device=cuda) for a in (g, r, vin))
+    feat, base, scale = net.features(tg, tr, tv)
+    base = base.cpu().numpy()
+    for b in range(3):
+        gd = g[b].astype(np.float64)
+        c = np.repeat(vin[b][:, None], 128, 1)
+        p1, q1 = _np_first_order(gd, r[b], c)
+        p2, q2 = _np_first_order(gd, r[b], c - p1 - q1)
+        np.testing.assert_allclose(base[b, 0], p2, rtol=1e-4, atol=1e-6 * np.abs(p2).max())
+        np.testing.assert_allclose(base[b, 1], q2, rtol=1e-4, atol=1e-6 * np.abs(q2).max())
+        s2 = max(np.abs(p2 - p1).max(), np.abs(q2 - q1).max())
+        assert scale[b].item() == pytest.approx(s2, rel=1e-4)
+        # the second-order field is a far better initial guess than zero
+        iex = oracle.exact_batch(g[b:b + 1], r[b], vin[b:b + 1])[0]
+        i2 = (gd * (c - p2 - q2)).sum(0)
+        assert np.max(np.abs(i2 - iex) / np.abs(iex)) < 3e-3
+
+
+def test_trained_warm_start_cuts_sweeps(cuda):
+    import torch
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    net = warmstart.load_default(cuda)
+    if net is None:
+        pytest.skip("no trained weights committed")
+    g, r, vin = warmstart.make_instances(256, seed=10_000_000, device=cuda)
+    plain = solve_batch(g, r, vin, tol=1e-6)
+    warm = solve_batch(g, r, vin, warm=net.predict(g, r, vin), tol=1e-6)
+    assert int((warm.status != 0).sum()) == 0
+    sp, sw = plain.sweeps.double().mean().item(), warm.sweeps.double().mean().item()
+    assert sw < 0.8 * sp, (sp, sw)
+    k = 8
+    iex = oracle.exact_batch(g[:k].cpu().numpy(), r[:k].cpu().numpy(), vin[:k].cpu().numpy())
+    err = np.max(np.abs(warm.iout[:k].cpu().numpy() - iex) / np.abs(iex))
+    assert err <= 1e-6, err
