@@ -59,7 +59,7 @@ class HookDesc(ctypes.Structure):
 # Every symbol include/fadesim.h declares.
 EXPORTED = [
     "fadesim_last_error", "fadesim_version", "fadesim_solve_batch", "fadesim_default_omega",
-    "fadesim_last_kernel_ms",
+    "fadesim_last_kernel_ms", "fadesim_cell_conductance",
     "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs",
     "fadesim_nn_features", "fadesim_nn_combine", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
     "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
@@ -114,6 +114,7 @@ def _declare_nn(L):
         "fadesim_nn_mse": [i, i, i, _vp, i, _vp, _vp, _vp, _vp],
         "fadesim_nn_adam": [sz, _vp, _vp, _vp, _vp, _vp, f, f, f, f, i, f, f, _vp],
         "fadesim_nn_cast": [sz, _vp, i, _vp, _vp],
+        "fadesim_cell_conductance": [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     }
     for name, args in sigs.items():
         if hasattr(L, name):

@@ -7,21 +7,26 @@ Restates the fixed-point, bit-sliced crossbar matrix multiply of PimTorch --
 mode) and ``Linear_sp`` (modules/linear_split.py:13-78) -- with the same
 arithmetic: Normal (two's complement) quantized inputs applied bit-serially,
 PN weights split into positive/negative magnitudes and sliced ``cellBits``
-per cell, conductance ``slice*dG + Gmin``, per-tile ADC
-``round(I/adcUseI*(2^adc-1))``, sum over row tiles, rescale and shift-add with
-the MSB subtracted (complement code).
+per cell, conductance ``slice*dG + Gmin`` in the float type, per-tile ADC
+``round(I/adcUseI*(2^adc-1))``, sum over row tiles, rescale and shift-add
+with the MSB subtracted (complement code).
 
 MI355X-first difference: instead of one hook call per (input bit, weight
-slice, pos/neg) -- 2 * inputBits * weightBits/cellBits calls per layer -- every
-tile solve of the layer goes into ONE batched solve: the distinct conductance
-tiles become slots of ``solve_batch(..., gidx=...)`` and every (bit, slice,
-sign, image, row tile, column tile) is an instance (all-zero input vectors
-are skipped: their current is exactly 0).  ``ir_mode`` selects the solver:
+slice, pos/neg) -- 2 * inputBits * weightBits/cellBits calls per layer --
+every tile solve of the layer goes into batched solves: the distinct
+conductance tiles become slots of ``solve_batch(..., gidx=...)`` and every
+(bit, slice, sign, image, row tile, column tile) with a non-zero input vector
+is one instance (an all-zero input vector gives exactly zero current and is
+not solved).  ``PhyArrParams.ir_mode`` selects the crossbar model:
 
-* ``"converged"`` -- fadesim_solve_batch to ``tol`` (optionally NN-warm-started);
+* ``"converged"`` -- fadesim_solve_batch to ``tol`` (linear cells), or the
+  Picard iteration of solve_batch_nonlinear (sinh cells); optionally seeded
+  by the learned warm start (``warm_start``);
 * ``"reference"`` -- the reference's own line-GS iteration through
-  fadesim_amd.hook.ir_drop_gs (iter_time sweeps at beta), per call like PimTorch;
-* ``"none"``      -- ideal crossbar (``no_ir_drop``).
+  fadesim_amd.hook.ir_drop_gs (``ir_drop_iter_time`` sweeps at
+  ``ir_drop_GS_method_beta_scale``), one call per (bit, slice, sign) as
+  PimTorch makes them;
+* ``"none"``      -- the ideal crossbar (``no_ir_drop``, ir_drop_solve.py:12-15).
 """
 from __future__ import annotations
 
@@ -45,6 +50,7 @@ class phyArrMode(Enum):
 
 
 class ConstForSplitArray:
+    """Default bit widths of commonConst.py:45-50."""
     input_bit_width = 8
     output_bit_width = 10
     weight_bit_width = 8
@@ -52,10 +58,20 @@ class ConstForSplitArray:
     dac_bit_width = 1
 
 
+def get_nonlinear_params_from_K2(nonlinear_V: float, R: float, k2: float) -> Tuple[float, float]:
+    """(a, b) of I = b sinh(a V) with chord resistance R at nonlinear_V and
+    I(V)/I(V/2) = k2 (conductance_variation.py:29-33)."""
+    x = (k2 + math.sqrt(k2 * k2 - 4)) / 2
+    a = 2 * math.log(x) / nonlinear_V
+    b = nonlinear_V / R / math.sinh(a * nonlinear_V)
+    return a, b
+
+
 class PhyArrParams:
     """The crossbar constants of PimTorch's phyArrParams (commonConst.py:187-325)
-    that the MVM path reads.  Derived values are properties so that changing
-    Ron/Roff/cellBits/arrRowSize keeps them consistent."""
+    that the MVM path reads, with the same defaults.  Derived values are
+    properties so that changing Ron/Roff/cellBits/arrRowSize keeps them
+    consistent."""
 
     def __init__(self, **kw):
         self.Ron = 1e3
@@ -72,10 +88,11 @@ class PhyArrParams:
         self.ir_drop_less_than_thresholdBreak_enable = False
         self.ir_drop_less_than_threshold = 1e-7
         self.nonlinear_cell_enable = False
-        self.nonlinear_params = []
-        self.tol = 1e-6                       # converged mode
-        self.warm_start = None                # WarmStartNet or None (converged mode)
-        self.max_batch = 131072               # instances per solve call
+        self.nonlinear_params = []            # Ron, Roff, a_on, b_on, a_off, b_off
+        self.tol = 1e-6                       # converged mode: relative I_out tolerance
+        self.warm_start = None                # converged mode: WarmStartNet or None
+        self.max_batch = 65536                # instances per batched solve
+        self.stats = {}                       # filled by the last converged call
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -97,8 +114,36 @@ class PhyArrParams:
     def adcUseI(self):
         return self.arrRowSize * (self.cellMaxConduct - self.cellMinConduct) * self.inputMaxVoltage
 
+    def set_sinh_cells(self, a: float, nonlinear_V: float = 1.0):
+        """Sinh cells I = b sinh(a V) for both states, b from the chord
+        resistance (Ron / Roff) at nonlinear_V as get_nonlinear_params_from_K2
+        does; a = 2 ln(x)/V  <=>  K2 = 2 cosh(a V / 2)."""
+        k2 = 2 * math.cosh(a * nonlinear_V / 2)
+        a_on, b_on = get_nonlinear_params_from_K2(nonlinear_V, self.Ron, k2)
+        a_off, b_off = get_nonlinear_params_from_K2(nonlinear_V, self.Roff, k2)
+        self.nonlinear_cell_enable = True
+        self.nonlinear_params = [self.Ron, self.Roff, a_on, b_on, a_off, b_off]
+        return self
+
 
 phyArrParams = PhyArrParams()
+
+
+def mlp_inference_params(**kw) -> PhyArrParams:
+    """The 784-256-10 MLP inference workload's crossbar: 128x128 tiles, 2 bits
+    per cell on the linear map Gmin = 1/2 MOhm .. Gmax = 1/1 MOhm (the
+    conductance range of the 4-state 128x128 solver workload the learned warm
+    start is trained on), 2.93 ohm wires, 0 / 0.2 V bit-serial inputs and sinh
+    cells with a = 2.5 / V (chord calibration at 1 V), every tile solved to
+    1e-6 by the batched solver."""
+    P = PhyArrParams(Ron=1e6, Roff=2e6, cellBits=2, arrRowSize=128, arrColSize=128, inputMaxVoltage=0.2,
+                     r_wire=2.93, ir_mode="converged")
+    P.set_sinh_cells(2.5)
+    for k, v in kw.items():
+        if not hasattr(P, k):
+            raise AttributeError(k)
+        setattr(P, k, v)
+    return P
 
 
 def _get_fixed_point_position(max_abs: float, bit_width: int, tensor_type: TensorType) -> int:
@@ -136,8 +181,9 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         raise Exception("[Error] Complement code input mode, but dac is not 1 bit. dacbits = {}".format(dacBits))
     if weightBits % P.cellBits != 0:
         raise Exception("[Error] cellbit is not factor of weight_bit")
+    if P.sigma != 0.0:
+        raise NotImplementedError("conductance variation (sigma != 0) is not modelled")
     with torch.no_grad():
-        dev = w.device
         bsize, insize = input.shape
         outsize = w.shape[1]
         qin, s1 = quantization_tensor(input, inputBits, TensorType.Normal)
@@ -150,88 +196,103 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         nslice = weightBits // P.cellBits
         mask_cell = (1 << P.cellBits) - 1
         inputV = P.inputMaxVoltage / (2 ** dacBits - 1)
-        # conductance slots [nslice][2][numX][numY][R][C] (linear map of the slice value)
+        # slices [nslice][2][Rp][Cp]; padded cells hold slice 0 (conductance Gmin)
         slices = torch.stack([torch.stack([(wpos >> (P.cellBits * j)) & mask_cell,
                                            (wneg >> (P.cellBits * j)) & mask_cell]) for j in range(nslice)])
-        slices = F.pad(slices, (0, Cp - outsize, 0, Rp - insize))  # padded cells hold slice 0
-        G = slices.to(torch.float64) * P.cellDeltaConduct + P.cellMinConduct
+        slices = F.pad(slices, (0, Cp - outsize, 0, Rp - insize))
+        G = slices.to(float_type).mul(P.cellDeltaConduct).add(P.cellMinConduct)  # matmul :753-754
         G = G.view(nslice, 2, numX, R, numY, C).permute(0, 1, 2, 4, 3, 5).contiguous()
         # input bits [inputBits][bsize][numX][R] (two's complement bits of qin)
         qpad = F.pad(qin, (0, Rp - insize))
         bits = torch.stack([(qpad >> i) & 1 for i in range(inputBits)]).view(inputBits, bsize, numX, R)
-        # currents [inputBits][nslice][2][bsize][numX][numY][C]
+        # tile currents [inputBits][nslice][2][bsize][numX][numY][C]
         if P.ir_mode == "none":
-            I = torch.einsum("ibxr,jsxyrc->ijsbxyc", bits.to(torch.float64) * inputV, G)
+            vin = bits.to(float_type).mul(inputV)
+            I = torch.einsum("ibxr,jsxyrc->ijsbxyc", vin, G)
         elif P.ir_mode == "converged":
             I = _solve_converged(bits, G, inputV, P)
         elif P.ir_mode == "reference":
             I = _solve_reference(bits, G, inputV, P, float_type)
         else:
             raise ValueError(f"unknown ir_mode {P.ir_mode}")
-        # ADC per tile (splitArrayArithmetic.py:831-836), sum over row tiles, rescale (:959-968)
-        I = I.to(float_type)
-        adc = I.div(P.adcUseI).mul(2 ** adcBits - 1).round()
+        # ADC per tile (matmul :833-835), sum over row tiles and rescale (:958-968)
+        adc = I.to(float_type).div(P.adcUseI).mul(2 ** adcBits - 1).round()
         out = adc.sum(4).reshape(inputBits, nslice, 2, bsize, Cp)[..., :outsize]
         out = out.mul(R * (2 ** dacBits - 1) * (2 ** P.cellBits - 1) / (2 ** adcBits - 1))
         out = out[:, :, 0] - out[:, :, 1]  # [inputBits][nslice][bsize][outsize]
         output = w.new_zeros((bsize, outsize))
-        for i in range(inputBits):
+        for i in range(inputBits):  # splitArr_matmul_nn :1197-1227
             for jj in range(nslice):
                 term = out[i, jj].to(w.dtype).mul(2.0 ** (i + P.cellBits * jj + s1 + s2))
                 output = output.sub(term) if i == inputBits - 1 else output.add(term)
         return output
 
 
-def _solve_converged(bits, G, inputV, P):
-    """All tile solves of the layer in batched calls of fadesim_solve_batch."""
-    from .solver import solve_batch
-    nbit, bsize, numX, R = bits.shape
-    nslice, _, _, numY, _, C = G.shape
-    dev = G.device
-    slots = G.reshape(-1, R, C).to(torch.float32).contiguous()  # slot id = ((j*2+s)*numX + x)*numY + y
-    I = torch.zeros((nbit, nslice, 2, bsize, numX, numY, C), dtype=torch.float64, device=dev)
-    nz = bits.view(nbit, bsize, numX, R).any(-1)  # non-zero input vectors
-    ii, bb, xx = torch.nonzero(nz, as_tuple=True)
-    if ii.numel() == 0:
-        return I
-    # instances: every non-zero input vector x every (slice, sign, column tile)
-    j = torch.arange(nslice, device=dev)
-    sgn = torch.arange(2, device=dev)
-    y = torch.arange(numY, device=dev)
-    J, S, Y = torch.meshgrid(j, sgn, y, indexing="ij")
+def tile_instances(bits: Tensor, nslice: int, numY: int):
+    """Instance list of a layer's crossbar solves: every (bit, image, row tile)
+    with a non-zero input vector x every (slice, sign, column tile).  Returns
+    index tensors (i, b, x, j, s, y), each [K]."""
+    dev = bits.device
+    ii, bb, xx = torch.nonzero(bits.any(-1), as_tuple=True)
+    J, S, Y = torch.meshgrid(torch.arange(nslice, device=dev), torch.arange(2, device=dev),
+                             torch.arange(numY, device=dev), indexing="ij")
     J, S, Y = J.reshape(-1), S.reshape(-1), Y.reshape(-1)
     n_in, n_t = ii.numel(), J.numel()
-    inst_i = ii.repeat_interleave(n_t)
-    inst_b = bb.repeat_interleave(n_t)
-    inst_x = xx.repeat_interleave(n_t)
-    inst_j, inst_s, inst_y = J.repeat(n_in), S.repeat(n_in), Y.repeat(n_in)
-    gidx = (((inst_j * 2 + inst_s) * numX + inst_x) * numY + inst_y).to(torch.int32)
-    total = gidx.numel()
+    return (ii.repeat_interleave(n_t), bb.repeat_interleave(n_t), xx.repeat_interleave(n_t),
+            J.repeat(n_in), S.repeat(n_in), Y.repeat(n_in))
+
+
+def _solve_converged(bits, G, inputV, P):
+    """All tile solves of the layer through the batched solver."""
+    from .solver import solve_batch, solve_batch_nonlinear
+    nbit, bsize, numX, R = bits.shape
+    nslice, _, _, numY, _, C = G.shape
+    if not G.is_cuda:
+        raise RuntimeError("the converged crossbar model runs on the GPU (G is on the CPU)")
+    dev = G.device
+    slots = G.reshape(-1, R, C).to(torch.float32).contiguous()  # slot ((j*2+s)*numX + x)*numY + y
+    I = torch.zeros((nbit, nslice, 2, bsize, numX, numY, C), dtype=torch.float64, device=dev)
+    ii, bb, xx, jj, ss, yy = tile_instances(bits, nslice, numY)
+    total = ii.numel()
+    gidx = (((jj * 2 + ss) * numX + xx) * numY + yy).to(torch.int32)
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
+    net = P.warm_start
+    stats = {"solves": total, "sweeps": 0, "picard_steps": 0, "calls": 0}
     for c0 in range(0, total, P.max_batch):
         c1 = min(total, c0 + P.max_batch)
-        vin = bits[inst_i[c0:c1], inst_b[c0:c1], inst_x[c0:c1]].to(torch.float64) * inputV
-        gi = gidx[c0:c1]
-        warm = None
-        if P.warm_start is not None:
-            warm = P.warm_start.predict(slots.index_select(0, gi.long()), r.expand(c1 - c0), vin)
-        res = solve_batch(slots, r.expand(c1 - c0), vin, warm=warm, gidx=gi, tol=P.tol)
+        n = c1 - c0
+        sl = slice(c0, c1)
+        vin = bits[ii[sl], bb[sl], xx[sl]].to(torch.float64) * inputV
+        gi = gidx[sl]
+        rr = r.expand(n)
+        if P.nonlinear_cell_enable:
+            warm_fn = (lambda Gz, rw, v: net.predict(Gz, rw, v)) if net is not None else None
+            res = solve_batch_nonlinear(slots, rr, vin, params=P.nonlinear_params, gidx=gi, tol=P.tol,
+                                        warm_fn=warm_fn)
+            stats["sweeps"] += int(res.total_sweeps.sum())
+            stats["picard_steps"] = max(stats["picard_steps"], res.outer)
+        else:
+            warm = net.predict(slots.index_select(0, gi.long()), rr, vin) if net is not None else None
+            res = solve_batch(slots, rr, vin, warm=warm, gidx=gi, tol=P.tol)
+            stats["sweeps"] += int(res.sweeps.sum())
+        stats["calls"] += 1
         if int((res.status != 0).sum()) != 0:
             raise RuntimeError("crossbar solve did not converge")
-        I[inst_i[c0:c1], inst_j[c0:c1], inst_s[c0:c1], inst_b[c0:c1], inst_x[c0:c1], inst_y[c0:c1]] = res.iout
+        I[ii[sl], jj[sl], ss[sl], bb[sl], xx[sl], yy[sl]] = res.iout
+    P.stats = stats
     return I
 
 
 def _solve_reference(bits, G, inputV, P, float_type):
     """The reference's iteration: one hook call per (input bit, weight slice,
-    sign), as splitArrayArithmetic.py:488-511 does."""
+    sign), as splitArrayArithmetic.py:488-511 makes them."""
     from .hook import ir_drop_gs
     nbit, bsize, numX, R = bits.shape
     nslice, _, _, numY, _, C = G.shape
     dev = G.device
     I = torch.zeros((nbit, nslice, 2, bsize, numX, numY, C), dtype=torch.float64, device=dev)
     for i in range(nbit):
-        vin = (bits[i].to(float_type) * inputV).contiguous()  # [bsize][numX][R]
+        vin = bits[i].to(float_type).mul(inputV).contiguous()  # [bsize][numX][R]
         for jj in range(nslice):
             for s in range(2):
                 Gh = G[jj, s].permute(0, 2, 1, 3).reshape(numX, R, numY * C).to(float_type).contiguous()
@@ -279,3 +340,24 @@ class Linear_sp(nn.Module):
         if self.hasBias and self.useFpBias:
             output = output + self.fp_bias
         return output
+
+
+class MLP(nn.Module):
+    """Perceptron of Linear_sp layers (784-256-10 by default) with ReLU between
+    them and seeded Gaussian weights."""
+
+    def __init__(self, sizes=(784, 256, 10), params=None, seed=0):
+        super().__init__()
+        gen = torch.Generator().manual_seed(seed)
+        self.layers = nn.ModuleList(Linear_sp(a, b, params=params) for a, b in zip(sizes[:-1], sizes[1:]))
+        with torch.no_grad():
+            for L in self.layers:
+                L.fp_weight.copy_(torch.randn(L.fp_weight.shape, generator=gen) / math.sqrt(L.inSize))
+                L.fp_bias.copy_(torch.randn(L.fp_bias.shape, generator=gen) * 0.01)
+
+    def forward(self, x):
+        for k, L in enumerate(self.layers):
+            x = L(x)
+            if k < len(self.layers) - 1:
+                x = torch.relu(x)
+        return x

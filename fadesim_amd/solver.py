@@ -116,3 +116,92 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
         rc = nat.lib().fadesim_solve_batch(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
     nat.check(rc, "fadesim_solve_batch")
     return SolveResult(iout, sweeps, status, est, field)
+
+
+def cell_conductance(g_lin: torch.Tensor, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
+                     field: Optional[torch.Tensor] = None, *, params, gidx: Optional[torch.Tensor] = None,
+                     out: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Sinh-nonlinear cell conductances [B, M, N] float32 of the state given by
+    the drop ``field`` (None: zero drops) -- ``fadesim_cell_conductance``.
+    ``params`` = (Ron, Roff, a_on, b_on, a_off, b_off) as in PimTorch's
+    ``phyArrParams.nonlinear_params`` (commonConst.py:200-207)."""
+    S, M, N = g_lin.shape
+    B = S if gidx is None else int(gidx.numel())
+    dev = g_lin.device
+    g_lin = g_lin.to(torch.float32).contiguous()
+    vin = vin.to(device=dev, dtype=torch.float64).reshape(B, M).contiguous()
+    if vout is not None:
+        vout = vout.to(device=dev, dtype=torch.float64).reshape(B, N).contiguous()
+    if field is not None:
+        field = field.to(device=dev, dtype=torch.float32).reshape(B, 2, M, N).contiguous()
+    if gidx is not None:
+        gidx = gidx.to(device=dev, dtype=torch.int32).reshape(B).contiguous()
+    p = [float(x) for x in params]
+    if len(p) != 6:
+        raise ValueError("params must be (Ron, Roff, a_on, b_on, a_off, b_off)")
+    prm = (ctypes.c_double * 6)(*p)
+    g = out if out is not None else torch.empty((B, M, N), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        rc = nat.lib().fadesim_cell_conductance(B, M, N, ctypes.c_void_p(g_lin.data_ptr()),
+                                                ctypes.c_void_p(_ptr(gidx)), ctypes.c_void_p(vin.data_ptr()),
+                                                ctypes.c_void_p(_ptr(vout)), ctypes.c_void_p(_ptr(field)), prm,
+                                                ctypes.c_void_p(g.data_ptr()),
+                                                ctypes.c_void_p(nat.stream_handle(dev)))
+    nat.check(rc, "fadesim_cell_conductance")
+    return g
+
+
+@dataclass
+class NonlinearResult(SolveResult):
+    outer: int = 0                 # Picard iterations (linear solves) performed
+    total_sweeps: Optional[torch.Tensor] = None  # [B] int64 sweeps over all Picard steps
+
+
+def solve_batch_nonlinear(g_lin: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
+                          warm: Optional[torch.Tensor] = None, *, params, gidx: Optional[torch.Tensor] = None,
+                          tol: float = 1e-6, max_outer: int = 40, warm_fn=None, **kw) -> NonlinearResult:
+    """Crossbars with sinh-nonlinear cells: Picard iteration G <- G(V) around
+    the linear batched solve, each solve warm-started from the previous
+    field, until the bitline currents of consecutive steps agree to tol/4
+    (relative to the largest current of the instance).
+
+    The fixed point is the nonlinear KCL solution of the network, the state
+    the reference's lagged-conductance sweeps (split_arr_matmul.cu:241-262)
+    converge to.  ``warm_fn(G, r, vin)``, if given, supplies the first solve's
+    initial field from the conductances of the zero-drop state (e.g. the
+    learned warm start); ``warm`` supplies it directly.
+    """
+    S, M, N = g_lin.shape
+    B = S if gidx is None else int(gidx.numel())
+    dev = g_lin.device
+    if not torch.is_tensor(r_wire):
+        r_wire = torch.full((B,), float(r_wire), dtype=torch.float64, device=dev)
+    r_wire = r_wire.to(device=dev, dtype=torch.float64).reshape(-1).expand(B).contiguous()
+    field = warm
+    G = cell_conductance(g_lin, vin, vout, field, params=params, gidx=gidx)
+    if field is None and warm_fn is not None:
+        field = warm_fn(G, r_wire, vin)
+    inner_tol = tol / 4
+    total = torch.zeros(B, dtype=torch.int64, device=dev)
+    prev = None
+    res = None
+    k = 0
+    for k in range(1, max_outer + 1):
+        res = solve_batch(G, r_wire, vin, vout, field, tol=inner_tol, want_field=True, **kw)
+        total += res.sweeps.to(torch.int64)
+        field = res.field
+        if int((res.status != 0).sum()) != 0:
+            break
+        if prev is not None:
+            scale = res.iout.abs().amax(dim=1).clamp_min(1e-300)
+            change = ((res.iout - prev).abs().amax(dim=1) / scale).max().item()
+            if change <= inner_tol:
+                break
+        prev = res.iout
+        G = cell_conductance(g_lin, vin, vout, field, params=params, gidx=gidx, out=G)
+    st = res.status.clone()
+    if k == max_outer and prev is not None and st.eq(0).all():
+        scale = res.iout.abs().amax(dim=1).clamp_min(1e-300)
+        if ((res.iout - prev).abs().amax(dim=1) / scale).max().item() > inner_tol:
+            st.fill_(1)
+    return NonlinearResult(res.iout, res.sweeps, st, res.est, res.field, outer=k, total_sweeps=total)

@@ -87,6 +87,22 @@ double fadesim_last_kernel_ms(void);
 double fadesim_default_omega(int rows, int cols);
 
 /*
+ * Sinh-nonlinear cell conductance for a crossbar state, the G update of the
+ * Picard iteration around fadesim_solve_batch.  For each instance b and cell
+ * (i,j): V = (vin_i - p_ij) - (vout_j + q_ij) with the drops (p, q) of
+ * d_field (NULL = zero drops), |V| clamped to >= 1e-5,
+ *   R = (1-z)|V|/(b_on sinh(a_on|V|)) + z|V|/(b_off sinh(a_off|V|)),
+ *   z = max(0, (1/g_lin - Ron)/(Roff - Ron)),  d_g = 1/R  (FP64, stored FP32).
+ * params (host) = {Ron, Roff, a_on, b_on, a_off, b_off}.  The cell model of
+ *   deal_v / get_nonlinear_ratio_z / nonlinear_R_tensor
+ *   pimtorch/pimfixedpoint/fixedPoint/SplitArrCuda/split_arr_matmul.cu:142-196
+ * d_glin is [B][n][m], or [S][n][m] slots with d_gidx [B]; d_g is [B][n][m].
+ */
+int fadesim_cell_conductance(int batch, int rows, int cols, const float *d_glin, const int *d_gidx,
+                             const double *d_vin, const double *d_vout, const float *d_field,
+                             const double *params, float *d_g, void *stream);
+
+/*
  * One crossbar, FP64 red-black SOR in node voltages with per-segment wire
  * resistances and the reference's stopping rule (stop after the first sweep
  * whose max |dv| < tol; fixed iter_number sweeps without a break).  The

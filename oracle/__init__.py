@@ -164,6 +164,26 @@ def exact_batch(g32, r, vin, vout=None):
     return iout
 
 
+def exact_slots(g32, r, slot, vin):
+    """Exact FP64 readout of K drives over S FP32 conductance slots [S, n, m]
+    (uniform wire resistance r[S] or scalar, vout = 0): instance k uses slot
+    slot[k] and wordline drive vin[k].  Returns iout [K, m]."""
+    g32 = _f(g32)
+    S, n, m = g32.shape
+    r = _d(np.broadcast_to(np.asarray(r, dtype=np.float64), (S,)))
+    slot = np.ascontiguousarray(slot, dtype=np.int32).reshape(-1)
+    K = slot.size
+    if K and (slot.min() < 0 or slot.max() >= S):
+        raise ValueError("slot out of range")
+    vin = _d(vin).reshape(K, n)
+    iout = np.empty((K, m))
+    rc = lib().oracle_exact_slots_f32g(S, n, m, _pf(g32), _pd(r), K,
+                                       slot.ctypes.data_as(ctypes.c_void_p), _pd(vin), _pd(iout))
+    if rc != 0:
+        raise RuntimeError("oracle_exact_slots_f32g failed")
+    return iout
+
+
 def rb_f64(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0):
     """FP64 red-black SOR.  Returns (u, d, iout, record, sweeps)."""
     g = _d(g)

@@ -16,6 +16,9 @@
  *                      Cholesky (the matrix is SPD).  Pinned against
  *                      scipy.sparse.linalg.spsolve, 2x2 closed forms and the
  *                      KCL/monotonicity invariants (tests/test_oracle.py).
+ *   oracle_exact_slots_f32g  the same solve for many drives over shared
+ *                      conductance slots (one factorisation per slot); the
+ *                      checker of the PimTorch-level crossbar currents.
  *   oracle_errisum     the partial-KCL "error I sum" of concrete_command.h:1293-1319.
  *   oracle_rb_f64      FP64 red-black SOR replay of the GPU drop-in kernel
  *                      (reference stopping rule max|dv| < tol).
@@ -183,22 +186,15 @@ double oracle_errisum(int n, int m, double g_wire, const double *g,
  * vout may be NULL (all bitlines grounded).  Half bandwidth kd = 2m.  Row-major lower band: L[k*(kd+1) + (kd - (k-c))]
  * holds entry (k, c) for c in [k-kd, k].
  */
-int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
-                 int rd_ld, const double *g, const double *vin,
-                 const double *vout, double *u_out, double *d_out, double *iout)
+/* Assembles the banded SPD matrix (lower band, width kd = 2m) of the network
+ * and factors it in place (A = L L^T).  Unknown order: u_ij at 2(i m + j),
+ * d_ij at 2(i m + j) + 1.  Returns 0, or -3 if the matrix is not SPD. */
+static int band_factor(int n, int m, const double *RU, int ru_ld, const double *RD,
+                       int rd_ld, const double *g, double *L)
 {
-    if (n < 1 || m < 1)
-        return -1;
     const long N = 2L * n * m;
     const long kd = 2L * m;
     const long W = kd + 1;
-    double *L = (double *)calloc((size_t)N * W, sizeof(double));
-    double *b = (double *)calloc((size_t)N, sizeof(double));
-    if (!L || !b) {
-        free(L);
-        free(b);
-        return -2;
-    }
 #define BAND(r, c) L[(size_t)(r) * W + (kd - ((r) - (c)))]
 #define RUL(i, j) RU[(size_t)(i) * ru_ld + (j)]
 #define RDL(j, i) RD[(size_t)(j) * rd_ld + (i)]
@@ -211,20 +207,15 @@ int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
             BAND(ku, ku) = gij + aj + aj1;
             if (j > 0)
                 BAND(ku, ku - 2) = -aj;
-            if (j == 0)
-                b[ku] += aj * vin[i];
             double bi = (i > 0) ? 1.0 / RDL(j, i) : 0.0;
             double bi1 = 1.0 / RDL(j, i + 1);
             BAND(kdn, kdn) = gij + bi + bi1;
             BAND(kdn, ku) = -gij;
             if (i > 0)
                 BAND(kdn, kdn - 2L * m) = -bi;
-            if (i == n - 1 && vout)
-                b[kdn] += bi1 * vout[j];
         }
 #undef RUL
 #undef RDL
-    /* in-place Cholesky A = L L^T */
     for (long k = 0; k < N; ++k) {
         long c0 = k - kd < 0 ? 0 : k - kd;
         /* off-diagonal entries of row k */
@@ -247,14 +238,19 @@ int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
 #pragma omp simd reduction(- : s)
         for (long t = 0; t < len; ++t)
             s -= rk[t] * rk[t];
-        if (!(s > 0)) {
-            free(L);
-            free(b);
+        if (!(s > 0))
             return -3;
-        }
         BAND(k, k) = sqrt(s);
     }
-    /* forward L y = b */
+    return 0;
+}
+
+/* Solves L L^T x = b in place for the factor of band_factor. */
+static void band_solve(int n, int m, const double *L, double *b)
+{
+    const long N = 2L * n * m;
+    const long kd = 2L * m;
+    const long W = kd + 1;
     for (long k = 0; k < N; ++k) {
         long c0 = k - kd < 0 ? 0 : k - kd;
         double s = b[k];
@@ -262,7 +258,6 @@ int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
             s -= BAND(k, c) * b[c];
         b[k] = s / BAND(k, k);
     }
-    /* backward L^T x = y */
     for (long k = N - 1; k >= 0; --k) {
         b[k] /= BAND(k, k);
         double xk = b[k];
@@ -271,6 +266,44 @@ int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
             b[c] -= BAND(k, c) * xk;
     }
 #undef BAND
+}
+
+/* Right-hand side of the network for drives vin (left) and vout (bottom, NULL = 0). */
+static void band_rhs(int n, int m, const double *RU, int ru_ld, const double *RD, int rd_ld,
+                     const double *vin, const double *vout, double *b)
+{
+    for (long k = 0; k < 2L * n * m; ++k)
+        b[k] = 0.0;
+    for (int i = 0; i < n; ++i)
+        b[2L * (i * (long)m)] += (1.0 / RU[(size_t)i * ru_ld]) * vin[i];
+    if (vout)
+        for (int j = 0; j < m; ++j)
+            b[2L * ((n - 1) * (long)m + j) + 1] += (1.0 / RD[(size_t)j * rd_ld + n]) * vout[j];
+}
+
+int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
+                 int rd_ld, const double *g, const double *vin,
+                 const double *vout, double *u_out, double *d_out, double *iout)
+{
+    if (n < 1 || m < 1)
+        return -1;
+    const long N = 2L * n * m;
+    const long W = 2L * m + 1;
+    double *L = (double *)calloc((size_t)N * W, sizeof(double));
+    double *b = (double *)calloc((size_t)N, sizeof(double));
+    if (!L || !b) {
+        free(L);
+        free(b);
+        return -2;
+    }
+    int e = band_factor(n, m, RU, ru_ld, RD, rd_ld, g, L);
+    if (e) {
+        free(L);
+        free(b);
+        return e;
+    }
+    band_rhs(n, m, RU, ru_ld, RD, rd_ld, vin, vout, b);
+    band_solve(n, m, L, b);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < m; ++j) {
             u_out[IDX(i, j, m)] = b[2 * IDX(i, j, m)];
@@ -284,6 +317,58 @@ int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
     free(L);
     free(b);
     return 0;
+}
+
+/* Exact solves of K drives over S shared FP32 conductance slots with uniform
+ * wire resistance r[s]: instance k uses slot[k] and vin[k][n] (vout = 0).
+ * One factorisation per slot, OpenMP over slots.  Output iout[k*m + j]. */
+int oracle_exact_slots_f32g(int S, int n, int m, const float *g32, const double *r,
+                            int K, const int *slot, const double *vin, double *iout)
+{
+    int err = 0;
+    const long N = 2L * n * m;
+    const long W = 2L * m + 1;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+    for (int s = 0; s < S; ++s) {
+        size_t nm = (size_t)n * m;
+        double *RU = (double *)malloc((size_t)n * (m + 1) * sizeof(double));
+        double *RD = (double *)malloc((size_t)m * (n + 1) * sizeof(double));
+        double *g = (double *)malloc(nm * sizeof(double));
+        double *L = (double *)calloc((size_t)N * W, sizeof(double));
+        double *b = (double *)malloc((size_t)N * sizeof(double));
+        if (!RU || !RD || !g || !L || !b) {
+            err |= 1;
+        } else {
+            for (size_t t = 0; t < (size_t)n * (m + 1); ++t)
+                RU[t] = r[s];
+            for (size_t t = 0; t < (size_t)m * (n + 1); ++t)
+                RD[t] = r[s];
+            for (size_t t = 0; t < nm; ++t)
+                g[t] = (double)g32[(size_t)s * nm + t];
+            if (band_factor(n, m, RU, m + 1, RD, n + 1, g, L)) {
+                err |= 1;
+            } else {
+                for (int k = 0; k < K; ++k) {
+                    if (slot[k] != s)
+                        continue;
+                    band_rhs(n, m, RU, m + 1, RD, n + 1, vin + (size_t)k * n, NULL, b);
+                    band_solve(n, m, L, b);
+                    double *io = iout + (size_t)k * m;
+                    for (int j = 0; j < m; ++j)
+                        io[j] = 0;
+                    for (int i = 0; i < n; ++i)
+                        for (int j = 0; j < m; ++j)
+                            io[j] += (b[2 * IDX(i, j, m)] - b[2 * IDX(i, j, m) + 1]) * g[IDX(i, j, m)];
+                }
+            }
+        }
+        free(RU);
+        free(RD);
+        free(g);
+        free(L);
+        free(b);
+    }
+    return err ? -1 : 0;
 }
 
 /* Batched exact solve for uniform wire resistance r[k] per instance, FP32

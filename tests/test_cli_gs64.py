@@ -159,7 +159,11 @@ def test_cli_matches_reference_golden(cuda, tmp_path, name):
     _, _, iex = oracle.exact(g, z["vin"], z["vout"], RU, z["RD"])
     assert np.max(np.abs(iout - iex) / np.abs(iex)) < 1e-9
     if int(z["record"]) >= 0 and float(z["tol"]) <= 1e-10:
-        assert np.max(np.abs(iout - z["iout"]) / np.abs(z["iout"])) < 5e-8
+        # the reference stopped on max|dv| < tol, a few 1e-8 short of the fixed
+        # point: the two answers differ by that iteration error and no more
+        ref_err = np.max(np.abs(z["iout"] - iex) / np.abs(iex))
+        assert ref_err < 1e-7
+        assert np.max(np.abs(iout - z["iout"]) / np.abs(z["iout"])) <= ref_err + 2e-9
     log = open(tmp_path / "default.log").read()
     assert re.search(r"threshould break, iter times =  \d+", log)
     assert "Duration time = " in log

@@ -1,0 +1,110 @@
+"""PimTorch-level crossbar MVM (fadesim_amd.pim) on the CPU: quantization
+against hand-computed cases and the numpy restatement, the ideal-crossbar
+pipeline against the restatement and against the integer product it
+approximates, and the nonlinear-cell constants against their closed forms."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pim as opim
+from fadesim_amd import pim
+
+
+def test_quantization_hand_cases():
+    x = torch.tensor([0.5, -1.0, 0.25, 0.0])
+    q, s = pim.quantization_tensor(x, 8, pim.TensorType.Normal)
+    assert s == -6 and q.tolist() == [32, -64, 16, 0]          # s = ceil(log2(1/127))
+    q, s = pim.quantization_tensor(x, 8, pim.TensorType.PN)
+    assert s == -7 and q.tolist() == [64, -128, 32, 0]         # s = ceil(log2(1/255))
+    # round half to even: 2.5 -> 2, 3.5 -> 4 at resolution 1 (max 127 -> s = 0)
+    q, s = pim.quantization_tensor(torch.tensor([127.0, 2.5, 3.5, -2.5]), 8, pim.TensorType.Normal)
+    assert s == 0 and q.tolist() == [127, 2, 4, -2]
+    # all-zero tensor: max taken as 1e-12
+    q, s = pim.quantization_tensor(torch.zeros(3), 8, pim.TensorType.Normal)
+    assert s == math.ceil(math.log2(1e-12 / 127)) and q.tolist() == [0, 0, 0]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_quantization_matches_restatement(seed):
+    rng = np.random.default_rng(seed)
+    x = (rng.standard_normal(1000) * 10 ** rng.uniform(-3, 3)).astype(np.float32)
+    for bits in (4, 8):
+        for kind, tt in (("normal", pim.TensorType.Normal), ("pn", pim.TensorType.PN)):
+            q, s = pim.quantization_tensor(torch.from_numpy(x), bits, tt)
+            qo, so = opim.quantize(x, bits, kind)
+            assert s == so and np.array_equal(q.numpy(), qo)
+
+
+def _near_tie_bound(info, rescale, outsize):
+    """Largest output change that ADC codes within 1e-4 of a rounding boundary
+    can cause (each such code may round either way)."""
+    total = 0.0
+    for (i, j, sign), a in info["adc_in"].items():
+        frac = np.abs(a - np.floor(a) - 0.5)
+        n = int((frac < 1e-4).sum())
+        total += n * rescale * 2.0 ** (i + j + info["s1"] + info["s2"])
+    return total
+
+
+@pytest.mark.parametrize("insize,outsize,rows,cell_bits,seed", [(100, 30, 64, 1, 0), (200, 70, 64, 2, 1),
+                                                                (130, 10, 128, 2, 2)])
+def test_ideal_crossbar_matches_restatement(insize, outsize, rows, cell_bits, seed):
+    gen = torch.Generator().manual_seed(seed)
+    x = torch.rand(3, insize, generator=gen) * 2 - 0.5
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="none", cellBits=cell_bits, arrRowSize=rows, arrColSize=rows)
+    out = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=P)
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), cell_bits=cell_bits, rows=rows, cols=rows,
+                                  vmax=P.inputMaxVoltage, ron=P.Ron, roff=P.Roff)
+    rescale = rows * (2 ** cell_bits - 1) / 1023
+    bound = _near_tie_bound(info, rescale, outsize)
+    assert np.max(np.abs(out.numpy() - ref)) <= bound + 1e-6 * np.abs(ref).max()
+
+
+def test_ideal_crossbar_approximates_integer_product():
+    """With no IR drop the crossbar computes the quantized product up to ADC
+    rounding: |out - q_x q_w 2^(s1+s2)| <= sum over (bit, slice, sign) of
+    numX * outsize-independent half-code errors."""
+    gen = torch.Generator().manual_seed(7)
+    insize, outsize, rows = 150, 20, 64
+    x = torch.rand(4, insize, generator=gen)
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="none")
+    out = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=P).double()
+    qx, s1 = pim.quantization_tensor(x, 8, pim.TensorType.Normal)
+    qw, s2 = pim.quantization_tensor(w, 8, pim.TensorType.PN)
+    exact = (qx.double() @ qw.double()) * 2.0 ** (s1 + s2)
+    numX = -(-insize // rows)
+    rescale = rows / 1023
+    # Gmin leakage cancels between the pos and neg arrays; each code rounds by <= 1/2
+    bound = sum(2 * numX * 0.5 * rescale * 2.0 ** (i + j + s1 + s2) for i in range(8) for j in range(8))
+    assert torch.max(torch.abs(out - exact)).item() <= bound
+    assert torch.max(torch.abs(out - exact)).item() < 0.05 * exact.abs().max().item()
+
+
+def test_sinh_parameters_closed_form():
+    P = pim.mlp_inference_params()
+    Ron, Roff, a_on, b_on, a_off, b_off = P.nonlinear_params
+    assert (Ron, Roff) == (1e6, 2e6)
+    assert a_on == pytest.approx(2.5, rel=1e-12) and a_off == pytest.approx(2.5, rel=1e-12)
+    # chord resistance at 1 V: V / (b sinh(a V)) = R
+    assert 1.0 / (b_on * math.sinh(a_on)) == pytest.approx(Ron, rel=1e-12)
+    assert 1.0 / (b_off * math.sinh(a_off)) == pytest.approx(Roff, rel=1e-12)
+    assert P.adcUseI == pytest.approx(128 * (1e-6 - 5e-7) * 0.2)
+
+
+def test_tile_instances_skip_zero_vectors():
+    bits = torch.zeros(3, 2, 2, 8, dtype=torch.int64)
+    bits[0, 1, 0, 3] = 1
+    bits[2, 0, 1, 0] = 1
+    ii, bb, xx, jj, ss, yy = pim.tile_instances(bits, nslice=4, numY=3)
+    assert ii.numel() == 2 * 4 * 2 * 3
+    assert set(zip(ii.tolist(), bb.tolist(), xx.tolist())) == {(0, 1, 0), (2, 0, 1)}
+
+
+def test_converged_mode_refuses_cpu():
+    P = pim.PhyArrParams(ir_mode="converged")
+    with pytest.raises(RuntimeError):
+        pim.splitArr_matmul_nn(torch.rand(1, 10), torch.randn(10, 3), 8, 1, 10, 10, 8, params=P)

@@ -1,0 +1,120 @@
+"""PimTorch-level crossbar MVM through the GPU solvers: the converged batched
+solve against the restatement driven by exact FP64 tile solves, the
+reference-iteration mode against the restatement driven by the CPU hook
+replay, and the sinh-nonlinear Picard solve against the converged fixed
+point of the CPU hook replay."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import pim as opim
+
+pytestmark = pytest.mark.gpu
+
+
+def _exact_currents(r, C):
+    def cur(vin, G):
+        bsize, numX, R = vin.shape
+        numY = G.shape[2] // C
+        slots = G.reshape(numX, R, numY, C).transpose(0, 2, 1, 3).reshape(numX * numY, R, C)
+        b, x, y = np.meshgrid(np.arange(bsize), np.arange(numX), np.arange(numY), indexing="ij")
+        b, x, y = b.ravel(), x.ravel(), y.ravel()
+        v = vin[b, x].astype(np.float64)
+        out = np.zeros((bsize, numX, numY, C))
+        nz = v.any(1)
+        if nz.any():
+            out[b[nz], x[nz], y[nz]] = oracle.exact_slots(slots, r, (x * numY + y)[nz], v[nz])
+        return out.reshape(bsize, numX, numY * C)
+    return cur
+
+
+def _tie_bound(info, rescale, window):
+    total = 0.0
+    for (i, j, sign), a in info["adc_in"].items():
+        frac = np.abs(a - np.floor(a) - 0.5)
+        n = int((frac < window).sum())
+        total += n * rescale * 2.0 ** (i + j + info["s1"] + info["s2"])
+    return total
+
+
+def test_converged_linear_layer_matches_exact_restatement(cuda):
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(11)
+    insize, outsize, bsize = 200, 40, 2
+    x = torch.rand(bsize, insize, generator=gen)
+    w = torch.randn(insize, outsize, generator=gen)
+    # 64x64 tiles keep the exact CPU solves short; the pipeline is the same
+    P = pim.mlp_inference_params(nonlinear_cell_enable=False, arrRowSize=64, arrColSize=64)
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, params=P).cpu().numpy()
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), cell_bits=2, rows=64, cols=64, vmax=0.2,
+                                  ron=P.Ron, roff=P.Roff, currents=_exact_currents(P.r_wire, 64))
+    # the solve is within 1e-6 of the exact currents: an ADC input a moves by at
+    # most ~1e-6 * 1023, so codes within that window of a tie may round either way
+    bound = _tie_bound(info, 64 * 3 / 1023, 2e-3)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-5 * np.abs(ref).max()
+    assert P.stats["solves"] > 0
+
+
+def test_reference_mode_layer_matches_hook_restatement(cuda):
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(12)
+    insize, outsize, bsize = 70, 20, 2
+    x = torch.rand(bsize, insize, generator=gen) - 0.3
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="reference", ir_drop_iter_time=40)
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, params=P).cpu().numpy()
+
+    def hook_currents(vin, G):
+        bsize_, numX, R = vin.shape
+        numY = G.shape[2] // 64
+        o, _ = oracle.hook_gs(vin.astype(np.float32), G.astype(np.float32), bsize_, R, 64, numX, numY,
+                              40, 1 / 2.93, 1.90, False, 1e-7, False, [])
+        return o
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=hook_currents)
+    bound = _tie_bound(info, 64 / 1023, 1e-4)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
+
+
+def test_nonlinear_picard_matches_hook_fixed_point(cuda):
+    import torch
+    from fadesim_amd import pim
+    from fadesim_amd.solver import solve_batch_nonlinear
+    P = pim.mlp_inference_params()
+    rng = np.random.default_rng(5)
+    B, n = 3, 64
+    s = rng.integers(0, 4, (B, n, n))
+    G = (s.astype(np.float32) * np.float32(P.cellDeltaConduct) + np.float32(P.cellMinConduct)).astype(np.float32)
+    vin = (rng.random((B, n)) < 0.5) * 0.2
+    res = solve_batch_nonlinear(torch.tensor(G, device=cuda), P.r_wire, torch.tensor(vin, device=cuda),
+                                params=P.nonlinear_params, tol=1e-6)
+    assert int((res.status != 0).sum()) == 0 and res.outer >= 2
+    got = res.iout.cpu().numpy()
+    for b in range(B):
+        ref, _ = oracle.hook_gs(vin[b][None, None].astype(np.float64), G[b][None].astype(np.float64), 1, n, n, 1, 1,
+                                200000, 1 / P.r_wire, 1.9, True, 1e-14, True, P.nonlinear_params)
+        ref = ref.reshape(n)
+        assert np.max(np.abs(got[b] - ref)) / np.max(np.abs(ref)) <= 1e-6
+
+
+def test_mlp_inference_plain_vs_warm(cuda):
+    """The MLP workload end to end on a few images: logits of the NN-warm-
+    started solves equal those of the plain solves (ADC codes can differ only
+    at rounding ties)."""
+    import torch
+    from fadesim_amd import pim, warmstart
+    net = warmstart.load_default(cuda)
+    if net is None:
+        pytest.skip("no trained weights committed")
+    gen = torch.Generator().manual_seed(3)
+    imgs = torch.rand(4, 784, generator=gen).to(cuda)
+    P = pim.mlp_inference_params()
+    model = pim.MLP(params=P, seed=0).to(cuda).eval()
+    plain = model(imgs)
+    s_plain = P.stats["sweeps"]
+    P.warm_start = net
+    warm = model(imgs)
+    s_warm = P.stats["sweeps"]
+    assert torch.max(torch.abs(plain - warm)).item() <= 0.02 * torch.max(torch.abs(plain)).item()
+    assert s_warm < s_plain
