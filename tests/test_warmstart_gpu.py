@@ -125,7 +125,8 @@ def test_network_gradients_match_torch_autograd(cuda):
     torch.cuda.synchronize()
     wref = net.wb.cpu().double().requires_grad_(True)
     o = _torch_net(net, x.double(), wref)
-    l = 0.5 * ((o[..., 0:2].permute(0, 3, 1, 2) - tgt.double()) ** 2).mean()
+    # the kernel's loss: sum over pixels and both output channels of d^2 / (2 P), P = pixels
+    l = ((o[..., 0:2].permute(0, 3, 1, 2) - tgt.double()) ** 2).sum() / (2 * B * 128 * 128)
     l.backward()
     assert abs(loss.item() - l.item()) <= 0.05 * l.item()
     g = grad.cpu().double()
