@@ -60,6 +60,8 @@ class HookDesc(ctypes.Structure):
 EXPORTED = [
     "fadesim_last_error", "fadesim_version", "fadesim_solve_batch", "fadesim_default_omega",
     "fadesim_last_kernel_ms", "fadesim_cell_conductance",
+    "fadesim_ragged_plan_create", "fadesim_ragged_plan_destroy", "fadesim_ragged_plan_info",
+    "fadesim_ragged_solve", "fadesim_ragged_aborted", "fadesim_ragged_last_ms",
     "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs",
     "fadesim_nn_features", "fadesim_nn_combine", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
     "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
@@ -93,6 +95,7 @@ def lib():
             L.fadesim_hook_ir_drop_gs.argtypes = [ctypes.POINTER(HookDesc), _vp]
             L.fadesim_hook_ir_drop_gs.restype = ctypes.c_int
         _declare_nn(L)
+        _declare_ragged(L)
         _LIB = L
     return _LIB
 
@@ -121,6 +124,42 @@ def _declare_nn(L):
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = ctypes.c_int
+
+
+class RaggedClassOpts(ctypes.Structure):
+    _fields_ = [("omega", ctypes.c_double), ("check_first", ctypes.c_int), ("check_min", ctypes.c_int),
+                ("check_max", ctypes.c_int)]
+
+
+class RaggedPlanStats(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int), ("cells", ctypes.c_longlong), ("rows_total", ctypes.c_longlong),
+                ("cols_total", ctypes.c_longlong), ("grid", ctypes.c_int), ("teams", ctypes.c_int),
+                ("big_tasks", ctypes.c_int), ("small_items", ctypes.c_int), ("omega", ctypes.c_double * 4),
+                ("check_first", ctypes.c_int * 4), ("check_min", ctypes.c_int * 4), ("check_max", ctypes.c_int * 4)]
+
+
+class RaggedDesc(ctypes.Structure):
+    _fields_ = [("d_g", _vp), ("d_r_wire", _vp), ("d_vin", _vp), ("d_vout", _vp), ("d_warm", _vp),
+                ("tol", ctypes.c_double), ("safety", ctypes.c_double), ("switch_tol", ctypes.c_double),
+                ("max_sweeps", ctypes.c_int), ("flags", ctypes.c_uint), ("d_iout", _vp), ("d_sweeps", _vp),
+                ("d_status", _vp), ("d_est", _vp), ("d_field", _vp)]
+
+
+def _declare_ragged(L):
+    if not hasattr(L, "fadesim_ragged_plan_create"):
+        return
+    L.fadesim_ragged_plan_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _vp, _vp, _vp]
+    L.fadesim_ragged_plan_create.restype = ctypes.c_int
+    L.fadesim_ragged_plan_destroy.argtypes = [_vp]
+    L.fadesim_ragged_plan_destroy.restype = None
+    L.fadesim_ragged_plan_info.argtypes = [_vp, ctypes.POINTER(RaggedPlanStats)]
+    L.fadesim_ragged_plan_info.restype = ctypes.c_int
+    L.fadesim_ragged_solve.argtypes = [_vp, ctypes.POINTER(RaggedDesc), _vp]
+    L.fadesim_ragged_solve.restype = ctypes.c_int
+    L.fadesim_ragged_aborted.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.fadesim_ragged_aborted.restype = ctypes.c_int
+    L.fadesim_ragged_last_ms.argtypes = [_vp]
+    L.fadesim_ragged_last_ms.restype = ctypes.c_double
 
 
 def check(rc: int, what: str) -> None:

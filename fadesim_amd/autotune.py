@@ -1,0 +1,107 @@
+"""Autotuner of the mixed-size solver's per-size schedule.
+
+For each crossbar size (64, 128, 256, 512) it times the persistent solve of a
+sample of the uneven-size workload (fadesim_amd.workloads.uneven_batch) on the
+GPU over a grid of candidates -- the relaxation factor
+w(k) = 2 / (1 + k sin(pi / (2n + 1))) and the bounds of the interval between
+convergence tests -- and keeps the fastest candidate whose solves all reach
+the tolerance.  The table is stored per GPU architecture under
+fadesim_amd/tuning/<arch>.json and read by RaggedPlan.
+
+python -m fadesim_amd.autotune [--per-size 16,32,64,256] [--tol 1e-6] [--out PATH]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import time
+from typing import Optional
+
+import torch
+
+TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
+KAPPAS = (0.55, 0.7, 0.83, 1.0, 1.25)
+CHECKS = ((32, 16, 256), (64, 32, 256), (32, 32, 512))
+
+
+def arch_name(device) -> str:
+    props = torch.cuda.get_device_properties(device)
+    return getattr(props, "gcnArchName", "unknown").split(":")[0]
+
+
+def table_path(device) -> str:
+    return os.path.join(TUNING_DIR, arch_name(device) + ".json")
+
+
+def load_table(device) -> Optional[dict]:
+    """The tuned per-size controls for this device's architecture, or None."""
+    try:
+        p = table_path(device)
+    except Exception:
+        return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("classes")
+
+
+def omega_of(n: int, kappa: float) -> float:
+    return 2.0 / (1.0 + kappa * math.sin(math.pi / (2 * n + 1)))
+
+
+def tune_size(n: int, count: int, tol: float, dev, log=print, reps: int = 2) -> dict:
+    from .ragged import RaggedPlan
+    from .workloads import uneven_batch
+    inst = uneven_batch({n: count}, seed=4242 + n, device=dev)
+    results = []
+    for kappa in KAPPAS:
+        for cf, cmin, cmax in CHECKS:
+            entry = {"omega": omega_of(n, kappa), "check_first": cf, "check_min": cmin, "check_max": cmax}
+            plan = RaggedPlan(inst["rows"], device=dev, tuning={str(n): entry})
+            res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=tol)  # warm-up, and correctness
+            ok = int((res.status != 0).sum()) == 0
+            times = []
+            for _ in range(reps):
+                plan.solve(inst["g"], inst["r"], inst["vin"], tol=tol, timing=True)
+                times.append(plan.last_ms())
+            ms = min(times)
+            sw = float(res.sweeps.double().mean())
+            results.append((ms if ok else math.inf, kappa, entry, sw))
+            log(f"  n={n} kappa={kappa:.2f} w={entry['omega']:.5f} checks={cf}/{cmin}/{cmax}: "
+                f"{ms:.3f} ms, mean sweeps {sw:.0f}{'' if ok else ' (not converged)'}")
+            del plan
+    best = min(results, key=lambda x: x[0])
+    out = dict(best[2])
+    out.update({"kappa": best[1], "ms": best[0], "mean_sweeps": best[3], "sample": count})
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--per-size", default="256,128,64,16",
+                    help="instances per size 64,128,256,512 in the timing sample")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    dev = torch.device("cuda:0")
+    counts = [int(x) for x in args.per_size.split(",")]
+    log = lambda s: print(s, flush=True)
+    t0 = time.time()
+    classes = {}
+    for n, c in zip((64, 128, 256, 512), counts):
+        log(f"size {n}: {c} instances")
+        classes[str(n)] = tune_size(n, c, args.tol, dev, log)
+        log(f"size {n}: best {classes[str(n)]}")
+    table = {"arch": arch_name(dev), "device": torch.cuda.get_device_name(dev), "tol": args.tol,
+             "seconds": time.time() - t0, "candidates": {"kappa": KAPPAS, "checks": CHECKS}, "classes": classes}
+    out = args.out or table_path(dev)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    with open(out, "w") as f:
+        json.dump(table, f, indent=1)
+    log("wrote " + out)
+
+
+if __name__ == "__main__":
+    main()

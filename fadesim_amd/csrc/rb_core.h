@@ -1,0 +1,779 @@
+// Device core of the red-black SOR crossbar solvers (uniform batches in
+// rbsor.hip, mixed sizes in ragged.hip): FP32 sweeps on drop variables, FP64
+// adjoint-weighted residual test, one FP64 iterative-refinement step, FP64
+// readout.
+//
+// Network and update rule follow NodebasedGSMethod::nodeBaseGS
+// (crossbarHspice/V2/concrete_command.h:1218-1253): the same KCL equations
+// relaxed by SOR, here in red-black order (colour of the wordline node u_ij is
+// (i+j)&1, of the bitline node d_ij (i+j+1)&1; every coupling joins opposite
+// colours, so one colour is updated in parallel while the other is frozen).
+//
+// Variables are the drops p = vin_i - u_ij >= 0 and q = d_ij - vout_j >= 0
+// with c_ij = vin_i - vout_j.  With wire conductance a = 1/r the KCL rows are
+//   (g + a_j + a_{j+1}) p - a p_{j-1} - a p_{j+1} + g q = g c   (p_{-1} = 0)
+//   (g + b_i + b_{i+1}) q - b q_{i-1} - b q_{i+1} + g p = g c   (q_n = 0)
+// (no a_{j+1} term at the open right end, no b_i term at the open top).
+// SOR with the diagonal taken as 2a (a at the open ends) -- the cell term is
+// ~1e-6 of it -- gives for a node x with line neighbours x1, x2:
+//   tt = fma(-h, x + x_other, rho),  x' = fma(w/2, x1 + x2, fma(1-w, x, tt))
+//   h = w g / (2a),  rho = w rhs / (2a)  (rhs = g c, or the FP64 residual
+//   after refinement).  At an open end (D~ = a) the missing neighbour is the
+//   ghost x2 = x1 + (2/w) tt, which turns the update into the exact one
+//   (1-w) x + w x1 + 2 tt.
+//
+// Layout: one workgroup per crossbar -- or, for crossbars too large for one
+// CU, per strip of full rows (a team of workgroups solves the crossbar, see
+// StripCtx).  Thread (ty, tx) owns the TR x TC cell tile rows TR*ty..,
+// columns TC*tx.. with p, q, h, rho_p, rho_q in VGPRs.  Per half-sweep a
+// thread reads (TR + TC) halo values from LDS (the neighbours' edge values of
+// the other colour), updates its TR*TC cells and publishes the TR + TC edge
+// values it changed.  Conductances come from HBM once per sweep batch
+// (coefficient reload after each convergence test).
+//
+// Convergence test (every few sweeps, FP64): residual r = b - A x at every
+// node, then the first-order error of the readout
+//   D_j = sum z_j . r,   I_j(x) - I_j* ~= D_j
+// with the adjoint z_j taken from the wire Green's functions (bitline:
+// r (n - max(i,i')) weighted by g of column j; wordline: r (min(j,j')+1)
+// weighted by g_ij).  Stop when max_j |D_j| / |I_j| * safety <= tol.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "fadesim.h"
+
+namespace fadesim {
+namespace rb {
+
+constexpr int kWsPlanes = 7; // workspace planes per slot: zq | rho p,q | snapshot p,q | residual p,q
+constexpr int kMaxStrips = 16;
+constexpr int kRowsLd = 512;                      // row stride of the strip exchange buffers
+constexpr unsigned long long kWaitTicks = 500000000ull; // 5 s of the 100 MHz constant clock
+
+// Solve controls shared by every instance of a size class.
+struct Ctl {
+    float omega;
+    double tol_eff;    // tol / safety
+    double switch_tol; // refinement switch point
+    double dwr_scale;  // DWR: stop at est <= tol_eff * dwr_scale
+    int max_sweeps;
+    unsigned flags;
+    int check_first;   // sweeps before the first test
+    int check_min, check_max; // bounds of the predicted test interval
+};
+
+// One crossbar.  rows is the full row count (all strips).
+struct Inst {
+    const float *g;    // [rows][N]
+    double rwire;
+    const double *vin; // [rows]
+    const double *vout; // [N]; never null (zeros supplied by the host)
+    const float *warm; // [2][rows][N] or null
+    float *field;      // [2][rows][N] or null
+    double *iout;      // [N]
+    int *sweeps;       // one element or null
+    int *status;
+    double *est;
+    int rows;
+};
+
+// Team of S workgroups solving one crossbar, strip s = rows [s*M, (s+1)*M).
+// Neighbouring strips exchange their boundary bitline rows after every
+// half-sweep through `rows` (double-buffered by publication parity) and
+// per-strip publication counters `flag` (release/acquire at agent scope);
+// column sums of the convergence test and of the set-up go through `part` /
+// `setup` and the counter barrier `cnt`.  A wait that does not end within
+// kWaitTicks raises `abort`, which every wait checks, so no workgroup spins
+// forever.
+struct StripCtx {
+    int S, s;
+    float *rows;   // [2 parity][S][2 top/bottom][kRowsLd]
+    int *flag;     // [S][2] publications of the top / bottom row
+    int *cnt;      // barrier arrivals
+    double *part;  // [2 parity][S][2][kRowsLd]
+    double *setup; // [S][4][kRowsLd]
+    int *abort;
+};
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+}
+
+// TC consecutive floats of a row (TC a multiple of 4).
+template <int TC>
+__device__ __forceinline__ void ldrow(const float *p, float (&v)[TC])
+{
+#pragma unroll
+    for (int k = 0; k < TC; k += 4) {
+        const float4 x = ld4(p + k);
+        v[k] = x.x;
+        v[k + 1] = x.y;
+        v[k + 2] = x.z;
+        v[k + 3] = x.w;
+    }
+}
+template <int TC>
+__device__ __forceinline__ void strow(float *p, const float (&v)[TC])
+{
+#pragma unroll
+    for (int k = 0; k < TC; k += 4)
+        st4(p + k, v[k], v[k + 1], v[k + 2], v[k + 3]);
+}
+
+// Geometry: TR x TC cells per thread, TY x TX threads, M x N cells per
+// workgroup (a whole crossbar, or one strip of full rows).
+template <int TR_, int TC_, int TY_, int TX_>
+struct Geo {
+    static constexpr int TR = TR_, TC = TC_, TY = TY_, TX = TX_;
+    static constexpr int NT = TY * TX;
+    static constexpr int M = TR * TY, N = TC * TX;
+    static constexpr int NW = NT / 64;
+    // LDS halo planes: p left/right columns [TR][NT]; q top/bottom rows
+    // [TC][NT + 2 TX] with thread t at TX + t -- the virtual thread rows -1 and
+    // TY hold the neighbouring strips' boundary rows.
+    static constexpr int QLD = NT + 2 * TX;
+    static constexpr int PL = 0, PR = TR * NT, QT = 2 * TR * NT, QB = 2 * TR * NT + TC * QLD;
+    static constexpr int HALO = 2 * TR * NT + 2 * TC * QLD;
+    static_assert(TR % 2 == 0 && TC % 4 == 0, "tile sides: TR even, TC multiple of 4");
+    static_assert(NT % 64 == 0 && 64 % TX == 0 || TX % 64 == 0, "tile rows must not straddle waves");
+    static_assert(NT >= M && NT >= N, "one thread per row and per column in the test");
+    static_assert(N <= kRowsLd, "strip rows exceed the exchange buffer");
+};
+
+template <int COL, class G>
+__device__ __forceinline__ void half_sweep(float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC],
+                                           const float (&hn)[G::TR][G::TC],
+                                           const float (&rp)[G::TR][G::TC],
+                                           const float (&rq)[G::TR][G::TC], float *halo, int t,
+                                           bool has_l, bool has_r, bool has_u, bool has_d,
+                                           float cA, float cB, float c2)
+{
+    constexpr int TR = G::TR, TC = G::TC, NT = G::NT, TX = G::TX, QLD = G::QLD;
+    float hl[TR], hr[TR], hu[TC], hd[TC];
+#pragma unroll
+    for (int k = 0; k < TR; ++k) {
+        hl[k] = hr[k] = 0.f;
+        if ((k & 1) == COL) { // left neighbour of p at (k,0); the source at j = 0
+            const float v = halo[G::PR + k * NT + (has_l ? t - 1 : t)];
+            hl[k] = has_l ? v : 0.f;
+        } else { // right neighbour of p at (k,TC-1)
+            hr[k] = halo[G::PL + k * NT + (has_r ? t + 1 : t)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TC; ++k) {
+        hu[k] = hd[k] = 0.f;
+        if ((k & 1) != COL) { // upper neighbour of q at (0,k)
+            hu[k] = halo[G::QB + k * QLD + t]; // thread row ty-1 (virtual row -1: upper strip)
+        } else { // lower neighbour of q at (TR-1,k); the sink below the last row
+            const float v = halo[G::QT + k * QLD + 2 * TX + t]; // thread row ty+1
+            hd[k] = has_d ? v : 0.f;
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const float s = yp[ii][jj] + yq[ii][jj];
+            if (((ii + jj) & 1) == COL) {
+                const float tt = fmaf(hn[ii][jj], s, rp[ii][jj]);
+                const float n1 = jj > 0 ? yp[ii][jj - 1] : hl[ii];
+                float n2;
+                if (jj < TC - 1)
+                    n2 = yp[ii][jj + 1];
+                else
+                    n2 = has_r ? hr[ii] : fmaf(c2, tt, n1); // open right end
+                yp[ii][jj] = fmaf(cB, n1 + n2, fmaf(cA, yp[ii][jj], tt));
+            } else {
+                const float tt = fmaf(hn[ii][jj], s, rq[ii][jj]);
+                const float n2 = ii < TR - 1 ? yq[ii + 1][jj] : hd[jj];
+                float n1;
+                if (ii > 0)
+                    n1 = yq[ii - 1][jj];
+                else
+                    n1 = has_u ? hu[jj] : fmaf(c2, tt, n2); // open top end
+                yq[ii][jj] = fmaf(cB, n1 + n2, fmaf(cA, yq[ii][jj], tt));
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TR; ++k) {
+        if ((k & 1) == COL)
+            halo[G::PL + k * NT + t] = yp[k][0];
+        else
+            halo[G::PR + k * NT + t] = yp[k][TC - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < TC; ++k) {
+        if ((k & 1) != COL)
+            halo[G::QT + k * QLD + TX + t] = yq[0][k];
+        else
+            halo[G::QB + k * QLD + TX + t] = yq[TR - 1][k];
+    }
+    __syncthreads();
+}
+
+template <class G>
+__device__ __forceinline__ void publish_all(const float (&yp)[G::TR][G::TC],
+                                            const float (&yq)[G::TR][G::TC], float *halo, int t)
+{
+    constexpr int NT = G::NT, QLD = G::QLD, TX = G::TX;
+#pragma unroll
+    for (int k = 0; k < G::TR; ++k) {
+        halo[G::PL + k * NT + t] = yp[k][0];
+        halo[G::PR + k * NT + t] = yp[k][G::TC - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < G::TC; ++k) {
+        halo[G::QT + k * QLD + TX + t] = yq[0][k];
+        halo[G::QB + k * QLD + TX + t] = yq[G::TR - 1][k];
+    }
+}
+
+// ---------------------------------------------------------------- strip exchange
+
+// Spin until *p >= target (agent-scope acquire).  All active lanes of the
+// wave call it together; false when the team aborted or the wait timed out.
+__device__ __forceinline__ bool wait_geq(const int *p, int target, int *abort)
+{
+    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        const unsigned long long t0 = wall_clock64();
+        while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                return false;
+            if (wall_clock64() - t0 > kWaitTicks) {
+                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Publication `pub` (0-based) of this strip's boundary bitline rows: the
+// top-row threads write row 0, the bottom-row threads row M-1, each group
+// then releases its counter to pub + 1.
+template <class G>
+__device__ __forceinline__ void strip_publish(const StripCtx &X, const float (&yq)[G::TR][G::TC], int ty,
+                                              int j0, int lane, int pub)
+{
+    const int par = pub & 1;
+    if (ty == 0 && X.s > 0) {
+        float *dst = X.rows + (((size_t)par * X.S + X.s) * 2 + 0) * kRowsLd + j0;
+        strow<G::TC>(dst, yq[0]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if ((lane % G::TX) == 0)
+            __hip_atomic_store(X.flag + 2 * X.s + 0, pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (ty == G::TY - 1 && X.s < X.S - 1) {
+        float *dst = X.rows + (((size_t)par * X.S + X.s) * 2 + 1) * kRowsLd + j0;
+        strow<G::TC>(dst, yq[G::TR - 1]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if ((lane % G::TX) == 0)
+            __hip_atomic_store(X.flag + 2 * X.s + 1, pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Loads the neighbours' publication `pub` into the virtual halo rows: the
+// upper strip's bottom row above thread row 0, the lower strip's top row below
+// thread row TY-1.  Ends with a workgroup barrier; returns false (uniformly)
+// when the team aborted.
+template <class G>
+__device__ __forceinline__ bool strip_fetch(const StripCtx &X, float *halo, int *s_dead, int tx, int ty, int pub)
+{
+    constexpr int TC = G::TC, TX = G::TX, QLD = G::QLD;
+    const int par = pub & 1;
+    const int j0 = TC * tx;
+    if (ty == 0 && X.s > 0) {
+        if (wait_geq(X.flag + 2 * (X.s - 1) + 1, pub + 1, X.abort)) {
+            const float *src = X.rows + (((size_t)par * X.S + X.s - 1) * 2 + 1) * kRowsLd + j0;
+            float v[TC];
+            ldrow<TC>(src, v);
+#pragma unroll
+            for (int k = 0; k < TC; ++k)
+                halo[G::QB + k * QLD + tx] = v[k]; // virtual thread row -1
+        } else {
+            *s_dead = 1;
+        }
+    }
+    if (ty == G::TY - 1 && X.s < X.S - 1) {
+        if (wait_geq(X.flag + 2 * (X.s + 1) + 0, pub + 1, X.abort)) {
+            const float *src = X.rows + (((size_t)par * X.S + X.s + 1) * 2 + 0) * kRowsLd + j0;
+            float v[TC];
+            ldrow<TC>(src, v);
+#pragma unroll
+            for (int k = 0; k < TC; ++k)
+                halo[G::QT + k * QLD + TX + G::NT + tx] = v[k]; // virtual thread row TY
+        } else {
+            *s_dead = 1;
+        }
+    }
+    __syncthreads();
+    return *s_dead == 0;
+}
+
+// Team barrier number `nbar` (0-based): every member's stores before it are
+// visible to every member after it.  Ends with a workgroup barrier; false
+// (uniformly) when the team aborted.
+__device__ __forceinline__ bool team_barrier(const StripCtx &X, int *s_dead, int t, int nbar)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (t == 0) {
+        __hip_atomic_fetch_add(X.cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!wait_geq(X.cnt, X.S * (nbar + 1), X.abort))
+            *s_dead = 1;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *s_dead == 0;
+}
+
+// ---------------------------------------------------------------- coefficients
+
+// Sweep coefficients: hn = -(w/2a) g and rho.  Before the refinement
+// rho = (w/2a) g c for both nodes of a cell; after it rho is the scaled FP64
+// residual the refinement step stored in the workspace.
+template <class G>
+__device__ __forceinline__ void load_coeffs(const Inst &I, const float *wsb, const float *gp, int i0, int j0,
+                                            int gi0, double hcoef, bool refined, float (&hn)[G::TR][G::TC],
+                                            float (&rp)[G::TR][G::TC], float (&rq)[G::TR][G::TC])
+{
+    constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        float g[TC];
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, g);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            hn[ii][jj] = (float)(-((double)g[jj] * hcoef));
+        if (refined) {
+            ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rp[ii]);
+            ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rq[ii]);
+        } else {
+            const double vi = I.vin[gi0 + i0 + ii];
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj) {
+                const double c = vi - I.vout[j0 + jj];
+                rp[ii][jj] = rq[ii][jj] = (float)(((double)g[jj] * c) * hcoef);
+            }
+        }
+    }
+}
+
+struct Shared {
+    float *halo;
+    double *S1, *Ifloor, *colI, *colD;
+    float *wmax;
+    int *dead;
+};
+
+// Convergence test, run on the snapshot of the drops in the workspace (only a
+// few values per thread are live).  For every node the FP64 residual
+// r = b - A x (b = g c before the refinement, the stored refinement residual
+// after it); then
+//   D_j = sum_i [ g_ij r W_i(j) + zq_ij r_q(i,j) ],  W_i(j) = sum_j' (min(j,j')+1) r_p(i,j')
+//   I_j = S1_j - sum_i g_ij (p_ij + q_ij)
+// with D_j the adjoint-weighted first-order error of I_j.  want_rho stores the
+// scaled residual as the refinement right-hand side instead of testing.
+// Strips: the rows above/below the strip come from the neighbours'
+// publication `pub`, and the column sums are completed over the team (team
+// barrier `nbar`, parity buffers `part`).
+// Results: colI[j], colD[j] (threads t < N), return value = max_j |D_j|/|I_j|
+// (negative when the team aborted).
+template <class G, bool STRIP>
+__device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0, int t,
+                            double a, double rwire, double hcoef, bool refined, bool want_rho,
+                            const Shared &sh, int pub, int nbar)
+{
+    constexpr int M = G::M, N = G::N, NT = G::NT, NW = G::NW;
+    const float *zq = wsb;
+    const float *rho_p = wsb + 1 * M * N, *rho_q = wsb + 2 * M * N;
+    const float *fp = wsb + 3 * M * N, *fq = wsb + 4 * M * N;
+    float *resp = wsb + 5 * M * N, *resq = wsb + 6 * M * N;
+    const float *g = gp;
+    const float *up_row = nullptr, *dn_row = nullptr;
+    bool top_open = true, bottom_sink = true;
+    if (STRIP) {
+        const int par = pub & 1;
+        if (X.s > 0) {
+            up_row = X.rows + (((size_t)par * X.S + X.s - 1) * 2 + 1) * kRowsLd;
+            top_open = false;
+        }
+        if (X.s < X.S - 1) {
+            dn_row = X.rows + (((size_t)par * X.S + X.s + 1) * 2 + 0) * kRowsLd;
+            bottom_sink = false;
+        }
+        // the neighbours' rows of publication pub must have arrived
+        if (t == 0) {
+            if (X.s > 0 && !wait_geq(X.flag + 2 * (X.s - 1) + 1, pub + 1, X.abort))
+                *sh.dead = 1;
+            if (X.s < X.S - 1 && !wait_geq(X.flag + 2 * (X.s + 1) + 0, pub + 1, X.abort))
+                *sh.dead = 1;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (*sh.dead)
+            return -1.0;
+    }
+    // 1. residuals, one node pair per thread and step
+    for (int k = t; k < M * N; k += NT) {
+        const int i = k / N, j = k - i * N;
+        const double gd = (double)g[k];
+        const double x = (double)fp[k], z = (double)fq[k];
+        const double cell = gd * (x + z);
+        double bp, bq;
+        if (refined) {
+            bp = (double)rho_p[k] / hcoef;
+            bq = (double)rho_q[k] / hcoef;
+        } else {
+            bp = bq = gd * (I.vin[gi0 + i] - I.vout[j]);
+        }
+        const double pl = j > 0 ? (double)fp[k - 1] : 0.0;
+        const double wp = (j < N - 1) ? a * ((pl - x) + ((double)fp[k + 1] - x)) : a * (pl - x);
+        double qd;
+        if (i < M - 1)
+            qd = (double)fq[k + N];
+        else
+            qd = bottom_sink ? 0.0 : (double)dn_row[j];
+        double wq;
+        if (i > 0)
+            wq = a * (((double)fq[k - N] - z) + (qd - z));
+        else
+            wq = top_open ? a * (qd - z) : a * (((double)up_row[j] - z) + (qd - z));
+        const double rpv = (bp - cell) + wp, rqv = (bq - cell) + wq;
+        if (want_rho) {
+            wsb[1 * M * N + k] = (float)(rpv * hcoef);
+            wsb[2 * M * N + k] = (float)(rqv * hcoef);
+        } else {
+            resp[k] = (float)rpv;
+            resq[k] = (float)rqv;
+        }
+    }
+    __syncthreads();
+    if (want_rho)
+        return 0.0;
+    // 2. wordline adjoint: one thread per row, v = g r W + zq r_q into resp
+    if (t < M) {
+        const int i = t;
+        double R = 0.0;
+        for (int j = 0; j < N; ++j)
+            R += (double)resp[i * N + j];
+        double P0 = 0.0, P1 = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const int k = i * N + j;
+            const double r = (double)resp[k];
+            const double jp1 = (double)(j + 1);
+            const double W = P1 + jp1 * (R - P0);
+            resp[k] = (float)((double)g[k] * rwire * W + (double)zq[k] * (double)resq[k]);
+            P0 += r;
+            P1 += jp1 * r;
+        }
+    }
+    __syncthreads();
+    // 3. column sums, one thread per column (over the team for strips)
+    double D = 0.0, Y = 0.0;
+    if (t < N) {
+        const int j = t;
+        for (int i = 0; i < M; ++i) {
+            const int k = i * N + j;
+            D += (double)resp[k];
+            Y += (double)g[k] * ((double)fp[k] + (double)fq[k]);
+        }
+    }
+    if (STRIP) {
+        double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
+        if (t < N) {
+            mine[t] = D;
+            mine[kRowsLd + t] = Y;
+        }
+        if (!team_barrier(X, sh.dead, t, nbar))
+            return -1.0;
+        if (t < N) {
+            D = Y = 0.0;
+            for (int s = 0; s < X.S; ++s) {
+                const double *ps = X.part + (((size_t)(nbar & 1) * X.S + s) * 2) * kRowsLd;
+                D += ps[t];
+                Y += ps[kRowsLd + t];
+            }
+        }
+    }
+    float emax = 0.f;
+    if (t < N) {
+        const double Ij = sh.S1[t] - Y;
+        sh.colI[t] = Ij;
+        sh.colD[t] = D;
+        const double e = fabs(D) / fmax(fabs(Ij), sh.Ifloor[t]);
+        emax = (e == e) ? (float)fmin(e, 3.0e38) : INFINITY;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        emax = fmaxf(emax, __shfl_xor(emax, off, 64));
+    if ((t & 63) == 0)
+        sh.wmax[t >> 6] = emax;
+    __syncthreads();
+    float em = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        em = fmaxf(em, sh.wmax[w]);
+    // identical in every lane: make it a scalar
+    em = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, em)));
+    return (double)em;
+}
+
+template <class G>
+__device__ __forceinline__ void snapshot(float *wsb, const float (&yp)[G::TR][G::TC],
+                                         const float (&yq)[G::TR][G::TC], int i0, int j0)
+{
+    constexpr int M = G::M, N = G::N;
+#pragma unroll
+    for (int ii = 0; ii < G::TR; ++ii) {
+        strow<G::TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
+        strow<G::TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+    }
+}
+
+// Ideal readout S1, the per-column floor of the relative test and the bitline
+// adjoint zq(i,j) = r sum_i' g(i',j) (n - max(i,i')) of this workgroup's rows
+// [gi0, gi0 + M) of an n-row crossbar (n = S*M for strips: column sums over the
+// team through `setup` and team barrier `nbar`).
+template <class G, bool STRIP>
+__device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, float *wsb, const float *gp,
+                                              int gi0, int t, double rwire, const Shared &sh, int nbar)
+{
+    constexpr int M = G::M, N = G::N;
+    const int n = I.rows;
+    double si = 0.0, sf = 0.0, sg = 0.0, sT = 0.0;
+    if (t < N) {
+        const double vo = I.vout[t];
+        for (int i = 0; i < M; ++i) {
+            const double gd = (double)gp[(size_t)i * N + t];
+            const double c = I.vin[gi0 + i] - vo;
+            si += gd * c;
+            sf += gd * fabs(c);
+            sg += gd;
+            sT += gd * (double)(n - (gi0 + i));
+        }
+    }
+    double Ac = 0.0, Bs = sT;
+    if (STRIP) {
+        if (t < N) {
+            double *mine = X.setup + ((size_t)X.s * 4) * kRowsLd;
+            mine[t] = si;
+            mine[kRowsLd + t] = sf;
+            mine[2 * kRowsLd + t] = sg;
+            mine[3 * kRowsLd + t] = sT;
+        }
+        if (!team_barrier(X, sh.dead, t, nbar))
+            return false;
+        if (t < N) {
+            si = sf = 0.0;
+            Bs = 0.0;
+            for (int s = 0; s < X.S; ++s) {
+                const double *ps = X.setup + ((size_t)s * 4) * kRowsLd;
+                si += ps[t];
+                sf += ps[kRowsLd + t];
+                if (s < X.s)
+                    Ac += ps[2 * kRowsLd + t];
+                else
+                    Bs += ps[3 * kRowsLd + t];
+            }
+        }
+    }
+    if (t < N) {
+        for (int i = 0; i < M; ++i) {
+            const double gd = (double)gp[(size_t)i * N + t];
+            Ac += gd;
+            Bs -= gd * (double)(n - (gi0 + i));
+            wsb[(size_t)i * N + t] = (float)(rwire * ((double)(n - (gi0 + i)) * Ac + Bs));
+        }
+        sh.S1[t] = si;
+        // a column carrying under 1e-9 of its driven current is judged absolutely
+        sh.Ifloor[t] = fmax(1e-9 * sf, 1e-300);
+    }
+    return true;
+}
+
+// Solve one crossbar (STRIP = false) or strip X.s of one (STRIP = true) with
+// workspace slot wsb.  Returns false when the team aborted.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ctl &C, float *wsb,
+                                      const Shared &sh)
+{
+    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY;
+    constexpr int M = G::M, N = G::N;
+    float *halo = sh.halo;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int tx = t % TX, ty = t / TX;
+    const int strip = STRIP ? X.s : 0;
+    const int gi0 = strip * M; // first global row of this workgroup
+    const float *gp = I.g + (size_t)gi0 * N;
+    const int i0 = TR * ty, j0 = TC * tx;
+    const bool top_ext = STRIP && X.s > 0, bot_ext = STRIP && X.s < X.S - 1;
+    const bool has_l = tx > 0, has_r = tx < TX - 1;
+    const bool has_u = ty > 0 || top_ext, has_d = ty < TY - 1 || bot_ext;
+    const size_t MNall = (size_t)I.rows * N;
+
+    const float wf = C.omega;
+    const float cA = 1.0f - wf;
+    const float cB = 0.5f * wf;
+    const float c2 = 2.0f / wf;
+    const double rwire = I.rwire;
+    const double a = 1.0 / rwire;
+    const double hcoef = 0.5 * (double)wf * rwire;
+    const bool fixed = (C.flags & FADESIM_F_FIXED_SWEEPS) != 0;
+    const bool dwr = (C.flags & FADESIM_F_DWR) != 0;
+    int pub = 0, nbar = 0;
+    if (t == 0)
+        *sh.dead = 0;
+    __syncthreads();
+
+    if (!setup_columns<G, STRIP>(I, X, wsb, gp, gi0, t, rwire, sh, nbar++))
+        return false;
+
+    float yp[TR][TC], yq[TR][TC], hn[TR][TC], rp[TR][TC], rq[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        if (I.warm) {
+            ldrow<TC>(I.warm + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
+            ldrow<TC>(I.warm + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                yp[ii][jj] = yq[ii][jj] = 0.f;
+        }
+    }
+    publish_all<G>(yp, yq, halo, t);
+    if (STRIP)
+        strip_publish<G>(X, yq, ty, j0, lane, pub++);
+    __syncthreads();
+
+    // check scheduling only: single-precision hardware log2
+    const float lrho = __log2f(fmaxf(wf - 1.0f, 1e-3f));
+    bool refined = false;
+    bool refine_pending = !(C.flags & FADESIM_F_NO_REFINE) && !fixed;
+    int sweep = 0;
+    int next_check = fixed ? C.max_sweeps : (C.max_sweeps < C.check_first ? C.max_sweeps : C.check_first);
+    int st = FADESIM_ST_MAXIT;
+    double est = INFINITY;
+    const double stop_tol = dwr ? C.tol_eff * C.dwr_scale : C.tol_eff;
+
+    for (;;) {
+        load_coeffs<G>(I, wsb, gp, i0, j0, gi0, hcoef, refined, hn, rp, rq);
+        while (sweep < next_check) {
+            if (STRIP && !strip_fetch<G>(X, halo, sh.dead, tx, ty, pub - 1))
+                return false;
+            half_sweep<0, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
+            if (STRIP) {
+                strip_publish<G>(X, yq, ty, j0, lane, pub++);
+                if (!strip_fetch<G>(X, halo, sh.dead, tx, ty, pub - 1))
+                    return false;
+            }
+            half_sweep<1, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
+            if (STRIP)
+                strip_publish<G>(X, yq, ty, j0, lane, pub++);
+            ++sweep;
+        }
+        snapshot<G>(wsb, yp, yq, i0, j0);
+        __syncthreads();
+        est = conv_test<G, STRIP>(I, X, wsb, gp, gi0, t, a, rwire, hcoef, refined, false, sh, pub - 1, nbar++);
+        if (est < 0.0)
+            return false;
+        if (fixed) {
+            st = FADESIM_ST_CONVERGED;
+            break;
+        }
+        if (!(est < INFINITY)) {
+            st = FADESIM_ST_NONFINITE;
+            break;
+        }
+        if (refine_pending && est <= C.switch_tol) {
+            // ---- one iterative-refinement step: rhs <- exact FP64 residual ----
+            conv_test<G, STRIP>(I, X, wsb, gp, gi0, t, a, rwire, hcoef, false, true, sh, pub - 1, nbar);
+            if (I.field) {
+#pragma unroll
+                for (int ii = 0; ii < TR; ++ii) {
+                    strow<TC>(I.field + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
+                    strow<TC>(I.field + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
+                }
+            }
+#pragma unroll
+            for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < TC; ++jj)
+                    yp[ii][jj] = yq[ii][jj] = 0.f;
+            if (t < N)
+                sh.S1[t] = sh.colI[t];
+            publish_all<G>(yp, yq, halo, t);
+            if (STRIP)
+                strip_publish<G>(X, yq, ty, j0, lane, pub++);
+            __syncthreads();
+            refine_pending = false;
+            refined = true;
+        } else if (!refine_pending && est <= stop_tol) {
+            st = FADESIM_ST_CONVERGED;
+            break;
+        }
+        if (sweep >= C.max_sweeps) {
+            st = FADESIM_ST_MAXIT;
+            break;
+        }
+        const double tgt = refine_pending ? C.switch_tol : stop_tol;
+        const float k = (est > tgt) ? 0.8f * __log2f((float)(tgt / est)) / lrho : (float)C.check_min;
+        const int kk = (int)fminf(fmaxf(k, (float)C.check_min), (float)C.check_max);
+        next_check = sweep + kk;
+        if (next_check > C.max_sweeps)
+            next_check = C.max_sweeps;
+    }
+
+    // ---- outputs ---------------------------------------------------------------
+    if (strip == 0) {
+        if (t < N) {
+            double Iv = sh.colI[t];
+            if (dwr && st == FADESIM_ST_CONVERGED)
+                Iv -= sh.colD[t];
+            I.iout[t] = Iv;
+        }
+        if (t == 0) {
+            if (I.sweeps)
+                *I.sweeps = sweep;
+            if (I.status)
+                *I.status = st;
+            if (I.est)
+                *I.est = fixed ? NAN : est;
+        }
+    }
+    if (I.field) {
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            float *P = I.field + (size_t)(gi0 + i0 + ii) * N + j0;
+            float *Q = P + MNall;
+            float pv[TC], qv[TC];
+            if (refined) {
+                ldrow<TC>(P, pv);
+                ldrow<TC>(Q, qv);
+            }
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj) {
+                pv[jj] = refined ? pv[jj] + yp[ii][jj] : yp[ii][jj];
+                qv[jj] = refined ? qv[jj] + yq[ii][jj] : yq[ii][jj];
+            }
+            strow<TC>(P, pv);
+            strow<TC>(Q, qv);
+        }
+    }
+    __syncthreads(); // colI / halo reads done before the next instance reuses them
+    return true;
+}
+
+} // namespace rb
+} // namespace fadesim

@@ -103,6 +103,69 @@ int fadesim_cell_conductance(int batch, int rows, int cols, const float *d_glin,
                              const double *params, float *d_g, void *stream);
 
 /*
+ * Mixed-size batched solve: crossbars of 64x64, 128x128, 256x256 and 512x512
+ * cells, in any order, in one persistent launch (same method and stopping
+ * rule as fadesim_solve_batch; 256x256 and 512x512 crossbars are solved by
+ * teams of workgroups, one strip of rows each).  Replaces, for uneven sizes,
+ * the per-crossbar sweep loops named at fadesim_solve_batch.
+ *
+ * A plan fixes the sizes, the queues and the per-size solve controls and owns
+ * all device memory the solve needs (allocated at create), so
+ * fadesim_ragged_solve allocates nothing and can be captured into a HIP graph.
+ * Packed layout, instance k in input order:
+ *   d_g, d_warm, d_field: conductances [rows][cols] at g_off[k] = sum of the
+ *     earlier rows*cols (d_warm / d_field: [2][rows][cols] at 2*g_off[k]);
+ *   d_vin at sum of the earlier rows; d_vout, d_iout at sum of the earlier cols.
+ * Calls with one plan are stream-ordered and must not overlap (the plan's
+ * queues and exchange buffers are reused).
+ */
+typedef struct fadesim_ragged_plan fadesim_ragged_plan;
+
+/* Per-size solve controls, index 0..3 = 64, 128, 256, 512; 0 = built-in. */
+typedef struct fadesim_ragged_class_opts {
+    double omega;     /* SOR factor in (0,2) */
+    int check_first;  /* sweeps before the first convergence test */
+    int check_min;    /* bounds of the predicted interval between tests */
+    int check_max;
+} fadesim_ragged_class_opts;
+
+typedef struct fadesim_ragged_plan_stats {
+    int count;
+    long long cells, rows_total, cols_total;
+    int grid, teams, big_tasks, small_items;
+    double omega[4];
+    int check_first[4], check_min[4], check_max[4];
+} fadesim_ragged_plan_stats;
+
+typedef struct fadesim_ragged_desc {
+    const float *d_g;        /* packed conductances (S) */
+    const double *d_r_wire;  /* [K] wire segment resistance (ohm) */
+    const double *d_vin;     /* packed wordline drives (V) */
+    const double *d_vout;    /* packed bitline sink voltages, or NULL = 0 V */
+    const float *d_warm;     /* packed initial drops, or NULL = 0 */
+    double tol, safety, switch_tol; /* as in fadesim_batch_desc */
+    int max_sweeps;
+    unsigned flags;          /* FADESIM_F_* */
+    double *d_iout;          /* packed bitline currents (A) */
+    int *d_sweeps;           /* [K] or NULL */
+    int *d_status;           /* [K] or NULL */
+    double *d_est;           /* [K] or NULL */
+    float *d_field;          /* packed final drops, or NULL */
+} fadesim_ragged_desc;
+
+/* rows, cols: host arrays [count].  opts: host array [4] or NULL. */
+int fadesim_ragged_plan_create(fadesim_ragged_plan **plan, int count, const int *rows, const int *cols,
+                               const fadesim_ragged_class_opts *opts);
+void fadesim_ragged_plan_destroy(fadesim_ragged_plan *plan);
+int fadesim_ragged_plan_info(const fadesim_ragged_plan *plan, fadesim_ragged_plan_stats *stats);
+int fadesim_ragged_solve(const fadesim_ragged_plan *plan, const fadesim_ragged_desc *desc, void *stream);
+/* 1 in *aborted when the last solve's team waits timed out (results invalid);
+ * synchronises with the device. */
+int fadesim_ragged_aborted(const fadesim_ragged_plan *plan, int *aborted);
+/* Kernel duration (ms) of the plan's last solve made with FADESIM_F_TIMING. */
+double fadesim_ragged_last_ms(const fadesim_ragged_plan *plan);
+
+/*
  * One crossbar, FP64 red-black SOR in node voltages with per-segment wire
  * resistances and the reference's stopping rule (stop after the first sweep
  * whose max |dv| < tol; fixed iter_number sweeps without a break).  The

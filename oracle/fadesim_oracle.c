@@ -502,7 +502,7 @@ int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
  * uniform wire conductance a = 1/r, hcoef = w/(2a) with w the FP32 omega:
  *   hn  = (float)(-(g * hcoef)),  rho = (float)((g * c) * hcoef)
  *   tt  = fmaf(hn, p + q, rho)
- *   x'  = (cA*x  -> fmaf(cB, N1 + N2, .)) + tt,  cA = 1-w, cB = w/2
+ *   x'  = fmaf(cB, N1 + N2, fmaf(cA, x, tt)),  cA = 1-w, cB = w/2
  *   p: N1 = j ? p_i,j-1 : 0 (source),  N2 = j<m-1 ? p_i,j+1 : fmaf(2/w, tt, N1)
  *   q: N2 = i<n-1 ? q_i+1,j : 0 (sink), N1 = i ? q_i-1,j : fmaf(2/w, tt, N2)
  * (the ghost at an open end makes the update the exact SOR step there).
@@ -537,15 +537,11 @@ void oracle_rb_f32(int n, int m, const float *g, double r, const double *vin,
                     if (((i + j) & 1) == col) {
                         const float n1 = j ? p[k - 1] : 0.0f;
                         const float n2 = (j < m - 1) ? p[k + 1] : fmaf(c2, tt, n1);
-                        float t2 = cA * p[k];
-                        t2 = fmaf(cB, n1 + n2, t2);
-                        p[k] = t2 + tt;
+                        p[k] = fmaf(cB, n1 + n2, fmaf(cA, p[k], tt));
                     } else {
                         const float n2 = (i < n - 1) ? q[k + m] : 0.0f;
                         const float n1 = i ? q[k - m] : fmaf(c2, tt, n2);
-                        float t2 = cA * q[k];
-                        t2 = fmaf(cB, n1 + n2, t2);
-                        q[k] = t2 + tt;
+                        q[k] = fmaf(cB, n1 + n2, fmaf(cA, q[k], tt));
                     }
                 }
     free(hn);

@@ -333,3 +333,19 @@ def test_hook_nonlinear_fixed_point_is_newton_solution():
     d = x[row * col:].reshape(row, col)
     I = (_nonlinear_G(u - d, z, nl) * (u - d)).sum(0)
     np.testing.assert_allclose(out[0, 0], I, rtol=1e-6)
+
+
+@pytest.mark.parametrize("n,m,r", [(64, 64, 2.93), (16, 24, 5.0), (33, 7, 1.0), (2, 2, 3.0)])
+def test_line_solver_matches_banded_cholesky(n, m, r):
+    """The block line Gauss-Seidel oracle (checker of 256x256 and 512x512
+    crossbars) reaches the exact solve of oracle.exact."""
+    from oracle.line import line_solve
+    rng = np.random.default_rng(n * 100 + m)
+    g = np.exp(rng.uniform(np.log(5e-7), np.log(1e-6), (n, m)))
+    vin = rng.uniform(0, 0.3, n)
+    vout = rng.uniform(0, 0.01, m)
+    p, q, I, it = line_solve(g, r, vin, vout)
+    u, d, Iex = oracle.exact(g, vin, vout, *oracle.segments_uniform(n, m, r))
+    np.testing.assert_allclose(I, Iex, rtol=1e-11)
+    np.testing.assert_allclose(vin[:, None] - p, u, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(q + vout[None, :], d, rtol=0, atol=1e-13)

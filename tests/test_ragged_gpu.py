@@ -1,0 +1,110 @@
+"""Mixed-size persistent solver (fadesim_ragged_*) on the GPU.
+
+* fixed sweeps, mixed sizes including 256x256 and 512x512 (strips with row
+  exchange between workgroups): every instance's drop field is bitwise equal
+  to the CPU FP32 red-black replay (oracle_rb_f32), from zero and random warm
+  starts -- this pins the strip exchange;
+* converged, the uneven-size workload's distribution: every instance within
+  1e-6 relative bitline current of the exact FP64 solve (oracle.line);
+* a batch of only 128x128 crossbars: bitwise the results of the uniform
+  solver (same method, schedule and relaxation);
+* the launch captured into a graph and replayed, with eager work between
+  replays, reproduces the eager results exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle.line import line_solve
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed(cuda, sizes, seed, numpy_too=True):
+    import torch
+    from fadesim_amd.workloads import uneven_batch
+    counts = {}
+    for n in sizes:
+        counts[n] = counts.get(n, 0) + 1
+    return uneven_batch(counts, seed=seed, device=cuda, numpy_too=numpy_too)
+
+
+def test_fixed_sweeps_bitwise_vs_replay(cuda):
+    import torch
+    from fadesim_amd.ragged import RaggedPlan, SIZES
+    inst = _mixed(cuda, [64, 128, 256, 512, 256, 64, 512, 128, 256], seed=3)
+    plan = RaggedPlan(inst["rows"], device=cuda)
+    rng = np.random.default_rng(9)
+    warm_list = []
+    for k, n in enumerate(inst["rows"]):
+        w = np.zeros((2, n, n), np.float32)
+        if k % 2:
+            w[:] = rng.uniform(0, 1e-3, (2, n, n)).astype(np.float32)
+        warm_list.append(w)
+    warm = torch.tensor(plan.pack(warm_list, "field", np.float32), device=cuda)
+    sweeps = 37
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], warm=warm, fixed_sweeps=sweeps, want_field=True)
+    torch.cuda.synchronize()
+    fields = plan.unpack(res.field, "field")
+    assert (res.sweeps.cpu().numpy() == sweeps).all()
+    for k, n in enumerate(inst["rows"]):
+        w = plan.stats["omega"][SIZES.index(n)]
+        p, q = oracle.rb_f32(inst["g_list"][k], inst["r_np"][k], inst["vin_list"][k], None, w, sweeps,
+                             warm_list[k][0], warm_list[k][1])
+        assert np.array_equal(fields[k][0], p), (k, n)
+        assert np.array_equal(fields[k][1], q), (k, n)
+
+
+def test_converged_mixed_vs_exact(cuda):
+    from fadesim_amd.ragged import RaggedPlan
+    inst = _mixed(cuda, [64, 64, 128, 128, 256, 256, 512, 512, 128, 64], seed=11)
+    plan = RaggedPlan(inst["rows"], device=cuda)
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    st = res.status.cpu().numpy()
+    assert (st == 0).all(), st
+    iouts = plan.unpack(res.iout, "cols")
+    for k, n in enumerate(inst["rows"]):
+        _, _, iex, _ = line_solve(inst["g_list"][k], inst["r_np"][k], inst["vin_list"][k])
+        err = np.max(np.abs(iouts[k] - iex) / np.abs(iex))
+        assert err <= 1e-6, (k, n, err)
+
+
+def test_uniform_128_batch_matches_uniform_solver(cuda):
+    import torch
+    from fadesim_amd.ragged import RaggedPlan
+    from fadesim_amd.solver import solve_batch
+    from fadesim_amd.workloads import uniform_batch
+    g, r, vin = uniform_batch(40, 128, seed=5, device=cuda)
+    plan = RaggedPlan([128] * 40, device=cuda, tuning={})
+    res = plan.solve(g.reshape(-1).contiguous(), r, vin.reshape(-1).contiguous(), tol=1e-6)
+    ref = solve_batch(g, r, vin, tol=1e-6)
+    assert torch.equal(res.iout.view(40, 128), ref.iout)
+    assert torch.equal(res.sweeps, ref.sweeps)
+
+
+def test_graph_replays_match_eager(cuda):
+    import torch
+    from fadesim_amd.ragged import RaggedPlan
+    inst = _mixed(cuda, [64, 128, 256, 512, 128], seed=17, numpy_too=False)
+    plan = RaggedPlan(inst["rows"], device=cuda)
+    eager = plan.solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    e_iout, e_sw = eager.iout.clone(), eager.sweeps.clone()
+    graph, res = plan.capture(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    for rep in range(3):
+        res.iout.fill_(0.0)
+        graph.replay()
+        torch.cuda.synchronize()
+        assert not plan.aborted()
+        assert torch.equal(res.iout, e_iout), rep
+        assert torch.equal(res.sweeps, e_sw), rep
+        # eager work between replays
+        plan.solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+        torch.cuda.synchronize()
+
+
+def test_plan_rejects_unsupported_sizes(cuda):
+    from fadesim_amd.ragged import RaggedPlan
+    with pytest.raises(RuntimeError):
+        RaggedPlan([64, 96], device=cuda)
+    with pytest.raises(RuntimeError):
+        RaggedPlan([128], cols=[64], device=cuda)
