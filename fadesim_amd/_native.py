@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libf
 
 FADESIM_OK = 0
 ST_CONVERGED, ST_MAXIT, ST_NONFINITE = 0, 1, 2
-F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING = 1, 2, 4, 8
+F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING, F_ZERO_CORRECTION = 1, 2, 4, 8, 16
 
 _vp = ctypes.c_void_p
 

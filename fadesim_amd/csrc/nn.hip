@@ -40,102 +40,161 @@ __device__ __forceinline__ bf16x8 zero8()
 }
 
 // ---------------------------------------------------------------- features
-// One workgroup per crossbar, one thread per row (pass over rows) then per
-// column.  First-order drops from the ideal cell currents I = g c:
+// One workgroup per crossbar (M, N <= 128).  First-order drops from the ideal
+// cell currents I = g c (c = vin_i, bitlines at 0 V):
 //   p1_ij = r sum_k (min(j,k)+1) I_ik      (wordline, source at the left)
 //   q1_ij = r sum_k (n - max(i,k)) I_kj    (bitline, sink at the bottom)
 // second order from I = g (c - p1 - q1).  Features (bf16, 8 channels):
 //   p1/s, q1/s, (p2-p1)/s2, (q2-q1)/s2, g/gmax, vin/vmax, r/5, 1
 // with s = max(p1, q1) and s2 = max|p2-p1|, |q2-q1| of the instance; base =
 // (p2, q2) in FP32 and the output scale s2 per instance.
-__global__ void __launch_bounds__(128) features_kernel(const float *__restrict__ g, const double *__restrict__ rw,
-                                                       const double *__restrict__ vin, int M, int N,
-                                                       float *__restrict__ tmp, __bf16 *__restrict__ feat,
-                                                       float *__restrict__ base, float *__restrict__ scale)
+//
+// Data flow: g and the row scans live in LDS (rows padded to N+1 floats, so a
+// thread per row and a thread per column both access it conflict-free); the
+// column thread computes q1 and, from the same column, q2 -- it needs p1 of
+// its column (LDS) -- and leaves the second-order cell current in the g plane;
+// the row threads then turn that into p2 in place.  Global traffic (g in, q1 /
+// q2 via the base and scratch planes, features out) is coalesced.  Scans
+// accumulate in FP64.
+constexpr int kFeatMax = 128;
+constexpr int kFeatThreads = 256;
+
+__global__ void __launch_bounds__(kFeatThreads) features_kernel(const float *__restrict__ g,
+                                                                const double *__restrict__ rw,
+                                                                const double *__restrict__ vin, int M, int N,
+                                                                float *__restrict__ tmp, __bf16 *__restrict__ feat,
+                                                                float *__restrict__ base, float *__restrict__ scale)
 {
+    __shared__ float gl[kFeatMax * (kFeatMax + 1)]; // g, then the 2nd-order current, then p2
+    __shared__ float pl[kFeatMax * (kFeatMax + 1)]; // p1
+    __shared__ double vl[kFeatMax];
+    __shared__ float red[4][kFeatThreads / 64];
     const int b = blockIdx.x, t = threadIdx.x;
+    const int ld = N + 1;
     const size_t MN = (size_t)M * N;
     const float *gb = g + b * MN;
     const double *vb = vin + (size_t)b * M;
     const double r = rw[b];
-    float *p1 = tmp + b * 4 * MN, *q1 = p1 + MN, *p2 = p1 + 2 * MN, *q2 = p1 + 3 * MN;
-    __shared__ float red[4][128];
-    // order 1 and 2 (order 2 uses the order-1 fields)
-    for (int order = 0; order < 2; ++order) {
-        float *P = order ? p2 : p1, *Q = order ? q2 : q1;
-        for (int i = t; i < M; i += blockDim.x) {
-            const double c = vb[i];
-            double R = 0.0;
-            for (int k = 0; k < N; ++k) {
-                const size_t id = (size_t)i * N + k;
-                const double cur = (double)gb[id] * (order ? c - (double)p1[id] - (double)q1[id] : c);
-                R += cur;
-            }
-            double P0 = 0.0, P1 = 0.0;
-            for (int j = 0; j < N; ++j) {
-                const size_t id = (size_t)i * N + j;
-                const double cur = (double)gb[id] * (order ? c - (double)p1[id] - (double)q1[id] : c);
-                P[id] = (float)(r * (P1 + (double)(j + 1) * (R - P0)));
-                P0 += cur;
-                P1 += (double)(j + 1) * cur;
-            }
-        }
-        for (int j = t; j < N; j += blockDim.x) {
-            double T = 0.0;
-            for (int k = 0; k < M; ++k) {
-                const size_t id = (size_t)k * N + j;
-                const double cur = (double)gb[id] * (order ? vb[k] - (double)p1[id] - (double)q1[id] : vb[k]);
-                T += cur * (double)(M - k);
-            }
-            double A = 0.0, Bs = T;
-            for (int i = 0; i < M; ++i) {
-                const size_t id = (size_t)i * N + j;
-                const double cur = (double)gb[id] * (order ? vb[i] - (double)p1[id] - (double)q1[id] : vb[i]);
-                A += cur;
-                Bs -= cur * (double)(M - i);
-                Q[id] = (float)(r * ((double)(M - i) * A + Bs));
-            }
-        }
-        __syncthreads();
+    float *q1g = tmp + b * MN;             // q1 scratch plane
+    float *bp = base + b * 2 * MN, *bq = bp + MN;
+    for (size_t id = t; id < MN; id += kFeatThreads) {
+        const int i = (int)(id / N), j = (int)(id % N);
+        gl[i * ld + j] = gb[id];
     }
+    for (int i = t; i < M; i += kFeatThreads)
+        vl[i] = vb[i];
+    __syncthreads();
+    // row pass 1: p1
+    if (t < M) {
+        const int i = t;
+        const double c = vl[i];
+        double R = 0.0;
+        for (int j = 0; j < N; ++j)
+            R += (double)gl[i * ld + j] * c;
+        double P0 = 0.0, P1 = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const double cur = (double)gl[i * ld + j] * c;
+            pl[i * ld + j] = (float)(r * (P1 + (double)(j + 1) * (R - P0)));
+            P0 += cur;
+            P1 += (double)(j + 1) * cur;
+        }
+    }
+    __syncthreads();
+    // column pass: q1, then the 2nd-order current of the column and q2
+    if (t < N) {
+        const int j = t;
+        double T = 0.0;
+        for (int k = 0; k < M; ++k)
+            T += (double)gl[k * ld + j] * vl[k] * (double)(M - k);
+        double A = 0.0, Bs = T;
+        for (int i = 0; i < M; ++i) {
+            const double cur = (double)gl[i * ld + j] * vl[i];
+            A += cur;
+            Bs -= cur * (double)(M - i);
+            const float q1 = (float)(r * ((double)(M - i) * A + Bs));
+            q1g[(size_t)i * N + j] = q1;
+            // second-order cell current, kept in the g plane for the row pass
+            gl[i * ld + j] = (float)((double)gl[i * ld + j] * (vl[i] - (double)pl[i * ld + j] - (double)q1));
+        }
+        T = 0.0;
+        for (int k = 0; k < M; ++k)
+            T += (double)gl[k * ld + j] * (double)(M - k);
+        A = 0.0;
+        Bs = T;
+        for (int i = 0; i < M; ++i) {
+            const double cur = (double)gl[i * ld + j];
+            A += cur;
+            Bs -= cur * (double)(M - i);
+            bq[(size_t)i * N + j] = (float)(r * ((double)(M - i) * A + Bs));
+        }
+    }
+    __syncthreads();
+    // row pass 2: p2 in place of the 2nd-order current
+    if (t < M) {
+        const int i = t;
+        double R = 0.0;
+        for (int j = 0; j < N; ++j)
+            R += (double)gl[i * ld + j];
+        double P0 = 0.0, P1 = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const double cur = (double)gl[i * ld + j];
+            gl[i * ld + j] = (float)(r * (P1 + (double)(j + 1) * (R - P0)));
+            P0 += cur;
+            P1 += (double)(j + 1) * cur;
+        }
+    }
+    __syncthreads();
     // scales
     float s1 = 0.f, s2 = 0.f, gm = 0.f, vm = 0.f;
-    for (size_t id = t; id < MN; id += blockDim.x) {
-        s1 = fmaxf(s1, fmaxf(fabsf(p1[id]), fabsf(q1[id])));
-        s2 = fmaxf(s2, fmaxf(fabsf(p2[id] - p1[id]), fabsf(q2[id] - q1[id])));
+    for (size_t id = t; id < MN; id += kFeatThreads) {
+        const int i = (int)(id / N), j = (int)(id % N);
+        const float p1 = pl[i * ld + j], p2 = gl[i * ld + j];
+        const float q1 = q1g[id], q2 = bq[id];
+        s1 = fmaxf(s1, fmaxf(fabsf(p1), fabsf(q1)));
+        s2 = fmaxf(s2, fmaxf(fabsf(p2 - p1), fabsf(q2 - q1)));
         gm = fmaxf(gm, gb[id]);
     }
-    for (int i = t; i < M; i += blockDim.x)
-        vm = fmaxf(vm, (float)fabs(vb[i]));
-    red[0][t] = s1;
-    red[1][t] = s2;
-    red[2][t] = gm;
-    red[3][t] = vm;
-    __syncthreads();
-    for (int off = 64; off > 0; off >>= 1) {
-        if (t < off)
-            for (int q = 0; q < 4; ++q)
-                red[q][t] = fmaxf(red[q][t], red[q][t + off]);
-        __syncthreads();
+    for (int i = t; i < M; i += kFeatThreads)
+        vm = fmaxf(vm, (float)fabs(vl[i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 = fmaxf(s1, __shfl_xor(s1, off, 64));
+        s2 = fmaxf(s2, __shfl_xor(s2, off, 64));
+        gm = fmaxf(gm, __shfl_xor(gm, off, 64));
+        vm = fmaxf(vm, __shfl_xor(vm, off, 64));
     }
-    const float S1 = fmaxf(red[0][0], 1e-30f), S2 = fmaxf(red[1][0], 1e-30f);
-    const float GM = fmaxf(red[2][0], 1e-30f), VM = fmaxf(red[3][0], 1e-30f);
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = s1;
+        red[1][t >> 6] = s2;
+        red[2][t >> 6] = gm;
+        red[3][t >> 6] = vm;
+    }
+    __syncthreads();
+    float S1 = 1e-30f, S2 = 1e-30f, GM = 1e-30f, VM = 1e-30f;
+#pragma unroll
+    for (int w = 0; w < kFeatThreads / 64; ++w) {
+        S1 = fmaxf(S1, red[0][w]);
+        S2 = fmaxf(S2, red[1][w]);
+        GM = fmaxf(GM, red[2][w]);
+        VM = fmaxf(VM, red[3][w]);
+    }
     if (t == 0)
         scale[b] = S2;
-    for (size_t id = t; id < MN; id += blockDim.x) {
-        const int i = (int)(id / N);
+    for (size_t id = t; id < MN; id += kFeatThreads) {
+        const int i = (int)(id / N), j = (int)(id % N);
+        const float p1 = pl[i * ld + j], p2 = gl[i * ld + j];
+        const float q1 = q1g[id], q2 = bq[id];
         bf16x8 f;
-        f[0] = (__bf16)(p1[id] / S1);
-        f[1] = (__bf16)(q1[id] / S1);
-        f[2] = (__bf16)((p2[id] - p1[id]) / S2);
-        f[3] = (__bf16)((q2[id] - q1[id]) / S2);
+        f[0] = (__bf16)(p1 / S1);
+        f[1] = (__bf16)(q1 / S1);
+        f[2] = (__bf16)((p2 - p1) / S2);
+        f[3] = (__bf16)((q2 - q1) / S2);
         f[4] = (__bf16)(gb[id] / GM);
-        f[5] = (__bf16)((float)vb[i] / VM);
+        f[5] = (__bf16)((float)vl[i] / VM);
         f[6] = (__bf16)((float)(r / 5.0));
         f[7] = (__bf16)1.0f;
         *reinterpret_cast<bf16x8 *>(feat + (b * MN + id) * 8) = f;
-        base[b * 2 * MN + id] = p2[id];
-        base[b * 2 * MN + MN + id] = q2[id];
+        bp[id] = p2;
     }
 }
 
@@ -210,6 +269,95 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(ConvArgs A)
             o[r] = (__bf16)v;
         }
         *reinterpret_cast<bf16x4 *>(yp + 16 * t) = o;
+    }
+}
+
+// LDS-tiled variant: a workgroup of 4 waves computes a 4-row x TW-pixel output
+// tile (TW = 16 P, P <= 4 segments of 16 pixels per wave) for all Cout.  The
+// input tile with its one-pixel halo, (4+2) x (TW+2) pixels x Cin, is staged
+// in LDS once (zeros outside the image; pixel stride Cin + 8 bf16 to spread
+// the banks); every wave then builds its B fragments (im2col) from LDS and
+// reuses each weight (A) fragment for its P pixel segments.
+constexpr int kConvRows = 4;
+
+template <int CT, int P>
+__global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(ConvArgs A)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16 *tile = reinterpret_cast<__bf16 *>(smem);
+    constexpr int TW = 16 * P;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Cin = A.Cin, Cs = Cin + 8; // LDS pixel stride (bf16)
+    const int tilesX = A.W / TW, tilesY = A.H / kConvRows;
+    int blk = blockIdx.x;
+    const int tx = blk % tilesX;
+    blk /= tilesX;
+    const int ty = blk % tilesY;
+    const long b = blk / tilesY;
+    const int x0 = tx * TW, y0 = ty * kConvRows;
+    // stage the input tile (8 channels = 16 bytes per element)
+    const int C8 = Cin / 8;
+    const int nvec = (kConvRows + 2) * (TW + 2) * C8;
+    for (int e = threadIdx.x; e < nvec; e += 256) {
+        const int c8 = e % C8;
+        const int px = (e / C8) % (TW + 2);
+        const int py = e / (C8 * (TW + 2));
+        const int yy = y0 + py - 1, xx = x0 + px - 1;
+        bf16x8 v = zero8();
+        if (yy >= 0 && yy < A.H && xx >= 0 && xx < A.W)
+            v = *reinterpret_cast<const bf16x8 *>(A.x + (((b * A.H + yy) * A.W + xx) * Cin + 8 * c8));
+        *reinterpret_cast<bf16x8 *>(tile + (py * (TW + 2) + px) * Cs + 8 * c8) = v;
+    }
+    __syncthreads();
+    const int j = lane & 15, g = lane >> 4;
+    const int K9 = 9 * Cin;
+    f4 acc[P][CT];
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int t = 0; t < CT; ++t)
+            acc[p][t] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < A.Kp; k0 += 32) {
+        const int k = k0 + 8 * g;
+        bf16x8 av[CT];
+#pragma unroll
+        for (int t = 0; t < CT; ++t)
+            av[t] = *reinterpret_cast<const bf16x8 *>(A.w + (long)(16 * t + j) * A.Kp + k);
+        int tap = 0, ci = 0;
+        const bool real = k < K9;
+        if (real) {
+            tap = k / Cin;
+            ci = k - tap * Cin;
+        }
+        const int ky = tap / 3, kx = tap - 3 * ky;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            bf16x8 bv = zero8();
+            if (real)
+                bv = *reinterpret_cast<const bf16x8 *>(tile + ((wave + ky) * (TW + 2) + 16 * p + j + kx) * Cs + ci);
+            else if (k == K9)
+                bv[0] = (__bf16)1.0f; // bias column
+#pragma unroll
+            for (int t = 0; t < CT; ++t)
+                acc[p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[p][t], 0, 0, 0);
+        }
+    }
+    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        __bf16 *yp = A.y + (((b * A.H + y0 + wave) * A.W + x0 + 16 * p + j) * A.Cout) + 4 * g;
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+            bf16x4 o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = acc[p][t][r];
+                if (A.relu)
+                    v = fmaxf(v, 0.f);
+                o[r] = (__bf16)v;
+            }
+            *reinterpret_cast<bf16x4 *>(yp + 16 * t) = o;
+        }
     }
 }
 
@@ -319,18 +467,26 @@ __global__ void __launch_bounds__(256) conv3x3_wgrad_kernel(WgradArgs A)
 // ---------------------------------------------------------------- elementwise
 __global__ void pool2_fwd_kernel(const __bf16 *__restrict__ x, int B, int H, int W, int C, __bf16 *__restrict__ y)
 {
-    const int Ho = H / 2, Wo = W / 2;
-    const size_t n = (size_t)B * Ho * Wo * C;
+    // 8 channels (16 bytes) per thread; C is a multiple of 8
+    const int Ho = H / 2, Wo = W / 2, C8 = C / 8;
+    const size_t n = (size_t)B * Ho * Wo * C8;
     for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(id % C);
-        size_t r = id / C;
+        const int c = (int)(id % C8) * 8;
+        size_t r = id / C8;
         const int xo = (int)(r % Wo);
         r /= Wo;
         const int yo = (int)(r % Ho);
         const size_t b = r / Ho;
         const size_t i00 = ((b * H + 2 * yo) * W + 2 * xo) * C + c;
-        const float s = (float)x[i00] + (float)x[i00 + C] + (float)x[i00 + (size_t)W * C] + (float)x[i00 + (size_t)W * C + C];
-        y[id] = (__bf16)(0.25f * s);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(x + i00);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(x + i00 + C);
+        const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(x + i00 + (size_t)W * C);
+        const bf16x8 a3 = *reinterpret_cast<const bf16x8 *>(x + i00 + (size_t)W * C + C);
+        bf16x8 o;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            o[e] = (__bf16)(0.25f * ((float)a[e] + (float)a1[e] + (float)a2[e] + (float)a3[e]));
+        *reinterpret_cast<bf16x8 *>(y + (((b * Ho + yo) * Wo + xo) * C + c)) = o;
     }
 }
 
@@ -354,20 +510,27 @@ __global__ void pool2_bwd_kernel(const __bf16 *__restrict__ dy, int B, int H, in
     }
 }
 
-// u (2H x 2W) = nearest-upsample(t (H x W)) + e
+// u (2H x 2W) = nearest-upsample(t (H x W)) + e, 8 channels per thread
 __global__ void up2_add_fwd_kernel(const __bf16 *__restrict__ t, const __bf16 *__restrict__ e, int B, int H, int W,
                                    int C, __bf16 *__restrict__ u)
 {
-    const int H2 = 2 * H, W2 = 2 * W;
-    const size_t n = (size_t)B * H2 * W2 * C;
+    const int H2 = 2 * H, W2 = 2 * W, C8 = C / 8;
+    const size_t n = (size_t)B * H2 * W2 * C8;
     for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(id % C);
-        size_t r = id / C;
+        const int c = (int)(id % C8) * 8;
+        size_t r = id / C8;
         const int xx = (int)(r % W2);
         r /= W2;
         const int yy = (int)(r % H2);
         const size_t b = r / H2;
-        u[id] = (__bf16)((float)t[((b * H + yy / 2) * W + xx / 2) * C + c] + (float)e[id]);
+        const size_t o = ((b * H2 + yy) * W2 + xx) * C + c;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(t + ((b * H + yy / 2) * W + xx / 2) * C + c);
+        const bf16x8 k = *reinterpret_cast<const bf16x8 *>(e + o);
+        bf16x8 v;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            v[q] = (__bf16)((float)a[q] + (float)k[q]);
+        *reinterpret_cast<bf16x8 *>(u + o) = v;
     }
 }
 
@@ -479,10 +642,12 @@ extern "C" int fadesim_nn_features(int B, int M, int N, const float *d_g, const 
         return set_error(FADESIM_E_INVALID, "fadesim_nn_features: bad arguments");
     if (B == 0)
         return FADESIM_OK;
-    float *tmp = (float *)workspace((size_t)B * 4 * M * N * sizeof(float));
+    if (M > kFeatMax || N > kFeatMax)
+        return set_error(FADESIM_E_UNSUPPORTED, "fadesim_nn_features: rows and cols must be <= 128");
+    float *tmp = (float *)workspace((size_t)B * M * N * sizeof(float));
     if (!tmp)
         return FADESIM_E_NOMEM;
-    hipLaunchKernelGGL(features_kernel, dim3(B), dim3(128), 0, (hipStream_t)stream, d_g, d_r, d_vin, M, N, tmp,
+    hipLaunchKernelGGL(features_kernel, dim3(B), dim3(kFeatThreads), 0, (hipStream_t)stream, d_g, d_r, d_vin, M, N, tmp,
                        (__bf16 *)d_feat, d_base, d_scale);
     NN_LAUNCH_CHECK();
     return FADESIM_OK;
@@ -500,15 +665,46 @@ extern "C" int fadesim_nn_combine(int B, int M, int N, const float *d_base, cons
     return FADESIM_OK;
 }
 
+template <int CT>
+static int launch_conv_lds(const ConvArgs &a, hipStream_t s)
+{
+    const int P = a.W >= 64 ? 4 : a.W / 16;
+    const int TW = 16 * P;
+    const long blocks = (long)a.B * (a.H / kConvRows) * (a.W / TW);
+    const size_t lds = (size_t)(kConvRows + 2) * (TW + 2) * (a.Cin + 8) * sizeof(__bf16);
+    const dim3 grid((unsigned)blocks);
+    switch (P) {
+    case 1: hipLaunchKernelGGL((conv3x3_fwd_lds_kernel<CT, 1>), grid, dim3(256), lds, s, a); break;
+    case 2: hipLaunchKernelGGL((conv3x3_fwd_lds_kernel<CT, 2>), grid, dim3(256), lds, s, a); break;
+    default: hipLaunchKernelGGL((conv3x3_fwd_lds_kernel<CT, 4>), grid, dim3(256), lds, s, a); break;
+    }
+    return FADESIM_OK;
+}
+
 extern "C" int fadesim_nn_conv3x3_fwd(int B, int H, int W, int Cin, int Cout, int Kp, const void *d_x, const void *d_w,
                                       void *d_y, int relu, void *stream)
 {
     if (B < 1 || H < 1 || W % 16 != 0 || Cin % 8 != 0 || Cout % 16 != 0 || Kp % 32 != 0 || Kp < 9 * Cin + 1)
         return set_error(FADESIM_E_INVALID, "conv3x3_fwd: need W%16==0, Cin%8==0, Cout%16==0, Kp%32==0, Kp>9*Cin");
     ConvArgs a{B, H, W, Cin, Cout, Kp, (const __bf16 *)d_x, (const __bf16 *)d_w, (__bf16 *)d_y, relu};
+    hipStream_t s = (hipStream_t)stream;
+    const int TWt = W >= 64 ? 64 : W;
+    const size_t lds_bytes = (size_t)(kConvRows + 2) * (TWt + 2) * (Cin + 8) * sizeof(__bf16);
+    const bool tiled = H % kConvRows == 0 && (W == 16 || W == 32 || W % 64 == 0) && lds_bytes <= 65536;
+    if (tiled) {
+        switch (Cout / 16) {
+        case 1: launch_conv_lds<1>(a, s); break;
+        case 2: launch_conv_lds<2>(a, s); break;
+        case 4: launch_conv_lds<4>(a, s); break;
+        case 6: launch_conv_lds<6>(a, s); break;
+        case 8: launch_conv_lds<8>(a, s); break;
+        default: return set_error(FADESIM_E_UNSUPPORTED, "conv3x3_fwd: Cout must be 16, 32, 64, 96 or 128");
+        }
+        NN_LAUNCH_CHECK();
+        return FADESIM_OK;
+    }
     const long npix = (long)B * H * W;
     const dim3 grid((unsigned)((npix + 63) / 64));
-    hipStream_t s = (hipStream_t)stream;
     switch (Cout / 16) {
     case 1: hipLaunchKernelGGL(conv3x3_fwd_kernel<1>, grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL(conv3x3_fwd_kernel<2>, grid, dim3(256), 0, s, a); break;
@@ -565,7 +761,9 @@ extern "C" int fadesim_nn_conv3x3_wgrad(int B, int H, int W, int Cin, int Cout, 
 
 extern "C" int fadesim_nn_pool2(int B, int H, int W, int C, const void *d_x, void *d_y, void *stream)
 {
-    const size_t n = (size_t)B * (H / 2) * (W / 2) * C;
+    if (C % 8 != 0 || H % 2 != 0 || W % 2 != 0)
+        return set_error(FADESIM_E_INVALID, "pool2: need C % 8 == 0 and even H, W");
+    const size_t n = (size_t)B * (H / 2) * (W / 2) * (C / 8);
     hipLaunchKernelGGL(pool2_fwd_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)d_x, B,
                        H, W, C, (__bf16 *)d_y);
     NN_LAUNCH_CHECK();
@@ -584,7 +782,9 @@ extern "C" int fadesim_nn_pool2_bwd(int B, int H, int W, int C, const void *d_dy
 
 extern "C" int fadesim_nn_up2_add(int B, int H, int W, int C, const void *d_t, const void *d_e, void *d_u, void *stream)
 {
-    const size_t n = (size_t)B * 4 * H * W * C;
+    if (C % 8 != 0)
+        return set_error(FADESIM_E_INVALID, "up2_add: need C % 8 == 0");
+    const size_t n = (size_t)B * 4 * H * W * (C / 8);
     hipLaunchKernelGGL(up2_add_fwd_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)d_t,
                        (const __bf16 *)d_e, B, H, W, C, (__bf16 *)d_u);
     NN_LAUNCH_CHECK();

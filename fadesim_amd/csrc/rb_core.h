@@ -37,6 +37,12 @@
 // with the adjoint z_j taken from the wire Green's functions (bitline:
 // r (n - max(i,i')) weighted by g of column j; wordline: r (min(j,j')+1)
 // weighted by g_ij).  Stop when max_j |D_j| / |I_j| * safety <= tol.
+//
+// FP32 sweeps stall near 1e-6 relative error, so once the estimate falls
+// below switch_tol the solve takes one iterative-refinement step: the exact
+// FP64 residual becomes the right-hand side of a correction solve, whose
+// initial guess is the second-order wire Green's-function estimate of the
+// correction (green_guess) rather than zero.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -424,6 +430,7 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
             return -1.0;
     }
     // 1. residuals, one node pair per thread and step
+    #pragma unroll 8
     for (int k = t; k < M * N; k += NT) {
         const int i = k / N, j = k - i * N;
         const double gd = (double)g[k];
@@ -464,9 +471,11 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     if (t < M) {
         const int i = t;
         double R = 0.0;
+        #pragma unroll 8
         for (int j = 0; j < N; ++j)
             R += (double)resp[i * N + j];
         double P0 = 0.0, P1 = 0.0;
+        #pragma unroll 8
         for (int j = 0; j < N; ++j) {
             const int k = i * N + j;
             const double r = (double)resp[k];
@@ -482,6 +491,7 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     double D = 0.0, Y = 0.0;
     if (t < N) {
         const int j = t;
+        #pragma unroll 8
         for (int i = 0; i < M; ++i) {
             const int k = i * N + j;
             D += (double)resp[k];
@@ -528,6 +538,120 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     return (double)em;
 }
 
+// Column Green's sums over the whole crossbar for rows [gi0, gi0 + M) of an
+// n-row crossbar: out(i,j) = r sum_i' (n - max(i,i')) v(i',j) with v read by
+// vfun(i, j) (FP64), written to out.  One thread per column; strips complete
+// the sums before/after their rows over the team (partials in `part`, team
+// barrier `nbar`).  Returns false when the team aborted.
+template <class G, bool STRIP, class VF>
+__device__ __forceinline__ bool column_green(const StripCtx &X, int gi0, int n, int t, double rwire, VF vfun,
+                                             float *out, int *dead, int nbar)
+{
+    constexpr int M = G::M, N = G::N;
+    double A = 0.0, B = 0.0, sA = 0.0, sB = 0.0;
+    if (t < N) {
+        #pragma unroll 8
+        for (int i = 0; i < M; ++i) {
+            const double v = vfun(i, t);
+            sA += v;
+            sB += v * (double)(n - (gi0 + i));
+        }
+        B = sB;
+    }
+    if (STRIP) {
+        double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
+        if (t < N) {
+            mine[t] = sA;
+            mine[kRowsLd + t] = sB;
+        }
+        if (!team_barrier(X, dead, t, nbar))
+            return false;
+        if (t < N) {
+            B = 0.0;
+            for (int s = 0; s < X.S; ++s) {
+                const double *ps = X.part + (((size_t)(nbar & 1) * X.S + s) * 2) * kRowsLd;
+                if (s < X.s)
+                    A += ps[t];
+                else
+                    B += ps[kRowsLd + t];
+            }
+        }
+    }
+    if (t < N) {
+        #pragma unroll 8
+        for (int i = 0; i < M; ++i) {
+            const double v = vfun(i, t);
+            A += v;
+            B -= v * (double)(n - (gi0 + i));
+            out[(size_t)i * N + t] = (float)(rwire * ((double)(n - (gi0 + i)) * A + B));
+        }
+    }
+    return true;
+}
+
+// Row Green's sums: out(i,j) = r sum_j' (min(j,j')+1) v(i,j'), one thread per row.
+template <class G, class VF>
+__device__ __forceinline__ void row_green(int t, double rwire, VF vfun, float *out)
+{
+    constexpr int M = G::M, N = G::N;
+    if (t < M) {
+        const int i = t;
+        double R = 0.0;
+        #pragma unroll 8
+        for (int j = 0; j < N; ++j)
+            R += vfun(i, j);
+        double P0 = 0.0, P1 = 0.0;
+        #pragma unroll 8
+        for (int j = 0; j < N; ++j) {
+            const double v = vfun(i, j);
+            const double jp1 = (double)(j + 1);
+            out[(size_t)i * N + j] = (float)(rwire * (P1 + jp1 * (R - P0)));
+            P0 += v;
+            P1 += jp1 * v;
+        }
+    }
+}
+
+// Initial guess of the refinement correction y (A y = residual): the
+// second-order superposition of the wire Green's functions,
+//   y1 = (aL)^-1 res,  y2 = y1 - (aL)^-1 G (y1p + y1q)
+// with (aL)^-1 the wordline (r (min(j,j')+1)) and bitline (r (n - max(i,i')))
+// line inverses; the cell coupling G is ~1e-6 of the wire conductance, so y2
+// is within a few 1e-3 of y and the correction converges in a few sweeps
+// instead of restarting from zero.  Reads the scaled residual (planes 1, 2),
+// writes y2 to planes 3, 4 (the snapshot is no longer needed), uses planes 5,
+// 6 for y1.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool green_guess(const StripCtx &X, float *wsb, const float *g, int gi0, int n, int t,
+                                            double rwire, double hcoef, int *dead, int &nbar)
+{
+    constexpr int M = G::M, N = G::N;
+    const float *rho_p = wsb + 1 * M * N, *rho_q = wsb + 2 * M * N;
+    float *y2p = wsb + 3 * M * N, *y2q = wsb + 4 * M * N;
+    float *y1p = wsb + 5 * M * N, *y1q = wsb + 6 * M * N;
+    const double ih = 1.0 / hcoef;
+    row_green<G>(t, rwire, [&](int i, int j) { return (double)rho_p[(size_t)i * N + j] * ih; }, y1p);
+    if (!column_green<G, STRIP>(X, gi0, n, t, rwire,
+                                [&](int i, int j) { return (double)rho_q[(size_t)i * N + j] * ih; }, y1q, dead,
+                                nbar++))
+        return false;
+    __syncthreads();
+    auto coupling = [&](int i, int j) {
+        const size_t k = (size_t)i * N + j;
+        return (double)g[k] * ((double)y1p[k] + (double)y1q[k]);
+    };
+    row_green<G>(t, rwire, coupling, y2p);
+    if (!column_green<G, STRIP>(X, gi0, n, t, rwire, coupling, y2q, dead, nbar++))
+        return false;
+    __syncthreads();
+    for (int k = t; k < M * N; k += G::NT) {
+        y2p[k] = y1p[k] - y2p[k];
+        y2q[k] = y1q[k] - y2q[k];
+    }
+    __syncthreads();
+    return true;
+}
+
 template <class G>
 __device__ __forceinline__ void snapshot(float *wsb, const float (&yp)[G::TR][G::TC],
                                          const float (&yq)[G::TR][G::TC], int i0, int j0)
@@ -553,6 +677,7 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
     double si = 0.0, sf = 0.0, sg = 0.0, sT = 0.0;
     if (t < N) {
         const double vo = I.vout[t];
+        #pragma unroll 8
         for (int i = 0; i < M; ++i) {
             const double gd = (double)gp[(size_t)i * N + t];
             const double c = I.vin[gi0 + i] - vo;
@@ -588,6 +713,7 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
         }
     }
     if (t < N) {
+        #pragma unroll 8
         for (int i = 0; i < M; ++i) {
             const double gd = (double)gp[(size_t)i * N + t];
             Ac += gd;
@@ -706,11 +832,21 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                     strow<TC>(I.field + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
                 }
             }
+            if (C.flags & FADESIM_F_ZERO_CORRECTION) {
 #pragma unroll
-            for (int ii = 0; ii < TR; ++ii)
+                for (int ii = 0; ii < TR; ++ii)
 #pragma unroll
-                for (int jj = 0; jj < TC; ++jj)
-                    yp[ii][jj] = yq[ii][jj] = 0.f;
+                    for (int jj = 0; jj < TC; ++jj)
+                        yp[ii][jj] = yq[ii][jj] = 0.f;
+            } else {
+                if (!green_guess<G, STRIP>(X, wsb, gp, gi0, I.rows, t, rwire, hcoef, sh.dead, nbar))
+                    return false;
+#pragma unroll
+                for (int ii = 0; ii < TR; ++ii) {
+                    ldrow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
+                    ldrow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+                }
+            }
             if (t < N)
                 sh.S1[t] = sh.colI[t];
             publish_all<G>(yp, yq, halo, t);

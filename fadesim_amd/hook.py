@@ -101,11 +101,12 @@ def _converged(input, G, batch_size, row, col, numX, numY, g_wire, ret_V):
     wire resistance 1/g_wire, vout = 0."""
     from .solver import solve_batch
     B = batch_size * numX * numY
-    # g[b, x, y, i, c] = G[x, i, y*col + c]
-    g = G.view(numX, row, numY, col).permute(0, 2, 1, 3).to(torch.float32)
-    g = g.unsqueeze(0).expand(batch_size, numX, numY, row, col).reshape(B, row, col)
+    # conductance slot x*numY + y holds G[x, :, y*col:(y+1)*col], shared by the batch
+    slots = G.view(numX, row, numY, col).permute(0, 2, 1, 3).reshape(numX * numY, row, col)
+    slots = slots.to(torch.float32).contiguous()
+    gidx = torch.arange(numX * numY, dtype=torch.int32, device=G.device).repeat(batch_size)
     vin = input.view(batch_size, numX, 1, row).expand(batch_size, numX, numY, row).reshape(B, row)
-    res = solve_batch(g, 1.0 / float(g_wire), vin, tol=tol, want_field=bool(ret_V))
+    res = solve_batch(slots, 1.0 / float(g_wire), vin, gidx=gidx, tol=tol, want_field=bool(ret_V))
     if ret_V:
         # u - d = (vin - p) - q
         f = res.field.to(torch.float64)

@@ -160,6 +160,7 @@ class RaggedPlan:
               warm: Optional[torch.Tensor] = None, *, tol: float = 1e-6, safety: float = 2.0,
               switch_tol: float = 2e-5, max_sweeps: int = 100000, fixed_sweeps: Optional[int] = None,
               refine: bool = True, want_field: bool = False, timing: bool = False,
+              zero_correction: bool = False,
               out: Optional[RaggedResult] = None, check_abort: bool = True) -> RaggedResult:
         """Solve every crossbar of the plan; packed float32 g / warm, float64
         r_wire [K], vin, vout (None = 0 V)."""
@@ -173,6 +174,8 @@ class RaggedPlan:
             flags |= nat.F_NO_REFINE
         if timing:
             flags |= nat.F_TIMING
+        if zero_correction:
+            flags |= nat.F_ZERO_CORRECTION
         d = self._desc(g, r_wire, vin, vout, warm, tol, safety, switch_tol, max_sweeps, flags, res)
         with torch.cuda.device(self.device):
             rc = nat.lib().fadesim_ragged_solve(self._h, ctypes.byref(d),

@@ -53,6 +53,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
                 omega: Optional[float] = None,
                 tol: float = 1e-6, safety: float = 2.0, switch_tol: float = 2e-5,
                 max_sweeps: int = 20000, refine: bool = True, dwr: bool = False,
+                zero_correction: bool = False,
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
                 out: Optional[torch.Tensor] = None, timing: bool = False) -> SolveResult:
     """Solve B crossbars of one shape.
@@ -60,7 +61,9 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     g [B, M, N] float32 (S) -- or [S, M, N] conductance slots with ``gidx`` [B]
     int32 naming each instance's slot; r_wire [B] or scalar (ohm), vin [B, M] (V),
     vout [B, N] or None (0 V), warm [B, 2, M, N] float32 initial drops or None.
-    ``fixed_sweeps=k`` runs exactly k sweeps with no test and no refinement.
+    ``fixed_sweeps=k`` runs exactly k sweeps with no test and no refinement;
+    ``zero_correction`` starts the refinement correction from zero instead of
+    the wire Green's-function estimate of it.
     """
     if g.dim() != 3:
         raise ValueError("g must be [B, M, N]")
@@ -100,6 +103,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
         flags |= nat.F_NO_REFINE
     if dwr:
         flags |= nat.F_DWR
+    if zero_correction:
+        flags |= nat.F_ZERO_CORRECTION
     if timing:
         flags |= nat.F_TIMING
     d = nat.BatchDesc()

@@ -37,6 +37,8 @@ extern "C" {
 #define FADESIM_F_NO_REFINE 2u    /* skip the FP64 iterative-refinement step */
 #define FADESIM_F_DWR 4u          /* return I - (adjoint-weighted residual) and stop on it */
 #define FADESIM_F_TIMING 8u       /* record HIP events around the solve kernel */
+#define FADESIM_F_ZERO_CORRECTION 16u /* start the refinement correction from zero instead of
+                                        * the second-order wire Green's-function estimate */
 
 const char *fadesim_last_error(void);
 int fadesim_version(void);
