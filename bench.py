@@ -17,7 +17,14 @@ of the solve kernel from HIP events on its launch stream; the CPU baseline
 over instances) on a bounded sample, and the measured error of a GPU sample
 against the exact FP64 oracle (checker of the cpu_baseline leg).
 
+Extra objects in the same line (not part of `value`): "uneven" -- 1024
+crossbars each of 64x64, 128x128, 256x256 and 512x512 (shuffled, seed 33)
+solved in one persistent launch and as a captured graph replay; "mlp" -- the
+784-256-10 crossbar MLP on 1024 images (sinh cells, every tile solve
+batched), plain vs NN-warm-started tile solves.
+
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+                [--no-uneven] [--no-mlp] [--mlp-images N]
 """
 from __future__ import annotations
 
@@ -78,6 +85,81 @@ def cpu_baseline(sample_B, seed):
                       f"{dt:.2f} s wall, max rel I_out err vs exact FP64 {err:.2e}"}, (g, r, vin, iex)
 
 
+def bench_uneven(args, dev):
+    """1024 crossbars of each of 64, 128, 256, 512 (the uneven-size workload)."""
+    import torch
+    from fadesim_amd.ragged import RaggedPlan
+    from fadesim_amd.workloads import uneven_batch
+    counts = {64: 1024, 128: 1024, 256: 1024, 512: 1024}
+    inst = uneven_batch(counts, seed=33, device=dev)
+    plan = RaggedPlan(inst["rows"], device=dev)
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=args.tol)
+    if int((res.status != 0).sum()) != 0:
+        raise RuntimeError("uneven batch: not every crossbar converged")
+    steps = max(1, min(args.steps, 3))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(steps):
+        plan.solve(inst["g"], inst["r"], inst["vin"], tol=args.tol, timing=True, out=res)
+        kms.append(plan.last_ms())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    graph, gres = plan.capture(inst["g"], inst["r"], inst["vin"], tol=args.tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    dtg = (time.perf_counter() - t0) / steps
+    if not torch.equal(gres.iout, res.iout):
+        raise RuntimeError("uneven batch: graph replay differs from the eager solve")
+    sizes = torch.tensor(inst["rows"], device=dev)
+    sw = res.sweeps.double()
+    K = len(inst["rows"])
+    return {"workload": "1024 each of 64x64, 128x128, 256x256, 512x512 crossbars, shuffled; g log-uniform "
+                        "[5e-7, 1e-6] S, r ~ U[1,5] ohm, vin = 0.3 k/7 V, one persistent launch",
+            "crossbars": K, "solves_per_s": K / dt, "ms_per_step": 1e3 * dt, "kernel_ms": float(np.mean(kms)),
+            "graph_replay_ms": 1e3 * dtg, "graph_solves_per_s": K / dtg,
+            "mean_sweeps": {str(n): float(sw[sizes == n].mean()) for n in (64, 128, 256, 512)},
+            "plan": plan.stats, "tuned": bool(plan.tuning)}
+
+
+def bench_mlp(args, dev, net):
+    """784-256-10 crossbar MLP (8-bit, 2 bits/cell, 128x128 PN tiles, sinh
+    cells a = 2.5/V), plain vs NN-warm-started tile solves."""
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(5)
+    imgs = torch.rand(args.mlp_images, 784, generator=gen).to(dev)
+    out = {"workload": f"784-256-10 MLP, {args.mlp_images} images U[0,1], 8-bit inputs bit-serial 0/0.2 V, "
+                       "8-bit PN weights 2 bits/cell on 128x128 tiles, sinh cells a=2.5/V, tiles to 1e-6"}
+    res = {}
+    for name in ("plain", "nn") if net is not None else ("plain",):
+        P = pim.mlp_inference_params(warm_start=net if name == "nn" else None, tol=args.tol)
+        model = pim.MLP(params=P, seed=0).to(dev).eval()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        logits = model(imgs)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = dict(P.stats)
+        res[name] = logits
+        out[name] = {"seconds": dt, "images_per_s": args.mlp_images / dt, "tile_solves": st.get("solves"),
+                     "tile_solves_per_s": st.get("solves", 0) / dt, "sweeps": st.get("sweeps"),
+                     "max_picard_steps": st.get("picard_steps")}
+    P0 = pim.PhyArrParams(Ron=1e6, Roff=2e6, cellBits=2, arrRowSize=128, arrColSize=128, inputMaxVoltage=0.2,
+                          ir_mode="none")
+    ideal = pim.MLP(params=P0, seed=0).to(dev).eval()(imgs)
+    if "nn" in res:
+        d = (res["nn"] - res["plain"]).abs()
+        out["nn_vs_plain_logits_max_abs_diff"] = float(d.max())
+        out["nn_vs_plain_argmax_agreement"] = float((res["nn"].argmax(1) == res["plain"].argmax(1)).double().mean())
+    out["ir_drop_effect_vs_ideal_argmax_agreement"] = float(
+        (res["plain"].argmax(1) == ideal.argmax(1)).double().mean())
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +168,9 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--mlp-images", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration) and exit")
@@ -189,6 +274,14 @@ def main():
             out["max_rel_err_warm"] = float(np.max(np.abs(rw.iout.cpu().numpy() - iex) / np.abs(iex)))
     else:
         out["cpu_baseline"] = None
+    for key, flag, fn in (("uneven", args.no_uneven, lambda: bench_uneven(args, dev)),
+                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net))):
+        if flag:
+            continue
+        try:
+            out[key] = fn()
+        except Exception as e:  # an extra workload never hides the headline line
+            out[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
 
 

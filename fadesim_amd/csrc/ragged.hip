@@ -45,6 +45,8 @@ constexpr int kNmax = 512;
 constexpr int kSlotCells = 16384; // cells of one workgroup's tile (all classes)
 constexpr int kTaskInts = 5;      // big task: class, up to four instances
 constexpr int kHaloMax = std::max(std::max(G64::HALO, G128::HALO), std::max(G256::HALO, G512::HALO));
+constexpr int kCbufMax = 2 * std::max(std::max(G64::TY * G64::N, G128::TY * G128::N),
+                                      std::max(G256::TY * G256::N, G512::TY * G512::N));
 static_assert(G64::NT == kThreads && G128::NT == kThreads && G256::NT == kThreads && G512::NT == kThreads,
               "one block size for every class");
 static_assert(G128::M * G128::N == kSlotCells && G256::M * G256::N == kSlotCells &&
@@ -115,7 +117,9 @@ __global__ void __launch_bounds__(kThreads) ragged_kernel(RaggedArgs A)
     __shared__ double colD[kNmax];
     __shared__ float wmax[kThreads / 64];
     __shared__ int s_task, s_dead;
-    const rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead};
+    __shared__ double cbuf[kCbufMax];
+    __shared__ double cext[3 * kNmax];
+    const rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     const rb::StripCtx none{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * kSlotCells;
     const int team = blockIdx.x / kTeam, rank = blockIdx.x % kTeam;

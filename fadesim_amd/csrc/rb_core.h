@@ -54,7 +54,7 @@
 namespace fadesim {
 namespace rb {
 
-constexpr int kWsPlanes = 7; // workspace planes per slot: zq | rho p,q | snapshot p,q | residual p,q
+constexpr int kWsPlanes = 5; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q
 constexpr int kMaxStrips = 16;
 constexpr int kRowsLd = 512;                      // row stride of the strip exchange buffers
 constexpr unsigned long long kWaitTicks = 500000000ull; // 5 s of the 100 MHz constant clock
@@ -379,148 +379,377 @@ struct Shared {
     double *S1, *Ifloor, *colI, *colD;
     float *wmax;
     int *dead;
+    double *cbuf; // [2][TY][N] per-thread-row column partials
+    double *cext; // [3][N] column values shared by the workgroup
 };
 
-// Convergence test, run on the snapshot of the drops in the workspace (only a
-// few values per thread are live).  For every node the FP64 residual
-// r = b - A x (b = g c before the refinement, the stored refinement residual
-// after it); then
-//   D_j = sum_i [ g_ij r W_i(j) + zq_ij r_q(i,j) ],  W_i(j) = sum_j' (min(j,j')+1) r_p(i,j')
-//   I_j = S1_j - sum_i g_ij (p_ij + q_ij)
-// with D_j the adjoint-weighted first-order error of I_j.  want_rho stores the
-// scaled residual as the refinement right-hand side instead of testing.
-// Strips: the rows above/below the strip come from the neighbours'
-// publication `pub`, and the column sums are completed over the team (team
-// barrier `nbar`, parity buffers `part`).
-// Results: colI[j], colD[j] (threads t < N), return value = max_j |D_j|/|I_j|
-// (negative when the team aborted).
-template <class G, bool STRIP>
-__device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0, int t,
-                            double a, double rwire, double hcoef, bool refined, bool want_rho,
-                            const Shared &sh, int pub, int nbar)
+// ---------------------------------------------------------------- tile reductions
+// The convergence test, the set-up and the refinement guess work on each
+// thread's register tile: sums along a row are shuffle scans over the TX
+// lanes of a thread row, sums along a column are LDS scans over the TY thread
+// rows (completed over the team for strips).  Accumulation is FP64.
+
+// Exclusive prefix over the TX lanes of a thread row, and the row total.
+template <int TX>
+__device__ __forceinline__ void row_scan(double x, int tx, double &excl, double &total)
 {
-    constexpr int M = G::M, N = G::N, NT = G::NT, NW = G::NW;
-    const float *zq = wsb;
-    const float *rho_p = wsb + 1 * M * N, *rho_q = wsb + 2 * M * N;
-    const float *fp = wsb + 3 * M * N, *fq = wsb + 4 * M * N;
-    float *resp = wsb + 5 * M * N, *resq = wsb + 6 * M * N;
-    const float *g = gp;
-    const float *up_row = nullptr, *dn_row = nullptr;
-    bool top_open = true, bottom_sink = true;
-    if (STRIP) {
-        const int par = pub & 1;
-        if (X.s > 0) {
-            up_row = X.rows + (((size_t)par * X.S + X.s - 1) * 2 + 1) * kRowsLd;
-            top_open = false;
-        }
-        if (X.s < X.S - 1) {
-            dn_row = X.rows + (((size_t)par * X.S + X.s + 1) * 2 + 0) * kRowsLd;
-            bottom_sink = false;
-        }
-        // the neighbours' rows of publication pub must have arrived
-        if (t == 0) {
-            if (X.s > 0 && !wait_geq(X.flag + 2 * (X.s - 1) + 1, pub + 1, X.abort))
-                *sh.dead = 1;
-            if (X.s < X.S - 1 && !wait_geq(X.flag + 2 * (X.s + 1) + 0, pub + 1, X.abort))
-                *sh.dead = 1;
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (*sh.dead)
-            return -1.0;
+    double incl = x;
+#pragma unroll
+    for (int d = 1; d < TX; d <<= 1) {
+        const double y = __shfl_up(incl, (unsigned)d, TX);
+        if (tx >= d)
+            incl += y;
     }
-    // 1. residuals, one node pair per thread and step
-    #pragma unroll 8
-    for (int k = t; k < M * N; k += NT) {
-        const int i = k / N, j = k - i * N;
-        const double gd = (double)g[k];
-        const double x = (double)fp[k], z = (double)fq[k];
-        const double cell = gd * (x + z);
-        double bp, bq;
-        if (refined) {
-            bp = (double)rho_p[k] / hcoef;
-            bq = (double)rho_q[k] / hcoef;
-        } else {
-            bp = bq = gd * (I.vin[gi0 + i] - I.vout[j]);
-        }
-        const double pl = j > 0 ? (double)fp[k - 1] : 0.0;
-        const double wp = (j < N - 1) ? a * ((pl - x) + ((double)fp[k + 1] - x)) : a * (pl - x);
-        double qd;
-        if (i < M - 1)
-            qd = (double)fq[k + N];
-        else
-            qd = bottom_sink ? 0.0 : (double)dn_row[j];
-        double wq;
-        if (i > 0)
-            wq = a * (((double)fq[k - N] - z) + (qd - z));
-        else
-            wq = top_open ? a * (qd - z) : a * (((double)up_row[j] - z) + (qd - z));
-        const double rpv = (bp - cell) + wp, rqv = (bq - cell) + wq;
-        if (want_rho) {
-            wsb[1 * M * N + k] = (float)(rpv * hcoef);
-            wsb[2 * M * N + k] = (float)(rqv * hcoef);
-        } else {
-            resp[k] = (float)rpv;
-            resq[k] = (float)rqv;
-        }
+    total = __shfl(incl, TX - 1, TX);
+    excl = incl - x;
+}
+
+// One tile row of the wordline Green's sum: out[jj] = scale sum_j' (min(j,j')+1) v(j')
+// over the whole row (j = j0 + jj), the row being spread over a thread row.
+template <class G>
+__device__ __forceinline__ void row_green_row(const double (&v)[G::TC], int j0, int tx, double scale,
+                                              double (&out)[G::TC])
+{
+    constexpr int TC = G::TC;
+    double L0 = 0.0, L1 = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        L0 += v[jj];
+        L1 += (double)(j0 + jj + 1) * v[jj];
+    }
+    double P0, P1, R, R1;
+    row_scan<G::TX>(L0, tx, P0, R);
+    row_scan<G::TX>(L1, tx, P1, R1);
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        const double jp1 = (double)(j0 + jj + 1);
+        out[jj] = scale * (P1 + jp1 * (R - P0));
+        P0 += v[jj];
+        P1 += jp1 * v[jj];
+    }
+}
+
+// Column sums over the workgroup's rows of two per-thread partials (thread
+// row ty, columns j0 + jj): outA[j], outB[j] (LDS, N entries each).
+template <class G>
+__device__ __forceinline__ void col_reduce2(const Shared &sh, const double (&a)[G::TC], const double (&b)[G::TC], int t,
+                                            int ty, int j0, double *outA, double *outB)
+{
+    constexpr int TC = G::TC, TY = G::TY, N = G::N;
+    double *ba = sh.cbuf, *bb = sh.cbuf + TY * N;
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        ba[ty * N + j0 + jj] = a[jj];
+        bb[ty * N + j0 + jj] = b[jj];
     }
     __syncthreads();
-    if (want_rho)
-        return 0.0;
-    // 2. wordline adjoint: one thread per row, v = g r W + zq r_q into resp
-    if (t < M) {
-        const int i = t;
-        double R = 0.0;
-        #pragma unroll 8
-        for (int j = 0; j < N; ++j)
-            R += (double)resp[i * N + j];
-        double P0 = 0.0, P1 = 0.0;
-        #pragma unroll 8
-        for (int j = 0; j < N; ++j) {
-            const int k = i * N + j;
-            const double r = (double)resp[k];
-            const double jp1 = (double)(j + 1);
-            const double W = P1 + jp1 * (R - P0);
-            resp[k] = (float)((double)g[k] * rwire * W + (double)zq[k] * (double)resq[k]);
-            P0 += r;
-            P1 += jp1 * r;
-        }
-    }
-    __syncthreads();
-    // 3. column sums, one thread per column (over the team for strips)
-    double D = 0.0, Y = 0.0;
     if (t < N) {
-        const int j = t;
-        #pragma unroll 8
-        for (int i = 0; i < M; ++i) {
-            const int k = i * N + j;
-            D += (double)resp[k];
-            Y += (double)g[k] * ((double)fp[k] + (double)fq[k]);
+        double sa = 0.0, sb = 0.0;
+        for (int q = 0; q < TY; ++q) {
+            sa += ba[q * N + t];
+            sb += bb[q * N + t];
+        }
+        outA[t] = sa;
+        outB[t] = sb;
+    }
+    __syncthreads();
+}
+
+// Bitline Green's sum of a tile: out(ii,jj) = scale sum_i' (n - max(i,i')) v(i',j)
+// over all n rows of the crossbar (i = gi0 + i0 + ii, j = j0 + jj), v given by
+// vf(ii, jj).  Strips complete the column prefix over the team (team barrier
+// nbar++).  Returns false when the team aborted.
+template <class G, bool STRIP, class VF>
+__device__ __forceinline__ bool col_green_tile(const Shared &sh, const StripCtx &X, VF vf, double scale, int n,
+                                               int gi0, int i0, int j0, int ty, int t, int &nbar,
+                                               float (&out)[G::TR][G::TC])
+{
+    constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
+    double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            const double v = vf(ii, jj);
+            s0 += v;
+            s1 += v * (double)(n - (gi0 + i0 + ii));
+        }
+        b0[ty * N + j0 + jj] = s0;
+        b1[ty * N + j0 + jj] = s1;
+    }
+    __syncthreads();
+    // exclusive prefix over the thread rows, in place; totals of this workgroup
+    double T0 = 0.0, T1 = 0.0;
+    if (t < N) {
+        for (int q = 0; q < TY; ++q) {
+            const double x0 = b0[q * N + t], x1 = b1[q * N + t];
+            b0[q * N + t] = T0;
+            b1[q * N + t] = T1;
+            T0 += x0;
+            T1 += x1;
         }
     }
+    double E0 = 0.0, E1 = 0.0, Tall = T1;
     if (STRIP) {
         double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
         if (t < N) {
-            mine[t] = D;
-            mine[kRowsLd + t] = Y;
+            mine[t] = T0;
+            mine[kRowsLd + t] = T1;
         }
-        if (!team_barrier(X, sh.dead, t, nbar))
+        if (!team_barrier(X, sh.dead, t, nbar++))
+            return false;
+        if (t < N) {
+            Tall = 0.0;
+            for (int s = 0; s < X.S; ++s) {
+                const double *ps = X.part + (((size_t)((nbar - 1) & 1) * X.S + s) * 2) * kRowsLd;
+                if (s < X.s) {
+                    E0 += ps[t];
+                    E1 += ps[kRowsLd + t];
+                }
+                Tall += ps[kRowsLd + t];
+            }
+        }
+    }
+    if (t < N) {
+        sh.cext[t] = E0;
+        sh.cext[N + t] = E1;
+        sh.cext[2 * N + t] = Tall;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        const int j = j0 + jj;
+        double A = sh.cext[j] + b0[ty * N + j];
+        double C = sh.cext[N + j] + b1[ty * N + j];
+        const double T = sh.cext[2 * N + j];
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            const double gi = (double)(n - (gi0 + i0 + ii));
+            const double v = vf(ii, jj);
+            A += v;
+            C += v * gi;
+            out[ii][jj] = (float)(scale * (gi * A + (T - C)));
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// Ideal readout S1, the per-column floor of the relative test and the bitline
+// adjoint zq(i,j) = r sum_i' g(i',j) (n - max(i,i')) (workspace plane 0) of
+// this workgroup's rows of an n-row crossbar (strips: column sums over the
+// team).  Returns false when the team aborted.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
+                                              int i0, int j0, int t, int ty, double rwire, const Shared &sh,
+                                              int &nbar)
+{
+    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    float gt[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
+    double si[TC], sf[TC];
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj)
+        si[jj] = sf[jj] = 0.0;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        const double vi = I.vin[gi0 + i0 + ii];
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const double c = vi - I.vout[j0 + jj];
+            si[jj] += (double)gt[ii][jj] * c;
+            sf[jj] += (double)gt[ii][jj] * fabs(c);
+        }
+    }
+    col_reduce2<G>(sh, si, sf, t, ty, j0, sh.S1, sh.Ifloor);
+    if (STRIP) {
+        double *mine = X.setup + ((size_t)X.s * 4) * kRowsLd;
+        if (t < N) {
+            mine[t] = sh.S1[t];
+            mine[kRowsLd + t] = sh.Ifloor[t];
+        }
+        if (!team_barrier(X, sh.dead, t, nbar++))
+            return false;
+        if (t < N) {
+            double a = 0.0, b = 0.0;
+            for (int s = 0; s < X.S; ++s) {
+                const double *ps = X.setup + ((size_t)s * 4) * kRowsLd;
+                a += ps[t];
+                b += ps[kRowsLd + t];
+            }
+            sh.S1[t] = a;
+            sh.Ifloor[t] = b;
+        }
+    }
+    if (t < N)
+        // a column carrying under 1e-9 of its driven current is judged absolutely
+        sh.Ifloor[t] = fmax(1e-9 * sh.Ifloor[t], 1e-300);
+    float zq[TR][TC];
+    if (!col_green_tile<G, STRIP>(sh, X, [&](int ii, int jj) { return (double)gt[ii][jj]; }, rwire, I.rows, gi0, i0,
+                                  j0, ty, t, nbar, zq))
+        return false;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+        strow<TC>(wsb + (size_t)(i0 + ii) * N + j0, zq[ii]);
+    return true;
+}
+
+// Convergence test on the register tiles.  For every node the FP64 residual
+// r = b - A x (b = g c before the refinement, the stored refinement residual
+// after it; neighbours from the tile and the LDS halos, which hold every
+// edge value after a full sweep); then
+//   D_j = sum_i [ g_ij r W_i(j) + zq_ij r_q(i,j) ],  W_i(j) = sum_j' (min(j,j')+1) r_p(i,j')
+//   I_j = S1_j - sum_i g_ij (p_ij + q_ij)
+// with D_j the adjoint-weighted first-order error of I_j.  want_rho stores the
+// scaled residual (planes 1, 2) as the refinement right-hand side instead.
+// Strips first refresh the neighbours' rows (publication pub) and complete the
+// column sums over the team.  Results: colI[j], colD[j]; return value =
+// max_j |D_j|/|I_j| (negative when the team aborted).
+template <class G, bool STRIP>
+__device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0, int i0, int j0,
+                            int t, int tx, int ty, double a, double rwire, double hcoef, bool refined, bool want_rho,
+                            bool has_l, bool has_r, bool has_u, bool has_d, const Shared &sh, int pub, int &nbar)
+{
+    // the drops come from this thread's snapshot rows (planes 3, 4): the sweep
+    // state is not live across the test, which leaves it the register file
+    const float *snp = wsb + 3 * G::M * G::N, *snq = wsb + 4 * G::M * G::N;
+    constexpr int TR = G::TR, TC = G::TC, N = G::N, M = G::M, NT = G::NT, NW = G::NW, TX = G::TX,
+                  QLD = G::QLD;
+    const float *halo = sh.halo;
+    if (STRIP && !strip_fetch<G>(X, sh.halo, sh.dead, tx, ty, pub))
+        return -1.0;
+    double Dp[TC], Yp[TC];
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj)
+        Dp[jj] = Yp[jj] = 0.0;
+    float qprev[TC], qcur[TC], qnext[TC];
+    ldrow<TC>(snq + (size_t)i0 * N + j0, qcur);
+#pragma unroll 1
+    for (int ii = 0; ii < TR; ++ii) { // rolled: one tile row at a time keeps the test's registers low
+        float grow[TC], rhp[TC], rhq[TC], prow[TC];
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, grow);
+        ldrow<TC>(snp + (size_t)(i0 + ii) * N + j0, prow);
+        if (ii < TR - 1)
+            ldrow<TC>(snq + (size_t)(i0 + ii + 1) * N + j0, qnext);
+        if (refined) {
+            ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rhp);
+            ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rhq);
+        }
+        float zrow[TC];
+        if (!want_rho)
+            ldrow<TC>(wsb + (size_t)(i0 + ii) * N + j0, zrow);
+        const double vi = I.vin[gi0 + i0 + ii];
+        float rpf[TC], rqf[TC]; // residuals of the row (FP64 arithmetic, FP32 storage)
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const double x = (double)prow[jj], z = (double)qcur[jj], gd = (double)grow[jj];
+            const double cell = gd * (x + z);
+            double bp, bq;
+            if (refined) {
+                bp = (double)rhp[jj] / hcoef;
+                bq = (double)rhq[jj] / hcoef;
+            } else {
+                bp = bq = gd * (vi - I.vout[j0 + jj]);
+            }
+            double pl = 0.0;
+            if (jj > 0)
+                pl = (double)prow[jj - 1];
+            else if (has_l)
+                pl = (double)halo[G::PR + ii * NT + t - 1];
+            double wp;
+            if (jj < TC - 1)
+                wp = a * ((pl - x) + ((double)prow[jj + 1] - x));
+            else if (has_r)
+                wp = a * ((pl - x) + ((double)halo[G::PL + ii * NT + t + 1] - x));
+            else
+                wp = a * (pl - x); // open right end
+            double qd = 0.0;       // sink below the last row
+            if (ii < TR - 1)
+                qd = (double)qnext[jj];
+            else if (has_d)
+                qd = (double)halo[G::QT + jj * QLD + 2 * TX + t];
+            double wq;
+            if (ii > 0)
+                wq = a * (((double)qprev[jj] - z) + (qd - z));
+            else if (has_u)
+                wq = a * (((double)halo[G::QB + jj * QLD + t] - z) + (qd - z));
+            else
+                wq = a * (qd - z); // open top
+            const double rpv = (bp - cell) + wp, rqv = (bq - cell) + wq;
+            if (want_rho) {
+                rpf[jj] = (float)(rpv * hcoef);
+                rqf[jj] = (float)(rqv * hcoef);
+            } else {
+                rpf[jj] = (float)rpv;
+                Dp[jj] += (double)zrow[jj] * rqv; // bitline part of the adjoint
+                Yp[jj] += cell;
+            }
+        }
+        if (want_rho) {
+            strow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rpf);
+            strow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rqf);
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj) {
+                qprev[jj] = qcur[jj];
+                qcur[jj] = qnext[jj];
+            }
+            continue;
+        }
+        // wordline part: g r W_i(j), W by a scan over the thread row
+        double L0 = 0.0, L1 = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            L0 += (double)rpf[jj];
+            L1 += (double)(j0 + jj + 1) * (double)rpf[jj];
+        }
+        double P0, P1, R, R1;
+        row_scan<TX>(L0, tx, P0, R);
+        row_scan<TX>(L1, tx, P1, R1);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const double jp1 = (double)(j0 + jj + 1);
+            Dp[jj] += (double)grow[jj] * rwire * (P1 + jp1 * (R - P0));
+            P0 += (double)rpf[jj];
+            P1 += jp1 * (double)rpf[jj];
+        }
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            qprev[jj] = qcur[jj];
+            qcur[jj] = qnext[jj];
+        }
+    }
+    if (want_rho) {
+        __syncthreads();
+        return 0.0;
+    }
+    col_reduce2<G>(sh, Dp, Yp, t, ty, j0, sh.colD, sh.colI); // colI holds sum g (p + q) for now
+    if (STRIP) {
+        double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
+        if (t < N) {
+            mine[t] = sh.colD[t];
+            mine[kRowsLd + t] = sh.colI[t];
+        }
+        if (!team_barrier(X, sh.dead, t, nbar++))
             return -1.0;
         if (t < N) {
-            D = Y = 0.0;
+            double D = 0.0, Y = 0.0;
             for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.part + (((size_t)(nbar & 1) * X.S + s) * 2) * kRowsLd;
+                const double *ps = X.part + (((size_t)((nbar - 1) & 1) * X.S + s) * 2) * kRowsLd;
                 D += ps[t];
                 Y += ps[kRowsLd + t];
             }
+            sh.colD[t] = D;
+            sh.colI[t] = Y;
         }
     }
     float emax = 0.f;
     if (t < N) {
-        const double Ij = sh.S1[t] - Y;
+        const double Ij = sh.S1[t] - sh.colI[t];
         sh.colI[t] = Ij;
-        sh.colD[t] = D;
-        const double e = fabs(D) / fmax(fabs(Ij), sh.Ifloor[t]);
+        const double e = fabs(sh.colD[t]) / fmax(fabs(Ij), sh.Ifloor[t]);
         emax = (e == e) ? (float)fmin(e, 3.0e38) : INFINITY;
     }
 #pragma unroll
@@ -535,200 +764,75 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
         em = fmaxf(em, sh.wmax[w]);
     // identical in every lane: make it a scalar
     em = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, em)));
+    __syncthreads(); // wmax / colI reads done before reuse
     return (double)em;
 }
 
-// Column Green's sums over the whole crossbar for rows [gi0, gi0 + M) of an
-// n-row crossbar: out(i,j) = r sum_i' (n - max(i,i')) v(i',j) with v read by
-// vfun(i, j) (FP64), written to out.  One thread per column; strips complete
-// the sums before/after their rows over the team (partials in `part`, team
-// barrier `nbar`).  Returns false when the team aborted.
-template <class G, bool STRIP, class VF>
-__device__ __forceinline__ bool column_green(const StripCtx &X, int gi0, int n, int t, double rwire, VF vfun,
-                                             float *out, int *dead, int nbar)
-{
-    constexpr int M = G::M, N = G::N;
-    double A = 0.0, B = 0.0, sA = 0.0, sB = 0.0;
-    if (t < N) {
-        #pragma unroll 8
-        for (int i = 0; i < M; ++i) {
-            const double v = vfun(i, t);
-            sA += v;
-            sB += v * (double)(n - (gi0 + i));
-        }
-        B = sB;
-    }
-    if (STRIP) {
-        double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
-        if (t < N) {
-            mine[t] = sA;
-            mine[kRowsLd + t] = sB;
-        }
-        if (!team_barrier(X, dead, t, nbar))
-            return false;
-        if (t < N) {
-            B = 0.0;
-            for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.part + (((size_t)(nbar & 1) * X.S + s) * 2) * kRowsLd;
-                if (s < X.s)
-                    A += ps[t];
-                else
-                    B += ps[kRowsLd + t];
-            }
-        }
-    }
-    if (t < N) {
-        #pragma unroll 8
-        for (int i = 0; i < M; ++i) {
-            const double v = vfun(i, t);
-            A += v;
-            B -= v * (double)(n - (gi0 + i));
-            out[(size_t)i * N + t] = (float)(rwire * ((double)(n - (gi0 + i)) * A + B));
-        }
-    }
-    return true;
-}
-
-// Row Green's sums: out(i,j) = r sum_j' (min(j,j')+1) v(i,j'), one thread per row.
-template <class G, class VF>
-__device__ __forceinline__ void row_green(int t, double rwire, VF vfun, float *out)
-{
-    constexpr int M = G::M, N = G::N;
-    if (t < M) {
-        const int i = t;
-        double R = 0.0;
-        #pragma unroll 8
-        for (int j = 0; j < N; ++j)
-            R += vfun(i, j);
-        double P0 = 0.0, P1 = 0.0;
-        #pragma unroll 8
-        for (int j = 0; j < N; ++j) {
-            const double v = vfun(i, j);
-            const double jp1 = (double)(j + 1);
-            out[(size_t)i * N + j] = (float)(rwire * (P1 + jp1 * (R - P0)));
-            P0 += v;
-            P1 += jp1 * v;
-        }
-    }
-}
-
-// Initial guess of the refinement correction y (A y = residual): the
-// second-order superposition of the wire Green's functions,
+// Initial guess of the refinement correction y (A y = residual), into yp, yq:
+// the second-order superposition of the wire Green's functions,
 //   y1 = (aL)^-1 res,  y2 = y1 - (aL)^-1 G (y1p + y1q)
 // with (aL)^-1 the wordline (r (min(j,j')+1)) and bitline (r (n - max(i,i')))
 // line inverses; the cell coupling G is ~1e-6 of the wire conductance, so y2
 // is within a few 1e-3 of y and the correction converges in a few sweeps
-// instead of restarting from zero.  Reads the scaled residual (planes 1, 2),
-// writes y2 to planes 3, 4 (the snapshot is no longer needed), uses planes 5,
-// 6 for y1.
+// instead of restarting from zero.  Reads the scaled residual (planes 1, 2).
 template <class G, bool STRIP>
-__device__ __forceinline__ bool green_guess(const StripCtx &X, float *wsb, const float *g, int gi0, int n, int t,
-                                            double rwire, double hcoef, int *dead, int &nbar)
+__device__ __forceinline__ bool green_guess(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
+                                            int i0, int j0, int t, int tx, int ty, double rwire, double hcoef,
+                                            const Shared &sh, int &nbar, float (&yp)[G::TR][G::TC],
+                                            float (&yq)[G::TR][G::TC])
 {
-    constexpr int M = G::M, N = G::N;
-    const float *rho_p = wsb + 1 * M * N, *rho_q = wsb + 2 * M * N;
-    float *y2p = wsb + 3 * M * N, *y2q = wsb + 4 * M * N;
-    float *y1p = wsb + 5 * M * N, *y1q = wsb + 6 * M * N;
+    constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
     const double ih = 1.0 / hcoef;
-    row_green<G>(t, rwire, [&](int i, int j) { return (double)rho_p[(size_t)i * N + j] * ih; }, y1p);
-    if (!column_green<G, STRIP>(X, gi0, n, t, rwire,
-                                [&](int i, int j) { return (double)rho_q[(size_t)i * N + j] * ih; }, y1q, dead,
-                                nbar++))
-        return false;
-    __syncthreads();
-    auto coupling = [&](int i, int j) {
-        const size_t k = (size_t)i * N + j;
-        return (double)g[k] * ((double)y1p[k] + (double)y1q[k]);
-    };
-    row_green<G>(t, rwire, coupling, y2p);
-    if (!column_green<G, STRIP>(X, gi0, n, t, rwire, coupling, y2q, dead, nbar++))
-        return false;
-    __syncthreads();
-    for (int k = t; k < M * N; k += G::NT) {
-        y2p[k] = y1p[k] - y2p[k];
-        y2q[k] = y1q[k] - y2q[k];
-    }
-    __syncthreads();
-    return true;
-}
-
-template <class G>
-__device__ __forceinline__ void snapshot(float *wsb, const float (&yp)[G::TR][G::TC],
-                                         const float (&yq)[G::TR][G::TC], int i0, int j0)
-{
-    constexpr int M = G::M, N = G::N;
+    float y1p[TR][TC], y1q[TR][TC];
 #pragma unroll
-    for (int ii = 0; ii < G::TR; ++ii) {
-        strow<G::TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
-        strow<G::TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+    for (int ii = 0; ii < TR; ++ii) {
+        float rr[TC];
+        ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rr);
+        double v[TC], o[TC];
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            v[jj] = (double)rr[jj] * ih;
+        row_green_row<G>(v, j0, tx, rwire, o);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            y1p[ii][jj] = (float)o[jj];
     }
-}
-
-// Ideal readout S1, the per-column floor of the relative test and the bitline
-// adjoint zq(i,j) = r sum_i' g(i',j) (n - max(i,i')) of this workgroup's rows
-// [gi0, gi0 + M) of an n-row crossbar (n = S*M for strips: column sums over the
-// team through `setup` and team barrier `nbar`).
-template <class G, bool STRIP>
-__device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, float *wsb, const float *gp,
-                                              int gi0, int t, double rwire, const Shared &sh, int nbar)
-{
-    constexpr int M = G::M, N = G::N;
-    const int n = I.rows;
-    double si = 0.0, sf = 0.0, sg = 0.0, sT = 0.0;
-    if (t < N) {
-        const double vo = I.vout[t];
-        #pragma unroll 8
-        for (int i = 0; i < M; ++i) {
-            const double gd = (double)gp[(size_t)i * N + t];
-            const double c = I.vin[gi0 + i] - vo;
-            si += gd * c;
-            sf += gd * fabs(c);
-            sg += gd;
-            sT += gd * (double)(n - (gi0 + i));
-        }
-    }
-    double Ac = 0.0, Bs = sT;
-    if (STRIP) {
-        if (t < N) {
-            double *mine = X.setup + ((size_t)X.s * 4) * kRowsLd;
-            mine[t] = si;
-            mine[kRowsLd + t] = sf;
-            mine[2 * kRowsLd + t] = sg;
-            mine[3 * kRowsLd + t] = sT;
-        }
-        if (!team_barrier(X, sh.dead, t, nbar))
+    {
+        float rq[TR][TC];
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii)
+            ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rq[ii]);
+        if (!col_green_tile<G, STRIP>(sh, X, [&](int ii, int jj) { return (double)rq[ii][jj] * ih; }, rwire, I.rows,
+                                      gi0, i0, j0, ty, t, nbar, y1q))
             return false;
-        if (t < N) {
-            si = sf = 0.0;
-            Bs = 0.0;
-            for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.setup + ((size_t)s * 4) * kRowsLd;
-                si += ps[t];
-                sf += ps[kRowsLd + t];
-                if (s < X.s)
-                    Ac += ps[2 * kRowsLd + t];
-                else
-                    Bs += ps[3 * kRowsLd + t];
-            }
-        }
     }
-    if (t < N) {
-        #pragma unroll 8
-        for (int i = 0; i < M; ++i) {
-            const double gd = (double)gp[(size_t)i * N + t];
-            Ac += gd;
-            Bs -= gd * (double)(n - (gi0 + i));
-            wsb[(size_t)i * N + t] = (float)(rwire * ((double)(n - (gi0 + i)) * Ac + Bs));
-        }
-        sh.S1[t] = si;
-        // a column carrying under 1e-9 of its driven current is judged absolutely
-        sh.Ifloor[t] = fmax(1e-9 * sf, 1e-300);
+    float gt[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
+    auto coupling = [&](int ii, int jj) { return (double)gt[ii][jj] * ((double)y1p[ii][jj] + (double)y1q[ii][jj]); };
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        double v[TC], o[TC];
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            v[jj] = coupling(ii, jj);
+        row_green_row<G>(v, j0, tx, rwire, o);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            yp[ii][jj] = y1p[ii][jj] - (float)o[jj];
     }
+    float cq[TR][TC];
+    if (!col_green_tile<G, STRIP>(sh, X, coupling, rwire, I.rows, gi0, i0, j0, ty, t, nbar, cq))
+        return false;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            yq[ii][jj] = y1q[ii][jj] - cq[ii][jj];
     return true;
 }
 
-// Solve one crossbar (STRIP = false) or strip X.s of one (STRIP = true) with
-// workspace slot wsb.  Returns false when the team aborted.
 template <class G, bool STRIP>
 __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ctl &C, float *wsb,
                                       const Shared &sh)
@@ -763,7 +867,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
         *sh.dead = 0;
     __syncthreads();
 
-    if (!setup_columns<G, STRIP>(I, X, wsb, gp, gi0, t, rwire, sh, nbar++))
+    if (!setup_columns<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, sh, nbar))
         return false;
 
     float yp[TR][TC], yq[TR][TC], hn[TR][TC], rp[TR][TC], rq[TR][TC];
@@ -809,29 +913,37 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 strip_publish<G>(X, yq, ty, j0, lane, pub++);
             ++sweep;
         }
-        snapshot<G>(wsb, yp, yq, i0, j0);
-        __syncthreads();
-        est = conv_test<G, STRIP>(I, X, wsb, gp, gi0, t, a, rwire, hcoef, refined, false, sh, pub - 1, nbar++);
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) { // snapshot: the sweep state is not live across the test
+            strow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
+            strow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+        }
+        est = conv_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, false, has_l,
+                                  has_r, has_u, has_d, sh, pub - 1, nbar);
         if (est < 0.0)
             return false;
+        bool reload = true, done = false;
         if (fixed) {
             st = FADESIM_ST_CONVERGED;
-            break;
-        }
-        if (!(est < INFINITY)) {
+            done = true;
+        } else if (!(est < INFINITY)) {
             st = FADESIM_ST_NONFINITE;
-            break;
-        }
-        if (refine_pending && est <= C.switch_tol) {
+            done = true;
+        } else if (refine_pending && est <= C.switch_tol) {
             // ---- one iterative-refinement step: rhs <- exact FP64 residual ----
-            conv_test<G, STRIP>(I, X, wsb, gp, gi0, t, a, rwire, hcoef, false, true, sh, pub - 1, nbar);
+            conv_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false, true, has_l, has_r,
+                                has_u, has_d, sh, pub - 1, nbar);
             if (I.field) {
 #pragma unroll
                 for (int ii = 0; ii < TR; ++ii) {
-                    strow<TC>(I.field + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
-                    strow<TC>(I.field + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
+                    float v[TC];
+                    ldrow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, v);
+                    strow<TC>(I.field + (size_t)(gi0 + i0 + ii) * N + j0, v);
+                    ldrow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, v);
+                    strow<TC>(I.field + MNall + (size_t)(gi0 + i0 + ii) * N + j0, v);
                 }
             }
+            reload = false;
             if (C.flags & FADESIM_F_ZERO_CORRECTION) {
 #pragma unroll
                 for (int ii = 0; ii < TR; ++ii)
@@ -839,13 +951,8 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                     for (int jj = 0; jj < TC; ++jj)
                         yp[ii][jj] = yq[ii][jj] = 0.f;
             } else {
-                if (!green_guess<G, STRIP>(X, wsb, gp, gi0, I.rows, t, rwire, hcoef, sh.dead, nbar))
+                if (!green_guess<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, sh, nbar, yp, yq))
                     return false;
-#pragma unroll
-                for (int ii = 0; ii < TR; ++ii) {
-                    ldrow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
-                    ldrow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
-                }
             }
             if (t < N)
                 sh.S1[t] = sh.colI[t];
@@ -857,12 +964,21 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             refined = true;
         } else if (!refine_pending && est <= stop_tol) {
             st = FADESIM_ST_CONVERGED;
-            break;
+            done = true;
         }
-        if (sweep >= C.max_sweeps) {
+        if (!done && sweep >= C.max_sweeps) {
             st = FADESIM_ST_MAXIT;
-            break;
+            done = true;
         }
+        if (reload) {
+#pragma unroll
+            for (int ii = 0; ii < TR; ++ii) {
+                ldrow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
+                ldrow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+            }
+        }
+        if (done)
+            break;
         const double tgt = refine_pending ? C.switch_tol : stop_tol;
         const float k = (est > tgt) ? 0.8f * __log2f((float)(tgt / est)) / lrho : (float)C.check_min;
         const int kk = (int)fminf(fmaxf(k, (float)C.check_min), (float)C.check_max);

@@ -54,7 +54,9 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
     __shared__ double colD[N];
     __shared__ float wmax[G::NW];
     __shared__ int s_inst, s_dead;
-    const rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead};
+    __shared__ double cbuf[2 * G::TY * N];
+    __shared__ double cext[3 * N];
+    const rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     const rb::StripCtx none{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     for (;;) {

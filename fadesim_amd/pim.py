@@ -92,7 +92,7 @@ class PhyArrParams:
         self.tol = 1e-6                       # converged mode: relative I_out tolerance
         self.warm_start = None                # converged mode: WarmStartNet or None
         self.max_batch = 65536                # instances per batched solve
-        self.stats = {}                       # filled by the last converged call
+        self.stats = {}                       # converged mode: solves, sweeps, Picard steps (accumulated)
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -257,7 +257,10 @@ def _solve_converged(bits, G, inputV, P):
     gidx = (((jj * 2 + ss) * numX + xx) * numY + yy).to(torch.int32)
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
     net = P.warm_start
-    stats = {"solves": total, "sweeps": 0, "picard_steps": 0, "calls": 0}
+    stats = P.stats  # accumulates over calls until the caller resets it
+    for k in ("solves", "sweeps", "picard_steps", "calls"):
+        stats.setdefault(k, 0)
+    stats["solves"] += total
     for c0 in range(0, total, P.max_batch):
         c1 = min(total, c0 + P.max_batch)
         n = c1 - c0
@@ -279,7 +282,6 @@ def _solve_converged(bits, G, inputV, P):
         if int((res.status != 0).sum()) != 0:
             raise RuntimeError("crossbar solve did not converge")
         I[ii[sl], jj[sl], ss[sl], bb[sl], xx[sl], yy[sl]] = res.iout
-    P.stats = stats
     return I
 
 

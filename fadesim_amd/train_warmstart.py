@@ -77,10 +77,11 @@ def main():
     ap.add_argument("--eval", type=int, default=4096)
     ap.add_argument("--max-seconds", type=float, default=600)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--arch", default="unet16", choices=sorted(warmstart.ARCHITECTURES))
     args = ap.parse_args()
     dev = torch.device("cuda:0")
     log = lambda s: print(s, flush=True)
-    net = warmstart.WarmStartNet(dev, seed=args.seed)
+    net = warmstart.WarmStartNet(dev, seed=args.seed, arch=args.arch)
     log(f"network: {net.n_params()} parameters")
     t0 = time.time()
     data = warmstart.generate_dataset(net, args.n, TRAIN_SEED0)

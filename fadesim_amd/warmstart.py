@@ -59,7 +59,7 @@ class Op:
     skip: str = ""   # up_add: tensor added after upsampling
 
 
-def architecture() -> List[Op]:
+def _unet16() -> List[Op]:
     C = [16, 32, 64, 96]
     return [
         Op("e0a", "conv", "x", 0, N_FEAT, C[0]),
@@ -86,6 +86,36 @@ def architecture() -> List[Op]:
     ]
 
 
+def _slim() -> List[Op]:
+    """Two full-resolution convolutions (input and output), 16/32/64/64
+    channels: about half the multiply-adds of unet16."""
+    return [
+        Op("e0", "conv", "x", 0, N_FEAT, 16),
+        Op("p1", "pool", "e0", 1, 16, 16),
+        Op("e1a", "conv", "p1", 1, 16, 32),
+        Op("e1", "conv", "e1a", 1, 32, 32),
+        Op("p2", "pool", "e1", 2, 32, 32),
+        Op("e2a", "conv", "p2", 2, 32, 64),
+        Op("e2", "conv", "e2a", 2, 64, 64),
+        Op("p3", "pool", "e2", 3, 64, 64),
+        Op("e3", "conv", "p3", 3, 64, 64),
+        Op("t3", "conv", "e3", 3, 64, 64),
+        Op("u2", "up_add", "t3", 2, 64, 64, skip="e2"),
+        Op("t2", "conv", "u2", 2, 64, 32),
+        Op("u1", "up_add", "t2", 1, 32, 32, skip="e1"),
+        Op("t1", "conv", "u1", 1, 32, 16),
+        Op("u0", "up_add", "t1", 0, 16, 16, skip="e0"),
+        Op("out", "conv", "u0", 0, 16, OUT_C, relu=False),
+    ]
+
+
+ARCHITECTURES = {"unet16": _unet16, "slim": _slim}
+
+
+def architecture(name: str = "unet16") -> List[Op]:
+    return ARCHITECTURES[name]()
+
+
 def _s(dev) -> ctypes.c_void_p:
     return ctypes.c_void_p(nat.stream_handle(dev))
 
@@ -97,10 +127,11 @@ def _chk(rc, what):
 class WarmStartNet:
     """Parameters, forward, backward and Adam of the warm-start network."""
 
-    def __init__(self, device, M: int = 128, N: int = 128, seed: int = 0):
+    def __init__(self, device, M: int = 128, N: int = 128, seed: int = 0, arch: str = "unet16"):
         self.dev = torch.device(device)
         self.M, self.N = M, N
-        self.ops = architecture()
+        self.arch = arch
+        self.ops = architecture(arch)
         self.offsets: Dict[str, int] = {}
         n = 0
         for op in self.ops:
@@ -131,7 +162,7 @@ class WarmStartNet:
         return sum(op.cout * (9 * op.cin + 1) for op in self.ops if op.kind == "conv")
 
     def describe(self) -> dict:
-        return {"params": self.n_params(), "layers": sum(op.kind == "conv" for op in self.ops),
+        return {"arch": self.arch, "params": self.n_params(), "layers": sum(op.kind == "conv" for op in self.ops),
                 "dtype": "bf16 (fp32 accumulate)", **{k: v for k, v in self.info.items() if k != "loss_curve"}}
 
     def save(self, name: str = DEFAULT_NAME, directory: str = WEIGHTS_DIR):
@@ -139,7 +170,7 @@ class WarmStartNet:
         self.w.detach().cpu().numpy().astype("<f4").tofile(os.path.join(directory, name + ".bin"))
         man = {"format": "raw little-endian float32, layers in order, each [Cout][Kp] with k = tap*Cin + ci, "
                          "bias at k = 9*Cin",
-               "rows": self.M, "cols": self.N, "features": N_FEAT, "out_channels": OUT_C,
+               "arch": self.arch, "rows": self.M, "cols": self.N, "features": N_FEAT, "out_channels": OUT_C,
                "layers": [{"name": op.name, "kind": op.kind, "src": op.src, "level": op.level, "cin": op.cin,
                            "cout": op.cout, "relu": op.relu, "skip": op.skip,
                            "offset": self.offsets.get(op.name), "kp": kpad(op.cin) if op.kind == "conv" else None}
@@ -152,7 +183,7 @@ class WarmStartNet:
     def load(cls, device, name: str = DEFAULT_NAME, directory: str = WEIGHTS_DIR) -> "WarmStartNet":
         with open(os.path.join(directory, name + ".json")) as f:
             man = json.load(f)
-        net = cls(device, man["rows"], man["cols"])
+        net = cls(device, man["rows"], man["cols"], arch=man.get("arch", "unet16"))
         import numpy as np
         w = np.fromfile(os.path.join(directory, name + ".bin"), dtype="<f4")
         if w.size != net.nparams_padded:

@@ -114,6 +114,7 @@ def test_mlp_inference_plain_vs_warm(cuda):
     plain = model(imgs)
     s_plain = P.stats["sweeps"]
     P.warm_start = net
+    P.stats = {}
     warm = model(imgs)
     s_warm = P.stats["sweeps"]
     assert torch.max(torch.abs(plain - warm)).item() <= 0.02 * torch.max(torch.abs(plain)).item()
