@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libf
 
 FADESIM_OK = 0
 ST_CONVERGED, ST_MAXIT, ST_NONFINITE = 0, 1, 2
-F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING, F_ZERO_CORRECTION = 1, 2, 4, 8, 16
+F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING, F_ZERO_CORRECTION, F_TEST_FIRST = 1, 2, 4, 8, 16, 32
+MAX_IR2S_ITERS = 64
 
 _vp = ctypes.c_void_p
 
@@ -27,6 +28,7 @@ class BatchDesc(ctypes.Structure):
         ("switch_tol", ctypes.c_double), ("max_sweeps", ctypes.c_int),
         ("flags", ctypes.c_uint),
         ("d_iout", _vp), ("d_sweeps", _vp), ("d_status", _vp), ("d_est", _vp), ("d_field", _vp),
+        ("ir2s_iters", ctypes.c_int), ("corr_iters", ctypes.c_int), ("ir2s_omega", ctypes.c_double),
     ]
 
 
@@ -56,13 +58,31 @@ class HookDesc(ctypes.Structure):
     ]
 
 
+class Ir2sDesc(ctypes.Structure):
+    """Mirror of ``fadesim_ir2s_desc``."""
+    _fields_ = [
+        ("dtype_bytes", ctypes.c_int), ("layout", ctypes.c_int),
+        ("d_vin", _vp), ("d_G", _vp), ("d_vout", _vp),
+        ("batch_size", ctypes.c_int), ("row_size", ctypes.c_int), ("col_size", ctypes.c_int),
+        ("numX", ctypes.c_int), ("numY", ctypes.c_int), ("iter_times", ctypes.c_int),
+        ("r_wire", ctypes.c_double), ("r_source", ctypes.c_double), ("r_sink", ctypes.c_double),
+        ("divide_by_g", ctypes.c_int), ("carry_residue", ctypes.c_int),
+        ("enable_break", ctypes.c_int), ("break_threshold", ctypes.c_double),
+        ("nonlinear", ctypes.c_int), ("nonlinear_params", ctypes.c_double * 6),
+        ("ret_v", ctypes.c_int), ("d_out", _vp), ("d_vup", _vp), ("d_vdown", _vp), ("d_iters", _vp),
+    ]
+
+
+IR2S_HOOK, IR2S_PLAIN = 0, 1
+IR2S_BREAK_BATCH, IR2S_BREAK_EACH = 1, 2
+
 # Every symbol include/fadesim.h declares.
 EXPORTED = [
     "fadesim_last_error", "fadesim_version", "fadesim_solve_batch", "fadesim_default_omega",
     "fadesim_last_kernel_ms", "fadesim_cell_conductance",
     "fadesim_ragged_plan_create", "fadesim_ragged_plan_destroy", "fadesim_ragged_plan_info",
     "fadesim_ragged_solve", "fadesim_ragged_aborted", "fadesim_ragged_last_ms",
-    "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs",
+    "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs", "fadesim_ir2s",
     "fadesim_nn_features", "fadesim_nn_combine", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
     "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
     "fadesim_nn_up2_bwd", "fadesim_nn_relu_mask", "fadesim_nn_add", "fadesim_nn_mse", "fadesim_nn_adam",
@@ -94,6 +114,9 @@ def lib():
         if hasattr(L, "fadesim_hook_ir_drop_gs"):
             L.fadesim_hook_ir_drop_gs.argtypes = [ctypes.POINTER(HookDesc), _vp]
             L.fadesim_hook_ir_drop_gs.restype = ctypes.c_int
+        if hasattr(L, "fadesim_ir2s"):
+            L.fadesim_ir2s.argtypes = [ctypes.POINTER(Ir2sDesc), _vp]
+            L.fadesim_ir2s.restype = ctypes.c_int
         _declare_nn(L)
         _declare_ragged(L)
         _LIB = L
@@ -128,14 +151,16 @@ def _declare_nn(L):
 
 class RaggedClassOpts(ctypes.Structure):
     _fields_ = [("omega", ctypes.c_double), ("check_first", ctypes.c_int), ("check_min", ctypes.c_int),
-                ("check_max", ctypes.c_int)]
+                ("check_max", ctypes.c_int), ("ir2s_iters", ctypes.c_int), ("corr_iters", ctypes.c_int),
+                ("ir2s_omega", ctypes.c_double)]
 
 
 class RaggedPlanStats(ctypes.Structure):
     _fields_ = [("count", ctypes.c_int), ("cells", ctypes.c_longlong), ("rows_total", ctypes.c_longlong),
                 ("cols_total", ctypes.c_longlong), ("grid", ctypes.c_int), ("teams", ctypes.c_int),
                 ("big_tasks", ctypes.c_int), ("small_items", ctypes.c_int), ("omega", ctypes.c_double * 4),
-                ("check_first", ctypes.c_int * 4), ("check_min", ctypes.c_int * 4), ("check_max", ctypes.c_int * 4)]
+                ("check_first", ctypes.c_int * 4), ("check_min", ctypes.c_int * 4), ("check_max", ctypes.c_int * 4),
+                ("ir2s_iters", ctypes.c_int * 4), ("corr_iters", ctypes.c_int * 4), ("ir2s_omega", ctypes.c_double * 4)]
 
 
 class RaggedDesc(ctypes.Structure):

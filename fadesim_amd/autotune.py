@@ -2,10 +2,13 @@
 
 For each crossbar size (64, 128, 256, 512) it times the persistent solve of a
 sample of the uneven-size workload (fadesim_amd.workloads.uneven_batch) on the
-GPU over a grid of candidates -- the relaxation factor
+GPU over a grid of candidates and keeps the fastest candidate whose solves
+all reach the tolerance: first the number of wire-solve (IR2S) iterations
+applied to the initial guess and to the refinement correction (their
+contraction per iteration grows with r g n^2, so small crossbars want several
+and 512x512 few), then, with that choice, the relaxation factor
 w(k) = 2 / (1 + k sin(pi / (2n + 1))) and the bounds of the interval between
-convergence tests -- and keeps the fastest candidate whose solves all reach
-the tolerance.  The table is stored per GPU architecture under
+convergence tests.  The table is stored per GPU architecture under
 fadesim_amd/tuning/<arch>.json and read by RaggedPlan.
 
 python -m fadesim_amd.autotune [--per-size 16,32,64,256] [--tol 1e-6] [--out PATH]
@@ -24,6 +27,8 @@ import torch
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 KAPPAS = (0.55, 0.7, 0.83, 1.0, 1.25)
 CHECKS = ((32, 16, 256), (64, 32, 256), (32, 32, 512))
+IR2S = (0, 2, 4, 6, 8, 12, 16)   # iterations on the initial guess (and the correction's)
+IR2S_OMEGA = (1.0, 0.85, 0.7)    # their damping: 2/(2+rho) for coupling radius rho
 
 
 def arch_name(device) -> str:
@@ -55,23 +60,37 @@ def tune_size(n: int, count: int, tol: float, dev, log=print, reps: int = 2) -> 
     from .ragged import RaggedPlan
     from .workloads import uneven_batch
     inst = uneven_batch({n: count}, seed=4242 + n, device=dev)
-    results = []
+
+    def trial(kappa, entry):
+        plan = RaggedPlan(inst["rows"], device=dev, tuning={str(n): entry})
+        res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=tol)  # warm-up, and correctness
+        ok = int((res.status != 0).sum()) == 0
+        times = []
+        for _ in range(reps):
+            plan.solve(inst["g"], inst["r"], inst["vin"], tol=tol, timing=True)
+            times.append(plan.last_ms())
+        ms = min(times)
+        sw = float(res.sweeps.double().mean())
+        log(f"  n={n} kappa={kappa:.2f} w={entry['omega']:.5f} checks={entry['check_first']}/"
+            f"{entry['check_min']}/{entry['check_max']} ir2s={entry['ir2s_iters']}/{entry['corr_iters']}"
+            f" w_ir2s={entry['ir2s_omega']:.2f}: "
+            f"{ms:.3f} ms, mean sweeps {sw:.0f}{'' if ok else ' (not converged)'}")
+        return (ms if ok else math.inf, kappa, entry, sw)
+
+    def make(kappa, checks, k, w):
+        cf, cmin, cmax = checks
+        return {"omega": omega_of(n, kappa), "check_first": cf, "check_min": cmin, "check_max": cmax,
+                "ir2s_iters": k, "corr_iters": max(2, k), "ir2s_omega": w}
+
+    stage1 = [trial(1.0, make(1.0, CHECKS[0], k, w)) for k in IR2S for w in IR2S_OMEGA if k > 0 or w == 1.0]
+    e1 = min(stage1, key=lambda x: x[0])[2]
+    k_best, w_best = e1["ir2s_iters"], e1["ir2s_omega"]
+    results = list(stage1)
     for kappa in KAPPAS:
-        for cf, cmin, cmax in CHECKS:
-            entry = {"omega": omega_of(n, kappa), "check_first": cf, "check_min": cmin, "check_max": cmax}
-            plan = RaggedPlan(inst["rows"], device=dev, tuning={str(n): entry})
-            res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=tol)  # warm-up, and correctness
-            ok = int((res.status != 0).sum()) == 0
-            times = []
-            for _ in range(reps):
-                plan.solve(inst["g"], inst["r"], inst["vin"], tol=tol, timing=True)
-                times.append(plan.last_ms())
-            ms = min(times)
-            sw = float(res.sweeps.double().mean())
-            results.append((ms if ok else math.inf, kappa, entry, sw))
-            log(f"  n={n} kappa={kappa:.2f} w={entry['omega']:.5f} checks={cf}/{cmin}/{cmax}: "
-                f"{ms:.3f} ms, mean sweeps {sw:.0f}{'' if ok else ' (not converged)'}")
-            del plan
+        for checks in CHECKS:
+            if kappa == 1.0 and checks == CHECKS[0]:
+                continue
+            results.append(trial(kappa, make(kappa, checks, k_best, w_best)))
     best = min(results, key=lambda x: x[0])
     out = dict(best[2])
     out.update({"kappa": best[1], "ms": best[0], "mean_sweeps": best[3], "sample": count})
@@ -95,7 +114,7 @@ def main():
         classes[str(n)] = tune_size(n, c, args.tol, dev, log)
         log(f"size {n}: best {classes[str(n)]}")
     table = {"arch": arch_name(dev), "device": torch.cuda.get_device_name(dev), "tol": args.tol,
-             "seconds": time.time() - t0, "candidates": {"kappa": KAPPAS, "checks": CHECKS}, "classes": classes}
+             "seconds": time.time() - t0, "candidates": {"kappa": KAPPAS, "checks": CHECKS, "ir2s_iters": IR2S, "ir2s_omega": IR2S_OMEGA}, "classes": classes}
     out = args.out or table_path(dev)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     with open(out, "w") as f:

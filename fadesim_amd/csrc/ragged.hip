@@ -288,7 +288,23 @@ extern "C" int fadesim_ragged_plan_create(fadesim_ragged_plan **out, int count, 
         C.check_first = 32;
         C.check_min = 16;
         C.check_max = 256;
+        C.ir2s_iters = 0;
+        C.corr_iters = 2;
+        C.ir2s_omega = 1.0;
         if (opts) {
+            if (opts[c].ir2s_iters < 0 || opts[c].ir2s_iters > FADESIM_MAX_IR2S_ITERS ||
+                opts[c].corr_iters < 0 || opts[c].corr_iters > FADESIM_MAX_IR2S_ITERS ||
+                !(opts[c].ir2s_omega >= 0.0 && opts[c].ir2s_omega <= 1.0)) {
+                delete P;
+                return set_error(FADESIM_E_INVALID, "ir2s_iters and corr_iters must lie in [0, " +
+                                                        std::to_string(FADESIM_MAX_IR2S_ITERS) +
+                                                        "], ir2s_omega in (0, 1] (0 selects 1)");
+            }
+            C.ir2s_iters = opts[c].ir2s_iters;
+            if (opts[c].corr_iters > 0)
+                C.corr_iters = opts[c].corr_iters;
+            if (opts[c].ir2s_omega > 0.0)
+                C.ir2s_omega = opts[c].ir2s_omega;
             if (opts[c].omega > 0) {
                 if (!(opts[c].omega < 2.0)) {
                     delete P;
@@ -460,6 +476,9 @@ extern "C" int fadesim_ragged_plan_info(const fadesim_ragged_plan *P, fadesim_ra
         st->check_first[c] = P->ctl[c].check_first;
         st->check_min[c] = P->ctl[c].check_min;
         st->check_max[c] = P->ctl[c].check_max;
+        st->ir2s_iters[c] = P->ctl[c].ir2s_iters;
+        st->corr_iters[c] = P->ctl[c].corr_iters;
+        st->ir2s_omega[c] = P->ctl[c].ir2s_omega;
     }
     return FADESIM_OK;
 }
@@ -519,6 +538,10 @@ extern "C" int fadesim_ragged_solve(const fadesim_ragged_plan *P, const fadesim_
         A.ctl[c].switch_tol = d->switch_tol > 0 ? d->switch_tol : 2e-5;
         A.ctl[c].max_sweeps = d->max_sweeps;
         A.ctl[c].flags = d->flags;
+        if ((d->flags & FADESIM_F_TEST_FIRST) && A.ctl[c].ir2s_iters > 0)
+            A.ctl[c].check_first = 0; // the IR2S guess is tested before the first sweep
+        else
+            A.ctl[c].flags &= ~FADESIM_F_TEST_FIRST;
     }
     // queues, abort flag, team broadcast slots and per-task counters start at 0
     // (a fill kernel, not a memset: the call may be captured into a graph)

@@ -41,8 +41,15 @@
 // FP32 sweeps stall near 1e-6 relative error, so once the estimate falls
 // below switch_tol the solve takes one iterative-refinement step: the exact
 // FP64 residual becomes the right-hand side of a correction solve, whose
-// initial guess is the second-order wire Green's-function estimate of the
-// correction (green_guess) rather than zero.
+// initial guess is corr_iters wire Green's-function (IR2S) iterations on the
+// correction (green_guess) rather than zero.  A solve that meets the
+// tolerance at a test stops there, refined or not.
+//
+// Initial guess: zero drops or the caller's field, then ir2s_iters IR2S
+// iterations (ir2s_guess: every wire solved exactly for the present cell
+// currents, optionally damped).  The iteration contracts by ~0.8 r gbar n m
+// per step, so a few of them bring a 128x128 crossbar to the tolerance before
+// the first sweep.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -69,6 +76,9 @@ struct Ctl {
     unsigned flags;
     int check_first;   // sweeps before the first test
     int check_min, check_max; // bounds of the predicted test interval
+    int ir2s_iters;    // wire-solve (IR2S) iterations applied to the initial guess
+    int corr_iters;    // IR2S iterations of the refinement correction's guess (>= 1)
+    double ir2s_omega; // damping of the IR2S iterations, (0, 1]
 };
 
 // One crossbar.  rows is the full row count (all strips).
@@ -768,21 +778,81 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     return (double)em;
 }
 
+// Wire-solve (IR2S) iterations on the drops, in place on yp, yq:
+//   s_ij = g_ij (c_ij - p_ij - q_ij)            (cell currents of the drops)
+//   p_ij <- r sum_j' (min(j,j')+1) s_ij'       (wordline: source segment + wires)
+//   q_ij <- r sum_i' (n - max(i,i')) s_i'j     (bitline: wires + sink segment)
+// i.e. every line solved exactly for the present cell currents, both layers
+// from the same currents: the iteration of ir_drop_fastSolve_cpu_singleMat
+// (crossbarHspice/ir_drop_fast_solver.h:24-80) and of ir2s
+// (pimtorch/pimfixedpoint/fixedPoint/SplitArrCuda/split_arr_matmul.cu:336-545)
+// in drop variables, uniform wire resistance.  In operator form x <- M^-1 (b -
+// K x) with M the wire Laplacians and K = [G G; G G] the cell coupling; the
+// eigenvalues of M^-1 K lie in [0, rho], rho ~ 0.8 r gbar n m (0.03 at
+// 128x128, r = 3 ohm, gbar ~ 7e-7 S; 0.15-0.8 at 512x512), so the iteration
+// contracts by rho, and the damped one, x <- (1-w) x + w M^-1 (b - K x), by
+// max(|1-w|, |1-w(1+rho)|) -- rho/(2+rho) at w = 2/(2+rho).  Returns false
+// when the team aborted.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool ir2s_guess(const Inst &I, const StripCtx &X, const float *gp, int gi0, int i0,
+                                           int j0, int t, int tx, int ty, double rwire, const Shared &sh, int &nbar,
+                                           int iters, double w, float (&yp)[G::TR][G::TC],
+                                           float (&yq)[G::TR][G::TC])
+{
+    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    const double w1 = 1.0 - w;
+    float gt[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
+#pragma unroll 1
+    for (int k = 0; k < iters; ++k) {
+        auto cur = [&](int ii, int jj) {
+            const double c = I.vin[gi0 + i0 + ii] - I.vout[j0 + jj];
+            return (double)gt[ii][jj] * ((c - (double)yp[ii][jj]) - (double)yq[ii][jj]);
+        };
+        float np[TR][TC], nq[TR][TC];
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            double v[TC], o[TC];
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                v[jj] = cur(ii, jj);
+            row_green_row<G>(v, j0, tx, rwire, o);
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                np[ii][jj] = (float)(w1 * (double)yp[ii][jj] + w * o[jj]);
+        }
+        if (!col_green_tile<G, STRIP>(sh, X, cur, rwire, I.rows, gi0, i0, j0, ty, t, nbar, nq))
+            return false;
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj) {
+                yp[ii][jj] = np[ii][jj];
+                yq[ii][jj] = (float)(w1 * (double)yq[ii][jj] + w * (double)nq[ii][jj]);
+            }
+    }
+    return true;
+}
+
 // Initial guess of the refinement correction y (A y = residual), into yp, yq:
-// the second-order superposition of the wire Green's functions,
-//   y1 = (aL)^-1 res,  y2 = y1 - (aL)^-1 G (y1p + y1q)
+// `iters` wire-solve iterations from zero, damped by w as in ir2s_guess,
+//   y1 = (aL)^-1 res,  y_{k+1} = (1-w) y_k + w (y1 - (aL)^-1 G (y_k,p + y_k,q))
 // with (aL)^-1 the wordline (r (min(j,j')+1)) and bitline (r (n - max(i,i')))
-// line inverses; the cell coupling G is ~1e-6 of the wire conductance, so y2
-// is within a few 1e-3 of y and the correction converges in a few sweeps
-// instead of restarting from zero.  Reads the scaled residual (planes 1, 2).
+// line inverses; the cell coupling G is ~1e-6 of the wire conductance, so
+// two iterations already put y within a few 1e-3 of the correction and the
+// correction converges in a few sweeps instead of restarting from zero.
+// Reads the scaled residual (planes 1, 2).
 template <class G, bool STRIP>
 __device__ __forceinline__ bool green_guess(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
                                             int i0, int j0, int t, int tx, int ty, double rwire, double hcoef,
-                                            const Shared &sh, int &nbar, float (&yp)[G::TR][G::TC],
-                                            float (&yq)[G::TR][G::TC])
+                                            const Shared &sh, int &nbar, int iters, double w,
+                                            float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC])
 {
     constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
     const double ih = 1.0 / hcoef;
+    const double w1 = 1.0 - w;
     float y1p[TR][TC], y1q[TR][TC];
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
@@ -808,28 +878,41 @@ __device__ __forceinline__ bool green_guess(const Inst &I, const StripCtx &X, fl
     }
     float gt[TR][TC];
 #pragma unroll
-    for (int ii = 0; ii < TR; ++ii)
-        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
-    auto coupling = [&](int ii, int jj) { return (double)gt[ii][jj] * ((double)y1p[ii][jj] + (double)y1q[ii][jj]); };
-#pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
-        double v[TC], o[TC];
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
 #pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            v[jj] = coupling(ii, jj);
-        row_green_row<G>(v, j0, tx, rwire, o);
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            yp[ii][jj] = y1p[ii][jj] - (float)o[jj];
+        for (int jj = 0; jj < TC; ++jj) {
+            yp[ii][jj] = (float)(w * (double)y1p[ii][jj]);
+            yq[ii][jj] = (float)(w * (double)y1q[ii][jj]);
+        }
     }
-    float cq[TR][TC];
-    if (!col_green_tile<G, STRIP>(sh, X, coupling, rwire, I.rows, gi0, i0, j0, ty, t, nbar, cq))
-        return false;
+#pragma unroll 1
+    for (int k = 1; k < iters; ++k) {
+        auto coupling = [&](int ii, int jj) {
+            return (double)gt[ii][jj] * ((double)yp[ii][jj] + (double)yq[ii][jj]);
+        };
+        float np[TR][TC], cq[TR][TC];
 #pragma unroll
-    for (int ii = 0; ii < TR; ++ii)
+        for (int ii = 0; ii < TR; ++ii) {
+            double v[TC], o[TC];
 #pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            yq[ii][jj] = y1q[ii][jj] - cq[ii][jj];
+            for (int jj = 0; jj < TC; ++jj)
+                v[jj] = coupling(ii, jj);
+            row_green_row<G>(v, j0, tx, rwire, o);
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                np[ii][jj] = (float)(w1 * (double)yp[ii][jj] + w * ((double)y1p[ii][jj] - o[jj]));
+        }
+        if (!col_green_tile<G, STRIP>(sh, X, coupling, rwire, I.rows, gi0, i0, j0, ty, t, nbar, cq))
+            return false;
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj) {
+                yp[ii][jj] = np[ii][jj];
+                yq[ii][jj] = (float)(w1 * (double)yq[ii][jj] + w * ((double)y1q[ii][jj] - (double)cq[ii][jj]));
+            }
+    }
     return true;
 }
 
@@ -882,6 +965,10 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 yp[ii][jj] = yq[ii][jj] = 0.f;
         }
     }
+    if (C.ir2s_iters > 0 &&
+        !ir2s_guess<G, STRIP>(I, X, gp, gi0, i0, j0, t, tx, ty, rwire, sh, nbar, C.ir2s_iters, C.ir2s_omega, yp,
+                              yq))
+        return false;
     publish_all<G>(yp, yq, halo, t);
     if (STRIP)
         strip_publish<G>(X, yq, ty, j0, lane, pub++);
@@ -922,12 +1009,17 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                                   has_r, has_u, has_d, sh, pub - 1, nbar);
         if (est < 0.0)
             return false;
-        bool reload = true, done = false;
+        bool reload = true, done = false, just_refined = false;
         if (fixed) {
             st = FADESIM_ST_CONVERGED;
             done = true;
         } else if (!(est < INFINITY)) {
             st = FADESIM_ST_NONFINITE;
+            done = true;
+        } else if (est <= stop_tol) {
+            // certified with or without the refinement step: the estimate is
+            // of the FP64 readout of the present field either way
+            st = FADESIM_ST_CONVERGED;
             done = true;
         } else if (refine_pending && est <= C.switch_tol) {
             // ---- one iterative-refinement step: rhs <- exact FP64 residual ----
@@ -951,7 +1043,8 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                     for (int jj = 0; jj < TC; ++jj)
                         yp[ii][jj] = yq[ii][jj] = 0.f;
             } else {
-                if (!green_guess<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, sh, nbar, yp, yq))
+                if (!green_guess<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, sh, nbar,
+                                           C.corr_iters, C.ir2s_omega, yp, yq))
                     return false;
             }
             if (t < N)
@@ -962,9 +1055,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             __syncthreads();
             refine_pending = false;
             refined = true;
-        } else if (!refine_pending && est <= stop_tol) {
-            st = FADESIM_ST_CONVERGED;
-            done = true;
+            just_refined = true;
         }
         if (!done && sweep >= C.max_sweeps) {
             st = FADESIM_ST_MAXIT;
@@ -981,7 +1072,9 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             break;
         const double tgt = refine_pending ? C.switch_tol : stop_tol;
         const float k = (est > tgt) ? 0.8f * __log2f((float)(tgt / est)) / lrho : (float)C.check_min;
-        const int kk = (int)fminf(fmaxf(k, (float)C.check_min), (float)C.check_max);
+        int kk = (int)fminf(fmaxf(k, (float)C.check_min), (float)C.check_max);
+        if (just_refined && (C.flags & FADESIM_F_TEST_FIRST))
+            kk = 0; // test the correction's initial guess before sweeping
         next_check = sweep + kk;
         if (next_check > C.max_sweeps)
             next_check = C.max_sweeps;

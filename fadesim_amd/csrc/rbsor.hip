@@ -187,6 +187,12 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     const double safety = d->safety > 0 ? d->safety : 2.0;
     if (safety < 1.0)
         return set_error(FADESIM_E_INVALID, "safety must be >= 1");
+    if (d->ir2s_iters < 0 || d->ir2s_iters > FADESIM_MAX_IR2S_ITERS || d->corr_iters < 0 ||
+        d->corr_iters > FADESIM_MAX_IR2S_ITERS)
+        return set_error(FADESIM_E_INVALID, "ir2s_iters and corr_iters must lie in [0, " +
+                                                std::to_string(FADESIM_MAX_IR2S_ITERS) + "]");
+    if (!(d->ir2s_omega >= 0.0 && d->ir2s_omega <= 1.0))
+        return set_error(FADESIM_E_INVALID, "ir2s_omega must lie in (0, 1] (0 selects 1)");
     hipStream_t s = (hipStream_t)stream;
 
     const double *vout = d->d_vout;
@@ -215,9 +221,12 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     args.ctl.dwr_scale = 16.0;
     args.ctl.max_sweeps = d->max_sweeps;
     args.ctl.flags = d->flags;
-    args.ctl.check_first = 32;
+    args.ctl.check_first = (d->flags & FADESIM_F_TEST_FIRST) ? 0 : 32;
     args.ctl.check_min = 16;
     args.ctl.check_max = 256;
+    args.ctl.ir2s_iters = d->ir2s_iters;
+    args.ctl.corr_iters = d->corr_iters > 0 ? d->corr_iters : 2;
+    args.ctl.ir2s_omega = d->ir2s_omega > 0.0 ? d->ir2s_omega : 1.0;
     args.iout = d->d_iout;
     args.sweeps = d->d_sweeps;
     args.status = d->d_status;

@@ -1,4 +1,4 @@
-"""Drop-in for the PimTorch array-simulation hook.
+"""Drop-in for the PimTorch array-simulation hook (``ir_drop_gs`` and ``ir2s``).
 
 PimTorch builds ``phyArrParams.cpp_ir_drop_accelerate`` at import time
 (pimtorch/pimfixedpoint/fixedPoint/nn/commonConst.py:273-280) and calls
@@ -115,9 +115,63 @@ def _converged(input, G, batch_size, row, col, numX, numY, g_wire, ret_V):
     return res.iout.view(batch_size, numX, numY * col).to(G.dtype)
 
 
+def ir2s(input: torch.Tensor, G: torch.Tensor, batch_size: int, row_size: int, col_size: int, numX: int,
+         numY: int, iter_times: int, r_wire: float, enable_break: bool, break_threshould: float,
+         nonlinear_cell_enable: bool, nonlinear_params: Sequence[float], ret_V: bool = False) -> torch.Tensor:
+    """IR2S solve of ``batch_size x numX x numY`` crossbars (the hook's
+    ``ir2s``, split_arr_matmul.cu:446-545): ``iter_times`` iterations of
+    exact wordline and bitline solves for the present cell currents, both
+    layers from the same currents, source and sink segments ``r_wire``,
+    grounded bitlines; with ``enable_break`` the loop ends after the first
+    iteration in which no node of the whole batch moved by more than
+    ``break_threshould``.  Same shapes, dtype rule and errors as ir_drop_gs;
+    the arithmetic is the reference's, operation for operation
+    (``fadesim_ir2s`` with the hook's settings)."""
+    _check_input(input, "input")
+    _check_input(G, "G")
+    if row_size < 2:
+        raise RuntimeError("row_sizeshould be >=2")
+    if col_size < 2:
+        raise RuntimeError("col_sizeshould be >=2")
+    if G.dtype not in (torch.float32, torch.float64):
+        raise RuntimeError(f"ir2s not implemented for '{G.dtype}'")
+    if input.dtype != G.dtype:
+        raise RuntimeError("expected input and G to have the same dtype")
+    if input.numel() != batch_size * numX * row_size or G.numel() != numX * row_size * numY * col_size:
+        raise RuntimeError("input/G sizes do not match batch_size, numX, numY, row_size, col_size")
+    nl = list(nonlinear_params) if nonlinear_cell_enable else []
+    if nonlinear_cell_enable and len(nl) < 6:
+        raise RuntimeError("nonlinear_params needs [Ron, Roff, a_on, b_on, a_off, b_off]")
+    dev = G.device
+    if ret_V:
+        out = torch.empty((batch_size, numX, numY, row_size, col_size), dtype=G.dtype, device=dev)
+    else:
+        out = torch.empty((batch_size, numX, numY * col_size), dtype=G.dtype, device=dev)
+    d = nat.Ir2sDesc()
+    d.dtype_bytes, d.layout = G.element_size(), nat.IR2S_HOOK
+    d.d_vin, d.d_G, d.d_vout = input.data_ptr(), G.data_ptr(), None
+    d.batch_size, d.row_size, d.col_size = int(batch_size), int(row_size), int(col_size)
+    d.numX, d.numY, d.iter_times = int(numX), int(numY), int(iter_times)
+    d.r_wire, d.r_source, d.r_sink = float(r_wire), 0.0, 0.0
+    d.divide_by_g, d.carry_residue = 0, 1
+    d.enable_break = nat.IR2S_BREAK_BATCH if enable_break else 0
+    d.break_threshold = float(break_threshould)
+    d.nonlinear = int(bool(nonlinear_cell_enable))
+    for k in range(6):
+        d.nonlinear_params[k] = float(nl[k]) if k < len(nl) else 0.0
+    d.ret_v = int(bool(ret_V))
+    d.d_out = out.data_ptr()
+    d.d_vup = d.d_vdown = d.d_iters = None
+    with torch.cuda.device(dev):
+        rc = nat.lib().fadesim_ir2s(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
+    nat.check(rc, "ir2s")
+    return out
+
+
 class _HookModule:
     """Module-like object with the reference extension's function names."""
     ir_drop_gs = staticmethod(ir_drop_gs)
+    ir2s = staticmethod(ir2s)
 
 
 def install(phyArrParams) -> None:

@@ -66,6 +66,9 @@ class RaggedPlan:
                     opts[c].check_first = int(t.get("check_first", 0))
                     opts[c].check_min = int(t.get("check_min", 0))
                     opts[c].check_max = int(t.get("check_max", 0))
+                    opts[c].ir2s_iters = int(t.get("ir2s_iters", 0))
+                    opts[c].corr_iters = int(t.get("corr_iters", 0))
+                    opts[c].ir2s_omega = float(t.get("ir2s_omega", 0.0))
         h = ctypes.c_void_p()
         L = nat.lib()
         with torch.cuda.device(self.device):
@@ -78,7 +81,9 @@ class RaggedPlan:
         nat.check(L.fadesim_ragged_plan_info(self._h, ctypes.byref(st)), "fadesim_ragged_plan_info")
         self.stats = {"grid": st.grid, "teams": st.teams, "big_tasks": st.big_tasks, "small_items": st.small_items,
                       "omega": list(st.omega), "check_first": list(st.check_first),
-                      "check_min": list(st.check_min), "check_max": list(st.check_max)}
+                      "check_min": list(st.check_min), "check_max": list(st.check_max),
+                      "ir2s_iters": list(st.ir2s_iters), "corr_iters": list(st.corr_iters),
+                      "ir2s_omega": list(st.ir2s_omega)}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -160,10 +165,12 @@ class RaggedPlan:
               warm: Optional[torch.Tensor] = None, *, tol: float = 1e-6, safety: float = 2.0,
               switch_tol: float = 2e-5, max_sweeps: int = 100000, fixed_sweeps: Optional[int] = None,
               refine: bool = True, want_field: bool = False, timing: bool = False,
-              zero_correction: bool = False,
+              zero_correction: bool = False, test_first: bool = True,
               out: Optional[RaggedResult] = None, check_abort: bool = True) -> RaggedResult:
         """Solve every crossbar of the plan; packed float32 g / warm, float64
-        r_wire [K], vin, vout (None = 0 V)."""
+        r_wire [K], vin, vout (None = 0 V).  ``test_first``: size classes whose
+        controls apply IR2S iterations to the initial guess test it before
+        the first sweep."""
         self._check_inputs(g, r_wire, vin, vout, warm)
         res = out if out is not None else self.alloc_result(want_field)
         flags = 0
@@ -176,6 +183,8 @@ class RaggedPlan:
             flags |= nat.F_TIMING
         if zero_correction:
             flags |= nat.F_ZERO_CORRECTION
+        if test_first:
+            flags |= nat.F_TEST_FIRST
         d = self._desc(g, r_wire, vin, vout, warm, tol, safety, switch_tol, max_sweeps, flags, res)
         with torch.cuda.device(self.device):
             rc = nat.lib().fadesim_ragged_solve(self._h, ctypes.byref(d),
@@ -195,7 +204,7 @@ class RaggedPlan:
         return float(nat.lib().fadesim_ragged_last_ms(self._h))
 
     def capture(self, g, r_wire, vin, vout=None, warm=None, *, tol=1e-6, safety=2.0, switch_tol=2e-5,
-                max_sweeps=100000, want_field=False):
+                max_sweeps=100000, want_field=False, test_first=True):
         """Record the solve (fill kernel + persistent kernel) into a graph.
         Returns (graph, result): ``graph.replay()`` re-solves from the current
         contents of the same input tensors into ``result``."""
@@ -203,10 +212,11 @@ class RaggedPlan:
         res = self.alloc_result(want_field)
         # one eager call first: nothing may be allocated during capture
         self.solve(g, r_wire, vin, vout, warm, tol=tol, safety=safety, switch_tol=switch_tol,
-                   max_sweeps=max_sweeps, out=res)
+                   max_sweeps=max_sweeps, out=res, test_first=test_first)
         torch.cuda.synchronize(self.device)
         graph = torch.cuda.CUDAGraph()
-        d = self._desc(g, r_wire, vin, vout, warm, tol, safety, switch_tol, max_sweeps, 0, res)
+        d = self._desc(g, r_wire, vin, vout, warm, tol, safety, switch_tol, max_sweeps,
+                       nat.F_TEST_FIRST if test_first else 0, res)
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         with torch.cuda.stream(s):

@@ -53,7 +53,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
                 omega: Optional[float] = None,
                 tol: float = 1e-6, safety: float = 2.0, switch_tol: float = 2e-5,
                 max_sweeps: int = 20000, refine: bool = True, dwr: bool = False,
-                zero_correction: bool = False,
+                zero_correction: bool = False, ir2s_iters: int = 0, corr_iters: int = 0,
+                ir2s_omega: float = 1.0, test_first: Optional[bool] = None,
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
                 out: Optional[torch.Tensor] = None, timing: bool = False) -> SolveResult:
     """Solve B crossbars of one shape.
@@ -64,6 +65,13 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     ``fixed_sweeps=k`` runs exactly k sweeps with no test and no refinement;
     ``zero_correction`` starts the refinement correction from zero instead of
     the wire Green's-function estimate of it.
+    ``ir2s_iters=k`` applies k wire-solve (IR2S) iterations to the initial
+    guess inside the kernel -- each solves every wordline and bitline exactly
+    for the cell currents of the present drops, the iteration of the
+    reference's ir2s (crossbarHspice/ir_drop_fast_solver.h:24-80) -- and
+    ``corr_iters`` sets the iterations of the refinement correction's guess
+    (0: 2), ``ir2s_omega`` their damping.  ``test_first`` tests the initial guess before the first sweep
+    (default: when ir2s_iters >= 3).
     """
     if g.dim() != 3:
         raise ValueError("g must be [B, M, N]")
@@ -107,6 +115,12 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
         flags |= nat.F_ZERO_CORRECTION
     if timing:
         flags |= nat.F_TIMING
+    if test_first is None:
+        test_first = ir2s_iters >= 3
+    if test_first:
+        flags |= nat.F_TEST_FIRST
+    if not (0 <= int(ir2s_iters) <= nat.MAX_IR2S_ITERS and 0 <= int(corr_iters) <= nat.MAX_IR2S_ITERS):
+        raise ValueError(f"ir2s_iters and corr_iters must lie in [0, {nat.MAX_IR2S_ITERS}]")
     d = nat.BatchDesc()
     d.batch, d.rows, d.cols = B, M, N
     d.d_g, d.d_r_wire, d.d_vin = g.data_ptr(), r_wire.data_ptr(), vin.data_ptr()
@@ -117,6 +131,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     d.max_sweeps, d.flags = int(max_sweeps), int(flags)
     d.d_iout, d.d_sweeps, d.d_status, d.d_est = iout.data_ptr(), sweeps.data_ptr(), status.data_ptr(), est.data_ptr()
     d.d_field = _ptr(field)
+    d.ir2s_iters, d.corr_iters, d.ir2s_omega = int(ir2s_iters), int(corr_iters), float(ir2s_omega)
     with torch.cuda.device(dev):
         rc = nat.lib().fadesim_solve_batch(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
     nat.check(rc, "fadesim_solve_batch")

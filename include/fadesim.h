@@ -38,7 +38,10 @@ extern "C" {
 #define FADESIM_F_DWR 4u          /* return I - (adjoint-weighted residual) and stop on it */
 #define FADESIM_F_TIMING 8u       /* record HIP events around the solve kernel */
 #define FADESIM_F_ZERO_CORRECTION 16u /* start the refinement correction from zero instead of
-                                        * the second-order wire Green's-function estimate */
+                                        * the wire Green's-function (IR2S) estimate */
+#define FADESIM_F_TEST_FIRST 32u  /* test the initial guess (and the refinement correction's
+                                   * initial guess) before sweeping: for accurate guesses */
+#define FADESIM_MAX_IR2S_ITERS 64
 
 const char *fadesim_last_error(void);
 int fadesim_version(void);
@@ -76,6 +79,16 @@ typedef struct fadesim_batch_desc {
     int *d_status;          /* [B] FADESIM_ST_*, or NULL */
     double *d_est;          /* [B] final relative error estimate, or NULL */
     float *d_field;         /* [B][2][n][m] final drops, or NULL */
+    int ir2s_iters;         /* wire-solve (IR2S) iterations applied to the initial guess
+                             * before the first sweep (0..FADESIM_MAX_IR2S_ITERS); each solves
+                             * every wordline and bitline exactly for the cell currents of the
+                             * present drops (crossbarHspice/ir_drop_fast_solver.h:24-80) */
+    int corr_iters;         /* IR2S iterations of the refinement correction's initial
+                             * guess (0 -> 2) */
+    double ir2s_omega;      /* damping w of the IR2S iterations, x <- (1-w) x + w x_IR2S,
+                             * in (0, 1]; 0 -> 1.  With the coupling's spectral radius rho
+                             * (~0.8 r gbar rows cols) the contraction is rho/(2+rho) at
+                             * w = 2/(2+rho), against rho undamped */
 } fadesim_batch_desc;
 
 int fadesim_solve_batch(const fadesim_batch_desc *desc, void *stream);
@@ -129,6 +142,9 @@ typedef struct fadesim_ragged_class_opts {
     int check_first;  /* sweeps before the first convergence test */
     int check_min;    /* bounds of the predicted interval between tests */
     int check_max;
+    int ir2s_iters;   /* IR2S iterations on the initial guess (0 = none) */
+    int corr_iters;   /* IR2S iterations of the refinement correction's guess (0 = built-in, 2) */
+    double ir2s_omega; /* damping of the IR2S iterations in (0, 1] (0 = built-in, 1) */
 } fadesim_ragged_class_opts;
 
 typedef struct fadesim_ragged_plan_stats {
@@ -137,6 +153,8 @@ typedef struct fadesim_ragged_plan_stats {
     int grid, teams, big_tasks, small_items;
     double omega[4];
     int check_first[4], check_min[4], check_max[4];
+    int ir2s_iters[4], corr_iters[4];
+    double ir2s_omega[4];
 } fadesim_ragged_plan_stats;
 
 typedef struct fadesim_ragged_desc {
@@ -231,6 +249,53 @@ typedef struct fadesim_hook_desc {
 } fadesim_hook_desc;
 
 int fadesim_hook_ir_drop_gs(const fadesim_hook_desc *desc, void *stream);
+
+/*
+ * IR2S with the reference's own semantics: every wordline and bitline solved
+ * exactly for the cell currents of the present state, both layers from the
+ * same currents, initial state u = vin, d = 0; a fixed iter_times, or a break
+ * rule; sinh-nonlinear cells by the lagged-conductance update.  Replaces
+ *   ir2s (PimTorch hook)  pimtorch/pimfixedpoint/fixedPoint/SplitArrCuda/
+ *                         split_arr_matmul.cu:336-545 (pybind11 entry :1432)
+ *   ir_drop_fastSolve_cpu_singleMat  crossbarHspice/ir_drop_fast_solver.h:24-163
+ *                         behind the V1 command `fastsolve` (2dArray.cc:318-327, 1691-1900)
+ * The hook's arithmetic: layout FADESIM_IR2S_HOOK, divide_by_g 0,
+ * carry_residue 1, r_source = r_sink = r_wire, vout NULL, break
+ * FADESIM_IR2S_BREAK_BATCH.  The CPU solver's: layout FADESIM_IR2S_PLAIN,
+ * divide_by_g 1, carry_residue 0, break FADESIM_IR2S_BREAK_EACH.
+ */
+#define FADESIM_IR2S_HOOK 0   /* vin [batch][numX][row], G [numX][row][numY*col] shared by the batch */
+#define FADESIM_IR2S_PLAIN 1  /* vin [batch][row], G [batch][row][col]; numX = numY = 1 */
+#define FADESIM_IR2S_BREAK_BATCH 1 /* stop after the first iteration in which no node of the
+                                    * whole call moved by more than break_threshold */
+#define FADESIM_IR2S_BREAK_EACH 2  /* per crossbar: stop after the first iteration whose largest
+                                    * move is below break_threshold */
+typedef struct fadesim_ir2s_desc {
+    int dtype_bytes;        /* 4 (float) or 8 (double): vin, G, vout, out and the fields */
+    int layout;             /* FADESIM_IR2S_HOOK or FADESIM_IR2S_PLAIN */
+    const void *d_vin;
+    const void *d_G;
+    const void *d_vout;     /* [crossbars][col] bitline sink voltages, or NULL = 0 V */
+    int batch_size, row_size, col_size, numX, numY; /* crossbars = batch_size * numX * numY */
+    int iter_times;
+    double r_wire;          /* inner wire segments (ohm) */
+    double r_source, r_sink; /* first wordline / last bitline segment; <= 0 -> r_wire */
+    int divide_by_g;        /* inner drop as I / (1/r) (CPU solver) instead of I * r (hook) */
+    int carry_residue;      /* keep each line's rounding residue of the running current sum
+                             * across iterations (the hook's never-reset I_row_sum) */
+    int enable_break;       /* 0, FADESIM_IR2S_BREAK_BATCH or FADESIM_IR2S_BREAK_EACH */
+    double break_threshold; /* volts */
+    int nonlinear;
+    double nonlinear_params[6]; /* Ron, Roff, a_on, b_on, a_off, b_off */
+    int ret_v;              /* out: [crossbars][row][col] u - d instead of currents [crossbars][col] */
+    void *d_out;
+    void *d_vup, *d_vdown;  /* optional [crossbars][row][col] final node voltages, or NULL */
+    int *d_iters;           /* optional (device): BREAK_EACH: [crossbars] counts as the CPU solver
+                             * returns them (break sweep + 1, or iter_times + 1 when never met);
+                             * otherwise one int: iterations run */
+} fadesim_ir2s_desc;
+
+int fadesim_ir2s(const fadesim_ir2s_desc *desc, void *stream);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,12 @@ def lib():
         L.oracle_rb_f64.restype = i
         L.oracle_rb_f32.argtypes = [i, i, _fp, d, _dp, _dp, d, i, _fp, _fp]
         L.oracle_rb_f32.restype = None
+        L.oracle_ir2s.argtypes = [i, i, _dp, _dp, _dp, d, d, i, i, d, _dp, _dp, _dp]
+        L.oracle_ir2s.restype = i
+        for name, fp in (("oracle_hook_ir2s_f64", _dp), ("oracle_hook_ir2s_f32", _fp)):
+            f = getattr(L, name)
+            f.argtypes = [fp, fp, i, i, i, i, i, i, d, i, d, i, _dp, i, fp]
+            f.restype = i
         for name, fp in (("oracle_hook_gs_f64", _dp), ("oracle_hook_gs_f32", _fp)):
             f = getattr(L, name)
             f.argtypes = [fp, fp, i, i, i, i, i, i, d, d, i, d, i, _dp, i, fp]
@@ -233,3 +239,40 @@ def hook_gs(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, g_wi
                float(g_wire), float(beta), int(bool(enable_break)), float(break_threshold),
                int(bool(nonlinear_cell_enable)), _pd(nl), int(bool(ret_V)), pp(out))
     return out, sweeps
+
+
+def ir2s(g, vin, vout, g_wire, r_load, times, enable_break=False, break_th=1e-9, vup=None, vdown=None):
+    """IR2S wire-solve iteration, voltage form (ir_drop_fast_solver.h:24-163).
+    Starts from vup = vin (broadcast), vdown = 0 unless given.  Returns
+    (vup, vdown, iout, count) with the reference's iteration count."""
+    g = _d(g)
+    n, m = g.shape
+    vin, vout = _d(vin).reshape(n), _d(vout).reshape(m)
+    vu = np.repeat(vin[:, None], m, 1).copy() if vup is None else _d(vup).copy()
+    vd = np.zeros((n, m)) if vdown is None else _d(vdown).copy()
+    out = np.empty(m)
+    cnt = lib().oracle_ir2s(n, m, _pd(g), _pd(vin), _pd(vout), float(g_wire), float(r_load), int(times),
+                            int(bool(enable_break)), float(break_th), _pd(vu), _pd(vd), _pd(out))
+    if cnt < 0:
+        raise MemoryError("oracle_ir2s")
+    return vu, vd, out, cnt
+
+
+def hook_ir2s(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, r_wire, enable_break,
+              break_threshould, nonlinear_cell_enable, nonlinear_params, ret_V=False):
+    """The hook's ir2s (split_arr_matmul.cu:336-545) on the CPU, in the dtype
+    of G (float32 or float64).  Returns (out, iterations)."""
+    f32 = np.asarray(G).dtype == np.float32
+    cv = _f if f32 else _d
+    inp, G = cv(inp), cv(G)
+    nl = _d(list(nonlinear_params) + [0.0] * (6 - len(nonlinear_params)) if nonlinear_cell_enable else np.zeros(6))
+    if ret_V:
+        out = np.empty((batch_size, numX, numY, row_size, col_size), dtype=G.dtype)
+    else:
+        out = np.empty((batch_size, numX, numY * col_size), dtype=G.dtype)
+    fn = lib().oracle_hook_ir2s_f32 if f32 else lib().oracle_hook_ir2s_f64
+    pp = _pf if f32 else _pd
+    it = fn(pp(inp), pp(G), int(batch_size), int(row_size), int(col_size), int(numX), int(numY), int(iter_times),
+            float(r_wire), int(bool(enable_break)), float(break_threshould), int(bool(nonlinear_cell_enable)),
+            _pd(nl), int(bool(ret_V)), pp(out))
+    return out, it

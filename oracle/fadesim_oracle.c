@@ -27,6 +27,12 @@
  *   oracle_hook_gs_*   the PimTorch hook's line-GS ordering
  *                      (pimtorch/.../SplitArrCuda/split_arr_matmul.cu:27-140,204-333),
  *                      linear and sinh-nonlinear cells, FP32 and FP64.
+ *   oracle_ir2s        the IR2S wire-solve iteration, a statement-by-statement
+ *                      restatement of ir_drop_fastSolve_cpu_singleMat
+ *                      (crossbarHspice/ir_drop_fast_solver.h:24-163), behind
+ *                      the V1 `fastsolve` command (2dArray.cc:318-327,
+ *                      1691-1900).  Pinned bitwise against golden vectors of
+ *                      the reference `sim_v1` binary (tests/golden/).
  *
  * Build: oracle/Makefile (gcc, -ffp-contract=off so that no multiply-add is
  * fused: the reference binary is built for baseline x86-64 which has no FMA).
@@ -741,4 +747,199 @@ int oracle_nodegs_batch_f32g(int B, int n, int m, const float *g32, const double
         free(d);
     }
     return nthreads;
+}
+
+
+/* ---------------------------------------------------------------- IR2S
+ * One sweep: cell currents of the present voltages, then every wordline and
+ * every bitline solved exactly for those currents (the source / sink segment
+ * r_load, inner segments 1/g_wire): wordline i carries, in segment j, the sum
+ * of the cell currents at columns >= j; bitline j carries, in segment i (below
+ * row i), the sum of the currents at rows <= i.  Accumulation orders and the
+ * division by g_wire follow ir_drop_fast_solver.h:24-80 (no break) and
+ * :81-163 (break when the largest voltage change of a sweep is < break_th).
+ * vup / vdown hold the starting voltages on entry.  Returns the reference's
+ * count: `times` without the break, else (0-based break sweep or times) + 1.
+ */
+int oracle_ir2s(int n, int m, const double *g, const double *vin, const double *vout, double g_wire,
+                double r_load, int times, int enable_break, double break_th, double *vup, double *vdown,
+                double *out)
+{
+    double *cur = (double *)malloc(sizeof(double) * (size_t)n * m);
+    double *wl = (double *)malloc(sizeof(double) * (size_t)n);
+    double *bl = (double *)malloc(sizeof(double) * (size_t)m);
+    if (!cur || !wl || !bl) {
+        free(cur); free(wl); free(bl);
+        return -1;
+    }
+    int record = times;
+    for (int k = 0; k < times; ++k) {
+        double dmax = 0.0;
+        for (int i = 0; i < n; ++i) wl[i] = 0.0;
+        for (int j = 0; j < m; ++j) bl[j] = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < m; ++j) {
+                const double c = (vup[i * m + j] - vdown[i * m + j]) * g[i * m + j];
+                cur[i * m + j] = c;
+                wl[i] += c;
+                bl[j] += c;
+            }
+        for (int i = 0; i < n; ++i) {
+            double carried = wl[i], v = vin[i] - carried * r_load;
+            dmax = fmax(dmax, fabs(v - vup[i * m]));
+            vup[i * m] = v;
+            carried -= cur[i * m];
+            for (int j = 1; j < m; ++j) {
+                v = vup[i * m + j - 1] - carried / g_wire;
+                dmax = fmax(dmax, fabs(v - vup[i * m + j]));
+                vup[i * m + j] = v;
+                carried -= cur[i * m + j];
+            }
+        }
+        for (int j = 0; j < m; ++j) {
+            const double v = bl[j] * r_load + vout[j];
+            dmax = fmax(dmax, fabs(v - vdown[(n - 1) * m + j]));
+            vdown[(n - 1) * m + j] = v;
+            bl[j] -= cur[(n - 1) * m + j];
+        }
+        for (int i = n - 2; i >= 0; --i)
+            for (int j = 0; j < m; ++j) {
+                const double v = vdown[(i + 1) * m + j] + bl[j] / g_wire;
+                dmax = fmax(dmax, fabs(v - vdown[i * m + j]));
+                vdown[i * m + j] = v;
+                bl[j] -= cur[i * m + j];
+            }
+        if (enable_break && dmax < break_th) {
+            record = k;
+            break;
+        }
+    }
+    for (int j = 0; j < m; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i)
+            s += (vup[i * m + j] - vdown[i * m + j]) * g[i * m + j];
+        out[j] = s;
+    }
+    free(cur); free(wl); free(bl);
+    return enable_break ? record + 1 : times;
+}
+
+/*
+ * The PimTorch hook's ir2s (split_arr_matmul.cu:336-545): the same grid,
+ * layout and nonlinear lag as oracle_hook_gs_*; per iteration the wordline
+ * kernel (:336-375) and the bitline kernel (:377-414) both read the newest
+ * state ("prev") and write the other buffer, each line's running current
+ * sum (I_row_sum / I_col_sum) persisting across iterations with its rounding
+ * residue; inner segments multiply by r_wire, source and sink segments too.
+ * Break: after an iteration, stop when no node of the whole grid moved by
+ * more than thr (:517-527).  Returns the number of iterations run.
+ */
+#define HOOK_IR2S_BODY(T, SINH, FABS)                                                  \
+    const size_t cells = (size_t)row * col;                                            \
+    const size_t nxb = (size_t)bsize * numX * numY;                                    \
+    T *ua = (T *)malloc(nxb * cells * sizeof(T)), *ub = (T *)malloc(nxb * cells * sizeof(T)); \
+    T *da = (T *)calloc(nxb * cells, sizeof(T)), *db = (T *)calloc(nxb * cells, sizeof(T));  \
+    T *Gk = (T *)malloc(nxb * cells * sizeof(T));                                      \
+    T *wsum = (T *)calloc(nxb * row, sizeof(T)), *bsum = (T *)calloc(nxb * col, sizeof(T)); \
+    const T rw = (T)r_wire;                                                            \
+    for (size_t xb = 0; xb < nxb; ++xb) {                                              \
+        const int b = (int)(xb / ((size_t)numX * numY)), x = (int)((xb / numY) % numX), y = (int)(xb % numY); \
+        for (int i = 0; i < row; ++i)                                                  \
+            for (int c = 0; c < col; ++c) {                                            \
+                const size_t k = xb * cells + (size_t)i * col + c;                     \
+                ua[k] = ub[k] = input[((size_t)b * numX + x) * row + i];               \
+                Gk[k] = G[((size_t)x * row + i) * (size_t)(numY * col) + (size_t)y * col + c]; \
+            }                                                                          \
+    }                                                                                  \
+    T *Glin = NULL;                                                                    \
+    if (nonlinear) {                                                                   \
+        Glin = (T *)malloc(nxb * cells * sizeof(T));                                   \
+        memcpy(Glin, Gk, nxb * cells * sizeof(T));                                     \
+    }                                                                                  \
+    T *ucur = ua, *uprev = ub, *dcur = da, *dprev = db;                                \
+    int it;                                                                            \
+    for (it = 0; it < iter_times; ++it) {                                              \
+        if (nonlinear)                                                                 \
+            for (size_t k = 0; k < nxb * cells; ++k) {                                 \
+                T V = ucur[k] - dcur[k];                                               \
+                if (V < (T)1e-5 && V > -(T)1e-5)                                       \
+                    V = V > 0 ? (T)1e-5 : -(T)1e-5;                                    \
+                V = FABS(V);                                                           \
+                T zz = ((T)1 / Glin[k] - (T)nl[0]) / (T)(nl[1] - nl[0]);               \
+                if (zz < 0) zz = 0;                                                    \
+                T R = ((T)1 - zz) * V / ((T)nl[3] * SINH(V * (T)nl[2])) +              \
+                      zz * V / ((T)nl[5] * SINH(V * (T)nl[4]));                        \
+                Gk[k] = (T)1 / R;                                                      \
+            }                                                                          \
+        for (size_t xb = 0; xb < nxb; ++xb) {                                          \
+            const int b = (int)(xb / ((size_t)numX * numY)), x = (int)((xb / numY) % numX); \
+            const size_t o = xb * cells;                                               \
+            for (int i = 0; i < row; ++i) {                                            \
+                T *s = wsum + xb * row + i;                                            \
+                const size_t ri = o + (size_t)i * col;                                 \
+                for (int j = 0; j < col; ++j)                                          \
+                    *s += (uprev[ri + j] - dprev[ri + j]) * Gk[ri + j];                \
+                ucur[ri] = input[((size_t)b * numX + x) * row + i] - *s * rw;          \
+                *s -= (uprev[ri] - dprev[ri]) * Gk[ri];                                \
+                for (int j = 1; j < col; ++j) {                                        \
+                    ucur[ri + j] = ucur[ri + j - 1] - *s * rw;                         \
+                    *s -= (uprev[ri + j] - dprev[ri + j]) * Gk[ri + j];                \
+                }                                                                      \
+            }                                                                          \
+            for (int c = 0; c < col; ++c) {                                            \
+                T *s = bsum + xb * col + c;                                            \
+                for (int i = 0; i < row; ++i) {                                        \
+                    const size_t k = o + (size_t)i * col + c;                          \
+                    *s += (uprev[k] - dprev[k]) * Gk[k];                               \
+                }                                                                      \
+                size_t k = o + (size_t)(row - 1) * col + c;                            \
+                dcur[k] = *s * rw;                                                     \
+                *s -= (uprev[k] - dprev[k]) * Gk[k];                                   \
+                for (int i = row - 2; i >= 0; --i) {                                   \
+                    k = o + (size_t)i * col + c;                                       \
+                    dcur[k] = dcur[k + col] + *s * rw;                                 \
+                    *s -= (uprev[k] - dprev[k]) * Gk[k];                               \
+                }                                                                      \
+            }                                                                          \
+        }                                                                              \
+        T *t = ucur; ucur = uprev; uprev = t;                                          \
+        t = dcur; dcur = dprev; dprev = t;                                             \
+        if (enable_break) {                                                            \
+            long cnt = 0;                                                              \
+            for (size_t k = 0; k < nxb * cells; ++k) {                                 \
+                cnt += FABS(ucur[k] - uprev[k]) > (T)thr;                              \
+                cnt += FABS(dcur[k] - dprev[k]) > (T)thr;                              \
+            }                                                                          \
+            if (cnt == 0) { ++it; break; }                                             \
+        }                                                                              \
+    }                                                                                  \
+    if (ret_v) {                                                                       \
+        for (size_t k = 0; k < nxb * cells; ++k)                                       \
+            out[k] = uprev[k] - dprev[k];                                              \
+    } else {                                                                           \
+        for (size_t xb = 0; xb < nxb; ++xb)                                            \
+            for (int c = 0; c < col; ++c) {                                            \
+                T s = 0;                                                               \
+                for (int i = 0; i < row; ++i) {                                        \
+                    const size_t k = xb * cells + (size_t)i * col + c;                 \
+                    s += (uprev[k] - dprev[k]) * Gk[k];                                \
+                }                                                                      \
+                out[xb * col + c] = s;                                                 \
+            }                                                                          \
+    }                                                                                  \
+    free(ua); free(ub); free(da); free(db); free(Gk); free(wsum); free(bsum); free(Glin); \
+    return it;
+
+int oracle_hook_ir2s_f64(const double *input, const double *G, int bsize, int row, int col, int numX, int numY,
+                         int iter_times, double r_wire, int enable_break, double thr, int nonlinear,
+                         const double *nl, int ret_v, double *out)
+{
+    HOOK_IR2S_BODY(double, sinh, fabs)
+}
+
+int oracle_hook_ir2s_f32(const float *input, const float *G, int bsize, int row, int col, int numX, int numY,
+                         int iter_times, double r_wire, int enable_break, double thr, int nonlinear,
+                         const double *nl, int ret_v, float *out)
+{
+    HOOK_IR2S_BODY(float, sinhf, fabsf)
 }

@@ -9,7 +9,7 @@ scratch directory and stores what it printed (``fastmode_nodebasedGS.out``,
 ``default.log`` or stdout) as small .npz fixtures.  Only data leaves this
 script: the inputs (states, voltages, resistances) and the outputs.
 
-Usage: python tests/golden/make_golden.py   (needs /root/reference)
+Usage: python tests/golden/make_golden.py [--only PREFIX]   (needs /root/reference)
 """
 from __future__ import annotations
 
@@ -26,6 +26,7 @@ REPO = os.path.dirname(os.path.dirname(HERE))
 REF_BIN = os.path.join(REPO, "oracle", "_ref")
 
 STATES4 = [2e6, 1.6e6, 1.3e6, 1e6]
+ONLY = ""  # --only PREFIX: write only the fixtures whose name starts with PREFIX
 
 
 def parse_out(path, n, m):
@@ -45,7 +46,7 @@ def parse_out(path, n, m):
 
 
 def write_config(path, version, n, m, states, cell_R, r, vin, vout, gs_args,
-                 seg_up=(), seg_down=()):
+                 seg_up=(), seg_down=(), solve_cmd="nodebasedGSMethod", rload_left=None):
     vin = [float(v) for v in vin]
     vout = [float(v) for v in vout]
     r = float(r)
@@ -59,6 +60,8 @@ def write_config(path, version, n, m, states, cell_R, r, vin, vout, gs_args,
         L.append(f"setLineR up {x} {y} {z!r}")
     for (x, y, z) in seg_down:
         L.append(f"setLineR down {x} {y} {z!r}")
+    if rload_left is not None:
+        L.append(f"rload left {float(rload_left)!r}")
     L.append("setUseLine left -1")
     L.append("setUseLine down -1")
     L.append("senseBitlineI down -1")
@@ -78,17 +81,21 @@ def write_config(path, version, n, m, states, cell_R, r, vin, vout, gs_args,
         for j in range(m):
             if states[i, j] != base:
                 L.append(f"setCellR {i} {j} {int(states[i, j])}")
-    L.append("nodebasedGSMethod " + " ".join(str(a) for a in gs_args))
+    L.append(solve_cmd + " " + " ".join(str(a) for a in gs_args))
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
 
 def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=(),
-             seg_down=(), keep_fields=True):
+             seg_down=(), keep_fields=True, solve_cmd="nodebasedGSMethod", rload_left=None):
+    if ONLY and not name.startswith(ONLY):
+        return
     exe = os.path.join(REF_BIN, f"sim_v{version}")
+    ir2s = solve_cmd == "fastsolve"
     with tempfile.TemporaryDirectory() as td:
         cfg = os.path.join(td, "cfg")
-        write_config(cfg, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up, seg_down)
+        write_config(cfg, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up, seg_down,
+                     solve_cmd, rload_left)
         res = subprocess.run([exe, cfg], cwd=td, capture_output=True, text=True, timeout=3600)
         text = res.stdout
         if "Error" in text:
@@ -96,9 +103,12 @@ def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=
         if version == 2:
             with open(os.path.join(td, "default.log")) as f:
                 text = f.read()
-        outp = os.path.join(td, "fastmode_nodebasedGS.out")
+        outp = os.path.join(td, "fastmode.out" if ir2s else "fastmode_nodebasedGS.out")
         errsum, vup, vdn, iout, iout_txt = parse_out(outp, n, m)
-    mrec = re.findall(r"threshould break, iter times =\s+(\d+)", text)
+    if ir2s:
+        mrec = re.findall(r"ir2s fast iter times = (\d+)", text)
+    else:
+        mrec = re.findall(r"threshould break, iter times =\s+(\d+)", text)
     record = int(mrec[-1]) if mrec else -1
     dur = [float(x) for x in re.findall(r"(?:Duration )?time = ([0-9.eE+-]+) s", text)]
     RU = np.full((n, m + 1), float(r))
@@ -109,9 +119,11 @@ def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=
         RD[x, y] = z
     data = dict(version=version, n=n, m=m, states=states.astype(np.uint8),
                 cell_R=np.array(cell_R, float), RU=RU, RD=RD, vin=np.array(vin, float),
-                vout=np.array(vout, float), iters=int(gs_args[0]), omega_text=str(gs_args[1]),
-                enable_break=int(len(gs_args) > 3 and gs_args[3] == "yes"),
-                tol=float(gs_args[4]) if len(gs_args) > 4 else 0.0,
+                vout=np.array(vout, float), iters=int(gs_args[0]),
+                omega_text="" if ir2s else str(gs_args[1]),
+                enable_break=int(gs_args[2]) if ir2s else int(len(gs_args) > 3 and gs_args[3] == "yes"),
+                tol=float(gs_args[3] if ir2s else (gs_args[4] if len(gs_args) > 4 else 0.0)),
+                command=solve_cmd, rload_left=float("nan") if rload_left is None else float(rload_left),
                 record=record, errsum=errsum, iout=iout, iout_txt=np.array(iout_txt),
                 solve_seconds=np.array(dur))
     if keep_fields:
@@ -130,6 +142,9 @@ def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=
 
 
 def main():
+    global ONLY
+    if len(sys.argv) > 2 and sys.argv[1] == "--only":
+        ONLY = sys.argv[2]
     if not (os.path.exists(os.path.join(REF_BIN, "sim_v2"))
             and os.path.exists(os.path.join(REF_BIN, "sim_v1"))):
         subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref"], check=True)
@@ -187,6 +202,24 @@ def main():
         vout = list(np.round(rng.uniform(0, 0.05, m), 3))
         run_case(f"small{idx}_v1_{n}x{m}", 1, n, m, st, STATES4, 2.93, vin, vout, args,
                  seg_up=su, seg_down=sd)
+
+    # IR2S (`fastsolve <iters> 0 <enable_break> <break_th>`) through V1:
+    # fixed sweep counts, the break rule, a source/sink load, non-square.
+    ir2s_cases = [(8, 8, 2.93, [1, 0, 0, "1e-9"], None), (16, 16, 2.93, [5, 0, 0, "1e-9"], 5.0),
+                  (12, 20, 4.1, [3, 0, 0, "1e-9"], None), (32, 32, 2.93, [1000, 0, 1, "1e-12"], None),
+                  (64, 64, 1.5, [1000, 0, 1, "1e-9"], 0.5)]
+    for idx, (n, m, r, args, rl) in enumerate(ir2s_cases):
+        rng = np.random.default_rng(700 + idx)
+        st = rng.integers(0, 4, (n, m))
+        vin = list(np.round(rng.uniform(0, 1, n), 3))
+        vout = list(np.round(rng.uniform(0, 0.05, m), 3))
+        run_case(f"ir2s{idx}_v1_{n}x{m}", 1, n, m, st, STATES4, r, vin, vout, args,
+                 solve_cmd="fastsolve", rload_left=rl)
+    rng = np.random.default_rng(777)
+    st = rng.integers(0, 4, (128, 128))
+    vin = list(np.where(rng.random(128) < 0.5, 0.2, 0.0))
+    run_case("ir2s5_v1_128x128", 1, 128, 128, st, STATES4, 3.5, vin, [0.0] * 128, [6, 0, 0, "1e-9"],
+             keep_fields=False, solve_cmd="fastsolve")
 
 
 if __name__ == "__main__":

@@ -7,7 +7,11 @@
 * the FP32/FP64 red-black replays against the exact solve (fixed point);
 * the hook line-GS restatement against a literal per-node Python replay of
   split_arr_matmul.cu:27-140 on tiny arrays, and its nonlinear fixed point
-  against a brute-force Newton solve.
+  against a brute-force Newton solve;
+* the IR2S restatement against golden vectors of the reference V1 binary's
+  `fastsolve` command (printed digits identical, identical iteration count),
+  the drop-form IR2S (checker of the GPU's in-kernel iteration) against it,
+  and both against the exact solve (fixed point, damping, contraction).
 """
 import glob
 import os
@@ -20,7 +24,9 @@ import scipy.sparse.linalg as spla
 import oracle
 from conftest import GOLDEN, STATES4, load_golden
 
-GOLDEN_FILES = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+_ALL = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+GOLDEN_FILES = [p for p in _ALL if not os.path.basename(p).startswith("ir2s")]
+IR2S_FILES = [p for p in _ALL if os.path.basename(p).startswith("ir2s")]
 
 
 def _case_inputs(z):
@@ -349,3 +355,112 @@ def test_line_solver_matches_banded_cholesky(n, m, r):
     np.testing.assert_allclose(I, Iex, rtol=1e-11)
     np.testing.assert_allclose(vin[:, None] - p, u, rtol=0, atol=1e-13)
     np.testing.assert_allclose(q + vout[None, :], d, rtol=0, atol=1e-13)
+
+
+def _ir2s_inputs(z):
+    g = 1.0 / z["cell_R"][z["states"]]
+    r = float(z["RD"][0, 0])
+    rl = float(z["rload_left"])
+    return g, r, (r if np.isnan(rl) else rl)
+
+
+@pytest.mark.parametrize("path", IR2S_FILES, ids=[os.path.basename(p)[:-4] for p in IR2S_FILES])
+def test_ir2s_matches_reference_binary(path):
+    z = np.load(path)
+    g, r, rl = _ir2s_inputs(z)
+    vup, vdn, iout, cnt = oracle.ir2s(g, z["vin"], z["vout"], 1.0 / r, rl, int(z["iters"]),
+                                      bool(z["enable_break"]), float(z["tol"]))
+    assert cnt == int(z["record"])
+    txt = np.array(["%.12e" % x for x in iout])
+    assert (txt == z["iout_txt"]).all()
+    if "vup" in z:
+        np.testing.assert_allclose(vup, z["vup"], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(vdn, z["vdown"], rtol=1e-12, atol=1e-300)
+    else:
+        rows = z["rows"]
+        np.testing.assert_allclose(vup[rows], z["vup_rows"], rtol=1e-12)
+        np.testing.assert_allclose(vdn.sum(1), z["vdown_rowsum"], rtol=1e-10, atol=1e-14)
+
+
+def test_ir2s_break_rule_and_count():
+    """Break at the first sweep whose largest voltage change is below the
+    threshold; the count returned is that 0-based sweep + 1 (and times + 1
+    when the threshold is never met: the reference's own off-by-one)."""
+    rng = np.random.default_rng(5)
+    g = 1.0 / np.array(STATES4)[rng.integers(0, 4, (16, 16))]
+    vin, vout = rng.uniform(0, 1, 16), np.zeros(16)
+    _, _, i_fix, c_fix = oracle.ir2s(g, vin, vout, 1 / 2.93, 2.93, 4)
+    assert c_fix == 4
+    _, _, i_brk, c_brk = oracle.ir2s(g, vin, vout, 1 / 2.93, 2.93, 40, True, 0.0)  # never < 0
+    assert c_brk == 41
+    _, _, i_b, c_b = oracle.ir2s(g, vin, vout, 1 / 2.93, 2.93, 1000, True, 1e-12)
+    # the same state as running exactly c_b sweeps
+    _, _, i_same, _ = oracle.ir2s(g, vin, vout, 1 / 2.93, 2.93, c_b)
+    assert np.array_equal(i_b, i_same)
+
+
+@pytest.mark.parametrize("n,m,r,iters", [(8, 8, 2.93, 1), (16, 12, 4.0, 3), (64, 64, 1.5, 6)])
+def test_ir2s_drop_form_matches_restatement(n, m, r, iters):
+    """The drop-variable iteration (cumulative sums) equals the reference's
+    voltage-form recurrences with uniform wire resistance (r_load = r)."""
+    from oracle.drops import ir2s_drops
+    rng = np.random.default_rng(n + m + iters)
+    g = np.exp(rng.uniform(np.log(5e-7), np.log(1e-6), (n, m)))
+    vin, vout = rng.uniform(0, 0.3, n), rng.uniform(0, 0.01, m)
+    # zero drops = vup at vin, vdown at vout (the reference starts vdown at 0 V)
+    vup, vdn, iout, _ = oracle.ir2s(g, vin, vout, 1.0 / r, r, iters, vdown=np.repeat(vout[None, :], n, 0))
+    p, q, io = ir2s_drops(g, r, vin, vout, iters)
+    np.testing.assert_allclose(vin[:, None] - p, vup, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(q + vout[None, :], vdn, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(io, iout, rtol=1e-12)
+
+
+def test_ir2s_first_sweep_is_first_order():
+    """One sweep from vup = vin, vdown = 0 puts the ideal currents g vin
+    through the wires: the closed form p = r W I with W_jk = min(j,k)+1."""
+    from oracle.drops import ir2s_drops
+    rng = np.random.default_rng(9)
+    n, m, r = 6, 5, 2.0
+    g = rng.uniform(5e-7, 1e-6, (n, m))
+    vin = rng.uniform(0, 1, n)
+    p, q, _ = ir2s_drops(g, r, vin, None, 1)
+    I = g * vin[:, None]
+    W = np.minimum.outer(np.arange(m), np.arange(m)) + 1.0
+    Wq = (n - np.maximum.outer(np.arange(n), np.arange(n))).astype(float)
+    np.testing.assert_allclose(p, r * I @ W.T, rtol=1e-14)
+    np.testing.assert_allclose(q, r * Wq @ I, rtol=1e-14)
+
+
+@pytest.mark.parametrize("omega", [1.0, 0.7])
+def test_ir2s_fixed_point_is_exact_solution(omega):
+    """The (damped) iteration converges to the exact solve, at the rate the
+    coupling's spectral radius predicts: rho undamped, max(|1-w|, |1-w(1+rho)|)
+    damped."""
+    from oracle.drops import coupling_radius, ir2s_drops
+    rng = np.random.default_rng(11)
+    n, r = 64, 4.0
+    g = 1.0 / np.array(STATES4)[rng.integers(0, 4, (n, n))]
+    vin = np.where(rng.random(n) < 0.5, 0.2, 0.0)
+    u, d, Iex = oracle.exact(g, vin, np.zeros(n), *oracle.segments_uniform(n, n, r))
+    rho = coupling_radius(g, r)
+    rate = rho if omega == 1.0 else max(abs(1 - omega), abs(1 - omega * (1 + rho)))
+    errs = []
+    for k in (2, 4):  # above the FP64 floor of the exact solve
+        p, q, io = ir2s_drops(g, r, vin, None, k, omega)
+        errs.append(np.max(np.abs(vin[:, None] - p - u)))
+    assert errs[1] / errs[0] <= 1.3 * rate ** 2
+    p, q, io = ir2s_drops(g, r, vin, None, 60, omega)
+    np.testing.assert_allclose(io, Iex, rtol=1e-11)
+
+
+def test_ir2s_contraction_scales_with_r_g_n2():
+    """rho ~ r * mean(g) * n * m (the estimate the damping is tuned from)."""
+    from oracle.drops import coupling_radius
+    rng = np.random.default_rng(3)
+    vals = []
+    for n in (32, 64):
+        g = np.exp(rng.uniform(np.log(5e-7), np.log(1e-6), (n, n)))
+        for r in (1.0, 4.0):
+            vals.append(coupling_radius(g, r) / (r * g.mean() * n * n))
+    assert max(vals) / min(vals) < 1.25
+    assert 0.5 < np.mean(vals) < 1.1

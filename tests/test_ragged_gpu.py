@@ -9,7 +9,10 @@
 * a batch of only 128x128 crossbars: bitwise the results of the uniform
   solver (same method, schedule and relaxation);
 * the launch captured into a graph and replayed, with eager work between
-  replays, reproduces the eager results exactly.
+  replays, reproduces the eager results exactly;
+* the in-kernel IR2S guess on strips (256x256, 512x512: column sums over the
+  team) against the FP64 drop-form oracle, and damped IR2S guesses with
+  converged solves against the exact solve.
 """
 import numpy as np
 import pytest
@@ -33,7 +36,7 @@ def test_fixed_sweeps_bitwise_vs_replay(cuda):
     import torch
     from fadesim_amd.ragged import RaggedPlan, SIZES
     inst = _mixed(cuda, [64, 128, 256, 512, 256, 64, 512, 128, 256], seed=3)
-    plan = RaggedPlan(inst["rows"], device=cuda)
+    plan = RaggedPlan(inst["rows"], device=cuda, tuning={})  # plain SOR: no IR2S guess
     rng = np.random.default_rng(9)
     warm_list = []
     for k, n in enumerate(inst["rows"]):
@@ -108,3 +111,37 @@ def test_plan_rejects_unsupported_sizes(cuda):
         RaggedPlan([64, 96], device=cuda)
     with pytest.raises(RuntimeError):
         RaggedPlan([128], cols=[64], device=cuda)
+
+
+def test_ir2s_guess_on_strips_matches_oracle(cuda):
+    from fadesim_amd.ragged import RaggedPlan, SIZES
+    from oracle.drops import ir2s_drops
+    inst = _mixed(cuda, [256, 512, 64, 128], seed=21)
+    tuning = {str(n): {"ir2s_iters": 3, "ir2s_omega": 0.8} for n in SIZES}
+    plan = RaggedPlan(inst["rows"], device=cuda, tuning=tuning)
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], fixed_sweeps=0, want_field=True)
+    fields = plan.unpack(res.field, "field")
+    for k, n in enumerate(inst["rows"]):
+        p, q, _ = ir2s_drops(inst["g_list"][k].astype(np.float64), inst["r_np"][k], inst["vin_list"][k],
+                             None, 3, 0.8)
+        for a, b in ((fields[k][0], p), (fields[k][1], q)):
+            err = np.max(np.abs(a.astype(np.float64) - b)) / np.abs(b).max()
+            assert err <= 2e-5, (k, n, err)
+
+
+def test_ir2s_damped_converged_mixed_vs_exact(cuda):
+    from fadesim_amd.ragged import RaggedPlan, SIZES
+    inst = _mixed(cuda, [64, 128, 256, 512, 512, 256], seed=23)
+    tuning = {"64": {"ir2s_iters": 4, "corr_iters": 4}, "128": {"ir2s_iters": 6, "corr_iters": 6},
+              "256": {"ir2s_iters": 8, "corr_iters": 8, "ir2s_omega": 0.85},
+              "512": {"ir2s_iters": 12, "corr_iters": 12, "ir2s_omega": 0.7}}
+    plan = RaggedPlan(inst["rows"], device=cuda, tuning=tuning)
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    assert (res.status.cpu().numpy() == 0).all()
+    base = RaggedPlan(inst["rows"], device=cuda, tuning={}).solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    iouts = plan.unpack(res.iout, "cols")
+    for k, n in enumerate(inst["rows"]):
+        _, _, iex, _ = line_solve(inst["g_list"][k], inst["r_np"][k], inst["vin_list"][k])
+        err = np.max(np.abs(iouts[k] - iex) / np.abs(iex))
+        assert err <= 1e-6, (k, n, err)
+    assert res.sweeps.double().sum().item() < 0.5 * base.sweeps.double().sum().item()

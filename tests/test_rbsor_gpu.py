@@ -6,7 +6,12 @@
   solve, on config-2 instances (4 states, r ~ U[1,5] ohm, vin in {0, 0.2} V)
   and other sizes, and against the reference CLI's golden vectors;
 * edge cases: exact warm start, nonzero vout, all-zero input, NaN input,
-  max-sweep exhaustion, unsupported shape.
+  max-sweep exhaustion, unsupported shape;
+* the in-kernel IR2S initial guess: with no sweeps, the drop field of k
+  (damped) wire-solve iterations against the FP64 drop-form oracle
+  (oracle/drops.py, itself pinned against the reference binary's `fastsolve`),
+  from zero and from a warm start; converged solves with it against the exact
+  solve, with far fewer sweeps.
 """
 import numpy as np
 import pytest
@@ -142,3 +147,70 @@ def test_unsupported_shape_raises(cuda):
     g = torch.ones((1, 48, 48), device=cuda)
     with pytest.raises(ValueError):
         solve_batch(g, 1.0, torch.zeros((1, 48), device=cuda))
+
+
+def _field_close(a, b, rel=2e-5):
+    """FP32 drop fields against FP64 ones: the FP32 rounding of each iterate."""
+    scale = max(np.abs(b).max(), 1e-30)
+    assert np.max(np.abs(a - b)) <= rel * scale, np.max(np.abs(a - b)) / scale
+
+
+@pytest.mark.parametrize("n,m", [(32, 32), (64, 64), (128, 128), (64, 128)])
+@pytest.mark.parametrize("iters,omega", [(1, 1.0), (3, 1.0), (4, 0.7)])
+def test_ir2s_guess_matches_oracle(cuda, n, m, iters, omega):
+    from oracle.drops import ir2s_drops
+    B = 3
+    g, r, vin = config2_instances(B, n, m, seed=7 * n + m + iters)
+    vout = np.random.default_rng(2).uniform(0, 0.01, (B, m))
+    res = _solve(cuda, g, r, vin, vout, None, fixed_sweeps=0, ir2s_iters=iters, ir2s_omega=omega,
+                 want_field=True)
+    field = res.field.cpu().numpy().astype(np.float64)
+    iout = res.iout.cpu().numpy()
+    assert (res.sweeps.cpu().numpy() == 0).all()
+    for b in range(B):
+        gd = g[b].astype(np.float64)
+        p, q, io = ir2s_drops(gd, r[b], vin[b], vout[b], iters, omega)
+        _field_close(field[b, 0], p)
+        _field_close(field[b, 1], q)
+        # the FP64 readout of the returned field, and the oracle's currents
+        c = vin[b][:, None] - vout[b][None, :]
+        np.testing.assert_allclose(iout[b], (gd * (c - field[b, 0] - field[b, 1])).sum(0), rtol=1e-12)
+        np.testing.assert_allclose(iout[b], io, rtol=1e-6)
+
+
+def test_ir2s_from_warm_start(cuda):
+    from oracle.drops import ir2s_drops
+    B, n = 2, 128
+    g, r, vin = config2_instances(B, n, seed=77)
+    w0 = np.random.default_rng(4).uniform(0, 2e-3, (B, 2, n, n)).astype(np.float32)
+    res = _solve(cuda, g, r, vin, None, w0, fixed_sweeps=0, ir2s_iters=2, want_field=True)
+    field = res.field.cpu().numpy().astype(np.float64)
+    for b in range(B):
+        p, q, _ = ir2s_drops(g[b].astype(np.float64), r[b], vin[b], None, 2, 1.0,
+                             w0[b, 0].astype(np.float64), w0[b, 1].astype(np.float64))
+        _field_close(field[b, 0], p)
+        _field_close(field[b, 1], q)
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_ir2s_converged_vs_exact(cuda, n):
+    B = 16
+    g, r, vin = config2_instances(B, n, seed=300 + n)
+    iex = oracle.exact_batch(g, r, vin)
+    plain = _solve(cuda, g, r, vin, tol=1e-6)
+    fast = _solve(cuda, g, r, vin, tol=1e-6, ir2s_iters=5, corr_iters=5)
+    assert (fast.status.cpu().numpy() == 0).all()
+    assert _relerr(fast.iout.cpu().numpy(), iex) <= 1e-6
+    sp, sf = plain.sweeps.double().mean().item(), fast.sweeps.double().mean().item()
+    assert sf <= sp / 4, (sp, sf)
+    # without the test before the first sweep: the same answer to tolerance
+    slow = _solve(cuda, g, r, vin, tol=1e-6, ir2s_iters=5, corr_iters=5, test_first=False)
+    assert _relerr(slow.iout.cpu().numpy(), iex) <= 1e-6
+
+
+def test_ir2s_bad_arguments(cuda):
+    g, r, vin = config2_instances(1, 32, seed=1)
+    with pytest.raises(ValueError):
+        _solve(cuda, g, r, vin, ir2s_iters=65)
+    with pytest.raises(RuntimeError):
+        _solve(cuda, g, r, vin, ir2s_iters=2, ir2s_omega=1.5)
