@@ -8,23 +8,25 @@ p = 0.5, bitlines grounded -- to <= 1e-6 relative bitline-current error
 (FP32 red-black SOR sweeps, FP64 residual test / refinement / readout).
 Inputs are synthetic, seeded, and resident in HBM before the timed region.
 
-Reported: whole-job solves/s of the plain (zero-drop initial guess) solver
-and, when a trained warm-start network is present
-(fadesim_amd/warmstart/weights), of the NN-warm-started solver with the
-network's inference inside the timed step; mean sweeps of both; the roofline
-of the solve kernel from HIP events on its launch stream; the CPU baseline
-(the oracle's restatement of the reference's lexicographic solver, OpenMP
-over instances) on a bounded sample, and the measured error of a GPU sample
-against the exact FP64 oracle (checker of the cpu_baseline leg).
+Reported: whole-job solves/s of three initial guesses for the same solve --
+zero drops (plain SOR), K wire-solve (IR2S) iterations inside the solve
+kernel (K from the autotuned table), and, when trained weights are present
+(fadesim_amd/weights), the learned warm start with the network's inference
+inside the timed step plus one IR2S iteration on its field (`value`, the
+metric's NN-warm-start figure); mean sweeps of each; the roofline of the
+solve kernel from HIP events on its launch stream; the CPU baseline (the
+oracle's restatement of the reference's lexicographic solver, OpenMP over
+instances) on a bounded sample, and the measured error of every variant on
+that sample against the exact FP64 oracle (checker of the cpu_baseline leg).
 
 Extra objects in the same line (not part of `value`): "uneven" -- 1024
 crossbars each of 64x64, 128x128, 256x256 and 512x512 (shuffled, seed 33)
 solved in one persistent launch and as a captured graph replay; "mlp" -- the
 784-256-10 crossbar MLP on 1024 images (sinh cells, every tile solve
-batched), plain vs NN-warm-started tile solves.
+batched), plain vs IR2S- and NN-warm-started tile solves.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
-                [--no-uneven] [--no-mlp] [--mlp-images N]
+                [--no-uneven] [--no-mlp] [--mlp-images N] [--only plain,ir2s,nn]
 """
 from __future__ import annotations
 
@@ -53,10 +55,11 @@ def make_batch(B, seed):
     return g, r, vin
 
 
-def algorithmic_bytes(sweeps_sum, B):
-    """Streaming-schedule bytes: 4 B of conductance per cell per sweep, plus
-    the inputs and the FP64 readout (4*M + 8*N per crossbar)."""
-    return 4.0 * M * N * sweeps_sum + B * (4.0 * M + 8.0 * N)
+def algorithmic_bytes(passes_sum, B):
+    """Streaming-schedule bytes: 4 B of conductance per cell per pass over the
+    crossbar (a red-black sweep or a wire-solve iteration), plus the inputs and
+    the FP64 readout (4*M + 8*N per crossbar)."""
+    return 4.0 * M * N * passes_sum + B * (4.0 * M + 8.0 * N)
 
 
 def load_traffic():
@@ -125,9 +128,9 @@ def bench_uneven(args, dev):
             "plan": plan.stats, "tuned": bool(plan.tuning)}
 
 
-def bench_mlp(args, dev, net):
+def bench_mlp(args, dev, net, k_ir2s):
     """784-256-10 crossbar MLP (8-bit, 2 bits/cell, 128x128 PN tiles, sinh
-    cells a = 2.5/V), plain vs NN-warm-started tile solves."""
+    cells a = 2.5/V): tile solves plain, IR2S-warm-started and NN-warm-started."""
     import torch
     from fadesim_amd import pim
     gen = torch.Generator().manual_seed(5)
@@ -135,8 +138,9 @@ def bench_mlp(args, dev, net):
     out = {"workload": f"784-256-10 MLP, {args.mlp_images} images U[0,1], 8-bit inputs bit-serial 0/0.2 V, "
                        "8-bit PN weights 2 bits/cell on 128x128 tiles, sinh cells a=2.5/V, tiles to 1e-6"}
     res = {}
-    for name in ("plain", "nn") if net is not None else ("plain",):
-        P = pim.mlp_inference_params(warm_start=net if name == "nn" else None, tol=args.tol)
+    for name in ("plain", "ir2s", "nn") if net is not None else ("plain", "ir2s"):
+        P = pim.mlp_inference_params(warm_start=net if name == "nn" else None, tol=args.tol,
+                                     ir2s_iters={"plain": 0, "ir2s": k_ir2s, "nn": 1}[name])
         model = pim.MLP(params=P, seed=0).to(dev).eval()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -151,13 +155,22 @@ def bench_mlp(args, dev, net):
     P0 = pim.PhyArrParams(Ron=1e6, Roff=2e6, cellBits=2, arrRowSize=128, arrColSize=128, inputMaxVoltage=0.2,
                           ir_mode="none")
     ideal = pim.MLP(params=P0, seed=0).to(dev).eval()(imgs)
-    if "nn" in res:
-        d = (res["nn"] - res["plain"]).abs()
-        out["nn_vs_plain_logits_max_abs_diff"] = float(d.max())
-        out["nn_vs_plain_argmax_agreement"] = float((res["nn"].argmax(1) == res["plain"].argmax(1)).double().mean())
+    for name in ("ir2s", "nn"):
+        if name in res:
+            d = (res[name] - res["plain"]).abs()
+            out[f"{name}_vs_plain_logits_max_abs_diff"] = float(d.max())
+            out[f"{name}_vs_plain_argmax_agreement"] = float(
+                (res[name].argmax(1) == res["plain"].argmax(1)).double().mean())
     out["ir_drop_effect_vs_ideal_argmax_agreement"] = float(
         (res["plain"].argmax(1) == ideal.argmax(1)).double().mean())
     return out
+
+
+def ir2s_iters_128(dev) -> int:
+    """IR2S iterations for 128x128 from the autotuned table (4 without one)."""
+    from fadesim_amd import autotune
+    t = autotune.load_table(dev) or {}
+    return int(t.get("128", {}).get("ir2s_iters", 4)) or 4
 
 
 def main():
@@ -172,6 +185,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-images", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn: run only these solve variants")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -200,82 +214,98 @@ def main():
     tr = torch.tensor(r, device=dev)
     tv = torch.tensor(vin, device=dev)
     net = warmstart.load_default(dev)
+    K = ir2s_iters_128(dev)
+    only = set(x for x in args.only.split(",") if x)
 
-    def step_plain(timing=False):
-        return solve_batch(tg, tr, tv, tol=args.tol, timing=timing)
-
-    def step_warm(timing=False):
-        w = net.predict(tg, tr, tv)
-        return solve_batch(tg, tr, tv, warm=w, tol=args.tol, timing=timing)
+    variants = {
+        # zero-drop initial guess, red-black SOR sweeps
+        "plain": (0, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, timing=timing)),
+        # K wire-solve (IR2S) iterations on the zero guess inside the solve kernel
+        "ir2s": (K, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, ir2s_iters=K, corr_iters=K,
+                                                    timing=timing)),
+    }
+    if net is not None:
+        # the network's field, one IR2S iteration on it, then the same test / SOR
+        variants["nn"] = (1, lambda timing=False: solve_batch(tg, tr, tv, warm=net.predict(tg, tr, tv), tol=args.tol,
+                                                              ir2s_iters=1, corr_iters=K, timing=timing))
+    if only:
+        variants = {k: v for k, v in variants.items() if k in only}
 
     def run(step):
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        kms = []
-        sweeps = []
-        res = None
+        kms, sweeps, res = [], [], None
         for _ in range(args.steps):
             res = step(timing=True)
             kms.append(last_kernel_ms())
             sweeps.append(res.sweeps)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        sw = torch.stack(sweeps).to(torch.float64)
-        return dt, kms, sw, res
+        return dt, kms, torch.stack(sweeps).to(torch.float64), res
 
-    dt_p, kms_p, sw_p, res_p = run(step_plain)
-    assert int((res_p.status != 0).sum()) == 0, "plain solve did not converge"
-    plain_sps = B * args.steps / dt_p
-    out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU"}
-    warm = None
-    if net is not None:
-        dt_w, kms_w, sw_w, res_w = run(step_warm)
-        assert int((res_w.status != 0).sum()) == 0, "warm-started solve did not converge"
-        warm = {"solves_per_s": B * args.steps / dt_w, "ms_per_step": 1e3 * dt_w / args.steps,
-                "mean_sweeps": float(sw_w.mean()), "kernel_ms": float(np.mean(kms_w)),
-                "network": net.describe()}
-    value = warm["solves_per_s"] if warm else plain_sps
-    dt_main = dt_w if warm else dt_p
-    kms_main = kms_w if warm else kms_p
-    sw_main = sw_w if warm else sw_p
-    kernel_ms = float(np.mean(kms_main))
-    ach = algorithmic_bytes(float(sw_main.sum()) / args.steps, B) / (kernel_ms * 1e-3) / 1e9
+    meas = {}
+    for name, (k_ir2s, step) in variants.items():
+        dt, kms, sw, res = run(step)
+        if int((res.status != 0).sum()) != 0:
+            raise RuntimeError(f"{name}: not every crossbar converged")
+        kernel_ms = float(np.mean(kms))
+        passes = float(sw.sum()) / args.steps + k_ir2s * B
+        ach = algorithmic_bytes(passes, B) / (kernel_ms * 1e-3) / 1e9
+        meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+                      "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
+                      "kernel_GBps_algorithmic": ach}
+    head = "nn" if "nn" in meas else ("ir2s" if "ir2s" in meas else "plain")
+    hm = meas[head]
     traffic = load_traffic()
-    out.update({
-        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt_main / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"{B} x 128x128 crossbars, 4 states {{2e6,1.6e6,1.3e6,1e6}} ohm, "
-                               "r ~ U[1,5] ohm, vin in {0,0.2} V, to 1e-6 rel I_out err",
-                   "batch": B, "rows": M, "cols": N, "tol": args.tol,
-                   "initial_guess": "nn-warm-start" if warm else "zero-drop"},
-        "plain": {"solves_per_s": plain_sps, "ms_per_step": 1e3 * dt_p / args.steps,
-                  "mean_sweeps": float(sw_p.mean()), "kernel_ms": float(np.mean(kms_p))},
-        "warm_start": warm,
-        "sweep_reduction": (float(sw_p.mean()) / warm["mean_sweeps"]) if warm else None,
-        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS,
-                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
-                     "kernel": "rbsor_kernel<Geo<4,8,32,16>>", "kernel_ms": kernel_ms,
-                     "bytes_model": "sweeps*4*M*N + 4*M + 8*N per crossbar (conductance streamed per sweep)"},
-    })
+    out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU",
+           "value": hm["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": hm["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"{B} x 128x128 crossbars, 4 states {{2e6,1.6e6,1.3e6,1e6}} ohm, "
+                                  "r ~ U[1,5] ohm, vin in {0,0.2} V, to 1e-6 rel I_out err",
+                      "batch": B, "rows": M, "cols": N, "tol": args.tol,
+                      "initial_guess": {"nn": "nn-warm-start (+1 IR2S iteration)", "ir2s": f"{K} IR2S iterations",
+                                        "plain": "zero-drop"}[head]},
+           "plain": meas.get("plain"), "ir2s_warm": meas.get("ir2s"),
+           "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None}
+    if "plain" in meas and "nn" in meas:
+        out["sweep_reduction"] = meas["plain"]["mean_sweeps"] / max(meas["nn"]["mean_sweeps"], 1e-9)
+        out["nn_vs_plain_speedup"] = meas["nn"]["solves_per_s"] / meas["plain"]["solves_per_s"]
+    if "plain" in meas and "ir2s" in meas:
+        out["ir2s_vs_plain_speedup"] = meas["ir2s"]["solves_per_s"] / meas["plain"]["solves_per_s"]
+    out["roofline"] = {"bound": "hbm", "achieved": hm["kernel_GBps_algorithmic"], "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": hm["kernel_GBps_algorithmic"] / HBM_PEAK_GBS,
+                       "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                       "kernel": "rbsor_kernel<Geo<4,8,32,16>>", "kernel_ms": hm["kernel_ms"],
+                       "bytes_model": "(sweeps + IR2S iterations) * 4*M*N + 4*M + 8*N per crossbar "
+                                      "(conductance streamed once per sweep or wire-solve pass)"}
+    if "plain" in meas:
+        pm = meas["plain"]
+        out["roofline_plain_sor"] = {"bound": "hbm", "achieved": pm["kernel_GBps_algorithmic"], "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": pm["kernel_GBps_algorithmic"] / HBM_PEAK_GBS,
+                                     "kernel_ms": pm["kernel_ms"]}
     if not args.no_cpu:
         cpu, (gs, rs, vs, iex) = cpu_baseline(args.cpu_sample, seed=77)
         out["cpu_baseline"] = cpu
-        # error of the GPU on the same sample, against the exact FP64 oracle
-        rp = solve_batch(torch.tensor(gs, device=dev), torch.tensor(rs, device=dev),
-                         torch.tensor(vs, device=dev), tol=args.tol)
-        out["max_rel_err_plain"] = float(np.max(np.abs(rp.iout.cpu().numpy() - iex) / np.abs(iex)))
-        if net is not None:
-            tgs, trs, tvs = (torch.tensor(a, device=dev) for a in (gs, rs, vs))
-            rw = solve_batch(tgs, trs, tvs, warm=net.predict(tgs, trs, tvs), tol=args.tol)
-            out["max_rel_err_warm"] = float(np.max(np.abs(rw.iout.cpu().numpy() - iex) / np.abs(iex)))
+        # error of every variant on the same sample, against the exact FP64 oracle
+        tgs, trs, tvs = (torch.tensor(a, device=dev) for a in (gs, rs, vs))
+        errs = {}
+        for name in meas:
+            if name == "plain":
+                rr = solve_batch(tgs, trs, tvs, tol=args.tol)
+            elif name == "ir2s":
+                rr = solve_batch(tgs, trs, tvs, tol=args.tol, ir2s_iters=K, corr_iters=K)
+            else:
+                rr = solve_batch(tgs, trs, tvs, warm=net.predict(tgs, trs, tvs), tol=args.tol, ir2s_iters=1,
+                                 corr_iters=K)
+            errs[name] = float(np.max(np.abs(rr.iout.cpu().numpy() - iex) / np.abs(iex)))
+        out["max_rel_err"] = errs
     else:
         out["cpu_baseline"] = None
     for key, flag, fn in (("uneven", args.no_uneven, lambda: bench_uneven(args, dev)),
-                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net))):
+                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net, K))):
         if flag:
             continue
         try:

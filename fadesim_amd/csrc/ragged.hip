@@ -119,7 +119,7 @@ __global__ void __launch_bounds__(kThreads) ragged_kernel(RaggedArgs A)
     __shared__ int s_task, s_dead;
     __shared__ double cbuf[kCbufMax];
     __shared__ double cext[3 * kNmax];
-    const rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     const rb::StripCtx none{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * kSlotCells;
     const int team = blockIdx.x / kTeam, rank = blockIdx.x % kTeam;
@@ -535,7 +535,7 @@ extern "C" int fadesim_ragged_solve(const fadesim_ragged_plan *P, const fadesim_
     for (int c = 0; c < 4; ++c) {
         A.ctl[c] = P->ctl[c];
         A.ctl[c].tol_eff = d->tol / safety;
-        A.ctl[c].switch_tol = d->switch_tol > 0 ? d->switch_tol : 2e-5;
+        A.ctl[c].switch_tol = d->switch_tol > 0 ? d->switch_tol : std::fmin(1e-3, std::fmax(2e-5, 1e3 * d->tol));
         A.ctl[c].max_sweeps = d->max_sweeps;
         A.ctl[c].flags = d->flags;
         if ((d->flags & FADESIM_F_TEST_FIRST) && A.ctl[c].ir2s_iters > 0)

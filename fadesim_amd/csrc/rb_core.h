@@ -42,11 +42,11 @@
 // below switch_tol the solve takes one iterative-refinement step: the exact
 // FP64 residual becomes the right-hand side of a correction solve, whose
 // initial guess is corr_iters wire Green's-function (IR2S) iterations on the
-// correction (green_guess) rather than zero.  A solve that meets the
+// correction (phase_green) rather than zero.  A solve that meets the
 // tolerance at a test stops there, refined or not.
 //
 // Initial guess: zero drops or the caller's field, then ir2s_iters IR2S
-// iterations (ir2s_guess: every wire solved exactly for the present cell
+// iterations (phase_ir2s: every wire solved exactly for the present cell
 // currents, optionally damped).  The iteration contracts by ~0.8 r gbar n m
 // per step, so a few of them bring a 128x128 crossbar to the tolerance before
 // the first sweep.
@@ -61,7 +61,7 @@
 namespace fadesim {
 namespace rb {
 
-constexpr int kWsPlanes = 5; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q
+constexpr int kWsPlanes = 6; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q | IR2S currents
 constexpr int kMaxStrips = 16;
 constexpr int kRowsLd = 512;                      // row stride of the strip exchange buffers
 constexpr unsigned long long kWaitTicks = 500000000ull; // 5 s of the 100 MHz constant clock
@@ -465,14 +465,13 @@ __device__ __forceinline__ void col_reduce2(const Shared &sh, const double (&a)[
     __syncthreads();
 }
 
-// Bitline Green's sum of a tile: out(ii,jj) = scale sum_i' (n - max(i,i')) v(i',j)
+// Bitline Green's sum of a tile: emit(ii, jj, scale sum_i' (n - max(i,i')) v(i',j))
 // over all n rows of the crossbar (i = gi0 + i0 + ii, j = j0 + jj), v given by
 // vf(ii, jj).  Strips complete the column prefix over the team (team barrier
 // nbar++).  Returns false when the team aborted.
-template <class G, bool STRIP, class VF>
-__device__ __forceinline__ bool col_green_tile(const Shared &sh, const StripCtx &X, VF vf, double scale, int n,
-                                               int gi0, int i0, int j0, int ty, int t, int &nbar,
-                                               float (&out)[G::TR][G::TC])
+template <class G, bool STRIP, class VF, class EM>
+__device__ __forceinline__ bool col_green_emit(const Shared &sh, const StripCtx &X, VF vf, double scale, int n,
+                                               int gi0, int i0, int j0, int ty, int t, int &nbar, EM emit)
 {
     constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
     double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
@@ -539,8 +538,110 @@ __device__ __forceinline__ bool col_green_tile(const Shared &sh, const StripCtx 
             const double v = vf(ii, jj);
             A += v;
             C += v * gi;
-            out[ii][jj] = (float)(scale * (gi * A + (T - C)));
+            emit(ii, jj, scale * (gi * A + (T - C)));
         }
+    }
+    __syncthreads();
+    return true;
+}
+
+// col_green_emit into a float tile.
+template <class G, bool STRIP, class VF>
+__device__ __forceinline__ bool col_green_tile(const Shared &sh, const StripCtx &X, VF vf, double scale, int n,
+                                               int gi0, int i0, int j0, int ty, int t, int &nbar,
+                                               float (&out)[G::TR][G::TC])
+{
+    return col_green_emit<G, STRIP>(sh, X, vf, scale, n, gi0, i0, j0, ty, t, nbar,
+                                    [&](int ii, int jj, double v) { out[ii][jj] = (float)v; });
+}
+
+// col_green_emit streamed by tile rows: src(ii, float (&)[TC]) supplies row ii
+// of v, emit(ii, const double (&)[TC]) takes row ii of the result.  The tile
+// rows are not unrolled, so only O(TC) values are live: for the phases whose
+// tile state lives in the workspace.
+template <class G, bool STRIP, class SRC, class EM>
+__device__ __forceinline__ bool col_green_rows(const Shared &sh, const StripCtx &X, SRC src, double scale, int n,
+                                               int gi0, int i0, int j0, int ty, int t, int &nbar, EM emit)
+{
+    constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
+    double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
+    double s0[TC], s1[TC];
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj)
+        s0[jj] = s1[jj] = 0.0;
+#pragma unroll 1
+    for (int ii = 0; ii < TR; ++ii) {
+        float v[TC];
+        src(ii, v);
+        const double gi = (double)(n - (gi0 + i0 + ii));
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            s0[jj] += (double)v[jj];
+            s1[jj] += (double)v[jj] * gi;
+        }
+    }
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        b0[ty * N + j0 + jj] = s0[jj];
+        b1[ty * N + j0 + jj] = s1[jj];
+    }
+    __syncthreads();
+    double T0 = 0.0, T1 = 0.0;
+    if (t < N) {
+        for (int q = 0; q < TY; ++q) {
+            const double x0 = b0[q * N + t], x1 = b1[q * N + t];
+            b0[q * N + t] = T0;
+            b1[q * N + t] = T1;
+            T0 += x0;
+            T1 += x1;
+        }
+    }
+    double E0 = 0.0, E1 = 0.0, Tall = T1;
+    if (STRIP) {
+        double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
+        if (t < N) {
+            mine[t] = T0;
+            mine[kRowsLd + t] = T1;
+        }
+        if (!team_barrier(X, sh.dead, t, nbar++))
+            return false;
+        if (t < N) {
+            Tall = 0.0;
+            for (int s = 0; s < X.S; ++s) {
+                const double *ps = X.part + (((size_t)((nbar - 1) & 1) * X.S + s) * 2) * kRowsLd;
+                if (s < X.s) {
+                    E0 += ps[t];
+                    E1 += ps[kRowsLd + t];
+                }
+                Tall += ps[kRowsLd + t];
+            }
+        }
+    }
+    if (t < N) {
+        sh.cext[t] = E0;
+        sh.cext[N + t] = E1;
+        sh.cext[2 * N + t] = Tall;
+    }
+    __syncthreads();
+    double A[TC], C[TC];
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        A[jj] = sh.cext[j0 + jj] + b0[ty * N + j0 + jj];
+        C[jj] = sh.cext[N + j0 + jj] + b1[ty * N + j0 + jj];
+    }
+#pragma unroll 1
+    for (int ii = 0; ii < TR; ++ii) {
+        float v[TC];
+        src(ii, v);
+        const double gi = (double)(n - (gi0 + i0 + ii));
+        double o[TC];
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            A[jj] += (double)v[jj];
+            C[jj] += (double)v[jj] * gi;
+            o[jj] = scale * (gi * A[jj] + (sh.cext[2 * N + j0 + jj] - C[jj]));
+        }
+        emit(ii, o);
     }
     __syncthreads();
     return true;
@@ -778,7 +879,63 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     return (double)em;
 }
 
-// Wire-solve (IR2S) iterations on the drops, in place on yp, yq:
+// ---------------------------------------------------------------- phases
+// The set-up, the test, the IR2S guess and the refinement guess run outside
+// the sweep loop.  Each is compiled as a function of its own so that its
+// register needs do not add to the pressure of the sweep loop (the tile state
+// p, q, h, rho_p, rho_q in VGPRs); the tile state crosses them through the
+// snapshot planes 3, 4 of the workspace.  The LDS pointers are passed one by
+// one (a struct of LDS addresses passed by value would become a constant
+// with address-space casts, which the backend cannot emit).
+#define RB_SH_PARAMS                                                                                    \
+    float *sh_halo, double *sh_S1, double *sh_Ifloor, double *sh_colI, double *sh_colD, float *sh_wmax, \
+        int *sh_dead, double *sh_cbuf, double *sh_cext
+#define RB_SH_ARGS(s) s.halo, s.S1, s.Ifloor, s.colI, s.colD, s.wmax, s.dead, s.cbuf, s.cext
+#define RB_SH_LOCAL Shared sh{sh_halo, sh_S1, sh_Ifloor, sh_colI, sh_colD, sh_wmax, sh_dead, sh_cbuf, sh_cext}
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) bool phase_setup(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                      int i0, int j0, int t, int ty, double rwire, RB_SH_PARAMS,
+                                                      int *nbar)
+{
+    RB_SH_LOCAL;
+    return setup_columns<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, sh, *nbar);
+}
+
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) double phase_test(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                       int i0, int j0, int t, int tx, int ty, double a, double rwire,
+                                                       double hcoef, bool refined, bool want_rho, bool has_l,
+                                                       bool has_r, bool has_u, bool has_d, RB_SH_PARAMS, int pub,
+                                                       int *nbar)
+{
+    RB_SH_LOCAL;
+    return conv_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, want_rho, has_l,
+                               has_r, has_u, has_d, sh, pub, *nbar);
+}
+
+template <class G>
+__device__ __forceinline__ void tile_load(const float *wsb, int i0, int j0, float (&yp)[G::TR][G::TC],
+                                          float (&yq)[G::TR][G::TC])
+{
+#pragma unroll
+    for (int ii = 0; ii < G::TR; ++ii) {
+        ldrow<G::TC>(wsb + 3 * G::M * G::N + (size_t)(i0 + ii) * G::N + j0, yp[ii]);
+        ldrow<G::TC>(wsb + 4 * G::M * G::N + (size_t)(i0 + ii) * G::N + j0, yq[ii]);
+    }
+}
+
+template <class G>
+__device__ __forceinline__ void tile_store(float *wsb, int i0, int j0, const float (&yp)[G::TR][G::TC],
+                                           const float (&yq)[G::TR][G::TC])
+{
+#pragma unroll
+    for (int ii = 0; ii < G::TR; ++ii) {
+        strow<G::TC>(wsb + 3 * G::M * G::N + (size_t)(i0 + ii) * G::N + j0, yp[ii]);
+        strow<G::TC>(wsb + 4 * G::M * G::N + (size_t)(i0 + ii) * G::N + j0, yq[ii]);
+    }
+}
+
+// Wire-solve (IR2S) iterations on the drops, in place on the drops held in planes 3, 4:
 //   s_ij = g_ij (c_ij - p_ij - q_ij)            (cell currents of the drops)
 //   p_ij <- r sum_j' (min(j,j')+1) s_ij'       (wordline: source segment + wires)
 //   q_ij <- r sum_i' (n - max(i,i')) s_i'j     (bitline: wires + sink segment)
@@ -793,125 +950,125 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
 // contracts by rho, and the damped one, x <- (1-w) x + w M^-1 (b - K x), by
 // max(|1-w|, |1-w(1+rho)|) -- rho/(2+rho) at w = 2/(2+rho).  Returns false
 // when the team aborted.
+// The tile is streamed by rows through the workspace (the currents of an
+// iteration through plane 5), so only O(TC) values are live.
 template <class G, bool STRIP>
-__device__ __forceinline__ bool ir2s_guess(const Inst &I, const StripCtx &X, const float *gp, int gi0, int i0,
-                                           int j0, int t, int tx, int ty, double rwire, const Shared &sh, int &nbar,
-                                           int iters, double w, float (&yp)[G::TR][G::TC],
-                                           float (&yq)[G::TR][G::TC])
+__device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                     int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
+                                                     int *nbar, int iters, double w)
 {
-    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    RB_SH_LOCAL;
+    constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
+    float *P = wsb + 3 * M * N, *Q = wsb + 4 * M * N, *S = wsb + 5 * M * N;
     const double w1 = 1.0 - w;
-    float gt[TR][TC];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii)
-        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
 #pragma unroll 1
     for (int k = 0; k < iters; ++k) {
-        auto cur = [&](int ii, int jj) {
-            const double c = I.vin[gi0 + i0 + ii] - I.vout[j0 + jj];
-            return (double)gt[ii][jj] * ((c - (double)yp[ii][jj]) - (double)yq[ii][jj]);
-        };
-        float np[TR][TC], nq[TR][TC];
-#pragma unroll
+#pragma unroll 1
         for (int ii = 0; ii < TR; ++ii) {
-            double v[TC], o[TC];
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                v[jj] = cur(ii, jj);
-            row_green_row<G>(v, j0, tx, rwire, o);
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                np[ii][jj] = (float)(w1 * (double)yp[ii][jj] + w * o[jj]);
-        }
-        if (!col_green_tile<G, STRIP>(sh, X, cur, rwire, I.rows, gi0, i0, j0, ty, t, nbar, nq))
-            return false;
-#pragma unroll
-        for (int ii = 0; ii < TR; ++ii)
+            const size_t o = (size_t)(i0 + ii) * N + j0;
+            float gr[TC], pr[TC], qr[TC], sr[TC];
+            ldrow<TC>(gp + o, gr);
+            ldrow<TC>(P + o, pr);
+            ldrow<TC>(Q + o, qr);
+            const double vi = I.vin[gi0 + i0 + ii];
+            double v[TC], out[TC];
 #pragma unroll
             for (int jj = 0; jj < TC; ++jj) {
-                yp[ii][jj] = np[ii][jj];
-                yq[ii][jj] = (float)(w1 * (double)yq[ii][jj] + w * (double)nq[ii][jj]);
+                v[jj] = (double)gr[jj] * (((vi - I.vout[j0 + jj]) - (double)pr[jj]) - (double)qr[jj]);
+                sr[jj] = (float)v[jj];
             }
+            strow<TC>(S + o, sr);
+            row_green_row<G>(v, j0, tx, rwire, out);
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                pr[jj] = (float)(w1 * (double)pr[jj] + w * out[jj]);
+            strow<TC>(P + o, pr);
+        }
+        if (!col_green_rows<G, STRIP>(
+                sh, X, [&](int ii, float (&v)[TC]) { ldrow<TC>(S + (size_t)(i0 + ii) * N + j0, v); }, rwire, I.rows,
+                gi0, i0, j0, ty, t, *nbar, [&](int ii, const double (&o)[TC]) {
+                    float qr[TC];
+                    float *q = Q + (size_t)(i0 + ii) * N + j0;
+                    ldrow<TC>(q, qr);
+#pragma unroll
+                    for (int jj = 0; jj < TC; ++jj)
+                        qr[jj] = (float)(w1 * (double)qr[jj] + w * o[jj]);
+                    strow<TC>(q, qr);
+                }))
+            return false;
     }
     return true;
 }
 
-// Initial guess of the refinement correction y (A y = residual), into yp, yq:
-// `iters` wire-solve iterations from zero, damped by w as in ir2s_guess,
-//   y1 = (aL)^-1 res,  y_{k+1} = (1-w) y_k + w (y1 - (aL)^-1 G (y_k,p + y_k,q))
+// Initial guess of the refinement correction y (A y = residual), into planes 3, 4:
+// `iters` wire-solve iterations from zero, damped by w as in phase_ir2s,
+//   y_{k+1} = (1-w) y_k + w (aL)^-1 (res - G (y_k,p + y_k,q))
 // with (aL)^-1 the wordline (r (min(j,j')+1)) and bitline (r (n - max(i,i')))
 // line inverses; the cell coupling G is ~1e-6 of the wire conductance, so
 // two iterations already put y within a few 1e-3 of the correction and the
 // correction converges in a few sweeps instead of restarting from zero.
-// Reads the scaled residual (planes 1, 2).
+// Reads the scaled residual (planes 1, 2); streamed by tile rows like
+// phase_ir2s.
 template <class G, bool STRIP>
-__device__ __forceinline__ bool green_guess(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
-                                            int i0, int j0, int t, int tx, int ty, double rwire, double hcoef,
-                                            const Shared &sh, int &nbar, int iters, double w,
-                                            float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC])
+__device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                      int i0, int j0, int t, int tx, int ty, double rwire,
+                                                      double hcoef, RB_SH_PARAMS, int *nbar, int iters, double w)
 {
+    RB_SH_LOCAL;
     constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
-    const double ih = 1.0 / hcoef;
-    const double w1 = 1.0 - w;
-    float y1p[TR][TC], y1q[TR][TC];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii) {
-        float rr[TC];
-        ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rr);
-        double v[TC], o[TC];
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            v[jj] = (double)rr[jj] * ih;
-        row_green_row<G>(v, j0, tx, rwire, o);
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            y1p[ii][jj] = (float)o[jj];
-    }
-    {
-        float rq[TR][TC];
-#pragma unroll
-        for (int ii = 0; ii < TR; ++ii)
-            ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rq[ii]);
-        if (!col_green_tile<G, STRIP>(sh, X, [&](int ii, int jj) { return (double)rq[ii][jj] * ih; }, rwire, I.rows,
-                                      gi0, i0, j0, ty, t, nbar, y1q))
-            return false;
-    }
-    float gt[TR][TC];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii) {
-        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            yp[ii][jj] = (float)(w * (double)y1p[ii][jj]);
-            yq[ii][jj] = (float)(w * (double)y1q[ii][jj]);
-        }
-    }
+    float *P = wsb + 3 * M * N, *Q = wsb + 4 * M * N, *S = wsb + 5 * M * N;
+    const float *RP = wsb + 1 * M * N, *RQ = wsb + 2 * M * N;
+    const double ih = 1.0 / hcoef, w1 = 1.0 - w;
+    (void)I;
 #pragma unroll 1
-    for (int k = 1; k < iters; ++k) {
-        auto coupling = [&](int ii, int jj) {
-            return (double)gt[ii][jj] * ((double)yp[ii][jj] + (double)yq[ii][jj]);
-        };
-        float np[TR][TC], cq[TR][TC];
-#pragma unroll
+    for (int k = 0; k < iters; ++k) {
+#pragma unroll 1
         for (int ii = 0; ii < TR; ++ii) {
-            double v[TC], o[TC];
+            const size_t o = (size_t)(i0 + ii) * N + j0;
+            float gr[TC], pr[TC], qr[TC], rp[TC], rq[TC], sr[TC];
+            ldrow<TC>(RP + o, rp);
+            ldrow<TC>(RQ + o, rq);
+            if (k > 0) {
+                ldrow<TC>(gp + o, gr);
+                ldrow<TC>(P + o, pr);
+                ldrow<TC>(Q + o, qr);
+            } else {
 #pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                v[jj] = coupling(ii, jj);
-            row_green_row<G>(v, j0, tx, rwire, o);
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                np[ii][jj] = (float)(w1 * (double)yp[ii][jj] + w * ((double)y1p[ii][jj] - o[jj]));
-        }
-        if (!col_green_tile<G, STRIP>(sh, X, coupling, rwire, I.rows, gi0, i0, j0, ty, t, nbar, cq))
-            return false;
-#pragma unroll
-        for (int ii = 0; ii < TR; ++ii)
+                for (int jj = 0; jj < TC; ++jj)
+                    gr[jj] = pr[jj] = qr[jj] = 0.f;
+            }
+            double v[TC], out[TC];
 #pragma unroll
             for (int jj = 0; jj < TC; ++jj) {
-                yp[ii][jj] = np[ii][jj];
-                yq[ii][jj] = (float)(w1 * (double)yq[ii][jj] + w * ((double)y1q[ii][jj] - (double)cq[ii][jj]));
+                const double cpl = (double)gr[jj] * ((double)pr[jj] + (double)qr[jj]);
+                v[jj] = (double)rp[jj] * ih - cpl;
+                sr[jj] = (float)((double)rq[jj] * ih - cpl);
             }
+            strow<TC>(S + o, sr);
+            row_green_row<G>(v, j0, tx, rwire, out);
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                pr[jj] = (float)(w1 * (double)pr[jj] + w * out[jj]);
+            strow<TC>(P + o, pr);
+            if (k == 0) {
+#pragma unroll
+                for (int jj = 0; jj < TC; ++jj)
+                    qr[jj] = 0.f;
+                strow<TC>(Q + o, qr);
+            }
+        }
+        if (!col_green_rows<G, STRIP>(
+                sh, X, [&](int ii, float (&v)[TC]) { ldrow<TC>(S + (size_t)(i0 + ii) * N + j0, v); }, rwire, I.rows,
+                gi0, i0, j0, ty, t, *nbar, [&](int ii, const double (&o)[TC]) {
+                    float qr[TC];
+                    float *q = Q + (size_t)(i0 + ii) * N + j0;
+                    ldrow<TC>(q, qr);
+#pragma unroll
+                    for (int jj = 0; jj < TC; ++jj)
+                        qr[jj] = (float)(w1 * (double)qr[jj] + w * o[jj]);
+                    strow<TC>(q, qr);
+                }))
+            return false;
     }
     return true;
 }
@@ -950,7 +1107,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
         *sh.dead = 0;
     __syncthreads();
 
-    if (!setup_columns<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, sh, nbar))
+    if (!phase_setup<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, RB_SH_ARGS(sh), &nbar))
         return false;
 
     float yp[TR][TC], yq[TR][TC], hn[TR][TC], rp[TR][TC], rq[TR][TC];
@@ -965,10 +1122,13 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 yp[ii][jj] = yq[ii][jj] = 0.f;
         }
     }
-    if (C.ir2s_iters > 0 &&
-        !ir2s_guess<G, STRIP>(I, X, gp, gi0, i0, j0, t, tx, ty, rwire, sh, nbar, C.ir2s_iters, C.ir2s_omega, yp,
-                              yq))
-        return false;
+    if (C.ir2s_iters > 0) {
+        tile_store<G>(wsb, i0, j0, yp, yq);
+        if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
+                                  C.ir2s_omega))
+            return false;
+        tile_load<G>(wsb, i0, j0, yp, yq);
+    }
     publish_all<G>(yp, yq, halo, t);
     if (STRIP)
         strip_publish<G>(X, yq, ty, j0, lane, pub++);
@@ -1005,8 +1165,8 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             strow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
             strow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
         }
-        est = conv_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, false, has_l,
-                                  has_r, has_u, has_d, sh, pub - 1, nbar);
+        est = phase_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, false, has_l,
+                                   has_r, has_u, has_d, RB_SH_ARGS(sh), pub - 1, &nbar);
         if (est < 0.0)
             return false;
         bool reload = true, done = false, just_refined = false;
@@ -1023,8 +1183,8 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             done = true;
         } else if (refine_pending && est <= C.switch_tol) {
             // ---- one iterative-refinement step: rhs <- exact FP64 residual ----
-            conv_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false, true, has_l, has_r,
-                                has_u, has_d, sh, pub - 1, nbar);
+            phase_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false, true, has_l, has_r,
+                                 has_u, has_d, RB_SH_ARGS(sh), pub - 1, &nbar);
             if (I.field) {
 #pragma unroll
                 for (int ii = 0; ii < TR; ++ii) {
@@ -1043,9 +1203,10 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                     for (int jj = 0; jj < TC; ++jj)
                         yp[ii][jj] = yq[ii][jj] = 0.f;
             } else {
-                if (!green_guess<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, sh, nbar,
-                                           C.corr_iters, C.ir2s_omega, yp, yq))
+                if (!phase_green<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, RB_SH_ARGS(sh), &nbar,
+                                           C.corr_iters, C.ir2s_omega))
                     return false;
+                tile_load<G>(wsb, i0, j0, yp, yq);
             }
             if (t < N)
                 sh.S1[t] = sh.colI[t];

@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
     __shared__ int s_inst, s_dead;
     __shared__ double cbuf[2 * G::TY * N];
     __shared__ double cext[3 * N];
-    const rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     const rb::StripCtx none{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     for (;;) {
@@ -217,7 +217,7 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     args.ws = nullptr;
     args.ctl.omega = (float)w;
     args.ctl.tol_eff = d->tol / safety;
-    args.ctl.switch_tol = d->switch_tol > 0 ? d->switch_tol : 2e-5;
+    args.ctl.switch_tol = d->switch_tol > 0 ? d->switch_tol : std::fmin(1e-3, std::fmax(2e-5, 1e3 * d->tol));
     args.ctl.dwr_scale = 16.0;
     args.ctl.max_sweeps = d->max_sweeps;
     args.ctl.flags = d->flags;

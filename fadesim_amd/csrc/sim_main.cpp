@@ -1,5 +1,6 @@
 // `sim <config>` -- the crossbar command language of the FADESim CLI, with
-// nodebasedGSMethod solved on the GPU through fadesim_gs64_solve.
+// nodebasedGSMethod solved on the GPU through fadesim_gs64_solve and the V1
+// IR2S command fastsolve through fadesim_ir2s.
 //
 // Command semantics follow the reference CLI (V2 registry
 // crossbarHspice/V2/main.cpp:68-117, command bodies
@@ -294,6 +295,139 @@ void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
     a.log(os.str());
 }
 
+// `fastsolve <iter_times> [table 0] [enable_break break_th]`: IR2S with the
+// V1 CPU solver's semantics (crossbarHspice/2dArray.cc:318-327, 1691-1900;
+// ir_drop_fast_solver.h:24-163) on the GPU through fadesim_ir2s; output
+// fastmode.out in the reference's format.
+void fastsolve(Array2D &a, const std::vector<std::string> &args)
+{
+    if (args.empty())
+        throw CmdError("fastsolve needs iter_times");
+    const int iter_times = parse<int>(args[0]);
+    int from_table = 0, enable_break = 0;
+    double break_th = 1e-9;
+    if (args.size() > 1)
+        from_table = parse<int>(args[1]);
+    if (args.size() > 2)
+        enable_break = parse<int>(args[2]);
+    if (args.size() > 3)
+        break_th = std::stod(args[3]);
+    if (from_table)
+        throw CmdError("fastsolve: initial voltages from a table are not supported");
+    if (!(a.m >= 2 && a.n >= 2))
+        throw CmdError("m and n should greater than 2");
+    const int n = a.n, m = a.m;
+    const size_t nm = (size_t)n * m;
+    std::vector<double> g(nm);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            const int st = a.arr[i][j];
+            if (st < 0 || st >= (int)a.cellR_states.size())
+                throw CmdError("cell state " + std::to_string(st) + " has no resistance (cellRstates)");
+            g[(size_t)i * m + j] = 1.0 / a.cellR_states[st];
+        }
+    const double r_wire = a.RD[0][0]; // one wire resistance: g_wire = 1 / RDLine[0][0]
+    const double gw = 1.0 / r_wire;
+    const double r_load = a.r_load[1] < 0 ? 1.0 / gw : a.r_load[1];
+    std::vector<double> vup(nm), vdown(nm), iout(m);
+    int count = 0;
+    double *d_g, *d_vin, *d_vout, *d_vup, *d_vdn, *d_iout;
+    int *d_cnt;
+    HIPCHK(hipMalloc(&d_g, nm * 8));
+    HIPCHK(hipMalloc(&d_vin, n * 8));
+    HIPCHK(hipMalloc(&d_vout, m * 8));
+    HIPCHK(hipMalloc(&d_vup, nm * 8));
+    HIPCHK(hipMalloc(&d_vdn, nm * 8));
+    HIPCHK(hipMalloc(&d_iout, m * 8));
+    HIPCHK(hipMalloc(&d_cnt, sizeof(int)));
+    HIPCHK(hipMemcpy(d_g, g.data(), nm * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_vin, a.v[1].data(), n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_vout, a.v[2].data(), m * 8, hipMemcpyHostToDevice));
+    fadesim_ir2s_desc d{};
+    d.dtype_bytes = 8;
+    d.layout = FADESIM_IR2S_PLAIN;
+    d.d_vin = d_vin;
+    d.d_G = d_g;
+    d.d_vout = d_vout;
+    d.batch_size = d.numX = d.numY = 1;
+    d.row_size = n;
+    d.col_size = m;
+    d.iter_times = iter_times;
+    d.r_wire = r_wire;
+    d.r_source = d.r_sink = r_load;
+    d.divide_by_g = 1;
+    d.carry_residue = 0;
+    d.enable_break = enable_break ? FADESIM_IR2S_BREAK_EACH : 0;
+    d.break_threshold = break_th;
+    d.d_out = d_iout;
+    d.d_vup = d_vup;
+    d.d_vdown = d_vdn;
+    d.d_iters = d_cnt;
+    HIPCHK(hipDeviceSynchronize());
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = fadesim_ir2s(&d, nullptr);
+    if (rc != FADESIM_OK)
+        throw CmdError(std::string("solver: ") + fadesim_last_error());
+    HIPCHK(hipDeviceSynchronize());
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    HIPCHK(hipMemcpy(vup.data(), d_vup, nm * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vdown.data(), d_vdn, nm * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(iout.data(), d_iout, m * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&count, d_cnt, sizeof(int), hipMemcpyDeviceToHost));
+    for (void *p : {(void *)d_g, (void *)d_vin, (void *)d_vout, (void *)d_vup, (void *)d_vdn, (void *)d_iout,
+                    (void *)d_cnt})
+        HIPCHK(hipFree(p));
+    // the V1 "error I sum" (2dArray.cc:1830-1866): every node's KCL with the end
+    // segments at g_wire, the sink at 0 V
+    auto U = [&](int i, int j) { return vup[(size_t)i * m + j]; };
+    auto D = [&](int i, int j) { return vdown[(size_t)i * m + j]; };
+    auto G = [&](int i, int j) { return g[(size_t)i * m + j]; };
+    const std::vector<double> &vin = a.v[1];
+    double err = 0;
+    for (int i = 0; i < n; ++i) {
+        err += std::fabs((vin[i] - U(i, 0) + (m > 1 ? U(i, 1) - U(i, 0) : 0)) * gw + (D(i, 0) - U(i, 0)) * G(i, 0));
+        for (int j = 1; j < m - 1; ++j)
+            err += std::fabs((U(i, j - 1) - U(i, j) + U(i, j + 1) - U(i, j)) * gw + (D(i, j) - U(i, j)) * G(i, j));
+        if (m > 1)
+            err += std::fabs((U(i, m - 2) - U(i, m - 1)) * gw + (D(i, m - 1) - U(i, m - 1)) * G(i, m - 1));
+    }
+    for (int j = 0; j < m; ++j)
+        err += std::fabs((U(0, j) - D(0, j)) * G(0, j) + (n > 1 ? D(1, j) - D(0, j) : 0) * gw);
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 0; j < m; ++j)
+            err += std::fabs((U(i, j) - D(i, j)) * G(i, j) + (D(i - 1, j) - D(i, j) + D(i + 1, j) - D(i, j)) * gw);
+    if (n > 1)
+        for (int j = 0; j < m; ++j)
+            err += std::fabs((0 - D(n - 1, j) + D(n - 2, j) - D(n - 1, j)) * gw + (U(n - 1, j) - D(n - 1, j)) * G(n - 1, j));
+    std::ofstream of("fastmode.out");
+    of << "error I sum = " << err << std::endl;
+    of << "vup = " << std::endl;
+    of << std::scientific << std::setprecision(12);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < m; ++j)
+            of << std::setw(20) << U(i, j);
+        of << std::endl;
+    }
+    of << std::endl << "vdown = " << std::endl;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < m; ++j)
+            of << std::setw(20) << D(i, j);
+        of << std::endl;
+    }
+    of << std::endl << "iout = " << std::endl;
+    for (int j = 0; j < m; ++j)
+        of << std::setw(20) << iout[j];
+    of << std::endl;
+    const int record = enable_break ? count : iter_times;
+    std::cout << "start fast solve" << std::endl << "ir2s fast mode time = " << seconds << " s" << std::endl;
+    std::cout << "ir2s fast iter times = " << record << std::endl;
+    std::cout << "error I sum = " << err << std::endl;
+    std::cout << "fast solve, iter_times = " << iter_times << ", print to file fastmode.out" << std::endl;
+    std::ostringstream os;
+    os << "fastsolve " << iter_times << " (GPU IR2S), iter times = " << record << ", error I sum = " << err;
+    a.log(os.str());
+}
+
 void run_line(Array2D &a, const std::string &line)
 {
     std::istringstream iss(line);
@@ -477,11 +611,14 @@ void run_line(Array2D &a, const std::string &line)
     } else if (cmd == "topString" || cmd == "bottomString" || cmd == "build" || cmd == "capacity" ||
                cmd == "capacitance" || cmd == "setLineV++" || cmd == "simpleWriteOne" ||
                cmd == "simpleWriteZero" || cmd == "simpleReadOut" || cmd == "fastmode" ||
-               cmd == "fastsolve" || cmd.rfind("ir_", 0) == 0) {
+               cmd.rfind("ir_", 0) == 0) {
         os << cmd << ": not handled by this build (SPICE generation and other solvers); ignored";
         std::cout << os.str() << std::endl;
     } else if (cmd == "nodebasedGSMethod") {
         nodebasedGS(a, args);
+        return;
+    } else if (cmd == "fastsolve") {
+        fastsolve(a, args);
         return;
     } else {
         throw CmdError("Unknown command: " + cmd);

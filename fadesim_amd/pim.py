@@ -91,6 +91,7 @@ class PhyArrParams:
         self.nonlinear_params = []            # Ron, Roff, a_on, b_on, a_off, b_off
         self.tol = 1e-6                       # converged mode: relative I_out tolerance
         self.warm_start = None                # converged mode: WarmStartNet or None
+        self.ir2s_iters = 0                   # converged mode: IR2S iterations on each solve's initial guess
         self.max_batch = 65536                # instances per batched solve
         self.stats = {}                       # converged mode: solves, sweeps, Picard steps (accumulated)
         for k, v in kw.items():
@@ -257,6 +258,8 @@ def _solve_converged(bits, G, inputV, P):
     gidx = (((jj * 2 + ss) * numX + xx) * numY + yy).to(torch.int32)
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
     net = P.warm_start
+    k = int(P.ir2s_iters)
+    ir2s = {"ir2s_iters": k, "corr_iters": max(2, k)} if k > 0 else {}
     stats = P.stats  # accumulates over calls until the caller resets it
     for k in ("solves", "sweeps", "picard_steps", "calls"):
         stats.setdefault(k, 0)
@@ -271,12 +274,12 @@ def _solve_converged(bits, G, inputV, P):
         if P.nonlinear_cell_enable:
             warm_fn = (lambda Gz, rw, v: net.predict(Gz, rw, v)) if net is not None else None
             res = solve_batch_nonlinear(slots, rr, vin, params=P.nonlinear_params, gidx=gi, tol=P.tol,
-                                        warm_fn=warm_fn)
+                                        warm_fn=warm_fn, **ir2s)
             stats["sweeps"] += int(res.total_sweeps.sum())
             stats["picard_steps"] = max(stats["picard_steps"], res.outer)
         else:
             warm = net.predict(slots.index_select(0, gi.long()), rr, vin) if net is not None else None
-            res = solve_batch(slots, rr, vin, warm=warm, gidx=gi, tol=P.tol)
+            res = solve_batch(slots, rr, vin, warm=warm, gidx=gi, tol=P.tol, **ir2s)
             stats["sweeps"] += int(res.sweeps.sum())
         stats["calls"] += 1
         if int((res.status != 0).sum()) != 0:

@@ -163,7 +163,7 @@ class RaggedPlan:
 
     def solve(self, g: torch.Tensor, r_wire: torch.Tensor, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
               warm: Optional[torch.Tensor] = None, *, tol: float = 1e-6, safety: float = 2.0,
-              switch_tol: float = 2e-5, max_sweeps: int = 100000, fixed_sweeps: Optional[int] = None,
+              switch_tol: float = 0.0, max_sweeps: int = 100000, fixed_sweeps: Optional[int] = None,
               refine: bool = True, want_field: bool = False, timing: bool = False,
               zero_correction: bool = False, test_first: bool = True,
               out: Optional[RaggedResult] = None, check_abort: bool = True) -> RaggedResult:
@@ -203,7 +203,7 @@ class RaggedPlan:
     def last_ms(self) -> float:
         return float(nat.lib().fadesim_ragged_last_ms(self._h))
 
-    def capture(self, g, r_wire, vin, vout=None, warm=None, *, tol=1e-6, safety=2.0, switch_tol=2e-5,
+    def capture(self, g, r_wire, vin, vout=None, warm=None, *, tol=1e-6, safety=2.0, switch_tol=0.0,
                 max_sweeps=100000, want_field=False, test_first=True):
         """Record the solve (fill kernel + persistent kernel) into a graph.
         Returns (graph, result): ``graph.replay()`` re-solves from the current

@@ -51,7 +51,7 @@ def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
 def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
                 warm: Optional[torch.Tensor] = None, *, gidx: Optional[torch.Tensor] = None,
                 omega: Optional[float] = None,
-                tol: float = 1e-6, safety: float = 2.0, switch_tol: float = 2e-5,
+                tol: float = 1e-6, safety: float = 2.0, switch_tol: float = 0.0,
                 max_sweeps: int = 20000, refine: bool = True, dwr: bool = False,
                 zero_correction: bool = False, ir2s_iters: int = 0, corr_iters: int = 0,
                 ir2s_omega: float = 1.0, test_first: Optional[bool] = None,

@@ -71,7 +71,8 @@ typedef struct fadesim_batch_desc {
     double omega;           /* SOR factor in (0,2); <= 0 selects the built-in rule */
     double tol;             /* target relative I_out error (e.g. 1e-6) */
     double safety;          /* stop when estimate * safety <= tol (>= 1; 0 -> 2) */
-    double switch_tol;      /* estimate at which the refinement step is taken (0 -> 2e-5) */
+    double switch_tol;      /* estimate at which the refinement step is taken
+                             * (0 -> min(1e-3, max(2e-5, 1e3 tol))) */
     int max_sweeps;         /* sweep budget (FIXED_SWEEPS: exact count) */
     unsigned flags;         /* FADESIM_F_* */
     double *d_iout;         /* [B][m] bitline currents (A), FP64 */

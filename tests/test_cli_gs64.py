@@ -1,10 +1,11 @@
-"""The `sim` CLI drop-in and its FP64 solver (fadesim_gs64_solve).
+"""The `sim` CLI drop-in and its FP64 solvers (fadesim_gs64_solve, fadesim_ir2s).
 
 GPU: the FP64 red-black kernel equals its CPU replay (oracle_rb_f64) bit for
 bit, sweep count included; the CLI run on the reference's own configs agrees
 with the reference's golden output to the precision both reach, and writes
-the reference's file formats.  CPU: command parsing and error behaviour
-(no GPU call is reached).
+the reference's file formats; `fastsolve` (IR2S) reproduces the reference
+binary's fastmode.out digits and iteration count.  CPU: command parsing and
+error behaviour (no GPU call is reached).
 """
 import ctypes
 import os
@@ -20,7 +21,8 @@ from conftest import REPO, STATES4, load_golden
 SIM = os.path.join(REPO, "fadesim_amd", "lib", "sim")
 
 
-def _write_cfg(path, n, m, states, cell_R, r, vin, vout, gs_args, v1_names=False, extra=()):
+def _write_cfg(path, n, m, states, cell_R, r, vin, vout, gs_args, v1_names=False, extra=(),
+               cmd="nodebasedGSMethod"):
     L = [f"arraySize {n} {m}", f"line_resistance {r!r}", "setUseLine left -1", "setUseLine down -1",
          "senseBitlineI down -1", "// a comment", "% another", ""]
     L += list(extra)
@@ -33,7 +35,7 @@ def _write_cfg(path, n, m, states, cell_R, r, vin, vout, gs_args, v1_names=False
     for i in range(n):
         for j in range(m):
             L.append(f"setCellR {i} {j} {int(states[i, j])}")
-    L.append("nodebasedGSMethod " + " ".join(str(a) for a in gs_args))
+    L.append(cmd + " " + " ".join(str(a) for a in gs_args))
     with open(path, "w") as f:
         f.write("\n".join(L) + "\n")
 
@@ -167,3 +169,41 @@ def test_cli_matches_reference_golden(cuda, tmp_path, name):
     log = open(tmp_path / "default.log").read()
     assert re.search(r"threshould break, iter times =  \d+", log)
     assert "Duration time = " in log
+
+
+IR2S_CASES = sorted(os.path.basename(p)[:-4] for p in
+                    __import__("glob").glob(os.path.join(REPO, "tests", "golden", "ir2s*.npz")))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", IR2S_CASES)
+def test_cli_fastsolve_matches_reference(tmp_path, name):
+    z = load_golden(name)
+    n, m = int(z["n"]), int(z["m"])
+    r = float(z["RD"][0, 0])
+    rl = float(z["rload_left"])
+    extra = () if np.isnan(rl) else (f"rload left {rl!r}",)
+    args = [int(z["iters"]), 0, int(z["enable_break"]), repr(float(z["tol"]))]
+    cfg = tmp_path / "cfg"
+    _write_cfg(str(cfg), n, m, z["states"], z["cell_R"], r, z["vin"], z["vout"], args, v1_names=True,
+               extra=extra, cmd="fastsolve")
+    out = _run(str(cfg), tmp_path)
+    assert "Error" not in out, out[:400]
+    cnt = int(re.findall(r"ir2s fast iter times = (\d+)", out)[-1])
+    assert cnt == int(z["record"])
+    err, vup, vdn, iout = _parse_out(str(tmp_path / "fastmode.out"), n, m)
+    np.testing.assert_allclose(iout, z["iout"], rtol=1e-12)
+    assert err == pytest.approx(float(z["errsum"]), rel=1e-5, abs=1e-17)
+    if "vup" in z:
+        np.testing.assert_allclose(vup, z["vup"], rtol=1e-12)
+        np.testing.assert_allclose(vdn, z["vdown"], rtol=1e-12, atol=1e-300)
+
+
+def test_cli_fastsolve_rejects_table_mode(tmp_path):
+    if not os.path.exists(SIM):
+        pytest.fail("sim not built")
+    cfg = tmp_path / "c"
+    cfg.write_text("arraySize 2 2\ncellRstates 1 1e6\nline_resistance 1\nfastsolve 3 1\n")
+    assert "initial voltages from a table are not supported" in _run(str(cfg), tmp_path)
+    cfg.write_text("arraySize 2 2\ncellRstates 1 1e6\nline_resistance 1\nfastsolve\n")
+    assert "fastsolve needs iter_times" in _run(str(cfg), tmp_path)

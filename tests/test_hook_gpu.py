@@ -2,7 +2,9 @@
 restatement of split_arr_matmul.cu (oracle.hook_gs): bit-exact for linear
 cells in FP32 and FP64 (fixed sweeps and break), to sinh rounding for the
 nonlinear Picard path; error messages of the reference's TORCH_CHECKs; the
-converged mode against the exact solve."""
+converged mode against the exact solve.  The hook's ir2s (fadesim_amd.hook.ir2s)
+against its CPU restatement (oracle.hook_ir2s) the same way, and its fixed
+point against the exact solve."""
 import numpy as np
 import pytest
 
@@ -81,3 +83,68 @@ def test_hook_converged_mode(cuda):
             _, _, iex = oracle.exact(G[x].astype(np.float64), inp[b, x].astype(np.float64), np.zeros(col),
                                      *oracle.segments_uniform(row, col, 2.93))
             assert np.max(np.abs(got[b, x] - iex) / np.abs(iex)) <= 1e-6
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("iters,brk,retv", [(1, False, False), (7, False, True), (200, True, False)])
+def test_hook_ir2s_linear_bitwise(cuda, dtype, iters, brk, retv):
+    import torch
+    from fadesim_amd.hook import ir2s
+    bsize, row, col, numX, numY = 3, 16, 12, 2, 2
+    inp, G = _case(dtype, bsize, row, col, numX, numY, seed=100 + iters)
+    thr = 1e-12 if dtype == np.float64 else 1e-7
+    out = ir2s(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, numX, numY, iters,
+               2.93, brk, thr, False, [], retv)
+    ref, it = oracle.hook_ir2s(inp, G, bsize, row, col, numX, numY, iters, 2.93, brk, thr, False, [], retv)
+    got = out.cpu().numpy()
+    assert got.dtype == dtype and got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    if brk:
+        assert it < iters  # the break was taken
+
+
+@pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+def test_hook_ir2s_nonlinear(cuda, dtype, rtol):
+    import torch
+    from fadesim_amd.hook import ir2s
+    bsize, row, col, numX, numY = 2, 8, 8, 1, 2
+    rng = np.random.default_rng(5)
+    inp = rng.uniform(0.05, 0.3, (bsize, numX, row)).astype(dtype)
+    Ron, Roff = 1e3, 1e6
+    G = (1.0 / rng.uniform(Ron, 1e4, (numX, row, numY * col))).astype(dtype)
+    a = 2.5
+    nl = [Ron, Roff, a, 1 / (Ron * np.sinh(a)), a, 1 / (Roff * np.sinh(a))]
+    out = ir2s(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, numX, numY, 12,
+               2.93, False, 0.0, True, nl)
+    ref, _ = oracle.hook_ir2s(inp, G, bsize, row, col, numX, numY, 12, 2.93, False, 0.0, True, nl)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=rtol)
+
+
+def test_hook_ir2s_fixed_point_is_exact_solve(cuda):
+    """Enough iterations reach the exact solve of the network with source
+    and sink segments r_wire and grounded bitlines."""
+    import torch
+    from fadesim_amd.hook import ir2s
+    bsize, row, col = 2, 64, 64
+    rng = np.random.default_rng(6)
+    inp = np.where(rng.random((bsize, 1, row)) < 0.5, 0.2, 0.0)
+    G = 1.0 / rng.choice([2e6, 1.6e6, 1.3e6, 1e6], (1, row, col))
+    out = ir2s(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, 1, 1, 40, 3.0,
+               False, 0.0, False, []).cpu().numpy()
+    RU, RD = oracle.segments_uniform(row, col, 3.0)
+    for b in range(bsize):
+        _, _, iex = oracle.exact(G[0], inp[b, 0], np.zeros(col), RU, RD)
+        np.testing.assert_allclose(out[b, 0], iex, rtol=1e-10)
+
+
+def test_hook_ir2s_reference_errors(cuda):
+    import torch
+    from fadesim_amd.hook import ir2s
+    inp, G = _case(np.float32)
+    ti, tG = torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda)
+    with pytest.raises(RuntimeError, match="G must be a CUDA tensor"):
+        ir2s(ti, torch.tensor(G), 3, 16, 12, 2, 2, 10, 2.93, False, 0, False, [])
+    with pytest.raises(RuntimeError, match="row_sizeshould be >=2"):
+        ir2s(ti, tG, 3, 1, 12, 2, 2, 10, 2.93, False, 0, False, [])
+    with pytest.raises(RuntimeError, match="same dtype"):
+        ir2s(ti.double(), tG, 3, 16, 12, 2, 2, 10, 2.93, False, 0, False, [])
