@@ -62,12 +62,17 @@ def algorithmic_bytes(passes_sum, B):
     return 4.0 * M * N * passes_sum + B * (4.0 * M + 8.0 * N)
 
 
-def load_traffic():
+def load_traffic(variant):
+    """HBM bytes per launch of the solve kernel of `variant`, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (scaled by
+    the same counters' reading of a plain device copy), or None."""
     p = os.path.join(REPO, "profiles", "rbsor_128x128_b4096_traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            return json.load(f)
-    return None
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        t = json.load(f)
+    v = t.get("hbm_bytes_per_launch", {})
+    return v.get(variant) if isinstance(v, dict) else v
 
 
 def cpu_baseline(sample_B, seed):
@@ -144,7 +149,8 @@ def bench_mlp(args, dev, net, k_ir2s):
         model = pim.MLP(params=P, seed=0).to(dev).eval()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        logits = model(imgs)
+        with torch.no_grad():
+            logits = model(imgs)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         st = dict(P.stats)
@@ -154,7 +160,8 @@ def bench_mlp(args, dev, net, k_ir2s):
                      "max_picard_steps": st.get("picard_steps")}
     P0 = pim.PhyArrParams(Ron=1e6, Roff=2e6, cellBits=2, arrRowSize=128, arrColSize=128, inputMaxVoltage=0.2,
                           ir_mode="none")
-    ideal = pim.MLP(params=P0, seed=0).to(dev).eval()(imgs)
+    with torch.no_grad():
+        ideal = pim.MLP(params=P0, seed=0).to(dev).eval()(imgs)
     for name in ("ir2s", "nn"):
         if name in res:
             d = (res[name] - res["plain"]).abs()
@@ -227,7 +234,8 @@ def main():
     if net is not None:
         # the network's field, one IR2S iteration on it, then the same test / SOR
         variants["nn"] = (1, lambda timing=False: solve_batch(tg, tr, tv, warm=net.predict(tg, tr, tv), tol=args.tol,
-                                                              ir2s_iters=1, corr_iters=K, timing=timing))
+                                                              ir2s_iters=1, corr_iters=K, test_first=True,
+                                                              timing=timing))
     if only:
         variants = {k: v for k, v in variants.items() if k in only}
 
@@ -258,7 +266,7 @@ def main():
                       "kernel_GBps_algorithmic": ach}
     head = "nn" if "nn" in meas else ("ir2s" if "ir2s" in meas else "plain")
     hm = meas[head]
-    traffic = load_traffic()
+    traffic = load_traffic(head)
     out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU",
            "value": hm["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": hm["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -277,7 +285,7 @@ def main():
         out["ir2s_vs_plain_speedup"] = meas["ir2s"]["solves_per_s"] / meas["plain"]["solves_per_s"]
     out["roofline"] = {"bound": "hbm", "achieved": hm["kernel_GBps_algorithmic"], "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": hm["kernel_GBps_algorithmic"] / HBM_PEAK_GBS,
-                       "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                       "traffic": traffic,
                        "kernel": "rbsor_kernel<Geo<4,8,32,16>>", "kernel_ms": hm["kernel_ms"],
                        "bytes_model": "(sweeps + IR2S iterations) * 4*M*N + 4*M + 8*N per crossbar "
                                       "(conductance streamed once per sweep or wire-solve pass)"}
@@ -299,7 +307,7 @@ def main():
                 rr = solve_batch(tgs, trs, tvs, tol=args.tol, ir2s_iters=K, corr_iters=K)
             else:
                 rr = solve_batch(tgs, trs, tvs, warm=net.predict(tgs, trs, tvs), tol=args.tol, ir2s_iters=1,
-                                 corr_iters=K)
+                                 corr_iters=K, test_first=True)
             errs[name] = float(np.max(np.abs(rr.iout.cpu().numpy() - iex) / np.abs(iex)))
         out["max_rel_err"] = errs
     else:

@@ -41,6 +41,8 @@ class Gs64Desc(ctypes.Structure):
         ("iter_number", ctypes.c_int), ("omega", ctypes.c_double),
         ("enable_break", ctypes.c_int), ("tol", ctypes.c_double),
         ("d_vup", _vp), ("d_vdown", _vp), ("d_iout", _vp), ("d_record", _vp), ("d_sweeps", _vp),
+        ("general", ctypes.c_int), ("d_on_top", _vp), ("d_on_left", _vp), ("d_on_bottom", _vp),
+        ("d_on_right", _vp), ("d_v_top", _vp), ("d_v_right", _vp),
     ]
 
 
@@ -83,7 +85,7 @@ EXPORTED = [
     "fadesim_ragged_plan_create", "fadesim_ragged_plan_destroy", "fadesim_ragged_plan_info",
     "fadesim_ragged_solve", "fadesim_ragged_aborted", "fadesim_ragged_last_ms",
     "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs", "fadesim_ir2s",
-    "fadesim_nn_features", "fadesim_nn_combine", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
+    "fadesim_nn_features", "fadesim_nn_combine", "fadesim_nn_combine_up", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
     "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
     "fadesim_nn_up2_bwd", "fadesim_nn_relu_mask", "fadesim_nn_add", "fadesim_nn_mse", "fadesim_nn_adam",
     "fadesim_nn_cast",
@@ -128,6 +130,7 @@ def _declare_nn(L):
     sigs = {
         "fadesim_nn_features": [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "fadesim_nn_combine": [i, i, i, _vp, _vp, _vp, i, _vp, _vp],
+        "fadesim_nn_combine_up": [i, i, i, _vp, _vp, _vp, i, i, _vp, _vp],
         "fadesim_nn_conv3x3_fwd": [i, i, i, i, i, i, _vp, _vp, _vp, i, _vp],
         "fadesim_nn_flip_weights": [i, i, i, i, _vp, _vp, _vp],
         "fadesim_nn_conv3x3_wgrad": [i, i, i, i, i, i, _vp, _vp, _vp, _vp],

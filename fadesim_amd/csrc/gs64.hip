@@ -14,6 +14,12 @@
 // Stop after the first sweep whose max |x' - x| < tol (concrete_command.h:1256-1272).
 // The readout sums (u-d)*g over ascending rows per column, like the reference.
 //
+// General boundary (array2D::nodeBasedGeneralGS, crossbarHspice/2dArray.cc:
+// 1209-1374): any edge driven (voltage source through its end segment) or
+// open, per line; initial state u = v_left on driven rows (0 elsewhere),
+// d = 0; readout the sink wire current (d_n-1,j - v_bottom_j) / RD[j][n] of
+// driven bottoms, 0 elsewhere.  The same kernel with per-node end terms.
+//
 // One workgroup; fields and coefficients live in the workspace (L2 resident
 // for the array sizes the CLI handles).
 #include <hip/hip_runtime.h>
@@ -34,6 +40,9 @@ struct GsArgs {
     const double *g, *ru, *rd;
     int ru_ld, rd_ld;
     const double *vin, *vout;
+    int general;                              // general-boundary network (nodeBasedGeneralGS)
+    const int *on_top, *on_left, *on_bottom, *on_right;
+    const double *vtop, *vright;
     int iter_number;
     double w;
     int enable_break;
@@ -52,16 +61,19 @@ __global__ void __launch_bounds__(kGsThreads) gs64_kernel(GsArgs A)
     double *u = A.u, *d = A.d;
     __shared__ double wred[kGsThreads / 64];
 
+    const bool gen = A.general != 0;
     for (size_t k = t; k < nm; k += kGsThreads) {
         const int i = (int)(k / m), j = (int)(k % m);
-        aL[k] = 1.0 / A.ru[(size_t)i * A.ru_ld + j];
-        aR[k] = (j < m - 1) ? 1.0 / A.ru[(size_t)i * A.ru_ld + j + 1] : 0.0;
-        bU[k] = (i > 0) ? 1.0 / A.rd[(size_t)j * A.rd_ld + i] : 0.0;
-        bD[k] = 1.0 / A.rd[(size_t)j * A.rd_ld + i + 1];
+        const bool onL = gen ? A.on_left[i] != 0 : true, onR = gen ? A.on_right[i] != 0 : false;
+        const bool onT = gen ? A.on_top[j] != 0 : false, onB = gen ? A.on_bottom[j] != 0 : true;
+        aL[k] = (j > 0 || onL) ? 1.0 / A.ru[(size_t)i * A.ru_ld + j] : 0.0;
+        aR[k] = (j < m - 1 || onR) ? 1.0 / A.ru[(size_t)i * A.ru_ld + j + 1] : 0.0;
+        bU[k] = (i > 0 || onT) ? 1.0 / A.rd[(size_t)j * A.rd_ld + i] : 0.0;
+        bD[k] = (i < n - 1 || onB) ? 1.0 / A.rd[(size_t)j * A.rd_ld + i + 1] : 0.0;
         wDu[k] = A.w / ((aL[k] + aR[k]) + A.g[k]);
         wDd[k] = A.w / ((bU[k] + bD[k]) + A.g[k]);
-        u[k] = A.vin[i];
-        d[k] = A.vout[j];
+        u[k] = gen ? (onL ? A.vin[i] : 0.0) : A.vin[i];
+        d[k] = gen ? 0.0 : A.vout[j];
     }
     __syncthreads();
     const double omw = 1.0 - A.w;
@@ -73,13 +85,13 @@ __global__ void __launch_bounds__(kGsThreads) gs64_kernel(GsArgs A)
                 const int i = (int)(k / m), j = (int)(k % m);
                 if (((i + j) & 1) == c) {
                     const double L = j ? u[k - 1] : A.vin[i];
-                    const double R = (j < m - 1) ? u[k + 1] : 0.0;
+                    const double R = (j < m - 1) ? u[k + 1] : (gen ? A.vright[i] : 0.0);
                     const double num = (aL[k] * L + aR[k] * R) + A.g[k] * d[k];
                     const double nv = wDu[k] * num + omw * u[k];
                     mx = fmax(mx, fabs(nv - u[k]));
                     u[k] = nv;
                 } else {
-                    const double U = i ? d[k - m] : 0.0;
+                    const double U = i ? d[k - m] : (gen ? A.vtop[j] : 0.0);
                     const double Dn = (i < n - 1) ? d[k + m] : A.vout[j];
                     const double num = (bU[k] * U + bD[k] * Dn) + A.g[k] * u[k];
                     const double nv = wDd[k] * num + omw * d[k];
@@ -111,6 +123,11 @@ __global__ void __launch_bounds__(kGsThreads) gs64_kernel(GsArgs A)
         A.vdown[k] = d[k];
     }
     for (int j = t; j < m; j += kGsThreads) {
+        if (gen) { // the sink wire current of a driven bottom (2dArray.cc:1351-1356)
+            const size_t k = (size_t)(n - 1) * m + j;
+            A.iout[j] = A.on_bottom[j] ? (d[k] - A.vout[j]) / A.rd[(size_t)j * A.rd_ld + n] : 0.0;
+            continue;
+        }
         double s = 0.0;
         for (int i = 0; i < n; ++i) {
             const size_t k = (size_t)i * m + j;
@@ -142,6 +159,9 @@ extern "C" int fadesim_gs64_solve(const fadesim_gs64_desc *d, void *stream)
         return set_error(FADESIM_E_INVALID, "ru_ld must be >= cols+1 and rd_ld >= rows+1");
     if (d->iter_number < 0)
         return set_error(FADESIM_E_INVALID, "iter_number must be >= 0");
+    if (d->general && (!d->d_on_top || !d->d_on_left || !d->d_on_bottom || !d->d_on_right || !d->d_v_top ||
+                       !d->d_v_right))
+        return set_error(FADESIM_E_INVALID, "the general boundary needs all edge flags and voltages");
     const size_t nm = (size_t)d->rows * d->cols;
     double *ws = (double *)workspace(8 * nm * sizeof(double));
     if (!ws)
@@ -156,6 +176,13 @@ extern "C" int fadesim_gs64_solve(const fadesim_gs64_desc *d, void *stream)
     a.rd_ld = d->rd_ld;
     a.vin = d->d_vin;
     a.vout = d->d_vout;
+    a.general = d->general;
+    a.on_top = d->d_on_top;
+    a.on_left = d->d_on_left;
+    a.on_bottom = d->d_on_bottom;
+    a.on_right = d->d_on_right;
+    a.vtop = d->d_v_top;
+    a.vright = d->d_v_right;
     a.iter_number = d->iter_number;
     a.w = d->omega;
     a.enable_break = d->enable_break;

@@ -210,6 +210,36 @@ __global__ void combine_kernel(const float *__restrict__ base, const float *__re
     }
 }
 
+// warm[b][c][i][j] = base + scale_b * bilinear(out[b][.][.][c]) for a network
+// output f = 2^level times coarser than the crossbar (M/f x N/f): sample point
+// ((i + 1/2)/f - 1/2, (j + 1/2)/f - 1/2), clamped to the coarse grid.
+__global__ void combine_up_kernel(const float *__restrict__ base, const float *__restrict__ scale,
+                                  const __bf16 *__restrict__ out, int Cout, int B, int M, int N, int level,
+                                  float *__restrict__ warm)
+{
+    const int f = 1 << level, Hc = M >> level, Wc = N >> level;
+    const float inv = 1.0f / (float)f;
+    const size_t MN = (size_t)M * N;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < (size_t)B * MN; k += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = k / MN, id = k % MN;
+        const int i = (int)(id / N), j = (int)(id % N);
+        const float y = fminf(fmaxf(((float)i + 0.5f) * inv - 0.5f, 0.0f), (float)(Hc - 1));
+        const float x = fminf(fmaxf(((float)j + 0.5f) * inv - 0.5f, 0.0f), (float)(Wc - 1));
+        const int y0 = (int)y, x0 = (int)x;
+        const int y1 = min(y0 + 1, Hc - 1), x1 = min(x0 + 1, Wc - 1);
+        const float wy = y - (float)y0, wx = x - (float)x0;
+        const __bf16 *o = out + b * (size_t)Hc * Wc * Cout;
+        const float s = scale[b];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float v00 = (float)o[((size_t)y0 * Wc + x0) * Cout + c], v01 = (float)o[((size_t)y0 * Wc + x1) * Cout + c];
+            const float v10 = (float)o[((size_t)y1 * Wc + x0) * Cout + c], v11 = (float)o[((size_t)y1 * Wc + x1) * Cout + c];
+            const float top = v00 + wx * (v01 - v00), bot = v10 + wx * (v11 - v10);
+            warm[b * 2 * MN + c * MN + id] = base[b * 2 * MN + c * MN + id] + s * (top + wy * (bot - top));
+        }
+    }
+}
+
 // ---------------------------------------------------------------- conv forward
 struct ConvArgs {
     int B, H, W, Cin, Cout, Kp;
@@ -661,6 +691,19 @@ extern "C" int fadesim_nn_combine(int B, int M, int N, const float *d_base, cons
     const size_t n = (size_t)B * M * N;
     hipLaunchKernelGGL(combine_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, d_base, d_scale,
                        (const __bf16 *)d_out, cout, B, M * N, d_warm);
+    NN_LAUNCH_CHECK();
+    return FADESIM_OK;
+}
+
+extern "C" int fadesim_nn_combine_up(int B, int M, int N, const float *d_base, const float *d_scale,
+                                     const void *d_out, int cout, int level, float *d_warm, void *stream)
+{
+    if (B < 0 || cout < 2 || level < 0 || level > 6 || !d_base || !d_scale || !d_out || !d_warm ||
+        (M >> level) < 1 || (N >> level) < 1 || ((M >> level) << level) != M || ((N >> level) << level) != N)
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_combine_up: bad arguments");
+    const size_t n = (size_t)B * M * N;
+    hipLaunchKernelGGL(combine_up_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, d_base, d_scale,
+                       (const __bf16 *)d_out, cout, B, M, N, level, d_warm);
     NN_LAUNCH_CHECK();
     return FADESIM_OK;
 }

@@ -887,11 +887,46 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
 // snapshot planes 3, 4 of the workspace.  The LDS pointers are passed one by
 // one (a struct of LDS addresses passed by value would become a constant
 // with address-space casts, which the backend cannot emit).
+// Uniform values reach a device function in VGPRs; moving them to SGPRs
+// (readfirstlane) lets the compiler keep addresses as scalar base + 32-bit
+// per-lane offset instead of 64-bit per-lane pointers.
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double uniform_f64(double x)
+{
+    const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ int uniform_i32(int x) { return (int)__builtin_amdgcn_readfirstlane((unsigned)x); }
+__device__ __forceinline__ void uniform_inst(Inst &I)
+{
+    I.g = uniform_ptr(I.g);
+    I.vin = uniform_ptr(I.vin);
+    I.vout = uniform_ptr(I.vout);
+    I.rwire = uniform_f64(I.rwire);
+    I.rows = uniform_i32(I.rows);
+}
+
 #define RB_SH_PARAMS                                                                                    \
     float *sh_halo, double *sh_S1, double *sh_Ifloor, double *sh_colI, double *sh_colD, float *sh_wmax, \
         int *sh_dead, double *sh_cbuf, double *sh_cext
 #define RB_SH_ARGS(s) s.halo, s.S1, s.Ifloor, s.colI, s.colD, s.wmax, s.dead, s.cbuf, s.cext
-#define RB_SH_LOCAL Shared sh{sh_halo, sh_S1, sh_Ifloor, sh_colI, sh_colD, sh_wmax, sh_dead, sh_cbuf, sh_cext}
+#define RB_SH_LOCAL                                                                                     \
+    Shared sh{uniform_ptr(sh_halo), uniform_ptr(sh_S1),   uniform_ptr(sh_Ifloor), uniform_ptr(sh_colI),     \
+              uniform_ptr(sh_colD), uniform_ptr(sh_wmax), uniform_ptr(sh_dead),   uniform_ptr(sh_cbuf),     \
+              uniform_ptr(sh_cext)};                                                                         \
+    uniform_inst(I);                                                                                         \
+    wsb = uniform_ptr(wsb);                                                                                  \
+    gp = uniform_ptr(gp);                                                                                    \
+    gi0 = uniform_i32(gi0)
 template <class G, bool STRIP>
 __device__ __attribute__((noinline)) bool phase_setup(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
                                                       int i0, int j0, int t, int ty, double rwire, RB_SH_PARAMS,

@@ -133,6 +133,150 @@ std::vector<int> sample(Array2D &a, const std::vector<double> &p, size_t len)
             throw CmdError(std::string("HIP error: ") + hipGetErrorString(r_));    \
     } while (0)
 
+// V1's dispatcher (crossbarHspice/2dArray.cc:84-110): a driven top or right
+// edge, or an undriven left or bottom line, needs the general solver.
+bool general_boundary(const Array2D &a)
+{
+    for (int x : a.bv[0])
+        if (x == 1)
+            return true;
+    for (int x : a.bv[1])
+        if (x == 0)
+            return true;
+    for (int x : a.bv[2])
+        if (x == 0)
+            return true;
+    for (int x : a.bv[3])
+        if (x == 1)
+            return true;
+    return false;
+}
+
+// The general-boundary solve (array2D::nodeBasedGeneralGS, 2dArray.cc:1209-1374)
+// on the GPU through fadesim_gs64_solve(general = 1); output
+// fastmode_nodebasedGeneralGS.out in V1's format (precision 6, width 14).
+void generalGS(Array2D &a, const std::vector<double> &g, int iter_time, double w, bool enable_break, double tol,
+               std::ostringstream &os)
+{
+    const int n = a.n, m = a.m;
+    const size_t nm = (size_t)n * m;
+    std::vector<double> ru((size_t)n * (m + 1)), rd((size_t)m * (n + 1));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= m; ++j)
+            ru[(size_t)i * (m + 1) + j] = a.RU[i][j];
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i <= n; ++i)
+            rd[(size_t)j * (n + 1) + i] = a.RD[j][i];
+    std::vector<double> vup(nm), vdown(nm), iout(m);
+    double *d_g, *d_ru, *d_rd, *d_v[4], *d_vup, *d_vdn, *d_iout;
+    int *d_on[4], *d_int;
+    HIPCHK(hipMalloc(&d_g, nm * 8));
+    HIPCHK(hipMalloc(&d_ru, ru.size() * 8));
+    HIPCHK(hipMalloc(&d_rd, rd.size() * 8));
+    for (int e = 0; e < 4; ++e) {
+        const size_t len = (e & 1) ? n : m;
+        HIPCHK(hipMalloc(&d_v[e], len * 8));
+        HIPCHK(hipMalloc(&d_on[e], len * sizeof(int)));
+        HIPCHK(hipMemcpy(d_v[e], a.v[e].data(), len * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_on[e], a.bv[e].data(), len * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&d_vup, nm * 8));
+    HIPCHK(hipMalloc(&d_vdn, nm * 8));
+    HIPCHK(hipMalloc(&d_iout, m * 8));
+    HIPCHK(hipMalloc(&d_int, 2 * sizeof(int)));
+    HIPCHK(hipMemcpy(d_g, g.data(), nm * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ru, ru.data(), ru.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_rd, rd.data(), rd.size() * 8, hipMemcpyHostToDevice));
+    fadesim_gs64_desc d{};
+    d.rows = n;
+    d.cols = m;
+    d.d_g = d_g;
+    d.d_ru = d_ru;
+    d.ru_ld = m + 1;
+    d.d_rd = d_rd;
+    d.rd_ld = n + 1;
+    d.d_vin = d_v[1];
+    d.d_vout = d_v[2];
+    d.iter_number = iter_time;
+    d.omega = (double)(float)w; // V1 reads omega with stof
+    d.enable_break = enable_break;
+    d.tol = tol;
+    d.d_vup = d_vup;
+    d.d_vdown = d_vdn;
+    d.d_iout = d_iout;
+    d.d_record = d_int;
+    d.d_sweeps = d_int + 1;
+    d.general = 1;
+    d.d_on_top = d_on[0];
+    d.d_on_left = d_on[1];
+    d.d_on_bottom = d_on[2];
+    d.d_on_right = d_on[3];
+    d.d_v_top = d_v[0];
+    d.d_v_right = d_v[3];
+    HIPCHK(hipDeviceSynchronize());
+    auto t0 = std::chrono::steady_clock::now();
+    if (fadesim_gs64_solve(&d, nullptr) != FADESIM_OK)
+        throw CmdError(std::string("solver: ") + fadesim_last_error());
+    HIPCHK(hipDeviceSynchronize());
+    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int hi[2];
+    HIPCHK(hipMemcpy(vup.data(), d_vup, nm * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(vdown.data(), d_vdn, nm * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(iout.data(), d_iout, m * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hi, d_int, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    for (void *p : {(void *)d_g, (void *)d_ru, (void *)d_rd, (void *)d_vup, (void *)d_vdn, (void *)d_iout,
+                    (void *)d_int})
+        HIPCHK(hipFree(p));
+    for (int e = 0; e < 4; ++e) {
+        HIPCHK(hipFree(d_v[e]));
+        HIPCHK(hipFree(d_on[e]));
+    }
+    std::cout << "evalutate ir drop using general GS-method, w = " << (float)w
+              << " , print to file fastmode_nodebasedGeneralGS.out" << std::endl;
+    std::cout << "time = " << seconds << " s" << std::endl;
+    if (hi[0] != -1)
+        std::cout << "threshould break, iter times =  " << hi[0] << std::endl;
+    // the general solve's "error I sum" (2dArray.cc:1316-1345): interior
+    // wordline nodes and bitline rows 0 .. n-2
+    const double gw = 1.0 / a.RD[0][0];
+    auto U = [&](int i, int j) { return vup[(size_t)i * m + j]; };
+    auto D = [&](int i, int j) { return vdown[(size_t)i * m + j]; };
+    auto G = [&](int i, int j) { return g[(size_t)i * m + j]; };
+    double err = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 1; j < m - 1; ++j)
+            err += std::fabs((U(i, j - 1) - U(i, j) + U(i, j + 1) - U(i, j)) * gw + (D(i, j) - U(i, j)) * G(i, j));
+        if (m > 1)
+            err += std::fabs((U(i, m - 2) - U(i, m - 1)) * gw + (D(i, m - 1) - U(i, m - 1)) * G(i, m - 1));
+    }
+    for (int j = 0; j < m; ++j)
+        err += std::fabs((U(0, j) - D(0, j)) * G(0, j) + (n > 1 ? D(1, j) - D(0, j) : 0) * gw);
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 0; j < m; ++j)
+            err += std::fabs((U(i, j) - D(i, j)) * G(i, j) + (D(i - 1, j) - D(i, j) + D(i + 1, j) - D(i, j)) * gw);
+    std::cout << "error I sum = " << err << std::endl;
+    std::ofstream of("fastmode_nodebasedGeneralGS.out");
+    of << "error I sum = " << err << std::endl;
+    of << "vup = " << std::endl;
+    of << std::scientific << std::setprecision(6);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < m; ++j)
+            of << std::setw(14) << U(i, j);
+        of << std::endl;
+    }
+    of << std::endl << "vdown = " << std::endl;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < m; ++j)
+            of << std::setw(14) << D(i, j);
+        of << std::endl;
+    }
+    of << std::endl << "iout = " << std::endl;
+    for (int j = 0; j < m; ++j)
+        of << std::setw(14) << iout[j];
+    of << std::endl;
+    os << "general GS (GPU, red-black), sweeps = " << hi[1] << ", error I sum = " << err << std::endl;
+}
+
 void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
 {
     if (args.size() < 3)
@@ -292,6 +436,8 @@ void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
     of << std::endl;
     std::cout << "evaluate ir drop using GS-method (GPU, red-black), w = " << w << ", sweeps = " << sweeps
               << ", print to file fastmode_nodebasedGS.out" << std::endl;
+    if (general_boundary(a))
+        generalGS(a, g, iter_time, w, enable_break, tol, os);
     a.log(os.str());
 }
 

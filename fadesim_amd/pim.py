@@ -259,7 +259,7 @@ def _solve_converged(bits, G, inputV, P):
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
     net = P.warm_start
     k = int(P.ir2s_iters)
-    ir2s = {"ir2s_iters": k, "corr_iters": max(2, k)} if k > 0 else {}
+    ir2s = {"ir2s_iters": k, "corr_iters": max(2, k), "test_first": True} if k > 0 else {}
     stats = P.stats  # accumulates over calls until the caller resets it
     for k in ("solves", "sweeps", "picard_steps", "calls"):
         stats.setdefault(k, 0)

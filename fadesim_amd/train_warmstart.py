@@ -7,8 +7,9 @@ Data: seeded instances of the 4096 x 128x128 benchmark distribution (4 cell
 states, r ~ U[1,5] ohm, vin in {0, 0.2} V), seeds [1e6, 1e6 + n); targets are
 the drop fields of fadesim_solve_batch converged to 1e-8.  Evaluation on
 held-out seeds [1e7, 1e7 + eval): mean sweeps to 1e-6 and solve throughput
-with the zero-drop guess, the second-order physics field alone, and the
-network -- printed and written as JSON next to the weights.
+with the zero-drop guess, the second-order physics field alone, the network,
+the network followed by one wire-solve (IR2S) iteration, and four IR2S
+iterations alone -- printed and written as JSON next to the weights.
 """
 from __future__ import annotations
 
@@ -51,6 +52,13 @@ def evaluate(net, n_eval, dev, log=print):
                                 "solves_per_s": n_eval / tph}
     nn, tn = timed(lambda: solve_batch(g, r, vin, warm=net.predict(g, r, vin), tol=1e-6))
     res["nn"] = {"mean_sweeps": nn.sweeps.double().mean().item(), "seconds": tn, "solves_per_s": n_eval / tn}
+    # the network's field refined by one wire-solve (IR2S) iteration and tested before any sweep
+    nn1, tn1 = timed(lambda: solve_batch(g, r, vin, warm=net.predict(g, r, vin), tol=1e-6, ir2s_iters=1,
+                                         corr_iters=4, test_first=True))
+    res["nn_ir2s1"] = {"mean_sweeps": nn1.sweeps.double().mean().item(), "seconds": tn1,
+                       "solves_per_s": n_eval / tn1}
+    ir4, ti4 = timed(lambda: solve_batch(g, r, vin, tol=1e-6, ir2s_iters=4, corr_iters=4))
+    res["ir2s4"] = {"mean_sweeps": ir4.sweeps.double().mean().item(), "seconds": ti4, "solves_per_s": n_eval / ti4}
     w = net.predict(g, r, vin)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -58,11 +66,11 @@ def evaluate(net, n_eval, dev, log=print):
         net.predict(g, r, vin)
     torch.cuda.synchronize()
     res["nn_inference_seconds"] = (time.perf_counter() - t0) / 3
-    for k in ("plain", "physics_2nd_order", "nn"):
+    for k in ("plain", "physics_2nd_order", "nn", "nn_ir2s1", "ir2s4"):
         log(f"eval {k}: mean sweeps {res[k]['mean_sweeps']:.1f}, {res[k]['solves_per_s']:.0f} solves/s")
     log(f"eval nn inference: {1e3 * res['nn_inference_seconds']:.2f} ms for {n_eval}")
     res["n_eval"] = n_eval
-    res["statuses_ok"] = bool(int((plain.status != 0).sum() + (phys.status != 0).sum() + (nn.status != 0).sum()) == 0)
+    res["statuses_ok"] = bool(int(sum((x.status != 0).sum() for x in (plain, phys, nn, nn1, ir4))) == 0)
     del w
     return res
 

@@ -109,7 +109,34 @@ def _slim() -> List[Op]:
     ]
 
 
-ARCHITECTURES = {"unet16": _unet16, "slim": _slim}
+def _coarse32() -> List[Op]:
+    """The same encoder-decoder moved to a 4x coarser grid: the features are
+    average-pooled to 32x32 and the network (32/64/64 channels at 32x32,
+    16x16, 8x8) predicts the correction there; predict() upsamples it
+    bilinearly (fadesim_nn_combine_up).  The correction is smooth on the
+    scale of a few cells, and one wire-solve iteration on the warm field
+    restores the per-cell detail: ~1/5 of the multiply-adds of unet16 and
+    no full-resolution convolution."""
+    return [
+        Op("p1", "pool", "x", 1, N_FEAT, N_FEAT),
+        Op("p2", "pool", "p1", 2, N_FEAT, N_FEAT),
+        Op("e2a", "conv", "p2", 2, N_FEAT, 32),
+        Op("e2", "conv", "e2a", 2, 32, 32),
+        Op("p3", "pool", "e2", 3, 32, 32),
+        Op("e3a", "conv", "p3", 3, 32, 64),
+        Op("e3", "conv", "e3a", 3, 64, 64),
+        Op("p4", "pool", "e3", 4, 64, 64),
+        Op("e4", "conv", "p4", 4, 64, 64),
+        Op("t4", "conv", "e4", 4, 64, 64),
+        Op("u3", "up_add", "t4", 3, 64, 64, skip="e3"),
+        Op("t3", "conv", "u3", 3, 64, 32),
+        Op("u2", "up_add", "t3", 2, 32, 32, skip="e2"),
+        Op("d2", "conv", "u2", 2, 32, 32),
+        Op("out", "conv", "d2", 2, 32, OUT_C, relu=False),
+    ]
+
+
+ARCHITECTURES = {"unet16": _unet16, "slim": _slim, "coarse32": _coarse32}
 
 
 def architecture(name: str = "unet16") -> List[Op]:
@@ -156,6 +183,11 @@ class WarmStartNet:
     def _view(self, flat, op):
         o = self.offsets[op.name]
         return flat[o:o + op.cout * kpad(op.cin)].view(op.cout, kpad(op.cin))
+
+    @property
+    def out_level(self) -> int:
+        """Resolution level of the network output (0 = the crossbar's)."""
+        return next(op.level for op in self.ops if op.name == "out")
 
     def n_params(self) -> int:
         """Trainable parameters (weights and biases, padding excluded)."""
@@ -259,10 +291,18 @@ class WarmStartNet:
             b1 = min(B, b0 + chunk)
             feat, base, scale = self.features(g[b0:b1], r[b0:b1], vin[b0:b1])
             out = self.forward(feat)
-            _chk(nat.lib().fadesim_nn_combine(b1 - b0, self.M, self.N, ctypes.c_void_p(base.data_ptr()),
-                                              ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                              OUT_C, ctypes.c_void_p(warm[b0:b1].data_ptr()), _s(self.dev)),
-                 "nn_combine")
+            lv = self.out_level
+            if lv == 0:
+                _chk(nat.lib().fadesim_nn_combine(b1 - b0, self.M, self.N, ctypes.c_void_p(base.data_ptr()),
+                                                  ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                                  OUT_C, ctypes.c_void_p(warm[b0:b1].data_ptr()), _s(self.dev)),
+                     "nn_combine")
+            else:
+                _chk(nat.lib().fadesim_nn_combine_up(b1 - b0, self.M, self.N, ctypes.c_void_p(base.data_ptr()),
+                                                     ctypes.c_void_p(scale.data_ptr()),
+                                                     ctypes.c_void_p(out.data_ptr()), OUT_C, lv,
+                                                     ctypes.c_void_p(warm[b0:b1].data_ptr()), _s(self.dev)),
+                     "nn_combine_up")
         return warm
 
     # ------------------------------------------------------------------ backward
@@ -322,7 +362,8 @@ class WarmStartNet:
 
     # ------------------------------------------------------------------ training
     def train(self, data, epochs=12, batch=64, lr=2e-3, wd=0.0, log=print, seed=0, max_seconds=None):
-        """data: dict with feat [P][M][N][8] bf16, target [P][2][M][N] FP32 (units of scale)."""
+        """data: dict with feat [P][M][N][8] bf16, target [P][2][M'][N'] FP32 (units of scale) at the
+        network's output resolution M' = M >> out_level."""
         L = nat.lib()
         s = _s(self.dev)
         P = data["feat"].shape[0]
@@ -348,7 +389,8 @@ class WarmStartNet:
                 out = acts["out"]
                 dout = torch.empty_like(out)
                 loss.zero_()
-                _chk(L.fadesim_nn_mse(batch, self.M, self.N, ctypes.c_void_p(out.data_ptr()), OUT_C,
+                _chk(L.fadesim_nn_mse(batch, self.M >> self.out_level, self.N >> self.out_level,
+                                      ctypes.c_void_p(out.data_ptr()), OUT_C,
                                       ctypes.c_void_p(tgt.data_ptr()), ctypes.c_void_p(dout.data_ptr()),
                                       ctypes.c_void_p(loss.data_ptr()), s), "mse")
                 grad.zero_()
@@ -400,6 +442,9 @@ def generate_dataset(net: WarmStartNet, n_instances: int, seed0: int, chunk: int
             raise RuntimeError("training-data solve did not converge")
         feat, base, scale = net.features(g, r, vin)
         tgt = (res.field - base) / scale.view(B, 1, 1, 1)
+        if net.out_level > 0:  # the network predicts block means on its coarse grid
+            f = 1 << net.out_level
+            tgt = torch.nn.functional.avg_pool2d(tgt, f)
         feats.append(feat)
         targets.append(tgt)
     return {"feat": torch.cat(feats), "target": torch.cat(targets)}

@@ -216,6 +216,14 @@ typedef struct fadesim_gs64_desc {
     double *d_iout;        /* [m] out: sum_i (vup - vdown) g */
     int *d_record;         /* out: 0-based sweep of the break, or -1 */
     int *d_sweeps;         /* out: sweeps run */
+    int general;           /* 1: general boundary, replacing array2D::nodeBasedGeneralGS
+                            * (crossbarHspice/2dArray.cc:1209-1374): each line end driven
+                            * or open by the flags below; initial u = vin on driven rows
+                            * (0 elsewhere), d = 0; d_iout = sink wire currents
+                            * (d_n-1,j - vout_j) / rd[j][n] of driven bottoms, 0 elsewhere */
+    const int *d_on_top, *d_on_left, *d_on_bottom, *d_on_right; /* general: [m], [n], [m], [n] */
+    const double *d_v_top;   /* general: [m] top drive voltages (left = d_vin, bottom = d_vout) */
+    const double *d_v_right; /* general: [n] right drive voltages */
 } fadesim_gs64_desc;
 
 int fadesim_gs64_solve(const fadesim_gs64_desc *desc, void *stream);

@@ -33,6 +33,12 @@ int fadesim_nn_features(int B, int M, int N, const float *d_g, const double *d_r
 int fadesim_nn_combine(int B, int M, int N, const float *d_base, const float *d_scale, const void *d_out, int cout,
                        float *d_warm, void *stream);
 
+/* warm = base + scale * bilinear(out[..., 0:2]) for a network output 2^level
+ * times coarser than the crossbar (out [B][M>>level][N>>level][cout]) ->
+ * [B][2][M][N] FP32; sample points at the fine cells' centres. */
+int fadesim_nn_combine_up(int B, int M, int N, const float *d_base, const float *d_scale, const void *d_out,
+                          int cout, int level, float *d_warm, void *stream);
+
 /* 3x3 same convolution (MFMA), optional ReLU: y [B][H][W][Cout] */
 int fadesim_nn_conv3x3_fwd(int B, int H, int W, int Cin, int Cout, int Kp, const void *d_x, const void *d_w, void *d_y,
                            int relu, void *stream);

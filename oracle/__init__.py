@@ -65,6 +65,16 @@ def lib():
         L.oracle_rb_f64.restype = i
         L.oracle_rb_f32.argtypes = [i, i, _fp, d, _dp, _dp, d, i, _fp, _fp]
         L.oracle_rb_f32.restype = None
+        L.oracle_exact_general.argtypes = [i, i, _dp, i, _dp, i, _dp] + [_ip] * 4 + [_dp] * 4 + [_dp, _dp, _dp]
+        L.oracle_exact_general.restype = i
+        L.oracle_nodegs_general.argtypes = ([i, i, _dp, i, _dp, i, _dp] + [_ip] * 4 + [_dp] * 4 +
+                                            [i, d, i, d, _dp, _dp, _dp, _ip])
+        L.oracle_nodegs_general.restype = i
+        L.oracle_rb_f64_general.argtypes = ([i, i, _dp, i, _dp, i, _dp] + [_ip] * 4 + [_dp] * 4 +
+                                            [i, d, i, d, _dp, _dp, _dp, _ip])
+        L.oracle_rb_f64_general.restype = i
+        L.oracle_errisum_general.argtypes = [i, i, d, _dp, _dp, _dp]
+        L.oracle_errisum_general.restype = d
         L.oracle_ir2s.argtypes = [i, i, _dp, _dp, _dp, d, d, i, i, d, _dp, _dp, _dp]
         L.oracle_ir2s.restype = i
         for name, fp in (("oracle_hook_ir2s_f64", _dp), ("oracle_hook_ir2s_f32", _fp)):
@@ -276,3 +286,66 @@ def hook_ir2s(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, r_
             float(r_wire), int(bool(enable_break)), float(break_threshould), int(bool(nonlinear_cell_enable)),
             _pd(nl), int(bool(ret_V)), pp(out))
     return out, it
+
+
+def _edges(n, m, on, v):
+    """on / v: four arrays (top [m], left [n], bottom [m], right [n])."""
+    sizes = (m, n, m, n)
+    on_c = [np.ascontiguousarray(np.asarray(on[e], dtype=np.int32).reshape(sizes[e])) for e in range(4)]
+    v_c = [_d(np.asarray(v[e], dtype=np.float64).reshape(sizes[e])) for e in range(4)]
+    return on_c, v_c
+
+
+def exact_general(g, RU, RD, on, v):
+    """Exact solve of the general-boundary network.  Returns (u, d, iout) with
+    iout the sink wire currents of driven bottoms (2dArray.cc:1353-1356)."""
+    g = _d(g)
+    n, m = g.shape
+    RU, RD = _d(RU), _d(RD)
+    on_c, v_c = _edges(n, m, on, v)
+    u, dn, io = np.empty((n, m)), np.empty((n, m)), np.empty(m)
+    rc = lib().oracle_exact_general(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                                    *[a.ctypes.data_as(_ip) for a in on_c], *[_pd(a) for a in v_c],
+                                    _pd(u), _pd(dn), _pd(io))
+    if rc == -3:
+        raise np.linalg.LinAlgError("singular network (a floating line)")
+    if rc:
+        raise MemoryError("oracle_exact_general")
+    return u, dn, io
+
+
+def nodegs_general(g, RU, RD, on, v, iters, w, enable_break=False, tol=0.0):
+    """nodeBasedGeneralGS restated.  Returns (vup, vdown, iout, record)."""
+    g = _d(g)
+    n, m = g.shape
+    RU, RD = _d(RU), _d(RD)
+    on_c, v_c = _edges(n, m, on, v)
+    vu, vd, io = np.empty((n, m)), np.empty((n, m)), np.empty(m)
+    rec = ctypes.c_int(0)
+    lib().oracle_nodegs_general(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                                *[a.ctypes.data_as(_ip) for a in on_c], *[_pd(a) for a in v_c],
+                                int(iters), float(w), int(bool(enable_break)), float(tol), _pd(vu), _pd(vd),
+                                _pd(io), ctypes.byref(rec))
+    return vu, vd, io, rec.value
+
+
+def errisum_general(g, vup, vdown, g_wire):
+    g = _d(g)
+    n, m = g.shape
+    return lib().oracle_errisum_general(n, m, float(g_wire), _pd(g), _pd(_d(vup)), _pd(_d(vdown)))
+
+
+def rb_f64_general(g, RU, RD, on, v, iters, w, enable_break=False, tol=0.0):
+    """FP64 red-black replay of the GPU general-boundary solve.  Returns
+    (vup, vdown, iout, record, sweeps)."""
+    g = _d(g)
+    n, m = g.shape
+    RU, RD = _d(RU), _d(RD)
+    on_c, v_c = _edges(n, m, on, v)
+    vu, vd, io = np.empty((n, m)), np.empty((n, m)), np.empty(m)
+    rec = ctypes.c_int(0)
+    sw = lib().oracle_rb_f64_general(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                                     *[a.ctypes.data_as(_ip) for a in on_c], *[_pd(a) for a in v_c],
+                                     int(iters), float(w), int(bool(enable_break)), float(tol), _pd(vu), _pd(vd),
+                                     _pd(io), ctypes.byref(rec))
+    return vu, vd, io, rec.value, sw

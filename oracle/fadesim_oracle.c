@@ -27,6 +27,14 @@
  *   oracle_hook_gs_*   the PimTorch hook's line-GS ordering
  *                      (pimtorch/.../SplitArrCuda/split_arr_matmul.cu:27-140,204-333),
  *                      linear and sinh-nonlinear cells, FP32 and FP64.
+ *   oracle_nodegs_general  general-boundary lexicographic SOR, a statement-by-
+ *                      statement restatement of array2D::nodeBasedGeneralGS
+ *                      (crossbarHspice/2dArray.cc:1209-1374): every edge driven
+ *                      or open; pinned against golden vectors of the
+ *                      reference `sim_v1` binary (tests/golden/).
+ *   oracle_exact_general  exact FP64 solve of the general-boundary network
+ *                      (the same banded Cholesky); pinned against scipy and
+ *                      against oracle_exact where the two networks coincide.
  *   oracle_ir2s        the IR2S wire-solve iteration, a statement-by-statement
  *                      restatement of ir_drop_fastSolve_cpu_singleMat
  *                      (crossbarHspice/ir_drop_fast_solver.h:24-163), behind
@@ -195,8 +203,27 @@ double oracle_errisum(int n, int m, double g_wire, const double *g,
 /* Assembles the banded SPD matrix (lower band, width kd = 2m) of the network
  * and factors it in place (A = L L^T).  Unknown order: u_ij at 2(i m + j),
  * d_ij at 2(i m + j) + 1.  Returns 0, or -3 if the matrix is not SPD. */
+/* Edge drives of the general-boundary network: on[e] flags (NULL = the
+ * nodeBaseGS network: left and bottom driven, top and right open), e = 0 top
+ * (bitline ends above row 0, [m]), 1 left ([n]), 2 bottom ([m]), 3 right ([n]). */
+static int edge_on(const int *const *on, int e, int k)
+{
+    if (!on || !on[e])
+        return e == 1 || e == 2;
+    return on[e][k] != 0;
+}
+
+static int band_factor_general(int n, int m, const double *RU, int ru_ld, const double *RD,
+                               int rd_ld, const double *g, const int *const *on, double *L);
+
 static int band_factor(int n, int m, const double *RU, int ru_ld, const double *RD,
                        int rd_ld, const double *g, double *L)
+{
+    return band_factor_general(n, m, RU, ru_ld, RD, rd_ld, g, NULL, L);
+}
+
+static int band_factor_general(int n, int m, const double *RU, int ru_ld, const double *RD,
+                               int rd_ld, const double *g, const int *const *on, double *L)
 {
     const long N = 2L * n * m;
     const long kd = 2L * m;
@@ -208,13 +235,13 @@ static int band_factor(int n, int m, const double *RU, int ru_ld, const double *
         for (int j = 0; j < m; ++j) {
             double gij = g[IDX(i, j, m)];
             long ku = 2L * (i * (long)m + j), kdn = ku + 1;
-            double aj = 1.0 / RUL(i, j);
-            double aj1 = (j < m - 1) ? 1.0 / RUL(i, j + 1) : 0.0;
+            double aj = (j > 0 || edge_on(on, 1, i)) ? 1.0 / RUL(i, j) : 0.0;
+            double aj1 = (j < m - 1 || edge_on(on, 3, i)) ? 1.0 / RUL(i, j + 1) : 0.0;
             BAND(ku, ku) = gij + aj + aj1;
             if (j > 0)
                 BAND(ku, ku - 2) = -aj;
-            double bi = (i > 0) ? 1.0 / RDL(j, i) : 0.0;
-            double bi1 = 1.0 / RDL(j, i + 1);
+            double bi = (i > 0 || edge_on(on, 0, j)) ? 1.0 / RDL(j, i) : 0.0;
+            double bi1 = (i < n - 1 || edge_on(on, 2, j)) ? 1.0 / RDL(j, i + 1) : 0.0;
             BAND(kdn, kdn) = gij + bi + bi1;
             BAND(kdn, ku) = -gij;
             if (i > 0)
@@ -285,6 +312,58 @@ static void band_rhs(int n, int m, const double *RU, int ru_ld, const double *RD
     if (vout)
         for (int j = 0; j < m; ++j)
             b[2L * ((n - 1) * (long)m + j) + 1] += (1.0 / RD[(size_t)j * rd_ld + n]) * vout[j];
+}
+
+/* Exact FP64 solve of the general-boundary network (nodeBasedGeneralGS,
+ * crossbarHspice/2dArray.cc:1209-1374): edge e driven where on[e][k] != 0, at
+ * the voltage v[e][k] through its end segment (top RD[j][0], left RU[i][0],
+ * bottom RD[j][n], right RU[i][m]), open otherwise.  Readout as the reference:
+ * iout_j = (d_{n-1,j} - v_bottom_j) / RD[j][n] for driven bottoms, else 0.
+ * Returns 0, -2 (memory) or -3 (singular: a floating line). */
+int oracle_exact_general(int n, int m, const double *RU, int ru_ld, const double *RD, int rd_ld, const double *g,
+                         const int *on_top, const int *on_left, const int *on_bottom, const int *on_right,
+                         const double *v_top, const double *v_left, const double *v_bottom, const double *v_right,
+                         double *u_out, double *d_out, double *iout)
+{
+    const int *on[4] = {on_top, on_left, on_bottom, on_right};
+    const long N = 2L * n * m;
+    const long W = 2L * m + 1;
+    double *L = (double *)calloc((size_t)N * W, sizeof(double));
+    double *b = (double *)calloc((size_t)N, sizeof(double));
+    if (!L || !b) {
+        free(L);
+        free(b);
+        return -2;
+    }
+    int e = band_factor_general(n, m, RU, ru_ld, RD, rd_ld, g, on, L);
+    if (e) {
+        free(L);
+        free(b);
+        return e;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (on_left[i])
+            b[2L * (i * (long)m)] += (1.0 / RU[(size_t)i * ru_ld]) * v_left[i];
+        if (on_right[i])
+            b[2L * (i * (long)m + m - 1)] += (1.0 / RU[(size_t)i * ru_ld + m]) * v_right[i];
+    }
+    for (int j = 0; j < m; ++j) {
+        if (on_top[j])
+            b[2L * j + 1] += (1.0 / RD[(size_t)j * rd_ld]) * v_top[j];
+        if (on_bottom[j])
+            b[2L * ((n - 1) * (long)m + j) + 1] += (1.0 / RD[(size_t)j * rd_ld + n]) * v_bottom[j];
+    }
+    band_solve(n, m, L, b);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            u_out[IDX(i, j, m)] = b[2 * IDX(i, j, m)];
+            d_out[IDX(i, j, m)] = b[2 * IDX(i, j, m) + 1];
+        }
+    for (int j = 0; j < m; ++j)
+        iout[j] = on_bottom[j] ? (d_out[IDX(n - 1, j, m)] - v_bottom[j]) / RD[(size_t)j * rd_ld + n] : 0.0;
+    free(L);
+    free(b);
+    return 0;
 }
 
 int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
@@ -429,11 +508,15 @@ int oracle_exact_batch_f32g(int B, int n, int m, const float *g32,
  * concrete_command.h:1256-1272).  iout_j = sum_{i ascending} (u-d)*g.
  * Returns sweeps performed; *record = 0-based break sweep or -1.
  */
-int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
-                  int rd_ld, const double *g, const double *vin,
-                  const double *vout, int iter_number, double w,
-                  int enable_break, double tol, double *u, double *d,
-                  double *iout, int *record_out)
+/* The FP64 red-black replay with general edges: e = top, left, bottom, right
+ * driven where on[e][k] != 0 at v[e][k] (NULL on: the nodeBaseGS network).
+ * general = 1 takes nodeBasedGeneralGS's initial state (u = v_left on driven
+ * rows, else 0; d = 0) and readout (sink wire currents of driven bottoms);
+ * general = 0 the nodeBaseGS ones (u = vin, d = vout; sum (u - d) g). */
+static int rb_f64_impl(int n, int m, const double *RU, int ru_ld, const double *RD, int rd_ld, const double *g,
+                       const int *const *on, const double *vt, const double *vin, const double *vout,
+                       const double *vr, int general, int iter_number, double w, int enable_break, double tol,
+                       double *u, double *d, double *iout, int *record_out)
 {
     size_t nm = (size_t)n * m;
     double *wDu = (double *)malloc(nm * sizeof(double));
@@ -445,14 +528,14 @@ int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < m; ++j) {
             size_t k = IDX(i, j, m);
-            aL[k] = 1.0 / RU[(size_t)i * ru_ld + j];
-            aR[k] = (j < m - 1) ? 1.0 / RU[(size_t)i * ru_ld + j + 1] : 0.0;
-            bU[k] = (i > 0) ? 1.0 / RD[(size_t)j * rd_ld + i] : 0.0;
-            bD[k] = 1.0 / RD[(size_t)j * rd_ld + i + 1];
+            aL[k] = (j > 0 || edge_on(on, 1, i)) ? 1.0 / RU[(size_t)i * ru_ld + j] : 0.0;
+            aR[k] = (j < m - 1 || edge_on(on, 3, i)) ? 1.0 / RU[(size_t)i * ru_ld + j + 1] : 0.0;
+            bU[k] = (i > 0 || edge_on(on, 0, j)) ? 1.0 / RD[(size_t)j * rd_ld + i] : 0.0;
+            bD[k] = (i < n - 1 || edge_on(on, 2, j)) ? 1.0 / RD[(size_t)j * rd_ld + i + 1] : 0.0;
             wDu[k] = w / ((aL[k] + aR[k]) + g[k]);
             wDd[k] = w / ((bU[k] + bD[k]) + g[k]);
-            u[k] = vin[i];
-            d[k] = vout[j];
+            u[k] = general ? (edge_on(on, 1, i) ? vin[i] : 0.0) : vin[i];
+            d[k] = general ? 0.0 : vout[j];
         }
     double omw = 1.0 - w;
     int record = -1, it;
@@ -464,13 +547,13 @@ int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
                     size_t k = IDX(i, j, m);
                     if (((i + j) & 1) == c) {
                         double L = j ? u[k - 1] : vin[i];
-                        double R = (j < m - 1) ? u[k + 1] : 0.0;
+                        double R = (j < m - 1) ? u[k + 1] : (vr ? vr[i] : 0.0);
                         double num = (aL[k] * L + aR[k] * R) + g[k] * d[k];
                         double nv = wDu[k] * num + omw * u[k];
                         mx = fmax(mx, fabs(nv - u[k]));
                         u[k] = nv;
                     } else {
-                        double U = i ? d[k - m] : 0.0;
+                        double U = i ? d[k - m] : (vt ? vt[j] : 0.0);
                         double Dn = (i < n - 1) ? d[k + m] : vout[j];
                         double num = (bU[k] * U + bD[k] * Dn) + g[k] * u[k];
                         double nv = wDd[k] * num + omw * d[k];
@@ -485,10 +568,14 @@ int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
         }
     }
     for (int j = 0; j < m; ++j) {
-        double s = 0;
-        for (int i = 0; i < n; ++i)
-            s += (u[IDX(i, j, m)] - d[IDX(i, j, m)]) * g[IDX(i, j, m)];
-        iout[j] = s;
+        if (general) {
+            iout[j] = edge_on(on, 2, j) ? (d[IDX(n - 1, j, m)] - vout[j]) / RD[(size_t)j * rd_ld + n] : 0.0;
+        } else {
+            double s = 0;
+            for (int i = 0; i < n; ++i)
+                s += (u[IDX(i, j, m)] - d[IDX(i, j, m)]) * g[IDX(i, j, m)];
+            iout[j] = s;
+        }
     }
     *record_out = record;
     free(wDu);
@@ -498,6 +585,27 @@ int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
     free(bU);
     free(bD);
     return it;
+}
+
+int oracle_rb_f64(int n, int m, const double *RU, int ru_ld, const double *RD,
+                  int rd_ld, const double *g, const double *vin,
+                  const double *vout, int iter_number, double w,
+                  int enable_break, double tol, double *u, double *d,
+                  double *iout, int *record_out)
+{
+    return rb_f64_impl(n, m, RU, ru_ld, RD, rd_ld, g, NULL, NULL, vin, vout, NULL, 0, iter_number, w, enable_break,
+                       tol, u, d, iout, record_out);
+}
+
+int oracle_rb_f64_general(int n, int m, const double *RU, int ru_ld, const double *RD, int rd_ld, const double *g,
+                          const int *on_top, const int *on_left, const int *on_bottom, const int *on_right,
+                          const double *v_top, const double *v_left, const double *v_bottom, const double *v_right,
+                          int iter_number, double w, int enable_break, double tol, double *u, double *d,
+                          double *iout, int *record_out)
+{
+    const int *on[4] = {on_top, on_left, on_bottom, on_right};
+    return rb_f64_impl(n, m, RU, ru_ld, RD, rd_ld, g, on, v_top, v_left, v_bottom, v_right, 1, iter_number, w,
+                       enable_break, tol, u, d, iout, record_out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -942,4 +1050,98 @@ int oracle_hook_ir2s_f32(const float *input, const float *G, int bsize, int row,
                          const double *nl, int ret_v, float *out)
 {
     HOOK_IR2S_BODY(float, sinhf, fabsf)
+}
+
+
+/* nodeBasedGeneralGS (crossbarHspice/2dArray.cc:1209-1374), statement by
+ * statement: lexicographic SOR over (i, j) updating u_ij then d_ij, edges
+ * driven or open per on[e] with the voltages v[e] (e = top, left, bottom,
+ * right), double-buffered iterates (the new left / upper neighbour, the old
+ * right / lower one), initial u = v_left on driven rows (0 elsewhere), d = 0;
+ * break after the first sweep whose max |change| < tol.  Outputs the final
+ * buffers, the reference's readout (sink wire currents of driven bottoms)
+ * and the 0-based break sweep (-1 without a break).  Returns 0. */
+int oracle_nodegs_general(int n, int m, const double *RU, int ru_ld, const double *RD, int rd_ld, const double *g,
+                          const int *on_top, const int *on_left, const int *on_bottom, const int *on_right,
+                          const double *v_top, const double *v_left, const double *v_bottom, const double *v_right,
+                          int iters, double w, int enable_break, double tol, double *vup, double *vdown,
+                          double *iout, int *record)
+{
+    const size_t nm = (size_t)n * m;
+    double *U[2], *D[2];
+    U[0] = (double *)malloc(nm * sizeof(double));
+    U[1] = (double *)malloc(nm * sizeof(double));
+    D[0] = (double *)calloc(nm, sizeof(double));
+    D[1] = (double *)calloc(nm, sizeof(double));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j)
+            U[0][IDX(i, j, m)] = U[1][IDX(i, j, m)] = on_left[i] ? v_left[i] : 0.0;
+#define RUL(i, j) RU[(size_t)(i) * ru_ld + (j)]
+#define RDL(j, i) RD[(size_t)(j) * rd_ld + (i)]
+    int rec = -1;
+    for (int it = 0; it < iters; ++it) {
+        const int x = it & 1, y = 1 - x;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < m; ++j) {
+                const size_t k = IDX(i, j, m);
+                double sumg = g[k] + ((j != 0 || on_left[i] == 1) ? 1.0 / RUL(i, j) : 0) +
+                              ((j != m - 1 || on_right[i] == 1) ? 1.0 / RUL(i, j + 1) : 0);
+                double l_side = (j == 0 ? (on_left[i] == 0 ? 0 : v_left[i]) : U[x][k - 1]) / RUL(i, j);
+                double r_side = (j == m - 1 ? (on_right[i] == 0 ? 0 : v_right[i]) : U[y][k + 1]) / RUL(i, j + 1);
+                double other = D[y][k] * g[k];
+                U[x][k] = w * (l_side + r_side + other) / sumg + (1 - w) * U[y][k];
+                sumg = g[k] + ((i != 0 || on_top[j] == 1) ? 1.0 / RDL(j, i) : 0) +
+                       ((i != n - 1 || on_bottom[j] == 1) ? 1.0 / RDL(j, i + 1) : 0);
+                l_side = (i == 0 ? (on_top[j] == 0 ? 0 : v_top[j]) : D[x][k - m]) / RDL(j, i);
+                r_side = (i == n - 1 ? (on_bottom[j] == 0 ? 0 : v_bottom[j]) : D[y][k + m]) / RDL(j, i + 1);
+                other = U[x][k] * g[k];
+                D[x][k] = w * (l_side + r_side + other) / sumg + (1 - w) * D[y][k];
+            }
+        if (enable_break) {
+            double mx = 0;
+            for (size_t k = 0; k < nm; ++k) {
+                mx = fmax(mx, fabs(D[x][k] - D[y][k]));
+                mx = fmax(mx, fabs(U[x][k] - U[y][k]));
+            }
+            if (mx < tol) {
+                rec = it;
+                break;
+            }
+        }
+    }
+    const int fin = rec != -1 ? (rec & 1) : 1 - (iters & 1);
+    memcpy(vup, U[fin], nm * sizeof(double));
+    memcpy(vdown, D[fin], nm * sizeof(double));
+    for (int j = 0; j < m; ++j)
+        iout[j] = on_bottom[j] == 1 ? (vdown[IDX(n - 1, j, m)] - v_bottom[j]) / RDL(j, n) : 0.0;
+#undef RUL
+#undef RDL
+    *record = rec;
+    free(U[0]); free(U[1]); free(D[0]); free(D[1]);
+    return 0;
+}
+
+/* The general solve's "error I sum" (2dArray.cc:1316-1345): interior wordline
+ * nodes j >= 1 and bitline rows 0 .. n-2, wire conductance 1/RD[0][0]. */
+double oracle_errisum_general(int n, int m, double g_wire, const double *g, const double *vup, const double *vdown)
+{
+    double e = 0;
+#define U(i, j) vup[IDX(i, j, m)]
+#define D(i, j) vdown[IDX(i, j, m)]
+#define G(i, j) g[IDX(i, j, m)]
+    for (int i = 0; i < n; ++i) {
+        for (int j = 1; j < m - 1; ++j)
+            e += fabs((U(i, j - 1) - U(i, j) + U(i, j + 1) - U(i, j)) * g_wire + (D(i, j) - U(i, j)) * G(i, j));
+        if (m > 1)
+            e += fabs((U(i, m - 2) - U(i, m - 1)) * g_wire + (D(i, m - 1) - U(i, m - 1)) * G(i, m - 1));
+    }
+    for (int j = 0; j < m; ++j)
+        e += fabs((U(0, j) - D(0, j)) * G(0, j) + (n > 1 ? D(1, j) - D(0, j) : 0) * g_wire);
+    for (int i = 1; i < n - 1; ++i)
+        for (int j = 0; j < m; ++j)
+            e += fabs((U(i, j) - D(i, j)) * G(i, j) + (D(i - 1, j) - D(i, j) + D(i + 1, j) - D(i, j)) * g_wire);
+#undef U
+#undef D
+#undef G
+    return e;
 }

@@ -141,6 +141,53 @@ def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=
           f"t={dur}", flush=True)
 
 
+DIRS = ("up", "left", "down", "right")
+
+
+def run_general(name, n, m, states, cell_R, r, on, v, gs_args):
+    """A general-boundary case through V1 (nodeBasedGeneralGS, chosen by the
+    dispatcher when the top or right edge is driven or the left or bottom edge
+    is not): per-edge setUseLine / setNotUseLine and setLineV."""
+    if ONLY and not name.startswith(ONLY):
+        return
+    exe = os.path.join(REF_BIN, "sim_v1")
+    L = ["// generated by tests/golden/make_golden.py", f"arraySize {n} {m}", "selector no", f"line_resistance {float(r)!r}"]
+    for e, dname in enumerate(DIRS):
+        L.append(f"setNotUseLine {dname} -1")
+        idx = [k for k in range(len(on[e])) if on[e][k]]
+        if idx:
+            L.append(f"setUseLine {dname} " + " ".join(str(k) for k in idx))
+        for k, x in enumerate(v[e]):
+            L.append(f"setLineV {dname} {k} {float(x)!r}")
+    L.append("senseBitlineI down -1")
+    L.append(f"cellRstates {len(cell_R)} " + " ".join(repr(float(x)) for x in cell_R))
+    for i in range(n):
+        for j in range(m):
+            L.append(f"setCellR {i} {j} {int(states[i, j])}")
+    L.append("nodebasedGSMethod " + " ".join(str(a) for a in gs_args))
+    with tempfile.TemporaryDirectory() as td:
+        cfg = os.path.join(td, "cfg")
+        with open(cfg, "w") as f:
+            f.write("\n".join(L) + "\n")
+        res = subprocess.run([exe, cfg], cwd=td, capture_output=True, text=True, timeout=3600)
+        text = res.stdout
+        if "using general GS-method" not in text:
+            raise RuntimeError(f"{name}: the reference did not take the general solver:\n{text[-400:]}")
+        text = text.split("using general GS-method", 1)[1]
+        errsum, vup, vdn, iout, iout_txt = parse_out(os.path.join(td, "fastmode_nodebasedGeneralGS.out"), n, m)
+    mrec = re.findall(r"threshould break, iter times =\s+(\d+)", text)
+    record = int(mrec[0]) if mrec else -1
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), version=1, n=n, m=m, states=states.astype(np.uint8),
+                        cell_R=np.array(cell_R, float), r=float(r),
+                        on=np.array([np.pad(np.asarray(x, int), (0, max(n, m) - len(x))) for x in on]),
+                        v=np.array([np.pad(np.asarray(x, float), (0, max(n, m) - len(x))) for x in v]),
+                        iters=int(gs_args[0]), omega_text=str(gs_args[1]),
+                        enable_break=int(len(gs_args) > 3 and gs_args[3] == "yes"),
+                        tol=float(gs_args[4]) if len(gs_args) > 4 else 0.0, record=record, errsum=errsum,
+                        vup=vup, vdown=vdn, iout=iout, iout_txt=np.array(iout_txt), command="general")
+    print(f"{name}: v1 general {n}x{m} record={record} iout[0]={iout[0]:.6e}", flush=True)
+
+
 def main():
     global ONLY
     if len(sys.argv) > 2 and sys.argv[1] == "--only":
@@ -215,6 +262,47 @@ def main():
         vout = list(np.round(rng.uniform(0, 0.05, m), 3))
         run_case(f"ir2s{idx}_v1_{n}x{m}", 1, n, m, st, STATES4, r, vin, vout, args,
                  solve_cmd="fastsolve", rload_left=rl)
+    # General boundaries (V1 nodeBasedGeneralGS): which edges drive, and at what voltage.
+    gen_cases = [
+        (8, 8, "both_sides", [100000, 1.9, 0, "yes", "1e-12"]),
+        (6, 10, "two_sinks", [57, 1.7, 0]),
+        (12, 7, "write_scheme", [100000, 1.85, 0, "yes", "1e-11"]),
+        (16, 16, "all_driven", [100000, 1.9, 0, "yes", "1e-12"]),
+        (5, 4, "open_bottoms", [200, 1.5, 0]),
+    ]
+    for idx, (n, m, kind, args) in enumerate(gen_cases):
+        rng = np.random.default_rng(900 + idx)
+        st = rng.integers(0, 4, (n, m))
+        on = [np.zeros(m, int), np.ones(n, int), np.ones(m, int), np.zeros(n, int)]
+        v = [np.zeros(m), np.round(rng.uniform(0, 1, n), 3), np.zeros(m), np.zeros(n)]
+        if kind == "both_sides":
+            on[3][:] = 1
+            v[3] = np.round(rng.uniform(0, 1, n), 3)
+        elif kind == "two_sinks":
+            on[0][:] = 1
+            v[0] = np.round(rng.uniform(0, 0.05, m), 3)
+        elif kind == "write_scheme":
+            on[1][:] = 0
+            on[1][[2, 5]] = 1
+            v[1][[2, 5]] = [1.2, 0.6]
+            on[3][[2, 9]] = 1
+            v[3][[2, 9]] = [0.6, 0.3]
+            on[2][:] = 0
+            on[2][[1, 4]] = 1
+            v[2][[1, 4]] = [0.0, 0.6]
+            on[0][[4]] = 1
+            v[0][[4]] = [0.3]
+        elif kind == "all_driven":
+            for e in range(4):
+                on[e][:] = 1
+            v = [np.round(rng.uniform(0, 0.1, m), 3), np.round(rng.uniform(0.5, 1, n), 3),
+                 np.round(rng.uniform(0, 0.1, m), 3), np.round(rng.uniform(0.5, 1, n), 3)]
+        elif kind == "open_bottoms":
+            on[2][[1, 3]] = 0
+            on[0][[1, 3]] = 1
+            v[0][[1, 3]] = [0.05, 0.0]
+        run_general(f"gen{idx}_v1_{n}x{m}", n, m, st, STATES4, 2.93, on, v, args)
+
     rng = np.random.default_rng(777)
     st = rng.integers(0, 4, (128, 128))
     vin = list(np.where(rng.random(128) < 0.5, 0.2, 0.0))

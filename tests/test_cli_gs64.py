@@ -207,3 +207,90 @@ def test_cli_fastsolve_rejects_table_mode(tmp_path):
     assert "initial voltages from a table are not supported" in _run(str(cfg), tmp_path)
     cfg.write_text("arraySize 2 2\ncellRstates 1 1e6\nline_resistance 1\nfastsolve\n")
     assert "fastsolve needs iter_times" in _run(str(cfg), tmp_path)
+
+
+# ------------------------------------------------------------------ general boundary
+GEN_CASES = sorted(os.path.basename(p)[:-4] for p in
+                   __import__("glob").glob(os.path.join(REPO, "tests", "golden", "gen*.npz")))
+DIRS = ("up", "left", "down", "right")
+
+
+def _gen(z):
+    n, m = int(z["n"]), int(z["m"])
+    sizes = (m, n, m, n)
+    on = [z["on"][e][:sizes[e]].astype(np.int32) for e in range(4)]
+    v = [z["v"][e][:sizes[e]].astype(np.float64) for e in range(4)]
+    g = 1.0 / z["cell_R"][z["states"]]
+    r = float(z["r"])
+    return n, m, g, r, np.full((n, m + 1), r), np.full((m, n + 1), r), on, v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", GEN_CASES)
+@pytest.mark.parametrize("iters,brk,tol", [(37, False, 0.0), (100000, True, 1e-12)])
+def test_gs64_general_bitwise_vs_replay(cuda, name, iters, brk, tol):
+    import torch
+    from fadesim_amd import _native as nat
+    n, m, g, r, RU, RD, on, v = _gen(load_golden(name))
+    T = lambda a, dt=torch.float64: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=cuda)
+    dg, dru, drd = T(g), T(RU), T(RD)
+    dv = [T(x) for x in v]
+    don = [T(x, torch.int32) for x in on]
+    vup = torch.empty((n, m), dtype=torch.float64, device=cuda)
+    vdn = torch.empty_like(vup)
+    iout = torch.empty(m, dtype=torch.float64, device=cuda)
+    ints = torch.zeros(2, dtype=torch.int32, device=cuda)
+    d = nat.Gs64Desc()
+    d.rows, d.cols = n, m
+    d.d_g, d.d_ru, d.ru_ld, d.d_rd, d.rd_ld = dg.data_ptr(), dru.data_ptr(), m + 1, drd.data_ptr(), n + 1
+    d.d_vin, d.d_vout = dv[1].data_ptr(), dv[2].data_ptr()
+    d.iter_number, d.omega, d.enable_break, d.tol = iters, 1.9, int(brk), tol
+    d.d_vup, d.d_vdown, d.d_iout = vup.data_ptr(), vdn.data_ptr(), iout.data_ptr()
+    d.d_record, d.d_sweeps = ints.data_ptr(), ints.data_ptr() + 4
+    d.general = 1
+    d.d_on_top, d.d_on_left, d.d_on_bottom, d.d_on_right = (x.data_ptr() for x in don)
+    d.d_v_top, d.d_v_right = dv[0].data_ptr(), dv[3].data_ptr()
+    nat.check(nat.lib().fadesim_gs64_solve(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(cuda))), "gs64")
+    torch.cuda.synchronize()
+    u, dd, io, rec, sweeps = oracle.rb_f64_general(g, RU, RD, on, v, iters, 1.9, brk, tol)
+    assert ints.cpu().tolist() == [rec, sweeps]
+    assert np.array_equal(vup.cpu().numpy(), u)
+    assert np.array_equal(vdn.cpu().numpy(), dd)
+    assert np.array_equal(iout.cpu().numpy(), io)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", GEN_CASES)
+def test_cli_general_boundary(cuda, tmp_path, name):
+    """V1's dispatcher: a general boundary also runs the general solver and
+    writes fastmode_nodebasedGeneralGS.out.  Where the reference converged
+    (break taken) the CLI agrees with its printed digits; otherwise the CLI's
+    red-black iterates are those of the CPU replay."""
+    z = load_golden(name)
+    n, m, g, r, RU, RD, on, v = _gen(z)
+    L = [f"arraySize {n} {m}", f"line_resistance {r!r}"]
+    for e, dname in enumerate(DIRS):
+        L.append(f"setNotUseLine {dname} -1")
+        idx = [k for k in range(len(on[e])) if on[e][k]]
+        if idx:
+            L.append(f"setUseLine {dname} " + " ".join(str(k) for k in idx))
+        for k, x in enumerate(v[e]):
+            L.append(f"setLineV {dname} {k} {float(x)!r}")
+    L.append(f"cellRstates {len(z['cell_R'])} " + " ".join(repr(float(x)) for x in z["cell_R"]))
+    for i in range(n):
+        for j in range(m):
+            L.append(f"setCellR {i} {j} {int(z['states'][i, j])}")
+    args = [int(z["iters"]), z["omega_text"], 0] + (["yes", repr(float(z["tol"]))] if int(z["enable_break"]) else [])
+    L.append("nodebasedGSMethod " + " ".join(str(a) for a in args))
+    (tmp_path / "cfg").write_text("\n".join(L) + "\n")
+    out = _run(str(tmp_path / "cfg"), tmp_path)
+    assert "using general GS-method" in out, out[-400:]
+    err, vup, vdn, iout = _parse_out(str(tmp_path / "fastmode_nodebasedGeneralGS.out"), n, m)
+    w = float(np.float32(float(z["omega_text"])))
+    u, dd, io, rec, sw = oracle.rb_f64_general(g, RU, RD, on, v, int(z["iters"]), w, bool(z["enable_break"]),
+                                               float(z["tol"]))
+    np.testing.assert_allclose(iout, io, rtol=1e-6, atol=1e-15)   # printed with 7 digits
+    np.testing.assert_allclose(vup, u, rtol=1e-6, atol=1e-12)
+    if int(z["record"]) != -1:  # the reference converged: same answer to its printed digits
+        np.testing.assert_allclose(iout, z["iout"], rtol=2e-6, atol=1e-15)
+        np.testing.assert_allclose(vup, z["vup"], rtol=2e-6, atol=1e-12)

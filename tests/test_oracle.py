@@ -8,6 +8,10 @@
 * the hook line-GS restatement against a literal per-node Python replay of
   split_arr_matmul.cu:27-140 on tiny arrays, and its nonlinear fixed point
   against a brute-force Newton solve;
+* the general-boundary restatement (nodeBasedGeneralGS) against golden vectors
+  of the reference V1 binary (identical break sweep, printed digits), the
+  general-boundary exact solve against scipy's sparse LU, and the FP64
+  red-black replay against the exact solve;
 * the IR2S restatement against golden vectors of the reference V1 binary's
   `fastsolve` command (printed digits identical, identical iteration count),
   the drop-form IR2S (checker of the GPU's in-kernel iteration) against it,
@@ -25,8 +29,9 @@ import oracle
 from conftest import GOLDEN, STATES4, load_golden
 
 _ALL = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
-GOLDEN_FILES = [p for p in _ALL if not os.path.basename(p).startswith("ir2s")]
+GOLDEN_FILES = [p for p in _ALL if not os.path.basename(p).startswith(("ir2s", "gen"))]
 IR2S_FILES = [p for p in _ALL if os.path.basename(p).startswith("ir2s")]
+GEN_FILES = [p for p in _ALL if os.path.basename(p).startswith("gen")]
 
 
 def _case_inputs(z):
@@ -464,3 +469,87 @@ def test_ir2s_contraction_scales_with_r_g_n2():
             vals.append(coupling_radius(g, r) / (r * g.mean() * n * n))
     assert max(vals) / min(vals) < 1.25
     assert 0.5 < np.mean(vals) < 1.1
+
+
+def _gen_inputs(z):
+    n, m = int(z["n"]), int(z["m"])
+    g = 1.0 / z["cell_R"][z["states"]]
+    r = float(z["r"])
+    sizes = (m, n, m, n)
+    on = [z["on"][e][:sizes[e]] for e in range(4)]
+    v = [z["v"][e][:sizes[e]] for e in range(4)]
+    return n, m, g, r, np.full((n, m + 1), r), np.full((m, n + 1), r), on, v
+
+
+@pytest.mark.parametrize("path", GEN_FILES, ids=[os.path.basename(p)[:-4] for p in GEN_FILES])
+def test_general_gs_matches_reference_binary(path):
+    z = np.load(path)
+    n, m, g, r, RU, RD, on, v = _gen_inputs(z)
+    w = float(np.float32(float(z["omega_text"])))  # V1 parses omega with stof
+    vup, vdn, iout, rec = oracle.nodegs_general(g, RU, RD, on, v, int(z["iters"]), w, bool(z["enable_break"]),
+                                                float(z["tol"]))
+    assert rec == int(z["record"])
+    assert (np.array(["%.6e" % x for x in iout]) == z["iout_txt"]).all()
+    np.testing.assert_allclose(vup, z["vup"], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(vdn, z["vdown"], rtol=1e-6, atol=1e-12)
+    assert oracle.errisum_general(g, vup, vdn, 1.0 / r) == pytest.approx(float(z["errsum"]), rel=1e-5)
+
+
+def _sparse_general(g, RU, RD, on, v):
+    """The general-boundary KCL system assembled literally, node by node."""
+    n, m = g.shape
+    idx = lambda i, j, t: 2 * (i * m + j) + t
+    A = sp.lil_matrix((2 * n * m, 2 * n * m))
+    b = np.zeros(2 * n * m)
+    for i in range(n):
+        for j in range(m):
+            ku, kd = idx(i, j, 0), idx(i, j, 1)
+            A[ku, ku] += g[i, j]; A[ku, kd] -= g[i, j]; A[kd, kd] += g[i, j]; A[kd, ku] -= g[i, j]
+            for jj, a in ((j - 1, 1 / RU[i, j]), (j + 1, 1 / RU[i, j + 1])):
+                if 0 <= jj < m:
+                    A[ku, ku] += a; A[ku, idx(i, jj, 0)] -= a
+                elif (jj < 0 and on[1][i]) or (jj >= m and on[3][i]):
+                    A[ku, ku] += a; b[ku] += a * (v[1][i] if jj < 0 else v[3][i])
+            for ii, a in ((i - 1, 1 / RD[j, i]), (i + 1, 1 / RD[j, i + 1])):
+                if 0 <= ii < n:
+                    A[kd, kd] += a; A[kd, idx(ii, j, 1)] -= a
+                elif (ii < 0 and on[0][j]) or (ii >= n and on[2][j]):
+                    A[kd, kd] += a; b[kd] += a * (v[0][j] if ii < 0 else v[2][j])
+    x = spla.spsolve(A.tocsc(), b)
+    return x[0::2].reshape(n, m), x[1::2].reshape(n, m)
+
+
+@pytest.mark.parametrize("path", GEN_FILES, ids=[os.path.basename(p)[:-4] for p in GEN_FILES])
+def test_exact_general_matches_scipy(path):
+    n, m, g, r, RU, RD, on, v = _gen_inputs(np.load(path))
+    u, d, io = oracle.exact_general(g, RU, RD, on, v)
+    us, ds = _sparse_general(g, RU, RD, on, v)
+    np.testing.assert_allclose(u, us, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(d, ds, rtol=0, atol=1e-9)
+    for j in range(m):
+        assert io[j] == (((d[n - 1, j] - v[2][j]) / RD[j, n]) if on[2][j] else 0.0)
+
+
+def test_exact_general_reduces_to_the_special_network():
+    rng = np.random.default_rng(12)
+    n, m = 9, 7
+    g = rng.uniform(5e-7, 1e-6, (n, m))
+    RU, RD = rng.uniform(1, 4, (n, m + 1)), rng.uniform(1, 4, (m, n + 1))
+    vin, vout = rng.uniform(0, 1, n), rng.uniform(0, 0.05, m)
+    u, d, io = oracle.exact(g, vin, vout, RU, RD)
+    on = [np.zeros(m), np.ones(n), np.ones(m), np.zeros(n)]
+    u2, d2, io2 = oracle.exact_general(g, RU, RD, on, [np.zeros(m), vin, vout, np.zeros(n)])
+    assert np.array_equal(u, u2) and np.array_equal(d, d2)
+    # the sink wire current equals the cell-current readout at the solution
+    np.testing.assert_allclose(io2, io, rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["gen0_v1_8x8", "gen3_v1_16x16", "gen4_v1_5x4"])
+def test_general_iterations_reach_the_exact_solve(name):
+    n, m, g, r, RU, RD, on, v = _gen_inputs(load_golden(name))
+    u, d, io = oracle.exact_general(g, RU, RD, on, v)
+    vu, vd, iog, rec = oracle.nodegs_general(g, RU, RD, on, v, 200000, 1.9, True, 1e-14)
+    np.testing.assert_allclose(vu, u, rtol=0, atol=1e-11)
+    vu2, vd2, io2, rec2, sw = oracle.rb_f64_general(g, RU, RD, on, v, 200000, 1.9, True, 1e-14)
+    np.testing.assert_allclose(vu2, u, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(vd2, d, rtol=0, atol=1e-11)

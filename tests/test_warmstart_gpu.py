@@ -191,3 +191,40 @@ def test_trained_warm_start_cuts_sweeps(cuda):
     iex = oracle.exact_batch(g[:k].cpu().numpy(), r[:k].cpu().numpy(), vin[:k].cpu().numpy())
     err = np.max(np.abs(warm.iout[:k].cpu().numpy() - iex) / np.abs(iex))
     assert err <= 1e-6, err
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_combine_up_matches_bilinear(cuda, level):
+    """Upsampling combine of a coarse network output against torch's bilinear
+    interpolation (half-pixel centres, edge clamping) in FP32."""
+    import torch
+    import torch.nn.functional as F
+    from fadesim_amd import _native as nat
+    B, M, N, C = 3, 128, 64, 16
+    gen = torch.Generator().manual_seed(level)
+    out = torch.randn(B, M >> level, N >> level, C, generator=gen).to(torch.bfloat16).to(cuda)
+    base = torch.randn(B, 2, M, N, generator=gen).to(cuda)
+    scale = torch.rand(B, generator=gen).to(cuda)
+    warm = torch.empty(B, 2, M, N, device=cuda)
+    rc = nat.lib().fadesim_nn_combine_up(B, M, N, ctypes.c_void_p(base.data_ptr()), ctypes.c_void_p(scale.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), C, level, ctypes.c_void_p(warm.data_ptr()),
+                                         ctypes.c_void_p(nat.stream_handle(cuda)))
+    nat.check(rc, "combine_up")
+    up = F.interpolate(out[..., :2].permute(0, 3, 1, 2).float(), scale_factor=1 << level, mode="bilinear",
+                       align_corners=False)
+    ref = base + scale.view(B, 1, 1, 1) * up
+    torch.testing.assert_close(warm, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_coarse_network_predicts_full_field(cuda):
+    import torch
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    net = warmstart.WarmStartNet(cuda, arch="coarse32", seed=1)
+    assert net.out_level == 2
+    g, r, vin = warmstart.make_instances(8, seed=3, device=cuda)
+    w = net.predict(g, r, vin)
+    assert w.shape == (8, 2, 128, 128) and bool(torch.isfinite(w).all())
+    # an untrained network's field is still only a warm start: the solve converges
+    res = solve_batch(g, r, vin, warm=w, tol=1e-6, ir2s_iters=1, corr_iters=4)
+    assert int((res.status != 0).sum()) == 0
