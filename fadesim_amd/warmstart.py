@@ -110,13 +110,12 @@ def _slim() -> List[Op]:
 
 
 def _coarse32() -> List[Op]:
-    """The same encoder-decoder moved to a 4x coarser grid: the features are
-    average-pooled to 32x32 and the network (32/64/64 channels at 32x32,
-    16x16, 8x8) predicts the correction there; predict() upsamples it
-    bilinearly (fadesim_nn_combine_up).  The correction is smooth on the
-    scale of a few cells, and one wire-solve iteration on the warm field
-    restores the per-cell detail: ~1/5 of the multiply-adds of unet16 and
-    no full-resolution convolution."""
+    """The encoder-decoder moved to a 4x coarser grid: the features are
+    average-pooled to 32x32 and the network (32 / 64 channels at 32x32 and
+    16x16) predicts the correction there; predict() upsamples it bilinearly
+    (fadesim_nn_combine_up).  The correction is smooth on the scale of a few
+    cells, and a wire-solve iteration on the warm field restores the per-cell
+    detail: no full-resolution convolution, ~1/8 of unet16's multiply-adds."""
     return [
         Op("p1", "pool", "x", 1, N_FEAT, N_FEAT),
         Op("p2", "pool", "p1", 2, N_FEAT, N_FEAT),
@@ -125,11 +124,7 @@ def _coarse32() -> List[Op]:
         Op("p3", "pool", "e2", 3, 32, 32),
         Op("e3a", "conv", "p3", 3, 32, 64),
         Op("e3", "conv", "e3a", 3, 64, 64),
-        Op("p4", "pool", "e3", 4, 64, 64),
-        Op("e4", "conv", "p4", 4, 64, 64),
-        Op("t4", "conv", "e4", 4, 64, 64),
-        Op("u3", "up_add", "t4", 3, 64, 64, skip="e3"),
-        Op("t3", "conv", "u3", 3, 64, 32),
+        Op("t3", "conv", "e3", 3, 64, 32),
         Op("u2", "up_add", "t3", 2, 32, 32, skip="e2"),
         Op("d2", "conv", "u2", 2, 32, 32),
         Op("out", "conv", "d2", 2, 32, OUT_C, relu=False),
