@@ -173,6 +173,12 @@ class WarmStartNet:
             self._view(self.w, op).copy_(wl.to(self.dev))
         self.wb = self.w.to(torch.bfloat16)
         self.info: dict = {}
+        # tensors whose gradient the backward pass needs: outputs of ops that
+        # depend on a convolution (pools of the input features do not)
+        self._grad_tensors = set()
+        for op in self.ops:
+            if op.kind == "conv" or op.src in self._grad_tensors or op.skip in self._grad_tensors:
+                self._grad_tensors.add(op.name)
 
     # ------------------------------------------------------------------ params
     def _view(self, flat, op):
@@ -309,6 +315,8 @@ class WarmStartNet:
         grads = {"out": dout}
         for op in reversed(self.ops):
             H, W = self._hw(op.level)
+            if op.name not in self._grad_tensors:
+                continue
             dy = grads.pop(op.name)
             if op.kind == "conv":
                 y = acts[op.name]
@@ -319,7 +327,7 @@ class WarmStartNet:
                 _chk(L.fadesim_nn_conv3x3_wgrad(B, H, W, op.cin, op.cout, kpad(op.cin),
                                                 ctypes.c_void_p(acts[op.src].data_ptr()), ctypes.c_void_p(dy.data_ptr()),
                                                 ctypes.c_void_p(gv.data_ptr()), s), "wgrad " + op.name)
-                if op.src != "x":
+                if op.src in self._grad_tensors:
                     kpt = kpad(op.cout)
                     wT = torch.empty((op.cin, kpt), dtype=torch.bfloat16, device=self.dev)
                     _chk(L.fadesim_nn_flip_weights(op.cin, op.cout, kpad(op.cin), kpt,

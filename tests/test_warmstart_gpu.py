@@ -103,20 +103,22 @@ def _torch_net(net, x_nhwc, wflat):
     return acts["out"].permute(0, 2, 3, 1)
 
 
-def test_network_gradients_match_torch_autograd(cuda):
+@pytest.mark.parametrize("arch", ["unet16", "coarse32"])
+def test_network_gradients_match_torch_autograd(cuda, arch):
     import torch
     from fadesim_amd import _native as nat
     from fadesim_amd.warmstart import OUT_C, WarmStartNet
-    net = WarmStartNet(cuda, seed=3)
+    net = WarmStartNet(cuda, seed=3, arch=arch)
     B = 1
+    R = 128 >> net.out_level
     gen = torch.Generator().manual_seed(5)
     x = _bf(torch.rand(B, 128, 128, 8, generator=gen))
-    tgt = torch.randn(B, 2, 128, 128, generator=gen).float()
+    tgt = torch.randn(B, 2, R, R, generator=gen).float()
     acts = net.forward(x.to(cuda), keep=True)
     out = acts["out"]
     dout = torch.empty_like(out)
     loss = torch.zeros(1, dtype=torch.float32, device=cuda)
-    nat.check(nat.lib().fadesim_nn_mse(B, 128, 128, ctypes.c_void_p(out.data_ptr()), OUT_C,
+    nat.check(nat.lib().fadesim_nn_mse(B, R, R, ctypes.c_void_p(out.data_ptr()), OUT_C,
                                        ctypes.c_void_p(tgt.to(cuda).data_ptr()), ctypes.c_void_p(dout.data_ptr()),
                                        ctypes.c_void_p(loss.data_ptr()), ctypes.c_void_p(nat.stream_handle(cuda))),
               "mse")
@@ -126,7 +128,7 @@ def test_network_gradients_match_torch_autograd(cuda):
     wref = net.wb.cpu().double().requires_grad_(True)
     o = _torch_net(net, x.double(), wref)
     # the kernel's loss: sum over pixels and both output channels of d^2 / (2 P), P = pixels
-    l = ((o[..., 0:2].permute(0, 3, 1, 2) - tgt.double()) ** 2).sum() / (2 * B * 128 * 128)
+    l = ((o[..., 0:2].permute(0, 3, 1, 2) - tgt.double()) ** 2).sum() / (2 * B * R * R)
     l.backward()
     assert abs(loss.item() - l.item()) <= 0.05 * l.item()
     g = grad.cpu().double()
