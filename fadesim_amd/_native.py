@@ -57,6 +57,8 @@ class HookDesc(ctypes.Structure):
         ("enable_break", ctypes.c_int), ("break_threshold", ctypes.c_double),
         ("nonlinear", ctypes.c_int), ("nonlinear_params", ctypes.c_double * 6),
         ("ret_v", ctypes.c_int), ("d_out", _vp), ("d_sweeps", _vp),
+        ("g_load_left", ctypes.c_double), ("g_load_down", ctypes.c_double),
+        ("ou_rows", ctypes.c_int), ("ou_cols", ctypes.c_int),
     ]
 
 
@@ -72,6 +74,7 @@ class Ir2sDesc(ctypes.Structure):
         ("enable_break", ctypes.c_int), ("break_threshold", ctypes.c_double),
         ("nonlinear", ctypes.c_int), ("nonlinear_params", ctypes.c_double * 6),
         ("ret_v", ctypes.c_int), ("d_out", _vp), ("d_vup", _vp), ("d_vdown", _vp), ("d_iters", _vp),
+        ("ou_rows", ctypes.c_int), ("ou_cols", ctypes.c_int),
     ]
 
 

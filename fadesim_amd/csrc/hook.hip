@@ -3,7 +3,10 @@
 // Gauss-Seidel/SOR, all wordlines left->right then all bitlines bottom->top
 // each sweep (pimtorch/.../SplitArrCuda/split_arr_matmul.cu:27-119), readout
 // sum_i (u - d) G (:121-140), sinh-nonlinear cells by the lagged Picard
-// update (:142-196, 241-262).
+// update (:142-196, 241-262).  Separate source / sink load conductances are
+// ir_gs_ou_local (:690-815); OU mode, every ou_rows x ou_cols block of a tile
+// solved as its own crossbar with position-dependent loads, is ir_gs_ou
+// (:1086-1327).
 //
 // One workgroup per crossbar (b, x, y).  The fields are updated in place:
 // walking a wordline left->right, the left neighbour is already new and the
@@ -35,6 +38,8 @@ struct HookArgs {
     const T *input;
     const T *G;
     int bsize, row, col, numX, numY;
+    int ou_rows, ou_cols, nOuX, nOuY; // OU mode: each ou_rows x ou_cols block is its own crossbar
+    T gl, gd;                         // source / sink load conductances (whole-tile mode)
     int sweeps_this_launch;
     int first_sweep; // 0 on the first launch: initialise the fields
     T gw, be, omb, thr;
@@ -64,20 +69,58 @@ __device__ __forceinline__ T nonlinear_g(T V, T z, const double *nl)
     return (T)1 / R;
 }
 
+// One solved crossbar: tile (b, x, y) or, in OU mode, OU (ou_x, ou_y) of it.
+template <typename T>
+struct HookUnit {
+    int b, x, y, oux, ouy, r0, c0, rows, cols;
+    T gl, gd;
+};
+
+template <typename T>
+__device__ __forceinline__ HookUnit<T> hook_unit(const HookArgs<T> &A, int xb)
+{
+    HookUnit<T> U;
+    U.ouy = xb % A.nOuY;
+    U.oux = (xb / A.nOuY) % A.nOuX;
+    const int t2 = xb / (A.nOuY * A.nOuX);
+    U.y = t2 % A.numY;
+    U.x = (t2 / A.numY) % A.numX;
+    U.b = t2 / (A.numY * A.numX);
+    if (A.ou_rows > 0) {
+        // the OU sees the source through c0 + 1 wire segments and the sink
+        // through row - (last row) segments (split_arr_matmul.cu:1109, 1166)
+        U.rows = A.ou_rows;
+        U.cols = A.ou_cols;
+        U.r0 = U.oux * A.ou_rows;
+        U.c0 = U.ouy * A.ou_cols;
+        U.gl = A.gw / (T)(U.c0 + 1);
+        U.gd = A.gw / (T)(A.row - (U.r0 + A.ou_rows - 1));
+    } else {
+        U.rows = A.row;
+        U.cols = A.col;
+        U.r0 = U.c0 = 0;
+        U.gl = A.gl;
+        U.gd = A.gd;
+    }
+    return U;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
 {
-    const int xb = blockIdx.x; // (b, x, y) flattened
-    const int b = xb / (A.numX * A.numY);
-    const int x = (xb / A.numY) % A.numX;
-    const int y = xb % A.numY;
-    const int row = A.row, col = A.col, t = threadIdx.x;
+    const int xb = blockIdx.x; // (b, x, y[, ou_x, ou_y]) flattened
+    const HookUnit<T> U = hook_unit(A, xb);
+    const int b = U.b, x = U.x, y = U.y;
+    const int row = U.rows, col = U.cols, t = threadIdx.x;
     const size_t cells = (size_t)row * col;
     T *u = A.u + (size_t)xb * cells;
     T *d = A.d + (size_t)xb * cells;
-    const T *in = A.input + ((size_t)b * A.numX + x) * row;
-    // linear G of cell (i, c): G[x][i][y*col + c]
-    auto Glin = [&](int i, int c) -> T { return A.G[((size_t)x * row + i) * ((size_t)A.numY * col) + (size_t)y * col + c]; };
+    const T *in = A.input + ((size_t)b * A.numX + x) * A.row + U.r0;
+    const T gl = U.gl, gd = U.gd;
+    // linear G of cell (i, c): G[x][r0 + i][y*col + c0 + c]
+    auto Glin = [&](int i, int c) -> T {
+        return A.G[((size_t)x * A.row + U.r0 + i) * ((size_t)A.numY * A.col) + (size_t)y * A.col + U.c0 + c];
+    };
     T *Gc = A.nonlinear ? A.Gc + (size_t)xb * cells : nullptr;
     T *Gn = A.nonlinear ? A.Gn + (size_t)xb * cells : nullptr;
     const T gw = A.gw, be = A.be, omb = A.omb;
@@ -124,8 +167,8 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
             T left = in[i];
             T old = ur[0];
             T G0 = Gof(i, 0);
-            T sumg = gw + gw + G0;
-            T nv = be * (gw * left + gw * ur[1] + G0 * dr[0]) / sumg + omb * old;
+            T sumg = gw + gl + G0;
+            T nv = be * (gl * left + gw * ur[1] + G0 * dr[0]) / sumg + omb * old;
             moved |= dev_abs(nv - old) > A.thr;
             ur[0] = nv;
             left = nv;
@@ -152,8 +195,8 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
             int i = row - 1;
             T old = d[(size_t)i * col + c];
             T Gi = Gof(i, c);
-            T sumg = gw + gw + Gi;
-            T nv = be * (gw * d[(size_t)(i - 1) * col + c] + gw * (T)0 + Gi * u[(size_t)i * col + c]) / sumg + omb * old;
+            T sumg = gw + gd + Gi;
+            T nv = be * (gw * d[(size_t)(i - 1) * col + c] + gd * (T)0 + Gi * u[(size_t)i * col + c]) / sumg + omb * old;
             moved |= dev_abs(nv - old) > A.thr;
             d[(size_t)i * col + c] = nv;
             T down = nv;
@@ -179,15 +222,15 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
         atomicOr(A.moved, 1u);
 }
 
-// Readout: out[b][x][y*col + c] = sum_i (u - d) G, or the voltage differences.
+// Readout: out[b][x][y*col + c] = sum_i (u - d) G, or the voltage differences;
+// OU mode: out[b][x][ou_x][y*col + c0 + c], the current of each OU.
 template <typename T>
 __global__ void hook_readout(HookArgs<T> A, T *out, int ret_v)
 {
     const int xb = blockIdx.x;
-    const int b = xb / (A.numX * A.numY);
-    const int x = (xb / A.numY) % A.numX;
-    const int y = xb % A.numY;
-    const int row = A.row, col = A.col;
+    const HookUnit<T> U = hook_unit(A, xb);
+    const int b = U.b, x = U.x, y = U.y;
+    const int row = U.rows, col = U.cols;
     const size_t cells = (size_t)row * col;
     const T *u = A.u + (size_t)xb * cells;
     const T *d = A.d + (size_t)xb * cells;
@@ -201,10 +244,11 @@ __global__ void hook_readout(HookArgs<T> A, T *out, int ret_v)
         for (int i = 0; i < row; ++i) {
             const size_t k = (size_t)i * col + c;
             const T G = A.nonlinear ? A.Gc[(size_t)xb * cells + k]
-                                    : A.G[((size_t)x * row + i) * ((size_t)A.numY * col) + (size_t)y * col + c];
+                                    : A.G[((size_t)x * A.row + U.r0 + i) * ((size_t)A.numY * A.col) +
+                                          (size_t)y * A.col + U.c0 + c];
             s += (u[k] - d[k]) * G;
         }
-        out[(((size_t)b * A.numX + x) * A.numY + y) * col + c] = s;
+        out[((((size_t)b * A.numX + x) * A.nOuX + U.oux) * A.numY + y) * A.col + U.c0 + c] = s;
     }
 }
 
@@ -214,8 +258,10 @@ __global__ void set_i32(int *p, int v) { *p = v; }
 template <typename T>
 int run_hook(const fadesim_hook_desc *D, hipStream_t s)
 {
-    const int grid = D->batch_size * D->numX * D->numY;
-    const size_t cells = (size_t)D->row_size * D->col_size;
+    const bool ou = D->ou_rows > 0;
+    const int nOuX = ou ? D->row_size / D->ou_rows : 1, nOuY = ou ? D->col_size / D->ou_cols : 1;
+    const int grid = D->batch_size * D->numX * D->numY * nOuX * nOuY;
+    const size_t cells = ou ? (size_t)D->ou_rows * D->ou_cols : (size_t)D->row_size * D->col_size;
     const size_t nfield = (size_t)grid * cells;
     const size_t planes = D->nonlinear ? 4 : 2;
     char *ws = (char *)workspace(planes * nfield * sizeof(T) + 256);
@@ -229,7 +275,13 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     A.col = D->col_size;
     A.numX = D->numX;
     A.numY = D->numY;
+    A.ou_rows = ou ? D->ou_rows : 0;
+    A.ou_cols = ou ? D->ou_cols : 0;
+    A.nOuX = nOuX;
+    A.nOuY = nOuY;
     A.gw = (T)D->g_wire;
+    A.gl = (T)(D->g_load_left > 0 ? D->g_load_left : D->g_wire);
+    A.gd = (T)(D->g_load_down > 0 ? D->g_load_down : D->g_wire);
     A.be = (T)D->beta;
     A.omb = (T)1 - A.be;
     A.thr = (T)D->break_threshold;
@@ -301,6 +353,12 @@ extern "C" int fadesim_hook_ir_drop_gs(const fadesim_hook_desc *d, void *stream)
         return set_error(FADESIM_E_INVALID, "batch_size, numX, numY must be >= 1 and iter_times >= 0");
     if (!d->d_input || !d->d_G || !d->d_out)
         return set_error(FADESIM_E_INVALID, "input, G and out are required");
+    if (d->ou_rows > 0 || d->ou_cols > 0) {
+        if (d->ou_rows < 2 || d->ou_cols < 2 || d->row_size % d->ou_rows || d->col_size % d->ou_cols)
+            return set_error(FADESIM_E_INVALID, "OU sizes must be >= 2 and divide row_size and col_size");
+        if (d->ret_v)
+            return set_error(FADESIM_E_INVALID, "OU mode returns currents only");
+    }
     if (d->dtype_bytes == 4)
         return run_hook<float>(d, (hipStream_t)stream);
     if (d->dtype_bytes == 8)

@@ -13,6 +13,11 @@
 //   d_n-1,j = r_snk T_n-1,j + vout_j,  d_ij = d_i+1,j + r T_ij   (T: prefix sums)
 // Both layers use the currents of the same state (Jacobi between layers).
 //
+// OU mode (ir2s_ou, split_arr_matmul.cu:816-1085): every ou_rows x ou_cols
+// block of a tile is solved as its own crossbar, its source c0 + 1 wire
+// segments away and its sink row - (last row) segments away; separate source
+// and sink loads on whole tiles are ir2s_ou_local (:546-688).
+//
 // One workgroup per crossbar.  The currents of an iteration go to a workspace
 // plane first (all cells in parallel); a thread per wordline then walks its
 // row and a thread per bitline its column, each with the reference's
@@ -42,6 +47,7 @@ struct Ir2sArgs {
     const T *vin, *G, *vout;
     int layout; // FADESIM_IR2S_HOOK or FADESIM_IR2S_PLAIN
     int bsize, row, col, numX, numY;
+    int ou_rows, ou_cols, nOuX, nOuY; // OU mode (hook layout): blocks solved as crossbars
     int iters_this_launch;
     int first; // initialise the state (u = vin, d = 0)
     T r, rsrc, rsnk, gwire;
@@ -75,25 +81,46 @@ __device__ __forceinline__ T cell_g(T V, T z, const double *nl)
 
 template <typename T>
 struct Cross {
-    const T *in;   // [row] drives
-    const T *vo;   // [col] sink voltages or null
+    const T *in;   // [rows] drives
+    const T *vo;   // [cols] sink voltages or null
     size_t gbase;  // G of cell (i, c) at G[gbase + i * gld + c]
     size_t gld;
+    int rows, cols;
+    T rsrc, rsnk;  // source / sink segment resistance
+    size_t obase;  // readout: out[obase + c]
 };
 
 template <typename T>
 __device__ __forceinline__ Cross<T> cross_of(const Ir2sArgs<T> &A, int xb)
 {
     Cross<T> X;
+    X.rows = A.row;
+    X.cols = A.col;
+    X.rsrc = A.rsrc;
+    X.rsnk = A.rsnk;
     if (A.layout == FADESIM_IR2S_HOOK) {
-        const int b = xb / (A.numX * A.numY), x = (xb / A.numY) % A.numX, y = xb % A.numY;
-        X.in = A.vin + ((size_t)b * A.numX + x) * A.row;
-        X.gbase = (size_t)x * A.row * ((size_t)A.numY * A.col) + (size_t)y * A.col;
+        const int ouy = xb % A.nOuY, oux = (xb / A.nOuY) % A.nOuX, t2 = xb / (A.nOuY * A.nOuX);
+        const int b = t2 / (A.numX * A.numY), x = (t2 / A.numY) % A.numX, y = t2 % A.numY;
+        int r0 = 0, c0 = 0;
+        if (A.ou_rows > 0) {
+            // the OU's source is c0 + 1 wire segments away, its sink
+            // row - (last row) segments (split_arr_matmul.cu:846, 891)
+            X.rows = A.ou_rows;
+            X.cols = A.ou_cols;
+            r0 = oux * A.ou_rows;
+            c0 = ouy * A.ou_cols;
+            X.rsrc = A.r * (T)(c0 + 1);
+            X.rsnk = A.r * (T)(A.row - (r0 + A.ou_rows - 1));
+        }
+        X.in = A.vin + ((size_t)b * A.numX + x) * A.row + r0;
+        X.gbase = ((size_t)x * A.row + r0) * ((size_t)A.numY * A.col) + (size_t)y * A.col + c0;
         X.gld = (size_t)A.numY * A.col;
+        X.obase = ((((size_t)b * A.numX + x) * A.nOuX + oux) * A.numY + y) * A.col + c0;
     } else {
         X.in = A.vin + (size_t)xb * A.row;
         X.gbase = (size_t)xb * A.row * A.col;
         X.gld = A.col;
+        X.obase = (size_t)xb * A.col;
     }
     X.vo = A.vout ? A.vout + (size_t)xb * A.col : nullptr;
     return X;
@@ -103,9 +130,9 @@ template <typename T>
 __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
 {
     const int xb = blockIdx.x;
-    const int row = A.row, col = A.col, t = threadIdx.x;
-    const size_t cells = (size_t)row * col;
     const Cross<T> X = cross_of(A, xb);
+    const int row = X.rows, col = X.cols, t = threadIdx.x;
+    const size_t cells = (size_t)row * col;
     T *u = A.u + (size_t)xb * cells, *d = A.d + (size_t)xb * cells, *cur = A.cur + (size_t)xb * cells;
     T *wres = A.wres + (size_t)xb * row, *bres = A.bres + (size_t)xb * col;
     T *Gc = A.nonlinear ? A.Gc + (size_t)xb * cells : nullptr;
@@ -160,7 +187,7 @@ __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
             T s = A.carry_residue ? wres[i] : (T)0;
             for (int j = 0; j < col; ++j)
                 s += ci[j];
-            T v = X.in[i] - s * A.rsrc;
+            T v = X.in[i] - s * X.rsrc;
             dmax = fmax(dmax, dabs(v - ui[0]));
             ui[0] = v;
             s -= ci[0];
@@ -179,7 +206,7 @@ __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
             for (int i = 0; i < row; ++i)
                 s += cur[(size_t)i * col + c];
             size_t k = (size_t)(row - 1) * col + c;
-            T v = s * A.rsnk + (X.vo ? X.vo[c] : (T)0);
+            T v = s * X.rsnk + (X.vo ? X.vo[c] : (T)0);
             dmax = fmax(dmax, dabs(v - d[k]));
             d[k] = v;
             s -= cur[k];
@@ -229,9 +256,9 @@ template <typename T>
 __global__ void ir2s_readout(Ir2sArgs<T> A, T *out, int ret_v, T *vup, T *vdown)
 {
     const int xb = blockIdx.x;
-    const int row = A.row, col = A.col;
-    const size_t cells = (size_t)row * col;
     const Cross<T> X = cross_of(A, xb);
+    const int row = X.rows, col = X.cols;
+    const size_t cells = (size_t)row * col;
     const T *u = A.u + (size_t)xb * cells, *d = A.d + (size_t)xb * cells;
     for (size_t k = threadIdx.x; k < cells; k += blockDim.x) {
         if (ret_v)
@@ -250,7 +277,7 @@ __global__ void ir2s_readout(Ir2sArgs<T> A, T *out, int ret_v, T *vup, T *vdown)
             const T G = A.nonlinear ? A.Gc[(size_t)xb * cells + k] : A.G[X.gbase + (size_t)i * X.gld + c];
             s += (u[k] - d[k]) * G;
         }
-        out[(size_t)xb * col + c] = s;
+        out[X.obase + c] = s;
     }
 }
 
@@ -260,11 +287,14 @@ __global__ void ir2s_set_i32(int *p, int v) { *p = v; }
 template <typename T>
 int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
 {
-    const int grid = D->batch_size * D->numX * D->numY;
-    const size_t cells = (size_t)D->row_size * D->col_size;
+    const bool ou = D->ou_rows > 0;
+    const int nOuX = ou ? D->row_size / D->ou_rows : 1, nOuY = ou ? D->col_size / D->ou_cols : 1;
+    const int grid = D->batch_size * D->numX * D->numY * nOuX * nOuY;
+    const int urows = ou ? D->ou_rows : D->row_size, ucols = ou ? D->ou_cols : D->col_size;
+    const size_t cells = (size_t)urows * ucols;
     const size_t nfield = (size_t)grid * cells;
     const size_t planes = D->nonlinear ? 5 : 3;
-    const size_t lines = (size_t)grid * (D->row_size + D->col_size);
+    const size_t lines = (size_t)grid * (urows + ucols);
     const size_t bytes = (planes * nfield + lines) * sizeof(T) + (size_t)grid * sizeof(int) + 256;
     char *ws = (char *)workspace(bytes);
     if (!ws)
@@ -279,6 +309,10 @@ int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
     A.col = D->col_size;
     A.numX = D->numX;
     A.numY = D->numY;
+    A.ou_rows = ou ? D->ou_rows : 0;
+    A.ou_cols = ou ? D->ou_cols : 0;
+    A.nOuX = nOuX;
+    A.nOuY = nOuY;
     A.r = (T)D->r_wire;
     A.gwire = (T)(1.0 / D->r_wire);
     A.rsrc = (T)(D->r_source > 0 ? D->r_source : D->r_wire);
@@ -296,8 +330,8 @@ int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
     A.Gc = D->nonlinear ? A.cur + nfield : nullptr;
     A.Gn = D->nonlinear ? A.Gc + nfield : nullptr;
     A.wres = A.u + planes * nfield;
-    A.bres = A.wres + (size_t)grid * D->row_size;
-    A.count = (int *)(A.bres + (size_t)grid * D->col_size);
+    A.bres = A.wres + (size_t)grid * urows;
+    A.count = (int *)(A.bres + (size_t)grid * ucols);
     A.moved = (unsigned *)(A.count + grid);
     int done = 0;
     if (D->enable_break != FADESIM_IR2S_BREAK_BATCH) {
@@ -367,6 +401,14 @@ extern "C" int fadesim_ir2s(const fadesim_ir2s_desc *d, void *stream)
         return set_error(FADESIM_E_INVALID, "enable_break must be 0, FADESIM_IR2S_BREAK_BATCH or _EACH");
     if (!(d->r_wire > 0.0))
         return set_error(FADESIM_E_INVALID, "r_wire must be > 0");
+    if (d->ou_rows > 0 || d->ou_cols > 0) {
+        if (d->layout != FADESIM_IR2S_HOOK || d->ou_rows < 2 || d->ou_cols < 2 || d->row_size % d->ou_rows ||
+            d->col_size % d->ou_cols)
+            return set_error(FADESIM_E_INVALID,
+                             "OU mode: hook layout, OU sizes >= 2 dividing row_size and col_size");
+        if (d->ret_v || d->d_vup || d->d_vdown)
+            return set_error(FADESIM_E_INVALID, "OU mode returns currents only");
+    }
     if (!d->d_vin || !d->d_G || !d->d_out)
         return set_error(FADESIM_E_INVALID, "vin, G and out are required");
     if (d->dtype_bytes == 4)

@@ -255,6 +255,12 @@ typedef struct fadesim_hook_desc {
     int ret_v;
     void *d_out;
     int *d_sweeps; /* out (device): sweeps run, or NULL */
+    double g_load_left, g_load_down; /* source / sink load conductances (<= 0: g_wire),
+                                      * ir_gs_ou_local (split_arr_matmul.cu:690-815) */
+    int ou_rows, ou_cols; /* > 0: OU mode, ir_gs_ou (split_arr_matmul.cu:1086-1327): every
+                           * ou_rows x ou_cols block solved as its own crossbar, source load
+                           * g_wire/(c0+1), sink load g_wire/(row_size - last row); out
+                           * [batch][numX][row_size/ou_rows][numY*col] currents */
 } fadesim_hook_desc;
 
 int fadesim_hook_ir_drop_gs(const fadesim_hook_desc *desc, void *stream);
@@ -302,6 +308,10 @@ typedef struct fadesim_ir2s_desc {
     int *d_iters;           /* optional (device): BREAK_EACH: [crossbars] counts as the CPU solver
                              * returns them (break sweep + 1, or iter_times + 1 when never met);
                              * otherwise one int: iterations run */
+    int ou_rows, ou_cols;   /* > 0 (hook layout): OU mode, ir2s_ou (split_arr_matmul.cu:816-1085):
+                             * every ou_rows x ou_cols block solved as its own crossbar, source
+                             * r_wire (c0+1), sink r_wire (row_size - last row); out
+                             * [batch][numX][row_size/ou_rows][numY*col] currents */
 } fadesim_ir2s_desc;
 
 int fadesim_ir2s(const fadesim_ir2s_desc *desc, void *stream);

@@ -85,6 +85,14 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [fp, fp, i, i, i, i, i, i, d, d, i, d, i, _dp, i, fp]
             f.restype = i
+        for name, fp in (("oracle_hook_gs_ou_f64", _dp), ("oracle_hook_gs_ou_f32", _fp)):
+            f = getattr(L, name)
+            f.argtypes = [fp, fp, i, i, i, i, i, i, d, d, d, d, i, d, i, _dp, i, i, i, fp]
+            f.restype = i
+        for name, fp in (("oracle_hook_ir2s_ou_f64", _dp), ("oracle_hook_ir2s_ou_f32", _fp)):
+            f = getattr(L, name)
+            f.argtypes = [fp, fp, i, i, i, i, i, i, d, d, d, i, d, i, _dp, i, i, i, fp]
+            f.restype = i
         _LIB = L
     return _LIB
 
@@ -285,6 +293,49 @@ def hook_ir2s(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, r_
     it = fn(pp(inp), pp(G), int(batch_size), int(row_size), int(col_size), int(numX), int(numY), int(iter_times),
             float(r_wire), int(bool(enable_break)), float(break_threshould), int(bool(nonlinear_cell_enable)),
             _pd(nl), int(bool(ret_V)), pp(out))
+    return out, it
+
+
+def hook_gs_ou(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, g_wire, beta, enable_break,
+               break_threshold, nonlinear_cell_enable, nonlinear_params, g_load_left=None, g_load_down=None,
+               ou_rows=0, ou_cols=0):
+    """Line-GS with separate source / sink loads (ir_gs_ou_local,
+    split_arr_matmul.cu:690-815) or in OU mode (ir_gs_ou, :1086-1327; loads
+    from the block's position).  Returns (out, sweeps), out
+    [bsize, numX, num_ou_X, numY*col] (num_ou_X = 1 without OUs)."""
+    f64 = np.asarray(G).dtype == np.float64
+    cv, pp = (_d, _pd) if f64 else (_f, _pf)
+    inp, G = cv(inp), cv(G)
+    nl = _d(list(nonlinear_params) + [0.0] * (6 - len(nonlinear_params)))
+    nOuX = row_size // ou_rows if ou_rows > 0 else 1
+    out = np.empty((batch_size, numX, nOuX, numY * col_size), G.dtype)
+    gl = g_wire if g_load_left is None else g_load_left
+    gd = g_wire if g_load_down is None else g_load_down
+    f = lib().oracle_hook_gs_ou_f64 if f64 else lib().oracle_hook_gs_ou_f32
+    sweeps = f(pp(inp), pp(G), batch_size, row_size, col_size, numX, numY, int(iter_times), float(g_wire),
+               float(gl), float(gd), float(beta), int(bool(enable_break)), float(break_threshold),
+               int(bool(nonlinear_cell_enable)), _pd(nl), 0, int(ou_rows), int(ou_cols), pp(out))
+    return out, sweeps
+
+
+def hook_ir2s_ou(inp, G, batch_size, row_size, col_size, numX, numY, iter_times, r_wire, enable_break,
+                 break_threshould, nonlinear_cell_enable, nonlinear_params, r_source=None, r_sink=None,
+                 ou_rows=0, ou_cols=0):
+    """The hook's IR2S with separate source / sink resistances (ir2s_ou_local,
+    split_arr_matmul.cu:546-688) or in OU mode (ir2s_ou, :816-1085).
+    Returns (out, iterations), out [bsize, numX, num_ou_X, numY*col]."""
+    f32 = np.asarray(G).dtype == np.float32
+    cv, pp = (_f, _pf) if f32 else (_d, _pd)
+    inp, G = cv(inp), cv(G)
+    nl = _d(list(nonlinear_params) + [0.0] * (6 - len(nonlinear_params)) if nonlinear_cell_enable else np.zeros(6))
+    nOuX = row_size // ou_rows if ou_rows > 0 else 1
+    out = np.empty((batch_size, numX, nOuX, numY * col_size), G.dtype)
+    rs = r_wire if r_source is None else r_source
+    rk = r_wire if r_sink is None else r_sink
+    fn = lib().oracle_hook_ir2s_ou_f32 if f32 else lib().oracle_hook_ir2s_ou_f64
+    it = fn(pp(inp), pp(G), int(batch_size), int(row_size), int(col_size), int(numX), int(numY), int(iter_times),
+            float(r_wire), float(rs), float(rk), int(bool(enable_break)), float(break_threshould),
+            int(bool(nonlinear_cell_enable)), _pd(nl), 0, int(ou_rows), int(ou_cols), pp(out))
     return out, it
 
 

@@ -681,86 +681,90 @@ void oracle_rb_f32(int n, int m, const float *g, double r, const double *vin,
  * Output: ret_v ? (u-d)[b][x][y][row][col] : I[b][x][y*col + c].
  * Returns the number of sweeps run.
  */
+/* Unit xb of the grid: tile (b, x, y), or in OU mode block (oux, ouy) of it,
+ * rows [r0, r0 + R) and columns [c0, c0 + C) of the tile. */
+#define HOOK_UNIT                                                                      \
+    const int nOuX = ou_rows > 0 ? row / ou_rows : 1, nOuY = ou_rows > 0 ? col / ou_cols : 1; \
+    const int R = ou_rows > 0 ? ou_rows : row, C = ou_rows > 0 ? ou_cols : col;        \
+    const size_t cells = (size_t)R * C;                                                \
+    const size_t nxb = (size_t)bsize * numX * numY * nOuX * nOuY;
+#define UNIT_DECODE(xb)                                                                \
+    const int ouy = (int)((xb) % nOuY), oux = (int)(((xb) / nOuY) % nOuX);             \
+    const size_t t2_ = (xb) / ((size_t)nOuY * nOuX);                                   \
+    const int y = (int)(t2_ % numY), x = (int)((t2_ / numY) % numX), b = (int)(t2_ / ((size_t)numY * numX)); \
+    const int r0 = oux * R, c0 = ouy * C;                                              \
+    (void)b;
+#define GLIN(i, c) G[((size_t)x * row + r0 + (i)) * (size_t)(numY * col) + (size_t)y * col + c0 + (c)]
+#define INP(i) input[((size_t)b * numX + x) * row + r0 + (i)]
+
 #define HOOK_GS_BODY(T, SINH, FABS)                                                    \
-    const size_t cells = (size_t)row * col;                                            \
-    const size_t nxb = (size_t)bsize * numX * numY;                                    \
+    HOOK_UNIT                                                                          \
     T *u0 = (T *)malloc(nxb * cells * sizeof(T)), *u1 = (T *)malloc(nxb * cells * sizeof(T)); \
     T *d0 = (T *)calloc(nxb * cells, sizeof(T)), *d1 = (T *)calloc(nxb * cells, sizeof(T));  \
     T *Gc = (T *)malloc(nxb * cells * sizeof(T));                                      \
-    T *z = nonlinear ? (T *)malloc((size_t)numX * numY * cells * sizeof(T)) : NULL;    \
+    T *z = nonlinear ? (T *)malloc(nxb * cells * sizeof(T)) : NULL;                    \
     const T gw = (T)g_wire, be = (T)beta, omb = (T)1 - be;                             \
-    for (int b = 0; b < bsize; ++b)                                                    \
-        for (int x = 0; x < numX; ++x)                                                 \
-            for (int y = 0; y < numY; ++y)                                             \
-                for (int i = 0; i < row; ++i)                                          \
-                    for (int c = 0; c < col; ++c) {                                    \
-                        size_t k = ((((size_t)b * numX + x) * numY + y) * row + i) * col + c; \
-                        u0[k] = u1[k] = input[((size_t)b * numX + x) * row + i];       \
-                        Gc[k] = G[((size_t)x * row + i) * (size_t)(numY * col) + (size_t)y * col + c]; \
-                    }                                                                  \
-    if (nonlinear)                                                                     \
-        for (int x = 0; x < numX; ++x)                                                 \
-            for (int y = 0; y < numY; ++y)                                             \
-                for (int i = 0; i < row; ++i)                                          \
-                    for (int c = 0; c < col; ++c) {                                    \
-                        T gl = G[((size_t)x * row + i) * (size_t)(numY * col) + (size_t)y * col + c]; \
-                        T zz = ((T)1 / gl - (T)nl[0]) / (T)(nl[1] - nl[0]);            \
-                        if (zz < 0) zz = 0;                                            \
-                        z[(((size_t)x * numY + y) * row + i) * col + c] = zz;          \
-                    }                                                                  \
+    for (size_t xb = 0; xb < nxb; ++xb) {                                              \
+        UNIT_DECODE(xb)                                                                \
+        for (int i = 0; i < R; ++i)                                                    \
+            for (int c = 0; c < C; ++c) {                                              \
+                const size_t k = xb * cells + (size_t)i * C + c;                       \
+                u0[k] = u1[k] = INP(i);                                                \
+                Gc[k] = GLIN(i, c);                                                    \
+                if (nonlinear) {                                                       \
+                    T zz = ((T)1 / GLIN(i, c) - (T)nl[0]) / (T)(nl[1] - nl[0]);        \
+                    z[k] = zz < 0 ? 0 : zz;                                            \
+                }                                                                      \
+            }                                                                          \
+    }                                                                                  \
     /* cur/prev buffers: the sweep writes cur, reads prev; then swap */                \
     T *ucur = u0, *uprev = u1, *dcur = d0, *dprev = d1;                                \
     int it;                                                                            \
     for (it = 0; it < iter_times; ++it) {                                              \
-        if (nonlinear) {                                                               \
-            for (int b = 0; b < bsize; ++b)                                            \
-                for (int x = 0; x < numX; ++x)                                         \
-                    for (int y = 0; y < numY; ++y)                                     \
-                        for (size_t e = 0; e < cells; ++e) {                           \
-                            size_t k = (((size_t)b * numX + x) * numY + y) * cells + e; \
-                            T V = ucur[k] - dcur[k];                                   \
-                            if (V < (T)1e-5 && V > -(T)1e-5)                           \
-                                V = V > 0 ? (T)1e-5 : -(T)1e-5;                        \
-                            V = FABS(V);                                               \
-                            T zz = z[((size_t)x * numY + y) * cells + e];              \
-                            T Ion = (T)nl[3] * SINH(V * (T)nl[2]);                     \
-                            T Ioff = (T)nl[5] * SINH(V * (T)nl[4]);                    \
-                            T R = ((T)1 - zz) * V / Ion + zz * V / Ioff;               \
-                            Gc[k] = (T)1 / R;                                          \
-                        }                                                              \
-        }                                                                              \
+        if (nonlinear)                                                                 \
+            for (size_t k = 0; k < nxb * cells; ++k) {                                 \
+                T V = ucur[k] - dcur[k];                                               \
+                if (V < (T)1e-5 && V > -(T)1e-5)                                       \
+                    V = V > 0 ? (T)1e-5 : -(T)1e-5;                                    \
+                V = FABS(V);                                                           \
+                T Ion = (T)nl[3] * SINH(V * (T)nl[2]);                                 \
+                T Ioff = (T)nl[5] * SINH(V * (T)nl[4]);                                \
+                T R_ = ((T)1 - z[k]) * V / Ion + z[k] * V / Ioff;                      \
+                Gc[k] = (T)1 / R_;                                                     \
+            }                                                                          \
         for (size_t xb = 0; xb < nxb; ++xb) {                                          \
-            int b = (int)(xb / ((size_t)numX * numY));                                 \
-            int x = (int)((xb / numY) % numX);                                         \
+            UNIT_DECODE(xb)                                                            \
+            const T gl = ou_rows > 0 ? gw / (T)(c0 + 1) : (T)g_load_left;              \
+            const T gd = ou_rows > 0 ? gw / (T)(row - (r0 + R - 1)) : (T)g_load_down;  \
             T *uc = ucur + xb * cells, *up = uprev + xb * cells;                       \
             T *dc = dcur + xb * cells, *dp = dprev + xb * cells;                       \
             const T *Gk = Gc + xb * cells;                                             \
-            for (int i = 0; i < row; ++i) {                                            \
-                T *ucr = uc + (size_t)i * col, *upr = up + (size_t)i * col;            \
-                const T *dpr = dp + (size_t)i * col, *Gr = Gk + (size_t)i * col;       \
-                T left = input[((size_t)b * numX + x) * row + i];                      \
-                T sumg = gw + gw + Gr[0];                                              \
-                ucr[0] = be * (gw * left + gw * upr[1] + Gr[0] * dpr[0]) / sumg + omb * upr[0]; \
+            for (int i = 0; i < R; ++i) {                                              \
+                T *ucr = uc + (size_t)i * C, *upr = up + (size_t)i * C;                \
+                const T *dpr = dp + (size_t)i * C, *Gr = Gk + (size_t)i * C;           \
+                T left = INP(i);                                                       \
+                T sumg = gw + gl + Gr[0];                                              \
+                ucr[0] = be * (gl * left + gw * upr[1] + Gr[0] * dpr[0]) / sumg + omb * upr[0]; \
                 int j;                                                                 \
-                for (j = 1; j < col - 1; ++j) {                                        \
+                for (j = 1; j < C - 1; ++j) {                                          \
                     sumg = gw * 2 + Gr[j];                                             \
                     ucr[j] = be * (gw * (ucr[j - 1] + upr[j + 1]) + Gr[j] * dpr[j]) / sumg + omb * upr[j]; \
                 }                                                                      \
                 sumg = gw + Gr[j];                                                     \
                 ucr[j] = be * (gw * ucr[j - 1] + Gr[j] * dpr[j]) / sumg + omb * upr[j];\
             }                                                                          \
-            for (int c = 0; c < col; ++c) {                                            \
-                int i = row - 1;                                                       \
-                T sumg = gw + gw + Gk[(size_t)i * col + c];                            \
-                dc[(size_t)i * col + c] = be * (gw * dp[(size_t)(i - 1) * col + c] + gw * (T)0 + \
-                    Gk[(size_t)i * col + c] * uc[(size_t)i * col + c]) / sumg + omb * dp[(size_t)i * col + c]; \
-                for (i = row - 2; i > 0; --i) {                                        \
-                    sumg = gw * 2 + Gk[(size_t)i * col + c];                           \
-                    dc[(size_t)i * col + c] = be * (gw * (dp[(size_t)(i - 1) * col + c] + dc[(size_t)(i + 1) * col + c]) + \
-                        Gk[(size_t)i * col + c] * uc[(size_t)i * col + c]) / sumg + omb * dp[(size_t)i * col + c]; \
+            for (int c = 0; c < C; ++c) {                                              \
+                int i = R - 1;                                                         \
+                T sumg = gw + gd + Gk[(size_t)i * C + c];                              \
+                dc[(size_t)i * C + c] = be * (gw * dp[(size_t)(i - 1) * C + c] + gd * (T)0 + \
+                    Gk[(size_t)i * C + c] * uc[(size_t)i * C + c]) / sumg + omb * dp[(size_t)i * C + c]; \
+                for (i = R - 2; i > 0; --i) {                                          \
+                    sumg = gw * 2 + Gk[(size_t)i * C + c];                             \
+                    dc[(size_t)i * C + c] = be * (gw * (dp[(size_t)(i - 1) * C + c] + dc[(size_t)(i + 1) * C + c]) + \
+                        Gk[(size_t)i * C + c] * uc[(size_t)i * C + c]) / sumg + omb * dp[(size_t)i * C + c]; \
                 }                                                                      \
                 sumg = gw + Gk[c];                                                     \
-                dc[c] = be * (gw * dc[(size_t)col + c] + Gk[c] * uc[c]) / sumg + omb * dp[c]; \
+                dc[c] = be * (gw * dc[(size_t)C + c] + Gk[c] * uc[c]) / sumg + omb * dp[c]; \
             }                                                                          \
         }                                                                              \
         T *t = ucur; ucur = uprev; uprev = t;                                          \
@@ -779,28 +783,47 @@ void oracle_rb_f32(int n, int m, const float *g, double r, const double *vin,
             out[k] = uprev[k] - dprev[k];                                              \
     } else {                                                                           \
         for (size_t xb = 0; xb < nxb; ++xb) {                                          \
-            int b = (int)(xb / ((size_t)numX * numY));                                 \
-            int x = (int)((xb / numY) % numX);                                         \
-            int y = (int)(xb % numY);                                                  \
-            for (int c = 0; c < col; ++c) {                                            \
+            UNIT_DECODE(xb)                                                            \
+            for (int c = 0; c < C; ++c) {                                              \
                 T s = 0;                                                               \
-                for (int i = 0; i < row; ++i) {                                       \
-                    size_t k = xb * cells + (size_t)i * col + c;                       \
+                for (int i = 0; i < R; ++i) {                                          \
+                    size_t k = xb * cells + (size_t)i * C + c;                         \
                     s += (uprev[k] - dprev[k]) * Gc[k];                                \
                 }                                                                      \
-                out[(((size_t)b * numX + x) * numY + y) * col + c] = s;                \
+                out[((((size_t)b * numX + x) * nOuX + oux) * numY + y) * col + c0 + c] = s; \
             }                                                                          \
         }                                                                              \
     }                                                                                  \
     free(u0); free(u1); free(d0); free(d1); free(Gc); free(z);                         \
     return it;
 
+/* With separate source / sink loads (ir_gs_ou_local, split_arr_matmul.cu:690-815)
+ * and OU mode (ir_gs_ou, :1086-1327: every ou_rows x ou_cols block its own
+ * crossbar with loads g_wire/(c0+1), g_wire/(row - last row); output
+ * I[b][x][oux][y*col + c0 + c]). */
+int oracle_hook_gs_ou_f64(const double *input, const double *G, int bsize, int row, int col, int numX, int numY,
+                          int iter_times, double g_wire, double g_load_left, double g_load_down, double beta,
+                          int enable_break, double thr, int nonlinear, const double *nl, int ret_v, int ou_rows,
+                          int ou_cols, double *out)
+{
+    HOOK_GS_BODY(double, sinh, fabs)
+}
+
+int oracle_hook_gs_ou_f32(const float *input, const float *G, int bsize, int row, int col, int numX, int numY,
+                          int iter_times, double g_wire, double g_load_left, double g_load_down, double beta,
+                          int enable_break, double thr, int nonlinear, const double *nl, int ret_v, int ou_rows,
+                          int ou_cols, float *out)
+{
+    HOOK_GS_BODY(float, sinhf, fabsf)
+}
+
 int oracle_hook_gs_f64(const double *input, const double *G, int bsize, int row,
                        int col, int numX, int numY, int iter_times,
                        double g_wire, double beta, int enable_break, double thr,
                        int nonlinear, const double *nl, int ret_v, double *out)
 {
-    HOOK_GS_BODY(double, sinh, fabs)
+    return oracle_hook_gs_ou_f64(input, G, bsize, row, col, numX, numY, iter_times, g_wire, g_wire, g_wire, beta,
+                                 enable_break, thr, nonlinear, nl, ret_v, 0, 0, out);
 }
 
 int oracle_hook_gs_f32(const float *input, const float *G, int bsize, int row,
@@ -808,7 +831,8 @@ int oracle_hook_gs_f32(const float *input, const float *G, int bsize, int row,
                        double g_wire, double beta, int enable_break, double thr,
                        int nonlinear, const double *nl, int ret_v, float *out)
 {
-    HOOK_GS_BODY(float, sinhf, fabsf)
+    return oracle_hook_gs_ou_f32(input, G, bsize, row, col, numX, numY, iter_times, g_wire, g_wire, g_wire, beta,
+                                 enable_break, thr, nonlinear, nl, ret_v, 0, 0, out);
 }
 
 /* CPU baseline: the lexicographic restatement (oracle_nodegs) over a batch of
@@ -943,26 +967,20 @@ int oracle_ir2s(int n, int m, const double *g, const double *vin, const double *
  * more than thr (:517-527).  Returns the number of iterations run.
  */
 #define HOOK_IR2S_BODY(T, SINH, FABS)                                                  \
-    const size_t cells = (size_t)row * col;                                            \
-    const size_t nxb = (size_t)bsize * numX * numY;                                    \
+    HOOK_UNIT                                                                          \
     T *ua = (T *)malloc(nxb * cells * sizeof(T)), *ub = (T *)malloc(nxb * cells * sizeof(T)); \
     T *da = (T *)calloc(nxb * cells, sizeof(T)), *db = (T *)calloc(nxb * cells, sizeof(T));  \
-    T *Gk = (T *)malloc(nxb * cells * sizeof(T));                                      \
-    T *wsum = (T *)calloc(nxb * row, sizeof(T)), *bsum = (T *)calloc(nxb * col, sizeof(T)); \
+    T *Gk = (T *)malloc(nxb * cells * sizeof(T)), *Glin = (T *)malloc(nxb * cells * sizeof(T)); \
+    T *wsum = (T *)calloc(nxb * R, sizeof(T)), *bsum = (T *)calloc(nxb * C, sizeof(T)); \
     const T rw = (T)r_wire;                                                            \
     for (size_t xb = 0; xb < nxb; ++xb) {                                              \
-        const int b = (int)(xb / ((size_t)numX * numY)), x = (int)((xb / numY) % numX), y = (int)(xb % numY); \
-        for (int i = 0; i < row; ++i)                                                  \
-            for (int c = 0; c < col; ++c) {                                            \
-                const size_t k = xb * cells + (size_t)i * col + c;                     \
-                ua[k] = ub[k] = input[((size_t)b * numX + x) * row + i];               \
-                Gk[k] = G[((size_t)x * row + i) * (size_t)(numY * col) + (size_t)y * col + c]; \
+        UNIT_DECODE(xb)                                                                \
+        for (int i = 0; i < R; ++i)                                                    \
+            for (int c = 0; c < C; ++c) {                                              \
+                const size_t k = xb * cells + (size_t)i * C + c;                       \
+                ua[k] = ub[k] = INP(i);                                                \
+                Gk[k] = Glin[k] = GLIN(i, c);                                          \
             }                                                                          \
-    }                                                                                  \
-    T *Glin = NULL;                                                                    \
-    if (nonlinear) {                                                                   \
-        Glin = (T *)malloc(nxb * cells * sizeof(T));                                   \
-        memcpy(Glin, Gk, nxb * cells * sizeof(T));                                     \
     }                                                                                  \
     T *ucur = ua, *uprev = ub, *dcur = da, *dprev = db;                                \
     int it;                                                                            \
@@ -975,37 +993,39 @@ int oracle_ir2s(int n, int m, const double *g, const double *vin, const double *
                 V = FABS(V);                                                           \
                 T zz = ((T)1 / Glin[k] - (T)nl[0]) / (T)(nl[1] - nl[0]);               \
                 if (zz < 0) zz = 0;                                                    \
-                T R = ((T)1 - zz) * V / ((T)nl[3] * SINH(V * (T)nl[2])) +              \
-                      zz * V / ((T)nl[5] * SINH(V * (T)nl[4]));                        \
-                Gk[k] = (T)1 / R;                                                      \
+                T R_ = ((T)1 - zz) * V / ((T)nl[3] * SINH(V * (T)nl[2])) +             \
+                       zz * V / ((T)nl[5] * SINH(V * (T)nl[4]));                       \
+                Gk[k] = (T)1 / R_;                                                     \
             }                                                                          \
         for (size_t xb = 0; xb < nxb; ++xb) {                                          \
-            const int b = (int)(xb / ((size_t)numX * numY)), x = (int)((xb / numY) % numX); \
+            UNIT_DECODE(xb)                                                            \
+            const T rsrc = ou_rows > 0 ? rw * (T)(c0 + 1) : (T)r_source;               \
+            const T rsnk = ou_rows > 0 ? rw * (T)(row - (r0 + R - 1)) : (T)r_sink;     \
             const size_t o = xb * cells;                                               \
-            for (int i = 0; i < row; ++i) {                                            \
-                T *s = wsum + xb * row + i;                                            \
-                const size_t ri = o + (size_t)i * col;                                 \
-                for (int j = 0; j < col; ++j)                                          \
+            for (int i = 0; i < R; ++i) {                                              \
+                T *s = wsum + xb * R + i;                                              \
+                const size_t ri = o + (size_t)i * C;                                   \
+                for (int j = 0; j < C; ++j)                                            \
                     *s += (uprev[ri + j] - dprev[ri + j]) * Gk[ri + j];                \
-                ucur[ri] = input[((size_t)b * numX + x) * row + i] - *s * rw;          \
+                ucur[ri] = INP(i) - *s * rsrc;                                         \
                 *s -= (uprev[ri] - dprev[ri]) * Gk[ri];                                \
-                for (int j = 1; j < col; ++j) {                                        \
+                for (int j = 1; j < C; ++j) {                                          \
                     ucur[ri + j] = ucur[ri + j - 1] - *s * rw;                         \
                     *s -= (uprev[ri + j] - dprev[ri + j]) * Gk[ri + j];                \
                 }                                                                      \
             }                                                                          \
-            for (int c = 0; c < col; ++c) {                                            \
-                T *s = bsum + xb * col + c;                                            \
-                for (int i = 0; i < row; ++i) {                                        \
-                    const size_t k = o + (size_t)i * col + c;                          \
+            for (int c = 0; c < C; ++c) {                                              \
+                T *s = bsum + xb * C + c;                                              \
+                for (int i = 0; i < R; ++i) {                                          \
+                    const size_t k = o + (size_t)i * C + c;                            \
                     *s += (uprev[k] - dprev[k]) * Gk[k];                               \
                 }                                                                      \
-                size_t k = o + (size_t)(row - 1) * col + c;                            \
-                dcur[k] = *s * rw;                                                     \
+                size_t k = o + (size_t)(R - 1) * C + c;                                \
+                dcur[k] = *s * rsnk;                                                   \
                 *s -= (uprev[k] - dprev[k]) * Gk[k];                                   \
-                for (int i = row - 2; i >= 0; --i) {                                   \
-                    k = o + (size_t)i * col + c;                                       \
-                    dcur[k] = dcur[k + col] + *s * rw;                                 \
+                for (int i = R - 2; i >= 0; --i) {                                     \
+                    k = o + (size_t)i * C + c;                                         \
+                    dcur[k] = dcur[k + C] + *s * rw;                                   \
                     *s -= (uprev[k] - dprev[k]) * Gk[k];                               \
                 }                                                                      \
             }                                                                          \
@@ -1025,31 +1045,53 @@ int oracle_ir2s(int n, int m, const double *g, const double *vin, const double *
         for (size_t k = 0; k < nxb * cells; ++k)                                       \
             out[k] = uprev[k] - dprev[k];                                              \
     } else {                                                                           \
-        for (size_t xb = 0; xb < nxb; ++xb)                                            \
-            for (int c = 0; c < col; ++c) {                                            \
+        for (size_t xb = 0; xb < nxb; ++xb) {                                          \
+            UNIT_DECODE(xb)                                                            \
+            for (int c = 0; c < C; ++c) {                                              \
                 T s = 0;                                                               \
-                for (int i = 0; i < row; ++i) {                                        \
-                    const size_t k = xb * cells + (size_t)i * col + c;                 \
+                for (int i = 0; i < R; ++i) {                                          \
+                    const size_t k = xb * cells + (size_t)i * C + c;                   \
                     s += (uprev[k] - dprev[k]) * Gk[k];                                \
                 }                                                                      \
-                out[xb * col + c] = s;                                                 \
+                out[((((size_t)b * numX + x) * nOuX + oux) * numY + y) * col + c0 + c] = s; \
             }                                                                          \
+        }                                                                              \
     }                                                                                  \
-    free(ua); free(ub); free(da); free(db); free(Gk); free(wsum); free(bsum); free(Glin); \
+    free(ua); free(ub); free(da); free(db); free(Gk); free(Glin); free(wsum); free(bsum); \
     return it;
+
+/* With separate source / sink resistances (ir2s_ou_local, split_arr_matmul.cu:546-688)
+ * and OU mode (ir2s_ou, :816-1085: loads r_wire (c0+1), r_wire (row - last row)). */
+int oracle_hook_ir2s_ou_f64(const double *input, const double *G, int bsize, int row, int col, int numX, int numY,
+                            int iter_times, double r_wire, double r_source, double r_sink, int enable_break,
+                            double thr, int nonlinear, const double *nl, int ret_v, int ou_rows, int ou_cols,
+                            double *out)
+{
+    HOOK_IR2S_BODY(double, sinh, fabs)
+}
+
+int oracle_hook_ir2s_ou_f32(const float *input, const float *G, int bsize, int row, int col, int numX, int numY,
+                            int iter_times, double r_wire, double r_source, double r_sink, int enable_break,
+                            double thr, int nonlinear, const double *nl, int ret_v, int ou_rows, int ou_cols,
+                            float *out)
+{
+    HOOK_IR2S_BODY(float, sinhf, fabsf)
+}
 
 int oracle_hook_ir2s_f64(const double *input, const double *G, int bsize, int row, int col, int numX, int numY,
                          int iter_times, double r_wire, int enable_break, double thr, int nonlinear,
                          const double *nl, int ret_v, double *out)
 {
-    HOOK_IR2S_BODY(double, sinh, fabs)
+    return oracle_hook_ir2s_ou_f64(input, G, bsize, row, col, numX, numY, iter_times, r_wire, r_wire, r_wire,
+                                   enable_break, thr, nonlinear, nl, ret_v, 0, 0, out);
 }
 
 int oracle_hook_ir2s_f32(const float *input, const float *G, int bsize, int row, int col, int numX, int numY,
                          int iter_times, double r_wire, int enable_break, double thr, int nonlinear,
                          const double *nl, int ret_v, float *out)
 {
-    HOOK_IR2S_BODY(float, sinhf, fabsf)
+    return oracle_hook_ir2s_ou_f32(input, G, bsize, row, col, numX, numY, iter_times, r_wire, r_wire, r_wire,
+                                   enable_break, thr, nonlinear, nl, ret_v, 0, 0, out);
 }
 
 

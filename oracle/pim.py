@@ -17,7 +17,12 @@ integer/fixed-point pipeline around the solver:
                         per tile (:833-835), sum over row tiles, [:outsize],
                         * rows (2^dac-1)(2^cell-1)/(2^adc-1) (:958-968),
                         out_pos - out_neg, then +-out * 2^(i+j+s1+s2) with the
-                        input MSB subtracted (:1224-1227).
+                        input MSB subtracted (:1224-1227).  OU mode
+                        (``ou_rows``): currents per OU row block, ADC full
+                        scale ou_rows (Gmax-Gmin) vmax (commonConst.py:242,
+                        matmul :838-840), rescale by ou_rows (:968-976).
+* ``ideal_currents_ou`` -- the ideal per-OU product (no_ir_drop_cal_ou,
+                        ir_drop_solve.py:21-30).
 
 Every array op is done in float32 where the reference's is (fp_type float32).
 """
@@ -53,8 +58,20 @@ def ideal_currents(vin: np.ndarray, G: np.ndarray) -> np.ndarray:
     return np.einsum("bxr,xrc->bxc", vin.astype(np.float64), G.astype(np.float64))
 
 
+def ideal_currents_ou(ou_rows: int):
+    """vin [bsize, numX, R] x G [numX, R, numY*C] -> per OU row block
+    [bsize, numX * R/ou_rows, numY*C], FP64."""
+    def cur(vin, G):
+        bsize, numX, R = vin.shape
+        k = R // ou_rows
+        v = vin.astype(np.float64).reshape(bsize, numX, k, ou_rows)
+        g = G.astype(np.float64).reshape(numX, k, ou_rows, G.shape[2])
+        return np.einsum("bxkr,xkrc->bxkc", v, g).reshape(bsize, numX * k, G.shape[2])
+    return cur
+
+
 def split_matmul(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, cell_bits=1,
-                 rows=64, cols=64, vmax=1.0, ron=1e3, roff=1e6, currents=ideal_currents):
+                 rows=64, cols=64, vmax=1.0, ron=1e3, roff=1e6, currents=ideal_currents, ou_rows=None):
     """output [bsize, outsize] = x [bsize, insize] @ w [insize, outsize] through
     PN crossbars.  ``currents(vin [bsize,numX,R] f32, G [numX,R,numY*C] f32)``
     returns the tile currents [bsize, numX, numY*C].  Returns (output f32,
@@ -75,11 +92,12 @@ def split_matmul(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, 
     Rp, Cp = numX * rows, numY * cols
     gmin, gmax = 1.0 / roff, 1.0 / ron
     dG = (gmax - gmin) / (2 ** cell_bits - 1)
-    adc_use_i = rows * (gmax - gmin) * vmax
+    per_read = ou_rows if ou_rows else rows  # rows behind one ADC reading
+    adc_use_i = per_read * (gmax - gmin) * vmax
     input_v = vmax / (2 ** dac_bits - 1)
     mask_cell = (1 << cell_bits) - 1
     lsb = np.float32(2 ** adc_bits - 1)
-    rescale = np.float32(rows * (2 ** dac_bits - 1) * (2 ** cell_bits - 1) / (2 ** adc_bits - 1))
+    rescale = np.float32(per_read * (2 ** dac_bits - 1) * (2 ** cell_bits - 1) / (2 ** adc_bits - 1))
     output = np.zeros((bsize, outsize), np.float32)
     codes, adc_in = {}, {}
     for i in range(0, input_bits, dac_bits):

@@ -148,3 +148,109 @@ def test_hook_ir2s_reference_errors(cuda):
         ir2s(ti, tG, 3, 1, 12, 2, 2, 10, 2.93, False, 0, False, [])
     with pytest.raises(RuntimeError, match="same dtype"):
         ir2s(ti.double(), tG, 3, 16, 12, 2, 2, 10, 2.93, False, 0, False, [])
+
+
+# --- separate loads and OU mode (ir_gs_ou_local, ir_gs_ou, ir2s_ou_local, ir2s_ou) ---
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("iters,brk", [(23, False), (3000, True)])
+def test_hook_gs_local_loads_bitwise(cuda, dtype, iters, brk):
+    import torch
+    from fadesim_amd.hook import ir_gs_ou_local
+    bsize, row, col, numX, numY = 2, 16, 12, 2, 3
+    inp, G = _case(dtype, bsize, row, col, numX, numY, seed=iters + 1)
+    thr = 1e-9 if dtype == np.float64 else 1e-7
+    gw = 1 / 2.93
+    out = ir_gs_ou_local(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, numX, numY,
+                         iters, gw, gw / 3, gw / 7, 1.8, brk, thr, False, [])
+    ref, _ = oracle.hook_gs_ou(inp, G, bsize, row, col, numX, numY, iters, gw, 1.8, brk, thr, False, [],
+                               g_load_left=gw / 3, g_load_down=gw / 7)
+    got = out.cpu().numpy()
+    assert got.dtype == dtype and got.shape == (bsize, numX, numY * col)
+    assert np.array_equal(got, ref[:, :, 0])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ou,iters,brk", [((4, 4), 31, False), ((8, 3), 2000, True), ((2, 12), 9, False)])
+def test_hook_gs_ou_bitwise(cuda, dtype, ou, iters, brk):
+    import torch
+    from fadesim_amd.hook import ir_gs_ou
+    bsize, row, col, numX, numY = 2, 16, 12, 2, 2
+    inp, G = _case(dtype, bsize, row, col, numX, numY, seed=ou[0] + iters)
+    thr = 1e-9 if dtype == np.float64 else 1e-7
+    out = ir_gs_ou(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, ou[0], ou[1],
+                   numX, numY, iters, 1 / 2.93, 1.8, brk, thr, False, [])
+    ref, _ = oracle.hook_gs_ou(inp, G, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.8, brk, thr, False, [],
+                               ou_rows=ou[0], ou_cols=ou[1])
+    got = out.cpu().numpy()
+    assert got.dtype == dtype and got.shape == (bsize, numX * (row // ou[0]), numY * col)
+    assert np.array_equal(got, ref.reshape(got.shape))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ou,iters,brk", [((4, 4), 5, False), ((8, 3), 100, True)])
+def test_hook_ir2s_ou_bitwise(cuda, dtype, ou, iters, brk):
+    import torch
+    from fadesim_amd.hook import ir2s_ou
+    bsize, row, col, numX, numY = 2, 16, 12, 2, 2
+    inp, G = _case(dtype, bsize, row, col, numX, numY, seed=ou[1] + iters)
+    thr = 1e-9 if dtype == np.float64 else 1e-7
+    out = ir2s_ou(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, ou[0], ou[1],
+                  numX, numY, iters, 2.93, brk, thr, False, [])
+    ref, _ = oracle.hook_ir2s_ou(inp, G, bsize, row, col, numX, numY, iters, 2.93, brk, thr, False, [],
+                                 ou_rows=ou[0], ou_cols=ou[1])
+    got = out.cpu().numpy()
+    assert got.dtype == dtype and got.shape == (bsize, numX * (row // ou[0]), numY * col)
+    assert np.array_equal(got, ref.reshape(got.shape))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_hook_ir2s_local_loads_bitwise(cuda, dtype):
+    import torch
+    from fadesim_amd.hook import ir2s_ou_local
+    bsize, row, col, numX, numY = 2, 16, 12, 1, 2
+    inp, G = _case(dtype, bsize, row, col, numX, numY, seed=9)
+    out = ir2s_ou_local(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, numX, numY,
+                        6, 2.93, 2.93 * 4, 2.93 * 2, False, 0.0, False, [])
+    ref, _ = oracle.hook_ir2s_ou(inp, G, bsize, row, col, numX, numY, 6, 2.93, False, 0.0, False, [],
+                                 r_source=2.93 * 4, r_sink=2.93 * 2)
+    assert np.array_equal(out.cpu().numpy(), ref[:, :, 0])
+
+
+@pytest.mark.parametrize("f", ["gs", "ir2s"])
+def test_hook_ou_nonlinear(cuda, f):
+    import torch
+    from fadesim_amd.hook import ir_gs_ou, ir2s_ou
+    bsize, row, col, numX, numY = 2, 8, 8, 1, 2
+    rng = np.random.default_rng(14)
+    inp = rng.uniform(0.05, 0.3, (bsize, numX, row))
+    Ron, Roff, a = 1e3, 1e6, 2.5
+    G = 1.0 / rng.uniform(Ron, 1e4, (numX, row, numY * col))
+    nl = [Ron, Roff, a, 1 / (Ron * np.sinh(a)), a, 1 / (Roff * np.sinh(a))]
+    ti, tG = torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda)
+    if f == "gs":
+        out = ir_gs_ou(ti, tG, bsize, row, col, 4, 4, numX, numY, 40, 1 / 2.93, 1.5, False, 0.0, True, nl)
+        ref, _ = oracle.hook_gs_ou(inp, G, bsize, row, col, numX, numY, 40, 1 / 2.93, 1.5, False, 0.0, True, nl,
+                                   ou_rows=4, ou_cols=4)
+    else:
+        out = ir2s_ou(ti, tG, bsize, row, col, 4, 4, numX, numY, 8, 2.93, False, 0.0, True, nl)
+        ref, _ = oracle.hook_ir2s_ou(inp, G, bsize, row, col, numX, numY, 8, 2.93, False, 0.0, True, nl,
+                                     ou_rows=4, ou_cols=4)
+    np.testing.assert_allclose(out.cpu().numpy(), ref.reshape(out.shape), rtol=1e-12)
+
+
+def test_hook_ou_errors(cuda):
+    import torch
+    from fadesim_amd.hook import ir_gs_ou, ir2s_ou, ir_gs_ou_local, ir2s_ou_local
+    inp, G = _case(np.float32)
+    ti, tG = torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda)
+    with pytest.raises(RuntimeError, match="OU sizes"):
+        ir_gs_ou(ti, tG, 3, 16, 12, 5, 4, 2, 2, 10, 1.0, 1.0, False, 0.0, False, [])
+    with pytest.raises(RuntimeError, match="OU sizes"):
+        ir2s_ou(ti, tG, 3, 16, 12, 4, 1, 2, 2, 10, 1.0, False, 0.0, False, [])
+    with pytest.raises(RuntimeError, match="row_sizeshould be >=2"):
+        ir_gs_ou(ti, tG, 3, 1, 12, 4, 4, 2, 2, 10, 1.0, 1.0, False, 0.0, False, [])
+    with pytest.raises(RuntimeError, match="must be > 0"):
+        ir_gs_ou_local(ti, tG, 3, 16, 12, 2, 2, 10, 1.0, 0.0, 1.0, 1.0, False, 0.0, False, [])
+    with pytest.raises(RuntimeError, match="must be > 0"):
+        ir2s_ou_local(ti, tG, 3, 16, 12, 2, 2, 10, 1.0, 1.0, -1.0, False, 0.0, False, [])

@@ -48,3 +48,45 @@ def test_shape_validation_without_gpu():
         solve_batch(torch.ones((2, 48, 48)), 1.0, torch.zeros((2, 48)))
     with pytest.raises(ValueError):
         solve_batch(torch.ones((2, 64, 64)), 1.0, torch.zeros((2, 64)))  # not a GPU tensor
+
+
+# positional parameters of the extension's six entry points, in the order
+# pybind11 binds them (split_arr_matmul.cu:204-220, 419-433, 565-581, 690-706,
+# 953-969, 1197-1213; module table :1431-1436)
+_HOOK_SIGNATURES = {
+    "ir_drop_gs": ["input", "G", "batch_size", "row_size", "col_size", "numX", "numY", "iter_times", "g_wire",
+                   "beta", "enable_break", "break_threshould", "nonlinear_cell_enable", "nonlinear_params", "ret_V"],
+    "ir2s": ["input", "G", "batch_size", "row_size", "col_size", "numX", "numY", "iter_times", "r_wire",
+             "enable_break", "break_threshould", "nonlinear_cell_enable", "nonlinear_params", "ret_V"],
+    "ir2s_ou_local": ["input", "G", "batch_size", "row_size", "col_size", "numX", "numY", "iter_times", "r_wire",
+                      "r_load_left", "r_load_down", "enable_break", "break_threshould", "nonlinear_cell_enable",
+                      "nonlinear_params"],
+    "ir_gs_ou_local": ["input", "G", "batch_size", "row_size", "col_size", "numX", "numY", "iter_times", "g_wire",
+                       "g_load_left", "g_load_down", "beta", "enable_break", "break_threshould",
+                       "nonlinear_cell_enable", "nonlinear_params"],
+    "ir2s_ou": ["input", "G", "batch_size", "row_size", "col_size", "ou_row_size", "ou_col_size", "numX", "numY",
+                "iter_times", "r_wire", "enable_break", "break_threshould", "nonlinear_cell_enable",
+                "nonlinear_params"],
+    "ir_gs_ou": ["input", "G", "batch_size", "row_size", "col_size", "ou_row_size", "ou_col_size", "numX", "numY",
+                 "iter_times", "g_wire", "beta", "enable_break", "break_threshould", "nonlinear_cell_enable",
+                 "nonlinear_params"],
+}
+
+
+def test_hook_module_has_the_extension_entry_points():
+    import inspect
+    from fadesim_amd import hook
+    for name, params in _HOOK_SIGNATURES.items():
+        for holder in (hook, hook._HookModule):
+            f = getattr(holder, name)
+            assert list(inspect.signature(f).parameters) == params, name
+
+
+def test_hook_refuses_cpu_tensors():
+    import torch
+    from fadesim_amd import hook
+    inp, G = torch.zeros((1, 1, 4)), torch.ones((1, 4, 4))
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        hook.ir_gs_ou(inp, G, 1, 4, 4, 2, 2, 1, 1, 1, 1.0, 1.0, False, 0.0, False, [])
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        hook.ir2s_ou_local(inp, G, 1, 4, 4, 1, 1, 1, 1.0, 1.0, 1.0, False, 0.0, False, [])

@@ -553,3 +553,96 @@ def test_general_iterations_reach_the_exact_solve(name):
     vu2, vd2, io2, rec2, sw = oracle.rb_f64_general(g, RU, RD, on, v, 200000, 1.9, True, 1e-14)
     np.testing.assert_allclose(vu2, u, rtol=0, atol=1e-11)
     np.testing.assert_allclose(vd2, d, rtol=0, atol=1e-11)
+
+
+# --- separate source / sink loads and OU mode (split_arr_matmul.cu:546-1327) ---
+
+def _loads_segments(row, col, r, r_src, r_snk):
+    RU, RD = oracle.segments_uniform(row, col, r)
+    RU[:, 0] = r_src
+    RD[:, row] = r_snk
+    return RU, RD
+
+
+def test_hook_local_loads_converge_to_exact():
+    """ir_gs_ou_local / ir2s_ou_local: the fixed point is the exact network with
+    source segment 1/g_load_left and sink segment 1/g_load_down."""
+    rng = np.random.default_rng(21)
+    row, col, r = 10, 7, 2.93
+    G = rng.uniform(1e-4, 1e-3, (1, row, col))
+    inp = rng.uniform(0, 1, (1, 1, row))
+    r_src, r_snk = 3 * r, 5 * r
+    _, _, iex = oracle.exact(G[0], inp[0, 0], np.zeros(col), *_loads_segments(row, col, r, r_src, r_snk))
+    out, _ = oracle.hook_gs_ou(inp, G, 1, row, col, 1, 1, 100000, 1 / r, 1.8, True, 1e-15, False, [],
+                               g_load_left=1 / r_src, g_load_down=1 / r_snk)
+    np.testing.assert_allclose(out[0, 0, 0], iex, rtol=1e-10)
+    out, _ = oracle.hook_ir2s_ou(inp, G, 1, row, col, 1, 1, 300, r, False, 0.0, False, [],
+                                 r_source=r_src, r_sink=r_snk)
+    np.testing.assert_allclose(out[0, 0, 0], iex, rtol=1e-10)
+    # a plausible slip (loads swapped) is far off
+    _, _, iswap = oracle.exact(G[0], inp[0, 0], np.zeros(col), *_loads_segments(row, col, r, r_snk, r_src))
+    assert np.max(np.abs(iswap / iex - 1)) > 1e-4
+
+
+@pytest.mark.parametrize("ou", [(4, 3), (2, 6), (8, 2)])
+def test_ou_block_is_the_tile_with_only_that_block_on(ou):
+    """An OU block solved alone (position loads g_wire/(c0+1), g_wire/(row-r1))
+    is exactly the whole tile with every other cell switched off: its
+    converged currents equal the exact solve of the masked tile."""
+    rng = np.random.default_rng(ou[0] * 10 + ou[1])
+    row, col, numX, numY, r = 8, 6, 2, 2, 2.93
+    our, ouc = ou
+    G = rng.uniform(1e-4, 1e-3, (numX, row, numY * col))
+    inp = rng.uniform(0, 1, (2, numX, row))
+    gs, _ = oracle.hook_gs_ou(inp, G, 2, row, col, numX, numY, 100000, 1 / r, 1.8, True, 1e-16, False, [],
+                              ou_rows=our, ou_cols=ouc)
+    irs, _ = oracle.hook_ir2s_ou(inp, G, 2, row, col, numX, numY, 500, r, True, 0.0, False, [],
+                                 ou_rows=our, ou_cols=ouc)
+    RU, RD = oracle.segments_uniform(row, col, r)
+    for b in range(2):
+        for x in range(numX):
+            for y in range(numY):
+                tile = G[x, :, y * col:(y + 1) * col]
+                for oux in range(row // our):
+                    for ouy in range(col // ouc):
+                        m = np.zeros_like(tile)
+                        rs, cs = slice(oux * our, (oux + 1) * our), slice(ouy * ouc, (ouy + 1) * ouc)
+                        m[rs, cs] = tile[rs, cs]
+                        _, _, iex = oracle.exact(m, inp[b, x], np.zeros(col), RU, RD)
+                        o = y * col + ouy * ouc
+                        np.testing.assert_allclose(gs[b, x, oux, o:o + ouc], iex[cs], rtol=1e-9)
+                        np.testing.assert_allclose(irs[b, x, oux, o:o + ouc], iex[cs], rtol=1e-9)
+
+
+@pytest.mark.parametrize("f", ["gs", "ir2s"])
+def test_ou_iterates_are_independent_crossbars(f):
+    """Unconverged OU iterates equal the local form run on each block cut out
+    of its tile with the position loads (bitwise): every block its own
+    crossbar, same arithmetic."""
+    rng = np.random.default_rng(5)
+    row, col, numX, numY, our, ouc, r = 8, 6, 2, 3, 4, 3, 2.93
+    G = rng.uniform(1e-4, 1e-3, (numX, row, numY * col))
+    inp = rng.uniform(0, 1, (2, numX, row))
+    if f == "gs":
+        ou, _ = oracle.hook_gs_ou(inp, G, 2, row, col, numX, numY, 9, 1 / r, 1.7, False, 0.0, False, [],
+                                  ou_rows=our, ou_cols=ouc)
+    else:
+        ou, _ = oracle.hook_ir2s_ou(inp, G, 2, row, col, numX, numY, 3, r, False, 0.0, False, [],
+                                    ou_rows=our, ou_cols=ouc)
+    for b in range(2):
+        for x in range(numX):
+            for y in range(numY):
+                for oux in range(row // our):
+                    for ouy in range(col // ouc):
+                        r0, c0 = oux * our, ouy * ouc
+                        blk = np.ascontiguousarray(G[x, r0:r0 + our, y * col + c0:y * col + c0 + ouc])[None]
+                        vin = np.ascontiguousarray(inp[b, x, r0:r0 + our])[None, None]
+                        if f == "gs":
+                            loc, _ = oracle.hook_gs_ou(vin, blk, 1, our, ouc, 1, 1, 9, 1 / r, 1.7, False, 0.0,
+                                                       False, [], g_load_left=(1 / r) / (c0 + 1),
+                                                       g_load_down=(1 / r) / (row - (r0 + our - 1)))
+                        else:
+                            loc, _ = oracle.hook_ir2s_ou(vin, blk, 1, our, ouc, 1, 1, 3, r, False, 0.0, False, [],
+                                                         r_source=r * (c0 + 1), r_sink=r * (row - (r0 + our - 1)))
+                        o = y * col + c0
+                        assert np.array_equal(ou[b, x, oux, o:o + ouc], loc[0, 0, 0])

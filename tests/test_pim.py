@@ -108,3 +108,39 @@ def test_converged_mode_refuses_cpu():
     P = pim.PhyArrParams(ir_mode="converged")
     with pytest.raises(RuntimeError):
         pim.splitArr_matmul_nn(torch.rand(1, 10), torch.randn(10, 3), 8, 1, 10, 10, 8, params=P)
+
+
+@pytest.mark.parametrize("ou,seed", [((8, 8), 0), ((16, 4), 1)])
+def test_ideal_ou_mode_matches_restatement(ou, seed):
+    """OU mode with the ideal crossbar: one ADC reading per OU row block at
+    full scale adcOUuseI, summed after the ADC."""
+    gen = torch.Generator().manual_seed(20 + seed)
+    insize, outsize, rows = 150, 40, 64
+    x = torch.rand(3, insize, generator=gen) - 0.2
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="none", useOUSize=True, OUSize=ou)
+    out = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=P)
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=rows, cols=rows, vmax=P.inputMaxVoltage,
+                                  ron=P.Ron, roff=P.Roff, currents=opim.ideal_currents_ou(ou[0]), ou_rows=ou[0])
+    assert P.adcOUuseI == pytest.approx(ou[0] * (1e-3 - 1e-6))
+    assert P.arrOUnum == (rows // ou[0], rows // ou[1])
+    bound = _near_tie_bound(info, ou[0] / 1023, outsize)
+    assert np.max(np.abs(out.numpy() - ref)) <= bound + 1e-6 * np.abs(ref).max()
+    # the OU reading is a finer ADC: closer to the integer product than whole-array reads
+    qx, s1 = pim.quantization_tensor(x, 8, pim.TensorType.Normal)
+    qw, s2 = pim.quantization_tensor(w, 8, pim.TensorType.PN)
+    exact = (qx.double() @ qw.double()) * 2.0 ** (s1 + s2)
+    whole = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="none")).double()
+    assert torch.abs(out.double() - exact).mean() < torch.abs(whole - exact).mean()
+
+
+def test_ou_size_checks():
+    x, w = torch.rand(1, 10), torch.randn(10, 3)
+    with pytest.raises(Exception, match="not the factor"):
+        pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="none", useOUSize=True,
+                                                                                 OUSize=(6, 8)))
+    with pytest.raises(Exception, match="not less than 2"):
+        pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="none", useOUSize=True,
+                                                                                 OUSize=(8, 1)))
+    with pytest.raises(NotImplementedError):
+        pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="converged", useOUSize=True))

@@ -119,3 +119,31 @@ def test_mlp_inference_plain_vs_warm(cuda):
     s_warm = P.stats["sweeps"]
     assert torch.max(torch.abs(plain - warm)).item() <= 0.02 * torch.max(torch.abs(plain)).item()
     assert s_warm < s_plain
+
+
+@pytest.mark.parametrize("mode", ["reference", "ir2s"])
+def test_ou_mode_layer_matches_hook_restatement(cuda, mode):
+    """OU mode through the hook's OU solvers (ir_gs_ou / ir2s_ou) against the
+    restatement driven by their CPU replays."""
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(13)
+    insize, outsize, bsize, ou = 70, 20, 2, (16, 8)
+    x = torch.rand(bsize, insize, generator=gen) - 0.3
+    w = torch.randn(insize, outsize, generator=gen)
+    iters = 30 if mode == "reference" else 4
+    P = pim.PhyArrParams(ir_mode=mode, ir_drop_iter_time=iters, useOUSize=True, OUSize=ou)
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, params=P).cpu().numpy()
+
+    def hook_currents(vin, G):
+        bsize_, numX, R = vin.shape
+        numY = G.shape[2] // 64
+        args = (vin.astype(np.float32), G.astype(np.float32), bsize_, R, 64, numX, numY, iters)
+        if mode == "reference":
+            o, _ = oracle.hook_gs_ou(*args, 1 / 2.93, 1.90, False, 1e-7, False, [], ou_rows=ou[0], ou_cols=ou[1])
+        else:
+            o, _ = oracle.hook_ir2s_ou(*args, 2.93, False, 1e-7, False, [], ou_rows=ou[0], ou_cols=ou[1])
+        return o.reshape(bsize_, -1, G.shape[2])
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=hook_currents, ou_rows=ou[0])
+    bound = _tie_bound(info, ou[0] / 1023, 1e-4)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
