@@ -61,7 +61,7 @@
 namespace fadesim {
 namespace rb {
 
-constexpr int kWsPlanes = 6; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q | IR2S currents
+constexpr int kWsPlanes = 6; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q | streamed currents
 constexpr int kMaxStrips = 16;
 constexpr int kRowsLd = 512;                      // row stride of the strip exchange buffers
 constexpr unsigned long long kWaitTicks = 500000000ull; // 5 s of the 100 MHz constant clock
@@ -439,6 +439,46 @@ __device__ __forceinline__ void row_green_row(const double (&v)[G::TC], int j0, 
     }
 }
 
+// FP32 forms for the wire-solve iterations of the initial guess, which only
+// need the drops to ~1e-5 relative (the drops are ~1e-3 of the cell voltage,
+// and every result is certified by the FP64 test).
+template <int TX>
+__device__ __forceinline__ void row_scan_f(float x, int tx, float &excl, float &total)
+{
+    float incl = x;
+#pragma unroll
+    for (int d = 1; d < TX; d <<= 1) {
+        const float y = __shfl_up(incl, (unsigned)d, TX);
+        if (tx >= d)
+            incl += y;
+    }
+    total = __shfl(incl, TX - 1, TX);
+    excl = incl - x;
+}
+
+template <class G>
+__device__ __forceinline__ void row_green_row_f(const float (&v)[G::TC], int j0, int tx, float scale,
+                                                float (&out)[G::TC])
+{
+    constexpr int TC = G::TC;
+    float L0 = 0.f, L1 = 0.f;
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        L0 += v[jj];
+        L1 += (float)(j0 + jj + 1) * v[jj];
+    }
+    float P0, P1, R, R1;
+    row_scan_f<G::TX>(L0, tx, P0, R);
+    row_scan_f<G::TX>(L1, tx, P1, R1);
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        const float jp1 = (float)(j0 + jj + 1);
+        out[jj] = scale * (P1 + jp1 * (R - P0));
+        P0 += v[jj];
+        P1 += jp1 * v[jj];
+    }
+}
+
 // Column sums over the workgroup's rows of two per-thread partials (thread
 // row ty, columns j0 + jj): outA[j], outB[j] (LDS, N entries each).
 template <class G>
@@ -465,28 +505,17 @@ __device__ __forceinline__ void col_reduce2(const Shared &sh, const double (&a)[
     __syncthreads();
 }
 
-// Bitline Green's sum of a tile: emit(ii, jj, scale sum_i' (n - max(i,i')) v(i',j))
-// over all n rows of the crossbar (i = gi0 + i0 + ii, j = j0 + jj), v given by
-// vf(ii, jj).  Strips complete the column prefix over the team (team barrier
-// nbar++).  Returns false when the team aborted.
-template <class G, bool STRIP, class VF, class EM>
-__device__ __forceinline__ bool col_green_emit(const Shared &sh, const StripCtx &X, VF vf, double scale, int n,
-                                               int gi0, int i0, int j0, int ty, int t, int &nbar, EM emit)
+// The middle of a bitline Green's sum: with every thread row's column partials
+// (sum v, sum v (n - i)) in cbuf [2][TY][N], turn them in place into
+// exclusive prefixes over the thread rows and put the prefix of the rows
+// above this workgroup (strips: over the team, team barrier nbar++) and the
+// crossbar total of sum v (n - i) in cext [3][N].  Returns false when the
+// team aborted.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool col_scan_exchange(const Shared &sh, const StripCtx &X, int t, int &nbar)
 {
-    constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
+    constexpr int TY = G::TY, N = G::N;
     double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
-#pragma unroll
-    for (int jj = 0; jj < TC; ++jj) {
-        double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-        for (int ii = 0; ii < TR; ++ii) {
-            const double v = vf(ii, jj);
-            s0 += v;
-            s1 += v * (double)(n - (gi0 + i0 + ii));
-        }
-        b0[ty * N + j0 + jj] = s0;
-        b1[ty * N + j0 + jj] = s1;
-    }
     __syncthreads();
     // exclusive prefix over the thread rows, in place; totals of this workgroup
     double T0 = 0.0, T1 = 0.0;
@@ -526,36 +555,55 @@ __device__ __forceinline__ bool col_green_emit(const Shared &sh, const StripCtx 
         sh.cext[2 * N + t] = Tall;
     }
     __syncthreads();
+    return true;
+}
+
+// Bitline Green's sum of a float tile: out(ii, jj) = scale sum_i' (n -
+// max(i,i')) v(i',j) over all n rows of the crossbar (i = gi0 + i0 + ii,
+// j = j0 + jj).  The sums across thread rows are FP64 (strips complete them
+// over the team, team barrier nbar++); within the tile FP32 from the FP64
+// anchors -- the current entering the tile and the weighted sum below it --
+// so no difference of large FP32 sums is formed.  Returns false when the
+// team aborted.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool col_green_tile_f(const Shared &sh, const StripCtx &X, const float (&v)[G::TR][G::TC],
+                                                 float scale, int n, int gi0, int i0, int j0, int ty, int t, int &nbar,
+                                                 float (&out)[G::TR][G::TC])
+{
+    constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
+    double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            s0 += v[ii][jj];
+            s1 += v[ii][jj] * (float)(n - (gi0 + i0 + ii));
+        }
+        b0[ty * N + j0 + jj] = (double)s0;
+        b1[ty * N + j0 + jj] = (double)s1;
+    }
+    if (!col_scan_exchange<G, STRIP>(sh, X, t, nbar))
+        return false;
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
         const int j = j0 + jj;
-        double A = sh.cext[j] + b0[ty * N + j];
-        double C = sh.cext[N + j] + b1[ty * N + j];
-        const double T = sh.cext[2 * N + j];
+        const float A0 = (float)(sh.cext[j] + b0[ty * N + j]);
+        const float Tb0 = (float)(sh.cext[2 * N + j] - (sh.cext[N + j] + b1[ty * N + j]));
+        float al = 0.f, cl = 0.f;
 #pragma unroll
         for (int ii = 0; ii < TR; ++ii) {
-            const double gi = (double)(n - (gi0 + i0 + ii));
-            const double v = vf(ii, jj);
-            A += v;
-            C += v * gi;
-            emit(ii, jj, scale * (gi * A + (T - C)));
+            const float gi = (float)(n - (gi0 + i0 + ii));
+            al += v[ii][jj];
+            cl += v[ii][jj] * gi;
+            out[ii][jj] = scale * (gi * (A0 + al) + (Tb0 - cl));
         }
     }
     __syncthreads();
     return true;
 }
 
-// col_green_emit into a float tile.
-template <class G, bool STRIP, class VF>
-__device__ __forceinline__ bool col_green_tile(const Shared &sh, const StripCtx &X, VF vf, double scale, int n,
-                                               int gi0, int i0, int j0, int ty, int t, int &nbar,
-                                               float (&out)[G::TR][G::TC])
-{
-    return col_green_emit<G, STRIP>(sh, X, vf, scale, n, gi0, i0, j0, ty, t, nbar,
-                                    [&](int ii, int jj, double v) { out[ii][jj] = (float)v; });
-}
-
-// col_green_emit streamed by tile rows: src(ii, float (&)[TC]) supplies row ii
+// Bitline Green's sum in FP64 streamed by tile rows: src(ii, float (&)[TC]) supplies row ii
 // of v, emit(ii, const double (&)[TC]) takes row ii of the result.  The tile
 // rows are not unrolled, so only O(TC) values are live: for the phases whose
 // tile state lives in the workspace.
@@ -585,44 +633,8 @@ __device__ __forceinline__ bool col_green_rows(const Shared &sh, const StripCtx 
         b0[ty * N + j0 + jj] = s0[jj];
         b1[ty * N + j0 + jj] = s1[jj];
     }
-    __syncthreads();
-    double T0 = 0.0, T1 = 0.0;
-    if (t < N) {
-        for (int q = 0; q < TY; ++q) {
-            const double x0 = b0[q * N + t], x1 = b1[q * N + t];
-            b0[q * N + t] = T0;
-            b1[q * N + t] = T1;
-            T0 += x0;
-            T1 += x1;
-        }
-    }
-    double E0 = 0.0, E1 = 0.0, Tall = T1;
-    if (STRIP) {
-        double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
-        if (t < N) {
-            mine[t] = T0;
-            mine[kRowsLd + t] = T1;
-        }
-        if (!team_barrier(X, sh.dead, t, nbar++))
-            return false;
-        if (t < N) {
-            Tall = 0.0;
-            for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.part + (((size_t)((nbar - 1) & 1) * X.S + s) * 2) * kRowsLd;
-                if (s < X.s) {
-                    E0 += ps[t];
-                    E1 += ps[kRowsLd + t];
-                }
-                Tall += ps[kRowsLd + t];
-            }
-        }
-    }
-    if (t < N) {
-        sh.cext[t] = E0;
-        sh.cext[N + t] = E1;
-        sh.cext[2 * N + t] = Tall;
-    }
-    __syncthreads();
+    if (!col_scan_exchange<G, STRIP>(sh, X, t, nbar))
+        return false;
     double A[TC], C[TC];
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
@@ -642,6 +654,85 @@ __device__ __forceinline__ bool col_green_rows(const Shared &sh, const StripCtx 
             o[jj] = scale * (gi * A[jj] + (sh.cext[2 * N + j0 + jj] - C[jj]));
         }
         emit(ii, o);
+    }
+    __syncthreads();
+    return true;
+}
+
+// One damped wire-solve iteration on a register tile (p, q):
+//   p <- (1-w) p + w r sum_j' (min(j,j')+1) a(i,j')     (every wordline solved)
+//   q <- (1-w) q + w r sum_i' (n - max(i,i')) b(i',j)   (every bitline solved)
+// for the currents of the present (p, q): cur(ii, prow, qrow, a, b) gives
+// tile row ii of a and b.  The currents are evaluated twice -- once for the
+// column partials, once in the pass that writes the new p and q row by row --
+// so no current tile is held: only p, q and the column accumulators are
+// live.  FP32 arithmetic, with the bitline sums across thread rows in FP64.
+// Returns false when the team aborted.
+template <class G, bool STRIP, class CUR>
+__device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx &X, CUR cur, double rwire, double w,
+                                               int n, int gi0, int i0, int j0, int tx, int ty, int t, int &nbar,
+                                               float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC])
+{
+    constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
+    double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
+    const float wf = (float)w, w1f = (float)(1.0 - w), rf = (float)rwire;
+    // opaque to the optimizer: the index arithmetic below is redone in every
+    // iteration of the caller's loop instead of being hoisted out of it and
+    // held in registers, which would spill
+    asm volatile("" : "+v"(i0), "+v"(j0));
+    {
+        float s0[TC], s1[TC];
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            s0[jj] = s1[jj] = 0.f;
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            float a[TC], b[TC];
+            cur(ii, p[ii], q[ii], a, b);
+            const float gi = (float)(n - (gi0 + i0 + ii));
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj) {
+                s0[jj] += b[jj];
+                s1[jj] += b[jj] * gi;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            b0[ty * N + j0 + jj] = (double)s0[jj];
+            b1[ty * N + j0 + jj] = (double)s1[jj];
+        }
+    }
+    if (!col_scan_exchange<G, STRIP>(sh, X, t, nbar))
+        return false;
+    // anchors of this thread row in FP64 -- the bitline current entering it
+    // (sum over the rows above) and the remaining weighted sum below it --
+    // then FP32 within the tile: q = r (gi (A0 + a) + (Tb0 - c)), no
+    // difference of large sums in FP32
+    float A0[TC], Tb0[TC], al[TC], cl[TC];
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        const double A = sh.cext[j0 + jj] + b0[ty * N + j0 + jj];
+        const double C = sh.cext[N + j0 + jj] + b1[ty * N + j0 + jj];
+        A0[jj] = (float)A;
+        Tb0[jj] = (float)(sh.cext[2 * N + j0 + jj] - C);
+        al[jj] = cl[jj] = 0.f;
+    }
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        float a[TC], b[TC], out[TC];
+        cur(ii, p[ii], q[ii], a, b);
+        const float gi = (float)(n - (gi0 + i0 + ii));
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            al[jj] += b[jj];
+            cl[jj] += b[jj] * gi;
+            const float o = rf * (gi * (A0[jj] + al[jj]) + (Tb0[jj] - cl[jj]));
+            q[ii][jj] = w1f * q[ii][jj] + wf * o;
+        }
+        row_green_row_f<G>(a, j0, tx, rf, out);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            p[ii][jj] = w1f * p[ii][jj] + wf * out[jj];
     }
     __syncthreads();
     return true;
@@ -699,8 +790,7 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
         // a column carrying under 1e-9 of its driven current is judged absolutely
         sh.Ifloor[t] = fmax(1e-9 * sh.Ifloor[t], 1e-300);
     float zq[TR][TC];
-    if (!col_green_tile<G, STRIP>(sh, X, [&](int ii, int jj) { return (double)gt[ii][jj]; }, rwire, I.rows, gi0, i0,
-                                  j0, ty, t, nbar, zq))
+    if (!col_green_tile_f<G, STRIP>(sh, X, gt, (float)rwire, I.rows, gi0, i0, j0, ty, t, nbar, zq))
         return false;
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii)
@@ -898,6 +988,21 @@ __device__ __forceinline__ T *uniform_ptr(T *p)
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
+// LDS arrays cross a function boundary as 32-bit LDS addresses: a pointer
+// rebuilt from one is known to point into LDS, so the callee addresses it
+// with ds_read / ds_write and a 32-bit offset instead of 64-bit flat addresses.
+template <class T>
+__device__ __forceinline__ unsigned lds_off(T *p)
+{
+    typedef __attribute__((address_space(3))) T lds_t;
+    return (unsigned)(size_t)(lds_t *)p;
+}
+template <class T>
+__device__ __forceinline__ T *lds_at(unsigned off)
+{
+    typedef __attribute__((address_space(3))) T lds_t;
+    return (T *)(lds_t *)(size_t)__builtin_amdgcn_readfirstlane(off);
+}
 __device__ __forceinline__ double uniform_f64(double x)
 {
     const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
@@ -915,15 +1020,30 @@ __device__ __forceinline__ void uniform_inst(Inst &I)
     I.rows = uniform_i32(I.rows);
 }
 
+__device__ __forceinline__ void uniform_strip(StripCtx &X)
+{
+    X.S = uniform_i32(X.S);
+    X.s = uniform_i32(X.s);
+    X.rows = uniform_ptr(X.rows);
+    X.flag = uniform_ptr(X.flag);
+    X.cnt = uniform_ptr(X.cnt);
+    X.part = uniform_ptr(X.part);
+    X.setup = uniform_ptr(X.setup);
+    X.abort = uniform_ptr(X.abort);
+}
+
 #define RB_SH_PARAMS                                                                                    \
-    float *sh_halo, double *sh_S1, double *sh_Ifloor, double *sh_colI, double *sh_colD, float *sh_wmax, \
-        int *sh_dead, double *sh_cbuf, double *sh_cext
-#define RB_SH_ARGS(s) s.halo, s.S1, s.Ifloor, s.colI, s.colD, s.wmax, s.dead, s.cbuf, s.cext
+    unsigned sh_halo, unsigned sh_S1, unsigned sh_Ifloor, unsigned sh_colI, unsigned sh_colD,                \
+        unsigned sh_wmax, unsigned sh_dead, unsigned sh_cbuf, unsigned sh_cext
+#define RB_SH_ARGS(s)                                                                                      \
+    lds_off(s.halo), lds_off(s.S1), lds_off(s.Ifloor), lds_off(s.colI), lds_off(s.colD), lds_off(s.wmax),    \
+        lds_off(s.dead), lds_off(s.cbuf), lds_off(s.cext)
 #define RB_SH_LOCAL                                                                                     \
-    Shared sh{uniform_ptr(sh_halo), uniform_ptr(sh_S1),   uniform_ptr(sh_Ifloor), uniform_ptr(sh_colI),     \
-              uniform_ptr(sh_colD), uniform_ptr(sh_wmax), uniform_ptr(sh_dead),   uniform_ptr(sh_cbuf),     \
-              uniform_ptr(sh_cext)};                                                                         \
+    Shared sh{lds_at<float>(sh_halo),  lds_at<double>(sh_S1),   lds_at<double>(sh_Ifloor),                 \
+              lds_at<double>(sh_colI), lds_at<double>(sh_colD), lds_at<float>(sh_wmax),                    \
+              lds_at<int>(sh_dead),    lds_at<double>(sh_cbuf), lds_at<double>(sh_cext)};                  \
     uniform_inst(I);                                                                                         \
+    uniform_strip(X);                                                                                        \
     wsb = uniform_ptr(wsb);                                                                                  \
     gp = uniform_ptr(gp);                                                                                    \
     gi0 = uniform_i32(gi0)
@@ -985,14 +1105,15 @@ __device__ __forceinline__ void tile_store(float *wsb, int i0, int j0, const flo
 // contracts by rho, and the damped one, x <- (1-w) x + w M^-1 (b - K x), by
 // max(|1-w|, |1-w(1+rho)|) -- rho/(2+rho) at w = 2/(2+rho).  Returns false
 // when the team aborted.
-// The tile is streamed by rows through the workspace (the currents of an
-// iteration through plane 5), so only O(TC) values are live.
+// Streamed forms of the two wire-solve phases, for strips (whose team
+// exchanges leave too few registers for the tile): tile rows go through the
+// workspace -- drops in planes 3, 4, currents in plane 5 -- so only O(TC)
+// values are live, at ~36 bytes of workspace traffic per cell and iteration.
 template <class G, bool STRIP>
-__device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
-                                                     int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
-                                                     int *nbar, int iters, double w)
+__device__ __forceinline__ bool ir2s_stream(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
+                                            int i0, int j0, int t, int tx, int ty, double rwire, const Shared &sh,
+                                            int &nbar, int iters, double w)
 {
-    RB_SH_LOCAL;
     constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
     float *P = wsb + 3 * M * N, *Q = wsb + 4 * M * N, *S = wsb + 5 * M * N;
     const double w1 = 1.0 - w;
@@ -1021,7 +1142,7 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
         }
         if (!col_green_rows<G, STRIP>(
                 sh, X, [&](int ii, float (&v)[TC]) { ldrow<TC>(S + (size_t)(i0 + ii) * N + j0, v); }, rwire, I.rows,
-                gi0, i0, j0, ty, t, *nbar, [&](int ii, const double (&o)[TC]) {
+                gi0, i0, j0, ty, t, nbar, [&](int ii, const double (&o)[TC]) {
                     float qr[TC];
                     float *q = Q + (size_t)(i0 + ii) * N + j0;
                     ldrow<TC>(q, qr);
@@ -1035,21 +1156,11 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
     return true;
 }
 
-// Initial guess of the refinement correction y (A y = residual), into planes 3, 4:
-// `iters` wire-solve iterations from zero, damped by w as in phase_ir2s,
-//   y_{k+1} = (1-w) y_k + w (aL)^-1 (res - G (y_k,p + y_k,q))
-// with (aL)^-1 the wordline (r (min(j,j')+1)) and bitline (r (n - max(i,i')))
-// line inverses; the cell coupling G is ~1e-6 of the wire conductance, so
-// two iterations already put y within a few 1e-3 of the correction and the
-// correction converges in a few sweeps instead of restarting from zero.
-// Reads the scaled residual (planes 1, 2); streamed by tile rows like
-// phase_ir2s.
 template <class G, bool STRIP>
-__device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
-                                                      int i0, int j0, int t, int tx, int ty, double rwire,
-                                                      double hcoef, RB_SH_PARAMS, int *nbar, int iters, double w)
+__device__ __forceinline__ bool green_stream(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
+                                             int i0, int j0, int t, int tx, int ty, double rwire, double hcoef,
+                                             const Shared &sh, int &nbar, int iters, double w)
 {
-    RB_SH_LOCAL;
     constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
     float *P = wsb + 3 * M * N, *Q = wsb + 4 * M * N, *S = wsb + 5 * M * N;
     const float *RP = wsb + 1 * M * N, *RQ = wsb + 2 * M * N;
@@ -1094,7 +1205,7 @@ __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float 
         }
         if (!col_green_rows<G, STRIP>(
                 sh, X, [&](int ii, float (&v)[TC]) { ldrow<TC>(S + (size_t)(i0 + ii) * N + j0, v); }, rwire, I.rows,
-                gi0, i0, j0, ty, t, *nbar, [&](int ii, const double (&o)[TC]) {
+                gi0, i0, j0, ty, t, nbar, [&](int ii, const double (&o)[TC]) {
                     float qr[TC];
                     float *q = Q + (size_t)(i0 + ii) * N + j0;
                     ldrow<TC>(q, qr);
@@ -1105,6 +1216,113 @@ __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float 
                 }))
             return false;
     }
+    return true;
+}
+
+// The caller's sweep state is dead across the call (it is reloaded from
+// planes 3, 4 afterwards), so the drops stay in VGPRs for all iterations;
+// from_zero starts them at zero instead of reading planes 3, 4 (strips always
+// read them)
+// (wire_iteration); per iteration only the conductance is read, twice, from
+// the cache.
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                     int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
+                                                     int *nbar, int iters, double w, bool from_zero)
+{
+    RB_SH_LOCAL;
+    if constexpr (STRIP)
+        return ir2s_stream<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, sh, *nbar, iters, w);
+    constexpr int TC = G::TC;
+    float p[G::TR][TC], q[G::TR][TC];
+    if (from_zero) {
+#pragma unroll
+        for (int ii = 0; ii < G::TR; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                p[ii][jj] = q[ii][jj] = 0.f;
+    } else {
+        tile_load<G>(wsb, i0, j0, p, q);
+    }
+    // the tile's conductances and line voltages stay in registers for all
+    // iterations: an iteration reads no memory but LDS
+    float gt[G::TR][TC], vi[G::TR], vo[TC];
+#pragma unroll
+    for (int ii = 0; ii < G::TR; ++ii) {
+        ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
+        vi[ii] = (float)I.vin[gi0 + i0 + ii];
+    }
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj)
+        vo[jj] = (float)I.vout[j0 + jj];
+    // s_ij = g_ij (c_ij - p_ij - q_ij) for both line solves
+    auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&b)[TC]) {
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            a[jj] = b[jj] = gt[ii][jj] * (((vi[ii] - vo[jj]) - pr[jj]) - qr[jj]);
+    };
+#pragma unroll 1
+    for (int k = 0; k < iters; ++k) {
+        // the cell voltages vi - vo are recomputed in every iteration rather
+        // than hoisted out of the loop as a register tile (which would spill)
+#pragma unroll
+        for (int ii = 0; ii < G::TR; ++ii)
+            asm volatile("" : "+v"(vi[ii]));
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            asm volatile("" : "+v"(vo[jj]));
+        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, *nbar, p, q))
+            return false;
+    }
+    tile_store<G>(wsb, i0, j0, p, q);
+    return true;
+}
+
+// Initial guess of the refinement correction y (A y = residual), into planes 3, 4:
+// `iters` wire-solve iterations from zero, damped by w as in phase_ir2s,
+//   y_{k+1} = (1-w) y_k + w (aL)^-1 (res - G (y_k,p + y_k,q))
+// with (aL)^-1 the wordline (r (min(j,j')+1)) and bitline (r (n - max(i,i')))
+// line inverses; the cell coupling G is ~1e-6 of the wire conductance, so
+// two iterations already put y within a few 1e-3 of the correction and the
+// correction converges in a few sweeps instead of restarting from zero.
+// Reads the scaled residual (planes 1, 2); the correction stays in VGPRs.
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                      int i0, int j0, int t, int tx, int ty, double rwire,
+                                                      double hcoef, RB_SH_PARAMS, int *nbar, int iters, double w)
+{
+    RB_SH_LOCAL;
+    if constexpr (STRIP)
+        return green_stream<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, sh, *nbar, iters, w);
+    constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
+    const float *RP = wsb + 1 * M * N, *RQ = wsb + 2 * M * N;
+    const float ih = (float)(1.0 / hcoef);
+    float p[TR][TC], q[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            p[ii][jj] = q[ii][jj] = 0.f;
+    (void)I;
+    // y starts at zero, so the first iteration's coupling term is exactly 0
+    auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&b)[TC]) {
+        const size_t o = (size_t)(i0 + ii) * N + j0;
+        float gr[TC], rp[TC], rq[TC];
+        ldrow<TC>(RP + o, rp);
+        ldrow<TC>(RQ + o, rq);
+        ldrow<TC>(gp + o, gr);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const float cpl = gr[jj] * (pr[jj] + qr[jj]);
+            a[jj] = rp[jj] * ih - cpl;
+            b[jj] = rq[jj] * ih - cpl;
+        }
+    };
+#pragma unroll 1
+    for (int k = 0; k < iters; ++k)
+        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, *nbar, p, q))
+            return false;
+    tile_store<G>(wsb, i0, j0, p, q);
     return true;
 }
 
@@ -1145,7 +1363,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
     if (!phase_setup<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, RB_SH_ARGS(sh), &nbar))
         return false;
 
-    float yp[TR][TC], yq[TR][TC], hn[TR][TC], rp[TR][TC], rq[TR][TC];
+    float yp[TR][TC], yq[TR][TC];
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
         if (I.warm) {
@@ -1157,12 +1375,17 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 yp[ii][jj] = yq[ii][jj] = 0.f;
         }
     }
+    // planes 3, 4 hold the present drops (no snapshot store needed before a test)
+    bool snap_valid = false;
     if (C.ir2s_iters > 0) {
-        tile_store<G>(wsb, i0, j0, yp, yq);
+        const bool from_zero = !STRIP && !I.warm;
+        if (!from_zero)
+            tile_store<G>(wsb, i0, j0, yp, yq);
         if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
-                                  C.ir2s_omega))
+                                  C.ir2s_omega, from_zero))
             return false;
         tile_load<G>(wsb, i0, j0, yp, yq);
+        snap_valid = true;
     }
     publish_all<G>(yp, yq, halo, t);
     if (STRIP)
@@ -1180,25 +1403,34 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
     const double stop_tol = dwr ? C.tol_eff * C.dwr_scale : C.tol_eff;
 
     for (;;) {
-        load_coeffs<G>(I, wsb, gp, i0, j0, gi0, hcoef, refined, hn, rp, rq);
-        while (sweep < next_check) {
-            if (STRIP && !strip_fetch<G>(X, halo, sh.dead, tx, ty, pub - 1))
-                return false;
-            half_sweep<0, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
-            if (STRIP) {
-                strip_publish<G>(X, yq, ty, j0, lane, pub++);
-                if (!strip_fetch<G>(X, halo, sh.dead, tx, ty, pub - 1))
+        if (sweep < next_check) {
+            // the sweep coefficients live only here: loaded for sweeps, dead
+            // across the test
+            float hn[TR][TC], rp[TR][TC], rq[TR][TC];
+            load_coeffs<G>(I, wsb, gp, i0, j0, gi0, hcoef, refined, hn, rp, rq);
+            snap_valid = false;
+            while (sweep < next_check) {
+                if (STRIP && !strip_fetch<G>(X, halo, sh.dead, tx, ty, pub - 1))
                     return false;
+                half_sweep<0, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
+                if (STRIP) {
+                    strip_publish<G>(X, yq, ty, j0, lane, pub++);
+                    if (!strip_fetch<G>(X, halo, sh.dead, tx, ty, pub - 1))
+                        return false;
+                }
+                half_sweep<1, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
+                if (STRIP)
+                    strip_publish<G>(X, yq, ty, j0, lane, pub++);
+                ++sweep;
             }
-            half_sweep<1, G>(yp, yq, hn, rp, rq, halo, t, has_l, has_r, has_u, has_d, cA, cB, c2);
-            if (STRIP)
-                strip_publish<G>(X, yq, ty, j0, lane, pub++);
-            ++sweep;
         }
+        if (!snap_valid) {
 #pragma unroll
-        for (int ii = 0; ii < TR; ++ii) { // snapshot: the sweep state is not live across the test
-            strow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
-            strow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+            for (int ii = 0; ii < TR; ++ii) { // snapshot: the sweep state is not live across the test
+                strow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
+                strow<TC>(wsb + 4 * M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+            }
+            snap_valid = true;
         }
         est = phase_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, false, has_l,
                                    has_r, has_u, has_d, RB_SH_ARGS(sh), pub - 1, &nbar);
@@ -1237,6 +1469,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
 #pragma unroll
                     for (int jj = 0; jj < TC; ++jj)
                         yp[ii][jj] = yq[ii][jj] = 0.f;
+                snap_valid = false;
             } else {
                 if (!phase_green<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, RB_SH_ARGS(sh), &nbar,
                                            C.corr_iters, C.ir2s_omega))
@@ -1257,7 +1490,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             st = FADESIM_ST_MAXIT;
             done = true;
         }
-        if (reload) {
+        if (reload && (!done || I.field)) {
 #pragma unroll
             for (int ii = 0; ii < TR; ++ii) {
                 ldrow<TC>(wsb + 3 * M * N + (size_t)(i0 + ii) * N + j0, yp[ii]);
