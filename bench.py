@@ -13,7 +13,8 @@ zero drops (plain SOR), K wire-solve (IR2S) iterations inside the solve
 kernel (K from the autotuned table), and, when trained weights are present
 (fadesim_amd/weights), the learned warm start with the network's inference
 inside the timed step plus one IR2S iteration on its field (`value`, the
-metric's NN-warm-start figure); mean sweeps of each; the roofline of the
+metric's NN-warm-start figure; "nn_unet16" the same with the 4-level
+full-resolution network); mean sweeps of each; the roofline of the
 solve kernel from HIP events on its launch stream; the CPU baseline (the
 oracle's restatement of the reference's lexicographic solver, OpenMP over
 instances) on a bounded sample, and the measured error of every variant on
@@ -192,7 +193,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-images", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=256)
-    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn: run only these solve variants")
+    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn,nn_unet16: run only these solve variants")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -236,6 +237,13 @@ def main():
         variants["nn"] = (1, lambda timing=False: solve_batch(tg, tr, tv, warm=net.predict(tg, tr, tv), tol=args.tol,
                                                               ir2s_iters=1, corr_iters=K, test_first=True,
                                                               timing=timing))
+    # the 4-level full-resolution encoder-decoder (~340k parameters), for comparison
+    net16 = None
+    if os.path.exists(os.path.join(warmstart.WEIGHTS_DIR, "warmstart_128_unet16.bin")):
+        net16 = warmstart.WarmStartNet.load(dev, name="warmstart_128_unet16")
+        variants["nn_unet16"] = (1, lambda timing=False: solve_batch(tg, tr, tv, warm=net16.predict(tg, tr, tv),
+                                                                     tol=args.tol, ir2s_iters=1, corr_iters=K,
+                                                                     test_first=True, timing=timing))
     if only:
         variants = {k: v for k, v in variants.items() if k in only}
 
@@ -277,9 +285,15 @@ def main():
                       "initial_guess": {"nn": "nn-warm-start (+1 IR2S iteration)", "ir2s": f"{K} IR2S iterations",
                                         "plain": "zero-drop"}[head]},
            "plain": meas.get("plain"), "ir2s_warm": meas.get("ir2s"),
-           "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None}
+           "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None,
+           "warm_start_unet16": dict(meas["nn_unet16"], network=net16.describe()) if "nn_unet16" in meas else None}
     if "plain" in meas and "nn" in meas:
-        out["sweep_reduction"] = meas["plain"]["mean_sweeps"] / max(meas["nn"]["mean_sweeps"], 1e-9)
+        ps, ns = meas["plain"]["mean_sweeps"], meas["nn"]["mean_sweeps"]
+        # SOR sweeps per solve, plain vs NN-warm (the NN-warm solve also runs one
+        # wire-solve pass); the ratio is null when the NN-warm solve needs none
+        out["sweeps_per_solve"] = {"plain": ps, "nn": ns}
+        out["sweep_reduction"] = ps / ns if ns > 0 else None
+        out["iteration_reduction"] = ps / (ns + meas["nn"]["ir2s_iters"])
         out["nn_vs_plain_speedup"] = meas["nn"]["solves_per_s"] / meas["plain"]["solves_per_s"]
     if "plain" in meas and "ir2s" in meas:
         out["ir2s_vs_plain_speedup"] = meas["ir2s"]["solves_per_s"] / meas["plain"]["solves_per_s"]
@@ -306,7 +320,8 @@ def main():
             elif name == "ir2s":
                 rr = solve_batch(tgs, trs, tvs, tol=args.tol, ir2s_iters=K, corr_iters=K)
             else:
-                rr = solve_batch(tgs, trs, tvs, warm=net.predict(tgs, trs, tvs), tol=args.tol, ir2s_iters=1,
+                nw = net16 if name == "nn_unet16" else net
+                rr = solve_batch(tgs, trs, tvs, warm=nw.predict(tgs, trs, tvs), tol=args.tol, ir2s_iters=1,
                                  corr_iters=K, test_first=True)
             errs[name] = float(np.max(np.abs(rr.iout.cpu().numpy() - iex) / np.abs(iex)))
         out["max_rel_err"] = errs

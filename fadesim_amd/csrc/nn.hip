@@ -23,6 +23,7 @@
 #include "common.h"
 #include "fadesim.h"
 #include "fadesim_nn.h"
+#include "rb_core.h"
 
 namespace fadesim {
 namespace {
@@ -37,6 +38,138 @@ __device__ __forceinline__ bf16x8 zero8()
     for (int e = 0; e < 8; ++e)
         z[e] = (__bf16)0.0f;
     return z;
+}
+
+// ---------------------------------------------------------------- pooled features
+// The same features for 128x128 crossbars, averaged over 4x4 cells (the
+// network's 32x32 input), with the drop fields computed the solver's way:
+// each thread holds a 4x8 register tile, (p1, q1) and (p2, q2) are two
+// undamped wire-solve iterations from zero (rb::wire_iteration: line scans
+// by lane shuffles, column sums across thread rows in FP64).  A thread's tile
+// is exactly two 4x4 pooling cells, so the pooled features come out of
+// registers; base = (p2, q2) at full resolution.
+using FeatGeo = rb::Geo<4, 8, 32, 16>;
+
+__global__ void __launch_bounds__(FeatGeo::NT) features_pooled_kernel(const float *__restrict__ g,
+                                                                     const double *__restrict__ rw,
+                                                                     const double *__restrict__ vin,
+                                                                     __bf16 *__restrict__ feat,
+                                                                     float *__restrict__ base,
+                                                                     float *__restrict__ scale)
+{
+    using G = FeatGeo;
+    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY, M = G::M, N = G::N, NT = G::NT;
+    constexpr int PW = N / 4; // pooled width
+    __shared__ double cbuf[2 * TY * N];
+    __shared__ double cext[3 * N];
+    __shared__ int dead;
+    __shared__ float red[4][NT / 64];
+    rb::Shared sh{};
+    sh.cbuf = cbuf;
+    sh.cext = cext;
+    sh.dead = &dead;
+    const rb::StripCtx X{};
+    const int b = blockIdx.x, t = threadIdx.x, tx = t % TX, ty = t / TX;
+    const int i0 = TR * ty, j0 = TC * tx;
+    const size_t MN = (size_t)M * N;
+    const float *gp = g + (size_t)b * MN;
+    const double r = rw[b];
+    float vi[TR];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+        vi[ii] = (float)vin[(size_t)b * M + i0 + ii];
+    auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&c)[TC]) {
+        float gr[TC];
+        rb::ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gr);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            a[jj] = c[jj] = gr[jj] * ((vi[ii] - pr[jj]) - qr[jj]);
+    };
+    float p[TR][TC], q[TR][TC], p1[TR][TC], q1[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            p[ii][jj] = q[ii][jj] = 0.f;
+    int nbar = 0;
+    rb::wire_iteration<G, false>(sh, X, cur, r, 1.0, M, 0, i0, j0, tx, ty, t, nbar, p, q);
+    float s1 = 0.f, gm = 0.f, vm = 0.f;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        vm = fmaxf(vm, fabsf(vi[ii]));
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            p1[ii][jj] = p[ii][jj];
+            q1[ii][jj] = q[ii][jj];
+            s1 = fmaxf(s1, fmaxf(fabsf(p[ii][jj]), fabsf(q[ii][jj])));
+        }
+    }
+    rb::wire_iteration<G, false>(sh, X, cur, r, 1.0, M, 0, i0, j0, tx, ty, t, nbar, p, q);
+    float s2 = 0.f;
+    float pool[2][5]; // per pooling cell: sums of p1, q1, p2 - p1, q2 - q1, g
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < 5; ++e)
+            pool[h][e] = 0.f;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        float gr[TC];
+        rb::ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gr);
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const float dp = p[ii][jj] - p1[ii][jj], dq = q[ii][jj] - q1[ii][jj];
+            s2 = fmaxf(s2, fmaxf(fabsf(dp), fabsf(dq)));
+            gm = fmaxf(gm, gr[jj]);
+            float *pc = pool[jj / 4];
+            pc[0] += p1[ii][jj];
+            pc[1] += q1[ii][jj];
+            pc[2] += dp;
+            pc[3] += dq;
+            pc[4] += gr[jj];
+        }
+        rb::strow<TC>(base + (size_t)b * 2 * MN + (size_t)(i0 + ii) * N + j0, p[ii]);
+        rb::strow<TC>(base + (size_t)b * 2 * MN + MN + (size_t)(i0 + ii) * N + j0, q[ii]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 = fmaxf(s1, __shfl_xor(s1, off, 64));
+        s2 = fmaxf(s2, __shfl_xor(s2, off, 64));
+        gm = fmaxf(gm, __shfl_xor(gm, off, 64));
+        vm = fmaxf(vm, __shfl_xor(vm, off, 64));
+    }
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = s1;
+        red[1][t >> 6] = s2;
+        red[2][t >> 6] = gm;
+        red[3][t >> 6] = vm;
+    }
+    __syncthreads();
+    float S1 = 1e-30f, S2 = 1e-30f, GM = 1e-30f, VM = 1e-30f;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) {
+        S1 = fmaxf(S1, red[0][w]);
+        S2 = fmaxf(S2, red[1][w]);
+        GM = fmaxf(GM, red[2][w]);
+        VM = fmaxf(VM, red[3][w]);
+    }
+    if (t == 0)
+        scale[b] = S2;
+    const float vavg = 0.25f * (((vi[0] + vi[1]) + vi[2]) + vi[3]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float *pc = pool[h];
+        bf16x8 f;
+        f[0] = (__bf16)(pc[0] * (1.f / 16.f) / S1);
+        f[1] = (__bf16)(pc[1] * (1.f / 16.f) / S1);
+        f[2] = (__bf16)(pc[2] * (1.f / 16.f) / S2);
+        f[3] = (__bf16)(pc[3] * (1.f / 16.f) / S2);
+        f[4] = (__bf16)(pc[4] * (1.f / 16.f) / GM);
+        f[5] = (__bf16)(vavg / VM);
+        f[6] = (__bf16)((float)(r / 5.0));
+        f[7] = (__bf16)1.0f;
+        *reinterpret_cast<bf16x8 *>(feat + (((size_t)b * (M / 4) + ty) * PW + 2 * tx + h) * 8) = f;
+    }
 }
 
 // ---------------------------------------------------------------- features
@@ -664,6 +797,22 @@ inline int grid_for(size_t n, int block = 256)
 using namespace fadesim;
 
 #define NN_LAUNCH_CHECK() FADESIM_HIP_CHECK(hipGetLastError())
+
+extern "C" int fadesim_nn_features_pooled(int B, int M, int N, int level, const float *d_g, const double *d_r,
+                                          const double *d_vin, void *d_feat, float *d_base, float *d_scale,
+                                          void *stream)
+{
+    if (B < 0 || !d_g || !d_r || !d_vin || !d_feat || !d_base || !d_scale)
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_features_pooled: bad arguments");
+    if (M != FeatGeo::M || N != FeatGeo::N || level != 2)
+        return set_error(FADESIM_E_UNSUPPORTED, "fadesim_nn_features_pooled: 128x128 crossbars, level 2 only");
+    if (B == 0)
+        return FADESIM_OK;
+    hipLaunchKernelGGL(features_pooled_kernel, dim3(B), dim3(FeatGeo::NT), 0, (hipStream_t)stream, d_g, d_r, d_vin,
+                       (__bf16 *)d_feat, d_base, d_scale);
+    NN_LAUNCH_CHECK();
+    return FADESIM_OK;
+}
 
 extern "C" int fadesim_nn_features(int B, int M, int N, const float *d_g, const double *d_r, const double *d_vin,
                                    void *d_feat, float *d_base, float *d_scale, void *stream)

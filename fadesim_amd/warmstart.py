@@ -131,7 +131,16 @@ def _coarse32() -> List[Op]:
     ]
 
 
-ARCHITECTURES = {"unet16": _unet16, "slim": _slim, "coarse32": _coarse32}
+def _coarse32f() -> List[Op]:
+    """coarse32 fed directly with features averaged over 4x4 cells
+    (fadesim_nn_features_pooled computes them in registers, with no
+    full-resolution feature planes and no pooling passes).  Same convolutions,
+    so coarse32 weights load into it."""
+    return [op if op.name != "e2a" else Op("e2a", "conv", "x", 2, N_FEAT, 32)
+            for op in _coarse32() if op.kind != "pool" or op.name == "p3"]
+
+
+ARCHITECTURES = {"unet16": _unet16, "slim": _slim, "coarse32": _coarse32, "coarse32f": _coarse32f}
 
 
 def architecture(name: str = "unet16") -> List[Op]:
@@ -186,6 +195,12 @@ class WarmStartNet:
         return flat[o:o + op.cout * kpad(op.cin)].view(op.cout, kpad(op.cin))
 
     @property
+    def in_level(self) -> int:
+        """Resolution level of the input features (0 = the crossbar's)."""
+        return min(op.level for op in self.ops if op.src == "x" and op.kind == "conv") if not any(
+            op.src == "x" and op.kind == "pool" for op in self.ops) else 0
+
+    @property
     def out_level(self) -> int:
         """Resolution level of the network output (0 = the crossbar's)."""
         return next(op.level for op in self.ops if op.name == "out")
@@ -228,14 +243,20 @@ class WarmStartNet:
 
     # ------------------------------------------------------------------ features
     def features(self, g, r, vin):
+        """Input features at the network's input level, the second-order base
+        field [B][2][M][N] and the per-instance output scale."""
         B = g.shape[0]
-        feat = torch.empty((B, self.M, self.N, N_FEAT), dtype=torch.bfloat16, device=self.dev)
+        lv = self.in_level
+        H, W = self._hw(lv)
+        feat = torch.empty((B, H, W, N_FEAT), dtype=torch.bfloat16, device=self.dev)
         base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
         scale = torch.empty(B, dtype=torch.float32, device=self.dev)
-        _chk(nat.lib().fadesim_nn_features(B, self.M, self.N, ctypes.c_void_p(g.data_ptr()),
-                                           ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(vin.data_ptr()),
-                                           ctypes.c_void_p(feat.data_ptr()), ctypes.c_void_p(base.data_ptr()),
-                                           ctypes.c_void_p(scale.data_ptr()), _s(self.dev)), "nn_features")
+        ptrs = [ctypes.c_void_p(x.data_ptr()) for x in (g, r, vin, feat, base, scale)]
+        if lv == 0:
+            _chk(nat.lib().fadesim_nn_features(B, self.M, self.N, *ptrs, _s(self.dev)), "nn_features")
+        else:
+            _chk(nat.lib().fadesim_nn_features_pooled(B, self.M, self.N, lv, *ptrs, _s(self.dev)),
+                 "nn_features_pooled")
         return feat, base, scale
 
     # ------------------------------------------------------------------ forward
@@ -243,7 +264,8 @@ class WarmStartNet:
         return self.M >> level, self.N >> level
 
     def forward(self, x, keep=False):
-        """x: features [B][M][N][8] bf16 -> out [B][M][N][16] bf16 (and all activations)."""
+        """x: features [B][H][W][8] bf16 at the input level -> out [B][H'][W'][16]
+        bf16 at the output level (and all activations)."""
         B = x.shape[0]
         L = nat.lib()
         acts = {"x": x}

@@ -230,3 +230,30 @@ def test_coarse_network_predicts_full_field(cuda):
     # an untrained network's field is still only a warm start: the solve converges
     res = solve_batch(g, r, vin, warm=w, tol=1e-6, ir2s_iters=1, corr_iters=4)
     assert int((res.status != 0).sum()) == 0
+
+
+def test_pooled_features_match_pooled_full_features(cuda):
+    """fadesim_nn_features_pooled (register-tile wire solves, FP32) against the
+    full-resolution features (FP64 scans) averaged over 4x4 cells, and its
+    second-order field against the closed form."""
+    import torch
+    import torch.nn.functional as F
+    from fadesim_amd.warmstart import WarmStartNet
+    full = WarmStartNet(cuda, arch="coarse32")
+    pooled = WarmStartNet(cuda, arch="coarse32f")
+    assert (full.in_level, pooled.in_level) == (0, 2)
+    g, r, vin = config2_instances(4, 128, seed=9)
+    tg, tr, tv = (torch.tensor(a, device=cuda) for a in (g, r, vin))
+    f0, b0, s0 = full.features(tg, tr, tv)
+    f2, b2, s2 = pooled.features(tg, tr, tv)
+    assert f2.shape == (4, 32, 32, 8)
+    torch.testing.assert_close(s2, s0, rtol=1e-4, atol=0)
+    for k in range(2):
+        scale = b0[:, k].abs().amax(dim=(1, 2)).view(4, 1, 1)
+        assert float(((b2[:, k] - b0[:, k]).abs() / scale).max()) < 1e-5
+    ref = F.avg_pool2d(f0.float().permute(0, 3, 1, 2), 4).permute(0, 2, 3, 1)
+    # features: bf16 values averaged (FP32) vs averages rounded to bf16
+    torch.testing.assert_close(f2.float(), ref, rtol=2e-2, atol=2e-3)
+    # a trained coarse network runs on them end to end
+    w = pooled.predict(tg, tr, tv)
+    assert w.shape == (4, 2, 128, 128) and bool(torch.isfinite(w).all())
