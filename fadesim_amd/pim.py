@@ -3,8 +3,9 @@
 Restates the fixed-point, bit-sliced crossbar matrix multiply of PimTorch --
 ``quantization_tensor`` (pimtorch/pimfixedpoint/fixedPoint/nn/fixedPointArithmetic.py:19-96),
 ``splitArr_matmul_nn`` (splitArrayArithmetic.py:1161-1260), ``matmul`` /
-``IRdrop_matmul`` (splitArrayArithmetic.py:442-1018, PN arrays, whole-array
-mode) and ``Linear_sp`` (modules/linear_split.py:13-78) -- with the same
+``IRdrop_matmul`` (splitArrayArithmetic.py:442-1018, PN and reference-column
+arrays, whole-array and OU mode), ``Linear_sp`` (modules/linear_split.py:
+13-78) and ``Conv2d_sp`` (modules/conv_split.py:31-136) -- with the same
 arithmetic: Normal (two's complement) quantized inputs applied bit-serially,
 PN weights split into positive/negative magnitudes and sliced ``cellBits``
 per cell, conductance ``slice*dG + Gmin`` in the float type, per-tile ADC
@@ -200,10 +201,19 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
                        float_type: torch.dtype = torch.float32,
                        params: Optional[PhyArrParams] = None) -> Tensor:
     """output [bsize, outsize] = input [bsize, insize] x w [insize, outsize]
-    through the simulated crossbars (PN arrays, complement-code inputs, 1-bit DAC)."""
+    through the simulated crossbars (complement-code inputs, 1-bit DAC).
+
+    arrType PN: positive/negative magnitude arrays (splitArrayArithmetic.py:
+    1183-1226).  arrType Ref: one array per weight slice holding the two's
+    complement weight offset by 2^(weightBits-1) (:1228-1258), each tile with
+    a zero column and a ones column appended (OU mode: a block of OUSize[1]
+    columns, cells past the ones column at 1e-20 S; :765-785); the zero
+    column's ADC code is subtracted from every column and, for slice 0, the
+    ones column's code times 2^(weightBits-1) (:996-1012)."""
     P = params or phyArrParams
-    if arrType != phyArrMode.PN:
-        raise NotImplementedError("only positive/negative (PN) arrays are supported")
+    ref = arrType == phyArrMode.Ref
+    if arrType not in (phyArrMode.PN, phyArrMode.Ref):
+        raise NotImplementedError(f"array mode {arrType}")
     if dacBits != 1:
         raise Exception("[Error] Complement code input mode, but dac is not 1 bit. dacbits = {}".format(dacBits))
     if weightBits % P.cellBits != 0:
@@ -214,40 +224,58 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         bsize, insize = input.shape
         outsize = w.shape[1]
         qin, s1 = quantization_tensor(input, inputBits, TensorType.Normal)
-        qw, s2 = quantization_tensor(w, weightBits, TensorType.PN)
-        wpos = qw.clamp(min=0)
-        wneg = (-qw).clamp(min=0)
         R, C = P.arrRowSize, P.arrColSize
         numX, numY = _tiles(insize, R), _tiles(outsize, C)
         Rp, Cp = numX * R, numY * C
         nslice = weightBits // P.cellBits
         mask_cell = (1 << P.cellBits) - 1
         inputV = P.inputMaxVoltage / (2 ** dacBits - 1)
-        # slices [nslice][2][Rp][Cp]; padded cells hold slice 0 (conductance Gmin)
-        slices = torch.stack([torch.stack([(wpos >> (P.cellBits * j)) & mask_cell,
-                                           (wneg >> (P.cellBits * j)) & mask_cell]) for j in range(nslice)])
-        slices = F.pad(slices, (0, Cp - outsize, 0, Rp - insize))
-        G = slices.to(float_type).mul(P.cellDeltaConduct).add(P.cellMinConduct)  # matmul :753-754
-        G = G.view(nslice, 2, numX, R, numY, C).permute(0, 1, 2, 4, 3, 5).contiguous()
+        if P.useOUSize:
+            P.check_ou()
+        if ref:
+            qw, s2 = quantization_tensor(w, weightBits, TensorType.Normal)
+            plus = 2 ** (weightBits - 1)
+            qw = (qw + plus).clamp(max=2 ** weightBits - 1)
+            arrays = [qw]
+            E = P.OUSize[1] if P.useOUSize else 2  # reference columns per tile
+        else:
+            qw, s2 = quantization_tensor(w, weightBits, TensorType.PN)
+            arrays = [qw.clamp(min=0), (-qw).clamp(min=0)]
+            E = 0
+        ns = len(arrays)
+        Cx = C + E
+        # slices [nslice][ns][Rp][Cp]; padded cells hold slice 0 (conductance Gmin)
+        slices = torch.stack([torch.stack([(a >> (P.cellBits * j)) & mask_cell for a in arrays])
+                              for j in range(nslice)])
+        slices = F.pad(slices, (0, Cp - outsize, 0, Rp - insize)).view(nslice, ns, numX, R, numY, C)
+        if ref:
+            extra = slices.new_zeros((nslice, ns, numX, R, numY, E))
+            extra[..., 1] = 1
+            slices = torch.cat((slices, extra), -1)
+        G = slices.to(float_type).mul(P.cellDeltaConduct).add(P.cellMinConduct)  # matmul :753-754, :777-783
+        if ref and P.useOUSize:
+            G[..., C + 2:] = 1e-20
+        G = G.permute(0, 1, 2, 4, 3, 5).contiguous()  # [nslice][ns][numX][numY][R][Cx]
         # input bits [inputBits][bsize][numX][R] (two's complement bits of qin)
         qpad = F.pad(qin, (0, Rp - insize))
         bits = torch.stack([(qpad >> i) & 1 for i in range(inputBits)]).view(inputBits, bsize, numX, R)
-        # tile currents [inputBits][nslice][2][bsize][numX][numY][C]; in OU
-        # mode one entry per OU row block: [..][numX * arrOUnum[0]][numY][C]
-        if P.useOUSize:
-            P.check_ou()
+        # tile currents [inputBits][nslice][ns][bsize][numX][numY][Cx]; in OU
+        # mode one entry per OU row block: [..][numX * arrOUnum[0]][numY][Cx]
         if P.ir_mode == "none":
             vin = bits.to(float_type).mul(inputV)
             if P.useOUSize:
                 our = P.OUSize[0]
                 I = torch.einsum("ibxkr,jsxykrc->ijsbxkyc", vin.view(inputBits, bsize, numX, R // our, our),
-                                 G.view(nslice, 2, numX, numY, R // our, our, C))
-                I = I.reshape(inputBits, nslice, 2, bsize, numX * (R // our), numY, C)
+                                 G.view(nslice, ns, numX, numY, R // our, our, Cx))
+                I = I.reshape(inputBits, nslice, ns, bsize, numX * (R // our), numY, Cx)
             else:
                 I = torch.einsum("ibxr,jsxyrc->ijsbxyc", vin, G)
         elif P.ir_mode == "converged":
             if P.useOUSize:
                 raise NotImplementedError("OU mode runs the hook's OU iterations (ir_mode 'reference' or 'ir2s')")
+            if ref:
+                raise NotImplementedError("reference columns make R x (C+2) tiles; the converged solver takes "
+                                          "square 64/128 tiles (ir_mode 'reference' or 'ir2s')")
             I = _solve_converged(bits, G, inputV, P)
         elif P.ir_mode in ("reference", "ir2s"):
             I = _solve_reference(bits, G, inputV, P, float_type)
@@ -256,11 +284,15 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         # ADC per tile or per OU (matmul :833-840), sum over row tiles / OUs and rescale (:958-976)
         adc_full, rows_per_read = (P.adcOUuseI, P.OUSize[0]) if P.useOUSize else (P.adcUseI, R)
         adc = I.to(float_type).div(adc_full).mul(2 ** adcBits - 1).round()
-        out = adc.sum(4).reshape(inputBits, nslice, 2, bsize, Cp)[..., :outsize]
+        if ref:  # reference columns (:996-1012)
+            adc = adc - adc[..., C:C + 1]
+            adc[:, 0] = adc[:, 0] - adc[:, 0, ..., C + 1:C + 2].mul(plus).round()
+            adc = adc[..., :C]
+        out = adc.sum(4).reshape(inputBits, nslice, ns, bsize, Cp)[..., :outsize]
         out = out.mul(rows_per_read * (2 ** dacBits - 1) * (2 ** P.cellBits - 1) / (2 ** adcBits - 1))
-        out = out[:, :, 0] - out[:, :, 1]  # [inputBits][nslice][bsize][outsize]
+        out = out[:, :, 0] if ref else out[:, :, 0] - out[:, :, 1]  # [inputBits][nslice][bsize][outsize]
         output = w.new_zeros((bsize, outsize))
-        for i in range(inputBits):  # splitArr_matmul_nn :1197-1227
+        for i in range(inputBits):  # splitArr_matmul_nn :1197-1227, :1245-1256
             for jj in range(nslice):
                 term = out[i, jj].to(w.dtype).mul(2.0 ** (i + P.cellBits * jj + s1 + s2))
                 output = output.sub(term) if i == inputBits - 1 else output.add(term)
@@ -328,20 +360,22 @@ def _solve_converged(bits, G, inputV, P):
 
 def _solve_reference(bits, G, inputV, P, float_type):
     """The reference's iterations: one hook call per (input bit, weight slice,
-    sign), as splitArrayArithmetic.py:466-511 (whole arrays) and :524-605 (OU
-    mode) make them -- line-GS (ir_drop_gs / ir_gs_ou) or IR2S (ir2s / ir2s_ou)."""
+    array), as splitArrayArithmetic.py:466-511 (whole arrays), :524-605 (OU
+    mode) and :612-701 (reference-column arrays, C + 2 or C + OUSize[1]
+    columns) make them -- line-GS (ir_drop_gs / ir_gs_ou) or IR2S (ir2s /
+    ir2s_ou)."""
     from . import hook
     nbit, bsize, numX, R = bits.shape
-    nslice, _, _, numY, _, C = G.shape
+    nslice, ns, _, numY, _, C = G.shape  # C includes reference columns
     dev = G.device
     nrb = numX * (R // P.OUSize[0]) if P.useOUSize else numX
-    I = torch.zeros((nbit, nslice, 2, bsize, nrb, numY, C), dtype=torch.float64, device=dev)
+    I = torch.zeros((nbit, nslice, ns, bsize, nrb, numY, C), dtype=torch.float64, device=dev)
     brk, thr = P.ir_drop_less_than_thresholdBreak_enable, P.ir_drop_less_than_threshold
     nl_on, nl = P.nonlinear_cell_enable, P.nonlinear_params
     for i in range(nbit):
         vin = bits[i].to(float_type).mul(inputV).contiguous()  # [bsize][numX][R]
         for jj in range(nslice):
-            for s in range(2):
+            for s in range(ns):
                 Gh = G[jj, s].permute(0, 2, 1, 3).reshape(numX, R, numY * C).to(float_type).contiguous()
                 it = P.ir_drop_iter_time
                 if P.ir_mode == "ir2s" and P.useOUSize:
@@ -396,6 +430,71 @@ class Linear_sp(nn.Module):
         if self.hasBias and self.useFpBias:
             output = output + self.fp_bias
         return output
+
+
+class Conv2d_sp(nn.Module):
+    """modules/conv_split.py:31-136: float training (F.conv2d); in eval mode
+    the input is unfolded (im2col), multiplied through the crossbars by
+    splitArr_matmul_nn with the weights as a [Cin*kh*kw, Cout] matrix, the fp
+    bias added after the crossbars (or mapped as an extra weight row with
+    use_fp_bias=False), and the result folded back to NCHW.  Zero padding,
+    groups = 1 in eval mode (the reference's eval path has the same limits)."""
+
+    def __init__(self, in_channels, out_channels, kernel_size, stride=1, padding=0, dilation=1, groups=1,
+                 bias=True, padding_mode="zeros",
+                 input_bit_width=ConstForSplitArray.input_bit_width,
+                 output_bit_width=ConstForSplitArray.output_bit_width,
+                 weight_bit_width=ConstForSplitArray.weight_bit_width,
+                 dac_bit_width=ConstForSplitArray.dac_bit_width,
+                 adc_bit_width=ConstForSplitArray.adc_bit_width,
+                 phy_arr_mode=phyArrMode.PN, fp_type=torch.float32, use_fp_bias=True, params=None):
+        super().__init__()
+        if groups <= 0:
+            raise ValueError("groups must be a positive integer")
+        if in_channels % groups != 0:
+            raise ValueError("in_channels must be divisible by groups")
+        if out_channels % groups != 0:
+            raise ValueError("out_channels must be divisible by groups")
+        pair = lambda v: tuple(v) if isinstance(v, (tuple, list)) else (v, v)  # noqa: E731
+        self.in_channels, self.out_channels = in_channels, out_channels
+        self.kernel_size, self.stride = pair(kernel_size), pair(stride)
+        self.padding, self.dilation, self.groups = pair(padding), pair(dilation), groups
+        self.hasBias, self.padding_mode, self.useFpBias = bias, padding_mode, use_fp_bias
+        self.inputBits, self.outputBits, self.weightBits = input_bit_width, output_bit_width, weight_bit_width
+        self.adcBits, self.dacBits = adc_bit_width, dac_bit_width
+        self.phyArrMode, self.fp_type, self.params = phy_arr_mode, fp_type, params
+        self.fp_weight = nn.Parameter(torch.empty((out_channels, in_channels // groups, *self.kernel_size),
+                                                  dtype=fp_type))
+        self.fp_bias = nn.Parameter(torch.empty(out_channels, dtype=fp_type)) if bias else None
+        nn.init.kaiming_uniform_(self.fp_weight, a=math.sqrt(5))
+        if self.fp_bias is not None:
+            fan_in = (in_channels // groups) * self.kernel_size[0] * self.kernel_size[1]
+            bound = 1 / math.sqrt(fan_in)
+            nn.init.uniform_(self.fp_bias, -bound, bound)
+
+    def forward(self, input: Tensor) -> Tensor:
+        if self.training:
+            return F.conv2d(input, self.fp_weight, self.fp_bias, self.stride, self.padding, self.dilation,
+                            self.groups)
+        if self.groups != 1 or self.padding_mode != "zeros":
+            raise NotImplementedError("crossbar inference of Conv2d_sp: groups = 1, zero padding")
+        B, _, H, W = input.shape
+        kh, kw = self.kernel_size
+        cols = F.unfold(input, self.kernel_size, dilation=self.dilation, padding=self.padding, stride=self.stride)
+        rows = cols.transpose(1, 2).reshape(-1, self.in_channels * kh * kw)
+        wmat = self.fp_weight.reshape(self.out_channels, -1)
+        if self.hasBias and not self.useFpBias:  # bias mapped onto the arrays as a ones row
+            rows = torch.cat((rows, rows.new_ones(rows.size(0), 1)), 1)
+            logicArr = torch.cat((wmat.t(), self.fp_bias.unsqueeze(0)), 0)
+        else:
+            logicArr = wmat.t()
+        out = splitArr_matmul_nn(rows, logicArr, self.inputBits, self.dacBits, self.adcBits, self.outputBits,
+                                 self.weightBits, self.phyArrMode, self.fp_type, params=self.params)
+        if self.hasBias and self.useFpBias:
+            out = out + self.fp_bias
+        Ho = (H + 2 * self.padding[0] - self.dilation[0] * (kh - 1) - 1) // self.stride[0] + 1
+        Wo = (W + 2 * self.padding[1] - self.dilation[1] * (kw - 1) - 1) // self.stride[1] + 1
+        return out.reshape(B, Ho * Wo, self.out_channels).transpose(1, 2).reshape(B, self.out_channels, Ho, Wo)
 
 
 class MLP(nn.Module):

@@ -23,6 +23,16 @@ integer/fixed-point pipeline around the solver:
                         matmul :838-840), rescale by ou_rows (:968-976).
 * ``ideal_currents_ou`` -- the ideal per-OU product (no_ir_drop_cal_ou,
                         ir_drop_solve.py:21-30).
+* ``split_matmul_ref`` -- the reference-column arrays (phyArrMode.Ref,
+                        splitArrayArithmetic.py:1228-1258): two's complement
+                        weights offset by 2^(wb-1), one array per slice with
+                        a zero column and a ones column per tile (OU mode: a
+                        block of OUSize[1] columns, the rest at 1e-20 S)
+                        (:765-785), the zero column's code subtracted from
+                        every column and, for slice 0, the ones column's
+                        code x 2^(wb-1) (:996-1012).
+* ``unfold`` / ``conv2d_sp`` -- Conv2d_sp's eval path (modules/conv_split.py:
+                        87-136): im2col, the split matmul, back to NCHW.
 
 Every array op is done in float32 where the reference's is (fp_type float32).
 """
@@ -120,3 +130,85 @@ def split_matmul(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, 
             term = out * np.float32(2.0 ** (i + j + s1 + s2))
             output = output - term if i == input_bits - 1 else output + term
     return output, {"codes": codes, "adc_in": adc_in, "s1": s1, "s2": s2}
+
+
+def split_matmul_ref(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, cell_bits=1,
+                     rows=64, cols=64, vmax=1.0, ron=1e3, roff=1e6, currents=ideal_currents, ou_rows=None,
+                     ou_cols=None):
+    """output [bsize, outsize] through reference-column arrays.
+    ``currents(vin [bsize,numX,R], G [numX,R,numY*(C+E)])`` returns
+    [bsize, row blocks, numY*(C+E)], E = 2 (OU mode: ou_cols)."""
+    if dac_bits != 1:
+        raise ValueError("complement-code inputs need a 1-bit DAC")
+    x = np.asarray(x, np.float32)
+    w = np.asarray(w, np.float32)
+    bsize, insize = x.shape
+    outsize = w.shape[1]
+    qin, s1 = quantize(x, input_bits, "normal")
+    qw, s2 = quantize(w, weight_bits, "normal")
+    plus = 2 ** (weight_bits - 1)
+    qw = np.minimum(qw + plus, 2 ** weight_bits - 1)
+    numX, numY = -(-insize // rows), -(-outsize // cols)
+    Rp, Cp = numX * rows, numY * cols
+    E = ou_cols if ou_rows else 2
+    gmin, gmax = 1.0 / roff, 1.0 / ron
+    dG = (gmax - gmin) / (2 ** cell_bits - 1)
+    per_read = ou_rows if ou_rows else rows
+    adc_use_i = per_read * (gmax - gmin) * vmax
+    mask_cell = (1 << cell_bits) - 1
+    lsb = np.float32(2 ** adc_bits - 1)
+    rescale = np.float32(per_read * (2 ** dac_bits - 1) * (2 ** cell_bits - 1) / (2 ** adc_bits - 1))
+    output = np.zeros((bsize, outsize), np.float32)
+    adc_in = {}
+    for i in range(0, input_bits, dac_bits):
+        in_b = np.pad((qin >> i) & 1, ((0, 0), (0, Rp - insize)))
+        vin = (in_b.astype(np.float32) * np.float32(vmax)).reshape(bsize, numX, rows)
+        for j in range(0, weight_bits, cell_bits):
+            sl = np.pad((qw >> j) & mask_cell, ((0, Rp - insize), (0, Cp - outsize))).reshape(Rp, numY, cols)
+            extra = np.zeros((Rp, numY, E), sl.dtype)
+            extra[..., 1] = 1
+            G = np.concatenate([sl, extra], 2).astype(np.float32) * np.float32(dG) + np.float32(gmin)
+            if ou_rows:
+                G[..., cols + 2:] = np.float32(1e-20)
+            G = G.reshape(numX, rows, numY * (cols + E))
+            I = np.asarray(currents(vin, G)).astype(np.float32)
+            a = I / np.float32(adc_use_i) * lsb
+            adc_in[(i, j, "ref")] = a
+            code = np.round(a).reshape(bsize, -1, numY, cols + E)
+            code = code - code[..., cols:cols + 1]
+            if j == 0:
+                code = code - np.round(code[..., cols + 1:cols + 2] * np.float32(plus))
+            out = code[..., :cols].sum(1, dtype=np.float32).reshape(bsize, -1)[:, :outsize] * rescale
+            term = out * np.float32(2.0 ** (i + j + s1 + s2))
+            output = output - term if i == input_bits - 1 else output + term
+    return output, {"adc_in": adc_in, "s1": s1, "s2": s2}
+
+
+def unfold(x, kh, kw, stride=(1, 1), padding=(0, 0), dilation=(1, 1)):
+    """im2col of x [B, C, H, W] in torch.nn.functional.unfold's order:
+    [B, C*kh*kw, L], row index c*kh*kw + ki*kw + kj."""
+    x = np.asarray(x)
+    B, C, H, W = x.shape
+    xp = np.pad(x, ((0, 0), (0, 0), (padding[0], padding[0]), (padding[1], padding[1])))
+    Ho = (H + 2 * padding[0] - dilation[0] * (kh - 1) - 1) // stride[0] + 1
+    Wo = (W + 2 * padding[1] - dilation[1] * (kw - 1) - 1) // stride[1] + 1
+    cols = np.empty((B, C, kh, kw, Ho, Wo), x.dtype)
+    for ki in range(kh):
+        for kj in range(kw):
+            r0, c0 = ki * dilation[0], kj * dilation[1]
+            cols[:, :, ki, kj] = xp[:, :, r0:r0 + stride[0] * Ho:stride[0], c0:c0 + stride[1] * Wo:stride[1]]
+    return cols.reshape(B, C * kh * kw, Ho * Wo), (Ho, Wo)
+
+
+def conv2d_sp(x, weight, bias=None, stride=(1, 1), padding=(0, 0), dilation=(1, 1), matmul=split_matmul, **kw):
+    """Conv2d_sp in eval mode (modules/conv_split.py:101-131): unfold, the
+    split matmul of the rows with weight [Cout, Cin*kh*kw]^T, fp bias added
+    after it, then NCHW."""
+    B = x.shape[0]
+    Cout, Cin, kh, kwd = weight.shape
+    cols, (Ho, Wo) = unfold(x, kh, kwd, stride, padding, dilation)
+    rows = cols.transpose(0, 2, 1).reshape(-1, Cin * kh * kwd)
+    out, _ = matmul(rows, np.asarray(weight, np.float32).reshape(Cout, -1).T, **kw)
+    if bias is not None:
+        out = out + np.asarray(bias, np.float32)
+    return out.reshape(B, Ho * Wo, Cout).transpose(0, 2, 1).reshape(B, Cout, Ho, Wo)

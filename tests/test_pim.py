@@ -144,3 +144,88 @@ def test_ou_size_checks():
                                                                                  OUSize=(8, 1)))
     with pytest.raises(NotImplementedError):
         pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="converged", useOUSize=True))
+
+
+# --- reference-column arrays and Conv2d_sp -------------------------------------
+
+@pytest.mark.parametrize("insize,outsize,rows,cell_bits,ou,seed", [(100, 30, 64, 1, None, 0),
+                                                                   (200, 70, 64, 2, None, 1),
+                                                                   (150, 40, 64, 2, (8, 8), 2),
+                                                                   (130, 20, 64, 1, (16, 4), 3)])
+def test_ref_arrays_match_restatement(insize, outsize, rows, cell_bits, ou, seed):
+    gen = torch.Generator().manual_seed(40 + seed)
+    x = torch.rand(3, insize, generator=gen) * 2 - 0.5
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="none", cellBits=cell_bits, arrRowSize=rows, arrColSize=rows,
+                         useOUSize=ou is not None, OUSize=ou or (8, 8))
+    out = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, pim.phyArrMode.Ref, params=P)
+    cur = opim.ideal_currents_ou(ou[0]) if ou else opim.ideal_currents
+    ref, info = opim.split_matmul_ref(x.numpy(), w.numpy(), cell_bits=cell_bits, rows=rows, cols=rows,
+                                      vmax=P.inputMaxVoltage, ron=P.Ron, roff=P.Roff, currents=cur,
+                                      ou_rows=ou[0] if ou else None, ou_cols=ou[1] if ou else None)
+    per_read = ou[0] if ou else rows
+    bound = _near_tie_bound(info, per_read * (2 ** cell_bits - 1) / 1023 * 129, outsize)
+    assert np.max(np.abs(out.numpy() - ref)) <= bound + 1e-6 * np.abs(ref).max()
+
+
+def test_ref_arrays_approximate_integer_product():
+    """Reference columns remove the Gmin leakage (zero column) and the
+    2^(wb-1) weight offset (ones column): the result is the two's complement
+    product up to ADC rounding -- each code +-1/2, the zero-column
+    subtraction +-1/2, and for slice 0 the ones column's +-1 code times 128."""
+    gen = torch.Generator().manual_seed(8)
+    insize, outsize, rows = 150, 20, 64
+    x = torch.rand(4, insize, generator=gen)
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="none")
+    out = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, pim.phyArrMode.Ref, params=P).double()
+    qx, s1 = pim.quantization_tensor(x, 8, pim.TensorType.Normal)
+    qw, s2 = pim.quantization_tensor(w, 8, pim.TensorType.Normal)
+    exact = (qx.double() @ qw.double()) * 2.0 ** (s1 + s2)
+    numX = -(-insize // rows)
+    rescale = rows / 1023
+    bound = sum(numX * (1.0 + (128.5 if j == 0 else 0.0)) * rescale * 2.0 ** (i + j + s1 + s2)
+                for i in range(8) for j in range(8))
+    err = torch.max(torch.abs(out - exact)).item()
+    assert err <= bound
+    assert err < 0.1 * exact.abs().max().item()
+    # a plain offset-binary array without the reference columns would be far off
+    assert torch.max(torch.abs(out - (qx.double() @ (qw.double() + 128)) * 2.0 ** (s1 + s2))).item() > 10 * err
+
+
+@pytest.mark.parametrize("arr,k,stride,pad,dil,bias_on_array", [
+    (pim.phyArrMode.PN, 3, 1, 1, 1, False), (pim.phyArrMode.PN, 2, 2, 0, 1, True),
+    (pim.phyArrMode.Ref, 3, 1, 2, 2, False)])
+def test_conv2d_sp_matches_restatement(arr, k, stride, pad, dil, bias_on_array):
+    torch.manual_seed(3)
+    P = pim.PhyArrParams(ir_mode="none")
+    conv = pim.Conv2d_sp(3, 5, k, stride=stride, padding=pad, dilation=dil, phy_arr_mode=arr,
+                         use_fp_bias=not bias_on_array, params=P)
+    x = torch.rand(2, 3, 9, 8)
+    conv.train()
+    torch.testing.assert_close(conv(x), F_conv2d(x, conv.fp_weight, conv.fp_bias, stride, pad, dil))
+    conv.eval()
+    with torch.no_grad():
+        y = conv(x)
+        fl = F_conv2d(x, conv.fp_weight, conv.fp_bias, stride, pad, dil)
+    assert y.shape == fl.shape
+    mm = opim.split_matmul if arr == pim.phyArrMode.PN else opim.split_matmul_ref
+    kw = dict(rows=64, cols=64, vmax=P.inputMaxVoltage, ron=P.Ron, roff=P.Roff)
+    if bias_on_array:
+        cols, (Ho, Wo) = opim.unfold(x.numpy(), k, k, (stride, stride), (pad, pad), (dil, dil))
+        rows = cols.transpose(0, 2, 1).reshape(-1, 3 * k * k)
+        rows = np.concatenate([rows, np.ones((rows.shape[0], 1), np.float32)], 1)
+        wm = np.concatenate([conv.fp_weight.detach().numpy().reshape(5, -1).T,
+                             conv.fp_bias.detach().numpy()[None]], 0)
+        o, _ = mm(rows, wm, **kw)
+        ref = o.reshape(2, Ho * Wo, 5).transpose(0, 2, 1).reshape(2, 5, Ho, Wo)
+    else:
+        ref = opim.conv2d_sp(x.numpy(), conv.fp_weight.detach().numpy(), conv.fp_bias.detach().numpy(),
+                             (stride, stride), (pad, pad), (dil, dil), matmul=mm, **kw)
+    np.testing.assert_allclose(y.numpy(), ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+    # and it is a quantized approximation of the float convolution
+    assert float((y - fl).abs().max()) < 0.15 * float(fl.abs().max())
+
+
+def F_conv2d(x, w, b, stride, pad, dil):
+    return torch.nn.functional.conv2d(x, w, b, stride, pad, dil)

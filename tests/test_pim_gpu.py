@@ -147,3 +147,61 @@ def test_ou_mode_layer_matches_hook_restatement(cuda, mode):
     ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=hook_currents, ou_rows=ou[0])
     bound = _tie_bound(info, ou[0] / 1023, 1e-4)
     assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("mode,ou", [("reference", None), ("ir2s", None), ("reference", (16, 8)), ("ir2s", (8, 8))])
+def test_ref_arrays_match_hook_restatement(cuda, mode, ou):
+    """Reference-column arrays through the hook's solvers (R x (C+2) tiles, or
+    C + OUSize[1] columns in OU mode) against the restatement driven by their
+    CPU replays."""
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(17)
+    insize, outsize, bsize = 70, 20, 2
+    x = torch.rand(bsize, insize, generator=gen) - 0.3
+    w = torch.randn(insize, outsize, generator=gen)
+    iters = 30 if mode == "reference" else 4
+    P = pim.PhyArrParams(ir_mode=mode, ir_drop_iter_time=iters, useOUSize=ou is not None, OUSize=ou or (8, 8))
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, pim.phyArrMode.Ref,
+                                 params=P).cpu().numpy()
+    E = ou[1] if ou else 2
+
+    def hook_currents(vin, G):
+        bsize_, numX, R = vin.shape
+        Cx = 64 + E
+        numY = G.shape[2] // Cx
+        args = (vin.astype(np.float32), G.astype(np.float32), bsize_, R, Cx, numX, numY, iters)
+        extra = dict(ou_rows=ou[0], ou_cols=ou[1]) if ou else {}
+        if mode == "reference":
+            o, _ = oracle.hook_gs_ou(*args, 1 / 2.93, 1.90, False, 1e-7, False, [], **extra)
+        else:
+            o, _ = oracle.hook_ir2s_ou(*args, 2.93, False, 1e-7, False, [], **extra)
+        return o.reshape(bsize_, -1, G.shape[2])
+    ref, info = opim.split_matmul_ref(x.numpy(), w.numpy(), rows=64, cols=64, currents=hook_currents,
+                                      ou_rows=ou[0] if ou else None, ou_cols=ou[1] if ou else None)
+    per_read = ou[0] if ou else 64
+    bound = _tie_bound(info, per_read / 1023 * 129, 1e-4)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
+
+
+def test_conv2d_sp_on_the_gpu_matches_restatement(cuda):
+    """A Conv2d_sp layer in eval mode with the hook's line-GS crossbars."""
+    import torch
+    from fadesim_amd import pim
+    torch.manual_seed(5)
+    P = pim.PhyArrParams(ir_mode="reference", ir_drop_iter_time=20)
+    conv = pim.Conv2d_sp(4, 6, 3, padding=1, params=P).to(cuda).eval()
+    x = torch.rand(2, 4, 6, 5, device=cuda)
+    with torch.no_grad():
+        y = conv(x).cpu().numpy()
+
+    def hook_currents(vin, G):
+        bsize_, numX, R = vin.shape
+        numY = G.shape[2] // 64
+        o, _ = oracle.hook_gs(vin.astype(np.float32), G.astype(np.float32), bsize_, R, 64, numX, numY, 20,
+                              1 / 2.93, 1.90, False, 1e-7, False, [])
+        return o
+    ref = opim.conv2d_sp(x.cpu().numpy(), conv.fp_weight.detach().cpu().numpy(), conv.fp_bias.detach().cpu().numpy(),
+                         (1, 1), (1, 1), (1, 1), rows=64, cols=64, currents=hook_currents)
+    assert y.shape == (2, 6, 6, 5)
+    np.testing.assert_allclose(y, ref, rtol=0, atol=2e-3 * np.abs(ref).max())
