@@ -88,7 +88,7 @@ EXPORTED = [
     "fadesim_ragged_plan_create", "fadesim_ragged_plan_destroy", "fadesim_ragged_plan_info",
     "fadesim_ragged_solve", "fadesim_ragged_aborted", "fadesim_ragged_last_ms",
     "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs", "fadesim_ir2s",
-    "fadesim_nn_features", "fadesim_nn_features_pooled", "fadesim_nn_combine", "fadesim_nn_combine_up", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
+    "fadesim_nn_features", "fadesim_nn_features_pooled", "fadesim_nn_coarse_forward", "fadesim_nn_combine", "fadesim_nn_combine_up", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
     "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
     "fadesim_nn_up2_bwd", "fadesim_nn_relu_mask", "fadesim_nn_add", "fadesim_nn_mse", "fadesim_nn_adam",
     "fadesim_nn_cast",
@@ -133,6 +133,7 @@ def _declare_nn(L):
     sigs = {
         "fadesim_nn_features": [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "fadesim_nn_features_pooled": [i, i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "fadesim_nn_coarse_forward": [i, _vp, _vp, _vp, _vp, _vp],
         "fadesim_nn_combine": [i, i, i, _vp, _vp, _vp, i, _vp, _vp],
         "fadesim_nn_combine_up": [i, i, i, _vp, _vp, _vp, i, i, _vp, _vp],
         "fadesim_nn_conv3x3_fwd": [i, i, i, i, i, i, _vp, _vp, _vp, i, _vp],

@@ -524,6 +524,144 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(ConvArgs A)
     }
 }
 
+// ---------------------------------------------------------------- fused coarse network
+// The whole coarse32f network for one crossbar in one workgroup, every
+// activation in LDS: 32x32x8 input -> e2a, e2 (32 ch, 32x32) -> 2x2 mean ->
+// e3a, e3 (64 ch), t3 (32 ch) at 16x16 -> nearest 2x upsample + e2 -> d2
+// (32 ch) -> out (16 ch, to HBM).  Each layer is the same implicit GEMM as
+// conv3x3_fwd_kernel (the same A and B fragments, the same k order), so the
+// result equals the layer-by-layer kernels'.  LDS: two 32x32x32 buffers (64
+// KB each) and one 32x32x8 / 16x16x32 buffer (16 KB).
+struct CoarseNetArgs {
+    const __bf16 *x;   // [B][32][32][8]
+    const __bf16 *w[7]; // e2a, e2, e3a, e3, t3, d2, out: [Cout][Kp]
+    int kp[7];
+    __bf16 *out;       // [B][32][32][16]
+};
+
+constexpr int kNetThreads = 1024;
+
+constexpr int net_kpad(int cin) { return ((9 * cin + 1 + 31) / 32) * 32; }
+
+// One layer: each of the 16 waves takes P = (H W / 16) / 16 pixel tiles of 16
+// pixels and all COUT channels, so every weight fragment loaded serves P
+// tiles; the depth is a compile-time constant (the k loop unrolls by 2 and
+// the next fragments load while the present ones multiply).
+template <int CIN, int COUT, int H, int W, bool RELU>
+__device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__restrict__ w, __bf16 *dst)
+{
+    constexpr int CT = COUT / 16, PT = H * W / 16, K9 = 9 * CIN, NW = kNetThreads / 64, KP = net_kpad(CIN);
+    constexpr int P = PT / NW;
+    static_assert(W % 16 == 0 && CIN % 8 == 0 && COUT % 16 == 0 && PT % NW == 0, "tile shapes");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, g = lane >> 4;
+    f4 acc[P][CT];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int t = 0; t < CT; ++t)
+            acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k0 = 0; k0 < KP; k0 += 32) {
+        const int k = k0 + 8 * g;
+        bf16x8 av[CT];
+#pragma unroll
+        for (int t = 0; t < CT; ++t)
+            av[t] = *reinterpret_cast<const bf16x8 *>(w + (long)(16 * t + j) * KP + k);
+        int tap = 0, ci = 0;
+        const bool real = k < K9;
+        if (real) {
+            tap = k / CIN;
+            ci = k - tap * CIN;
+        }
+        const int ky = tap / 3, kx = tap - 3 * ky;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int p0 = 16 * (wave + NW * q), y = p0 / W, x0 = p0 % W;
+            bf16x8 bv = zero8();
+            if (real) {
+                const int yy = y + ky - 1, xx = x0 + j + kx - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                    bv = *reinterpret_cast<const bf16x8 *>(src + ((yy * W + xx) * CIN + ci));
+            } else if (k == K9) {
+                bv[0] = (__bf16)1.0f; // bias column
+            }
+#pragma unroll
+            for (int t = 0; t < CT; ++t)
+                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[q][t], 0, 0, 0);
+        }
+    }
+    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int p0 = 16 * (wave + NW * q);
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+            bf16x4 o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = acc[q][t][r];
+                if (RELU)
+                    v = fmaxf(v, 0.f);
+                o[r] = (__bf16)v;
+            }
+            *reinterpret_cast<bf16x4 *>(dst + (p0 + j) * COUT + 16 * t + 4 * g) = o;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A)
+{
+    __shared__ __align__(16) __bf16 bufA[32 * 32 * 32];
+    __shared__ __align__(16) __bf16 bufS[32 * 32 * 32];
+    __shared__ __align__(16) __bf16 bufX[32 * 32 * 8];
+    const long b = blockIdx.x;
+    const int t = threadIdx.x;
+    // input features (16 bytes per copy)
+    for (int e = t; e < 32 * 32; e += kNetThreads)
+        *reinterpret_cast<bf16x8 *>(bufX + 8 * e) = *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
+    __syncthreads();
+    net_conv<8, 32, 32, 32, true>(bufX, A.w[0], bufA); // e2a
+    __syncthreads();
+    net_conv<32, 32, 32, 32, true>(bufA, A.w[1], bufS); // e2 (kept for the skip)
+    __syncthreads();
+    // p3: 2x2 mean of e2 -> 16x16x32 in bufX
+    for (int e = t; e < 16 * 16 * 4; e += kNetThreads) {
+        const int c = (e % 4) * 8, po = e / 4, yo = po / 16, xo = po % 16;
+        const int i00 = ((2 * yo) * 32 + 2 * xo) * 32 + c;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufS + i00);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32);
+        const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * 32);
+        const bf16x8 a3 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * 32 + 32);
+        bf16x8 o;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            o[q] = (__bf16)(0.25f * ((float)a[q] + (float)a1[q] + (float)a2[q] + (float)a3[q]));
+        *reinterpret_cast<bf16x8 *>(bufX + po * 32 + c) = o;
+    }
+    __syncthreads();
+    net_conv<32, 64, 16, 16, true>(bufX, A.w[2], bufA); // e3a -> bufA[0 : 16K)
+    __syncthreads();
+    net_conv<64, 64, 16, 16, true>(bufA, A.w[3], bufA + 16 * 16 * 64); // e3 -> bufA[16K : 32K)
+    __syncthreads();
+    net_conv<64, 32, 16, 16, true>(bufA + 16 * 16 * 64, A.w[4], bufX); // t3 -> bufX
+    __syncthreads();
+    // u2 = nearest upsample of t3 + e2, in place in bufS
+    for (int e = t; e < 32 * 32 * 4; e += kNetThreads) {
+        const int c = (e % 4) * 8, p = e / 4, yy = p / 32, xx = p % 32;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufX + ((yy / 2) * 16 + xx / 2) * 32 + c);
+        bf16x8 k = *reinterpret_cast<const bf16x8 *>(bufS + p * 32 + c);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            k[q] = (__bf16)((float)a[q] + (float)k[q]);
+        *reinterpret_cast<bf16x8 *>(bufS + p * 32 + c) = k;
+    }
+    __syncthreads();
+    net_conv<32, 32, 32, 32, true>(bufS, A.w[5], bufA); // d2
+    __syncthreads();
+    net_conv<32, 16, 32, 32, false>(bufA, A.w[6], A.out + b * 1024 * 16); // out
+}
+
 // wT[ci][tap'*Cout + co] = w[co][(8-tap')*Cin + ci]; bias and pad columns 0.
 __global__ void flip_weights_kernel(const __bf16 *__restrict__ w, int Cin, int Cout, int Kp, int KpT,
                                     __bf16 *__restrict__ wT)
@@ -870,6 +1008,28 @@ static int launch_conv_lds(const ConvArgs &a, hipStream_t s)
     case 2: hipLaunchKernelGGL((conv3x3_fwd_lds_kernel<CT, 2>), grid, dim3(256), lds, s, a); break;
     default: hipLaunchKernelGGL((conv3x3_fwd_lds_kernel<CT, 4>), grid, dim3(256), lds, s, a); break;
     }
+    return FADESIM_OK;
+}
+
+extern "C" int fadesim_nn_coarse_forward(int B, const void *d_x, const void *const *d_w, const int *kp, void *d_out,
+                                         void *stream)
+{
+    static const int cin[7] = {8, 32, 32, 64, 64, 32, 32};
+    if (B < 0 || !d_x || !d_w || !kp || !d_out)
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_coarse_forward: bad arguments");
+    CoarseNetArgs a{};
+    a.x = (const __bf16 *)d_x;
+    a.out = (__bf16 *)d_out;
+    for (int l = 0; l < 7; ++l) {
+        if (!d_w[l] || kp[l] != net_kpad(cin[l]))
+            return set_error(FADESIM_E_INVALID, "fadesim_nn_coarse_forward: layer weights / Kp do not fit");
+        a.w[l] = (const __bf16 *)d_w[l];
+        a.kp[l] = kp[l];
+    }
+    if (B == 0)
+        return FADESIM_OK;
+    hipLaunchKernelGGL(coarse_net_kernel, dim3(B), dim3(kNetThreads), 0, (hipStream_t)stream, a);
+    NN_LAUNCH_CHECK();
     return FADESIM_OK;
 }
 

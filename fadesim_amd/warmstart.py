@@ -294,6 +294,25 @@ class WarmStartNet:
                         del acts[k]
         return acts if keep else acts["out"]
 
+    FUSED_LAYERS = ("e2a", "e2", "e3a", "e3", "t3", "d2", "out")
+
+    @property
+    def fused(self) -> bool:
+        """coarse32f at 128x128 runs as one kernel (fadesim_nn_coarse_forward)."""
+        return self.arch == "coarse32f" and (self.M, self.N) == (128, 128)
+
+    def forward_fused(self, x):
+        """x [B][32][32][8] bf16 -> out [B][32][32][16] bf16, the whole network
+        in one launch (every activation of a crossbar in LDS)."""
+        B = x.shape[0]
+        out = torch.empty((B, 32, 32, OUT_C), dtype=torch.bfloat16, device=self.dev)
+        byname = {op.name: op for op in self.ops}
+        ptrs = (ctypes.c_void_p * 7)(*[self._view(self.wb, byname[n]).data_ptr() for n in self.FUSED_LAYERS])
+        kps = (ctypes.c_int * 7)(*[kpad(byname[n].cin) for n in self.FUSED_LAYERS])
+        _chk(nat.lib().fadesim_nn_coarse_forward(B, ctypes.c_void_p(x.data_ptr()), ptrs, kps,
+                                                 ctypes.c_void_p(out.data_ptr()), _s(self.dev)), "coarse_forward")
+        return out
+
     def _needed_later(self, tensor, current):
         seen = False
         for op in self.ops:
@@ -313,7 +332,7 @@ class WarmStartNet:
         for b0 in range(0, B, chunk):
             b1 = min(B, b0 + chunk)
             feat, base, scale = self.features(g[b0:b1], r[b0:b1], vin[b0:b1])
-            out = self.forward(feat)
+            out = self.forward_fused(feat) if self.fused else self.forward(feat)
             lv = self.out_level
             if lv == 0:
                 _chk(nat.lib().fadesim_nn_combine(b1 - b0, self.M, self.N, ctypes.c_void_p(base.data_ptr()),

@@ -45,6 +45,13 @@ int fadesim_nn_combine(int B, int M, int N, const float *d_base, const float *d_
 int fadesim_nn_combine_up(int B, int M, int N, const float *d_base, const float *d_scale, const void *d_out,
                           int cout, int level, float *d_warm, void *stream);
 
+/* The coarse32f network of B crossbars in one launch, one workgroup per
+ * crossbar with every activation in LDS: x [B][32][32][8] bf16 (pooled
+ * features) -> out [B][32][32][16] bf16.  d_w: 7 layer weight pointers
+ * [Cout][Kp] (e2a, e2, e3a, e3, t3, d2, out), kp: their padded depths. */
+int fadesim_nn_coarse_forward(int B, const void *d_x, const void *const *d_w, const int *kp, void *d_out,
+                              void *stream);
+
 /* 3x3 same convolution (MFMA), optional ReLU: y [B][H][W][Cout] */
 int fadesim_nn_conv3x3_fwd(int B, int H, int W, int Cin, int Cout, int Kp, const void *d_x, const void *d_w, void *d_y,
                            int relu, void *stream);

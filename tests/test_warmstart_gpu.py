@@ -257,3 +257,20 @@ def test_pooled_features_match_pooled_full_features(cuda):
     # a trained coarse network runs on them end to end
     w = pooled.predict(tg, tr, tv)
     assert w.shape == (4, 2, 128, 128) and bool(torch.isfinite(w).all())
+
+
+def test_fused_coarse_network_equals_layered(cuda):
+    """The one-kernel coarse network (activations in LDS) against the same
+    network run layer by layer: the same GEMM fragments and order, so the
+    outputs agree to the last bf16 bit."""
+    import torch
+    from fadesim_amd.warmstart import WarmStartNet
+    net = WarmStartNet(cuda, arch="coarse32f", seed=3)
+    assert net.fused
+    gen = torch.Generator().manual_seed(5)
+    x = (torch.randn(37, 32, 32, 8, generator=gen) * 0.5).to(torch.bfloat16).to(cuda)
+    a = net.forward(x)
+    b = net.forward_fused(x)
+    torch.cuda.synchronize()
+    assert a.shape == b.shape == (37, 32, 32, 16)
+    assert torch.equal(a, b), float((a.float() - b.float()).abs().max())
