@@ -543,31 +543,34 @@ constexpr int kNetThreads = 1024;
 
 constexpr int net_kpad(int cin) { return ((9 * cin + 1 + 31) / 32) * 32; }
 
-// One layer: each of the 16 waves takes P = (H W / 16) / 16 pixel tiles of 16
-// pixels and all COUT channels, so every weight fragment loaded serves P
-// tiles; the depth is a compile-time constant (the k loop unrolls by 2 and
-// the next fragments load while the present ones multiply).
-template <int CIN, int COUT, int H, int W, bool RELU>
+// One layer on the 16 waves of the workgroup: wave (cg, pg) computes CTW
+// output-channel tiles (16 channels each) of P pixel tiles (16 pixels each),
+// so a weight fragment serves P tiles and an input fragment CTW tiles; the
+// depth is a compile-time constant (the k loop unrolls by 2 so that the next
+// fragments load while the present ones multiply).
+template <int CIN, int COUT, int H, int W, bool RELU, int CTW, int P>
 __device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__restrict__ w, __bf16 *dst)
 {
     constexpr int CT = COUT / 16, PT = H * W / 16, K9 = 9 * CIN, NW = kNetThreads / 64, KP = net_kpad(CIN);
-    constexpr int P = PT / NW;
-    static_assert(W % 16 == 0 && CIN % 8 == 0 && COUT % 16 == 0 && PT % NW == 0, "tile shapes");
+    constexpr int NPG = PT / P; // pixel groups
+    static_assert(W % 16 == 0 && CIN % 8 == 0 && COUT % 16 == 0, "tile shapes");
+    static_assert(CT % CTW == 0 && PT % P == 0 && (CT / CTW) * NPG == NW, "waves x tiles");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = lane & 15, g = lane >> 4;
-    f4 acc[P][CT];
+    const int pg = wave % NPG, cg = wave / NPG;
+    f4 acc[P][CTW];
 #pragma unroll
     for (int q = 0; q < P; ++q)
 #pragma unroll
-        for (int t = 0; t < CT; ++t)
+        for (int t = 0; t < CTW; ++t)
             acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 2
     for (int k0 = 0; k0 < KP; k0 += 32) {
         const int k = k0 + 8 * g;
-        bf16x8 av[CT];
+        bf16x8 av[CTW];
 #pragma unroll
-        for (int t = 0; t < CT; ++t)
-            av[t] = *reinterpret_cast<const bf16x8 *>(w + (long)(16 * t + j) * KP + k);
+        for (int t = 0; t < CTW; ++t)
+            av[t] = *reinterpret_cast<const bf16x8 *>(w + (long)(16 * (cg * CTW + t) + j) * KP + k);
         int tap = 0, ci = 0;
         const bool real = k < K9;
         if (real) {
@@ -577,7 +580,7 @@ __device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__rest
         const int ky = tap / 3, kx = tap - 3 * ky;
 #pragma unroll
         for (int q = 0; q < P; ++q) {
-            const int p0 = 16 * (wave + NW * q), y = p0 / W, x0 = p0 % W;
+            const int p0 = 16 * (pg * P + q), y = p0 / W, x0 = p0 % W;
             bf16x8 bv = zero8();
             if (real) {
                 const int yy = y + ky - 1, xx = x0 + j + kx - 1;
@@ -587,16 +590,16 @@ __device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__rest
                 bv[0] = (__bf16)1.0f; // bias column
             }
 #pragma unroll
-            for (int t = 0; t < CT; ++t)
+            for (int t = 0; t < CTW; ++t)
                 acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[q][t], 0, 0, 0);
         }
     }
     typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 #pragma unroll
     for (int q = 0; q < P; ++q) {
-        const int p0 = 16 * (wave + NW * q);
+        const int p0 = 16 * (pg * P + q);
 #pragma unroll
-        for (int t = 0; t < CT; ++t) {
+        for (int t = 0; t < CTW; ++t) {
             bf16x4 o;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -605,7 +608,7 @@ __device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__rest
                     v = fmaxf(v, 0.f);
                 o[r] = (__bf16)v;
             }
-            *reinterpret_cast<bf16x4 *>(dst + (p0 + j) * COUT + 16 * t + 4 * g) = o;
+            *reinterpret_cast<bf16x4 *>(dst + (p0 + j) * COUT + 16 * (cg * CTW + t) + 4 * g) = o;
         }
     }
 }
@@ -621,9 +624,9 @@ __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A
     for (int e = t; e < 32 * 32; e += kNetThreads)
         *reinterpret_cast<bf16x8 *>(bufX + 8 * e) = *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
     __syncthreads();
-    net_conv<8, 32, 32, 32, true>(bufX, A.w[0], bufA); // e2a
+    net_conv<8, 32, 32, 32, true, 1, 8>(bufX, A.w[0], bufA); // e2a
     __syncthreads();
-    net_conv<32, 32, 32, 32, true>(bufA, A.w[1], bufS); // e2 (kept for the skip)
+    net_conv<32, 32, 32, 32, true, 1, 8>(bufA, A.w[1], bufS); // e2 (kept for the skip)
     __syncthreads();
     // p3: 2x2 mean of e2 -> 16x16x32 in bufX
     for (int e = t; e < 16 * 16 * 4; e += kNetThreads) {
@@ -640,11 +643,11 @@ __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A
         *reinterpret_cast<bf16x8 *>(bufX + po * 32 + c) = o;
     }
     __syncthreads();
-    net_conv<32, 64, 16, 16, true>(bufX, A.w[2], bufA); // e3a -> bufA[0 : 16K)
+    net_conv<32, 64, 16, 16, true, 1, 4>(bufX, A.w[2], bufA); // e3a -> bufA[0 : 16K)
     __syncthreads();
-    net_conv<64, 64, 16, 16, true>(bufA, A.w[3], bufA + 16 * 16 * 64); // e3 -> bufA[16K : 32K)
+    net_conv<64, 64, 16, 16, true, 1, 4>(bufA, A.w[3], bufA + 16 * 16 * 64); // e3 -> bufA[16K : 32K)
     __syncthreads();
-    net_conv<64, 32, 16, 16, true>(bufA + 16 * 16 * 64, A.w[4], bufX); // t3 -> bufX
+    net_conv<64, 32, 16, 16, true, 1, 2>(bufA + 16 * 16 * 64, A.w[4], bufX); // t3 -> bufX
     __syncthreads();
     // u2 = nearest upsample of t3 + e2, in place in bufS
     for (int e = t; e < 32 * 32 * 4; e += kNetThreads) {
@@ -657,9 +660,9 @@ __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A
         *reinterpret_cast<bf16x8 *>(bufS + p * 32 + c) = k;
     }
     __syncthreads();
-    net_conv<32, 32, 32, 32, true>(bufS, A.w[5], bufA); // d2
+    net_conv<32, 32, 32, 32, true, 1, 8>(bufS, A.w[5], bufA); // d2
     __syncthreads();
-    net_conv<32, 16, 32, 32, false>(bufA, A.w[6], A.out + b * 1024 * 16); // out
+    net_conv<32, 16, 32, 32, false, 1, 4>(bufA, A.w[6], A.out + b * 1024 * 16); // out
 }
 
 // wT[ci][tap'*Cout + co] = w[co][(8-tap')*Cin + ci]; bias and pad columns 0.
