@@ -613,6 +613,122 @@ __device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__rest
     }
 }
 
+// Lane shifts within rows of 16 lanes (DPP row_shr:1 / row_shl:1): lane j
+// receives lane j-1's (j+1's) fragment, 0 where that lane is outside the row.
+__device__ __forceinline__ bf16x8 frag_from_left(bf16x8 v)
+{
+    typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i4;
+    i4 x = __builtin_bit_cast(i4, v), y;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        y[e] = __builtin_amdgcn_update_dpp(0, x[e], 0x111, 0xF, 0xF, true);
+    return __builtin_bit_cast(bf16x8, y);
+}
+__device__ __forceinline__ bf16x8 frag_from_right(bf16x8 v)
+{
+    typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i4;
+    i4 x = __builtin_bit_cast(i4, v), y;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        y[e] = __builtin_amdgcn_update_dpp(0, x[e], 0x101, 0xF, 0xF, true);
+    return __builtin_bit_cast(bf16x8, y);
+}
+
+// net_conv for CIN a multiple of 32: the k blocks of one kernel row (taps
+// kx = 0, 1, 2 of row ky, each CIN/32 blocks) are built from ONE load of the
+// centre tap: the left and right taps are the centre fragments of the
+// neighbouring lanes (DPP row shifts), only the two edge pixels of the 16-pixel
+// segment are loaded again.  Same fragments, same k order as net_conv.
+template <int CIN, int COUT, int H, int W, bool RELU, int CTW, int P>
+__device__ __forceinline__ void net_conv_rows(const __bf16 *src, const __bf16 *__restrict__ w, __bf16 *dst)
+{
+    constexpr int CT = COUT / 16, PT = H * W / 16, K9 = 9 * CIN, NW = kNetThreads / 64, KP = net_kpad(CIN);
+    constexpr int NPG = PT / P, CB = CIN / 32; // pixel groups, k blocks per tap
+    static_assert(W % 16 == 0 && CIN % 32 == 0 && COUT % 16 == 0 && KP == K9 + 32, "tile shapes");
+    static_assert(CT % CTW == 0 && PT % P == 0 && (CT / CTW) * NPG == NW, "waves x tiles");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = lane & 15, g = lane >> 4;
+    const int pg = wave % NPG, cg = wave / NPG;
+    f4 acc[P][CTW];
+#pragma unroll
+    for (int q = 0; q < P; ++q)
+#pragma unroll
+        for (int t = 0; t < CTW; ++t)
+            acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int ky = 0; ky < 3; ++ky) {
+        bf16x8 av[3][CB][CTW]; // weights of the three taps of this kernel row
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                for (int t = 0; t < CTW; ++t)
+                    av[kx][cb][t] = *reinterpret_cast<const bf16x8 *>(
+                        w + (long)(16 * (cg * CTW + t) + j) * KP + ((3 * ky + kx) * CIN + 32 * cb + 8 * g));
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int p0 = 16 * (pg * P + q), y = p0 / W, x0 = p0 % W, yy = y + ky - 1;
+            if (yy < 0 || yy >= H)
+                continue; // the whole segment reads padding: the fragments are zero
+            const __bf16 *rowp = src + (yy * W) * CIN + 8 * g;
+            bf16x8 b[3][CB];
+#pragma unroll
+            for (int cb = 0; cb < CB; ++cb) {
+                const bf16x8 c = *reinterpret_cast<const bf16x8 *>(rowp + (x0 + j) * CIN + 32 * cb);
+                bf16x8 l = frag_from_left(c), r = frag_from_right(c);
+                if (j == 0 && x0 > 0)
+                    l = *reinterpret_cast<const bf16x8 *>(rowp + (x0 - 1) * CIN + 32 * cb);
+                if (j == 15 && x0 + 16 < W)
+                    r = *reinterpret_cast<const bf16x8 *>(rowp + (x0 + 16) * CIN + 32 * cb);
+                b[0][cb] = l;
+                b[1][cb] = c;
+                b[2][cb] = r;
+            }
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx)
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+                    for (int t = 0; t < CTW; ++t)
+                        acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[kx][cb][t], b[kx][cb], acc[q][t], 0,
+                                                                           0, 0);
+        }
+    }
+    // the bias block: k = K9 is the ones column
+    {
+        bf16x8 av[CTW];
+#pragma unroll
+        for (int t = 0; t < CTW; ++t)
+            av[t] = *reinterpret_cast<const bf16x8 *>(w + (long)(16 * (cg * CTW + t) + j) * KP + K9 + 8 * g);
+        bf16x8 bv = zero8();
+        if (g == 0)
+            bv[0] = (__bf16)1.0f;
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+#pragma unroll
+            for (int t = 0; t < CTW; ++t)
+                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[q][t], 0, 0, 0);
+    }
+    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        const int p0 = 16 * (pg * P + q);
+#pragma unroll
+        for (int t = 0; t < CTW; ++t) {
+            bf16x4 o;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = acc[q][t][r];
+                if (RELU)
+                    v = fmaxf(v, 0.f);
+                o[r] = (__bf16)v;
+            }
+            *reinterpret_cast<bf16x4 *>(dst + (p0 + j) * COUT + 16 * (cg * CTW + t) + 4 * g) = o;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A)
 {
     __shared__ __align__(16) __bf16 bufA[32 * 32 * 32];
@@ -626,7 +742,7 @@ __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A
     __syncthreads();
     net_conv<8, 32, 32, 32, true, 1, 8>(bufX, A.w[0], bufA); // e2a
     __syncthreads();
-    net_conv<32, 32, 32, 32, true, 1, 8>(bufA, A.w[1], bufS); // e2 (kept for the skip)
+    net_conv_rows<32, 32, 32, 32, true, 1, 8>(bufA, A.w[1], bufS); // e2 (kept for the skip)
     __syncthreads();
     // p3: 2x2 mean of e2 -> 16x16x32 in bufX
     for (int e = t; e < 16 * 16 * 4; e += kNetThreads) {
@@ -643,11 +759,11 @@ __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A
         *reinterpret_cast<bf16x8 *>(bufX + po * 32 + c) = o;
     }
     __syncthreads();
-    net_conv<32, 64, 16, 16, true, 1, 4>(bufX, A.w[2], bufA); // e3a -> bufA[0 : 16K)
+    net_conv_rows<32, 64, 16, 16, true, 1, 4>(bufX, A.w[2], bufA); // e3a -> bufA[0 : 16K)
     __syncthreads();
-    net_conv<64, 64, 16, 16, true, 1, 4>(bufA, A.w[3], bufA + 16 * 16 * 64); // e3 -> bufA[16K : 32K)
+    net_conv_rows<64, 64, 16, 16, true, 1, 4>(bufA, A.w[3], bufA + 16 * 16 * 64); // e3 -> bufA[16K : 32K)
     __syncthreads();
-    net_conv<64, 32, 16, 16, true, 1, 2>(bufA + 16 * 16 * 64, A.w[4], bufX); // t3 -> bufX
+    net_conv_rows<64, 32, 16, 16, true, 1, 2>(bufA + 16 * 16 * 64, A.w[4], bufX); // t3 -> bufX
     __syncthreads();
     // u2 = nearest upsample of t3 + e2, in place in bufS
     for (int e = t; e < 32 * 32 * 4; e += kNetThreads) {
@@ -660,9 +776,9 @@ __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A
         *reinterpret_cast<bf16x8 *>(bufS + p * 32 + c) = k;
     }
     __syncthreads();
-    net_conv<32, 32, 32, 32, true, 1, 8>(bufS, A.w[5], bufA); // d2
+    net_conv_rows<32, 32, 32, 32, true, 1, 8>(bufS, A.w[5], bufA); // d2
     __syncthreads();
-    net_conv<32, 16, 32, 32, false, 1, 4>(bufA, A.w[6], A.out + b * 1024 * 16); // out
+    net_conv_rows<32, 16, 32, 32, false, 1, 4>(bufA, A.w[6], A.out + b * 1024 * 16); // out
 }
 
 // wT[ci][tap'*Cout + co] = w[co][(8-tap')*Cin + ci]; bias and pad columns 0.
