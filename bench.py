@@ -56,11 +56,12 @@ def make_batch(B, seed):
     return g, r, vin
 
 
-def algorithmic_bytes(passes_sum, B):
+def algorithmic_bytes(passes_sum, B, warm=False):
     """Streaming-schedule bytes: 4 B of conductance per cell per pass over the
     crossbar (a red-black sweep or a wire-solve iteration), plus the inputs and
-    the FP64 readout (4*M + 8*N per crossbar)."""
-    return 4.0 * M * N * passes_sum + B * (4.0 * M + 8.0 * N)
+    the FP64 readout (4*M + 8*N per crossbar), plus the FP32 warm-start field
+    (2*M*N*4) when the solve reads one from HBM."""
+    return 4.0 * M * N * passes_sum + B * (4.0 * M + 8.0 * N + (8.0 * M * N if warm else 0.0))
 
 
 def load_traffic(variant):
@@ -268,7 +269,7 @@ def main():
             raise RuntimeError(f"{name}: not every crossbar converged")
         kernel_ms = float(np.mean(kms))
         passes = float(sw.sum()) / args.steps + k_ir2s * B
-        ach = algorithmic_bytes(passes, B) / (kernel_ms * 1e-3) / 1e9
+        ach = algorithmic_bytes(passes, B, warm=name.startswith("nn")) / (kernel_ms * 1e-3) / 1e9
         meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                       "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
                       "kernel_GBps_algorithmic": ach}
@@ -302,7 +303,8 @@ def main():
                        "traffic": traffic,
                        "kernel": "rbsor_kernel<Geo<4,8,32,16>>", "kernel_ms": hm["kernel_ms"],
                        "bytes_model": "(sweeps + IR2S iterations) * 4*M*N + 4*M + 8*N per crossbar "
-                                      "(conductance streamed once per sweep or wire-solve pass)"}
+                                      "(conductance streamed once per sweep or wire-solve pass), + 8*M*N "
+                                      "for the warm-start field read from HBM"}
     if "plain" in meas:
         pm = meas["plain"]
         out["roofline_plain_sor"] = {"bound": "hbm", "achieved": pm["kernel_GBps_algorithmic"], "peak": HBM_PEAK_GBS,
