@@ -373,6 +373,39 @@ __global__ void combine_up_kernel(const float *__restrict__ base, const float *_
     }
 }
 
+// The same combination with 32-bit cell indexing: block (cells, crossbar),
+// one cell per thread (the 64-bit divisions of the flat form are the cost of
+// that kernel, not its bytes).
+__global__ void __launch_bounds__(256) combine_up2d_kernel(const float *__restrict__ base,
+                                                          const float *__restrict__ scale,
+                                                          const __bf16 *__restrict__ out, int Cout, int M, int N,
+                                                          int level, float *__restrict__ warm)
+{
+    const int id = blockIdx.x * 256 + threadIdx.x, MN = M * N;
+    if (id >= MN)
+        return;
+    const int f = 1 << level, Hc = M >> level, Wc = N >> level;
+    const float inv = 1.0f / (float)f;
+    const int i = id / N, j = id - i * N;
+    const size_t b = blockIdx.y;
+    const float y = fminf(fmaxf(((float)i + 0.5f) * inv - 0.5f, 0.0f), (float)(Hc - 1));
+    const float x = fminf(fmaxf(((float)j + 0.5f) * inv - 0.5f, 0.0f), (float)(Wc - 1));
+    const int y0 = (int)y, x0 = (int)x;
+    const int y1 = min(y0 + 1, Hc - 1), x1 = min(x0 + 1, Wc - 1);
+    const float wy = y - (float)y0, wx = x - (float)x0;
+    const __bf16 *o = out + b * (size_t)Hc * Wc * Cout;
+    const float s = scale[b];
+    const float *bb = base + b * 2 * (size_t)MN;
+    float *ww = warm + b * 2 * (size_t)MN;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const float v00 = (float)o[(y0 * Wc + x0) * Cout + c], v01 = (float)o[(y0 * Wc + x1) * Cout + c];
+        const float v10 = (float)o[(y1 * Wc + x0) * Cout + c], v11 = (float)o[(y1 * Wc + x1) * Cout + c];
+        const float top = v00 + wx * (v01 - v00), bot = v10 + wx * (v11 - v10);
+        ww[c * MN + id] = bb[c * MN + id] + s * (top + wy * (bot - top));
+    }
+}
+
 // ---------------------------------------------------------------- conv forward
 struct ConvArgs {
     int B, H, W, Cin, Cout, Kp;
@@ -1107,6 +1140,14 @@ extern "C" int fadesim_nn_combine_up(int B, int M, int N, const float *d_base, c
     if (B < 0 || cout < 2 || level < 0 || level > 6 || !d_base || !d_scale || !d_out || !d_warm ||
         (M >> level) < 1 || (N >> level) < 1 || ((M >> level) << level) != M || ((N >> level) << level) != N)
         return set_error(FADESIM_E_INVALID, "fadesim_nn_combine_up: bad arguments");
+    if (B == 0)
+        return FADESIM_OK;
+    if ((size_t)M * N * 2 < (1u << 31) && B <= 65535) {
+        hipLaunchKernelGGL(combine_up2d_kernel, dim3((M * N + 255) / 256, B), dim3(256), 0, (hipStream_t)stream,
+                           d_base, d_scale, (const __bf16 *)d_out, cout, M, N, level, d_warm);
+        NN_LAUNCH_CHECK();
+        return FADESIM_OK;
+    }
     const size_t n = (size_t)B * M * N;
     hipLaunchKernelGGL(combine_up_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, d_base, d_scale,
                        (const __bf16 *)d_out, cout, B, M, N, level, d_warm);
