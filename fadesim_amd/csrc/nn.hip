@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(ConvArgs A)
 // KB each) and one 32x32x8 / 16x16x32 buffer (16 KB).
 struct CoarseNetArgs {
     const __bf16 *x;   // [B][32][32][8]
-    const __bf16 *w[7]; // e2a, e2, e3a, e3, t3, d2, out: [Cout][Kp]
+    const __bf16 *w[7]; // e2a, e2, e3a, e3, t3, d2, out: fragment-major [Kp/32][Cout][32]
     int kp[7];
     __bf16 *out;       // [B][32][32][16]
 };
@@ -602,8 +602,8 @@ __device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__rest
         const int k = k0 + 8 * g;
         bf16x8 av[CTW];
 #pragma unroll
-        for (int t = 0; t < CTW; ++t)
-            av[t] = *reinterpret_cast<const bf16x8 *>(w + (long)(16 * (cg * CTW + t) + j) * KP + k);
+        for (int t = 0; t < CTW; ++t) // fragment-major weights: one contiguous 1 KB per wave
+            av[t] = *reinterpret_cast<const bf16x8 *>(w + ((k0 / 32) * COUT + 16 * (cg * CTW + t) + j) * 32 + 8 * g);
         int tap = 0, ci = 0;
         const bool real = k < K9;
         if (real) {
@@ -698,7 +698,7 @@ __device__ __forceinline__ void net_conv_rows(const __bf16 *src, const __bf16 *_
 #pragma unroll
                 for (int t = 0; t < CTW; ++t)
                     av[kx][cb][t] = *reinterpret_cast<const bf16x8 *>(
-                        w + (long)(16 * (cg * CTW + t) + j) * KP + ((3 * ky + kx) * CIN + 32 * cb + 8 * g));
+                        w + (((3 * ky + kx) * CB + cb) * COUT + 16 * (cg * CTW + t) + j) * 32 + 8 * g);
 #pragma unroll
         for (int q = 0; q < P; ++q) {
             const int p0 = 16 * (pg * P + q), y = p0 / W, x0 = p0 % W, yy = y + ky - 1;
@@ -733,7 +733,7 @@ __device__ __forceinline__ void net_conv_rows(const __bf16 *src, const __bf16 *_
         bf16x8 av[CTW];
 #pragma unroll
         for (int t = 0; t < CTW; ++t)
-            av[t] = *reinterpret_cast<const bf16x8 *>(w + (long)(16 * (cg * CTW + t) + j) * KP + K9 + 8 * g);
+            av[t] = *reinterpret_cast<const bf16x8 *>(w + ((K9 / 32) * COUT + 16 * (cg * CTW + t) + j) * 32 + 8 * g);
         bf16x8 bv = zero8();
         if (g == 0)
             bv[0] = (__bf16)1.0f;

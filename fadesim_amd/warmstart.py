@@ -181,6 +181,7 @@ class WarmStartNet:
             wl[:, :9 * op.cin] = torch.randn(op.cout, 9 * op.cin, generator=g) * std
             self._view(self.w, op).copy_(wl.to(self.dev))
         self.wb = self.w.to(torch.bfloat16)
+        self._wver = 0  # bumped whenever the bf16 weights change (caches derived layouts)
         self.info: dict = {}
         # tensors whose gradient the backward pass needs: outputs of ops that
         # depend on a convolution (pools of the input features do not)
@@ -238,6 +239,7 @@ class WarmStartNet:
             raise RuntimeError(f"weight file has {w.size} floats, architecture needs {net.nparams_padded}")
         net.w.copy_(torch.from_numpy(w).to(net.dev))
         net.wb = net.w.to(torch.bfloat16)
+        net._wver += 1
         net.info = man.get("info", {})
         return net
 
@@ -307,7 +309,12 @@ class WarmStartNet:
         B = x.shape[0]
         out = torch.empty((B, 32, 32, OUT_C), dtype=torch.bfloat16, device=self.dev)
         byname = {op.name: op for op in self.ops}
-        ptrs = (ctypes.c_void_p * 7)(*[self._view(self.wb, byname[n]).data_ptr() for n in self.FUSED_LAYERS])
+        if getattr(self, "_frag_ver", None) != self._wver:
+            # fragment-major weights [Kp/32][Cout][32]: a wave's fragment is 1 KB contiguous
+            self._frag = [self._view(self.wb, byname[n]).view(byname[n].cout, -1, 32).transpose(0, 1).contiguous()
+                          for n in self.FUSED_LAYERS]
+            self._frag_ver = self._wver
+        ptrs = (ctypes.c_void_p * 7)(*[f.data_ptr() for f in self._frag])
         kps = (ctypes.c_int * 7)(*[kpad(byname[n].cin) for n in self.FUSED_LAYERS])
         _chk(nat.lib().fadesim_nn_coarse_forward(B, ctypes.c_void_p(x.data_ptr()), ptrs, kps,
                                                  ctypes.c_void_p(out.data_ptr()), _s(self.dev)), "coarse_forward")
@@ -446,6 +453,7 @@ class WarmStartNet:
                                        ctypes.c_void_p(self.wb.data_ptr()), ctypes.c_float(lr_t),
                                        ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8), step,
                                        ctypes.c_float(wd), ctypes.c_float(1.0), s), "adam")
+                self._wver += 1
                 if it % 50 == 0:
                     lv = float(loss.item())
                     ep_loss = lv
