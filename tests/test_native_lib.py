@@ -90,3 +90,37 @@ def test_hook_refuses_cpu_tensors():
         hook.ir_gs_ou(inp, G, 1, 4, 4, 2, 2, 1, 1, 1, 1.0, 1.0, False, 0.0, False, [])
     with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
         hook.ir2s_ou_local(inp, G, 1, 4, 4, 1, 1, 1, 1.0, 1.0, 1.0, False, 0.0, False, [])
+
+
+def test_c_abi_rejects_bad_descriptors_without_a_gpu():
+    """Argument validation of the C entry points happens before any device
+    work: each call returns FADESIM_E_INVALID / _UNSUPPORTED with a message."""
+    import ctypes
+    from fadesim_amd import _native as nat
+    L = nat.lib()
+    E_INVALID = -1
+
+    def err(rc):
+        assert rc != 0
+        return L.fadesim_last_error().decode()
+
+    assert "null descriptor" in err(L.fadesim_solve_batch(None, None))
+    assert "null descriptor" in err(L.fadesim_hook_ir_drop_gs(None, None))
+    h = nat.HookDesc()
+    h.row_size, h.col_size, h.batch_size, h.numX, h.numY = 16, 12, 1, 1, 1
+    h.d_input = h.d_G = h.d_out = 8  # never dereferenced: validation fails first
+    h.ou_rows, h.ou_cols = 5, 4
+    assert "OU sizes" in err(L.fadesim_hook_ir_drop_gs(ctypes.byref(h), None))
+    h.row_size = 1
+    assert "row_size" in err(L.fadesim_hook_ir_drop_gs(ctypes.byref(h), None))
+    q = nat.Ir2sDesc()
+    q.dtype_bytes, q.layout, q.batch_size, q.row_size, q.col_size, q.numX, q.numY = 8, nat.IR2S_HOOK, 1, 8, 8, 1, 1
+    q.r_wire = 0.0
+    assert "r_wire" in err(L.fadesim_ir2s(ctypes.byref(q), None))
+    kp = (ctypes.c_int * 7)(96, 320, 320, 608, 608, 320, 320)
+    w = (ctypes.c_void_p * 7)(*([8] * 7))
+    assert L.fadesim_nn_coarse_forward(-1, ctypes.c_void_p(8), w, kp, ctypes.c_void_p(8), None) == E_INVALID
+    kp[3] = 600
+    assert "Kp" in err(L.fadesim_nn_coarse_forward(1, ctypes.c_void_p(8), w, kp, ctypes.c_void_p(8), None))
+    p = [ctypes.c_void_p(8)] * 6
+    assert "128x128" in err(L.fadesim_nn_features_pooled(1, 64, 64, 2, *p, None))
