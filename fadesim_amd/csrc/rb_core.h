@@ -61,7 +61,7 @@
 namespace fadesim {
 namespace rb {
 
-constexpr int kWsPlanes = 6; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q | streamed currents
+constexpr int kWsPlanes = 5; // workspace planes per slot: zq | refinement rho p, q | snapshot p, q
 constexpr int kMaxStrips = 16;
 constexpr int kRowsLd = 512;                      // row stride of the strip exchange buffers
 constexpr unsigned long long kWaitTicks = 500000000ull; // 5 s of the 100 MHz constant clock
@@ -99,7 +99,8 @@ struct Inst {
 // Team of S workgroups solving one crossbar, strip s = rows [s*M, (s+1)*M).
 // Neighbouring strips exchange their boundary bitline rows after every
 // half-sweep through `rows` (double-buffered by publication parity) and
-// per-strip publication counters `flag` (release/acquire at agent scope);
+// per-strip publication counters `flag` (device-coherent stores and loads,
+// team_st / team_ld, with no cache maintenance);
 // column sums of the convergence test and of the set-up go through `part` /
 // `setup` and the counter barrier `cnt`.  A wait that does not end within
 // kWaitTicks raises `abort`, which every wait checks, so no workgroup spins
@@ -253,9 +254,10 @@ __device__ __forceinline__ void publish_all(const float (&yp)[G::TR][G::TC],
 
 // ---------------------------------------------------------------- strip exchange
 
-// Spin until *p >= target (agent-scope acquire).  All active lanes of the
-// wave call it together; false when the team aborted or the wait timed out.
-__device__ __forceinline__ bool wait_geq(const int *p, int target, int *abort)
+// Spin until *p >= target (relaxed: no cache maintenance).  All active
+// lanes of the wave call it together; false when the team aborted or the wait
+// timed out.
+__device__ __forceinline__ bool poll_geq(const int *p, int target, int *abort)
 {
     if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         const unsigned long long t0 = wall_clock64();
@@ -269,13 +271,50 @@ __device__ __forceinline__ bool wait_geq(const int *p, int target, int *abort)
             __builtin_amdgcn_s_sleep(2);
         }
     }
+    return true;
+}
+
+// poll_geq, then an agent-scope acquire: ordinary loads after it see what the
+// releasing side stored before its release.
+__device__ __forceinline__ bool wait_geq(const int *p, int target, int *abort)
+{
+    if (!poll_geq(p, target, abort))
+        return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return true;
 }
 
+// Team exchanges carry no whole-cache maintenance (an agent-scope release
+// writes back the XCD's L2, an acquire invalidates it: ~15 us per exchange
+// on a busy chip).  Instead every value a strip publishes is stored with a
+// device-coherent store (team_st), the wave waits until those stores have
+// completed (team_commit) before the publishing counter moves, and readers,
+// once they have seen the counter, read with device-coherent loads (team_ld),
+// which do not return stale cached lines.
+__device__ __forceinline__ void team_st(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void team_st(float *p, float v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double team_ld(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float team_ld(const float *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// This wave's earlier vector-memory operations have completed (a store: it
+// is visible at device scope); also a compiler barrier.
+__device__ __forceinline__ void team_commit() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Publication `pub` (0-based) of this strip's boundary bitline rows: the
 // top-row threads write row 0, the bottom-row threads row M-1, each group
-// then releases its counter to pub + 1.
+// then moves its counter to pub + 1 (a thread row lies within one wave, so
+// the wave's team_commit covers all of the row's stores).
 template <class G>
 __device__ __forceinline__ void strip_publish(const StripCtx &X, const float (&yq)[G::TR][G::TC], int ty,
                                               int j0, int lane, int pub)
@@ -283,15 +322,19 @@ __device__ __forceinline__ void strip_publish(const StripCtx &X, const float (&y
     const int par = pub & 1;
     if (ty == 0 && X.s > 0) {
         float *dst = X.rows + (((size_t)par * X.S + X.s) * 2 + 0) * kRowsLd + j0;
-        strow<G::TC>(dst, yq[0]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+#pragma unroll
+        for (int k = 0; k < G::TC; ++k)
+            team_st(dst + k, yq[0][k]);
+        team_commit();
         if ((lane % G::TX) == 0)
             __hip_atomic_store(X.flag + 2 * X.s + 0, pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (ty == G::TY - 1 && X.s < X.S - 1) {
         float *dst = X.rows + (((size_t)par * X.S + X.s) * 2 + 1) * kRowsLd + j0;
-        strow<G::TC>(dst, yq[G::TR - 1]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+#pragma unroll
+        for (int k = 0; k < G::TC; ++k)
+            team_st(dst + k, yq[G::TR - 1][k]);
+        team_commit();
         if ((lane % G::TX) == 0)
             __hip_atomic_store(X.flag + 2 * X.s + 1, pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -308,10 +351,12 @@ __device__ __forceinline__ bool strip_fetch(const StripCtx &X, float *halo, int 
     const int par = pub & 1;
     const int j0 = TC * tx;
     if (ty == 0 && X.s > 0) {
-        if (wait_geq(X.flag + 2 * (X.s - 1) + 1, pub + 1, X.abort)) {
+        if (poll_geq(X.flag + 2 * (X.s - 1) + 1, pub + 1, X.abort)) {
             const float *src = X.rows + (((size_t)par * X.S + X.s - 1) * 2 + 1) * kRowsLd + j0;
             float v[TC];
-            ldrow<TC>(src, v);
+#pragma unroll
+            for (int k = 0; k < TC; ++k)
+                v[k] = team_ld(src + k);
 #pragma unroll
             for (int k = 0; k < TC; ++k)
                 halo[G::QB + k * QLD + tx] = v[k]; // virtual thread row -1
@@ -320,10 +365,12 @@ __device__ __forceinline__ bool strip_fetch(const StripCtx &X, float *halo, int 
         }
     }
     if (ty == G::TY - 1 && X.s < X.S - 1) {
-        if (wait_geq(X.flag + 2 * (X.s + 1) + 0, pub + 1, X.abort)) {
+        if (poll_geq(X.flag + 2 * (X.s + 1) + 0, pub + 1, X.abort)) {
             const float *src = X.rows + (((size_t)par * X.S + X.s + 1) * 2 + 0) * kRowsLd + j0;
             float v[TC];
-            ldrow<TC>(src, v);
+#pragma unroll
+            for (int k = 0; k < TC; ++k)
+                v[k] = team_ld(src + k);
 #pragma unroll
             for (int k = 0; k < TC; ++k)
                 halo[G::QT + k * QLD + TX + G::NT + tx] = v[k]; // virtual thread row TY
@@ -335,20 +382,55 @@ __device__ __forceinline__ bool strip_fetch(const StripCtx &X, float *halo, int 
     return *s_dead == 0;
 }
 
-// Team barrier number `nbar` (0-based): every member's stores before it are
-// visible to every member after it.  Ends with a workgroup barrier; false
+// Team sums of the two FP64 values every strip published for column t, at
+// base + s stride + t and base + s stride + kRowsLd + t: over the strips
+// above this one (E0, E1) and over the whole team (T0, T1), each accumulated
+// in strip order.  The loads go out in batches of kBatch strips, each batch
+// in flight together, so the exchange costs kMaxStrips / kBatch memory
+// latencies instead of one per strip, at 4 kBatch VGPRs.
+__device__ __forceinline__ void team_sums(const double *base, size_t stride, int S, int below, int t, double &E0,
+                                          double &E1, double &T0, double &T1)
+{
+    constexpr int kBatch = 8;
+    E0 = E1 = T0 = T1 = 0.0;
+#pragma unroll
+    for (int s0 = 0; s0 < kMaxStrips; s0 += kBatch) {
+        if (s0 >= S)
+            break;
+        double a[kBatch], b[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const double *ps = base + (size_t)(s0 + k < S ? s0 + k : 0) * stride;
+            a[k] = team_ld(ps + t);
+            b[k] = team_ld(ps + kRowsLd + t);
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            if (s0 + k < below) {
+                E0 += a[k];
+                E1 += b[k];
+            }
+            if (s0 + k < S) {
+                T0 += a[k];
+                T1 += b[k];
+            }
+        }
+    }
+}
+
+// Team barrier number `nbar` (0-based): every member's team_st stores before
+// it are visible to every member's team_ld loads after it.  Ends with a workgroup barrier; false
 // (uniformly) when the team aborted.
 __device__ __forceinline__ bool team_barrier(const StripCtx &X, int *s_dead, int t, int nbar)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    team_commit();
     __syncthreads();
     if (t == 0) {
-        __hip_atomic_fetch_add(X.cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (!wait_geq(X.cnt, X.S * (nbar + 1), X.abort))
+        __hip_atomic_fetch_add(X.cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!poll_geq(X.cnt, X.S * (nbar + 1), X.abort))
             *s_dead = 1;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return *s_dead == 0;
 }
 
@@ -412,31 +494,6 @@ __device__ __forceinline__ void row_scan(double x, int tx, double &excl, double 
     }
     total = __shfl(incl, TX - 1, TX);
     excl = incl - x;
-}
-
-// One tile row of the wordline Green's sum: out[jj] = scale sum_j' (min(j,j')+1) v(j')
-// over the whole row (j = j0 + jj), the row being spread over a thread row.
-template <class G>
-__device__ __forceinline__ void row_green_row(const double (&v)[G::TC], int j0, int tx, double scale,
-                                              double (&out)[G::TC])
-{
-    constexpr int TC = G::TC;
-    double L0 = 0.0, L1 = 0.0;
-#pragma unroll
-    for (int jj = 0; jj < TC; ++jj) {
-        L0 += v[jj];
-        L1 += (double)(j0 + jj + 1) * v[jj];
-    }
-    double P0, P1, R, R1;
-    row_scan<G::TX>(L0, tx, P0, R);
-    row_scan<G::TX>(L1, tx, P1, R1);
-#pragma unroll
-    for (int jj = 0; jj < TC; ++jj) {
-        const double jp1 = (double)(j0 + jj + 1);
-        out[jj] = scale * (P1 + jp1 * (R - P0));
-        P0 += v[jj];
-        P1 += jp1 * v[jj];
-    }
 }
 
 // FP32 forms for the wire-solve iterations of the initial guess, which only
@@ -532,21 +589,15 @@ __device__ __forceinline__ bool col_scan_exchange(const Shared &sh, const StripC
     if (STRIP) {
         double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
         if (t < N) {
-            mine[t] = T0;
-            mine[kRowsLd + t] = T1;
+            team_st(mine + t, T0);
+            team_st(mine + kRowsLd + t, T1);
         }
         if (!team_barrier(X, sh.dead, t, nbar++))
             return false;
         if (t < N) {
-            Tall = 0.0;
-            for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.part + (((size_t)((nbar - 1) & 1) * X.S + s) * 2) * kRowsLd;
-                if (s < X.s) {
-                    E0 += ps[t];
-                    E1 += ps[kRowsLd + t];
-                }
-                Tall += ps[kRowsLd + t];
-            }
+            double T0;
+            team_sums(X.part + ((size_t)((nbar - 1) & 1) * X.S * 2) * kRowsLd, 2 * kRowsLd, X.S, X.s, t, E0, E1, T0,
+                      Tall);
         }
     }
     if (t < N) {
@@ -598,62 +649,6 @@ __device__ __forceinline__ bool col_green_tile_f(const Shared &sh, const StripCt
             cl += v[ii][jj] * gi;
             out[ii][jj] = scale * (gi * (A0 + al) + (Tb0 - cl));
         }
-    }
-    __syncthreads();
-    return true;
-}
-
-// Bitline Green's sum in FP64 streamed by tile rows: src(ii, float (&)[TC]) supplies row ii
-// of v, emit(ii, const double (&)[TC]) takes row ii of the result.  The tile
-// rows are not unrolled, so only O(TC) values are live: for the phases whose
-// tile state lives in the workspace.
-template <class G, bool STRIP, class SRC, class EM>
-__device__ __forceinline__ bool col_green_rows(const Shared &sh, const StripCtx &X, SRC src, double scale, int n,
-                                               int gi0, int i0, int j0, int ty, int t, int &nbar, EM emit)
-{
-    constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
-    double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
-    double s0[TC], s1[TC];
-#pragma unroll
-    for (int jj = 0; jj < TC; ++jj)
-        s0[jj] = s1[jj] = 0.0;
-#pragma unroll 1
-    for (int ii = 0; ii < TR; ++ii) {
-        float v[TC];
-        src(ii, v);
-        const double gi = (double)(n - (gi0 + i0 + ii));
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            s0[jj] += (double)v[jj];
-            s1[jj] += (double)v[jj] * gi;
-        }
-    }
-#pragma unroll
-    for (int jj = 0; jj < TC; ++jj) {
-        b0[ty * N + j0 + jj] = s0[jj];
-        b1[ty * N + j0 + jj] = s1[jj];
-    }
-    if (!col_scan_exchange<G, STRIP>(sh, X, t, nbar))
-        return false;
-    double A[TC], C[TC];
-#pragma unroll
-    for (int jj = 0; jj < TC; ++jj) {
-        A[jj] = sh.cext[j0 + jj] + b0[ty * N + j0 + jj];
-        C[jj] = sh.cext[N + j0 + jj] + b1[ty * N + j0 + jj];
-    }
-#pragma unroll 1
-    for (int ii = 0; ii < TR; ++ii) {
-        float v[TC];
-        src(ii, v);
-        const double gi = (double)(n - (gi0 + i0 + ii));
-        double o[TC];
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            A[jj] += (double)v[jj];
-            C[jj] += (double)v[jj] * gi;
-            o[jj] = scale * (gi * A[jj] + (sh.cext[2 * N + j0 + jj] - C[jj]));
-        }
-        emit(ii, o);
     }
     __syncthreads();
     return true;
@@ -770,18 +765,14 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
     if (STRIP) {
         double *mine = X.setup + ((size_t)X.s * 4) * kRowsLd;
         if (t < N) {
-            mine[t] = sh.S1[t];
-            mine[kRowsLd + t] = sh.Ifloor[t];
+            team_st(mine + t, sh.S1[t]);
+            team_st(mine + kRowsLd + t, sh.Ifloor[t]);
         }
         if (!team_barrier(X, sh.dead, t, nbar++))
             return false;
         if (t < N) {
-            double a = 0.0, b = 0.0;
-            for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.setup + ((size_t)s * 4) * kRowsLd;
-                a += ps[t];
-                b += ps[kRowsLd + t];
-            }
+            double e0, e1, a, b;
+            team_sums(X.setup, 4 * kRowsLd, X.S, 0, t, e0, e1, a, b);
             sh.S1[t] = a;
             sh.Ifloor[t] = b;
         }
@@ -930,18 +921,14 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     if (STRIP) {
         double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
         if (t < N) {
-            mine[t] = sh.colD[t];
-            mine[kRowsLd + t] = sh.colI[t];
+            team_st(mine + t, sh.colD[t]);
+            team_st(mine + kRowsLd + t, sh.colI[t]);
         }
         if (!team_barrier(X, sh.dead, t, nbar++))
             return -1.0;
         if (t < N) {
-            double D = 0.0, Y = 0.0;
-            for (int s = 0; s < X.S; ++s) {
-                const double *ps = X.part + (((size_t)((nbar - 1) & 1) * X.S + s) * 2) * kRowsLd;
-                D += ps[t];
-                Y += ps[kRowsLd + t];
-            }
+            double e0, e1, D, Y;
+            team_sums(X.part + ((size_t)((nbar - 1) & 1) * X.S * 2) * kRowsLd, 2 * kRowsLd, X.S, 0, t, e0, e1, D, Y);
             sh.colD[t] = D;
             sh.colI[t] = Y;
         }
@@ -1105,124 +1092,9 @@ __device__ __forceinline__ void tile_store(float *wsb, int i0, int j0, const flo
 // contracts by rho, and the damped one, x <- (1-w) x + w M^-1 (b - K x), by
 // max(|1-w|, |1-w(1+rho)|) -- rho/(2+rho) at w = 2/(2+rho).  Returns false
 // when the team aborted.
-// Streamed forms of the two wire-solve phases, for strips (whose team
-// exchanges leave too few registers for the tile): tile rows go through the
-// workspace -- drops in planes 3, 4, currents in plane 5 -- so only O(TC)
-// values are live, at ~36 bytes of workspace traffic per cell and iteration.
-template <class G, bool STRIP>
-__device__ __forceinline__ bool ir2s_stream(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
-                                            int i0, int j0, int t, int tx, int ty, double rwire, const Shared &sh,
-                                            int &nbar, int iters, double w)
-{
-    constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
-    float *P = wsb + 3 * M * N, *Q = wsb + 4 * M * N, *S = wsb + 5 * M * N;
-    const double w1 = 1.0 - w;
-#pragma unroll 1
-    for (int k = 0; k < iters; ++k) {
-#pragma unroll 1
-        for (int ii = 0; ii < TR; ++ii) {
-            const size_t o = (size_t)(i0 + ii) * N + j0;
-            float gr[TC], pr[TC], qr[TC], sr[TC];
-            ldrow<TC>(gp + o, gr);
-            ldrow<TC>(P + o, pr);
-            ldrow<TC>(Q + o, qr);
-            const double vi = I.vin[gi0 + i0 + ii];
-            double v[TC], out[TC];
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj) {
-                v[jj] = (double)gr[jj] * (((vi - I.vout[j0 + jj]) - (double)pr[jj]) - (double)qr[jj]);
-                sr[jj] = (float)v[jj];
-            }
-            strow<TC>(S + o, sr);
-            row_green_row<G>(v, j0, tx, rwire, out);
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                pr[jj] = (float)(w1 * (double)pr[jj] + w * out[jj]);
-            strow<TC>(P + o, pr);
-        }
-        if (!col_green_rows<G, STRIP>(
-                sh, X, [&](int ii, float (&v)[TC]) { ldrow<TC>(S + (size_t)(i0 + ii) * N + j0, v); }, rwire, I.rows,
-                gi0, i0, j0, ty, t, nbar, [&](int ii, const double (&o)[TC]) {
-                    float qr[TC];
-                    float *q = Q + (size_t)(i0 + ii) * N + j0;
-                    ldrow<TC>(q, qr);
-#pragma unroll
-                    for (int jj = 0; jj < TC; ++jj)
-                        qr[jj] = (float)(w1 * (double)qr[jj] + w * o[jj]);
-                    strow<TC>(q, qr);
-                }))
-            return false;
-    }
-    return true;
-}
-
-template <class G, bool STRIP>
-__device__ __forceinline__ bool green_stream(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
-                                             int i0, int j0, int t, int tx, int ty, double rwire, double hcoef,
-                                             const Shared &sh, int &nbar, int iters, double w)
-{
-    constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
-    float *P = wsb + 3 * M * N, *Q = wsb + 4 * M * N, *S = wsb + 5 * M * N;
-    const float *RP = wsb + 1 * M * N, *RQ = wsb + 2 * M * N;
-    const double ih = 1.0 / hcoef, w1 = 1.0 - w;
-    (void)I;
-#pragma unroll 1
-    for (int k = 0; k < iters; ++k) {
-#pragma unroll 1
-        for (int ii = 0; ii < TR; ++ii) {
-            const size_t o = (size_t)(i0 + ii) * N + j0;
-            float gr[TC], pr[TC], qr[TC], rp[TC], rq[TC], sr[TC];
-            ldrow<TC>(RP + o, rp);
-            ldrow<TC>(RQ + o, rq);
-            if (k > 0) {
-                ldrow<TC>(gp + o, gr);
-                ldrow<TC>(P + o, pr);
-                ldrow<TC>(Q + o, qr);
-            } else {
-#pragma unroll
-                for (int jj = 0; jj < TC; ++jj)
-                    gr[jj] = pr[jj] = qr[jj] = 0.f;
-            }
-            double v[TC], out[TC];
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj) {
-                const double cpl = (double)gr[jj] * ((double)pr[jj] + (double)qr[jj]);
-                v[jj] = (double)rp[jj] * ih - cpl;
-                sr[jj] = (float)((double)rq[jj] * ih - cpl);
-            }
-            strow<TC>(S + o, sr);
-            row_green_row<G>(v, j0, tx, rwire, out);
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                pr[jj] = (float)(w1 * (double)pr[jj] + w * out[jj]);
-            strow<TC>(P + o, pr);
-            if (k == 0) {
-#pragma unroll
-                for (int jj = 0; jj < TC; ++jj)
-                    qr[jj] = 0.f;
-                strow<TC>(Q + o, qr);
-            }
-        }
-        if (!col_green_rows<G, STRIP>(
-                sh, X, [&](int ii, float (&v)[TC]) { ldrow<TC>(S + (size_t)(i0 + ii) * N + j0, v); }, rwire, I.rows,
-                gi0, i0, j0, ty, t, nbar, [&](int ii, const double (&o)[TC]) {
-                    float qr[TC];
-                    float *q = Q + (size_t)(i0 + ii) * N + j0;
-                    ldrow<TC>(q, qr);
-#pragma unroll
-                    for (int jj = 0; jj < TC; ++jj)
-                        qr[jj] = (float)(w1 * (double)qr[jj] + w * o[jj]);
-                    strow<TC>(q, qr);
-                }))
-            return false;
-    }
-    return true;
-}
-
 // The caller's sweep state is dead across the call (it is reloaded from
 // planes 3, 4 afterwards), so the drops stay in VGPRs for all iterations;
-// from_zero starts them at zero instead of reading planes 3, 4 (strips always
-// read them)
+// from_zero starts them at zero instead of reading planes 3, 4
 // (wire_iteration); per iteration only the conductance is read, twice, from
 // the cache.
 template <class G, bool STRIP>
@@ -1231,8 +1103,6 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
                                                      int *nbar, int iters, double w, bool from_zero)
 {
     RB_SH_LOCAL;
-    if constexpr (STRIP)
-        return ir2s_stream<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, sh, *nbar, iters, w);
     constexpr int TC = G::TC;
     float p[G::TR][TC], q[G::TR][TC];
     if (from_zero) {
@@ -1292,8 +1162,6 @@ __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float 
                                                       double hcoef, RB_SH_PARAMS, int *nbar, int iters, double w)
 {
     RB_SH_LOCAL;
-    if constexpr (STRIP)
-        return green_stream<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, sh, *nbar, iters, w);
     constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
     const float *RP = wsb + 1 * M * N, *RQ = wsb + 2 * M * N;
     const float ih = (float)(1.0 / hcoef);
@@ -1378,7 +1246,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
     // planes 3, 4 hold the present drops (no snapshot store needed before a test)
     bool snap_valid = false;
     if (C.ir2s_iters > 0) {
-        const bool from_zero = !STRIP && !I.warm;
+        const bool from_zero = !I.warm;
         if (!from_zero)
             tile_store<G>(wsb, i0, j0, yp, yq);
         if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
