@@ -498,18 +498,50 @@ __device__ __forceinline__ void row_scan(double x, int tx, double &excl, double 
 
 // FP32 forms for the wire-solve iterations of the initial guess, which only
 // need the drops to ~1e-5 relative (the drops are ~1e-3 of the cell voltage,
-// and every result is certified by the FP64 test).
+// and every result is certified by the FP64 test); products are fused
+// (fmaf) here, unlike in the sweeps, whose CPU replays are bit-exact.
+//
+// v from the lane `CTRL` selects (a DPP control), 0 where that lane lies
+// outside the row of 16 or the destination row is masked off by ROW_MASK.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, true));
+}
+
+// Inclusive prefix over the TX lanes of a thread row.  TX = 16, 32 or 64
+// (rows aligned to 16-lane DPP rows): shifts by 1, 2, 4, 8 within each row of
+// 16, then row 15 -> rows 1, 3 and lane 31 -> rows 2, 3 (wave64 row
+// broadcasts); narrower rows use lane shuffles.  The row total comes from the
+// row's last lane.
 template <int TX>
 __device__ __forceinline__ void row_scan_f(float x, int tx, float &excl, float &total)
 {
+    static_assert(TX == 4 || TX == 8 || TX == 16 || TX == 32 || TX == 64, "thread rows of 4 to 64 lanes");
     float incl = x;
+    if constexpr (TX < 16) {
 #pragma unroll
-    for (int d = 1; d < TX; d <<= 1) {
-        const float y = __shfl_up(incl, (unsigned)d, TX);
-        if (tx >= d)
-            incl += y;
+        for (int d = 1; d < TX; d <<= 1) {
+            const float y = __shfl_up(incl, (unsigned)d, TX);
+            if (tx >= d)
+                incl += y;
+        }
+        total = __shfl(incl, TX - 1, TX);
+        excl = incl - x;
+        return;
     }
-    total = __shfl(incl, TX - 1, TX);
+    incl += dpp_f<0x111>(incl); // row_shr:1
+    incl += dpp_f<0x112>(incl); // row_shr:2
+    incl += dpp_f<0x114>(incl); // row_shr:4
+    incl += dpp_f<0x118>(incl); // row_shr:8
+    if constexpr (TX >= 32)
+        incl += dpp_f<0x142, 0xa>(incl); // row_bcast:15
+    if constexpr (TX == 64) {
+        incl += dpp_f<0x143, 0xc>(incl); // row_bcast:31
+        total = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(incl), 63));
+    } else {
+        total = __shfl(incl, TX - 1, TX);
+    }
     excl = incl - x;
 }
 
@@ -522,7 +554,7 @@ __device__ __forceinline__ void row_green_row_f(const float (&v)[G::TC], int j0,
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
         L0 += v[jj];
-        L1 += (float)(j0 + jj + 1) * v[jj];
+        L1 = fmaf((float)(j0 + jj + 1), v[jj], L1);
     }
     float P0, P1, R, R1;
     row_scan_f<G::TX>(L0, tx, P0, R);
@@ -530,9 +562,9 @@ __device__ __forceinline__ void row_green_row_f(const float (&v)[G::TC], int j0,
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
         const float jp1 = (float)(j0 + jj + 1);
-        out[jj] = scale * (P1 + jp1 * (R - P0));
+        out[jj] = scale * fmaf(jp1, R - P0, P1);
         P0 += v[jj];
-        P1 += jp1 * v[jj];
+        P1 = fmaf(jp1, v[jj], P1);
     }
 }
 
@@ -688,7 +720,7 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
 #pragma unroll
             for (int jj = 0; jj < TC; ++jj) {
                 s0[jj] += b[jj];
-                s1[jj] += b[jj] * gi;
+                s1[jj] = fmaf(b[jj], gi, s1[jj]);
             }
         }
 #pragma unroll
@@ -720,14 +752,14 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj) {
             al[jj] += b[jj];
-            cl[jj] += b[jj] * gi;
-            const float o = rf * (gi * (A0[jj] + al[jj]) + (Tb0[jj] - cl[jj]));
-            q[ii][jj] = w1f * q[ii][jj] + wf * o;
+            cl[jj] = fmaf(b[jj], gi, cl[jj]);
+            const float o = rf * fmaf(gi, A0[jj] + al[jj], Tb0[jj] - cl[jj]);
+            q[ii][jj] = fmaf(wf, o, w1f * q[ii][jj]);
         }
         row_green_row_f<G>(a, j0, tx, rf, out);
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj)
-            p[ii][jj] = w1f * p[ii][jj] + wf * out[jj];
+            p[ii][jj] = fmaf(wf, out[jj], w1f * p[ii][jj]);
     }
     __syncthreads();
     return true;
@@ -1182,8 +1214,8 @@ __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float 
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj) {
             const float cpl = gr[jj] * (pr[jj] + qr[jj]);
-            a[jj] = rp[jj] * ih - cpl;
-            b[jj] = rq[jj] * ih - cpl;
+            a[jj] = fmaf(rp[jj], ih, -cpl);
+            b[jj] = fmaf(rq[jj], ih, -cpl);
         }
     };
 #pragma unroll 1
