@@ -307,6 +307,18 @@ __device__ __forceinline__ float team_ld(const float *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// A pair of floats as one 8-byte access.
+__device__ __forceinline__ void team_st2(unsigned long long *p, float lo, float hi)
+{
+    const unsigned long long v = (unsigned long long)__float_as_uint(lo) | ((unsigned long long)__float_as_uint(hi) << 32);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void team_ld2(const unsigned long long *p, float &lo, float &hi)
+{
+    const unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    lo = __uint_as_float((unsigned)v);
+    hi = __uint_as_float((unsigned)(v >> 32));
+}
 // This wave's earlier vector-memory operations have completed (a store: it
 // is visible at device scope); also a compiler barrier.
 __device__ __forceinline__ void team_commit() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -418,6 +430,35 @@ __device__ __forceinline__ void team_sums(const double *base, size_t stride, int
     }
 }
 
+// team_sums over FP32 pairs (lo, hi) published at base + s stride + t, one
+// 8-byte load per strip, accumulated in FP64 in strip order.
+__device__ __forceinline__ void team_sums2(const unsigned long long *base, size_t stride, int S, int below, int t,
+                                           double &E0, double &E1, double &T0, double &T1)
+{
+    constexpr int kBatch = 8;
+    E0 = E1 = T0 = T1 = 0.0;
+#pragma unroll
+    for (int s0 = 0; s0 < kMaxStrips; s0 += kBatch) {
+        if (s0 >= S)
+            break;
+        float a[kBatch], b[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k)
+            team_ld2(base + (size_t)(s0 + k < S ? s0 + k : 0) * stride + t, a[k], b[k]);
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            if (s0 + k < below) {
+                E0 += (double)a[k];
+                E1 += (double)b[k];
+            }
+            if (s0 + k < S) {
+                T0 += (double)a[k];
+                T1 += (double)b[k];
+            }
+        }
+    }
+}
+
 // Team barrier number `nbar` (0-based): every member's team_st stores before
 // it are visible to every member's team_ld loads after it.  Ends with a workgroup barrier; false
 // (uniformly) when the team aborted.
@@ -430,6 +471,24 @@ __device__ __forceinline__ bool team_barrier(const StripCtx &X, int *s_dead, int
         if (!poll_geq(X.cnt, X.S * (nbar + 1), X.abort))
             *s_dead = 1;
     }
+    __syncthreads();
+    return *s_dead == 0;
+}
+
+// The same barrier in two halves, for work that needs no team data in
+// between: team_arrive counts this member in (after its team stores have
+// completed), team_wait waits for the others.
+__device__ __forceinline__ void team_arrive(const StripCtx &X, int t)
+{
+    team_commit();
+    __syncthreads();
+    if (t == 0)
+        __hip_atomic_fetch_add(X.cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool team_wait(const StripCtx &X, int *s_dead, int t, int nbar)
+{
+    if (t == 0 && !poll_geq(X.cnt, X.S * (nbar + 1), X.abort))
+        *s_dead = 1;
     __syncthreads();
     return *s_dead == 0;
 }
@@ -600,8 +659,10 @@ __device__ __forceinline__ void col_reduce2(const Shared &sh, const double (&a)[
 // above this workgroup (strips: over the team, team barrier nbar++) and the
 // crossbar total of sum v (n - i) in cext [3][N].  Returns false when the
 // team aborted.
-template <class G, bool STRIP>
-__device__ __forceinline__ bool col_scan_exchange(const Shared &sh, const StripCtx &X, int t, int &nbar)
+// F32X: the team exchange carries the workgroup totals as FP32 pairs (half
+// the bytes of the FP64 exchange; for the FP32 wire-solve iterations).
+template <class G, bool STRIP, bool F32X = false>
+__device__ __forceinline__ double col_scan_begin(const Shared &sh, const StripCtx &X, int t, int nbar)
 {
     constexpr int TY = G::TY, N = G::N;
     double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
@@ -617,19 +678,37 @@ __device__ __forceinline__ bool col_scan_exchange(const Shared &sh, const StripC
             T1 += x1;
         }
     }
-    double E0 = 0.0, E1 = 0.0, Tall = T1;
     if (STRIP) {
         double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
         if (t < N) {
-            team_st(mine + t, T0);
-            team_st(mine + kRowsLd + t, T1);
+            if (F32X) {
+                team_st2(reinterpret_cast<unsigned long long *>(mine) + t, (float)T0, (float)T1);
+            } else {
+                team_st(mine + t, T0);
+                team_st(mine + kRowsLd + t, T1);
+            }
         }
-        if (!team_barrier(X, sh.dead, t, nbar++))
+        team_arrive(X, t);
+    }
+    return T1;
+}
+
+template <class G, bool STRIP, bool F32X = false>
+__device__ __forceinline__ bool col_scan_end(const Shared &sh, const StripCtx &X, int t, int &nbar, double T1)
+{
+    constexpr int N = G::N;
+    double E0 = 0.0, E1 = 0.0, Tall = T1;
+    if (STRIP) {
+        if (!team_wait(X, sh.dead, t, nbar++))
             return false;
         if (t < N) {
             double T0;
-            team_sums(X.part + ((size_t)((nbar - 1) & 1) * X.S * 2) * kRowsLd, 2 * kRowsLd, X.S, X.s, t, E0, E1, T0,
-                      Tall);
+            const double *base = X.part + ((size_t)((nbar - 1) & 1) * X.S * 2) * kRowsLd;
+            if (F32X)
+                team_sums2(reinterpret_cast<const unsigned long long *>(base), 2 * kRowsLd, X.S, X.s, t, E0, E1, T0,
+                           Tall);
+            else
+                team_sums(base, 2 * kRowsLd, X.S, X.s, t, E0, E1, T0, Tall);
         }
     }
     if (t < N) {
@@ -639,6 +718,15 @@ __device__ __forceinline__ bool col_scan_exchange(const Shared &sh, const StripC
     }
     __syncthreads();
     return true;
+}
+
+// col_scan_begin then col_scan_end: for strips, work placed between the two
+// overlaps the team barrier.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool col_scan_exchange(const Shared &sh, const StripCtx &X, int t, int &nbar)
+{
+    const double T1 = col_scan_begin<G, STRIP>(sh, X, t, nbar);
+    return col_scan_end<G, STRIP>(sh, X, t, nbar, T1);
 }
 
 // Bitline Green's sum of a float tile: out(ii, jj) = scale sum_i' (n -
@@ -729,7 +817,22 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
             b1[ty * N + j0 + jj] = (double)s1[jj];
         }
     }
-    if (!col_scan_exchange<G, STRIP>(sh, X, t, nbar))
+    const double T1 = col_scan_begin<G, STRIP, true>(sh, X, t, nbar);
+    // strips: the wordline half of the iteration runs while the team barrier
+    // completes; the bitline inputs b are kept for the second half
+    float sb[STRIP ? TR : 1][TC];
+    if constexpr (STRIP) {
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            float a[TC], out[TC];
+            cur(ii, p[ii], q[ii], a, sb[ii]);
+            row_green_row_f<G>(a, j0, tx, rf, out);
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                p[ii][jj] = fmaf(wf, out[jj], w1f * p[ii][jj]);
+        }
+    }
+    if (!col_scan_end<G, STRIP, true>(sh, X, t, nbar, T1))
         return false;
     // anchors of this thread row in FP64 -- the bitline current entering it
     // (sum over the rows above) and the remaining weighted sum below it --
@@ -747,7 +850,13 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
         float a[TC], b[TC], out[TC];
-        cur(ii, p[ii], q[ii], a, b);
+        if constexpr (STRIP) {
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                b[jj] = sb[ii][jj];
+        } else {
+            cur(ii, p[ii], q[ii], a, b);
+        }
         const float gi = (float)(n - (gi0 + i0 + ii));
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj) {
@@ -756,10 +865,12 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
             const float o = rf * fmaf(gi, A0[jj] + al[jj], Tb0[jj] - cl[jj]);
             q[ii][jj] = fmaf(wf, o, w1f * q[ii][jj]);
         }
-        row_green_row_f<G>(a, j0, tx, rf, out);
+        if constexpr (!STRIP) {
+            row_green_row_f<G>(a, j0, tx, rf, out);
 #pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            p[ii][jj] = fmaf(wf, out[jj], w1f * p[ii][jj]);
+            for (int jj = 0; jj < TC; ++jj)
+                p[ii][jj] = fmaf(wf, out[jj], w1f * p[ii][jj]);
+        }
     }
     __syncthreads();
     return true;
