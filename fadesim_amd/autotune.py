@@ -27,8 +27,8 @@ import torch
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 KAPPAS = (0.55, 0.7, 0.83, 1.0, 1.25)
 CHECKS = ((32, 16, 256), (64, 32, 256), (32, 32, 512))
-IR2S = (0, 2, 4, 6, 8, 12, 16)   # iterations on the initial guess (and the correction's)
-IR2S_OMEGA = (1.0, 0.85, 0.7)    # their damping: 2/(2+rho) for coupling radius rho
+IR2S = (0, 2, 4, 6, 8, 10, 12, 14, 16)   # iterations on the initial guess (and the correction's)
+IR2S_OMEGA = (1.0, 0.85, 0.75, 0.7, 0.6)  # their damping: 2/(2+rho) for coupling radius rho
 
 
 def arch_name(device) -> str:
@@ -99,7 +99,7 @@ def tune_size(n: int, count: int, tol: float, dev, log=print, reps: int = 2) -> 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--per-size", default="256,128,64,16",
+    ap.add_argument("--per-size", default="256,128,64,64",
                     help="instances per size 64,128,256,512 in the timing sample")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--out", default=None)
