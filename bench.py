@@ -235,9 +235,10 @@ def main():
     }
     if net is not None:
         # the network's field, one IR2S iteration on it, then the same test / SOR
-        variants["nn"] = (1, lambda timing=False: solve_batch(tg, tr, tv, warm=net.predict(tg, tr, tv), tol=args.tol,
-                                                              ir2s_iters=1, corr_iters=K, test_first=True,
-                                                              timing=timing))
+        # (the network output is combined with the base field while the solve loads it)
+        variants["nn"] = (1, lambda timing=False: solve_batch(tg, tr, tv, **net.warm_kwargs(tg, tr, tv),
+                                                              tol=args.tol, ir2s_iters=1, corr_iters=K,
+                                                              test_first=True, timing=timing))
     # the 4-level full-resolution encoder-decoder (~340k parameters), for comparison
     net16 = None
     if os.path.exists(os.path.join(warmstart.WEIGHTS_DIR, "warmstart_128_unet16.bin")):

@@ -29,6 +29,7 @@ class BatchDesc(ctypes.Structure):
         ("flags", ctypes.c_uint),
         ("d_iout", _vp), ("d_sweeps", _vp), ("d_status", _vp), ("d_est", _vp), ("d_field", _vp),
         ("ir2s_iters", ctypes.c_int), ("corr_iters", ctypes.c_int), ("ir2s_omega", ctypes.c_double),
+        ("d_warm_coarse", _vp), ("d_warm_scale", _vp), ("warm_coarse_ch", ctypes.c_int), ("warm_level", ctypes.c_int),
     ]
 
 

@@ -105,6 +105,9 @@ __device__ __forceinline__ rb::Inst make_inst(const RaggedArgs &A, int k, int n)
     I.status = A.status ? A.status + k : nullptr;
     I.est = A.est ? A.est + k : nullptr;
     I.rows = n;
+    I.coarse = nullptr;
+    I.cscale = 0.f;
+    I.cch = I.clevel = 0;
     return I;
 }
 

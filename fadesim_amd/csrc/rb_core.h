@@ -94,6 +94,11 @@ struct Inst {
     int *status;
     double *est;
     int rows;
+    // optional: warm = base + cscale * bilinear(coarse[.][.][0..1]), the
+    // coarse grid 2^clevel times coarser (rbsor only; null elsewhere)
+    const __bf16 *coarse;
+    float cscale;
+    int cch, clevel;
 };
 
 // Team of S workgroups solving one crossbar, strip s = rows [s*M, (s+1)*M).
@@ -250,6 +255,56 @@ __device__ __forceinline__ void publish_all(const float (&yp)[G::TR][G::TC],
         halo[G::QT + k * QLD + TX + t] = yq[0][k];
         halo[G::QB + k * QLD + TX + t] = yq[G::TR - 1][k];
     }
+}
+
+// Adds scale x bilinear(coarse) to the loaded warm-start drops: the network
+// output of a grid 2^level times coarser than the crossbar (channel 0 to the
+// wordline drops, 1 to the bitline drops; an even channel count, so the pair
+// is one 4-byte load), staged as FP32 pairs in `stage` (LDS, 2 (rows >>
+// level) (N >> level) floats).  Sample point ((i + 1/2)/f - 1/2,
+// (j + 1/2)/f - 1/2) clamped to the coarse grid, the arithmetic of
+// fadesim_nn_combine_up term by term, so the drops equal that kernel's field
+// bit for bit.  Ends with a workgroup barrier (the stage is free again).
+template <class G>
+__device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, int gi0, int i0, int j0,
+                                           float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC])
+{
+    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    const int lvl = I.clevel, f = 1 << lvl, Hc = I.rows >> lvl, Wc = N >> lvl, HW = Hc * Wc;
+    __syncthreads();
+    const unsigned *src = reinterpret_cast<const unsigned *>(I.coarse);
+    float2 *st2 = reinterpret_cast<float2 *>(stage);
+    for (int k = t; k < HW; k += G::NT) {
+        const unsigned v = src[(size_t)k * (I.cch >> 1)];
+        st2[k] = make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
+    }
+    __syncthreads();
+    const float inv = 1.0f / (float)f, s = I.cscale;
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        const int i = gi0 + i0 + ii;
+        const float y = fminf(fmaxf(((float)i + 0.5f) * inv - 0.5f, 0.0f), (float)(Hc - 1));
+        const int y0 = (int)y, y1 = min(y0 + 1, Hc - 1);
+        const float wy = y - (float)y0;
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            const int j = j0 + jj;
+            const float x = fminf(fmaxf(((float)j + 0.5f) * inv - 0.5f, 0.0f), (float)(Wc - 1));
+            const int x0 = (int)x, x1 = min(x0 + 1, Wc - 1);
+            const float wx = x - (float)x0;
+            const float2 a00 = st2[y0 * Wc + x0], a01 = st2[y0 * Wc + x1];
+            const float2 a10 = st2[y1 * Wc + x0], a11 = st2[y1 * Wc + x1];
+            {
+                const float top = a00.x + wx * (a01.x - a00.x), bot = a10.x + wx * (a11.x - a10.x);
+                yp[ii][jj] = yp[ii][jj] + s * (top + wy * (bot - top));
+            }
+            {
+                const float top = a00.y + wx * (a01.y - a00.y), bot = a10.y + wx * (a11.y - a10.y);
+                yq[ii][jj] = yq[ii][jj] + s * (top + wy * (bot - top));
+            }
+        }
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------- strip exchange
@@ -1220,6 +1275,28 @@ __device__ __forceinline__ void tile_store(float *wsb, int i0, int j0, const flo
     }
 }
 
+// Warm start combined at load (Inst::coarse): planes 3, 4 = base + scale x
+// bilinear(coarse) over this thread's tile, out of line so that its
+// registers do not weigh on the sweep loop.
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) void phase_coarse(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                       int i0, int j0, int t, RB_SH_PARAMS)
+{
+    RB_SH_LOCAL;
+    (void)X;
+    (void)gp;
+    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    const size_t MNall = (size_t)I.rows * N;
+    float yp[TR][TC], yq[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii) {
+        ldrow<TC>(I.warm + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
+        ldrow<TC>(I.warm + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
+    }
+    add_coarse<G>(I, sh.halo, t, gi0, i0, j0, yp, yq);
+    tile_store<G>(wsb, i0, j0, yp, yq);
+}
+
 // Wire-solve (IR2S) iterations on the drops, in place on the drops held in planes 3, 4:
 //   s_ij = g_ij (c_ij - p_ij - q_ij)            (cell currents of the drops)
 //   p_ij <- r sum_j' (min(j,j')+1) s_ij'       (wordline: source segment + wires)
@@ -1375,22 +1452,30 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
         return false;
 
     float yp[TR][TC], yq[TR][TC];
+    const bool coarse = !STRIP && I.coarse;
+    if (coarse) {
+        // the warm start combined while loading, into planes 3, 4
+        phase_coarse<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
+        if (C.ir2s_iters == 0)
+            tile_load<G>(wsb, i0, j0, yp, yq);
+    } else {
 #pragma unroll
-    for (int ii = 0; ii < TR; ++ii) {
-        if (I.warm) {
-            ldrow<TC>(I.warm + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
-            ldrow<TC>(I.warm + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
-        } else {
+        for (int ii = 0; ii < TR; ++ii) {
+            if (I.warm) {
+                ldrow<TC>(I.warm + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
+                ldrow<TC>(I.warm + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
+            } else {
 #pragma unroll
-            for (int jj = 0; jj < TC; ++jj)
-                yp[ii][jj] = yq[ii][jj] = 0.f;
+                for (int jj = 0; jj < TC; ++jj)
+                    yp[ii][jj] = yq[ii][jj] = 0.f;
+            }
         }
     }
     // planes 3, 4 hold the present drops (no snapshot store needed before a test)
     bool snap_valid = false;
     if (C.ir2s_iters > 0) {
         const bool from_zero = !I.warm;
-        if (!from_zero)
+        if (!from_zero && !coarse)
             tile_store<G>(wsb, i0, j0, yp, yq);
         if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
                                   C.ir2s_omega, from_zero))

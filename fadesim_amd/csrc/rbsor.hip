@@ -24,7 +24,10 @@ struct RbArgs {
     const double *vin;
     const double *vout; // never null (zeros supplied by the host)
     const float *warm;
-    float *ws;         // per workgroup slot [7][M][N]
+    const __bf16 *coarse; // optional network output added to warm at load
+    const float *cscale;
+    int cch, clevel;
+    float *ws;         // per workgroup slot [kWsPlanes][M][N]
     rb::Ctl ctl;
     double *iout;
     int *sweeps;
@@ -80,6 +83,10 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         I.status = A.status ? A.status + b : nullptr;
         I.est = A.est ? A.est + b : nullptr;
         I.rows = M;
+        I.coarse = A.coarse ? A.coarse + (size_t)b * (M >> A.clevel) * (N >> A.clevel) * A.cch : nullptr;
+        I.cscale = A.coarse ? A.cscale[b] : 0.f;
+        I.cch = A.cch;
+        I.clevel = A.clevel;
         rb::solve<G, false>(I, none, A.ctl, wsb, sh);
     }
 }
@@ -109,8 +116,9 @@ int resident_slots()
 template <class G>
 int launch_rb(RbArgs args, int M, int N, hipStream_t s)
 {
-    (void)M;
-    (void)N;
+    // the coarse warm-start grid is staged in the halo planes' LDS
+    if (args.coarse && 2 * (M >> args.clevel) * (N >> args.clevel) > G::HALO)
+        return set_error(FADESIM_E_UNSUPPORTED, "d_warm_coarse: coarse grid too large for this shape (raise warm_level)");
     int grid = resident_slots<G>();
     if (grid > args.batch)
         grid = args.batch;
@@ -193,6 +201,13 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
                                                 std::to_string(FADESIM_MAX_IR2S_ITERS) + "]");
     if (!(d->ir2s_omega >= 0.0 && d->ir2s_omega <= 1.0))
         return set_error(FADESIM_E_INVALID, "ir2s_omega must lie in (0, 1] (0 selects 1)");
+    if (d->d_warm_coarse) {
+        const int L = d->warm_level;
+        if (!d->d_warm || !d->d_warm_scale || d->warm_coarse_ch < 2 || (d->warm_coarse_ch & 1) || L < 1 || L > 6 || ((M >> L) << L) != M ||
+            ((N >> L) << L) != N || (M >> L) < 1 || (N >> L) < 1)
+            return set_error(FADESIM_E_INVALID, "d_warm_coarse needs d_warm, d_warm_scale, an even warm_coarse_ch "
+                                                ">= 2 and warm_level in 1..6 dividing rows and cols");
+    }
     hipStream_t s = (hipStream_t)stream;
 
     const double *vout = d->d_vout;
@@ -214,6 +229,10 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     args.vin = d->d_vin;
     args.vout = vout;
     args.warm = d->d_warm;
+    args.coarse = (const __bf16 *)d->d_warm_coarse;
+    args.cscale = d->d_warm_scale;
+    args.cch = d->warm_coarse_ch;
+    args.clevel = d->warm_level;
     args.ws = nullptr;
     args.ctl.omega = (float)w;
     args.ctl.tol_eff = d->tol / safety;

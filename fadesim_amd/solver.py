@@ -56,7 +56,9 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
                 zero_correction: bool = False, ir2s_iters: int = 0, corr_iters: int = 0,
                 ir2s_omega: float = 1.0, test_first: Optional[bool] = None,
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
-                out: Optional[torch.Tensor] = None, timing: bool = False) -> SolveResult:
+                out: Optional[torch.Tensor] = None, timing: bool = False,
+                warm_coarse: Optional[torch.Tensor] = None, warm_scale: Optional[torch.Tensor] = None,
+                warm_level: int = 2) -> SolveResult:
     """Solve B crossbars of one shape.
 
     g [B, M, N] float32 (S) -- or [S, M, N] conductance slots with ``gidx`` [B]
@@ -72,6 +74,10 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     ``corr_iters`` sets the iterations of the refinement correction's guess
     (0: 2), ``ir2s_omega`` their damping.  ``test_first`` tests the initial guess before the first sweep
     (default: when ir2s_iters >= 3).
+    ``warm_coarse`` [B, M >> warm_level, N >> warm_level, C] bf16 with
+    ``warm_scale`` [B]: the initial drops are warm + warm_scale x
+    bilinear(warm_coarse[..., 0:2]), formed while loading (the arithmetic of
+    fadesim_nn_combine_up; WarmStartNet.warm_kwargs builds these).
     """
     if g.dim() != 3:
         raise ValueError("g must be [B, M, N]")
@@ -95,6 +101,16 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
         vout = vout.to(device=dev, dtype=torch.float64).reshape(B, N).contiguous()
     if warm is not None:
         warm = warm.to(device=dev, dtype=torch.float32).reshape(B, 2, M, N).contiguous()
+    if warm_coarse is not None:
+        L = int(warm_level)
+        if warm is None or warm_scale is None:
+            raise ValueError("warm_coarse needs warm (the base field) and warm_scale")
+        if warm_coarse.dtype != torch.bfloat16 or warm_coarse.dim() != 4 or \
+                tuple(warm_coarse.shape[:3]) != (B, M >> L, N >> L) or warm_coarse.shape[3] < 2 or \
+                warm_coarse.shape[3] % 2:
+            raise ValueError(f"warm_coarse must be bf16 [B, M >> {L}, N >> {L}, C] with C even, >= 2")
+        warm_coarse = warm_coarse.to(dev).contiguous()
+        warm_scale = warm_scale.to(device=dev, dtype=torch.float32).reshape(B).contiguous()
     iout = out if out is not None else torch.empty((B, N), dtype=torch.float64, device=dev)
     if iout.shape != (B, N) or iout.dtype != torch.float64 or not iout.is_contiguous():
         raise ValueError("out must be a contiguous float64 [B, N] tensor")
@@ -132,6 +148,9 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     d.d_iout, d.d_sweeps, d.d_status, d.d_est = iout.data_ptr(), sweeps.data_ptr(), status.data_ptr(), est.data_ptr()
     d.d_field = _ptr(field)
     d.ir2s_iters, d.corr_iters, d.ir2s_omega = int(ir2s_iters), int(corr_iters), float(ir2s_omega)
+    if warm_coarse is not None:
+        d.d_warm_coarse, d.d_warm_scale = warm_coarse.data_ptr(), warm_scale.data_ptr()
+        d.warm_coarse_ch, d.warm_level = int(warm_coarse.shape[3]), int(warm_level)
     with torch.cuda.device(dev):
         rc = nat.lib().fadesim_solve_batch(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
     nat.check(rc, "fadesim_solve_batch")

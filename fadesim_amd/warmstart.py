@@ -244,15 +244,18 @@ class WarmStartNet:
         return net
 
     # ------------------------------------------------------------------ features
-    def features(self, g, r, vin):
+    def features(self, g, r, vin, base=None, scale=None):
         """Input features at the network's input level, the second-order base
-        field [B][2][M][N] and the per-instance output scale."""
+        field [B][2][M][N] and the per-instance output scale (written into
+        ``base`` / ``scale`` when given)."""
         B = g.shape[0]
         lv = self.in_level
         H, W = self._hw(lv)
         feat = torch.empty((B, H, W, N_FEAT), dtype=torch.bfloat16, device=self.dev)
-        base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
-        scale = torch.empty(B, dtype=torch.float32, device=self.dev)
+        if base is None:
+            base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
+        if scale is None:
+            scale = torch.empty(B, dtype=torch.float32, device=self.dev)
         ptrs = [ctypes.c_void_p(x.data_ptr()) for x in (g, r, vin, feat, base, scale)]
         if lv == 0:
             _chk(nat.lib().fadesim_nn_features(B, self.M, self.N, *ptrs, _s(self.dev)), "nn_features")
@@ -303,11 +306,12 @@ class WarmStartNet:
         """coarse32f at 128x128 runs as one kernel (fadesim_nn_coarse_forward)."""
         return self.arch == "coarse32f" and (self.M, self.N) == (128, 128)
 
-    def forward_fused(self, x):
+    def forward_fused(self, x, out=None):
         """x [B][32][32][8] bf16 -> out [B][32][32][16] bf16, the whole network
         in one launch (every activation of a crossbar in LDS)."""
         B = x.shape[0]
-        out = torch.empty((B, 32, 32, OUT_C), dtype=torch.bfloat16, device=self.dev)
+        if out is None:
+            out = torch.empty((B, 32, 32, OUT_C), dtype=torch.bfloat16, device=self.dev)
         byname = {op.name: op for op in self.ops}
         if getattr(self, "_frag_ver", None) != self._wver:
             # fragment-major weights [Kp/32][Cout][32]: a wave's fragment is 1 KB contiguous
@@ -353,6 +357,42 @@ class WarmStartNet:
                                                      ctypes.c_void_p(warm[b0:b1].data_ptr()), _s(self.dev)),
                      "nn_combine_up")
         return warm
+
+    def predict_parts(self, g, r, vin, chunk: int = 2048):
+        """The warm start before its final combination: base field [B][2][M][N]
+        FP32, network output [B][M >> level][N >> level][16] BF16, scale [B]
+        FP32 and the output level (>= 1), for solve_batch(warm=base,
+        warm_coarse=out, warm_scale=scale, warm_level=level), which forms
+        base + scale x bilinear(out) while loading -- the same field predict()
+        returns, without writing it."""
+        lv = self.out_level
+        if lv < 1:
+            raise ValueError("predict_parts needs a network whose output is coarser than the crossbar")
+        B = g.shape[0]
+        g = g.to(torch.float32).contiguous()
+        r = r.to(torch.float64).reshape(-1).expand(B).contiguous()
+        vin = vin.to(torch.float64).contiguous()
+        base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
+        scale = torch.empty(B, dtype=torch.float32, device=self.dev)
+        H, W = self._hw(lv)
+        out = torch.empty((B, H, W, OUT_C), dtype=torch.bfloat16, device=self.dev)
+        for b0 in range(0, B, chunk):
+            b1 = min(B, b0 + chunk)
+            feat, _, _ = self.features(g[b0:b1], r[b0:b1], vin[b0:b1], base[b0:b1], scale[b0:b1])
+            if self.fused:
+                self.forward_fused(feat, out=out[b0:b1])
+            else:
+                out[b0:b1] = self.forward(feat)
+        return base, out, scale, lv
+
+    def warm_kwargs(self, g, r, vin):
+        """solve_batch keyword arguments carrying this network's warm start:
+        the parts (combined while loading) when the output is coarser than the
+        crossbar, else the combined field."""
+        if self.out_level >= 1:
+            base, out, scale, lv = self.predict_parts(g, r, vin)
+            return {"warm": base, "warm_coarse": out, "warm_scale": scale, "warm_level": lv}
+        return {"warm": self.predict(g, r, vin)}
 
     # ------------------------------------------------------------------ backward
     def backward(self, acts, dout, grad):

@@ -274,3 +274,47 @@ def test_fused_coarse_network_equals_layered(cuda):
     torch.cuda.synchronize()
     assert a.shape == b.shape == (37, 32, 32, 16)
     assert torch.equal(a, b), float((a.float() - b.float()).abs().max())
+
+
+@pytest.mark.parametrize("arch", ["coarse32f", "coarse32"])
+def test_warm_start_combined_at_load_equals_combined_field(cuda, arch):
+    """solve_batch(warm=base, warm_coarse=out, warm_scale=s) forms the warm
+    start while loading with the arithmetic of fadesim_nn_combine_up: the
+    initial field (zero sweeps) is that kernel's field bit for bit, and a
+    converged solve from it is the same solve."""
+    import torch
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    net = warmstart.WarmStartNet(cuda, arch=arch, seed=11)
+    g, r, vin = warmstart.make_instances(96, seed=20_000_000, device=cuda)
+    warm = net.predict(g, r, vin)
+    kw = net.warm_kwargs(g, r, vin)
+    assert "warm_coarse" in kw and kw["warm_level"] == net.out_level >= 1
+    f0 = solve_batch(g, r, vin, **kw, fixed_sweeps=0, want_field=True)
+    torch.cuda.synchronize()
+    assert torch.equal(f0.field, warm)
+    a = solve_batch(g, r, vin, warm=warm, tol=1e-6, ir2s_iters=1, corr_iters=4, test_first=True, want_field=True)
+    b = solve_batch(g, r, vin, **kw, tol=1e-6, ir2s_iters=1, corr_iters=4, test_first=True, want_field=True)
+    torch.cuda.synchronize()
+    assert int((b.status != 0).sum()) == 0
+    assert torch.equal(a.sweeps, b.sweeps)
+    assert torch.equal(a.iout, b.iout)
+    assert torch.equal(a.field, b.field)
+
+
+def test_warm_coarse_arguments_checked(cuda):
+    import torch
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    g, r, vin = warmstart.make_instances(4, seed=5, device=cuda)
+    base = torch.zeros((4, 2, 128, 128), dtype=torch.float32, device=cuda)
+    out = torch.zeros((4, 32, 32, 16), dtype=torch.bfloat16, device=cuda)
+    s = torch.ones(4, dtype=torch.float32, device=cuda)
+    with pytest.raises(ValueError):
+        solve_batch(g, r, vin, warm=None, warm_coarse=out, warm_scale=s, warm_level=2)
+    with pytest.raises(ValueError):
+        solve_batch(g, r, vin, warm=base, warm_coarse=out, warm_scale=s, warm_level=1)
+    with pytest.raises(RuntimeError):
+        # a full-resolution "coarse" grid does not fit the kernel's LDS staging
+        solve_batch(g, r, vin, warm=base, warm_coarse=torch.zeros((4, 128, 128, 2), dtype=torch.bfloat16,
+                                                                   device=cuda), warm_scale=s, warm_level=0)
