@@ -12,16 +12,25 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libfadesim.so")
 
 FADESIM_OK = 0
-ST_CONVERGED, ST_MAXIT, ST_NONFINITE = 0, 1, 2
+ST_CONVERGED, ST_MAXIT, ST_NONFINITE, ST_ABORTED = 0, 1, 2, 3
 F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING, F_ZERO_CORRECTION, F_TEST_FIRST = 1, 2, 4, 8, 16, 32
 MAX_IR2S_ITERS = 64
 
 _vp = ctypes.c_void_p
 
 
-class BatchDesc(ctypes.Structure):
+class _SizedDesc(ctypes.Structure):
+    """A descriptor whose leading ``struct_size`` is filled in at creation."""
+
+    def __init__(self, *a, **kw):
+        super().__init__(*a, **kw)
+        self.struct_size = ctypes.sizeof(self)
+
+
+class BatchDesc(_SizedDesc):
     """Mirror of ``fadesim_batch_desc``."""
     _fields_ = [
+        ("struct_size", ctypes.c_size_t),
         ("batch", ctypes.c_int), ("rows", ctypes.c_int), ("cols", ctypes.c_int),
         ("d_g", _vp), ("d_gidx", _vp), ("d_r_wire", _vp), ("d_vin", _vp), ("d_vout", _vp), ("d_warm", _vp),
         ("omega", ctypes.c_double), ("tol", ctypes.c_double), ("safety", ctypes.c_double),
@@ -33,9 +42,10 @@ class BatchDesc(ctypes.Structure):
     ]
 
 
-class Gs64Desc(ctypes.Structure):
+class Gs64Desc(_SizedDesc):
     """Mirror of ``fadesim_gs64_desc``."""
     _fields_ = [
+        ("struct_size", ctypes.c_size_t),
         ("rows", ctypes.c_int), ("cols", ctypes.c_int),
         ("d_g", _vp), ("d_ru", _vp), ("ru_ld", ctypes.c_int), ("d_rd", _vp), ("rd_ld", ctypes.c_int),
         ("d_vin", _vp), ("d_vout", _vp),
@@ -44,7 +54,11 @@ class Gs64Desc(ctypes.Structure):
         ("d_vup", _vp), ("d_vdown", _vp), ("d_iout", _vp), ("d_record", _vp), ("d_sweeps", _vp),
         ("general", ctypes.c_int), ("d_on_top", _vp), ("d_on_left", _vp), ("d_on_bottom", _vp),
         ("d_on_right", _vp), ("d_v_top", _vp), ("d_v_right", _vp),
+        ("order", ctypes.c_int), ("d_init_up", _vp), ("d_init_down", _vp),
     ]
+
+
+GS64_LEXICOGRAPHIC, GS64_REDBLACK = 0, 1
 
 
 class HookDesc(ctypes.Structure):

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", os.path.join(REPO, "include"), "-I", CSRC,
 ]
 
-LIB_SOURCES = ["common.cpp", "rbsor.hip", "gs64.hip", "hook.hip", "nn.hip", "cells.hip", "ragged.hip", "ir2s.hip"]
+LIB_SOURCES = ["common.cpp", "rbsor.hip", "gs64.hip", "gslex.hip", "hook.hip", "nn.hip", "cells.hip", "ragged.hip", "ir2s.hip"]
 CLI_SOURCES = ["sim_main.cpp"]
 
 

@@ -1,6 +1,7 @@
-// FP64 red-black SOR for one crossbar with per-segment wire resistances and
-// the reference stopping rule -- the solver behind the `sim` CLI's
-// nodebasedGSMethod command.
+// fadesim_gs64_solve: one crossbar in FP64 with per-segment wire resistances
+// and the reference stopping rule -- the solver behind the `sim` CLI's
+// nodebasedGSMethod command.  The reference's lexicographic order is
+// gslex.hip; this file holds the entry point and the red-black alternative:
 //
 // Same network and update as NodebasedGSMethod::nodeBaseGS
 // (crossbarHspice/V2/concrete_command.h:1218-1273) in absolute node voltages,
@@ -29,6 +30,7 @@
 
 #include "common.h"
 #include "fadesim.h"
+#include "gslex.h"
 
 namespace fadesim {
 namespace {
@@ -43,6 +45,7 @@ struct GsArgs {
     int general;                              // general-boundary network (nodeBasedGeneralGS)
     const int *on_top, *on_left, *on_bottom, *on_right;
     const double *vtop, *vright;
+    const double *init_u, *init_d;
     int iter_number;
     double w;
     int enable_break;
@@ -72,8 +75,13 @@ __global__ void __launch_bounds__(kGsThreads) gs64_kernel(GsArgs A)
         bD[k] = (i < n - 1 || onB) ? 1.0 / A.rd[(size_t)j * A.rd_ld + i + 1] : 0.0;
         wDu[k] = A.w / ((aL[k] + aR[k]) + A.g[k]);
         wDd[k] = A.w / ((bU[k] + bD[k]) + A.g[k]);
-        u[k] = gen ? (onL ? A.vin[i] : 0.0) : A.vin[i];
-        d[k] = gen ? 0.0 : A.vout[j];
+        if (A.init_u) {
+            u[k] = A.init_u[k];
+            d[k] = A.init_d[k];
+        } else {
+            u[k] = gen ? (onL ? A.vin[i] : 0.0) : A.vin[i];
+            d[k] = gen ? 0.0 : A.vout[j];
+        }
     }
     __syncthreads();
     const double omw = 1.0 - A.w;
@@ -150,6 +158,14 @@ extern "C" int fadesim_gs64_solve(const fadesim_gs64_desc *d, void *stream)
 {
     if (!d)
         return set_error(FADESIM_E_INVALID, "null descriptor");
+    if (d->struct_size != sizeof(fadesim_gs64_desc))
+        return set_error(FADESIM_E_INVALID, "fadesim_gs64_desc.struct_size is " + std::to_string(d->struct_size) +
+                                                ", the library's descriptor has " +
+                                                std::to_string(sizeof(fadesim_gs64_desc)) + " bytes");
+    if (d->order != FADESIM_GS64_LEXICOGRAPHIC && d->order != FADESIM_GS64_REDBLACK)
+        return set_error(FADESIM_E_INVALID, "order must be FADESIM_GS64_LEXICOGRAPHIC or FADESIM_GS64_REDBLACK");
+    if ((d->d_init_up == nullptr) != (d->d_init_down == nullptr))
+        return set_error(FADESIM_E_INVALID, "d_init_up and d_init_down are given together or not at all");
     if (!(d->rows >= 2 && d->cols >= 2))
         return set_error(FADESIM_E_INVALID, "m and n should greater than 2");
     if (!d->d_g || !d->d_ru || !d->d_rd || !d->d_vin || !d->d_vout || !d->d_vup || !d->d_vdown ||
@@ -163,6 +179,12 @@ extern "C" int fadesim_gs64_solve(const fadesim_gs64_desc *d, void *stream)
                        !d->d_v_right))
         return set_error(FADESIM_E_INVALID, "the general boundary needs all edge flags and voltages");
     const size_t nm = (size_t)d->rows * d->cols;
+    if (d->order == FADESIM_GS64_LEXICOGRAPHIC) {
+        void *lws = workspace(gslex_workspace_bytes(d->rows, d->cols));
+        if (!lws)
+            return FADESIM_E_NOMEM;
+        return gslex_solve(d, lws, (hipStream_t)stream);
+    }
     double *ws = (double *)workspace(8 * nm * sizeof(double));
     if (!ws)
         return FADESIM_E_NOMEM;
@@ -183,6 +205,8 @@ extern "C" int fadesim_gs64_solve(const fadesim_gs64_desc *d, void *stream)
     a.on_right = d->d_on_right;
     a.vtop = d->d_v_top;
     a.vright = d->d_v_right;
+    a.init_u = d->d_init_up;
+    a.init_d = d->d_init_down;
     a.iter_number = d->iter_number;
     a.w = d->omega;
     a.enable_break = d->enable_break;

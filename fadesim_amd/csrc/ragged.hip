@@ -113,7 +113,7 @@ __device__ __forceinline__ rb::Inst make_inst(const RaggedArgs &A, int k, int n)
 
 __global__ void __launch_bounds__(kThreads) ragged_kernel(RaggedArgs A)
 {
-    __shared__ float halo[kHaloMax];
+    __shared__ __align__(16) float halo[kHaloMax];
     __shared__ double S1[kNmax];
     __shared__ double Ifloor[kNmax];
     __shared__ double colI[kNmax];
@@ -551,6 +551,13 @@ extern "C" int fadesim_ragged_solve(const fadesim_ragged_plan *P, const fadesim_
     hipLaunchKernelGGL(fill_i32, dim3((unsigned)std::min<size_t>((P->n_ints + 255) / 256, 4096)), dim3(256), 0, s,
                        P->d_ints, P->n_ints, 0);
     FADESIM_HIP_CHECK(hipGetLastError());
+    // every status starts as "aborted": an instance the launch never finishes
+    // (a team wait that timed out) keeps it, whatever an earlier call left there
+    if (d->d_status) {
+        hipLaunchKernelGGL(fill_i32, dim3((unsigned)std::min<size_t>(((size_t)P->count + 255) / 256, 4096)),
+                           dim3(256), 0, s, d->d_status, (size_t)P->count, FADESIM_ST_ABORTED);
+        FADESIM_HIP_CHECK(hipGetLastError());
+    }
     auto *PP = const_cast<fadesim_ragged_plan *>(P);
     const bool timing = (d->flags & FADESIM_F_TIMING) != 0;
     if (timing)

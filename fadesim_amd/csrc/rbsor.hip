@@ -50,7 +50,7 @@ template <class G>
 __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
 {
     constexpr int M = G::M, N = G::N;
-    __shared__ float halo[G::HALO];
+    __shared__ __align__(16) float halo[G::HALO]; // also read as float2 (the coarse warm-start grid)
     __shared__ double S1[N];
     __shared__ double Ifloor[N];
     __shared__ double colI[N];
@@ -177,6 +177,10 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
 {
     if (!d)
         return set_error(FADESIM_E_INVALID, "null descriptor");
+    if (d->struct_size != sizeof(fadesim_batch_desc))
+        return set_error(FADESIM_E_INVALID, "fadesim_batch_desc.struct_size is " + std::to_string(d->struct_size) +
+                                                ", the library's descriptor has " +
+                                                std::to_string(sizeof(fadesim_batch_desc)) + " bytes");
     const int B = d->batch, M = d->rows, N = d->cols;
     if (B < 0)
         return set_error(FADESIM_E_INVALID, "batch must be >= 0");

@@ -152,6 +152,168 @@ bool general_boundary(const Array2D &a)
     return false;
 }
 
+// Command-line options: -v1 selects the V1 CLI's semantics
+// (crossbarHspice/2dArray.cc: omega parsed with stof, wordline segments from
+// RULine, mode bit 0); the default is V2's (concrete_command.h: stod, the
+// wordline segments of a square array read from RDLine, :1214).  The
+// environment variable FADESIM_GS64_REDBLACK=1 solves with red-black order
+// instead of the reference's lexicographic one.
+bool g_v1 = false;
+
+struct GsRun {
+    std::vector<double> vup, vdown, iout;
+    int record = -1, sweeps = 0;
+    double seconds = 0;
+    bool redblack = false;
+};
+
+// One fadesim_gs64_solve call: stages the host arrays on the device, solves,
+// and returns the fields, the readout and the break record.
+GsRun run_gs64(const Array2D &a, const std::vector<double> &g, const std::vector<double> &ru,
+               const std::vector<double> &rd, int iter_time, double w, bool enable_break, double tol, bool general,
+               const std::vector<double> *init_u, const std::vector<double> *init_d)
+{
+    const int n = a.n, m = a.m;
+    const size_t nm = (size_t)n * m;
+    GsRun res;
+    res.vup.resize(nm);
+    res.vdown.resize(nm);
+    res.iout.resize(m);
+    const char *rbenv = std::getenv("FADESIM_GS64_REDBLACK");
+    res.redblack = (rbenv && std::string(rbenv) == "1") || n > 512;
+    std::vector<void *> owned;
+    auto up = [&](const void *src, size_t bytes) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes));
+        owned.push_back(p);
+        if (src)
+            HIPCHK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+        return p;
+    };
+    struct Guard {
+        std::vector<void *> &v;
+        ~Guard()
+        {
+            for (void *p : v)
+                (void)hipFree(p);
+        }
+    } guard{owned};
+    fadesim_gs64_desc d{};
+    d.struct_size = sizeof d;
+    d.rows = n;
+    d.cols = m;
+    d.d_g = (const double *)up(g.data(), nm * 8);
+    d.d_ru = (const double *)up(ru.data(), ru.size() * 8);
+    d.ru_ld = m + 1;
+    d.d_rd = (const double *)up(rd.data(), rd.size() * 8);
+    d.rd_ld = n + 1;
+    d.d_vin = (const double *)up(a.v[1].data(), n * 8);
+    d.d_vout = (const double *)up(a.v[2].data(), m * 8);
+    d.iter_number = iter_time;
+    d.omega = w;
+    d.enable_break = enable_break;
+    d.tol = tol;
+    d.d_vup = (double *)up(nullptr, nm * 8);
+    d.d_vdown = (double *)up(nullptr, nm * 8);
+    d.d_iout = (double *)up(nullptr, m * 8);
+    int *d_int = (int *)up(nullptr, 2 * sizeof(int));
+    d.d_record = d_int;
+    d.d_sweeps = d_int + 1;
+    if (general) {
+        d.general = 1;
+        int *on[4];
+        for (int e = 0; e < 4; ++e)
+            on[e] = (int *)up(a.bv[e].data(), a.bv[e].size() * sizeof(int));
+        d.d_on_top = on[0];
+        d.d_on_left = on[1];
+        d.d_on_bottom = on[2];
+        d.d_on_right = on[3];
+        d.d_v_top = (const double *)up(a.v[0].data(), m * 8);
+        d.d_v_right = (const double *)up(a.v[3].data(), n * 8);
+    }
+    if (init_u) {
+        d.d_init_up = (const double *)up(init_u->data(), nm * 8);
+        d.d_init_down = (const double *)up(init_d->data(), nm * 8);
+    }
+    d.order = res.redblack ? FADESIM_GS64_REDBLACK : FADESIM_GS64_LEXICOGRAPHIC;
+    HIPCHK(hipDeviceSynchronize());
+    auto t0 = std::chrono::steady_clock::now();
+    if (fadesim_gs64_solve(&d, nullptr) != FADESIM_OK)
+        throw CmdError(std::string("solver: ") + fadesim_last_error());
+    HIPCHK(hipDeviceSynchronize());
+    res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int hi[2];
+    HIPCHK(hipMemcpy(res.vup.data(), d.d_vup, nm * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(res.vdown.data(), d.d_vdown, nm * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(res.iout.data(), d.d_iout, m * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hi, d_int, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    res.record = hi[0];
+    res.sweeps = hi[1];
+    return res;
+}
+
+// Segment arrays in the solver's layout: ru [n][m+1] and rd [m][n+1].  V2
+// reads the wordline segments of its (square) arrays from RDLine.
+void segments(const Array2D &a, std::vector<double> &ru, std::vector<double> &rd, bool v2_alias)
+{
+    const int n = a.n, m = a.m;
+    ru.assign((size_t)n * (m + 1), 0.0);
+    rd.assign((size_t)m * (n + 1), 0.0);
+    for (int j = 0; j < m; ++j)
+        for (int i = 0; i <= n; ++i)
+            rd[(size_t)j * (n + 1) + i] = a.RD[j][i];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= m; ++j)
+            ru[(size_t)i * (m + 1) + j] = v2_alias ? a.RD[i][j] : a.RU[i][j];
+}
+
+// The initial state of V1 mode bit 0 (crossbarHspice/2dArray.cc:1425-1535):
+// an equivalent-resistance estimate of the wordline voltages and the bitline
+// voltages under them, each expression in the reference's evaluation order.
+void scichina_guess(const Array2D &a, const std::vector<double> &g, std::vector<double> &vup,
+                    std::vector<double> &vdown)
+{
+    const int M = a.n, N = a.m;
+    const double g_wire = 1.0 / a.RD[0][0];
+    const double r_wire = 1 / g_wire, Rs = 1 / g_wire;
+    std::vector<std::vector<double>> R(M, std::vector<double>(N)), RR = R, RRR = R, Vwl = R, Rup = R, Rdown = R,
+                                                                    Req = R;
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            R[i][j] = 1.0 / g[(size_t)i * N + j];
+    std::vector<double> gsum(N, 0.0);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i)
+            gsum[j] += g[(size_t)i * N + j];
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < N; ++j)
+            RR[k][j] = R[k][j] + (M - k) * r_wire + (gsum[j] * R[k][j]) * Rs;
+    for (int k = 0; k < M; ++k)
+        for (int j = N - 1; j >= 0; --j)
+            RRR[k][j] = (j == N - 1) ? RR[k][N - 1] + r_wire
+                                     : (RR[k][j] * RRR[k][j + 1]) / (RR[k][j] + RRR[k][j + 1]) + r_wire;
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < N; ++j)
+            Vwl[k][j] = (RRR[k][j] - r_wire) / RRR[k][j] * (j == 0 ? a.v[1][k] : Vwl[k][j - 1]);
+    auto parallel = [](double x, double y) { return x * y / (x + y); };
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            Rup[i][j] = (i == 0) ? r_wire + R[i][j] : parallel(Rup[i - 1][j], R[i][j]) + r_wire;
+    for (int i = M - 1; i >= 0; --i)
+        for (int j = 0; j < N; ++j)
+            Rdown[i][j] = (i != M - 1) ? r_wire + parallel(Rdown[i + 1][j], R[i + 1][j]) : Rs;
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            Req[i][j] = (i == 0) ? Rdown[i][j] : parallel(Rup[i - 1][j], Rdown[i][j]);
+    vup.assign((size_t)M * N, 0.0);
+    vdown.assign((size_t)M * N, 0.0);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i) {
+            vup[(size_t)i * N + j] = Vwl[i][j];
+            vdown[(size_t)i * N + j] = Vwl[i][j] * (Req[i][j]) / (Req[i][j] + R[i][j]);
+        }
+}
+
 // The general-boundary solve (array2D::nodeBasedGeneralGS, 2dArray.cc:1209-1374)
 // on the GPU through fadesim_gs64_solve(general = 1); output
 // fastmode_nodebasedGeneralGS.out in V1's format (precision 6, width 14).
@@ -159,88 +321,20 @@ void generalGS(Array2D &a, const std::vector<double> &g, int iter_time, double w
                std::ostringstream &os)
 {
     const int n = a.n, m = a.m;
-    const size_t nm = (size_t)n * m;
-    std::vector<double> ru((size_t)n * (m + 1)), rd((size_t)m * (n + 1));
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j <= m; ++j)
-            ru[(size_t)i * (m + 1) + j] = a.RU[i][j];
-    for (int j = 0; j < m; ++j)
-        for (int i = 0; i <= n; ++i)
-            rd[(size_t)j * (n + 1) + i] = a.RD[j][i];
-    std::vector<double> vup(nm), vdown(nm), iout(m);
-    double *d_g, *d_ru, *d_rd, *d_v[4], *d_vup, *d_vdn, *d_iout;
-    int *d_on[4], *d_int;
-    HIPCHK(hipMalloc(&d_g, nm * 8));
-    HIPCHK(hipMalloc(&d_ru, ru.size() * 8));
-    HIPCHK(hipMalloc(&d_rd, rd.size() * 8));
-    for (int e = 0; e < 4; ++e) {
-        const size_t len = (e & 1) ? n : m;
-        HIPCHK(hipMalloc(&d_v[e], len * 8));
-        HIPCHK(hipMalloc(&d_on[e], len * sizeof(int)));
-        HIPCHK(hipMemcpy(d_v[e], a.v[e].data(), len * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_on[e], a.bv[e].data(), len * sizeof(int), hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipMalloc(&d_vup, nm * 8));
-    HIPCHK(hipMalloc(&d_vdn, nm * 8));
-    HIPCHK(hipMalloc(&d_iout, m * 8));
-    HIPCHK(hipMalloc(&d_int, 2 * sizeof(int)));
-    HIPCHK(hipMemcpy(d_g, g.data(), nm * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_ru, ru.data(), ru.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(d_rd, rd.data(), rd.size() * 8, hipMemcpyHostToDevice));
-    fadesim_gs64_desc d{};
-    d.rows = n;
-    d.cols = m;
-    d.d_g = d_g;
-    d.d_ru = d_ru;
-    d.ru_ld = m + 1;
-    d.d_rd = d_rd;
-    d.rd_ld = n + 1;
-    d.d_vin = d_v[1];
-    d.d_vout = d_v[2];
-    d.iter_number = iter_time;
-    d.omega = (double)(float)w; // V1 reads omega with stof
-    d.enable_break = enable_break;
-    d.tol = tol;
-    d.d_vup = d_vup;
-    d.d_vdown = d_vdn;
-    d.d_iout = d_iout;
-    d.d_record = d_int;
-    d.d_sweeps = d_int + 1;
-    d.general = 1;
-    d.d_on_top = d_on[0];
-    d.d_on_left = d_on[1];
-    d.d_on_bottom = d_on[2];
-    d.d_on_right = d_on[3];
-    d.d_v_top = d_v[0];
-    d.d_v_right = d_v[3];
-    HIPCHK(hipDeviceSynchronize());
-    auto t0 = std::chrono::steady_clock::now();
-    if (fadesim_gs64_solve(&d, nullptr) != FADESIM_OK)
-        throw CmdError(std::string("solver: ") + fadesim_last_error());
-    HIPCHK(hipDeviceSynchronize());
-    const double seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    int hi[2];
-    HIPCHK(hipMemcpy(vup.data(), d_vup, nm * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(vdown.data(), d_vdn, nm * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(iout.data(), d_iout, m * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hi, d_int, 2 * sizeof(int), hipMemcpyDeviceToHost));
-    for (void *p : {(void *)d_g, (void *)d_ru, (void *)d_rd, (void *)d_vup, (void *)d_vdn, (void *)d_iout,
-                    (void *)d_int})
-        HIPCHK(hipFree(p));
-    for (int e = 0; e < 4; ++e) {
-        HIPCHK(hipFree(d_v[e]));
-        HIPCHK(hipFree(d_on[e]));
-    }
+    std::vector<double> ru, rd;
+    segments(a, ru, rd, false);
+    GsRun r = run_gs64(a, g, ru, rd, iter_time, (double)(float)w /* V1 reads omega with stof */, enable_break, tol,
+                       true, nullptr, nullptr);
     std::cout << "evalutate ir drop using general GS-method, w = " << (float)w
               << " , print to file fastmode_nodebasedGeneralGS.out" << std::endl;
-    std::cout << "time = " << seconds << " s" << std::endl;
-    if (hi[0] != -1)
-        std::cout << "threshould break, iter times =  " << hi[0] << std::endl;
+    std::cout << "time = " << r.seconds << " s" << std::endl;
+    if (r.record != -1)
+        std::cout << "threshould break, iter times =  " << r.record << std::endl;
     // the general solve's "error I sum" (2dArray.cc:1316-1345): interior
     // wordline nodes and bitline rows 0 .. n-2
     const double gw = 1.0 / a.RD[0][0];
-    auto U = [&](int i, int j) { return vup[(size_t)i * m + j]; };
-    auto D = [&](int i, int j) { return vdown[(size_t)i * m + j]; };
+    auto U = [&](int i, int j) { return r.vup[(size_t)i * m + j]; };
+    auto D = [&](int i, int j) { return r.vdown[(size_t)i * m + j]; };
     auto G = [&](int i, int j) { return g[(size_t)i * m + j]; };
     double err = 0;
     for (int i = 0; i < n; ++i) {
@@ -272,9 +366,10 @@ void generalGS(Array2D &a, const std::vector<double> &g, int iter_time, double w
     }
     of << std::endl << "iout = " << std::endl;
     for (int j = 0; j < m; ++j)
-        of << std::setw(14) << iout[j];
+        of << std::setw(14) << r.iout[j];
     of << std::endl;
-    os << "general GS (GPU, red-black), sweeps = " << hi[1] << ", error I sum = " << err << std::endl;
+    os << "general GS (GPU, " << (r.redblack ? "red-black" : "lexicographic") << "), sweeps = " << r.sweeps
+       << ", error I sum = " << err << std::endl;
 }
 
 void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
@@ -282,18 +377,22 @@ void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
     if (args.size() < 3)
         throw CmdError("args number not matched (<3)");
     const int iter_time = parse<int>(args[0]);
-    const double w = std::stod(args[1]);
+    // V1 parses omega with stof (2dArray.cc:65), V2 with stod (concrete_command.h:1111)
+    const double w = g_v1 ? (double)std::stof(args[1]) : std::stod(args[1]);
     const int mode = parse<int>(args[2]);
     bool enable_break = false;
     double tol = 0;
     if (args.size() > 3) {
-        if (args.size() != 5)
+        if (args.size() != 5 && !g_v1)
             throw CmdError("args number not matched (!=5)");
         enable_break = checkYes(args[3]);
-        if (enable_break)
+        if (enable_break) {
+            if (args.size() < 5)
+                throw CmdError("args number not matched (!=5)");
             tol = std::stod(args[4]);
+        }
     }
-    if (mode & 1)
+    if ((mode & 1) && !g_v1)
         throw CmdError("mode bit 0 is not supported now");
     if (!(a.m >= 2 && a.n >= 2))
         throw CmdError("m and n should greater than 2");
@@ -301,7 +400,7 @@ void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
         throw CmdError("mode bit 2 is not supported now");
     const int n = a.n, m = a.m;
     const size_t nm = (size_t)n * m;
-    std::vector<double> g(nm), ru((size_t)n * (m + 1)), rd((size_t)m * (n + 1));
+    std::vector<double> g(nm), ru, rd;
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < m; ++j) {
             const int s = a.arr[i][j];
@@ -309,76 +408,30 @@ void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
                 throw CmdError("cell state " + std::to_string(s) + " has no resistance (cellRStates)");
             g[(size_t)i * m + j] = 1.0 / a.cellR_states[s];
         }
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j <= m; ++j)
-            ru[(size_t)i * (m + 1) + j] = a.RU[i][j];
-    for (int j = 0; j < m; ++j)
-        for (int i = 0; i <= n; ++i)
-            rd[(size_t)j * (n + 1) + i] = a.RD[j][i];
+    segments(a, ru, rd, !g_v1 && n == m);
     std::ostringstream os;
     std::vector<double> vup(nm), vdown(nm), iout(m);
+    std::vector<double> gu, gd;
+    if (mode & 1)
+        scichina_guess(a, g, gu, gd);
     int record = -1, sweeps = 0;
-    double seconds = 0;
+    bool redblack = false;
     if ((mode & 2) == 0) {
-        double *d_g, *d_ru, *d_rd, *d_vin, *d_vout, *d_vup, *d_vdn, *d_iout;
-        int *d_int;
-        HIPCHK(hipMalloc(&d_g, nm * 8));
-        HIPCHK(hipMalloc(&d_ru, ru.size() * 8));
-        HIPCHK(hipMalloc(&d_rd, rd.size() * 8));
-        HIPCHK(hipMalloc(&d_vin, n * 8));
-        HIPCHK(hipMalloc(&d_vout, m * 8));
-        HIPCHK(hipMalloc(&d_vup, nm * 8));
-        HIPCHK(hipMalloc(&d_vdn, nm * 8));
-        HIPCHK(hipMalloc(&d_iout, m * 8));
-        HIPCHK(hipMalloc(&d_int, 2 * sizeof(int)));
-        HIPCHK(hipMemcpy(d_g, g.data(), nm * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_ru, ru.data(), ru.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_rd, rd.data(), rd.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_vin, a.v[1].data(), n * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_vout, a.v[2].data(), m * 8, hipMemcpyHostToDevice));
-        fadesim_gs64_desc d{};
-        d.rows = n;
-        d.cols = m;
-        d.d_g = d_g;
-        d.d_ru = d_ru;
-        d.ru_ld = m + 1;
-        d.d_rd = d_rd;
-        d.rd_ld = n + 1;
-        d.d_vin = d_vin;
-        d.d_vout = d_vout;
-        d.iter_number = iter_time;
-        d.omega = w;
-        d.enable_break = enable_break;
-        d.tol = tol;
-        d.d_vup = d_vup;
-        d.d_vdown = d_vdn;
-        d.d_iout = d_iout;
-        d.d_record = d_int;
-        d.d_sweeps = d_int + 1;
-        HIPCHK(hipDeviceSynchronize());
-        auto t0 = std::chrono::steady_clock::now();
-        int rc = fadesim_gs64_solve(&d, nullptr);
-        if (rc != FADESIM_OK)
-            throw CmdError(std::string("solver: ") + fadesim_last_error());
-        HIPCHK(hipDeviceSynchronize());
-        seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        int hi[2];
-        HIPCHK(hipMemcpy(vup.data(), d_vup, nm * 8, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(vdown.data(), d_vdn, nm * 8, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(iout.data(), d_iout, m * 8, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(hi, d_int, 2 * sizeof(int), hipMemcpyDeviceToHost));
-        record = hi[0];
-        sweeps = hi[1];
-        for (void *p : {(void *)d_g, (void *)d_ru, (void *)d_rd, (void *)d_vin, (void *)d_vout,
-                        (void *)d_vup, (void *)d_vdn, (void *)d_iout, (void *)d_int})
-            HIPCHK(hipFree(p));
-        os << "Duration time = " << seconds << " s" << std::endl;
+        GsRun r = run_gs64(a, g, ru, rd, iter_time, w, enable_break, tol, false, (mode & 1) ? &gu : nullptr,
+                           (mode & 1) ? &gd : nullptr);
+        vup = std::move(r.vup);
+        vdown = std::move(r.vdown);
+        iout = std::move(r.iout);
+        record = r.record;
+        sweeps = r.sweeps;
+        redblack = r.redblack;
+        os << "Duration time = " << r.seconds << " s" << std::endl;
     } else {
-        // mode bit 1: no sweeps, the initial guess is reported
+        // mode bit 1: no sweeps, the initial state is reported
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < m; ++j) {
-                vup[(size_t)i * m + j] = a.v[1][i];
-                vdown[(size_t)i * m + j] = a.v[2][j];
+                vup[(size_t)i * m + j] = (mode & 1) ? gu[(size_t)i * m + j] : a.v[1][i];
+                vdown[(size_t)i * m + j] = (mode & 1) ? gd[(size_t)i * m + j] : a.v[2][j];
             }
         for (int j = 0; j < m; ++j) {
             double s = 0;
@@ -434,9 +487,11 @@ void nodebasedGS(Array2D &a, const std::vector<std::string> &args)
     for (int j = 0; j < m; ++j)
         of << std::setw(20) << iout[j];
     of << std::endl;
-    std::cout << "evaluate ir drop using GS-method (GPU, red-black), w = " << w << ", sweeps = " << sweeps
-              << ", print to file fastmode_nodebasedGS.out" << std::endl;
-    if (general_boundary(a))
+    std::cout << "evaluate ir drop using GS-method (GPU, " << (redblack ? "red-black" : "lexicographic")
+              << "), w = " << w << ", sweeps = " << sweeps << ", print to file fastmode_nodebasedGS.out" << std::endl;
+    if (record != -1)
+        std::cout << "threshould break, iter times =  " << record << std::endl;
+    if (general_boundary(a)) // V1's dispatcher (2dArray.cc:84-110); V2 has no general solver
         generalGS(a, g, iter_time, w, enable_break, tol, os);
     a.log(os.str());
 }
@@ -776,7 +831,19 @@ void run_line(Array2D &a, const std::string &line)
 
 int main(int argc, char *argv[])
 {
-    const std::string input = argc >= 2 ? argv[1] : "config";
+    std::string input = "config";
+    bool have_input = false;
+    for (int k = 1; k < argc; ++k) {
+        const std::string s = argv[k];
+        if (s == "-v1")
+            g_v1 = true;
+        else if (s == "-debug" || s == "-3DH" || s == "-3dh")
+            std::cout << "option " << s << " is not handled by this build; ignored" << std::endl;
+        else if (!have_input) {
+            input = s;
+            have_input = true;
+        }
+    }
     Array2D arr;
     int line = 0;
     const bool strict = std::getenv("FADESIM_STRICT_EXIT") && std::string(std::getenv("FADESIM_STRICT_EXIT")) == "1";

@@ -17,6 +17,8 @@
 #ifndef FADESIM_H
 #define FADESIM_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -31,6 +33,8 @@ extern "C" {
 #define FADESIM_ST_CONVERGED 0
 #define FADESIM_ST_MAXIT 1
 #define FADESIM_ST_NONFINITE 2
+#define FADESIM_ST_ABORTED 3      /* fadesim_ragged_solve: the launch was aborted before this
+                                   * instance was solved (its other outputs are not valid) */
 
 /* flags of fadesim_batch_desc */
 #define FADESIM_F_FIXED_SWEEPS 1u /* run exactly max_sweeps sweeps, no test, no refinement */
@@ -60,6 +64,8 @@ int fadesim_version(void);
  * tolerance instead of a fixed or max|dv| sweep rule.
  */
 typedef struct fadesim_batch_desc {
+    size_t struct_size;     /* sizeof(fadesim_batch_desc) as the caller compiled it; a call whose
+                             * size differs from the library's is refused (FADESIM_E_INVALID) */
     int batch;              /* B instances */
     int rows, cols;         /* n, m: multiples of 4, 8..128 */
     const float *d_g;       /* [B][n][m] cell conductance (S), or [S][n][m] with d_gidx */
@@ -95,7 +101,8 @@ typedef struct fadesim_batch_desc {
      * with the arithmetic of fadesim_nn_combine_up (channel 0 -> wordline drops,
      * 1 -> bitline drops), so that kernel's field is never written.  Needs
      * d_warm; warm_level 1..6 with the coarse grid small enough for the
-     * kernel's LDS staging ((n >> level) x (m >> level) <= 1024 at 128x128). */
+     * kernel's LDS staging: 2 (n >> level)(m >> level) <= 12800 floats, so every
+     * level >= 1 fits at 128x128. */
     const void *d_warm_coarse;  /* [B][n >> warm_level][m >> warm_level][warm_coarse_ch] BF16, or NULL */
     const float *d_warm_scale;  /* [B] */
     int warm_coarse_ch;         /* even, >= 2 */
@@ -190,25 +197,31 @@ int fadesim_ragged_plan_create(fadesim_ragged_plan **plan, int count, const int 
 void fadesim_ragged_plan_destroy(fadesim_ragged_plan *plan);
 int fadesim_ragged_plan_info(const fadesim_ragged_plan *plan, fadesim_ragged_plan_stats *stats);
 int fadesim_ragged_solve(const fadesim_ragged_plan *plan, const fadesim_ragged_desc *desc, void *stream);
-/* 1 in *aborted when the last solve's team waits timed out (results invalid);
+/* 1 in *aborted when the last solve's team waits timed out (results invalid:
+ * every instance the launch did not finish has status FADESIM_ST_ABORTED);
  * synchronises with the device. */
 int fadesim_ragged_aborted(const fadesim_ragged_plan *plan, int *aborted);
 /* Kernel duration (ms) of the plan's last solve made with FADESIM_F_TIMING. */
 double fadesim_ragged_last_ms(const fadesim_ragged_plan *plan);
 
 /*
- * One crossbar, FP64 red-black SOR in node voltages with per-segment wire
- * resistances and the reference's stopping rule (stop after the first sweep
- * whose max |dv| < tol; fixed iter_number sweeps without a break).  The
- * solver behind the `sim` CLI command
+ * One crossbar, FP64 SOR in node voltages with per-segment wire resistances
+ * and the reference's stopping rule (stop after the first sweep whose
+ * max |dv| < tol; fixed iter_number sweeps without a break).  The solver
+ * behind the `sim` CLI command
  *   nodebasedGSMethod <iter> <omega> <mode> [yes <tol>]
  * replacing NodebasedGSMethod::nodeBaseGS
  *   crossbarHspice/V2/concrete_command.h:1136-1361 (command :1097-1127)
  * and array2D::nodeBasedGS  crossbarHspice/2dArray.cc:1377-1687.
- * The sweep order is red-black, not lexicographic: the fixed point is the
- * same, the iterates are not.
+ * order FADESIM_GS64_LEXICOGRAPHIC (0, the default) is the reference's own
+ * sweep order and arithmetic: the same iterates, break sweep and output bits
+ * (rows <= 512); FADESIM_GS64_REDBLACK colours the nodes instead: the same
+ * fixed point, different iterates.
  */
+#define FADESIM_GS64_LEXICOGRAPHIC 0
+#define FADESIM_GS64_REDBLACK 1
 typedef struct fadesim_gs64_desc {
+    size_t struct_size;    /* sizeof(fadesim_gs64_desc) as the caller compiled it */
     int rows, cols;        /* n, m >= 2 */
     const double *d_g;     /* [n][m] cell conductance (S) */
     const double *d_ru;    /* [n][ru_ld] wordline segments (ohm), j in [0, m] */
@@ -234,6 +247,10 @@ typedef struct fadesim_gs64_desc {
     const int *d_on_top, *d_on_left, *d_on_bottom, *d_on_right; /* general: [m], [n], [m], [n] */
     const double *d_v_top;   /* general: [m] top drive voltages (left = d_vin, bottom = d_vout) */
     const double *d_v_right; /* general: [n] right drive voltages */
+    int order;               /* FADESIM_GS64_LEXICOGRAPHIC or FADESIM_GS64_REDBLACK */
+    const double *d_init_up;   /* [n][m] initial wordline node voltages, or NULL (the reference's
+                                * initial state); with d_init_down: V1 mode bit 0's guess */
+    const double *d_init_down; /* [n][m] initial bitline node voltages, or NULL */
 } fadesim_gs64_desc;
 
 int fadesim_gs64_solve(const fadesim_gs64_desc *desc, void *stream);

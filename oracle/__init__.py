@@ -52,6 +52,11 @@ def lib():
         L.oracle_nodegs.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, i, d, i, d,
                                     _dp, _dp, _dp, _ip]
         L.oracle_nodegs.restype = i
+        L.oracle_nodegs_init.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, _dp, _dp, i, d, i, d,
+                                         _dp, _dp, _dp, ctypes.POINTER(i)]
+        L.oracle_nodegs_init.restype = i
+        L.oracle_scichina_guess.argtypes = [i, i, d, _dp, _dp, _dp, _dp]
+        L.oracle_scichina_guess.restype = None
         L.oracle_errisum.argtypes = [i, i, d, _dp, _dp, _dp]
         L.oracle_errisum.restype = d
         L.oracle_exact.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -118,8 +123,9 @@ def segments_uniform(n: int, m: int, r: float):
     return np.full((n, m + 1), float(r)), np.full((m, n + 1), float(r))
 
 
-def nodegs(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0):
-    """Lexicographic SOR.  Returns (vup, vdown, iout, record)."""
+def nodegs(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0, init=None):
+    """Lexicographic SOR (from the initial state `init` = (vup0, vdown0) when
+    given).  Returns (vup, vdown, iout, record)."""
     g = _d(g)
     n, m = g.shape
     RU, RD = _d(RU), _d(RD)
@@ -127,13 +133,24 @@ def nodegs(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0):
     vdn = np.empty((n, m))
     iout = np.empty(m)
     rec = ctypes.c_int(0)
-    rc = lib().oracle_nodegs(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
-                             _pd(_d(vin)), _pd(_d(vout)), int(iters), float(w),
-                             int(bool(enable_break)), float(tol), _pd(vup), _pd(vdn),
-                             _pd(iout), ctypes.byref(rec))
+    iu, idn = (None, None) if init is None else (_d(init[0]), _d(init[1]))
+    rc = lib().oracle_nodegs_init(n, m, _pd(RU), RU.shape[1], _pd(RD), RD.shape[1], _pd(g),
+                                  _pd(_d(vin)), _pd(_d(vout)), None if iu is None else _pd(iu),
+                                  None if idn is None else _pd(idn), int(iters), float(w),
+                                  int(bool(enable_break)), float(tol), _pd(vup), _pd(vdn),
+                                  _pd(iout), ctypes.byref(rec))
     if rc != 0:
         raise ValueError("m and n should greater than 2")
     return vup, vdn, iout, rec.value
+
+
+def scichina_guess(g, vin, r_seg):
+    """V1 mode bit 0's initial state (2dArray.cc:1425-1535).  Returns (vup0, vdown0)."""
+    g = _d(g)
+    n, m = g.shape
+    vu, vd = np.empty((n, m)), np.empty((n, m))
+    lib().oracle_scichina_guess(n, m, float(r_seg), _pd(g), _pd(_d(vin)), _pd(vu), _pd(vd))
+    return vu, vd
 
 
 def nodegs_batch(g32, r, vin, iters, w, tol):

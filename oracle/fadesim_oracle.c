@@ -12,6 +12,9 @@
  *                      array2D::nodeBasedGS (crossbarHspice/2dArray.cc:1377-1687).
  *                      Pinned bitwise against golden vectors written by the
  *                      reference `sim` binaries (tests/golden/).
+ *   oracle_scichina_guess, oracle_nodegs_init  V1 mode bit 0: the initial
+ *                      state of crossbarHspice/2dArray.cc:1425-1535 and the same
+ *                      SOR from it; pinned against golden vectors of `sim_v1`.
  *   oracle_exact       exact FP64 solve of the same linear network by banded
  *                      Cholesky (the matrix is SPD).  Pinned against
  *                      scipy.sparse.linalg.spsolve, 2x2 closed forms and the
@@ -70,11 +73,85 @@
  * vup/vdown/iout are the final iterate and the readout sum_i (vup-vdown)*g
  * (concrete_command.h:1344-1348); *record is the 0-based break sweep or -1.
  */
+int oracle_nodegs_init(int n, int m, const double *RU, int ru_ld, const double *RD,
+                       int rd_ld, const double *g, const double *vin,
+                       const double *vout, const double *init_up, const double *init_down,
+                       int iter_number, double w, int enable_break, double tol, double *vup_out,
+                       double *vdown_out, double *iout, int *record_out);
+
 int oracle_nodegs(int n, int m, const double *RU, int ru_ld, const double *RD,
                   int rd_ld, const double *g, const double *vin,
                   const double *vout, int iter_number, double w,
                   int enable_break, double tol, double *vup_out,
                   double *vdown_out, double *iout, int *record_out)
+{
+    return oracle_nodegs_init(n, m, RU, ru_ld, RD, rd_ld, g, vin, vout, NULL, NULL, iter_number, w,
+                              enable_break, tol, vup_out, vdown_out, iout, record_out);
+}
+
+/*
+ * The initial state of V1 mode bit 0 ("sci_china way", crossbarHspice/
+ * 2dArray.cc:1425-1535): equivalent resistances of the bitline ladders above
+ * and below each cell, the wordline voltages of a ladder of those, and the
+ * bitline voltages under them.  r is the wire resistance RDLine[0][0] (the
+ * reference uses g_wire = 1/RDLine[0][0] for every segment here).
+ */
+void oracle_scichina_guess(int M, int N, double r_seg, const double *g, const double *vin,
+                           double *vup, double *vdown)
+{
+    const double g_wire = 1.0 / r_seg;
+    const double r_wire = 1 / g_wire, Rs = 1 / g_wire;
+    const size_t nm = (size_t)M * N;
+    double *R = malloc(nm * sizeof(double)), *RR = malloc(nm * sizeof(double));
+    double *RRR = malloc(nm * sizeof(double)), *Vwl = malloc(nm * sizeof(double));
+    double *Rup = malloc(nm * sizeof(double)), *Rdn = malloc(nm * sizeof(double));
+    double *Req = malloc(nm * sizeof(double)), *gs = calloc(N, sizeof(double));
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            R[IDX(i, j, N)] = 1.0 / g[IDX(i, j, N)];
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < M; ++i)
+            gs[j] += g[IDX(i, j, N)];
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < N; ++j)
+            RR[IDX(k, j, N)] = R[IDX(k, j, N)] + (M - k) * r_wire + (gs[j] * R[IDX(k, j, N)]) * Rs;
+    for (int k = 0; k < M; ++k) {
+        RRR[IDX(k, N - 1, N)] = RR[IDX(k, N - 1, N)] + r_wire;
+        for (int j = N - 2; j >= 0; --j) {
+            const double a = RR[IDX(k, j, N)], b = RRR[IDX(k, j + 1, N)];
+            RRR[IDX(k, j, N)] = (a * b) / (a + b) + r_wire;
+        }
+    }
+    for (int k = 0; k < M; ++k)
+        for (int j = 0; j < N; ++j) {
+            const double x = RRR[IDX(k, j, N)];
+            Vwl[IDX(k, j, N)] = (x - r_wire) / x * (j == 0 ? vin[k] : Vwl[IDX(k, j - 1, N)]);
+        }
+#define PAR(x, y) ((x) * (y) / ((x) + (y)))
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            Rup[IDX(i, j, N)] = i == 0 ? r_wire + R[IDX(i, j, N)] : PAR(Rup[IDX(i - 1, j, N)], R[IDX(i, j, N)]) + r_wire;
+    for (int i = M - 1; i >= 0; --i)
+        for (int j = 0; j < N; ++j)
+            Rdn[IDX(i, j, N)] = i != M - 1 ? r_wire + PAR(Rdn[IDX(i + 1, j, N)], R[IDX(i + 1, j, N)]) : Rs;
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            Req[IDX(i, j, N)] = i == 0 ? Rdn[IDX(i, j, N)] : PAR(Rup[IDX(i - 1, j, N)], Rdn[IDX(i, j, N)]);
+#undef PAR
+    for (size_t k = 0; k < nm; ++k) {
+        vup[k] = Vwl[k];
+        vdown[k] = Vwl[k] * (Req[k]) / (Req[k] + R[k]);
+    }
+    free(R); free(RR); free(RRR); free(Vwl); free(Rup); free(Rdn); free(Req); free(gs);
+}
+
+/* oracle_nodegs from a given initial state (NULL: the reference's u = vin,
+ * d = vout), as V1 mode bit 0 starts from oracle_scichina_guess. */
+int oracle_nodegs_init(int n, int m, const double *RU, int ru_ld, const double *RD,
+                       int rd_ld, const double *g, const double *vin,
+                       const double *vout, const double *init_up, const double *init_down,
+                       int iter_number, double w, int enable_break, double tol, double *vup_out,
+                       double *vdown_out, double *iout, int *record_out)
 {
     if (!(m >= 2 && n >= 2))
         return -1;
@@ -86,8 +163,8 @@ int oracle_nodegs(int n, int m, const double *RU, int ru_ld, const double *RD,
     vdown[1] = (double *)malloc(nm * sizeof(double));
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < m; ++j) {
-            vup[1][IDX(i, j, m)] = vup[0][IDX(i, j, m)] = vin[i];
-            vdown[1][IDX(i, j, m)] = vdown[0][IDX(i, j, m)] = vout[j];
+            vup[1][IDX(i, j, m)] = vup[0][IDX(i, j, m)] = init_up ? init_up[IDX(i, j, m)] : vin[i];
+            vdown[1][IDX(i, j, m)] = vdown[0][IDX(i, j, m)] = init_down ? init_down[IDX(i, j, m)] : vout[j];
         }
 #define RUL(i, j) RU[(size_t)(i) * ru_ld + (j)]
 #define RDL(j, i) RD[(size_t)(j) * rd_ld + (i)]

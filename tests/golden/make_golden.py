@@ -120,6 +120,7 @@ def run_case(name, version, n, m, states, cell_R, r, vin, vout, gs_args, seg_up=
     data = dict(version=version, n=n, m=m, states=states.astype(np.uint8),
                 cell_R=np.array(cell_R, float), RU=RU, RD=RD, vin=np.array(vin, float),
                 vout=np.array(vout, float), iters=int(gs_args[0]),
+                mode=0 if ir2s else int(gs_args[2]),
                 omega_text="" if ir2s else str(gs_args[1]),
                 enable_break=int(gs_args[2]) if ir2s else int(len(gs_args) > 3 and gs_args[3] == "yes"),
                 tol=float(gs_args[3] if ir2s else (gs_args[4] if len(gs_args) > 4 else 0.0)),
@@ -249,6 +250,16 @@ def main():
         vout = list(np.round(rng.uniform(0, 0.05, m), 3))
         run_case(f"small{idx}_v1_{n}x{m}", 1, n, m, st, STATES4, 2.93, vin, vout, args,
                  seg_up=su, seg_down=sd)
+
+    # V1 mode bit 0: the "sci_china" initial state (2dArray.cc:1425-1535), then
+    # sweeps; with a break; with mode bit 1 (the initial state reported); none.
+    sci_cases = [(6, 9, [60, 1.8, 1]), (16, 16, [100000, 1.9, 1, "yes", "1e-12"]), (7, 5, [10, 1.7, 3]),
+                 (12, 12, [0, 1.9, 1])]
+    for idx, (n, m, args) in enumerate(sci_cases):
+        rng = np.random.default_rng(1100 + idx)
+        st = rng.integers(0, 4, (n, m))
+        vin = list(np.round(rng.uniform(0, 1, n), 3))
+        run_case(f"sci{idx}_v1_{n}x{m}", 1, n, m, st, STATES4, 2.93, vin, [0.0] * m, args)
 
     # IR2S (`fastsolve <iters> 0 <enable_break> <break_th>`) through V1:
     # fixed sweep counts, the break rule, a source/sink load, non-square.

@@ -96,7 +96,13 @@ def test_c_abi_rejects_bad_descriptors_without_a_gpu():
     """Argument validation of the C entry points happens before any device
     work: each call returns FADESIM_E_INVALID / _UNSUPPORTED with a message."""
     import ctypes
+
+    import torch
     from fadesim_amd import _native as nat
+    if torch.cuda.is_available():
+        # the descriptors below hold fake device addresses: a validation
+        # regression must never reach a real device
+        pytest.skip("fake-pointer validation runs only where no GPU is visible")
     L = nat.lib()
     E_INVALID = -1
 
@@ -137,3 +143,32 @@ def test_c_abi_rejects_bad_descriptors_without_a_gpu():
     assert "d_warm_coarse" in err(L.fadesim_solve_batch(ctypes.byref(d), None))
     d.warm_coarse_ch, d.warm_level = 16, 7
     assert "d_warm_coarse" in err(L.fadesim_solve_batch(ctypes.byref(d), None))
+
+
+def test_c_abi_refuses_a_descriptor_of_another_size():
+    """struct_size guards the descriptor layout: a caller built against an
+    older or newer header is refused before any field past the size is read."""
+    import ctypes
+
+    import torch
+    from fadesim_amd import _native as nat
+    if torch.cuda.is_available():
+        pytest.skip("fake-pointer validation runs only where no GPU is visible")
+    L = nat.lib()
+    d = nat.BatchDesc()
+    assert d.struct_size == ctypes.sizeof(nat.BatchDesc)
+    d.batch, d.rows, d.cols, d.tol = 4, 128, 128, 1e-6
+    d.d_g = d.d_r_wire = d.d_vin = d.d_iout = 8
+    d.struct_size = ctypes.sizeof(nat.BatchDesc) - 24  # the descriptor without its last four fields
+    assert L.fadesim_solve_batch(ctypes.byref(d), None) == -1
+    assert "struct_size" in L.fadesim_last_error().decode()
+    q = nat.Gs64Desc()
+    assert q.struct_size == ctypes.sizeof(nat.Gs64Desc)
+    q.rows = q.cols = 4
+    q.struct_size = ctypes.sizeof(nat.Gs64Desc) - 24
+    assert L.fadesim_gs64_solve(ctypes.byref(q), None) == -1
+    assert "struct_size" in L.fadesim_last_error().decode()
+    q.struct_size = ctypes.sizeof(nat.Gs64Desc)
+    q.order = 7
+    assert L.fadesim_gs64_solve(ctypes.byref(q), None) == -1
+    assert "order" in L.fadesim_last_error().decode()

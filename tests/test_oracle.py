@@ -49,8 +49,12 @@ def _case_inputs(z):
 def test_nodegs_matches_reference_binary(path):
     z = np.load(path)
     g, w, RU, RD = _case_inputs(z)
-    vup, vdn, iout, rec = oracle.nodegs(g, z["vin"], z["vout"], RU, RD, int(z["iters"]), w,
-                                        bool(z["enable_break"]), float(z["tol"]))
+    mode = int(z["mode"]) if "mode" in z else 0
+    # V1 mode bit 0 starts from the "sci_china" state, bit 1 skips the sweeps
+    init = oracle.scichina_guess(g, z["vin"], RD[0, 0]) if mode & 1 else None
+    iters = 0 if mode & 2 else int(z["iters"])
+    vup, vdn, iout, rec = oracle.nodegs(g, z["vin"], z["vout"], RU, RD, iters, w,
+                                        bool(z["enable_break"]), float(z["tol"]), init=init)
     assert rec == int(z["record"])
     txt = np.array(["%.12e" % x for x in iout])
     assert (txt == z["iout_txt"]).all()

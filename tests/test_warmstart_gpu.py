@@ -315,6 +315,32 @@ def test_warm_coarse_arguments_checked(cuda):
     with pytest.raises(ValueError):
         solve_batch(g, r, vin, warm=base, warm_coarse=out, warm_scale=s, warm_level=1)
     with pytest.raises(RuntimeError):
-        # a full-resolution "coarse" grid does not fit the kernel's LDS staging
+        # level 0 (a full-resolution "coarse" grid) is outside warm_level's range 1..6
         solve_batch(g, r, vin, warm=base, warm_coarse=torch.zeros((4, 128, 128, 2), dtype=torch.bfloat16,
                                                                    device=cuda), warm_scale=s, warm_level=0)
+
+
+@pytest.mark.parametrize("level,ch", [(1, 2), (1, 4), (3, 16)])
+def test_warm_coarse_levels_at_load(cuda, level, ch):
+    """Every level the LDS staging takes at 128x128 (level 1: a 64x64 grid,
+    2 x 64 x 64 floats of the 12800 available): the initial field formed at
+    load equals fadesim_nn_combine_up's field bit for bit."""
+    import torch
+    from fadesim_amd import _native as nat
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    B, M, N = 5, 128, 128
+    g, r, vin = warmstart.make_instances(B, seed=9, device=cuda)
+    gen = torch.Generator().manual_seed(level * 10 + ch)
+    out = (torch.randn(B, M >> level, N >> level, ch, generator=gen) * 1e-2).to(torch.bfloat16).to(cuda)
+    base = (torch.randn(B, 2, M, N, generator=gen) * 1e-3).to(cuda)
+    scale = torch.rand(B, generator=gen).to(cuda)
+    warm = torch.empty(B, 2, M, N, device=cuda)
+    nat.check(nat.lib().fadesim_nn_combine_up(B, M, N, ctypes.c_void_p(base.data_ptr()),
+                                              ctypes.c_void_p(scale.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                              ch, level, ctypes.c_void_p(warm.data_ptr()),
+                                              ctypes.c_void_p(nat.stream_handle(cuda))), "combine_up")
+    f0 = solve_batch(g, r, vin, warm=base, warm_coarse=out, warm_scale=scale, warm_level=level, fixed_sweeps=0,
+                     want_field=True)
+    torch.cuda.synchronize()
+    assert torch.equal(f0.field, warm)
