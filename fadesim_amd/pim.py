@@ -103,6 +103,7 @@ class PhyArrParams:
         self.nonlinear_params = []            # Ron, Roff, a_on, b_on, a_off, b_off
         self.tol = 1e-6                       # converged mode: relative I_out tolerance
         self.warm_start = None                # converged mode: WarmStartNet or None
+        self.record = None                    # a list: every layer call appends its tile currents
         self.ir2s_iters = 0                   # converged mode: IR2S iterations on each solve's initial guess
         self.max_batch = 65536                # instances per batched solve
         self.stats = {}                       # converged mode: solves, sweeps, Picard steps (accumulated)
@@ -355,6 +356,8 @@ def _solve_converged(bits, G, inputV, P):
         if int((res.status != 0).sum()) != 0:
             raise RuntimeError("crossbar solve did not converge")
         I[ii[sl], jj[sl], ss[sl], bb[sl], xx[sl], yy[sl]] = res.iout
+    if P.record is not None:
+        P.record.append(I)
     return I
 
 

@@ -190,6 +190,13 @@ def cell_conductance(g_lin: torch.Tensor, vin: torch.Tensor, vout: Optional[torc
     return g
 
 
+def _column_change(cur: torch.Tensor, prev: torch.Tensor) -> float:
+    """Largest relative change of any bitline current between two Picard
+    steps, each column measured against its own current (a column with no
+    current in either step counts as unchanged)."""
+    return float(((cur - prev).abs() / cur.abs().clamp_min(1e-300)).amax().item())
+
+
 @dataclass
 class NonlinearResult(SolveResult):
     outer: int = 0                 # Picard iterations (linear solves) performed
@@ -202,7 +209,7 @@ def solve_batch_nonlinear(g_lin: torch.Tensor, r_wire, vin: torch.Tensor, vout: 
     """Crossbars with sinh-nonlinear cells: Picard iteration G <- G(V) around
     the linear batched solve, each solve warm-started from the previous
     field, until the bitline currents of consecutive steps agree to tol/4
-    (relative to the largest current of the instance).
+    relative in every column (each column against its own current).
 
     The fixed point is the nonlinear KCL solution of the network, the state
     the reference's lagged-conductance sweeps (split_arr_matmul.cu:241-262)
@@ -231,16 +238,11 @@ def solve_batch_nonlinear(g_lin: torch.Tensor, r_wire, vin: torch.Tensor, vout: 
         field = res.field
         if int((res.status != 0).sum()) != 0:
             break
-        if prev is not None:
-            scale = res.iout.abs().amax(dim=1).clamp_min(1e-300)
-            change = ((res.iout - prev).abs().amax(dim=1) / scale).max().item()
-            if change <= inner_tol:
-                break
+        if prev is not None and _column_change(res.iout, prev) <= inner_tol:
+            break
         prev = res.iout
         G = cell_conductance(g_lin, vin, vout, field, params=params, gidx=gidx, out=G)
     st = res.status.clone()
-    if k == max_outer and prev is not None and st.eq(0).all():
-        scale = res.iout.abs().amax(dim=1).clamp_min(1e-300)
-        if ((res.iout - prev).abs().amax(dim=1) / scale).max().item() > inner_tol:
-            st.fill_(1)
+    if k == max_outer and prev is not None and st.eq(0).all() and _column_change(res.iout, prev) > inner_tol:
+        st.fill_(1)
     return NonlinearResult(res.iout, res.sweeps, st, res.est, res.field, outer=k, total_sweeps=total)

@@ -57,6 +57,8 @@ def lib():
         L.oracle_nodegs_init.restype = i
         L.oracle_scichina_guess.argtypes = [i, i, d, _dp, _dp, _dp, _dp]
         L.oracle_scichina_guess.restype = None
+        L.oracle_exact_nonlinear_batch.argtypes = [i, i, i, _fp, _dp, _dp, _dp, d, i, _dp, _ip]
+        L.oracle_exact_nonlinear_batch.restype = None
         L.oracle_errisum.argtypes = [i, i, d, _dp, _dp, _dp]
         L.oracle_errisum.restype = d
         L.oracle_exact.argtypes = [i, i, _dp, i, _dp, i, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -142,6 +144,25 @@ def nodegs(g, vin, vout, RU, RD, iters, w, enable_break=False, tol=0.0, init=Non
     if rc != 0:
         raise ValueError("m and n should greater than 2")
     return vup, vdn, iout, rec.value
+
+
+def exact_nonlinear_batch(glin, r, vin, params, tol=1e-14, max_iter=60):
+    """Sinh-nonlinear cells converged: Picard iteration of the hook's cell
+    model around exact FP64 solves (uniform wire r per instance, grounded
+    bitlines).  glin [B, n, m] float32, r [B] or scalar, vin [B, n].
+    Returns (iout [B, m], solves [B])."""
+    glin = np.ascontiguousarray(glin, dtype=np.float32)
+    B, n, m = glin.shape
+    r = np.ascontiguousarray(np.broadcast_to(np.asarray(r, dtype=np.float64), (B,)))
+    vin = _d(np.asarray(vin).reshape(B, n))
+    prm = _d(np.asarray(params, dtype=np.float64).reshape(6))
+    iout = np.empty((B, m))
+    its = np.empty(B, dtype=np.int32)
+    lib().oracle_exact_nonlinear_batch(B, n, m, glin.ctypes.data_as(_fp), _pd(r), _pd(vin), _pd(prm), float(tol),
+                                       int(max_iter), _pd(iout), its.ctypes.data_as(_ip))
+    if (its < 0).any():
+        raise RuntimeError(f"oracle_exact_nonlinear_batch failed: {its.tolist()}")
+    return iout, its
 
 
 def scichina_guess(g, vin, r_seg):

@@ -19,6 +19,10 @@
  *                      Cholesky (the matrix is SPD).  Pinned against
  *                      scipy.sparse.linalg.spsolve, 2x2 closed forms and the
  *                      KCL/monotonicity invariants (tests/test_oracle.py).
+ *   oracle_exact_nonlinear_batch  sinh-nonlinear cells: Picard iteration of
+ *                      the hook's cell model around exact solves, to the
+ *                      nonlinear fixed point; pinned against the converged
+ *                      hook replay and a Newton solve (tests/test_oracle.py).
  *   oracle_exact_slots_f32g  the same solve for many drives over shared
  *                      conductance slots (one factorisation per slot); the
  *                      checker of the PimTorch-level crossbar currents.
@@ -479,6 +483,95 @@ int oracle_exact(int n, int m, const double *RU, int ru_ld, const double *RD,
     free(L);
     free(b);
     return 0;
+}
+
+/*
+ * Sinh-nonlinear cells, converged: the fixed point of G <- G(V) with exact
+ * FP64 solves of the network at each G (uniform wire resistance r[b], vout =
+ * 0) -- an independent route to the state that the reference's lagged-
+ * conductance sweeps (split_arr_matmul.cu:241-262) and the GPU's Picard
+ * solve both converge to.  The cell model is the hook's (:142-196):
+ *   V = u - d, |V| < 1e-5 clamped to 1e-5,
+ *   R(V) = (1-z)|V|/(b_on sinh(a_on|V|)) + z|V|/(b_off sinh(a_off|V|)),
+ *   z = max(0, (1/g_lin - Ron)/(Roff - Ron)),  G = 1/R,
+ * params = {Ron, Roff, a_on, b_on, a_off, b_off}.  The first G is that of
+ * u = vin, d = 0 (the hook's initial state).  Iterates until consecutive
+ * readouts agree to tol relative in every column, at most max_iter solves.
+ * glin [B][n][m] FP32, vin [B][n]; iout [B][m]; iters [B] = solves used, or
+ * -1 (not converged), -2 (memory), -3 (singular).  OpenMP over instances.
+ */
+void oracle_exact_nonlinear_batch(int B, int n, int m, const float *glin, const double *r, const double *vin,
+                                  const double *nl, double tol, int max_iter, double *iout, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < B; ++b) {
+        const size_t nm = (size_t)n * m;
+        const long N = 2L * n * m, W = 2L * m + 1;
+        double *RU = malloc((size_t)n * (m + 1) * sizeof(double));
+        double *RD = malloc((size_t)m * (n + 1) * sizeof(double));
+        double *z = malloc(nm * sizeof(double)), *G = malloc(nm * sizeof(double));
+        double *u = malloc(nm * sizeof(double)), *d = malloc(nm * sizeof(double));
+        double *L = malloc((size_t)N * W * sizeof(double)), *rhs = malloc((size_t)N * sizeof(double));
+        double *io = iout + (size_t)b * m, *prev = malloc((size_t)m * sizeof(double));
+        const float *gl = glin + (size_t)b * nm;
+        const double *vi = vin + (size_t)b * n;
+        int status = -2;
+        if (RU && RD && z && G && u && d && L && rhs && prev) {
+            for (size_t k = 0; k < (size_t)n * (m + 1); ++k)
+                RU[k] = r[b];
+            for (size_t k = 0; k < (size_t)m * (n + 1); ++k)
+                RD[k] = r[b];
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < m; ++j) {
+                    const double zz = (1.0 / (double)gl[IDX(i, j, m)] - nl[0]) / (nl[1] - nl[0]);
+                    z[IDX(i, j, m)] = zz < 0 ? 0 : zz;
+                    u[IDX(i, j, m)] = vi[i];
+                    d[IDX(i, j, m)] = 0.0;
+                }
+            status = -1;
+            for (int it = 1; it <= max_iter; ++it) {
+                for (size_t k = 0; k < nm; ++k) {
+                    double V = u[k] - d[k];
+                    if (V < 1e-5 && V > -1e-5)
+                        V = V > 0 ? 1e-5 : -1e-5;
+                    V = fabs(V);
+                    const double R_ = (1.0 - z[k]) * V / (nl[3] * sinh(V * nl[2])) + z[k] * V / (nl[5] * sinh(V * nl[4]));
+                    G[k] = 1.0 / R_;
+                }
+                memset(L, 0, (size_t)N * W * sizeof(double));
+                if (band_factor(n, m, RU, m + 1, RD, n + 1, G, L)) {
+                    status = -3;
+                    break;
+                }
+                band_rhs(n, m, RU, m + 1, RD, n + 1, vi, NULL, rhs);
+                band_solve(n, m, L, rhs);
+                for (size_t k = 0; k < nm; ++k) {
+                    u[k] = rhs[2 * k];
+                    d[k] = rhs[2 * k + 1];
+                }
+                for (int j = 0; j < m; ++j)
+                    io[j] = 0.0;
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j < m; ++j)
+                        io[j] += (u[IDX(i, j, m)] - d[IDX(i, j, m)]) * G[IDX(i, j, m)];
+                if (it > 1) {
+                    double ch = 0.0;
+                    for (int j = 0; j < m; ++j) {
+                        const double den = fabs(io[j]) > 0 ? fabs(io[j]) : 1e-300;
+                        const double c = fabs(io[j] - prev[j]) / den;
+                        ch = c > ch ? c : ch;
+                    }
+                    if (ch <= tol) {
+                        status = it;
+                        break;
+                    }
+                }
+                memcpy(prev, io, (size_t)m * sizeof(double));
+            }
+        }
+        iters[b] = status;
+        free(RU); free(RD); free(z); free(G); free(u); free(d); free(L); free(rhs); free(prev);
+    }
 }
 
 /* Exact solves of K drives over S shared FP32 conductance slots with uniform

@@ -304,6 +304,30 @@ def _nonlinear_G(V, z, nl):
     return 1.0 / ((1 - z) * V / Ion + z * V / Ioff)
 
 
+def test_exact_nonlinear_equals_hook_fixed_point_and_newton():
+    """oracle.exact_nonlinear_batch (Picard around exact solves) reaches the
+    same nonlinear fixed point as the converged hook replay (a different
+    iteration entirely) on a 16x24 network, and the brute-force Newton
+    solution of the 2x3 network below."""
+    from fadesim_amd import pim
+    P = pim.mlp_inference_params()
+    rng = np.random.default_rng(8)
+    n, m = 16, 24
+    G = (rng.integers(0, 4, (2, n, m)).astype(np.float32) * np.float32(P.cellDeltaConduct)
+         + np.float32(P.cellMinConduct)).astype(np.float32)
+    vin = (rng.random((2, n)) < 0.5) * 0.2
+    vin[:, 0] = 0.2
+    io, its = oracle.exact_nonlinear_batch(G, P.r_wire, vin, P.nonlinear_params)
+    assert (its >= 2).all()
+    for b in range(2):
+        ref, _ = oracle.hook_gs(vin[b][None, None], G[b][None].astype(np.float64), 1, n, m, 1, 1, 200000,
+                                1 / P.r_wire, 1.9, True, 1e-15, True, P.nonlinear_params)
+        np.testing.assert_allclose(io[b], ref.ravel(), rtol=1e-11)
+    # the cells are nonlinear: linear cells of the same G give other currents
+    lin = oracle.exact_batch(G, np.full(2, P.r_wire), vin)
+    assert np.max(np.abs(lin - io) / np.abs(io)) > 1e-2
+
+
 def test_hook_nonlinear_fixed_point_is_newton_solution():
     """The Picard fixed point of the nonlinear hook satisfies the nonlinear KCL:
     checked against a brute-force Newton solve of the 2x3 network."""
