@@ -39,6 +39,7 @@ class BatchDesc(_SizedDesc):
         ("d_iout", _vp), ("d_sweeps", _vp), ("d_status", _vp), ("d_est", _vp), ("d_field", _vp),
         ("ir2s_iters", ctypes.c_int), ("corr_iters", ctypes.c_int), ("ir2s_omega", ctypes.c_double),
         ("d_warm_coarse", _vp), ("d_warm_scale", _vp), ("warm_coarse_ch", ctypes.c_int), ("warm_level", ctypes.c_int),
+        ("d_guess_rejected", _vp),
     ]
 
 

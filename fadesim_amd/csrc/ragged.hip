@@ -104,6 +104,7 @@ __device__ __forceinline__ rb::Inst make_inst(const RaggedArgs &A, int k, int n)
     I.sweeps = A.sweeps ? A.sweeps + k : nullptr;
     I.status = A.status ? A.status + k : nullptr;
     I.est = A.est ? A.est + k : nullptr;
+    I.rejected = nullptr;
     I.rows = n;
     I.coarse = nullptr;
     I.cscale = 0.f;

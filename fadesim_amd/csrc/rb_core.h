@@ -93,6 +93,7 @@ struct Inst {
     int *sweeps;       // one element or null
     int *status;
     double *est;
+    int *rejected;     // one element or null: 1 when the initial guess was discarded for the zero guess
     int rows;
     // optional: warm = base + cscale * bilinear(coarse[.][.][0..1]), the
     // coarse grid 2^clevel times coarser (rbsor only; null elsewhere)
@@ -1154,6 +1155,37 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     return (double)em;
 }
 
+// Estimated relative error of the zero-drop guess, from the test just run:
+// that guess reads out S1_j (the ideal currents), and I_j - D_j (D_j being the
+// estimated error of the present readout I_j) is the test's first-order
+// estimate of the solution's current, so the zero guess is off by about
+// |I_j - D_j - S1_j|.  The largest over the columns, uniform over the
+// workgroup (strips: colI, colD and S1 are team totals, so over the team).
+template <class G>
+__device__ __forceinline__ double zero_guess_est(const Shared &sh, int t)
+{
+    constexpr int N = G::N, NW = G::NW;
+    float e = 0.f;
+    if (t < N) {
+        const double It = sh.colI[t] - sh.colD[t];
+        const double e0 = fabs(It - sh.S1[t]) / fmax(fabs(It), sh.Ifloor[t]);
+        e = (e0 == e0) ? (float)fmin(e0, 3.0e38) : INFINITY;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        e = fmaxf(e, __shfl_xor(e, off, 64));
+    if ((t & 63) == 0)
+        sh.wmax[t >> 6] = e;
+    __syncthreads();
+    float em = 0.f;
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        em = fmaxf(em, sh.wmax[w]);
+    em = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, em)));
+    __syncthreads();
+    return (double)em;
+}
+
 // ---------------------------------------------------------------- phases
 // The set-up, the test, the IR2S guess and the refinement guess run outside
 // the sweep loop.  Each is compiled as a function of its own so that its
@@ -1497,6 +1529,9 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
     int st = FADESIM_ST_MAXIT;
     double est = INFINITY;
     const double stop_tol = dwr ? C.tol_eff * C.dwr_scale : C.tol_eff;
+    // a given initial guess is judged at the first test against the zero guess
+    bool judge_guess = (I.warm != nullptr || coarse) && !fixed;
+    bool rejected = false;
 
     for (;;) {
         if (sweep < next_check) {
@@ -1532,6 +1567,35 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                                    has_r, has_u, has_d, RB_SH_ARGS(sh), pub - 1, &nbar);
         if (est < 0.0)
             return false;
+        if (judge_guess) {
+            judge_guess = false;
+            // a guess estimated worse than the zero guess (or not finite) is
+            // dropped: the solve restarts from zero drops (and the IR2S
+            // iterations on them, when asked for) and tests that at once
+            if (!(est <= zero_guess_est<G>(sh, t))) {
+                rejected = true;
+                if (C.ir2s_iters > 0) {
+                    if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
+                                              C.ir2s_iters, C.ir2s_omega, true))
+                        return false;
+                    tile_load<G>(wsb, i0, j0, yp, yq);
+                } else {
+#pragma unroll
+                    for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+                        for (int jj = 0; jj < TC; ++jj)
+                            yp[ii][jj] = yq[ii][jj] = 0.f;
+                    tile_store<G>(wsb, i0, j0, yp, yq);
+                }
+                publish_all<G>(yp, yq, halo, t);
+                if (STRIP)
+                    strip_publish<G>(X, yq, ty, j0, lane, pub++);
+                __syncthreads();
+                snap_valid = true;
+                next_check = sweep;
+                continue;
+            }
+        }
         bool reload = true, done = false, just_refined = false;
         if (fixed) {
             st = FADESIM_ST_CONVERGED;
@@ -1620,6 +1684,8 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 *I.status = st;
             if (I.est)
                 *I.est = fixed ? NAN : est;
+            if (I.rejected)
+                *I.rejected = rejected ? 1 : 0;
         }
     }
     if (I.field) {

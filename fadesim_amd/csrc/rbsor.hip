@@ -33,6 +33,7 @@ struct RbArgs {
     int *sweeps;
     int *status;
     double *est;
+    int *rejected;
     float *field;
 };
 
@@ -82,6 +83,7 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         I.sweeps = A.sweeps ? A.sweeps + b : nullptr;
         I.status = A.status ? A.status + b : nullptr;
         I.est = A.est ? A.est + b : nullptr;
+        I.rejected = A.rejected ? A.rejected + b : nullptr;
         I.rows = M;
         I.coarse = A.coarse ? A.coarse + (size_t)b * (M >> A.clevel) * (N >> A.clevel) * A.cch : nullptr;
         I.cscale = A.coarse ? A.cscale[b] : 0.f;
@@ -254,6 +256,7 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     args.sweeps = d->d_sweeps;
     args.status = d->d_status;
     args.est = d->d_est;
+    args.rejected = d->d_guess_rejected;
     args.field = d->d_field;
 
 #define FADESIM_RB_CASE(TRv, TCv, TYv, TXv)                                 \

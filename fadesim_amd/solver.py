@@ -32,6 +32,7 @@ class SolveResult:
     status: torch.Tensor          # [B] int32 (0 converged, 1 max sweeps, 2 non-finite)
     est: torch.Tensor             # [B] float64 final relative error estimate
     field: Optional[torch.Tensor]  # [B, 2, M, N] float32 drops (vin-u, d-vout) or None
+    rejected: Optional[torch.Tensor] = None  # [B] int32: 1 where the given guess was dropped for zero drops
 
 
 def last_kernel_ms() -> float:
@@ -118,6 +119,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     status = torch.empty(B, dtype=torch.int32, device=dev)
     est = torch.empty(B, dtype=torch.float64, device=dev)
     field = torch.empty((B, 2, M, N), dtype=torch.float32, device=dev) if want_field else None
+    rejected = torch.empty(B, dtype=torch.int32, device=dev) if warm is not None else None
 
     flags = 0
     if fixed_sweeps is not None:
@@ -147,6 +149,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     d.max_sweeps, d.flags = int(max_sweeps), int(flags)
     d.d_iout, d.d_sweeps, d.d_status, d.d_est = iout.data_ptr(), sweeps.data_ptr(), status.data_ptr(), est.data_ptr()
     d.d_field = _ptr(field)
+    d.d_guess_rejected = _ptr(rejected)
     d.ir2s_iters, d.corr_iters, d.ir2s_omega = int(ir2s_iters), int(corr_iters), float(ir2s_omega)
     if warm_coarse is not None:
         d.d_warm_coarse, d.d_warm_scale = warm_coarse.data_ptr(), warm_scale.data_ptr()
@@ -154,7 +157,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     with torch.cuda.device(dev):
         rc = nat.lib().fadesim_solve_batch(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
     nat.check(rc, "fadesim_solve_batch")
-    return SolveResult(iout, sweeps, status, est, field)
+    return SolveResult(iout, sweeps, status, est, field, rejected)
 
 
 def cell_conductance(g_lin: torch.Tensor, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,

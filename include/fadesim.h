@@ -107,6 +107,11 @@ typedef struct fadesim_batch_desc {
     const float *d_warm_scale;  /* [B] */
     int warm_coarse_ch;         /* even, >= 2 */
     int warm_level;
+    /* A given initial guess (d_warm, with or without d_warm_coarse) is judged at
+     * the first convergence test: when its estimated error exceeds that of the
+     * zero-drop guess (estimated from the same test), the solve restarts from
+     * zero drops, with ir2s_iters IR2S iterations on them. */
+    int *d_guess_rejected;      /* [B] out: 1 where that happened, else 0; or NULL */
 } fadesim_batch_desc;
 
 int fadesim_solve_batch(const fadesim_batch_desc *desc, void *stream);

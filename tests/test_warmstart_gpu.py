@@ -85,34 +85,63 @@ def test_conv_fwd_and_wgrad_match_torch(cuda, H, W, cin, cout):
         assert torch.max(torch.abs(dx.cpu().double() - gx)) <= 1e-2 * torch.max(torch.abs(gx)) + 1e-3
 
 
+class _Bf16(object):
+    """Rounds to bf16 where the kernels store bf16: the value going forward and
+    the gradient coming back (torch autograd in FP64 otherwise)."""
+
+    @staticmethod
+    def fn():
+        import torch
+
+        class F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, x):
+                return x.to(torch.bfloat16).to(x.dtype)
+
+            @staticmethod
+            def backward(ctx, g):
+                return g.to(torch.bfloat16).to(g.dtype)
+        return F.apply
+
+
 def _torch_net(net, x_nhwc, wflat):
-    """Same graph as fadesim_amd.warmstart.architecture() in torch FP64 (NCHW)."""
+    """Same graph as fadesim_amd.warmstart.architecture() in torch FP64 (NCHW),
+    rounded to bf16 wherever the kernels store bf16: every activation, every
+    gradient an op passes back to its input, and the sum of the gradients of
+    an activation used twice."""
     import torch.nn.functional as F
     from fadesim_amd.warmstart import kpad
-    acts = {"x": x_nhwc.permute(0, 3, 1, 2)}
+    rnd = _Bf16.fn()
+    acts = {"x": rnd(x_nhwc.permute(0, 3, 1, 2))}
     for op in net.ops:
         if op.kind == "conv":
             o = net.offsets[op.name]
             wt, b = _torch_weight(wflat[o:o + op.cout * kpad(op.cin)], op.cin, op.cout, kpad(op.cin))
-            y = F.conv2d(acts[op.src], wt, b, padding=1)
-            acts[op.name] = F.relu(y) if op.relu else y
+            y = F.conv2d(rnd(acts[op.src]), wt, b, padding=1)
+            acts[op.name] = rnd(F.relu(y) if op.relu else y)
         elif op.kind == "pool":
-            acts[op.name] = F.avg_pool2d(acts[op.src], 2)
+            acts[op.name] = rnd(F.avg_pool2d(rnd(acts[op.src]), 2))
         else:
-            acts[op.name] = F.interpolate(acts[op.src], scale_factor=2, mode="nearest") + acts[op.skip]
+            acts[op.name] = rnd(F.interpolate(rnd(acts[op.src]), scale_factor=2, mode="nearest") + rnd(acts[op.skip]))
     return acts["out"].permute(0, 2, 3, 1)
 
 
-@pytest.mark.parametrize("arch", ["unet16", "coarse32"])
+@pytest.mark.parametrize("arch", ["unet16", "coarse32", "coarse32f"])
 def test_network_gradients_match_torch_autograd(cuda, arch):
+    """Weight gradients of the whole network (MFMA forward, weight- and
+    data-gradient kernels, pooling, upsampling, skip sums) against torch FP64
+    autograd of the same graph with the kernels' bf16 storage reproduced:
+    what remains is FP32 accumulation order and the odd bf16 rounding it
+    flips, so each layer's gradient agrees to a few parts in a thousand."""
     import torch
     from fadesim_amd import _native as nat
     from fadesim_amd.warmstart import OUT_C, WarmStartNet
     net = WarmStartNet(cuda, seed=3, arch=arch)
-    B = 1
+    B = 2
     R = 128 >> net.out_level
+    Rin = 128 >> net.in_level
     gen = torch.Generator().manual_seed(5)
-    x = _bf(torch.rand(B, 128, 128, 8, generator=gen))
+    x = _bf(torch.rand(B, Rin, Rin, 8, generator=gen))
     tgt = torch.randn(B, 2, R, R, generator=gen).float()
     acts = net.forward(x.to(cuda), keep=True)
     out = acts["out"]
@@ -130,17 +159,19 @@ def test_network_gradients_match_torch_autograd(cuda, arch):
     # the kernel's loss: sum over pixels and both output channels of d^2 / (2 P), P = pixels
     l = ((o[..., 0:2].permute(0, 3, 1, 2) - tgt.double()) ** 2).sum() / (2 * B * R * R)
     l.backward()
-    assert abs(loss.item() - l.item()) <= 0.05 * l.item()
+    assert abs(loss.item() - l.item()) <= 1e-3 * l.item()
     g = grad.cpu().double()
+    worst = 0.0
     for op in net.ops:
         if op.kind != "conv":
             continue
         o0 = net.offsets[op.name]
         n = op.cout * (((9 * op.cin + 1 + 31) // 32) * 32)
         a, b = g[o0:o0 + n], wref.grad[o0:o0 + n]
-        cos = float(torch.dot(a, b) / (a.norm() * b.norm() + 1e-300))
         rel = float((a - b).norm() / (b.norm() + 1e-300))
-        assert cos > 0.995 and rel < 0.1, (op.name, cos, rel)
+        worst = max(worst, rel)
+        assert rel < 2e-2, (op.name, rel)
+    print(f"{arch}: largest relative gradient error of a layer {worst:.2e}")
 
 
 def _np_first_order(g, r, c):
@@ -188,7 +219,10 @@ def test_trained_warm_start_cuts_sweeps(cuda):
     warm = solve_batch(g, r, vin, warm=net.predict(g, r, vin), tol=1e-6)
     assert int((warm.status != 0).sum()) == 0
     sp, sw = plain.sweeps.double().mean().item(), warm.sweeps.double().mean().item()
-    assert sw < 0.8 * sp, (sp, sw)
+    # the trained network's field needs well under half of plain SOR's sweeps
+    # (about a sixth when measured)
+    assert sw < 0.4 * sp, (sp, sw)
+    assert int(warm.rejected.sum()) == 0
     k = 8
     iex = oracle.exact_batch(g[:k].cpu().numpy(), r[:k].cpu().numpy(), vin[:k].cpu().numpy())
     err = np.max(np.abs(warm.iout[:k].cpu().numpy() - iex) / np.abs(iex))
@@ -344,3 +378,65 @@ def test_warm_coarse_levels_at_load(cuda, level, ch):
                      want_field=True)
     torch.cuda.synchronize()
     assert torch.equal(f0.field, warm)
+
+
+def _headline_nn_call(net, g, r, vin, K):
+    """The NN-warm step that bench.py times: the network's output combined
+    with the base field while the solve loads it, one IR2S iteration on it,
+    the first test before any sweep, K IR2S iterations for the refinement
+    correction's guess."""
+    from fadesim_amd.solver import solve_batch
+    return solve_batch(g, r, vin, **net.warm_kwargs(g, r, vin), tol=1e-6, ir2s_iters=1, corr_iters=K,
+                       test_first=True)
+
+
+def test_headline_nn_call_meets_the_tolerance(cuda):
+    """bench.py's NN-warm call on 128 held-out instances (seeds outside the
+    training range) against the exact FP64 solve: within 1e-6 relative in
+    every bitline, the guess kept for every instance."""
+    import torch
+    from fadesim_amd import warmstart
+    net = warmstart.load_default(cuda)
+    if net is None:
+        pytest.skip("no trained weights committed")
+    g, r, vin = warmstart.make_instances(128, seed=30_000_000, device=cuda)
+    res = _headline_nn_call(net, g, r, vin, 4)
+    torch.cuda.synchronize()
+    assert int((res.status != 0).sum()) == 0
+    assert int(res.rejected.sum()) == 0
+    iex = oracle.exact_batch(g.cpu().numpy(), r.cpu().numpy(), vin.cpu().numpy())
+    err = np.abs(res.iout.cpu().numpy() - iex) / np.abs(iex)
+    print(f"headline NN-warm call: max rel err {err.max():.2e}, sweeps {np.bincount(res.sweeps.cpu().numpy())}")
+    assert err.max() <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["scaled1000", "noise"])
+def test_bad_warm_start_falls_back_to_the_zero_guess(cuda, kind):
+    """A guess estimated worse than the zero guess at the first test (after
+    its IR2S iteration, which shrinks any error by the coupling's spectral
+    radius, 0.01-0.05 here) is dropped: the solve continues exactly as the
+    solve from zero drops with the same IR2S iterations (bit for bit), and
+    says so in `rejected`."""
+    import torch
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    net = warmstart.load_default(cuda)
+    if net is None:
+        pytest.skip("no trained weights committed")
+    g, r, vin = warmstart.make_instances(64, seed=31_000_000, device=cuda)
+    w = net.predict(g, r, vin)
+    if kind == "scaled1000":
+        bad = 1000.0 * w
+    else:
+        bad = torch.randn(w.shape, generator=torch.Generator().manual_seed(1)).to(cuda) * (300 * w.abs().max())
+    kw = dict(tol=1e-6, ir2s_iters=1, corr_iters=4, test_first=True, want_field=True)
+    res = solve_batch(g, r, vin, warm=bad, **kw)
+    zero = solve_batch(g, r, vin, **kw)
+    good = solve_batch(g, r, vin, warm=w, **kw)
+    torch.cuda.synchronize()
+    assert int(res.rejected.sum()) == 64 and int(good.rejected.sum()) == 0
+    assert int((res.status != 0).sum()) == 0
+    assert torch.equal(res.iout, zero.iout) and torch.equal(res.sweeps, zero.sweeps)
+    assert torch.equal(res.field, zero.field)
+    iex = oracle.exact_batch(g[:8].cpu().numpy(), r[:8].cpu().numpy(), vin[:8].cpu().numpy())
+    assert np.max(np.abs(res.iout[:8].cpu().numpy() - iex) / np.abs(iex)) <= 1e-6
