@@ -23,6 +23,7 @@ struct RbArgs {
     const double *rw;
     const double *vin;
     const double *vout; // never null (zeros supplied by the host)
+    int vout_ld;        // elements between instances' vout (0: one row shared by all)
     const float *warm;
     const __bf16 *coarse; // optional network output added to warm at load
     const float *cscale;
@@ -76,7 +77,7 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         I.g = A.g + (size_t)(A.gidx ? A.gidx[b] : b) * M * N;
         I.rwire = A.rw[b];
         I.vin = A.vin + (size_t)b * M;
-        I.vout = A.vout + (size_t)b * N;
+        I.vout = A.vout + (size_t)b * A.vout_ld;
         I.warm = A.warm ? A.warm + (size_t)b * 2 * M * N : nullptr;
         I.field = A.field ? A.field + (size_t)b * 2 * M * N : nullptr;
         I.iout = A.iout + (size_t)b * N;
@@ -115,21 +116,33 @@ int resident_slots()
     return cached[dev];
 }
 
+// fixed: the workgroup slots come from `fixed` (fixed_bytes long) instead of
+// the library-owned workspace; the grid shrinks to the slots that fit.
 template <class G>
-int launch_rb(RbArgs args, int M, int N, hipStream_t s)
+int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, size_t fixed_bytes = 0)
 {
     // the coarse warm-start grid is staged in the halo planes' LDS
     if (args.coarse && 2 * (M >> args.clevel) * (N >> args.clevel) > G::HALO)
         return set_error(FADESIM_E_UNSUPPORTED, "d_warm_coarse: coarse grid too large for this shape (raise warm_level)");
+    const size_t slot_bytes = (size_t)rb::kWsPlanes * G::M * G::N * sizeof(float);
     int grid = resident_slots<G>();
     if (grid > args.batch)
         grid = args.batch;
-    const size_t ws_bytes = (size_t)grid * rb::kWsPlanes * G::M * G::N * sizeof(float) + 256;
-    char *ws = (char *)workspace_aux(ws_bytes);
-    if (!ws)
-        return FADESIM_E_NOMEM;
+    char *ws;
+    if (fixed) {
+        const size_t fit = (fixed_bytes - 256) / slot_bytes;
+        if (fit < 1)
+            return set_error(FADESIM_E_UNSUPPORTED, "scratch too small for one workgroup of this shape");
+        if ((size_t)grid > fit)
+            grid = (int)fit;
+        ws = fixed;
+    } else {
+        ws = (char *)workspace_aux((size_t)grid * slot_bytes + 256);
+        if (!ws)
+            return FADESIM_E_NOMEM;
+    }
     args.ws = (float *)ws;
-    args.queue = (int *)(ws + (size_t)grid * rb::kWsPlanes * G::M * G::N * sizeof(float));
+    args.queue = (int *)(ws + (size_t)grid * slot_bytes);
     hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, args.queue);
     FADESIM_HIP_CHECK(hipGetLastError());
     const bool timing = (args.ctl.flags & FADESIM_F_TIMING) != 0;
@@ -175,8 +188,14 @@ extern "C" double fadesim_default_omega(int rows, int cols)
     return 2.0 / (1.0 + 0.83 * s);
 }
 
-extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
+namespace {
+
+// Validates d and fills the kernel arguments (everything but the workspace
+// and vout).  Returns FADESIM_OK, or an error; *empty = true when there is
+// nothing to solve.
+int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
 {
+    *empty = false;
     if (!d)
         return set_error(FADESIM_E_INVALID, "null descriptor");
     if (d->struct_size != sizeof(fadesim_batch_desc))
@@ -186,8 +205,10 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     const int B = d->batch, M = d->rows, N = d->cols;
     if (B < 0)
         return set_error(FADESIM_E_INVALID, "batch must be >= 0");
-    if (B == 0)
+    if (B == 0) {
+        *empty = true;
         return FADESIM_OK;
+    }
     if (!d->d_g || !d->d_r_wire || !d->d_vin || !d->d_iout)
         return set_error(FADESIM_E_INVALID, "d_g, d_r_wire, d_vin and d_iout are required");
     double w = d->omega > 0 ? d->omega : fadesim_default_omega(M, N);
@@ -214,26 +235,14 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
             return set_error(FADESIM_E_INVALID, "d_warm_coarse needs d_warm, d_warm_scale, an even warm_coarse_ch "
                                                 ">= 2 and warm_level in 1..6 dividing rows and cols");
     }
-    hipStream_t s = (hipStream_t)stream;
-
-    const double *vout = d->d_vout;
-    if (!vout) {
-        double *z = (double *)workspace((size_t)B * N * sizeof(double));
-        if (!z)
-            return FADESIM_E_NOMEM;
-        hipLaunchKernelGGL(fill_zero_f64, dim3(256), dim3(256), 0, s, z, (size_t)B * N);
-        FADESIM_HIP_CHECK(hipGetLastError());
-        vout = z;
-    }
-
-    RbArgs args;
     args.batch = B;
     args.g = d->d_g;
     args.gidx = d->d_gidx;
     args.queue = nullptr;
     args.rw = d->d_r_wire;
     args.vin = d->d_vin;
-    args.vout = vout;
+    args.vout = d->d_vout;
+    args.vout_ld = N;
     args.warm = d->d_warm;
     args.coarse = (const __bf16 *)d->d_warm_coarse;
     args.cscale = d->d_warm_scale;
@@ -258,10 +267,14 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     args.est = d->d_est;
     args.rejected = d->d_guess_rejected;
     args.field = d->d_field;
+    return FADESIM_OK;
+}
 
+int launch_shape(const RbArgs &args, int M, int N, hipStream_t s, char *fixed, size_t fixed_bytes)
+{
 #define FADESIM_RB_CASE(TRv, TCv, TYv, TXv)                                 \
     if (M == TRv * TYv && N == TCv * TXv)                                   \
-        return launch_rb<rb::Geo<TRv, TCv, TYv, TXv>>(args, M, N, s);
+        return launch_rb<rb::Geo<TRv, TCv, TYv, TXv>>(args, M, N, s, fixed, fixed_bytes);
     FADESIM_RB_CASE(4, 8, 32, 16) // 128 x 128, 512 threads
     FADESIM_RB_CASE(4, 8, 16, 8)  // 64 x 64, 128 threads
     FADESIM_RB_CASE(2, 4, 16, 8)  // 32 x 32, 128 threads
@@ -271,4 +284,26 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     return set_error(FADESIM_E_UNSUPPORTED,
                      "rows x cols = " + std::to_string(M) + " x " + std::to_string(N) +
                          " not supported (batched kernel: 32x32, 64x64, 128x128, 128x64, 64x128)");
+}
+
+} // namespace
+
+extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
+{
+    RbArgs args;
+    bool empty;
+    int rc = batch_args(d, args, &empty);
+    if (rc != FADESIM_OK || empty)
+        return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const int B = d->batch, M = d->rows, N = d->cols;
+    if (!args.vout) {
+        double *z = (double *)workspace((size_t)B * N * sizeof(double));
+        if (!z)
+            return FADESIM_E_NOMEM;
+        hipLaunchKernelGGL(fill_zero_f64, dim3(256), dim3(256), 0, s, z, (size_t)B * N);
+        FADESIM_HIP_CHECK(hipGetLastError());
+        args.vout = z;
+    }
+    return launch_shape(args, M, N, s, nullptr, 0);
 }
