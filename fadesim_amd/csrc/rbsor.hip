@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -93,6 +94,15 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         rb::solve<G, false>(I, none, A.ctl, wsb, sh);
     }
 }
+
+// Scratch of fadesim_solve_batch_noalloc: a zero row for an absent vout
+// (kScratchZeroBytes), then the persistent grid's queue counter and
+// per-workgroup slots, as launch_rb lays them out in the workspace.  Sized
+// for 256 workgroups of the 128x128 geometry (one per CU of an MI355X).
+constexpr size_t kScratchZeroBytes = 4096;
+constexpr size_t kScratchSlotsBytes = (size_t)256 * rb::kWsPlanes * 128 * 128 * sizeof(float) + 256;
+constexpr size_t kScratchBytes = kScratchZeroBytes + kScratchSlotsBytes;
+static __device__ __align__(256) char scratch[kScratchBytes];
 
 thread_local hipEvent_t t_ev[2] = {nullptr, nullptr};
 thread_local bool t_ev_valid = false;
@@ -306,4 +316,59 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
         args.vout = z;
     }
     return launch_shape(args, M, N, s, nullptr, 0);
+}
+
+namespace {
+std::mutex g_scratch_mutex;
+char *g_scratch_ptr[64] = {nullptr};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+char *scratch_on_current_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error(FADESIM_E_HIP, "hipGetDevice failed");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(FADESIM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = (char *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+} // namespace
+
+extern "C" int fadesim_solve_batch_noalloc(const fadesim_batch_desc *d, void *stream)
+{
+    RbArgs args;
+    bool empty;
+    int rc = batch_args(d, args, &empty);
+    if (rc != FADESIM_OK || empty)
+        return rc;
+    if (d->flags & FADESIM_F_TIMING)
+        return set_error(FADESIM_E_INVALID, "fadesim_solve_batch_noalloc: FADESIM_F_TIMING creates events; "
+                                            "time the call with the caller's own events");
+    if ((size_t)d->cols * sizeof(double) > kScratchZeroBytes)
+        return set_error(FADESIM_E_UNSUPPORTED, "fadesim_solve_batch_noalloc: cols beyond the scratch zero row");
+    char *base = scratch_on_current_device();
+    if (!base)
+        return FADESIM_E_HIP;
+    hipStream_t s = (hipStream_t)stream;
+    if (!args.vout) {
+        // one zero row for every instance, cleared by a kernel (no memset node
+        // if the caller captures the call)
+        double *z = (double *)base;
+        hipLaunchKernelGGL(fill_zero_f64, dim3(1), dim3(256), 0, s, z, (size_t)d->cols);
+        FADESIM_HIP_CHECK(hipGetLastError());
+        args.vout = z;
+        args.vout_ld = 0;
+    }
+    return launch_shape(args, d->rows, d->cols, s, base + kScratchZeroBytes, kScratchSlotsBytes);
 }

@@ -59,7 +59,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
                 out: Optional[torch.Tensor] = None, timing: bool = False,
                 warm_coarse: Optional[torch.Tensor] = None, warm_scale: Optional[torch.Tensor] = None,
-                warm_level: int = 2) -> SolveResult:
+                warm_level: int = 2, noalloc: bool = False) -> SolveResult:
     """Solve B crossbars of one shape.
 
     g [B, M, N] float32 (S) -- or [S, M, N] conductance slots with ``gidx`` [B]
@@ -79,6 +79,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     ``warm_scale`` [B]: the initial drops are warm + warm_scale x
     bilinear(warm_coarse[..., 0:2]), formed while loading (the arithmetic of
     fadesim_nn_combine_up; WarmStartNet.warm_kwargs builds these).
+    ``noalloc`` solves through fadesim_solve_batch_noalloc (the library
+    allocates nothing; the outputs here are still allocated by torch).
     """
     if g.dim() != 3:
         raise ValueError("g must be [B, M, N]")
@@ -154,9 +156,10 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     if warm_coarse is not None:
         d.d_warm_coarse, d.d_warm_scale = warm_coarse.data_ptr(), warm_scale.data_ptr()
         d.warm_coarse_ch, d.warm_level = int(warm_coarse.shape[3]), int(warm_level)
+    fn = nat.lib().fadesim_solve_batch_noalloc if noalloc else nat.lib().fadesim_solve_batch
     with torch.cuda.device(dev):
-        rc = nat.lib().fadesim_solve_batch(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
-    nat.check(rc, "fadesim_solve_batch")
+        rc = fn(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))
+    nat.check(rc, "fadesim_solve_batch_noalloc" if noalloc else "fadesim_solve_batch")
     return SolveResult(iout, sweeps, status, est, field, rejected)
 
 

@@ -116,6 +116,20 @@ typedef struct fadesim_batch_desc {
 
 int fadesim_solve_batch(const fadesim_batch_desc *desc, void *stream);
 
+/*
+ * fadesim_solve_batch that allocates nothing: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * results as fadesim_solve_batch.  Its workspace is a file-scope device array
+ * of 83,890,432 bytes (80 MiB + 4.25 KiB) that every GPU the library runs on
+ * holds until the process ends: a zero row for an absent d_vout, the queue
+ * counter and the slots of up to 256 resident workgroups at 128x128 (more at
+ * smaller shapes; a larger batch is worked through by the same workgroups).  Every call on one GPU shares that array, so calls that overlap on
+ * one GPU race: order them on one stream.  FADESIM_F_TIMING is refused (it
+ * would create events).  Nothing is cleared with a memset node.
+ */
+int fadesim_solve_batch_noalloc(const fadesim_batch_desc *desc, void *stream);
+
 /* Duration (ms) of the solve kernel of the last fadesim_solve_batch call made
  * with FADESIM_F_TIMING on this thread, from HIP events on the call's stream;
  * waits for that kernel.  Negative when no timed call was made. */
