@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libf
 
 FADESIM_OK = 0
 ST_CONVERGED, ST_MAXIT, ST_NONFINITE, ST_ABORTED = 0, 1, 2, 3
-F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING, F_ZERO_CORRECTION, F_TEST_FIRST = 1, 2, 4, 8, 16, 32
+F_FIXED_SWEEPS, F_NO_REFINE, F_DWR, F_TIMING, F_ZERO_CORRECTION, F_TEST_FIRST, F_NO_FASTPATH = 1, 2, 4, 8, 16, 32, 64
 MAX_IR2S_ITERS = 64
 
 _vp = ctypes.c_void_p

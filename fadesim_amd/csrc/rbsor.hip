@@ -1,9 +1,13 @@
 // Batched IR-drop solver for gfx950, one shape per call: a persistent
 // work-queue kernel over the red-black SOR core of rb_core.h (FP32 sweeps on
 // drop variables, FP64 adjoint-weighted residual test, one FP64 refinement
-// step, FP64 readout), one workgroup per crossbar.
+// step, FP64 readout), one workgroup per crossbar.  Solves that test their
+// guess before sweeping go first through the register-resident fast path of
+// rb_fast.h (fast_kernel); the sweeping kernel then takes only the crossbars
+// that path did not certify, from a device-side list.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
@@ -12,12 +16,15 @@
 #include "common.h"
 #include "fadesim.h"
 #include "rb_core.h"
+#include "rb_fast.h"
 
 namespace fadesim {
 namespace {
 
 struct RbArgs {
     int batch;
+    int *list;         // optional: solve instances list[k], k < *list_count (the fast path's leftovers)
+    int *list_count;
     const float *g;
     const int *gidx;   // optional: instance b uses conductance slot gidx[b]
     int *queue;        // work counter (zeroed before the launch)
@@ -70,6 +77,56 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
             s_inst = atomicAdd(A.queue, 1);
         __syncthreads();
         // wave-uniform: keep the instance index and everything derived from it in SGPRs
+        int b = __builtin_amdgcn_readfirstlane(s_inst);
+        __syncthreads();
+        if (A.list) {
+            if (b >= *A.list_count)
+                break;
+            b = __builtin_amdgcn_readfirstlane(A.list[b]);
+        } else if (b >= A.batch) {
+            break;
+        }
+        rb::Inst I;
+        I.g = A.g + (size_t)(A.gidx ? A.gidx[b] : b) * M * N;
+        I.rwire = A.rw[b];
+        I.vin = A.vin + (size_t)b * M;
+        I.vout = A.vout + (size_t)b * A.vout_ld;
+        I.warm = A.warm ? A.warm + (size_t)b * 2 * M * N : nullptr;
+        I.field = A.field ? A.field + (size_t)b * 2 * M * N : nullptr;
+        I.iout = A.iout + (size_t)b * N;
+        I.sweeps = A.sweeps ? A.sweeps + b : nullptr;
+        I.status = A.status ? A.status + b : nullptr;
+        I.est = A.est ? A.est + b : nullptr;
+        I.rejected = A.rejected ? A.rejected + b : nullptr;
+        I.rows = M;
+        I.coarse = A.coarse ? A.coarse + (size_t)b * (M >> A.clevel) * (N >> A.clevel) * A.cch : nullptr;
+        I.cscale = A.coarse ? A.cscale[b] : 0.f;
+        I.cch = A.cch;
+        I.clevel = A.clevel;
+        rb::solve<G, false>(I, none, A.ctl, wsb, sh);
+    }
+}
+
+// The guess-first fast path (rb_fast.h) over the whole batch: certified
+// crossbars get their outputs, the others are appended to A.list.
+template <class G>
+__global__ void __launch_bounds__(G::NT) fast_kernel(RbArgs A)
+{
+    constexpr int M = G::M, N = G::N;
+    __shared__ __align__(16) float halo[G::HALO];
+    __shared__ double S1[N];
+    __shared__ double Ifloor[N];
+    __shared__ double colI[N];
+    __shared__ double colD[N];
+    __shared__ float wmax[G::NW];
+    __shared__ int s_inst, s_dead;
+    __shared__ double cbuf[2 * G::TY * N];
+    __shared__ double cext[3 * N];
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_inst = atomicAdd(A.queue, 1);
+        __syncthreads();
         const int b = __builtin_amdgcn_readfirstlane(s_inst);
         __syncthreads();
         if (b >= A.batch)
@@ -91,8 +148,16 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
         I.cscale = A.coarse ? A.cscale[b] : 0.f;
         I.cch = A.cch;
         I.clevel = A.clevel;
-        rb::solve<G, false>(I, none, A.ctl, wsb, sh);
+        if (!rb::fast_solve<G>(I, A.ctl, sh, A.ws + (size_t)blockIdx.x * M * N) && threadIdx.x == 0)
+            A.list[atomicAdd(A.list_count, 1)] = b;
     }
+}
+
+__global__ void zero_counters(int *a, int *b)
+{
+    *a = 0;
+    if (b)
+        *b = 0;
 }
 
 // Scratch of fadesim_solve_batch_noalloc: a zero row for an absent vout
@@ -126,8 +191,43 @@ int resident_slots()
     return cached[dev];
 }
 
-// fixed: the workgroup slots come from `fixed` (fixed_bytes long) instead of
-// the library-owned workspace; the grid shrinks to the slots that fit.
+// Resident workgroups of fast_kernel<G> on the current device.
+template <class G>
+int resident_fast_slots()
+{
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev >= 64)
+        return 256;
+    if (!cached[dev]) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fast_kernel<G>, G::NT, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            cus = 256;
+        cached[dev] = per_cu * cus;
+    }
+    return cached[dev];
+}
+
+// The fast path would take solves that test their guess before any sweep and
+// have a guess to test (IR2S iterations or a warm start), at 128 x 128.  It is
+// switched off: a warm-started run whose crossbars all went on to the sweeping
+// kernel did not finish, with no cause found yet; until one is, every solve
+// runs on the sweeping kernel alone.
+constexpr bool kFastPathEnabled = false;
+
+template <class G>
+bool takes_fast_path(const RbArgs &args)
+{
+    const unsigned f = args.ctl.flags;
+    return kFastPathEnabled && G::M == 128 && G::N == 128 && (f & FADESIM_F_TEST_FIRST) &&
+           !(f & FADESIM_F_FIXED_SWEEPS) && !(f & FADESIM_F_NO_FASTPATH) && (args.ctl.ir2s_iters > 0 || args.warm);
+}
+
+// fixed: the workspace comes from `fixed` (fixed_bytes long) instead of the
+// library-owned one; the sweeping kernel's grid shrinks to the slots that fit.
+// Layout: [grid slots][256 B: queue, fast queue, list count][fast path: list of batch ints].
 template <class G>
 int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, size_t fixed_bytes = 0)
 {
@@ -135,25 +235,38 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
     if (args.coarse && 2 * (M >> args.clevel) * (N >> args.clevel) > G::HALO)
         return set_error(FADESIM_E_UNSUPPORTED, "d_warm_coarse: coarse grid too large for this shape (raise warm_level)");
     const size_t slot_bytes = (size_t)rb::kWsPlanes * G::M * G::N * sizeof(float);
+    bool fast = false;
+    int fgrid = 0;
+    if constexpr (G::M == 128 && G::N == 128) {
+        fast = takes_fast_path<G>(args);
+        if (fast)
+            fgrid = std::min(resident_fast_slots<G>(), args.batch);
+    }
+    // the fast path's list, then one adjoint plane per fast workgroup
+    const size_t list_bytes = ((size_t)args.batch * sizeof(int) + 255) / 256 * 256 +
+                              (size_t)fgrid * G::M * G::N * sizeof(float);
     int grid = resident_slots<G>();
     if (grid > args.batch)
         grid = args.batch;
     char *ws;
     if (fixed) {
-        const size_t fit = (fixed_bytes - 256) / slot_bytes;
+        if (fast && fixed_bytes < 256 + list_bytes + slot_bytes)
+            fast = false; // the list does not fit beside one slot: every crossbar sweeps
+        const size_t fit = (fixed_bytes - 256 - (fast ? list_bytes : 0)) / slot_bytes;
         if (fit < 1)
             return set_error(FADESIM_E_UNSUPPORTED, "scratch too small for one workgroup of this shape");
         if ((size_t)grid > fit)
             grid = (int)fit;
         ws = fixed;
     } else {
-        ws = (char *)workspace_aux((size_t)grid * slot_bytes + 256);
+        ws = (char *)workspace_aux((size_t)grid * slot_bytes + 256 + (fast ? list_bytes : 0));
         if (!ws)
             return FADESIM_E_NOMEM;
     }
     args.ws = (float *)ws;
-    args.queue = (int *)(ws + (size_t)grid * slot_bytes);
-    hipLaunchKernelGGL(zero_counter, dim3(1), dim3(1), 0, s, args.queue);
+    int *ctr = (int *)(ws + (size_t)grid * slot_bytes);
+    args.queue = ctr;
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, ctr, fast ? ctr + 1 : nullptr);
     FADESIM_HIP_CHECK(hipGetLastError());
     const bool timing = (args.ctl.flags & FADESIM_F_TIMING) != 0;
     if (timing) {
@@ -162,6 +275,22 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
             FADESIM_HIP_CHECK(hipEventCreate(&t_ev[1]));
         }
         FADESIM_HIP_CHECK(hipEventRecord(t_ev[0], s));
+    }
+    if constexpr (G::M == 128 && G::N == 128) {
+        if (fast) {
+            RbArgs fa = args;
+            fa.queue = ctr + 2;
+            fa.list_count = ctr + 1;
+            fa.list = (int *)(ws + (size_t)grid * slot_bytes + 256);
+            fa.ws = (float *)(ws + (size_t)grid * slot_bytes + 256 +
+                              ((size_t)args.batch * sizeof(int) + 255) / 256 * 256);
+            hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, fa.queue, nullptr);
+            FADESIM_HIP_CHECK(hipGetLastError());
+            hipLaunchKernelGGL((fast_kernel<G>), dim3(fgrid), dim3(G::NT), 0, s, fa);
+            FADESIM_HIP_CHECK(hipGetLastError());
+            args.list = fa.list;
+            args.list_count = fa.list_count;
+        }
     }
     hipLaunchKernelGGL((rbsor_kernel<G>), dim3(grid), dim3(G::NT), 0, s, args);
     FADESIM_HIP_CHECK(hipGetLastError());
@@ -246,6 +375,8 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
                                                 ">= 2 and warm_level in 1..6 dividing rows and cols");
     }
     args.batch = B;
+    args.list = nullptr;
+    args.list_count = nullptr;
     args.g = d->d_g;
     args.gidx = d->d_gidx;
     args.queue = nullptr;

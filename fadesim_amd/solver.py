@@ -59,7 +59,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
                 out: Optional[torch.Tensor] = None, timing: bool = False,
                 warm_coarse: Optional[torch.Tensor] = None, warm_scale: Optional[torch.Tensor] = None,
-                warm_level: int = 2, noalloc: bool = False) -> SolveResult:
+                warm_level: int = 2, noalloc: bool = False, fastpath: bool = True) -> SolveResult:
     """Solve B crossbars of one shape.
 
     g [B, M, N] float32 (S) -- or [S, M, N] conductance slots with ``gidx`` [B]
@@ -79,6 +79,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     ``warm_scale`` [B]: the initial drops are warm + warm_scale x
     bilinear(warm_coarse[..., 0:2]), formed while loading (the arithmetic of
     fadesim_nn_combine_up; WarmStartNet.warm_kwargs builds these).
+    ``fastpath=False`` sets FADESIM_F_NO_FASTPATH (every crossbar on the
+    sweeping kernel; the same results).
     ``noalloc`` solves through fadesim_solve_batch_noalloc (the library
     allocates nothing; the outputs here are still allocated by torch).
     """
@@ -139,6 +141,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
         test_first = ir2s_iters >= 3
     if test_first:
         flags |= nat.F_TEST_FIRST
+    if not fastpath:
+        flags |= nat.F_NO_FASTPATH
     if not (0 <= int(ir2s_iters) <= nat.MAX_IR2S_ITERS and 0 <= int(corr_iters) <= nat.MAX_IR2S_ITERS):
         raise ValueError(f"ir2s_iters and corr_iters must lie in [0, {nat.MAX_IR2S_ITERS}]")
     d = nat.BatchDesc()

@@ -214,3 +214,42 @@ def test_ir2s_bad_arguments(cuda):
         _solve(cuda, g, r, vin, ir2s_iters=65)
     with pytest.raises(RuntimeError):
         _solve(cuda, g, r, vin, ir2s_iters=2, ir2s_omega=1.5)
+
+
+@pytest.mark.skip(reason="the guess-first fast path is switched off: its run of the ir2s1_warm case did not "
+                         "finish and no cause has been found (kFastPathEnabled in rbsor.hip)")
+@pytest.mark.parametrize("case", ["ir2s4", "ir2s1_warm", "coarse_warm", "dwr", "rejected", "some_sweep"])
+def test_fast_path_equals_the_sweeping_kernel(cuda, case):
+    """Guess-first solves at 128x128 go through the register-resident fast
+    path (certified crossbars) and the sweeping kernel (the rest): every
+    output equals the sweeping kernel's alone (FADESIM_F_NO_FASTPATH) bit for
+    bit, including crossbars that the first test does not certify."""
+    import torch
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    g, r, vin = config2_instances(200, 128, seed=77)
+    tg, tr, tv = (torch.tensor(a, device=cuda) for a in (g, r, vin))
+    kw = dict(tol=1e-6, ir2s_iters=4, corr_iters=4, test_first=True, want_field=True)
+    if case == "ir2s1_warm":
+        kw.update(warm=solve_batch(tg, tr, tv, tol=1e-3, want_field=True).field * 0.999, ir2s_iters=1)
+    elif case == "coarse_warm":
+        net = warmstart.load_default(cuda)
+        if net is None:
+            pytest.skip("no trained weights committed")
+        kw.update(net.warm_kwargs(tg, tr, tv), ir2s_iters=1)
+    elif case == "dwr":
+        kw.update(dwr=True)
+    elif case == "rejected":
+        kw.update(warm=torch.full((200, 2, 128, 128), 0.05, device=cuda), ir2s_iters=1)
+    elif case == "some_sweep":
+        kw.update(ir2s_iters=1, tol=1e-8)   # one IR2S iteration is not enough for 1e-8: most crossbars sweep
+    a = solve_batch(tg, tr, tv, **kw)
+    b = solve_batch(tg, tr, tv, fastpath=False, **kw)
+    torch.cuda.synchronize()
+    for name in ("iout", "sweeps", "status", "field"):
+        assert torch.equal(getattr(a, name), getattr(b, name)), name
+    assert torch.equal(a.est, b.est)
+    if a.rejected is not None:
+        assert torch.equal(a.rejected, b.rejected)
+    print(case, "sweeps", torch.bincount(a.sweeps).nonzero().ravel().tolist()[:8],
+          "rejected", None if a.rejected is None else int(a.rejected.sum()))
