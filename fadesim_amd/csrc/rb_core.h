@@ -121,10 +121,34 @@ struct StripCtx {
     int *abort;
 };
 
-__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+// Global-memory views of plain pointers.  The phases below are functions of
+// their own that receive their buffers as generic pointers; loads through a
+// generic pointer are FLAT instructions, which count against the LDS wait
+// counter as well, so every wait for an LDS result would also wait for the
+// global loads in flight.  Every buffer passed to the solver is device
+// global memory.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ T ldg(const T *p) { return *gmem(p); }
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4(const float *p)
+{
+    const f32x4 v = ldg(reinterpret_cast<const f32x4 *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
 __device__ __forceinline__ void st4(float *p, float a, float b, float c, float d)
 {
-    *reinterpret_cast<float4 *>(p) = make_float4(a, b, c, d);
+    *gmem(reinterpret_cast<f32x4 *>(p)) = f32x4{a, b, c, d};
 }
 
 // TC consecutive floats of a row (TC a multiple of 4).
@@ -276,7 +300,7 @@ __device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, i
     const unsigned *src = reinterpret_cast<const unsigned *>(I.coarse);
     float2 *st2 = reinterpret_cast<float2 *>(stage);
     for (int k = t; k < HW; k += G::NT) {
-        const unsigned v = src[(size_t)k * (I.cch >> 1)];
+        const unsigned v = ldg(src + (size_t)k * (I.cch >> 1));
         st2[k] = make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
     }
     __syncthreads();
@@ -315,13 +339,13 @@ __device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, i
 // timed out.
 __device__ __forceinline__ bool poll_geq(const int *p, int target, int *abort)
 {
-    if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         const unsigned long long t0 = wall_clock64();
-        while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        while (__hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(gmem(abort), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 return false;
             if (wall_clock64() - t0 > kWaitTicks) {
-                __hip_atomic_store(abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(gmem(abort), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return false;
             }
             __builtin_amdgcn_s_sleep(2);
@@ -349,29 +373,29 @@ __device__ __forceinline__ bool wait_geq(const int *p, int target, int *abort)
 // which do not return stale cached lines.
 __device__ __forceinline__ void team_st(double *p, double v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void team_st(float *p, float v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double team_ld(const double *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ float team_ld(const float *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // A pair of floats as one 8-byte access.
 __device__ __forceinline__ void team_st2(unsigned long long *p, float lo, float hi)
 {
     const unsigned long long v = (unsigned long long)__float_as_uint(lo) | ((unsigned long long)__float_as_uint(hi) << 32);
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void team_ld2(const unsigned long long *p, float &lo, float &hi)
 {
-    const unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long v = __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     lo = __uint_as_float((unsigned)v);
     hi = __uint_as_float((unsigned)(v >> 32));
 }
@@ -395,7 +419,7 @@ __device__ __forceinline__ void strip_publish(const StripCtx &X, const float (&y
             team_st(dst + k, yq[0][k]);
         team_commit();
         if ((lane % G::TX) == 0)
-            __hip_atomic_store(X.flag + 2 * X.s + 0, pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gmem(X.flag + 2 * X.s + 0), pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (ty == G::TY - 1 && X.s < X.S - 1) {
         float *dst = X.rows + (((size_t)par * X.S + X.s) * 2 + 1) * kRowsLd + j0;
@@ -404,7 +428,7 @@ __device__ __forceinline__ void strip_publish(const StripCtx &X, const float (&y
             team_st(dst + k, yq[G::TR - 1][k]);
         team_commit();
         if ((lane % G::TX) == 0)
-            __hip_atomic_store(X.flag + 2 * X.s + 1, pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gmem(X.flag + 2 * X.s + 1), pub + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -523,7 +547,7 @@ __device__ __forceinline__ bool team_barrier(const StripCtx &X, int *s_dead, int
     team_commit();
     __syncthreads();
     if (t == 0) {
-        __hip_atomic_fetch_add(X.cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gmem(X.cnt), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!poll_geq(X.cnt, X.S * (nbar + 1), X.abort))
             *s_dead = 1;
     }
@@ -539,7 +563,7 @@ __device__ __forceinline__ void team_arrive(const StripCtx &X, int t)
     team_commit();
     __syncthreads();
     if (t == 0)
-        __hip_atomic_fetch_add(X.cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gmem(X.cnt), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ bool team_wait(const StripCtx &X, int *s_dead, int t, int nbar)
 {
@@ -571,10 +595,10 @@ __device__ __forceinline__ void load_coeffs(const Inst &I, const float *wsb, con
             ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rp[ii]);
             ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rq[ii]);
         } else {
-            const double vi = I.vin[gi0 + i0 + ii];
+            const double vi = ldg(I.vin + (gi0 + i0 + ii));
 #pragma unroll
             for (int jj = 0; jj < TC; ++jj) {
-                const double c = vi - I.vout[j0 + jj];
+                const double c = vi - ldg(I.vout + (j0 + jj));
                 rp[ii][jj] = rq[ii][jj] = (float)(((double)g[jj] * c) * hcoef);
             }
         }
@@ -952,10 +976,10 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
         si[jj] = sf[jj] = 0.0;
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
-        const double vi = I.vin[gi0 + i0 + ii];
+        const double vi = ldg(I.vin + (gi0 + i0 + ii));
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj) {
-            const double c = vi - I.vout[j0 + jj];
+            const double c = vi - ldg(I.vout + (j0 + jj));
             si[jj] += (double)gt[ii][jj] * c;
             sf[jj] += (double)gt[ii][jj] * fabs(c);
         }
@@ -1032,7 +1056,7 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
         float zrow[TC];
         if (!want_rho)
             ldrow<TC>(wsb + (size_t)(i0 + ii) * N + j0, zrow);
-        const double vi = I.vin[gi0 + i0 + ii];
+        const double vi = ldg(I.vin + (gi0 + i0 + ii));
         float rpf[TC], rqf[TC]; // residuals of the row (FP64 arithmetic, FP32 storage)
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj) {
@@ -1043,7 +1067,7 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
                 bp = (double)rhp[jj] / hcoef;
                 bq = (double)rhq[jj] / hcoef;
             } else {
-                bp = bq = gd * (vi - I.vout[j0 + jj]);
+                bp = bq = gd * (vi - ldg(I.vout + (j0 + jj)));
             }
             double pl = 0.0;
             if (jj > 0)
@@ -1372,11 +1396,11 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
 #pragma unroll
     for (int ii = 0; ii < G::TR; ++ii) {
         ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
-        vi[ii] = (float)I.vin[gi0 + i0 + ii];
+        vi[ii] = (float)ldg(I.vin + (gi0 + i0 + ii));
     }
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj)
-        vo[jj] = (float)I.vout[j0 + jj];
+        vo[jj] = (float)ldg(I.vout + (j0 + jj));
     // s_ij = g_ij (c_ij - p_ij - q_ij) for both line solves
     auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&b)[TC]) {
 #pragma unroll
