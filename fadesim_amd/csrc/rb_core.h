@@ -186,6 +186,9 @@ struct Geo {
     static constexpr int QLD = NT + 2 * TX;
     static constexpr int PL = 0, PR = TR * NT, QT = 2 * TR * NT, QB = 2 * TR * NT + TC * QLD;
     static constexpr int HALO = 2 * TR * NT + 2 * TC * QLD;
+    // true: the IR2S iterations reload the conductance row by row instead of
+    // holding the tile in registers (lower register pressure)
+    static constexpr bool kLean = false;
     static_assert(TR % 2 == 0 && TC % 4 == 0, "tile sides: TR even, TC multiple of 4");
     static_assert(NT % 64 == 0 && 64 % TX == 0 || TX % 64 == 0, "tile rows must not straddle waves");
     static_assert(NT >= M && NT >= N, "one thread per row and per column in the test");
@@ -1390,12 +1393,14 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
     } else {
         tile_load<G>(wsb, i0, j0, p, q);
     }
-    // the tile's conductances and line voltages stay in registers for all
-    // iterations: an iteration reads no memory but LDS
-    float gt[G::TR][TC], vi[G::TR], vo[TC];
+    // the tile's conductances (lean geometries: only the line voltages) stay
+    // in registers for all iterations: an iteration reads no memory but LDS
+    constexpr bool kHold = !G::kLean;
+    float gt[kHold ? G::TR : 1][TC], vi[G::TR], vo[TC];
 #pragma unroll
     for (int ii = 0; ii < G::TR; ++ii) {
-        ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
+        if constexpr (kHold)
+            ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
         vi[ii] = (float)ldg(I.vin + (gi0 + i0 + ii));
     }
 #pragma unroll
@@ -1403,10 +1408,19 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
         vo[jj] = (float)ldg(I.vout + (j0 + jj));
     // s_ij = g_ij (c_ij - p_ij - q_ij) for both line solves
     auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&b)[TC]) {
+        float gr[TC];
+        if constexpr (kHold) {
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                gr[jj] = gt[ii][jj];
+        } else {
+            ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gr);
+        }
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj)
-            a[jj] = b[jj] = gt[ii][jj] * (((vi[ii] - vo[jj]) - pr[jj]) - qr[jj]);
+            a[jj] = b[jj] = gr[jj] * (((vi[ii] - vo[jj]) - pr[jj]) - qr[jj]);
     };
+    iters = uniform_i32(iters); // a loop with barriers: its trip count a scalar
 #pragma unroll 1
     for (int k = 0; k < iters; ++k) {
         // the cell voltages vi - vo are recomputed in every iteration rather
@@ -1462,6 +1476,7 @@ __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float 
             b[jj] = fmaf(rq[jj], ih, -cpl);
         }
     };
+    iters = uniform_i32(iters); // a loop with barriers: its trip count a scalar
 #pragma unroll 1
     for (int k = 0; k < iters; ++k)
         if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, *nbar, p, q))
@@ -1587,8 +1602,10 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             }
             snap_valid = true;
         }
-        est = phase_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, false, has_l,
-                                   has_r, has_u, has_d, RB_SH_ARGS(sh), pub - 1, &nbar);
+        // the same in every lane, but returned in VGPRs: a scalar keeps the
+        // decisions below uniform branches
+        est = uniform_f64(phase_test<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, refined, false,
+                                               has_l, has_r, has_u, has_d, RB_SH_ARGS(sh), pub - 1, &nbar));
         if (est < 0.0)
             return false;
         if (judge_guess) {
@@ -1729,6 +1746,137 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             }
             strow<TC>(P, pv);
             strow<TC>(Q, qv);
+        }
+    }
+    __syncthreads(); // colI / halo reads done before the next instance reuses them
+    return true;
+}
+
+// A geometry of its own type for the guess-first kernel: the phase functions
+// instantiated for it are called only from that kernel (and compiled for its
+// register needs), and its IR2S iterations reload the conductance.
+template <class G>
+struct GuessGeo : G {
+    static constexpr bool kLean = true;
+};
+
+// The part of solve() before its first sweep, for whole crossbars whose
+// guess is tested first (FADESIM_F_TEST_FIRST, not FADESIM_F_FIXED_SWEEPS):
+// the set-up, the initial guess (warm start, coarse combine, IR2S
+// iterations), its test, and -- when a given guess is judged worse than the
+// zero guess -- the zero guess with its IR2S iterations and their test.  The
+// same phase functions in the same order as solve(), so whatever this writes
+// equals what solve() writes bit for bit.  Returns true when a test ended the
+// solve before any sweep (certified, or not finite), with every output
+// written as solve() writes it; false when the solve needs sweeps (no output
+// written; solve() then runs the crossbar from the start).
+template <class G>
+__device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *wsb, const Shared &sh)
+{
+    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY;
+    constexpr int M = G::M, N = G::N;
+    float *halo = sh.halo;
+    const StripCtx X{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const int t = threadIdx.x;
+    const int tx = t % TX, ty = t / TX;
+    const int gi0 = 0;
+    const float *gp = I.g;
+    const int i0 = TR * ty, j0 = TC * tx;
+    const bool has_l = tx > 0, has_r = tx < TX - 1, has_u = ty > 0, has_d = ty < TY - 1;
+    const double rwire = I.rwire;
+    const double a = 1.0 / rwire;
+    const double hcoef = 0.5 * (double)C.omega * rwire;
+    const bool dwr = (C.flags & FADESIM_F_DWR) != 0;
+    int nbar = 0;
+    if (t == 0)
+        *sh.dead = 0;
+    __syncthreads();
+
+    phase_setup<G, false>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, RB_SH_ARGS(sh), &nbar);
+
+    float yp[TR][TC], yq[TR][TC];
+    const bool coarse = I.coarse != nullptr;
+    if (coarse) {
+        phase_coarse<G, false>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
+        if (C.ir2s_iters == 0)
+            tile_load<G>(wsb, i0, j0, yp, yq);
+    } else {
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            if (I.warm) {
+                ldrow<TC>(I.warm + (size_t)(i0 + ii) * N + j0, yp[ii]);
+                ldrow<TC>(I.warm + (size_t)M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < TC; ++jj)
+                    yp[ii][jj] = yq[ii][jj] = 0.f;
+            }
+        }
+    }
+    if (C.ir2s_iters > 0) {
+        const bool from_zero = !I.warm;
+        if (!from_zero && !coarse)
+            tile_store<G>(wsb, i0, j0, yp, yq);
+        phase_ir2s<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
+                             C.ir2s_omega, from_zero);
+        tile_load<G>(wsb, i0, j0, yp, yq);
+    }
+    publish_all<G>(yp, yq, halo, t);
+    __syncthreads();
+    if (C.ir2s_iters == 0)
+        tile_store<G>(wsb, i0, j0, yp, yq); // the snapshot the test reads
+    // the test's value is the same in every lane but comes back from the call
+    // in VGPRs: made a scalar, every decision below is a uniform branch (a
+    // branch the compiler takes as divergent lets it split lanes of one wave
+    // across the caller's loop and its barriers)
+    double est = uniform_f64(phase_test<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false,
+                                                  false, has_l, has_r, has_u, has_d, RB_SH_ARGS(sh), -1, &nbar));
+    bool rejected = false;
+    if ((I.warm != nullptr || coarse) && !(est <= zero_guess_est<G>(sh, t))) {
+        rejected = true;
+        if (C.ir2s_iters > 0) {
+            phase_ir2s<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
+                                 C.ir2s_iters, C.ir2s_omega, true);
+            tile_load<G>(wsb, i0, j0, yp, yq);
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < TR; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < TC; ++jj)
+                    yp[ii][jj] = yq[ii][jj] = 0.f;
+            tile_store<G>(wsb, i0, j0, yp, yq);
+        }
+        publish_all<G>(yp, yq, halo, t);
+        __syncthreads();
+        est = uniform_f64(phase_test<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false, false,
+                                               has_l, has_r, has_u, has_d, RB_SH_ARGS(sh), -1, &nbar));
+    }
+    const bool nonfinite = !(est < INFINITY);
+    if (!nonfinite && !(est <= (dwr ? C.tol_eff * C.dwr_scale : C.tol_eff)))
+        return false;
+    const int st = nonfinite ? FADESIM_ST_NONFINITE : FADESIM_ST_CONVERGED;
+    if (t < N) {
+        double Iv = sh.colI[t];
+        if (dwr && st == FADESIM_ST_CONVERGED)
+            Iv -= sh.colD[t];
+        I.iout[t] = Iv;
+    }
+    if (t == 0) {
+        if (I.sweeps)
+            *I.sweeps = 0;
+        if (I.status)
+            *I.status = st;
+        if (I.est)
+            *I.est = est;
+        if (I.rejected)
+            *I.rejected = rejected ? 1 : 0;
+    }
+    if (I.field) {
+        tile_load<G>(wsb, i0, j0, yp, yq); // the tested field (snapshot planes)
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            strow<TC>(I.field + (size_t)(i0 + ii) * N + j0, yp[ii]);
+            strow<TC>(I.field + (size_t)M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
         }
     }
     __syncthreads(); // colI / halo reads done before the next instance reuses them

@@ -2,9 +2,10 @@
 // work-queue kernel over the red-black SOR core of rb_core.h (FP32 sweeps on
 // drop variables, FP64 adjoint-weighted residual test, one FP64 refinement
 // step, FP64 readout), one workgroup per crossbar.  Solves that test their
-// guess before sweeping go first through the register-resident fast path of
-// rb_fast.h (fast_kernel); the sweeping kernel then takes only the crossbars
-// that path did not certify, from a device-side list.
+// guess before sweeping go first through guess_kernel, which runs the part of
+// the solve before the first sweep without the sweep loop's register state
+// and marks the crossbars its test certified; the sweeping kernel then solves
+// only the others.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,15 +17,13 @@
 #include "common.h"
 #include "fadesim.h"
 #include "rb_core.h"
-#include "rb_fast.h"
 
 namespace fadesim {
 namespace {
 
 struct RbArgs {
     int batch;
-    int *list;         // optional: solve instances list[k], k < *list_count (the fast path's leftovers)
-    int *list_count;
+    int *done;         // optional [batch]: 1 = solved by guess_kernel (skip), written by it for every instance
     const float *g;
     const int *gidx;   // optional: instance b uses conductance slot gidx[b]
     int *queue;        // work counter (zeroed before the launch)
@@ -52,8 +51,6 @@ __global__ void fill_zero_f64(double *p, size_t n)
         p[i] = 0.0;
 }
 
-__global__ void zero_counter(int *c) { *c = 0; }
-
 // Persistent work queue: each resident workgroup takes instances until the
 // batch is exhausted.
 template <class G>
@@ -77,15 +74,12 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
             s_inst = atomicAdd(A.queue, 1);
         __syncthreads();
         // wave-uniform: keep the instance index and everything derived from it in SGPRs
-        int b = __builtin_amdgcn_readfirstlane(s_inst);
+        const int b = __builtin_amdgcn_readfirstlane(s_inst);
         __syncthreads();
-        if (A.list) {
-            if (b >= *A.list_count)
-                break;
-            b = __builtin_amdgcn_readfirstlane(A.list[b]);
-        } else if (b >= A.batch) {
+        if (b >= A.batch)
             break;
-        }
+        if (A.done && __builtin_amdgcn_readfirstlane(rb::ldg(A.done + b)))
+            continue;
         rb::Inst I;
         I.g = A.g + (size_t)(A.gidx ? A.gidx[b] : b) * M * N;
         I.rwire = A.rw[b];
@@ -107,11 +101,17 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
     }
 }
 
-// The guess-first fast path (rb_fast.h) over the whole batch: certified
-// crossbars get their outputs, the others are appended to A.list.
+// The guess-first part of every solve of the batch (rb::guess_solve), one
+// workgroup per crossbar as in rbsor_kernel.  It carries none of the sweep
+// loop's state, and its phases are instantiated for it alone (GuessGeo: the
+// IR2S iterations reload the conductance row by row instead of holding the
+// tile), so set-up, guess and test cost less here than inside rbsor_kernel.
+// done[b] = 1 for each crossbar whose solve ended at a test before any sweep
+// (outputs written), else 0.
 template <class G>
-__global__ void __launch_bounds__(G::NT) fast_kernel(RbArgs A)
+__global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
 {
+    using GG = rb::GuessGeo<G>;
     constexpr int M = G::M, N = G::N;
     __shared__ __align__(16) float halo[G::HALO];
     __shared__ double S1[N];
@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(G::NT) fast_kernel(RbArgs A)
     __shared__ double cbuf[2 * G::TY * N];
     __shared__ double cext[3 * N];
     rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     for (;;) {
         if (threadIdx.x == 0)
             s_inst = atomicAdd(A.queue, 1);
@@ -148,8 +149,14 @@ __global__ void __launch_bounds__(G::NT) fast_kernel(RbArgs A)
         I.cscale = A.coarse ? A.cscale[b] : 0.f;
         I.cch = A.cch;
         I.clevel = A.clevel;
-        if (!rb::fast_solve<G>(I, A.ctl, sh, A.ws + (size_t)blockIdx.x * M * N) && threadIdx.x == 0)
-            A.list[atomicAdd(A.list_count, 1)] = b;
+        const bool done = rb::guess_solve<GG>(I, A.ctl, wsb, sh);
+        if (threadIdx.x == 0)
+            A.done[b] = done ? 1 : 0;
+        // a barrier between this thread-0 store and the next iteration's
+        // thread-0 queue update: without one the compiler joins the two into
+        // one path for thread 0 around the loop's barriers, so the other lanes
+        // of its wave would read s_inst before thread 0 has updated it
+        __syncthreads();
     }
 }
 
@@ -172,8 +179,9 @@ static __device__ __align__(256) char scratch[kScratchBytes];
 thread_local hipEvent_t t_ev[2] = {nullptr, nullptr};
 thread_local bool t_ev_valid = false;
 
-// Resident workgroups of rbsor_kernel<G> on the current device (persistent grid).
-template <class G>
+// Resident workgroups of rbsor_kernel<G> (GUESS: guess_kernel<G>) on the
+// current device: the persistent grid.
+template <class G, bool GUESS>
 int resident_slots()
 {
     static int cached[64] = {0};
@@ -182,7 +190,12 @@ int resident_slots()
         return 256;
     if (!cached[dev]) {
         int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rbsor_kernel<G>, G::NT, 0) != hipSuccess || per_cu < 1)
+        hipError_t e;
+        if constexpr (GUESS)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, guess_kernel<G>, G::NT, 0);
+        else
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rbsor_kernel<G>, G::NT, 0);
+        if (e != hipSuccess || per_cu < 1)
             per_cu = 1;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
             cus = 256;
@@ -191,43 +204,20 @@ int resident_slots()
     return cached[dev];
 }
 
-// Resident workgroups of fast_kernel<G> on the current device.
-template <class G>
-int resident_fast_slots()
-{
-    static int cached[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev >= 64)
-        return 256;
-    if (!cached[dev]) {
-        int per_cu = 0, cus = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fast_kernel<G>, G::NT, 0) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
-            cus = 256;
-        cached[dev] = per_cu * cus;
-    }
-    return cached[dev];
-}
-
-// The fast path would take solves that test their guess before any sweep and
-// have a guess to test (IR2S iterations or a warm start), at 128 x 128.  It is
-// switched off: a warm-started run whose crossbars all went on to the sweeping
-// kernel did not finish, with no cause found yet; until one is, every solve
-// runs on the sweeping kernel alone.
-constexpr bool kFastPathEnabled = false;
-
-template <class G>
-bool takes_fast_path(const RbArgs &args)
+// Solves whose guess is tested before any sweep go through guess_kernel
+// first: they test a guess (IR2S iterations, a warm start or a coarse
+// combine), and no fixed sweep count bypasses the test.
+bool takes_guess_path(const RbArgs &args)
 {
     const unsigned f = args.ctl.flags;
-    return kFastPathEnabled && G::M == 128 && G::N == 128 && (f & FADESIM_F_TEST_FIRST) &&
-           !(f & FADESIM_F_FIXED_SWEEPS) && !(f & FADESIM_F_NO_FASTPATH) && (args.ctl.ir2s_iters > 0 || args.warm);
+    return (f & FADESIM_F_TEST_FIRST) && !(f & FADESIM_F_FIXED_SWEEPS) && !(f & FADESIM_F_NO_FASTPATH) &&
+           (args.ctl.ir2s_iters > 0 || args.warm || args.coarse);
 }
 
 // fixed: the workspace comes from `fixed` (fixed_bytes long) instead of the
-// library-owned one; the sweeping kernel's grid shrinks to the slots that fit.
-// Layout: [grid slots][256 B: queue, fast queue, list count][fast path: list of batch ints].
+// library-owned one; the grids shrink to the slots that fit.
+// Layout: [slots][256 B: sweeping queue, guess queue][guess path: done flags, batch ints].
+// The two kernels run one after the other on the stream and share the slots.
 template <class G>
 int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, size_t fixed_bytes = 0)
 {
@@ -235,38 +225,30 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
     if (args.coarse && 2 * (M >> args.clevel) * (N >> args.clevel) > G::HALO)
         return set_error(FADESIM_E_UNSUPPORTED, "d_warm_coarse: coarse grid too large for this shape (raise warm_level)");
     const size_t slot_bytes = (size_t)rb::kWsPlanes * G::M * G::N * sizeof(float);
-    bool fast = false;
-    int fgrid = 0;
-    if constexpr (G::M == 128 && G::N == 128) {
-        fast = takes_fast_path<G>(args);
-        if (fast)
-            fgrid = std::min(resident_fast_slots<G>(), args.batch);
-    }
-    // the fast path's list, then one adjoint plane per fast workgroup
-    const size_t list_bytes = ((size_t)args.batch * sizeof(int) + 255) / 256 * 256 +
-                              (size_t)fgrid * G::M * G::N * sizeof(float);
-    int grid = resident_slots<G>();
-    if (grid > args.batch)
-        grid = args.batch;
+    const size_t done_bytes = ((size_t)args.batch * sizeof(int) + 255) / 256 * 256;
+    bool guess = takes_guess_path(args);
+    int grid = std::min(resident_slots<G, false>(), args.batch);
+    int ggrid = guess ? std::min(resident_slots<G, true>(), args.batch) : 0;
     char *ws;
     if (fixed) {
-        if (fast && fixed_bytes < 256 + list_bytes + slot_bytes)
-            fast = false; // the list does not fit beside one slot: every crossbar sweeps
-        const size_t fit = (fixed_bytes - 256 - (fast ? list_bytes : 0)) / slot_bytes;
+        if (guess && fixed_bytes < 256 + done_bytes + slot_bytes)
+            guess = false; // the flags do not fit beside one slot: every crossbar goes to the sweeping kernel
+        const size_t fit = (fixed_bytes - 256 - (guess ? done_bytes : 0)) / slot_bytes;
         if (fit < 1)
             return set_error(FADESIM_E_UNSUPPORTED, "scratch too small for one workgroup of this shape");
-        if ((size_t)grid > fit)
-            grid = (int)fit;
+        grid = (int)std::min((size_t)grid, fit);
+        ggrid = (int)std::min((size_t)ggrid, fit);
         ws = fixed;
     } else {
-        ws = (char *)workspace_aux((size_t)grid * slot_bytes + 256 + (fast ? list_bytes : 0));
+        ws = (char *)workspace_aux((size_t)std::max(grid, ggrid) * slot_bytes + 256 + (guess ? done_bytes : 0));
         if (!ws)
             return FADESIM_E_NOMEM;
     }
+    const size_t slots_bytes = (size_t)std::max(grid, ggrid) * slot_bytes;
     args.ws = (float *)ws;
-    int *ctr = (int *)(ws + (size_t)grid * slot_bytes);
+    int *ctr = (int *)(ws + slots_bytes);
     args.queue = ctr;
-    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, ctr, fast ? ctr + 1 : nullptr);
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, ctr, guess ? ctr + 1 : nullptr);
     FADESIM_HIP_CHECK(hipGetLastError());
     const bool timing = (args.ctl.flags & FADESIM_F_TIMING) != 0;
     if (timing) {
@@ -276,21 +258,13 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
         }
         FADESIM_HIP_CHECK(hipEventRecord(t_ev[0], s));
     }
-    if constexpr (G::M == 128 && G::N == 128) {
-        if (fast) {
-            RbArgs fa = args;
-            fa.queue = ctr + 2;
-            fa.list_count = ctr + 1;
-            fa.list = (int *)(ws + (size_t)grid * slot_bytes + 256);
-            fa.ws = (float *)(ws + (size_t)grid * slot_bytes + 256 +
-                              ((size_t)args.batch * sizeof(int) + 255) / 256 * 256);
-            hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, fa.queue, nullptr);
-            FADESIM_HIP_CHECK(hipGetLastError());
-            hipLaunchKernelGGL((fast_kernel<G>), dim3(fgrid), dim3(G::NT), 0, s, fa);
-            FADESIM_HIP_CHECK(hipGetLastError());
-            args.list = fa.list;
-            args.list_count = fa.list_count;
-        }
+    if (guess) {
+        RbArgs ga = args;
+        ga.queue = ctr + 1;
+        ga.done = (int *)(ws + slots_bytes + 256);
+        hipLaunchKernelGGL((guess_kernel<G>), dim3(ggrid), dim3(G::NT), 0, s, ga);
+        FADESIM_HIP_CHECK(hipGetLastError());
+        args.done = ga.done;
     }
     hipLaunchKernelGGL((rbsor_kernel<G>), dim3(grid), dim3(G::NT), 0, s, args);
     FADESIM_HIP_CHECK(hipGetLastError());
@@ -375,8 +349,7 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
                                                 ">= 2 and warm_level in 1..6 dividing rows and cols");
     }
     args.batch = B;
-    args.list = nullptr;
-    args.list_count = nullptr;
+    args.done = nullptr;
     args.g = d->d_g;
     args.gidx = d->d_gidx;
     args.queue = nullptr;

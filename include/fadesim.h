@@ -45,9 +45,11 @@ extern "C" {
                                         * the wire Green's-function (IR2S) estimate */
 #define FADESIM_F_TEST_FIRST 32u  /* test the initial guess (and the refinement correction's
                                    * initial guess) before sweeping: for accurate guesses */
-#define FADESIM_F_NO_FASTPATH 64u /* solve every crossbar on the sweeping kernel, never on the
-                                   * register-resident guess-first fast path (which is switched off
-                                   * in this build, so the flag changes nothing at present) */
+#define FADESIM_F_NO_FASTPATH 64u /* solve every crossbar on the sweeping kernel alone (by default a
+                                   * solve with FADESIM_F_TEST_FIRST and a guess to test runs its
+                                   * steps up to the first sweep in a kernel of twice the occupancy,
+                                   * and only the crossbars that test does not certify reach the
+                                   * sweeping kernel; the outputs are the same bit for bit) */
 #define FADESIM_MAX_IR2S_ITERS 64
 
 const char *fadesim_last_error(void);

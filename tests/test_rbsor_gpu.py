@@ -216,14 +216,12 @@ def test_ir2s_bad_arguments(cuda):
         _solve(cuda, g, r, vin, ir2s_iters=2, ir2s_omega=1.5)
 
 
-@pytest.mark.skip(reason="the guess-first fast path is switched off: its run of the ir2s1_warm case did not "
-                         "finish and no cause has been found (kFastPathEnabled in rbsor.hip)")
-@pytest.mark.parametrize("case", ["ir2s4", "ir2s1_warm", "coarse_warm", "dwr", "rejected", "some_sweep"])
-def test_fast_path_equals_the_sweeping_kernel(cuda, case):
-    """Guess-first solves at 128x128 go through the register-resident fast
-    path (certified crossbars) and the sweeping kernel (the rest): every
-    output equals the sweeping kernel's alone (FADESIM_F_NO_FASTPATH) bit for
-    bit, including crossbars that the first test does not certify."""
+@pytest.mark.parametrize("case", ["ir2s4", "some_sweep", "dwr", "rejected", "coarse_warm", "ir2s1_warm"])
+def test_guess_kernel_equals_the_sweeping_kernel(cuda, case):
+    """Guess-first solves go through guess_kernel (the crossbars its test
+    certifies) and the sweeping kernel (the rest): every output equals the
+    sweeping kernel's alone (FADESIM_F_NO_FASTPATH) bit for bit, including
+    crossbars whose first test does not certify them and rejected guesses."""
     import torch
     from fadesim_amd import warmstart
     from fadesim_amd.solver import solve_batch
