@@ -710,18 +710,36 @@ __device__ __forceinline__ void row_green_row_f(const float (&v)[G::TC], int j0,
     }
 }
 
+// Per-column values in LDS (S1, Ifloor, colI, colD, cext and the rows of
+// cbuf) are kept in slot order: column j0 + jj of thread column tx sits in
+// slot jj * TX + tx, so the TX threads of a thread row write one jj at
+// consecutive addresses (no LDS bank conflicts), and the thread t < N that
+// scans a column handles slot t, i.e. column col_of_slot(t).  The order of
+// every sum is unchanged.
+template <class G>
+__device__ __forceinline__ int slot_of(int tx, int jj)
+{
+    return jj * G::TX + tx;
+}
+template <class G>
+__device__ __forceinline__ int col_of_slot(int s)
+{
+    return (s % G::TX) * G::TC + s / G::TX;
+}
+
 // Column sums over the workgroup's rows of two per-thread partials (thread
-// row ty, columns j0 + jj): outA[j], outB[j] (LDS, N entries each).
+// row ty, columns j0 + jj): outA, outB (LDS, N entries each, slot order).
 template <class G>
 __device__ __forceinline__ void col_reduce2(const Shared &sh, const double (&a)[G::TC], const double (&b)[G::TC], int t,
                                             int ty, int j0, double *outA, double *outB)
 {
     constexpr int TC = G::TC, TY = G::TY, N = G::N;
     double *ba = sh.cbuf, *bb = sh.cbuf + TY * N;
+    const int tx = j0 / TC;
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
-        ba[ty * N + j0 + jj] = a[jj];
-        bb[ty * N + j0 + jj] = b[jj];
+        ba[ty * N + slot_of<G>(tx, jj)] = a[jj];
+        bb[ty * N + slot_of<G>(tx, jj)] = b[jj];
     }
     __syncthreads();
     if (t < N) {
@@ -826,6 +844,7 @@ __device__ __forceinline__ bool col_green_tile_f(const Shared &sh, const StripCt
 {
     constexpr int TR = G::TR, TC = G::TC, TY = G::TY, N = G::N;
     double *b0 = sh.cbuf, *b1 = sh.cbuf + TY * N;
+    const int tx = j0 / TC;
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
         float s0 = 0.f, s1 = 0.f;
@@ -834,16 +853,16 @@ __device__ __forceinline__ bool col_green_tile_f(const Shared &sh, const StripCt
             s0 += v[ii][jj];
             s1 += v[ii][jj] * (float)(n - (gi0 + i0 + ii));
         }
-        b0[ty * N + j0 + jj] = (double)s0;
-        b1[ty * N + j0 + jj] = (double)s1;
+        b0[ty * N + slot_of<G>(tx, jj)] = (double)s0;
+        b1[ty * N + slot_of<G>(tx, jj)] = (double)s1;
     }
     if (!col_scan_exchange<G, STRIP>(sh, X, t, nbar))
         return false;
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
-        const int j = j0 + jj;
-        const float A0 = (float)(sh.cext[j] + b0[ty * N + j]);
-        const float Tb0 = (float)(sh.cext[2 * N + j] - (sh.cext[N + j] + b1[ty * N + j]));
+        const int sj = slot_of<G>(tx, jj);
+        const float A0 = (float)(sh.cext[sj] + b0[ty * N + sj]);
+        const float Tb0 = (float)(sh.cext[2 * N + sj] - (sh.cext[N + sj] + b1[ty * N + sj]));
         float al = 0.f, cl = 0.f;
 #pragma unroll
         for (int ii = 0; ii < TR; ++ii) {
@@ -896,8 +915,8 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
         }
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj) {
-            b0[ty * N + j0 + jj] = (double)s0[jj];
-            b1[ty * N + j0 + jj] = (double)s1[jj];
+            b0[ty * N + slot_of<G>(tx, jj)] = (double)s0[jj];
+            b1[ty * N + slot_of<G>(tx, jj)] = (double)s1[jj];
         }
     }
     const double T1 = col_scan_begin<G, STRIP, true>(sh, X, t, nbar);
@@ -924,10 +943,11 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
     float A0[TC], Tb0[TC], al[TC], cl[TC];
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
-        const double A = sh.cext[j0 + jj] + b0[ty * N + j0 + jj];
-        const double C = sh.cext[N + j0 + jj] + b1[ty * N + j0 + jj];
+        const int sj = slot_of<G>(tx, jj);
+        const double A = sh.cext[sj] + b0[ty * N + sj];
+        const double C = sh.cext[N + sj] + b1[ty * N + sj];
         A0[jj] = (float)A;
-        Tb0[jj] = (float)(sh.cext[2 * N + j0 + jj] - C);
+        Tb0[jj] = (float)(sh.cext[2 * N + sj] - C);
         al[jj] = cl[jj] = 0.f;
     }
 #pragma unroll
@@ -1716,7 +1736,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
             double Iv = sh.colI[t];
             if (dwr && st == FADESIM_ST_CONVERGED)
                 Iv -= sh.colD[t];
-            I.iout[t] = Iv;
+            I.iout[col_of_slot<G>(t)] = Iv;
         }
         if (t == 0) {
             if (I.sweeps)
@@ -1859,7 +1879,7 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
         double Iv = sh.colI[t];
         if (dwr && st == FADESIM_ST_CONVERGED)
             Iv -= sh.colD[t];
-        I.iout[t] = Iv;
+        I.iout[col_of_slot<G>(t)] = Iv;
     }
     if (t == 0) {
         if (I.sweeps)
