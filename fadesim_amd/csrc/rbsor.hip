@@ -23,7 +23,8 @@ namespace {
 
 struct RbArgs {
     int batch;
-    int *done;         // optional [batch]: 1 = solved by guess_kernel (skip), written by it for every instance
+    int *list;         // optional: solve only instances list[k], k < *list_count (those guess_kernel
+    int *list_count;   // did not finish); guess_kernel appends to them
     const float *g;
     const int *gidx;   // optional: instance b uses conductance slot gidx[b]
     int *queue;        // work counter (zeroed before the launch)
@@ -74,12 +75,15 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
             s_inst = atomicAdd(A.queue, 1);
         __syncthreads();
         // wave-uniform: keep the instance index and everything derived from it in SGPRs
-        const int b = __builtin_amdgcn_readfirstlane(s_inst);
+        int b = __builtin_amdgcn_readfirstlane(s_inst);
         __syncthreads();
-        if (b >= A.batch)
+        if (A.list) {
+            if (b >= __builtin_amdgcn_readfirstlane(rb::ldg(A.list_count)))
+                break;
+            b = __builtin_amdgcn_readfirstlane(rb::ldg(A.list + b));
+        } else if (b >= A.batch) {
             break;
-        if (A.done && __builtin_amdgcn_readfirstlane(rb::ldg(A.done + b)))
-            continue;
+        }
         rb::Inst I;
         I.g = A.g + (size_t)(A.gidx ? A.gidx[b] : b) * M * N;
         I.rwire = A.rw[b];
@@ -106,8 +110,8 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
 // loop's state, and its phases are instantiated for it alone (GuessGeo: the
 // IR2S iterations reload the conductance row by row instead of holding the
 // tile), so set-up, guess and test cost less here than inside rbsor_kernel.
-// done[b] = 1 for each crossbar whose solve ended at a test before any sweep
-// (outputs written), else 0.
+// A crossbar whose solve ended at a test before any sweep has its outputs
+// written; every other one is appended to list (list_count entries).
 template <class G>
 __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
 {
@@ -150,9 +154,9 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
         I.cch = A.cch;
         I.clevel = A.clevel;
         const bool done = rb::guess_solve<GG>(I, A.ctl, wsb, sh);
-        if (threadIdx.x == 0)
-            A.done[b] = done ? 1 : 0;
-        // a barrier between this thread-0 store and the next iteration's
+        if (!done && threadIdx.x == 0)
+            A.list[atomicAdd(A.list_count, 1)] = b;
+        // a barrier between this thread-0 append and the next iteration's
         // thread-0 queue update: without one the compiler joins the two into
         // one path for thread 0 around the loop's barriers, so the other lanes
         // of its wave would read s_inst before thread 0 has updated it
@@ -160,11 +164,10 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
     }
 }
 
-__global__ void zero_counters(int *a, int *b)
+__global__ void zero_counters(int *a, int n)
 {
-    *a = 0;
-    if (b)
-        *b = 0;
+    for (int k = 0; k < n; ++k)
+        a[k] = 0;
 }
 
 // Scratch of fadesim_solve_batch_noalloc: a zero row for an absent vout
@@ -216,7 +219,7 @@ bool takes_guess_path(const RbArgs &args)
 
 // fixed: the workspace comes from `fixed` (fixed_bytes long) instead of the
 // library-owned one; the grids shrink to the slots that fit.
-// Layout: [slots][256 B: sweeping queue, guess queue][guess path: done flags, batch ints].
+// Layout: [slots][256 B: sweeping queue, guess queue, list count][guess path: list, batch ints].
 // The two kernels run one after the other on the stream and share the slots.
 template <class G>
 int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, size_t fixed_bytes = 0)
@@ -225,22 +228,22 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
     if (args.coarse && 2 * (M >> args.clevel) * (N >> args.clevel) > G::HALO)
         return set_error(FADESIM_E_UNSUPPORTED, "d_warm_coarse: coarse grid too large for this shape (raise warm_level)");
     const size_t slot_bytes = (size_t)rb::kWsPlanes * G::M * G::N * sizeof(float);
-    const size_t done_bytes = ((size_t)args.batch * sizeof(int) + 255) / 256 * 256;
+    const size_t list_bytes = ((size_t)args.batch * sizeof(int) + 255) / 256 * 256;
     bool guess = takes_guess_path(args);
     int grid = std::min(resident_slots<G, false>(), args.batch);
     int ggrid = guess ? std::min(resident_slots<G, true>(), args.batch) : 0;
     char *ws;
     if (fixed) {
-        if (guess && fixed_bytes < 256 + done_bytes + slot_bytes)
-            guess = false; // the flags do not fit beside one slot: every crossbar goes to the sweeping kernel
-        const size_t fit = (fixed_bytes - 256 - (guess ? done_bytes : 0)) / slot_bytes;
+        if (guess && fixed_bytes < 256 + list_bytes + slot_bytes)
+            guess = false; // the list does not fit beside one slot: every crossbar goes to the sweeping kernel
+        const size_t fit = (fixed_bytes - 256 - (guess ? list_bytes : 0)) / slot_bytes;
         if (fit < 1)
             return set_error(FADESIM_E_UNSUPPORTED, "scratch too small for one workgroup of this shape");
         grid = (int)std::min((size_t)grid, fit);
         ggrid = (int)std::min((size_t)ggrid, fit);
         ws = fixed;
     } else {
-        ws = (char *)workspace_aux((size_t)std::max(grid, ggrid) * slot_bytes + 256 + (guess ? done_bytes : 0));
+        ws = (char *)workspace_aux((size_t)std::max(grid, ggrid) * slot_bytes + 256 + (guess ? list_bytes : 0));
         if (!ws)
             return FADESIM_E_NOMEM;
     }
@@ -248,7 +251,7 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
     args.ws = (float *)ws;
     int *ctr = (int *)(ws + slots_bytes);
     args.queue = ctr;
-    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, ctr, guess ? ctr + 1 : nullptr);
+    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(1), 0, s, ctr, guess ? 3 : 1);
     FADESIM_HIP_CHECK(hipGetLastError());
     const bool timing = (args.ctl.flags & FADESIM_F_TIMING) != 0;
     if (timing) {
@@ -261,10 +264,12 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
     if (guess) {
         RbArgs ga = args;
         ga.queue = ctr + 1;
-        ga.done = (int *)(ws + slots_bytes + 256);
+        ga.list = (int *)(ws + slots_bytes + 256);
+        ga.list_count = ctr + 2;
         hipLaunchKernelGGL((guess_kernel<G>), dim3(ggrid), dim3(G::NT), 0, s, ga);
         FADESIM_HIP_CHECK(hipGetLastError());
-        args.done = ga.done;
+        args.list = ga.list;
+        args.list_count = ga.list_count;
     }
     hipLaunchKernelGGL((rbsor_kernel<G>), dim3(grid), dim3(G::NT), 0, s, args);
     FADESIM_HIP_CHECK(hipGetLastError());
@@ -349,7 +354,8 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
                                                 ">= 2 and warm_level in 1..6 dividing rows and cols");
     }
     args.batch = B;
-    args.done = nullptr;
+    args.list = nullptr;
+    args.list_count = nullptr;
     args.g = d->d_g;
     args.gidx = d->d_gidx;
     args.queue = nullptr;
@@ -410,14 +416,16 @@ extern "C" int fadesim_solve_batch(const fadesim_batch_desc *d, void *stream)
     if (rc != FADESIM_OK || empty)
         return rc;
     hipStream_t s = (hipStream_t)stream;
-    const int B = d->batch, M = d->rows, N = d->cols;
+    const int M = d->rows, N = d->cols;
     if (!args.vout) {
-        double *z = (double *)workspace((size_t)B * N * sizeof(double));
+        // one zero row shared by every instance
+        double *z = (double *)workspace((size_t)N * sizeof(double));
         if (!z)
             return FADESIM_E_NOMEM;
-        hipLaunchKernelGGL(fill_zero_f64, dim3(256), dim3(256), 0, s, z, (size_t)B * N);
+        hipLaunchKernelGGL(fill_zero_f64, dim3(1), dim3(256), 0, s, z, (size_t)N);
         FADESIM_HIP_CHECK(hipGetLastError());
         args.vout = z;
+        args.vout_ld = 0;
     }
     return launch_shape(args, M, N, s, nullptr, 0);
 }
