@@ -56,25 +56,39 @@ def make_batch(B, seed):
     return g, r, vin
 
 
-def algorithmic_bytes(passes_sum, B, warm=False):
+def algorithmic_bytes(passes_sum, B, warm=False, coarse=False):
     """Streaming-schedule bytes: 4 B of conductance per cell per pass over the
     crossbar (a red-black sweep or a wire-solve iteration), plus the inputs and
     the FP64 readout (4*M + 8*N per crossbar), plus the FP32 warm-start field
-    (2*M*N*4) when the solve reads one from HBM."""
-    return 4.0 * M * N * passes_sum + B * (4.0 * M + 8.0 * N + (8.0 * M * N if warm else 0.0))
+    (2*M*N*4) when the solve reads one from HBM, plus the network's coarse
+    output ((M/4)*(N/4) pixels x 16 BF16 channels) when it is combined at
+    load."""
+    per = 4.0 * M + 8.0 * N + (8.0 * M * N if warm else 0.0) + (2.0 * 16 * (M // 4) * (N // 4) if coarse else 0.0)
+    return 4.0 * M * N * passes_sum + B * per
 
 
-def load_traffic(variant):
-    """HBM bytes per launch of the solve kernel of `variant`, from the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (scaled by
-    the same counters' reading of a plain device copy), or None."""
-    p = os.path.join(REPO, "profiles", "rbsor_128x128_b4096_traffic.json")
+def load_counters(variant, kernel):
+    """rocprofv3 counter figures of `kernel` in the run of `variant`, as
+    summarised in profiles/ (HBM bytes per launch from FETCH_SIZE /
+    WRITE_SIZE scaled by the same counters' reading of a plain device copy;
+    LDS bytes and VALU / wait fractions from the SQ counters), or {}."""
+    p = os.path.join(REPO, "profiles", "solve_128x128_b4096_counters.json")
     if not os.path.exists(p):
-        return None
+        return {}
     with open(p) as f:
         t = json.load(f)
-    v = t.get("hbm_bytes_per_launch", {})
-    return v.get(variant) if isinstance(v, dict) else v
+    return t.get("variants", {}).get(variant, {}).get(kernel, {})
+
+
+# Per-cell FP32 operations of one red-black sweep (both nodes of a cell):
+# tt = fma(h, p + q, rho) and x' = fma(w/2, n1 + n2, fma(1 - w, x, tt)),
+# 8 flops per node (an FMA counted as 2).
+SWEEP_FLOPS_PER_CELL = 16
+# MI355X vector FP32 peak (packed FP32 included; this build emits no packed
+# FP32 instructions, so half of it is the reachable rate) and LDS bandwidth
+# (128 bytes per clock per CU, 256 CUs, 2.4 GHz).
+VALU_FP32_PEAK_TFLOPS = 157.3
+LDS_PEAK_GBS = 128 * 256 * 2.4
 
 
 # Time of the restatement (oracle_nodegs, one core) over the reference V2 `sim`
@@ -235,7 +249,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": 4 * n}), flush=True)
         return
-    from fadesim_amd.solver import last_kernel_ms, solve_batch
+    from fadesim_amd.solver import last_guess_ms, last_kernel_ms, solve_batch
     from fadesim_amd import warmstart
 
     dev = torch.device("cuda:0")
@@ -276,32 +290,34 @@ def main():
             step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        kms, sweeps, res = [], [], None
+        kms, gms, sweeps, res = [], [], [], None
         for _ in range(args.steps):
             res = step(timing=True)
             kms.append(last_kernel_ms())
+            gms.append(last_guess_ms())
             sweeps.append(res.sweeps)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        return dt, kms, torch.stack(sweeps).to(torch.float64), res
+        return dt, kms, gms, torch.stack(sweeps).to(torch.float64), res
 
     meas, last = {}, {}
     for name, (k_ir2s, step) in variants.items():
-        dt, kms, sw, res = run(step)
+        dt, kms, gms, sw, res = run(step)
         last[name] = res
         if int((res.status != 0).sum()) != 0:
             raise RuntimeError(f"{name}: not every crossbar converged")
         if res.rejected is not None and int(res.rejected.sum()) != 0:
             raise RuntimeError(f"{name}: the warm start was rejected for {int(res.rejected.sum())} crossbars")
         kernel_ms = float(np.mean(kms))
+        guess_ms = float(np.mean(gms)) if min(gms) > 0 else None
         passes = float(sw.sum()) / args.steps + k_ir2s * B
-        ach = algorithmic_bytes(passes, B, warm=name.startswith("nn")) / (kernel_ms * 1e-3) / 1e9
+        ach = algorithmic_bytes(passes, B, warm=name.startswith("nn"),
+                                coarse=name == "nn") / (kernel_ms * 1e-3) / 1e9
         meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                       "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
-                      "kernel_GBps_algorithmic": ach}
+                      "guess_kernel_ms": guess_ms, "kernel_GBps_algorithmic": ach}
     head = "nn" if "nn" in meas else ("ir2s" if "ir2s" in meas else "plain")
     hm = meas[head]
-    traffic = load_traffic(head)
     out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU",
            "value": hm["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": hm["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -327,18 +343,42 @@ def main():
         out["nn_vs_plain_speedup"] = meas["nn"]["solves_per_s"] / meas["plain"]["solves_per_s"]
     if "plain" in meas and "ir2s" in meas:
         out["ir2s_vs_plain_speedup"] = meas["ir2s"]["solves_per_s"] / meas["plain"]["solves_per_s"]
-    out["roofline"] = {"bound": "hbm", "achieved": hm["kernel_GBps_algorithmic"], "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": hm["kernel_GBps_algorithmic"] / HBM_PEAK_GBS,
-                       "traffic": traffic,
-                       "kernel": "rbsor_kernel<Geo<4,8,32,16>>", "kernel_ms": hm["kernel_ms"],
+    # the dominant kernel of the reported step: guess_kernel when the step's
+    # solves end at their first test (every crossbar of the nn and ir2s
+    # variants), timed alone between HIP events on its stream
+    gk = hm["guess_kernel_ms"] is not None and hm["mean_sweeps"] == 0
+    kname = "guess_kernel<Geo<4,8,32,16>>" if gk else "rbsor_kernel<Geo<4,8,32,16>>"
+    kms_dom = hm["guess_kernel_ms"] if gk else hm["kernel_ms"]
+    passes_dom = hm["ir2s_iters"] * B + hm["mean_sweeps"] * B
+    ach_dom = algorithmic_bytes(passes_dom, B, warm=head == "nn", coarse=head == "nn") / (kms_dom * 1e-3) / 1e9
+    ctr = load_counters(head, "guess_kernel" if gk else "rbsor_kernel")
+    out["roofline"] = {"bound": "hbm", "achieved": ach_dom, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": ach_dom / HBM_PEAK_GBS,
+                       "traffic": ctr.get("hbm_bytes_per_launch"),
+                       "kernel": kname, "kernel_ms": kms_dom, "step_kernels_ms": hm["kernel_ms"],
                        "bytes_model": "(sweeps + IR2S iterations) * 4*M*N + 4*M + 8*N per crossbar "
                                       "(conductance streamed once per sweep or wire-solve pass), + 8*M*N "
-                                      "for the warm-start field read from HBM"}
+                                      "for the warm-start field read from HBM, + 2*16*(M/4)*(N/4) for the "
+                                      "network's coarse output combined at load",
+                       "lds": ({"achieved": ctr["lds_bytes_per_launch"] / (kms_dom * 1e-3) / 1e9,
+                                "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                                "frac": ctr["lds_bytes_per_launch"] / (kms_dom * 1e-3) / 1e9 / LDS_PEAK_GBS}
+                               if "lds_bytes_per_launch" in ctr else None),
+                       "valu_busy": ctr.get("valu_busy"), "wait_frac": ctr.get("wait_frac"),
+                       "counters": "profiles/solve_128x128_b4096_counters.md" if ctr else None}
     if "plain" in meas:
         pm = meas["plain"]
-        out["roofline_plain_sor"] = {"bound": "hbm", "achieved": pm["kernel_GBps_algorithmic"], "peak": HBM_PEAK_GBS,
-                                     "unit": "GB/s", "frac": pm["kernel_GBps_algorithmic"] / HBM_PEAK_GBS,
-                                     "kernel_ms": pm["kernel_ms"]}
+        tflops = SWEEP_FLOPS_PER_CELL * M * N * pm["mean_sweeps"] * B / (pm["kernel_ms"] * 1e-3) / 1e12
+        pc = load_counters("plain", "rbsor_kernel")
+        # plain SOR keeps the conductance in registers across its sweeps: its
+        # binding resource is VALU issue, not HBM
+        out["roofline_plain_sor"] = {"bound": "valu", "achieved": tflops, "peak": VALU_FP32_PEAK_TFLOPS,
+                                     "unit": "TFLOP/s", "frac": tflops / VALU_FP32_PEAK_TFLOPS,
+                                     "flops_model": f"{SWEEP_FLOPS_PER_CELL} FP32 flops per cell per sweep",
+                                     "kernel_ms": pm["kernel_ms"], "traffic": pc.get("hbm_bytes_per_launch"),
+                                     "valu_busy": pc.get("valu_busy"),
+                                     "hbm_GBps_measured": (pc["hbm_bytes_per_launch"] / (pm["kernel_ms"] * 1e-3) / 1e9
+                                                           if "hbm_bytes_per_launch" in pc else None)}
     if not args.no_cpu:
         S = min(args.cpu_sample, B)
         cpu, iex = cpu_baseline(g[:S], r[:S], vin[:S])

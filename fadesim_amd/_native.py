@@ -101,7 +101,7 @@ IR2S_BREAK_BATCH, IR2S_BREAK_EACH = 1, 2
 EXPORTED = [
     "fadesim_last_error", "fadesim_version", "fadesim_solve_batch", "fadesim_solve_batch_noalloc",
     "fadesim_default_omega",
-    "fadesim_last_kernel_ms", "fadesim_cell_conductance",
+    "fadesim_last_kernel_ms", "fadesim_last_guess_ms", "fadesim_cell_conductance",
     "fadesim_ragged_plan_create", "fadesim_ragged_plan_destroy", "fadesim_ragged_plan_info",
     "fadesim_ragged_solve", "fadesim_ragged_aborted", "fadesim_ragged_last_ms",
     "fadesim_gs64_solve", "fadesim_hook_ir_drop_gs", "fadesim_ir2s",
@@ -131,6 +131,8 @@ def lib():
             L.fadesim_solve_batch_noalloc.argtypes = [ctypes.POINTER(BatchDesc), _vp]
             L.fadesim_solve_batch_noalloc.restype = ctypes.c_int
         L.fadesim_last_kernel_ms.restype = ctypes.c_double
+        if hasattr(L, "fadesim_last_guess_ms"):
+            L.fadesim_last_guess_ms.restype = ctypes.c_double
         L.fadesim_default_omega.argtypes = [ctypes.c_int, ctypes.c_int]
         L.fadesim_default_omega.restype = ctypes.c_double
         if hasattr(L, "fadesim_gs64_solve"):

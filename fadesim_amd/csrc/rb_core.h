@@ -1392,26 +1392,41 @@ __device__ __attribute__((noinline)) void phase_coarse(Inst I, StripCtx X, float
 // max(|1-w|, |1-w(1+rho)|) -- rho/(2+rho) at w = 2/(2+rho).  Returns false
 // when the team aborted.
 // The caller's sweep state is dead across the call (it is reloaded from
-// planes 3, 4 afterwards), so the drops stay in VGPRs for all iterations;
-// from_zero starts them at zero instead of reading planes 3, 4
+// planes 3, 4 afterwards), so the drops stay in VGPRs for all iterations
 // (wire_iteration); per iteration only the conductance is read, twice, from
-// the cache.
-template <class G, bool STRIP>
+// the cache.  start: the drops begin at zero (kStartZero), at the warm start
+// (kStartWarm), or at the warm start plus the network's coarse correction
+// combined while loading (kStartWarmCoarse, whole crossbars) -- read from
+// HBM here, not from a workspace plane.  The result goes to planes 3, 4;
+// PUBLISH (whole crossbars) also publishes its halo and ends with a barrier,
+// for a caller that only tests it.
+constexpr int kStartZero = 0, kStartWarm = 1, kStartWarmCoarse = 2;
+
+template <class G, bool STRIP, bool PUBLISH = false>
 __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
                                                      int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
-                                                     int *nbar, int iters, double w, bool from_zero)
+                                                     int *nbar, int iters, double w, int start)
 {
     RB_SH_LOCAL;
-    constexpr int TC = G::TC;
+    static_assert(!(STRIP && PUBLISH), "strips publish through the team exchange");
+    constexpr int TC = G::TC, N = G::N;
     float p[G::TR][TC], q[G::TR][TC];
-    if (from_zero) {
+    start = uniform_i32(start);
+    if (start == kStartZero) {
 #pragma unroll
         for (int ii = 0; ii < G::TR; ++ii)
 #pragma unroll
             for (int jj = 0; jj < TC; ++jj)
                 p[ii][jj] = q[ii][jj] = 0.f;
     } else {
-        tile_load<G>(wsb, i0, j0, p, q);
+        const size_t MNall = (size_t)I.rows * N;
+#pragma unroll
+        for (int ii = 0; ii < G::TR; ++ii) {
+            ldrow<TC>(I.warm + (size_t)(gi0 + i0 + ii) * N + j0, p[ii]);
+            ldrow<TC>(I.warm + MNall + (size_t)(gi0 + i0 + ii) * N + j0, q[ii]);
+        }
+        if (!STRIP && start == kStartWarmCoarse)
+            add_coarse<G>(I, sh.halo, t, gi0, i0, j0, p, q);
     }
     // the tile's conductances (lean geometries: only the line voltages) stay
     // in registers for all iterations: an iteration reads no memory but LDS
@@ -1455,6 +1470,10 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
             return false;
     }
     tile_store<G>(wsb, i0, j0, p, q);
+    if constexpr (PUBLISH) {
+        publish_all<G>(p, q, sh.halo, t);
+        __syncthreads();
+    }
     return true;
 }
 
@@ -1544,11 +1563,23 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
 
     float yp[TR][TC], yq[TR][TC];
     const bool coarse = !STRIP && I.coarse;
-    if (coarse) {
+    // snap_valid: planes 3, 4 hold the present drops (no snapshot store
+    // needed before a test)
+    bool snap_valid = false;
+    if (C.ir2s_iters > 0) {
+        // the IR2S phase reads the initial guess itself (the coarse
+        // correction combined while loading)
+        const int start = !I.warm ? kStartZero : coarse ? kStartWarmCoarse : kStartWarm;
+        if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
+                                  C.ir2s_omega, start))
+            return false;
+        tile_load<G>(wsb, i0, j0, yp, yq);
+        snap_valid = true;
+    } else if (coarse) {
         // the warm start combined while loading, into planes 3, 4
         phase_coarse<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
-        if (C.ir2s_iters == 0)
-            tile_load<G>(wsb, i0, j0, yp, yq);
+        tile_load<G>(wsb, i0, j0, yp, yq);
+        snap_valid = true;
     } else {
 #pragma unroll
         for (int ii = 0; ii < TR; ++ii) {
@@ -1561,18 +1592,6 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                     yp[ii][jj] = yq[ii][jj] = 0.f;
             }
         }
-    }
-    // planes 3, 4 hold the present drops (no snapshot store needed before a test)
-    bool snap_valid = false;
-    if (C.ir2s_iters > 0) {
-        const bool from_zero = !I.warm;
-        if (!from_zero && !coarse)
-            tile_store<G>(wsb, i0, j0, yp, yq);
-        if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
-                                  C.ir2s_omega, from_zero))
-            return false;
-        tile_load<G>(wsb, i0, j0, yp, yq);
-        snap_valid = true;
     }
     publish_all<G>(yp, yq, halo, t);
     if (STRIP)
@@ -1637,7 +1656,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 rejected = true;
                 if (C.ir2s_iters > 0) {
                     if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
-                                              C.ir2s_iters, C.ir2s_omega, true))
+                                              C.ir2s_iters, C.ir2s_omega, kStartZero))
                         return false;
                     tile_load<G>(wsb, i0, j0, yp, yq);
                 } else {
@@ -1814,37 +1833,35 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
 
     phase_setup<G, false>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, RB_SH_ARGS(sh), &nbar);
 
+    // the guess into planes 3, 4 (the snapshot the test reads) and the halo;
+    // the drops themselves are not needed in registers here
     float yp[TR][TC], yq[TR][TC];
     const bool coarse = I.coarse != nullptr;
-    if (coarse) {
-        phase_coarse<G, false>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
-        if (C.ir2s_iters == 0)
-            tile_load<G>(wsb, i0, j0, yp, yq);
-    } else {
-#pragma unroll
-        for (int ii = 0; ii < TR; ++ii) {
-            if (I.warm) {
-                ldrow<TC>(I.warm + (size_t)(i0 + ii) * N + j0, yp[ii]);
-                ldrow<TC>(I.warm + (size_t)M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
-            } else {
-#pragma unroll
-                for (int jj = 0; jj < TC; ++jj)
-                    yp[ii][jj] = yq[ii][jj] = 0.f;
-            }
-        }
-    }
     if (C.ir2s_iters > 0) {
-        const bool from_zero = !I.warm;
-        if (!from_zero && !coarse)
+        const int start = !I.warm ? kStartZero : coarse ? kStartWarmCoarse : kStartWarm;
+        phase_ir2s<G, false, true>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
+                                   C.ir2s_iters, C.ir2s_omega, start);
+    } else {
+        if (coarse) {
+            phase_coarse<G, false>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
+            tile_load<G>(wsb, i0, j0, yp, yq);
+        } else {
+#pragma unroll
+            for (int ii = 0; ii < TR; ++ii) {
+                if (I.warm) {
+                    ldrow<TC>(I.warm + (size_t)(i0 + ii) * N + j0, yp[ii]);
+                    ldrow<TC>(I.warm + (size_t)M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < TC; ++jj)
+                        yp[ii][jj] = yq[ii][jj] = 0.f;
+                }
+            }
             tile_store<G>(wsb, i0, j0, yp, yq);
-        phase_ir2s<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
-                             C.ir2s_omega, from_zero);
-        tile_load<G>(wsb, i0, j0, yp, yq);
+        }
+        publish_all<G>(yp, yq, halo, t);
+        __syncthreads();
     }
-    publish_all<G>(yp, yq, halo, t);
-    __syncthreads();
-    if (C.ir2s_iters == 0)
-        tile_store<G>(wsb, i0, j0, yp, yq); // the snapshot the test reads
     // the test's value is the same in every lane but comes back from the call
     // in VGPRs: made a scalar, every decision below is a uniform branch (a
     // branch the compiler takes as divergent lets it split lanes of one wave
@@ -1855,9 +1872,8 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
     if ((I.warm != nullptr || coarse) && !(est <= zero_guess_est<G>(sh, t))) {
         rejected = true;
         if (C.ir2s_iters > 0) {
-            phase_ir2s<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
-                                 C.ir2s_iters, C.ir2s_omega, true);
-            tile_load<G>(wsb, i0, j0, yp, yq);
+            phase_ir2s<G, false, true>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
+                                       C.ir2s_iters, C.ir2s_omega, kStartZero);
         } else {
 #pragma unroll
             for (int ii = 0; ii < TR; ++ii)
@@ -1865,9 +1881,9 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
                 for (int jj = 0; jj < TC; ++jj)
                     yp[ii][jj] = yq[ii][jj] = 0.f;
             tile_store<G>(wsb, i0, j0, yp, yq);
+            publish_all<G>(yp, yq, halo, t);
+            __syncthreads();
         }
-        publish_all<G>(yp, yq, halo, t);
-        __syncthreads();
         est = uniform_f64(phase_test<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false, false,
                                                has_l, has_r, has_u, has_d, RB_SH_ARGS(sh), -1, &nbar));
     }

@@ -179,8 +179,10 @@ constexpr size_t kScratchSlotsBytes = (size_t)256 * rb::kWsPlanes * 128 * 128 * 
 constexpr size_t kScratchBytes = kScratchZeroBytes + kScratchSlotsBytes;
 static __device__ __align__(256) char scratch[kScratchBytes];
 
-thread_local hipEvent_t t_ev[2] = {nullptr, nullptr};
-thread_local bool t_ev_valid = false;
+// t_ev[0] .. t_ev[1]: both kernels of the last timed call; t_ev[2]: the end
+// of its guess_kernel (t_guess: that kernel ran)
+thread_local hipEvent_t t_ev[3] = {nullptr, nullptr, nullptr};
+thread_local bool t_ev_valid = false, t_guess = false;
 
 // Resident workgroups of rbsor_kernel<G> (GUESS: guess_kernel<G>) on the
 // current device: the persistent grid.
@@ -258,6 +260,7 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
         if (!t_ev[0]) {
             FADESIM_HIP_CHECK(hipEventCreate(&t_ev[0]));
             FADESIM_HIP_CHECK(hipEventCreate(&t_ev[1]));
+            FADESIM_HIP_CHECK(hipEventCreate(&t_ev[2]));
         }
         FADESIM_HIP_CHECK(hipEventRecord(t_ev[0], s));
     }
@@ -268,6 +271,8 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
         ga.list_count = ctr + 2;
         hipLaunchKernelGGL((guess_kernel<G>), dim3(ggrid), dim3(G::NT), 0, s, ga);
         FADESIM_HIP_CHECK(hipGetLastError());
+        if (timing)
+            FADESIM_HIP_CHECK(hipEventRecord(t_ev[2], s));
         args.list = ga.list;
         args.list_count = ga.list_count;
     }
@@ -276,6 +281,7 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
     if (timing) {
         FADESIM_HIP_CHECK(hipEventRecord(t_ev[1], s));
         t_ev_valid = true;
+        t_guess = guess;
     }
     return FADESIM_OK;
 }
@@ -295,6 +301,18 @@ extern "C" double fadesim_last_kernel_ms(void)
         return -2.0;
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t_ev[0], t_ev[1]) != hipSuccess)
+        return -3.0;
+    return (double)ms;
+}
+
+extern "C" double fadesim_last_guess_ms(void)
+{
+    if (!t_ev_valid || !t_guess)
+        return -1.0;
+    if (hipEventSynchronize(t_ev[2]) != hipSuccess)
+        return -2.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, t_ev[0], t_ev[2]) != hipSuccess)
         return -3.0;
     return (double)ms;
 }

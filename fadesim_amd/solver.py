@@ -41,6 +41,12 @@ def last_kernel_ms() -> float:
     return float(nat.lib().fadesim_last_kernel_ms())
 
 
+def last_guess_ms() -> float:
+    """Of the last ``solve_batch(..., timing=True)``, the duration of its
+    guess-first kernel alone (negative when it ran none)."""
+    return float(nat.lib().fadesim_last_guess_ms())
+
+
 def default_omega(rows: int, cols: int) -> float:
     return float(nat.lib().fadesim_default_omega(int(rows), int(cols)))
 

@@ -135,10 +135,15 @@ int fadesim_solve_batch(const fadesim_batch_desc *desc, void *stream);
  */
 int fadesim_solve_batch_noalloc(const fadesim_batch_desc *desc, void *stream);
 
-/* Duration (ms) of the solve kernel of the last fadesim_solve_batch call made
- * with FADESIM_F_TIMING on this thread, from HIP events on the call's stream;
- * waits for that kernel.  Negative when no timed call was made. */
+/* Duration (ms) of the solve kernels of the last fadesim_solve_batch call
+ * made with FADESIM_F_TIMING on this thread, from HIP events on the call's
+ * stream; waits for them.  Negative when no timed call was made. */
 double fadesim_last_kernel_ms(void);
+
+/* Of that call, the duration (ms) of its guess-first kernel alone (the part of
+ * a FADESIM_F_TEST_FIRST solve before its first sweep); negative when the call
+ * did not run one.  No reference counterpart: a timing query. */
+double fadesim_last_guess_ms(void);
 
 /* The built-in relaxation factor for an n x m array (used when omega <= 0). */
 double fadesim_default_omega(int rows, int cols);
