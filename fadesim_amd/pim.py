@@ -59,6 +59,60 @@ class phyArrMode(Enum):
     Ref = 1
 
 
+class GS_method(Enum):
+    """Update order of PimTorch's torch line-GS fallback (commonConst.py:174-177,
+    ir_drop_solve.py:191-292).  Whole arrays: both orders give the same
+    iterates -- row by row from the bottom, every wordline node reads its new
+    left and old right neighbour and its old bitline node, every bitline node
+    its new lower and old upper neighbour and its new wordline node, in the
+    same float expressions -- so both run the hook kernel.  OU blocks: the
+    nodeBynode form (:315-360) leaves out the OU's source and sink loads (plain
+    wire ends), i.e. the whole-array iteration on each OU block."""
+    nodeBynode = 1
+    batchBybatch = 2
+
+
+class variations:
+    """The seeded CPU generators of the conductance variation
+    (commonConst.py:122-154): network arrays draw from rand_gen_network (seed
+    + 2); reset_all / network_gen_reset restart them."""
+    var_defaultSeed = 555
+    rand_gen_memCMOS = torch.Generator()
+    rand_gen_mirrored = torch.Generator()
+    rand_gen_network = torch.Generator()
+
+    @staticmethod
+    def network_gen_reset(seed: int = 555):
+        variations.rand_gen_network.manual_seed(seed + 2)
+
+    @staticmethod
+    def mirrored_gen_reset(seed: int = 555):
+        variations.rand_gen_mirrored.manual_seed(seed + 1)
+
+    @staticmethod
+    def memCMOS_gen_reset(seed: int = 555):
+        variations.rand_gen_memCMOS.manual_seed(seed)
+
+    @staticmethod
+    def reset_all(seed: int = 555):
+        variations.memCMOS_gen_reset(seed)
+        variations.mirrored_gen_reset(seed)
+        variations.network_gen_reset(seed)
+
+
+variations.reset_all()
+
+
+def add_variation(conductance: Tensor, sigma: float = 0.0, rand_gen: Optional[torch.Generator] = None) -> Tensor:
+    """Lognormal cell variation (conductance_variation.py:6-14): R = R0 exp(q),
+    q ~ N(0, sigma^2) per cell, so g = g0 / exp(q).  q is drawn on the CPU
+    (torch.randn from rand_gen, FP32, in the tensor's shape) and the factor
+    moved to the tensor's device, as the reference does, so a seeded
+    generator gives the reference's draws."""
+    factor = torch.randn(conductance.shape, generator=rand_gen).mul_(sigma).exp_()
+    return conductance / factor.to(conductance.device)
+
+
 class ConstForSplitArray:
     """Default bit widths of commonConst.py:45-50."""
     input_bit_width = 8
@@ -97,6 +151,7 @@ class PhyArrParams:
         self.OUSize = (8, 8)
         self.ir_drop_iter_time = 600
         self.ir_drop_GS_method_beta_scale = 1.90
+        self.ir_drop_GS_method_mode = GS_method.batchBybatch  # update order (both orders: same iterates)
         self.ir_drop_less_than_thresholdBreak_enable = False
         self.ir_drop_less_than_threshold = 1e-7
         self.nonlinear_cell_enable = False
@@ -219,8 +274,6 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         raise Exception("[Error] Complement code input mode, but dac is not 1 bit. dacbits = {}".format(dacBits))
     if weightBits % P.cellBits != 0:
         raise Exception("[Error] cellbit is not factor of weight_bit")
-    if P.sigma != 0.0:
-        raise NotImplementedError("conductance variation (sigma != 0) is not modelled")
     with torch.no_grad():
         bsize, insize = input.shape
         outsize = w.shape[1]
@@ -256,7 +309,19 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         G = slices.to(float_type).mul(P.cellDeltaConduct).add(P.cellMinConduct)  # matmul :753-754, :777-783
         if ref and P.useOUSize:
             G[..., C + 2:] = 1e-20
-        G = G.permute(0, 1, 2, 4, 3, 5).contiguous()  # [nslice][ns][numX][numY][R][Cx]
+        if P.sigma != 0.0:
+            # the reference draws a fresh variation for every (input bit,
+            # slice, array) call of matmul, in that loop order (:1197-1227
+            # around :757-760 / :788), over the padded [numX, R, numY*Cx]
+            # array: one conductance set per input bit.  (At sigma = 0 the
+            # reference still draws; this path skips the draws.)
+            gen = variations.rand_gen_network
+            G = torch.stack([torch.stack([torch.stack([
+                add_variation(G[j, s].reshape(numX, R, numY * Cx), P.sigma, gen).view(numX, R, numY, Cx)
+                for s in range(ns)]) for j in range(nslice)]) for _ in range(inputBits)])
+        else:
+            G = G.unsqueeze(0)
+        G = G.permute(0, 1, 2, 3, 5, 4, 6).contiguous()  # [nv][nslice][ns][numX][numY][R][Cx], nv = 1 or inputBits
         # input bits [inputBits][bsize][numX][R] (two's complement bits of qin)
         qpad = F.pad(qin, (0, Rp - insize))
         bits = torch.stack([(qpad >> i) & 1 for i in range(inputBits)]).view(inputBits, bsize, numX, R)
@@ -264,13 +329,14 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         # mode one entry per OU row block: [..][numX * arrOUnum[0]][numY][Cx]
         if P.ir_mode == "none":
             vin = bits.to(float_type).mul(inputV)
+            Gb = G.expand(inputBits, *G.shape[1:])
             if P.useOUSize:
                 our = P.OUSize[0]
-                I = torch.einsum("ibxkr,jsxykrc->ijsbxkyc", vin.view(inputBits, bsize, numX, R // our, our),
-                                 G.view(nslice, ns, numX, numY, R // our, our, Cx))
+                I = torch.einsum("ibxkr,ijsxykrc->ijsbxkyc", vin.view(inputBits, bsize, numX, R // our, our),
+                                 Gb.reshape(inputBits, nslice, ns, numX, numY, R // our, our, Cx))
                 I = I.reshape(inputBits, nslice, ns, bsize, numX * (R // our), numY, Cx)
             else:
-                I = torch.einsum("ibxr,jsxyrc->ijsbxyc", vin, G)
+                I = torch.einsum("ibxr,ijsxyrc->ijsbxyc", vin, Gb)
         elif P.ir_mode == "converged":
             if P.useOUSize:
                 raise NotImplementedError("OU mode runs the hook's OU iterations (ir_mode 'reference' or 'ir2s')")
@@ -318,15 +384,18 @@ def _solve_converged(bits, G, inputV, P):
     """All tile solves of the layer through the batched solver."""
     from .solver import solve_batch, solve_batch_nonlinear
     nbit, bsize, numX, R = bits.shape
-    nslice, _, _, numY, _, C = G.shape
+    nv, nslice, _, _, numY, _, C = G.shape
     if not G.is_cuda:
         raise RuntimeError("the converged crossbar model runs on the GPU (G is on the CPU)")
     dev = G.device
-    slots = G.reshape(-1, R, C).to(torch.float32).contiguous()  # slot ((j*2+s)*numX + x)*numY + y
+    # slot (((v*nslice + j)*2 + s)*numX + x)*numY + y, v the input bit when
+    # every bit has its own conductances (variation), else 0
+    slots = G.reshape(-1, R, C).to(torch.float32).contiguous()
     I = torch.zeros((nbit, nslice, 2, bsize, numX, numY, C), dtype=torch.float64, device=dev)
     ii, bb, xx, jj, ss, yy = tile_instances(bits, nslice, numY)
     total = ii.numel()
-    gidx = (((jj * 2 + ss) * numX + xx) * numY + yy).to(torch.int32)
+    vv = ii if nv > 1 else torch.zeros_like(ii)
+    gidx = ((((vv * nslice + jj) * 2 + ss) * numX + xx) * numY + yy).to(torch.int32)
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
     net = P.warm_start
     k = int(P.ir2s_iters)
@@ -369,9 +438,10 @@ def _solve_reference(bits, G, inputV, P, float_type):
     ir2s_ou)."""
     from . import hook
     nbit, bsize, numX, R = bits.shape
-    nslice, ns, _, numY, _, C = G.shape  # C includes reference columns
+    nv, nslice, ns, _, numY, _, C = G.shape  # C includes reference columns
     dev = G.device
     nrb = numX * (R // P.OUSize[0]) if P.useOUSize else numX
+    node_ou = P.useOUSize and P.ir_mode == "reference" and P.ir_drop_GS_method_mode == GS_method.nodeBynode
     I = torch.zeros((nbit, nslice, ns, bsize, nrb, numY, C), dtype=torch.float64, device=dev)
     brk, thr = P.ir_drop_less_than_thresholdBreak_enable, P.ir_drop_less_than_threshold
     nl_on, nl = P.nonlinear_cell_enable, P.nonlinear_params
@@ -379,9 +449,18 @@ def _solve_reference(bits, G, inputV, P, float_type):
         vin = bits[i].to(float_type).mul(inputV).contiguous()  # [bsize][numX][R]
         for jj in range(nslice):
             for s in range(ns):
-                Gh = G[jj, s].permute(0, 2, 1, 3).reshape(numX, R, numY * C).to(float_type).contiguous()
+                Gh = G[i if nv > 1 else 0, jj, s].permute(0, 2, 1, 3).reshape(numX, R, numY * C)
+                Gh = Gh.to(float_type).contiguous()
                 it = P.ir_drop_iter_time
-                if P.ir_mode == "ir2s" and P.useOUSize:
+                if node_ou:
+                    # ir_drop_accurate_GSMethod_OU's nodeBynode form (ir_drop_solve.py:315-360): every
+                    # OU block a crossbar of its own with plain wire ends
+                    our, ouc = P.OUSize
+                    vb = vin.reshape(bsize, nrb, our).contiguous()
+                    Gb = Gh.reshape(nrb, our, numY * C).contiguous()
+                    out = hook.ir_drop_gs(vb, Gb, bsize, our, ouc, nrb, numY * C // ouc, it, 1.0 / P.r_wire,
+                                          P.ir_drop_GS_method_beta_scale, brk, thr, nl_on, nl)
+                elif P.ir_mode == "ir2s" and P.useOUSize:
                     out = hook.ir2s_ou(vin, Gh, bsize, R, C, P.OUSize[0], P.OUSize[1], numX, numY, it, P.r_wire,
                                        brk, thr, nl_on, nl)
                 elif P.ir_mode == "ir2s":

@@ -33,6 +33,13 @@ integer/fixed-point pipeline around the solver:
                         code x 2^(wb-1) (:996-1012).
 * ``unfold`` / ``conv2d_sp`` -- Conv2d_sp's eval path (modules/conv_split.py:
                         87-136): im2col, the split matmul, back to NCHW.
+* ``vary``           -- the conductance variation hook of both matmuls: called
+                        on every array's conductances [numX, R, numY*C(+E)]
+                        right after they are formed, once per (input bit,
+                        slice, array) in the reference's loop order (the
+                        add_variation calls of matmul, splitArrayArithmetic.py:
+                        757-760 and :788, inside splitArr_matmul_nn's loops
+                        :1197-1227 / :1245-1256); the caller supplies the draw.
 
 Every array op is done in float32 where the reference's is (fp_type float32).
 """
@@ -81,7 +88,8 @@ def ideal_currents_ou(ou_rows: int):
 
 
 def split_matmul(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, cell_bits=1,
-                 rows=64, cols=64, vmax=1.0, ron=1e3, roff=1e6, currents=ideal_currents, ou_rows=None):
+                 rows=64, cols=64, vmax=1.0, ron=1e3, roff=1e6, currents=ideal_currents, ou_rows=None,
+                 vary=None):
     """output [bsize, outsize] = x [bsize, insize] @ w [insize, outsize] through
     PN crossbars.  ``currents(vin [bsize,numX,R] f32, G [numX,R,numY*C] f32)``
     returns the tile currents [bsize, numX, numY*C].  Returns (output f32,
@@ -120,6 +128,8 @@ def split_matmul(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, 
                 sl = (wm >> j) & mask_cell
                 sl = np.pad(sl, ((0, Rp - insize), (0, Cp - outsize)))
                 G = (sl.astype(np.float32) * np.float32(dG) + np.float32(gmin)).reshape(numX, rows, Cp)
+                if vary is not None:
+                    G = np.asarray(vary(G), np.float32)
                 I = np.asarray(currents(vin, G)).astype(np.float32)
                 a = I / np.float32(adc_use_i) * lsb
                 code = np.round(a)
@@ -134,7 +144,7 @@ def split_matmul(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, 
 
 def split_matmul_ref(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits=8, cell_bits=1,
                      rows=64, cols=64, vmax=1.0, ron=1e3, roff=1e6, currents=ideal_currents, ou_rows=None,
-                     ou_cols=None):
+                     ou_cols=None, vary=None):
     """output [bsize, outsize] through reference-column arrays.
     ``currents(vin [bsize,numX,R], G [numX,R,numY*(C+E)])`` returns
     [bsize, row blocks, numY*(C+E)], E = 2 (OU mode: ou_cols)."""
@@ -171,6 +181,8 @@ def split_matmul_ref(x, w, *, input_bits=8, dac_bits=1, adc_bits=10, weight_bits
             if ou_rows:
                 G[..., cols + 2:] = np.float32(1e-20)
             G = G.reshape(numX, rows, numY * (cols + E))
+            if vary is not None:
+                G = np.asarray(vary(G), np.float32)
             I = np.asarray(currents(vin, G)).astype(np.float32)
             a = I / np.float32(adc_use_i) * lsb
             adc_in[(i, j, "ref")] = a

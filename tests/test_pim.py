@@ -229,3 +229,173 @@ def test_conv2d_sp_matches_restatement(arr, k, stride, pad, dil, bias_on_array):
 
 def F_conv2d(x, w, b, stride, pad, dil):
     return torch.nn.functional.conv2d(x, w, b, stride, pad, dil)
+
+
+# ------------------------------------------------------------------ variation
+# conductance_variation.py:6-14 with the generators of commonConst.py:122-154
+
+def _draws(sigma, seed=557):
+    """The variation hook of the restatement: a CPU generator seeded like the
+    reference's network generator (555 + 2), one draw per call."""
+    gen = torch.Generator().manual_seed(seed)
+
+    def vary(G):
+        q = torch.randn(G.shape, generator=gen)
+        return (torch.from_numpy(np.ascontiguousarray(G)) / torch.exp(q * sigma)).numpy()
+    return vary
+
+
+def test_variation_draw_is_the_seeded_lognormal():
+    """The factor is exp(sigma q), q one randn draw per cell from the network
+    generator (seed 557 after reset): the first draw equals that generator's
+    first randn, and over many cells log(g0 / g) has mean 0 and standard
+    deviation sigma (sigma is the std of q; a variance would give 0.3**0.5)."""
+    pim.variations.reset_all()
+    g0 = torch.full((2, 64, 128), 1e-6, dtype=torch.float32)
+    g = pim.add_variation(g0, 0.3, pim.variations.rand_gen_network)
+    q = torch.randn(g0.shape, generator=torch.Generator().manual_seed(557))
+    assert torch.equal(g, g0 / torch.exp(q * 0.3))
+    lq = torch.log(g0.double() / g.double())
+    assert abs(lq.mean().item()) < 0.01 and abs(lq.std().item() - 0.3) < 0.01
+    # the generator advances: the next call draws new values
+    assert not torch.equal(pim.add_variation(g0, 0.3, pim.variations.rand_gen_network), g)
+    # reset restarts the sequence
+    pim.variations.network_gen_reset()
+    assert torch.equal(pim.add_variation(g0, 0.3, pim.variations.rand_gen_network), g)
+
+
+@pytest.mark.parametrize("arr,ou,seed", [(pim.phyArrMode.PN, None, 0), (pim.phyArrMode.PN, (8, 8), 1),
+                                         (pim.phyArrMode.Ref, None, 2)])
+def test_variation_matches_restatement(arr, ou, seed):
+    """sigma != 0 through the ideal crossbar: one draw per (input bit, slice,
+    array) over the padded [numX, R, numY*C] array, in the reference's loop
+    order -- the restatement makes the draws in that loop itself."""
+    gen = torch.Generator().manual_seed(seed)
+    insize, outsize, rows = 100, 30, 64
+    x = torch.rand(3, insize, generator=gen) * 2 - 0.5
+    w = torch.randn(insize, outsize, generator=gen)
+    kw = dict(useOUSize=True, OUSize=ou) if ou else {}
+    P = pim.PhyArrParams(ir_mode="none", cellBits=2, arrRowSize=rows, arrColSize=rows, sigma=0.2, **kw)
+    pim.variations.reset_all()
+    out = pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, arrType=arr, params=P).numpy()
+    mm = opim.split_matmul if arr == pim.phyArrMode.PN else opim.split_matmul_ref
+    okw = dict(cell_bits=2, rows=rows, cols=rows, vmax=P.inputMaxVoltage, ron=P.Ron, roff=P.Roff)
+    if ou:
+        okw.update(currents=opim.ideal_currents_ou(ou[0]), ou_rows=ou[0])
+        if arr == pim.phyArrMode.Ref:
+            okw["ou_cols"] = ou[1]
+    ref, info = mm(x.numpy(), w.numpy(), vary=_draws(0.2), **okw)
+    per_read = ou[0] if ou else rows
+    bound = _near_tie_bound(info, per_read * 3 / 1023, outsize)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
+    # the variation is visible, and a different draw order (slices outside
+    # input bits) gives another answer: the comparison pins the order
+    ref0, _ = mm(x.numpy(), w.numpy(), **okw)
+    tol = bound + 1e-6 * np.abs(ref).max()
+    assert np.max(np.abs(ref - ref0)) > 3 * tol
+    ns = 2 if arr == pim.phyArrMode.PN else 1
+    calls = iter(range(8 * 4 * ns))
+
+    def vary_swapped(G):
+        # the draw this call would get if slices were the outer loop
+        i, rest = divmod(next(calls), 4 * ns)
+        j, s = divmod(rest, ns)
+        g2 = torch.Generator().manual_seed(557)
+        for _ in range((j * 8 + i) * ns + s):
+            torch.randn(G.shape, generator=g2)
+        q = torch.randn(G.shape, generator=g2)
+        return (torch.from_numpy(np.ascontiguousarray(G)) / torch.exp(q * 0.2)).numpy()
+    wrong, _ = mm(x.numpy(), w.numpy(), vary=vary_swapped, **okw)
+    assert np.max(np.abs(wrong - ref)) > 3 * tol
+
+
+def test_gs_orders_give_the_same_iterates():
+    """PimTorch's torch fallback (ir_drop_solve.py:191-292), both update orders
+    transcribed with torch ops on the CPU: nodeBynode (:201-249) and
+    batchBybatch (:250-292) give the same iterates bit for bit, so either
+    order is the iteration the hook kernel runs; against the hook's CPU
+    replay (the CUDA hook's arithmetic) they agree to FP32 rounding (the
+    torch and CUDA forms round differently).  The OU form of nodeBynode
+    (:315-360) is the whole-array iteration on each OU block."""
+    import oracle
+    gen = torch.Generator().manual_seed(4)
+    bsize, numX, numY, R, C, it, beta = 2, 2, 2, 6, 5, 7, 1.9
+    r_wire = 2.93
+    inv = (torch.rand(bsize, numX, R, generator=gen) * 0.3).float()
+    x = (1.0 / (1e3 + torch.rand(numX, R, numY * C, generator=gen) * 2e4)).float()
+    g_wire = 1.0 / r_wire
+
+    def start(inv, x, numX, numY, R, C):
+        last_vup = inv.unsqueeze(-1).unsqueeze(-1).repeat(1, 1, 1, numY, C)
+        last_vdown = x.new_zeros((inv.shape[0], numX, R, numY, C))
+        return last_vup, last_vdown, last_vup.clone(), last_vdown.clone(), x.reshape(numX, R, numY, C)
+
+    def node_by_node(inv, x, numX, numY, R, C):
+        last_vup, last_vdown, now_vup, now_vdown, xx = start(inv, x, numX, numY, R, C)
+        for _ in range(it):
+            for i in range(R - 1, -1, -1):
+                for j in range(C):
+                    left = inv[..., i].unsqueeze(-1).repeat(1, 1, numY) if j == 0 else now_vup[:, :, i, :, j - 1]
+                    lu, ld = last_vup[:, :, i, :, j], last_vdown[:, :, i, :, j]
+                    xij = xx[:, i, :, j]
+                    xo = xij.mul(ld)
+                    lu1b, ld1b = lu.mul(1.0 - beta), ld.mul(1.0 - beta)
+                    if j == C - 1:
+                        now_vup[:, :, i, :, j] = (left.mul(g_wire) + xo).mul(beta).div(xij.add(g_wire)) + lu1b
+                    else:
+                        right = last_vup[:, :, i, :, j + 1]
+                        now_vup[:, :, i, :, j] = ((left + right).mul(g_wire) + xo).mul(beta).div(
+                            xij.add(2 * g_wire)) + lu1b
+                    right = inv.new_zeros((inv.shape[0], numX, numY)) if i == R - 1 else now_vdown[:, :, i + 1, :, j]
+                    xo = xij.mul(now_vup[:, :, i, :, j])
+                    if i == 0:
+                        now_vdown[:, :, i, :, j] = (right.mul(g_wire) + xo).mul(beta).div(xij.add(g_wire)) + ld1b
+                    else:
+                        left = last_vdown[:, :, i - 1, :, j]
+                        now_vdown[:, :, i, :, j] = ((left + right).mul(g_wire) + xo).mul(beta).div(
+                            xij.add(2 * g_wire)) + ld1b
+            last_vup, now_vup = now_vup, last_vup
+            last_vdown, now_vdown = now_vdown, last_vdown
+        return (last_vup - last_vdown).mul(xx).sum(2).reshape(inv.shape[0], numX, -1)
+
+    def batch_by_batch(inv, x, numX, numY, R, C):
+        last_vup, last_vdown, now_vup, now_vdown, xx = start(inv, x, numX, numY, R, C)
+        for _ in range(it):
+            left = inv.unsqueeze(-1).repeat(1, 1, 1, numY)
+            for j in range(C - 1):
+                right = last_vup[..., j + 1]
+                sumg = xx[..., j].add(g_wire * 2)
+                now_vup[..., j] = ((left + right).mul(g_wire) + xx[..., j] * last_vdown[..., j]).mul(beta).div(
+                    sumg) + last_vup[..., j].mul(1.0 - beta)
+                left = now_vup[..., j]
+            sumg = xx[..., -1].add(g_wire)
+            now_vup[..., -1] = (left.mul(g_wire) + xx[..., -1] * last_vdown[..., -1]).mul(beta).div(sumg) + \
+                last_vup[..., -1].mul(1.0 - beta)
+            down = inv.new_zeros((inv.shape[0], numX, numY, C))
+            for i in range(R - 1, 0, -1):
+                up = last_vdown[:, :, i - 1, ...]
+                sumg = xx[:, i, ...].add(g_wire * 2)
+                now_vdown[:, :, i, ...] = ((up + down).mul(g_wire) + xx[:, i, ...] * now_vup[:, :, i, ...]).mul(
+                    beta).div(sumg) + last_vdown[:, :, i, ...].mul(1.0 - beta)
+                down = now_vdown[:, :, i, ...]
+            sumg = xx[:, 0, ...].add(g_wire)
+            now_vdown[:, :, 0, ...] = (down.mul(g_wire) + xx[:, 0, ...] * now_vup[:, :, 0, ...]).mul(beta).div(
+                sumg) + last_vdown[:, :, 0, ...].mul(1.0 - beta)
+            last_vup, now_vup = now_vup, last_vup
+            last_vdown, now_vdown = now_vdown, last_vdown
+        return (last_vup - last_vdown).mul(xx).sum(2).reshape(inv.shape[0], numX, -1)
+
+    nb = node_by_node(inv, x, numX, numY, R, C).numpy()
+    bb = batch_by_batch(inv, x, numX, numY, R, C).numpy()
+    assert np.array_equal(nb, bb)
+    hk, _ = oracle.hook_gs(inv.numpy(), x.numpy(), bsize, R, C, numX, numY, it, g_wire, beta, False, 0.0,
+                           False, [])
+    np.testing.assert_allclose(nb, hk, rtol=2e-6, atol=0)
+    # OU blocks (3 x 5 of the 6 x 5 array here): nodeBynode on each block alone
+    our, nrb = 3, numX * (R // 3)
+    inv_b, x_b = inv.reshape(bsize, nrb, our), x.reshape(nrb, our, numY * C)
+    nb_ou = node_by_node(inv_b, x_b, nrb, numY, our, C).numpy()
+    assert np.array_equal(nb_ou, batch_by_batch(inv_b, x_b, nrb, numY, our, C).numpy())
+    hk_ou, _ = oracle.hook_gs(inv_b.numpy(), x_b.numpy(), bsize, our, C, nrb, numY, it, g_wire, beta, False, 0.0,
+                              False, [])
+    np.testing.assert_allclose(nb_ou, hk_ou, rtol=2e-6, atol=0)

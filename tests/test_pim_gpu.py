@@ -37,7 +37,20 @@ def _tie_bound(info, rescale, window):
     return total
 
 
-def test_converged_linear_layer_matches_exact_restatement(cuda):
+def _draws(sigma, seed=557):
+    """Variation draws in call order from a generator seeded like the
+    reference's network generator (commonConst.py:122-130)."""
+    import torch
+    gen = torch.Generator().manual_seed(seed)
+
+    def vary(G):
+        q = torch.randn(G.shape, generator=gen)
+        return (torch.from_numpy(np.ascontiguousarray(G)) / torch.exp(q * sigma)).numpy()
+    return vary
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.2])
+def test_converged_linear_layer_matches_exact_restatement(cuda, sigma):
     import torch
     from fadesim_amd import pim
     gen = torch.Generator().manual_seed(11)
@@ -46,9 +59,12 @@ def test_converged_linear_layer_matches_exact_restatement(cuda):
     w = torch.randn(insize, outsize, generator=gen)
     # 64x64 tiles keep the exact CPU solves short; the pipeline is the same
     P = pim.mlp_inference_params(nonlinear_cell_enable=False, arrRowSize=64, arrColSize=64)
+    P.sigma = sigma
+    pim.variations.reset_all()
     out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, params=P).cpu().numpy()
     ref, info = opim.split_matmul(x.numpy(), w.numpy(), cell_bits=2, rows=64, cols=64, vmax=0.2,
-                                  ron=P.Ron, roff=P.Roff, currents=_exact_currents(P.r_wire, 64))
+                                  ron=P.Ron, roff=P.Roff, currents=_exact_currents(P.r_wire, 64),
+                                  vary=_draws(sigma) if sigma else None)
     # the solve is within 1e-6 of the exact currents: an ADC input a moves by at
     # most ~1e-6 * 1023, so codes within that window of a tie may round either way
     bound = _tie_bound(info, 64 * 3 / 1023, 2e-3)
@@ -56,14 +72,17 @@ def test_converged_linear_layer_matches_exact_restatement(cuda):
     assert P.stats["solves"] > 0
 
 
-def test_reference_mode_layer_matches_hook_restatement(cuda):
+@pytest.mark.parametrize("sigma,order", [(0.0, "batchBybatch"), (0.3, "nodeBynode")])
+def test_reference_mode_layer_matches_hook_restatement(cuda, sigma, order):
     import torch
     from fadesim_amd import pim
     gen = torch.Generator().manual_seed(12)
     insize, outsize, bsize = 70, 20, 2
     x = torch.rand(bsize, insize, generator=gen) - 0.3
     w = torch.randn(insize, outsize, generator=gen)
-    P = pim.PhyArrParams(ir_mode="reference", ir_drop_iter_time=40)
+    P = pim.PhyArrParams(ir_mode="reference", ir_drop_iter_time=40, sigma=sigma,
+                         ir_drop_GS_method_mode=getattr(pim.GS_method, order))
+    pim.variations.reset_all()
     out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, params=P).cpu().numpy()
 
     def hook_currents(vin, G):
@@ -72,7 +91,8 @@ def test_reference_mode_layer_matches_hook_restatement(cuda):
         o, _ = oracle.hook_gs(vin.astype(np.float32), G.astype(np.float32), bsize_, R, 64, numX, numY,
                               40, 1 / 2.93, 1.90, False, 1e-7, False, [])
         return o
-    ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=hook_currents)
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=hook_currents,
+                                  vary=_draws(sigma) if sigma else None)
     bound = _tie_bound(info, 64 / 1023, 1e-4)
     assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
 
@@ -305,3 +325,30 @@ def test_conv2d_sp_on_the_gpu_matches_restatement(cuda):
                          (1, 1), (1, 1), (1, 1), rows=64, cols=64, currents=hook_currents)
     assert y.shape == (2, 6, 6, 5)
     np.testing.assert_allclose(y, ref, rtol=0, atol=2e-3 * np.abs(ref).max())
+
+
+def test_ou_node_by_node_layer_runs_each_ou_block_alone(cuda):
+    """OU mode with the nodeBynode order (ir_drop_solve.py:315-360): every OU
+    block iterated as a crossbar of its own with plain wire ends (no source or
+    sink loads) -- the restatement driven by the hook replay on the blocks."""
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(13)
+    insize, outsize, bsize, our, ouc = 70, 20, 2, 16, 8
+    x = torch.rand(bsize, insize, generator=gen) - 0.3
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="reference", ir_drop_iter_time=30, useOUSize=True, OUSize=(our, ouc),
+                         ir_drop_GS_method_mode=pim.GS_method.nodeBynode)
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, params=P).cpu().numpy()
+
+    def block_currents(vin, G):
+        bsize_, numX, R = vin.shape
+        nrb = numX * (R // our)
+        vb = vin.astype(np.float32).reshape(bsize_, nrb, our)
+        Gb = G.astype(np.float32).reshape(nrb, our, G.shape[2])
+        o, _ = oracle.hook_gs(vb, Gb, bsize_, our, ouc, nrb, G.shape[2] // ouc, 30, 1 / 2.93, 1.90, False, 1e-7,
+                              False, [])
+        return o
+    ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=block_currents, ou_rows=our)
+    bound = _tie_bound(info, our / 1023, 1e-4)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
