@@ -15,7 +15,11 @@ kernel (K from the autotuned table), and, when trained weights are present
 inside the timed step plus one IR2S iteration on its field (`value`, the
 metric's NN-warm-start figure; "nn_unet16" the same with the 4-level
 full-resolution network); mean sweeps of each; the roofline of the
-solve kernel from HIP events on its launch stream; the CPU baseline (the
+step's dominant kernel (the guess-first kernel when every crossbar is
+certified at its first test, else the sweeping kernel) from HIP events on
+its launch stream, with the HBM / LDS / VALU figures of the rocprofv3
+counter runs summarised in profiles/, and plain SOR against the FP32 VALU
+peak (its binding resource); the CPU baseline (the
 oracle's restatement of the reference's lexicographic solver, OpenMP over
 instances) on a bounded sample, and the measured error of every variant on
 that sample against the exact FP64 oracle (checker of the cpu_baseline leg).
