@@ -558,10 +558,10 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(ConvArgs A)
 }
 
 // ---------------------------------------------------------------- fused coarse network
-// The whole coarse32f network for one crossbar in one workgroup, every
-// activation in LDS: 32x32x8 input -> e2a, e2 (32 ch, 32x32) -> 2x2 mean ->
-// e3a, e3 (64 ch), t3 (32 ch) at 16x16 -> nearest 2x upsample + e2 -> d2
-// (32 ch) -> out (16 ch, to HBM).  Each layer is the same implicit GEMM as
+// The whole coarse network (coarse32f, or coarse16f at half the width) for
+// one crossbar in one workgroup, every activation in LDS: 32x32x8 input ->
+// e2a, e2 (C1 ch, 32x32) -> 2x2 mean -> e3a, e3 (2 C1 ch), t3 (C1 ch) at
+// 16x16 -> nearest 2x upsample + e2 -> d2 (C1 ch) -> out (16 ch, to HBM).  Each layer is the same implicit GEMM as
 // conv3x3_fwd_kernel (the same A and B fragments, the same k order), so the
 // result equals the layer-by-layer kernels'.  LDS: two 32x32x32 buffers (64
 // KB each) and one 32x32x8 / 16x16x32 buffer (16 KB).
@@ -762,56 +762,88 @@ __device__ __forceinline__ void net_conv_rows(const __bf16 *src, const __bf16 *_
     }
 }
 
+// The fused network's layers for first width C1 (C1 channels at 32x32, 2 C1
+// at 16x16): the implicit-GEMM form and the split of the 16 waves over
+// output-channel tiles and pixel tiles per layer.
+template <int C1>
+struct CoarseLayers;
+template <>
+struct CoarseLayers<32> { // coarse32f
+    static __device__ __forceinline__ void e2a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<8, 32, 32, 32, true, 1, 8>(x, w, y); }
+    static __device__ __forceinline__ void e2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 32, 32, 32, true, 1, 8>(x, w, y); }
+    static __device__ __forceinline__ void e3a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 64, 16, 16, true, 1, 4>(x, w, y); }
+    static __device__ __forceinline__ void e3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<64, 64, 16, 16, true, 1, 4>(x, w, y); }
+    static __device__ __forceinline__ void t3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<64, 32, 16, 16, true, 1, 2>(x, w, y); }
+    static __device__ __forceinline__ void d2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 32, 32, 32, true, 1, 8>(x, w, y); }
+    static __device__ __forceinline__ void out(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 16, 32, 32, false, 1, 4>(x, w, y); }
+};
+template <>
+struct CoarseLayers<16> { // coarse16f
+    static __device__ __forceinline__ void e2a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<8, 16, 32, 32, true, 1, 4>(x, w, y); }
+    static __device__ __forceinline__ void e2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 16, 32, 32, true, 1, 4>(x, w, y); }
+    static __device__ __forceinline__ void e3a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 32, 16, 16, true, 1, 2>(x, w, y); }
+    static __device__ __forceinline__ void e3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 32, 16, 16, true, 1, 2>(x, w, y); }
+    static __device__ __forceinline__ void t3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 16, 16, 16, true, 1, 1>(x, w, y); }
+    static __device__ __forceinline__ void d2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 16, 32, 32, true, 1, 4>(x, w, y); }
+    static __device__ __forceinline__ void out(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 16, 32, 32, false, 1, 4>(x, w, y); }
+};
+
+template <int C1>
 __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A)
 {
-    __shared__ __align__(16) __bf16 bufA[32 * 32 * 32];
-    __shared__ __align__(16) __bf16 bufS[32 * 32 * 32];
-    __shared__ __align__(16) __bf16 bufX[32 * 32 * 8];
+    using L = CoarseLayers<C1>;
+    constexpr int C2 = 2 * C1, G1 = C1 / 8; // 8-channel groups of a C1-channel pixel
+    // bufA: e2a / d2 (32x32xC1), or e3a and e3 (16x16xC2 each); bufS: e2 and
+    // u2 (32x32xC1); bufX: the input (32x32x8), p3 and t3 (16x16xC1)
+    constexpr int kX = 32 * 32 * 8 > 16 * 16 * C1 ? 32 * 32 * 8 : 16 * 16 * C1;
+    __shared__ __align__(16) __bf16 bufA[32 * 32 * C1];
+    __shared__ __align__(16) __bf16 bufS[32 * 32 * C1];
+    __shared__ __align__(16) __bf16 bufX[kX];
     const long b = blockIdx.x;
     const int t = threadIdx.x;
     // input features (16 bytes per copy)
     for (int e = t; e < 32 * 32; e += kNetThreads)
         *reinterpret_cast<bf16x8 *>(bufX + 8 * e) = *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
     __syncthreads();
-    net_conv<8, 32, 32, 32, true, 1, 8>(bufX, A.w[0], bufA); // e2a
+    L::e2a(bufX, A.w[0], bufA);
     __syncthreads();
-    net_conv_rows<32, 32, 32, 32, true, 1, 8>(bufA, A.w[1], bufS); // e2 (kept for the skip)
+    L::e2(bufA, A.w[1], bufS); // kept for the skip
     __syncthreads();
-    // p3: 2x2 mean of e2 -> 16x16x32 in bufX
-    for (int e = t; e < 16 * 16 * 4; e += kNetThreads) {
-        const int c = (e % 4) * 8, po = e / 4, yo = po / 16, xo = po % 16;
-        const int i00 = ((2 * yo) * 32 + 2 * xo) * 32 + c;
+    // p3: 2x2 mean of e2 -> 16x16xC1 in bufX
+    for (int e = t; e < 16 * 16 * G1; e += kNetThreads) {
+        const int c = (e % G1) * 8, po = e / G1, yo = po / 16, xo = po % 16;
+        const int i00 = ((2 * yo) * 32 + 2 * xo) * C1 + c;
         const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufS + i00);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32);
-        const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * 32);
-        const bf16x8 a3 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * 32 + 32);
+        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + C1);
+        const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * C1);
+        const bf16x8 a3 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * C1 + C1);
         bf16x8 o;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
             o[q] = (__bf16)(0.25f * ((float)a[q] + (float)a1[q] + (float)a2[q] + (float)a3[q]));
-        *reinterpret_cast<bf16x8 *>(bufX + po * 32 + c) = o;
+        *reinterpret_cast<bf16x8 *>(bufX + po * C1 + c) = o;
     }
     __syncthreads();
-    net_conv_rows<32, 64, 16, 16, true, 1, 4>(bufX, A.w[2], bufA); // e3a -> bufA[0 : 16K)
+    L::e3a(bufX, A.w[2], bufA); // -> bufA[0 : 16x16xC2)
     __syncthreads();
-    net_conv_rows<64, 64, 16, 16, true, 1, 4>(bufA, A.w[3], bufA + 16 * 16 * 64); // e3 -> bufA[16K : 32K)
+    L::e3(bufA, A.w[3], bufA + 16 * 16 * C2); // -> bufA[16x16xC2 : 2 x 16x16xC2)
     __syncthreads();
-    net_conv_rows<64, 32, 16, 16, true, 1, 2>(bufA + 16 * 16 * 64, A.w[4], bufX); // t3 -> bufX
+    L::t3(bufA + 16 * 16 * C2, A.w[4], bufX); // -> bufX
     __syncthreads();
     // u2 = nearest upsample of t3 + e2, in place in bufS
-    for (int e = t; e < 32 * 32 * 4; e += kNetThreads) {
-        const int c = (e % 4) * 8, p = e / 4, yy = p / 32, xx = p % 32;
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufX + ((yy / 2) * 16 + xx / 2) * 32 + c);
-        bf16x8 k = *reinterpret_cast<const bf16x8 *>(bufS + p * 32 + c);
+    for (int e = t; e < 32 * 32 * G1; e += kNetThreads) {
+        const int c = (e % G1) * 8, p = e / G1, yy = p / 32, xx = p % 32;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufX + ((yy / 2) * 16 + xx / 2) * C1 + c);
+        bf16x8 k = *reinterpret_cast<const bf16x8 *>(bufS + p * C1 + c);
 #pragma unroll
         for (int q = 0; q < 8; ++q)
             k[q] = (__bf16)((float)a[q] + (float)k[q]);
-        *reinterpret_cast<bf16x8 *>(bufS + p * 32 + c) = k;
+        *reinterpret_cast<bf16x8 *>(bufS + p * C1 + c) = k;
     }
     __syncthreads();
-    net_conv_rows<32, 32, 32, 32, true, 1, 8>(bufS, A.w[5], bufA); // d2
+    L::d2(bufS, A.w[5], bufA);
     __syncthreads();
-    net_conv_rows<32, 16, 32, 32, false, 1, 4>(bufA, A.w[6], A.out + b * 1024 * 16); // out
+    L::out(bufA, A.w[6], A.out + b * 1024 * 16);
 }
 
 // wT[ci][tap'*Cout + co] = w[co][(8-tap')*Cin + ci]; bias and pad columns 0.
@@ -1174,9 +1206,13 @@ static int launch_conv_lds(const ConvArgs &a, hipStream_t s)
 extern "C" int fadesim_nn_coarse_forward(int B, const void *d_x, const void *const *d_w, const int *kp, void *d_out,
                                          void *stream)
 {
-    static const int cin[7] = {8, 32, 32, 64, 64, 32, 32};
     if (B < 0 || !d_x || !d_w || !kp || !d_out)
         return set_error(FADESIM_E_INVALID, "fadesim_nn_coarse_forward: bad arguments");
+    // the first width, from the depth of the second layer (C1 -> C1)
+    const int c1 = kp[1] == net_kpad(32) ? 32 : kp[1] == net_kpad(16) ? 16 : 0;
+    if (!c1)
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_coarse_forward: layer 2 depth fits neither width 32 nor 16");
+    const int cin[7] = {8, c1, c1, 2 * c1, 2 * c1, c1, c1};
     CoarseNetArgs a{};
     a.x = (const __bf16 *)d_x;
     a.out = (__bf16 *)d_out;
@@ -1188,7 +1224,10 @@ extern "C" int fadesim_nn_coarse_forward(int B, const void *d_x, const void *con
     }
     if (B == 0)
         return FADESIM_OK;
-    hipLaunchKernelGGL(coarse_net_kernel, dim3(B), dim3(kNetThreads), 0, (hipStream_t)stream, a);
+    if (c1 == 32)
+        hipLaunchKernelGGL(coarse_net_kernel<32>, dim3(B), dim3(kNetThreads), 0, (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(coarse_net_kernel<16>, dim3(B), dim3(kNetThreads), 0, (hipStream_t)stream, a);
     NN_LAUNCH_CHECK();
     return FADESIM_OK;
 }

@@ -85,7 +85,7 @@ def main():
     ap.add_argument("--eval", type=int, default=4096)
     ap.add_argument("--max-seconds", type=float, default=600)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--arch", default="coarse32f", choices=sorted(warmstart.ARCHITECTURES))
+    ap.add_argument("--arch", default="coarse16f", choices=sorted(warmstart.ARCHITECTURES))
     args = ap.parse_args()
     dev = torch.device("cuda:0")
     log = lambda s: print(s, flush=True)

@@ -140,7 +140,24 @@ def _coarse32f() -> List[Op]:
             for op in _coarse32() if op.kind != "pool" or op.name == "p3"]
 
 
-ARCHITECTURES = {"unet16": _unet16, "slim": _slim, "coarse32": _coarse32, "coarse32f": _coarse32f}
+def _coarse16f() -> List[Op]:
+    """coarse32f at half the width (16 / 32 channels): a quarter of its
+    multiply-adds."""
+    return [
+        Op("e2a", "conv", "x", 2, N_FEAT, 16),
+        Op("e2", "conv", "e2a", 2, 16, 16),
+        Op("p3", "pool", "e2", 3, 16, 16),
+        Op("e3a", "conv", "p3", 3, 16, 32),
+        Op("e3", "conv", "e3a", 3, 32, 32),
+        Op("t3", "conv", "e3", 3, 32, 16),
+        Op("u2", "up_add", "t3", 2, 16, 16, skip="e2"),
+        Op("d2", "conv", "u2", 2, 16, 16),
+        Op("out", "conv", "d2", 2, 16, OUT_C, relu=False),
+    ]
+
+
+ARCHITECTURES = {"unet16": _unet16, "slim": _slim, "coarse32": _coarse32, "coarse32f": _coarse32f,
+                 "coarse16f": _coarse16f}
 
 
 def architecture(name: str = "unet16") -> List[Op]:
@@ -303,8 +320,9 @@ class WarmStartNet:
 
     @property
     def fused(self) -> bool:
-        """coarse32f at 128x128 runs as one kernel (fadesim_nn_coarse_forward)."""
-        return self.arch == "coarse32f" and (self.M, self.N) == (128, 128)
+        """coarse32f and coarse16f at 128x128 run as one kernel
+        (fadesim_nn_coarse_forward)."""
+        return self.arch in ("coarse32f", "coarse16f") and (self.M, self.N) == (128, 128)
 
     def forward_fused(self, x, out=None):
         """x [B][32][32][8] bf16 -> out [B][32][32][16] bf16, the whole network

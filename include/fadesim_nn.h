@@ -45,12 +45,13 @@ int fadesim_nn_combine(int B, int M, int N, const float *d_base, const float *d_
 int fadesim_nn_combine_up(int B, int M, int N, const float *d_base, const float *d_scale, const void *d_out,
                           int cout, int level, float *d_warm, void *stream);
 
-/* The coarse32f network of B crossbars in one launch, one workgroup per
- * crossbar with every activation in LDS: x [B][32][32][8] bf16 (pooled
- * features) -> out [B][32][32][16] bf16.  d_w: 7 layer weight pointers
- * (e2a, e2, e3a, e3, t3, d2, out) in fragment-major order [Kp/32][Cout][32]
- * (w'[kb][co][kk] = w[co][32 kb + kk] of the [Cout][Kp] layout), kp: their
- * padded depths. */
+/* The coarse32f (widths 32 / 64) or coarse16f (16 / 32) network of B
+ * crossbars in one launch, one workgroup per crossbar with every activation
+ * in LDS: x [B][32][32][8] bf16 (pooled features) -> out [B][32][32][16]
+ * bf16.  d_w: 7 layer weight pointers (e2a, e2, e3a, e3, t3, d2, out) in
+ * fragment-major order [Kp/32][Cout][32] (w'[kb][co][kk] = w[co][32 kb + kk]
+ * of the [Cout][Kp] layout), kp: their padded depths (the width is read from
+ * kp[1]). */
 int fadesim_nn_coarse_forward(int B, const void *d_x, const void *const *d_w, const int *kp, void *d_out,
                               void *stream);
 

@@ -126,7 +126,7 @@ def _torch_net(net, x_nhwc, wflat):
     return acts["out"].permute(0, 2, 3, 1)
 
 
-@pytest.mark.parametrize("arch", ["unet16", "coarse32", "coarse32f"])
+@pytest.mark.parametrize("arch", ["unet16", "coarse32", "coarse32f", "coarse16f"])
 def test_network_gradients_match_torch_autograd(cuda, arch):
     """Weight gradients of the whole network (MFMA forward, weight- and
     data-gradient kernels, pooling, upsampling, skip sums) against torch FP64
@@ -293,13 +293,14 @@ def test_pooled_features_match_pooled_full_features(cuda):
     assert w.shape == (4, 2, 128, 128) and bool(torch.isfinite(w).all())
 
 
-def test_fused_coarse_network_equals_layered(cuda):
+@pytest.mark.parametrize("arch", ["coarse32f", "coarse16f"])
+def test_fused_coarse_network_equals_layered(cuda, arch):
     """The one-kernel coarse network (activations in LDS) against the same
     network run layer by layer: the same GEMM fragments and order, so the
     outputs agree to the last bf16 bit."""
     import torch
     from fadesim_amd.warmstart import WarmStartNet
-    net = WarmStartNet(cuda, arch="coarse32f", seed=3)
+    net = WarmStartNet(cuda, arch=arch, seed=3)
     assert net.fused
     gen = torch.Generator().manual_seed(5)
     x = (torch.randn(37, 32, 32, 8, generator=gen) * 0.5).to(torch.bfloat16).to(cuda)
@@ -310,7 +311,7 @@ def test_fused_coarse_network_equals_layered(cuda):
     assert torch.equal(a, b), float((a.float() - b.float()).abs().max())
 
 
-@pytest.mark.parametrize("arch", ["coarse32f", "coarse32"])
+@pytest.mark.parametrize("arch", ["coarse32f", "coarse32", "coarse16f"])
 def test_warm_start_combined_at_load_equals_combined_field(cuda, arch):
     """solve_batch(warm=base, warm_coarse=out, warm_scale=s) forms the warm
     start while loading with the arithmetic of fadesim_nn_combine_up: the
