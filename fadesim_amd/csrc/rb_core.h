@@ -172,6 +172,27 @@ __device__ __forceinline__ void strow(float *p, const float (&v)[TC])
         st4(p + k, v[k], v[k + 1], v[k + 2], v[k + 3]);
 }
 
+// The same for a row in LDS (16-byte aligned).
+template <int TC>
+__device__ __forceinline__ void ldrow_lds(const float *p, float (&v)[TC])
+{
+#pragma unroll
+    for (int k = 0; k < TC; k += 4) {
+        const float4 x = *reinterpret_cast<const float4 *>(p + k);
+        v[k] = x.x;
+        v[k + 1] = x.y;
+        v[k + 2] = x.z;
+        v[k + 3] = x.w;
+    }
+}
+template <int TC>
+__device__ __forceinline__ void strow_lds(float *p, const float (&v)[TC])
+{
+#pragma unroll
+    for (int k = 0; k < TC; k += 4)
+        *reinterpret_cast<float4 *>(p + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+}
+
 // Geometry: TR x TC cells per thread, TY x TX threads, M x N cells per
 // workgroup (a whole crossbar, or one strip of full rows).
 template <int TR_, int TC_, int TY_, int TX_>
@@ -186,11 +207,8 @@ struct Geo {
     static constexpr int QLD = NT + 2 * TX;
     static constexpr int PL = 0, PR = TR * NT, QT = 2 * TR * NT, QB = 2 * TR * NT + TC * QLD;
     static constexpr int HALO = 2 * TR * NT + 2 * TC * QLD;
-    // true: the IR2S iterations reload the conductance row by row instead of
-    // holding the tile in registers (lower register pressure)
-    static constexpr bool kLean = false;
     static_assert(TR % 2 == 0 && TC % 4 == 0, "tile sides: TR even, TC multiple of 4");
-    static_assert(NT % 64 == 0 && 64 % TX == 0 || TX % 64 == 0, "tile rows must not straddle waves");
+    static_assert((NT % 64 == 0 && 64 % TX == 0) || TX % 64 == 0, "tile rows must not straddle waves");
     static_assert(NT >= M && NT >= N, "one thread per row and per column in the test");
     static_assert(N <= kRowsLd, "strip rows exceed the exchange buffer");
 };
@@ -299,6 +317,10 @@ __device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, i
 {
     constexpr int TR = G::TR, TC = G::TC, N = G::N;
     const int lvl = I.clevel, f = 1 << lvl, Hc = I.rows >> lvl, Wc = N >> lvl, HW = Hc * Wc;
+    // opaque to the optimizer: the sample positions below are computed here,
+    // not hoisted out of a caller's loop over crossbars as a register tile
+    // (which would spill)
+    asm volatile("" : "+v"(i0), "+v"(j0));
     __syncthreads();
     const unsigned *src = reinterpret_cast<const unsigned *>(I.coarse);
     float2 *st2 = reinterpret_cast<float2 *>(stage);
@@ -980,19 +1002,15 @@ __device__ __forceinline__ bool wire_iteration(const Shared &sh, const StripCtx 
 }
 
 // Ideal readout S1, the per-column floor of the relative test and the bitline
-// adjoint zq(i,j) = r sum_i' g(i',j) (n - max(i,i')) (workspace plane 0) of
-// this workgroup's rows of an n-row crossbar (strips: column sums over the
-// team).  Returns false when the team aborted.
+// adjoint zq(i,j) = r sum_i' g(i',j) (n - max(i,i')) (this thread's tile, from
+// its conductance tile gt) of this workgroup's rows of an n-row crossbar
+// (strips: column sums over the team).  Returns false when the team aborted.
 template <class G, bool STRIP>
-__device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
-                                              int i0, int j0, int t, int ty, double rwire, const Shared &sh,
-                                              int &nbar)
+__device__ __forceinline__ bool setup_tile(const Inst &I, const StripCtx &X, const float (&gt)[G::TR][G::TC],
+                                           int gi0, int i0, int j0, int t, int ty, double rwire, const Shared &sh,
+                                           int &nbar, float (&zq)[G::TR][G::TC])
 {
     constexpr int TR = G::TR, TC = G::TC, N = G::N;
-    float gt[TR][TC];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii)
-        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
     double si[TC], sf[TC];
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj)
@@ -1026,8 +1044,22 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
     if (t < N)
         // a column carrying under 1e-9 of its driven current is judged absolutely
         sh.Ifloor[t] = fmax(1e-9 * sh.Ifloor[t], 1e-300);
-    float zq[TR][TC];
-    if (!col_green_tile_f<G, STRIP>(sh, X, gt, (float)rwire, I.rows, gi0, i0, j0, ty, t, nbar, zq))
+    return col_green_tile_f<G, STRIP>(sh, X, gt, (float)rwire, I.rows, gi0, i0, j0, ty, t, nbar, zq);
+}
+
+// setup_tile on the conductance tile read from gp, the adjoint stored in
+// workspace plane 0.
+template <class G, bool STRIP>
+__device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0,
+                                              int i0, int j0, int t, int ty, double rwire, const Shared &sh,
+                                              int &nbar)
+{
+    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    float gt[TR][TC], zq[TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < TR; ++ii)
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
+    if (!setup_tile<G, STRIP>(I, X, gt, gi0, i0, j0, t, ty, rwire, sh, nbar, zq))
         return false;
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii)
@@ -1046,123 +1078,96 @@ __device__ __forceinline__ bool setup_columns(const Inst &I, const StripCtx &X, 
 // Strips first refresh the neighbours' rows (publication pub) and complete the
 // column sums over the team.  Results: colI[j], colD[j]; return value =
 // max_j |D_j|/|I_j| (negative when the team aborted).
-template <class G, bool STRIP>
-__device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0, int i0, int j0,
-                            int t, int tx, int ty, double a, double rwire, double hcoef, bool refined, bool want_rho,
-                            bool has_l, bool has_r, bool has_u, bool has_d, const Shared &sh, int pub, int &nbar)
+//
+// test_row: tile row ii (crossbar row gi0 + i0 + ii) -- its conductances
+// grow and wordline drops prow, the wordline drops left and right of the tile
+// (pl, pr; has_l / has_r false at an open end), the bitline drops of the row
+// itself (qcur), of the row above (qup; none at the top: has_up false) and of
+// the row below (qdown; none below the last row: has_down false), the stored
+// refinement residual rows rhp, rhq (refined only) and the adjoint row zrow
+// (not want_rho) -- adds its terms to Dp (adjoint-weighted residual) and Yp
+// (sum g (p + q)); want_rho returns the scaled residuals in rpf, rqf instead.
+template <class G>
+__device__ __forceinline__ void test_row(const Inst &I, int gi0, int i0, int j0, int ii, int tx, double a,
+                                         double rwire, double hcoef, bool refined, bool want_rho,
+                                         const float (&grow)[G::TC], const float (&prow)[G::TC], float pl_edge,
+                                         bool has_l, float pr_edge, bool has_r, const float (&qup)[G::TC], bool has_up,
+                                         const float (&qcur)[G::TC], const float (&qdown)[G::TC], bool has_down,
+                                         const float (&rhp)[G::TC], const float (&rhq)[G::TC],
+                                         const float (&zrow)[G::TC], double (&Dp)[G::TC], double (&Yp)[G::TC],
+                                         float (&rpf)[G::TC], float (&rqf)[G::TC])
 {
-    // the drops come from this thread's snapshot rows (planes 3, 4): the sweep
-    // state is not live across the test, which leaves it the register file
-    const float *snp = wsb + 3 * G::M * G::N, *snq = wsb + 4 * G::M * G::N;
-    constexpr int TR = G::TR, TC = G::TC, N = G::N, M = G::M, NT = G::NT, NW = G::NW, TX = G::TX,
-                  QLD = G::QLD;
-    const float *halo = sh.halo;
-    if (STRIP && !strip_fetch<G>(X, sh.halo, sh.dead, tx, ty, pub))
-        return -1.0;
-    double Dp[TC], Yp[TC];
+    constexpr int TC = G::TC, TX = G::TX;
+    const double vi = ldg(I.vin + (gi0 + i0 + ii));
+    // residuals of the row: FP64 arithmetic, FP32 storage
 #pragma unroll
-    for (int jj = 0; jj < TC; ++jj)
-        Dp[jj] = Yp[jj] = 0.0;
-    float qprev[TC], qcur[TC], qnext[TC];
-    ldrow<TC>(snq + (size_t)i0 * N + j0, qcur);
-#pragma unroll 1
-    for (int ii = 0; ii < TR; ++ii) { // rolled: one tile row at a time keeps the test's registers low
-        float grow[TC], rhp[TC], rhq[TC], prow[TC];
-        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, grow);
-        ldrow<TC>(snp + (size_t)(i0 + ii) * N + j0, prow);
-        if (ii < TR - 1)
-            ldrow<TC>(snq + (size_t)(i0 + ii + 1) * N + j0, qnext);
+    for (int jj = 0; jj < TC; ++jj) {
+        const double x = (double)prow[jj], z = (double)qcur[jj], gd = (double)grow[jj];
+        const double cell = gd * (x + z);
+        double bp, bq;
         if (refined) {
-            ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rhp);
-            ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rhq);
+            bp = (double)rhp[jj] / hcoef;
+            bq = (double)rhq[jj] / hcoef;
+        } else {
+            bp = bq = gd * (vi - ldg(I.vout + (j0 + jj)));
         }
-        float zrow[TC];
-        if (!want_rho)
-            ldrow<TC>(wsb + (size_t)(i0 + ii) * N + j0, zrow);
-        const double vi = ldg(I.vin + (gi0 + i0 + ii));
-        float rpf[TC], rqf[TC]; // residuals of the row (FP64 arithmetic, FP32 storage)
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            const double x = (double)prow[jj], z = (double)qcur[jj], gd = (double)grow[jj];
-            const double cell = gd * (x + z);
-            double bp, bq;
-            if (refined) {
-                bp = (double)rhp[jj] / hcoef;
-                bq = (double)rhq[jj] / hcoef;
-            } else {
-                bp = bq = gd * (vi - ldg(I.vout + (j0 + jj)));
-            }
-            double pl = 0.0;
-            if (jj > 0)
-                pl = (double)prow[jj - 1];
-            else if (has_l)
-                pl = (double)halo[G::PR + ii * NT + t - 1];
-            double wp;
-            if (jj < TC - 1)
-                wp = a * ((pl - x) + ((double)prow[jj + 1] - x));
-            else if (has_r)
-                wp = a * ((pl - x) + ((double)halo[G::PL + ii * NT + t + 1] - x));
-            else
-                wp = a * (pl - x); // open right end
-            double qd = 0.0;       // sink below the last row
-            if (ii < TR - 1)
-                qd = (double)qnext[jj];
-            else if (has_d)
-                qd = (double)halo[G::QT + jj * QLD + 2 * TX + t];
-            double wq;
-            if (ii > 0)
-                wq = a * (((double)qprev[jj] - z) + (qd - z));
-            else if (has_u)
-                wq = a * (((double)halo[G::QB + jj * QLD + t] - z) + (qd - z));
-            else
-                wq = a * (qd - z); // open top
-            const double rpv = (bp - cell) + wp, rqv = (bq - cell) + wq;
-            if (want_rho) {
-                rpf[jj] = (float)(rpv * hcoef);
-                rqf[jj] = (float)(rqv * hcoef);
-            } else {
-                rpf[jj] = (float)rpv;
-                Dp[jj] += (double)zrow[jj] * rqv; // bitline part of the adjoint
-                Yp[jj] += cell;
-            }
-        }
+        double pl = 0.0;
+        if (jj > 0)
+            pl = (double)prow[jj - 1];
+        else if (has_l)
+            pl = (double)pl_edge;
+        double wp;
+        if (jj < TC - 1)
+            wp = a * ((pl - x) + ((double)prow[jj + 1] - x));
+        else if (has_r)
+            wp = a * ((pl - x) + ((double)pr_edge - x));
+        else
+            wp = a * (pl - x); // open right end
+        const double qd = has_down ? (double)qdown[jj] : 0.0; // sink below the last row
+        double wq;
+        if (has_up)
+            wq = a * (((double)qup[jj] - z) + (qd - z));
+        else
+            wq = a * (qd - z); // open top
+        const double rpv = (bp - cell) + wp, rqv = (bq - cell) + wq;
         if (want_rho) {
-            strow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rpf);
-            strow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rqf);
-#pragma unroll
-            for (int jj = 0; jj < TC; ++jj) {
-                qprev[jj] = qcur[jj];
-                qcur[jj] = qnext[jj];
-            }
-            continue;
-        }
-        // wordline part: g r W_i(j), W by a scan over the thread row
-        double L0 = 0.0, L1 = 0.0;
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            L0 += (double)rpf[jj];
-            L1 += (double)(j0 + jj + 1) * (double)rpf[jj];
-        }
-        double P0, P1, R, R1;
-        row_scan<TX>(L0, tx, P0, R);
-        row_scan<TX>(L1, tx, P1, R1);
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            const double jp1 = (double)(j0 + jj + 1);
-            Dp[jj] += (double)grow[jj] * rwire * (P1 + jp1 * (R - P0));
-            P0 += (double)rpf[jj];
-            P1 += jp1 * (double)rpf[jj];
-        }
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            qprev[jj] = qcur[jj];
-            qcur[jj] = qnext[jj];
+            rpf[jj] = (float)(rpv * hcoef);
+            rqf[jj] = (float)(rqv * hcoef);
+        } else {
+            rpf[jj] = (float)rpv;
+            Dp[jj] += (double)zrow[jj] * rqv; // bitline part of the adjoint
+            Yp[jj] += cell;
         }
     }
-    if (want_rho) {
-        __syncthreads();
-        return 0.0;
+    if (want_rho)
+        return;
+    // wordline part: g r W_i(j), W by a scan over the thread row
+    double L0 = 0.0, L1 = 0.0;
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        L0 += (double)rpf[jj];
+        L1 += (double)(j0 + jj + 1) * (double)rpf[jj];
     }
+    double P0, P1, R, R1;
+    row_scan<TX>(L0, tx, P0, R);
+    row_scan<TX>(L1, tx, P1, R1);
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj) {
+        const double jp1 = (double)(j0 + jj + 1);
+        Dp[jj] += (double)grow[jj] * rwire * (P1 + jp1 * (R - P0));
+        P0 += (double)rpf[jj];
+        P1 += jp1 * (double)rpf[jj];
+    }
+}
+
+// test_reduce: the column sums of every thread's Dp, Yp (strips: over the
+// team), colD, colI = S1 - sum g (p + q), and the largest |D_j| / |I_j|,
+// uniform over the workgroup (negative when the team aborted).
+template <class G, bool STRIP>
+__device__ __forceinline__ double test_reduce(const Shared &sh, const StripCtx &X, const double (&Dp)[G::TC],
+                                              const double (&Yp)[G::TC], int t, int ty, int j0, int &nbar)
+{
+    constexpr int N = G::N, NW = G::NW;
     col_reduce2<G>(sh, Dp, Yp, t, ty, j0, sh.colD, sh.colI); // colI holds sum g (p + q) for now
     if (STRIP) {
         double *mine = X.part + (((size_t)(nbar & 1) * X.S + X.s) * 2) * kRowsLd;
@@ -1200,6 +1205,70 @@ __device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const 
     em = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, em)));
     __syncthreads(); // wmax / colI reads done before reuse
     return (double)em;
+}
+
+// conv_test: the drops from this thread's snapshot rows (planes 3, 4; the
+// sweep state is not live across the test, which leaves it the register
+// file), the adjoint from plane 0, the refinement residual from planes 1, 2.
+template <class G, bool STRIP>
+__device__ double conv_test(const Inst &I, const StripCtx &X, float *wsb, const float *gp, int gi0, int i0, int j0,
+                            int t, int tx, int ty, double a, double rwire, double hcoef, bool refined, bool want_rho,
+                            bool has_l, bool has_r, bool has_u, bool has_d, const Shared &sh, int pub, int &nbar)
+{
+    const float *snp = wsb + 3 * G::M * G::N, *snq = wsb + 4 * G::M * G::N;
+    constexpr int TR = G::TR, TC = G::TC, N = G::N, M = G::M, NT = G::NT, TX = G::TX, QLD = G::QLD;
+    const float *halo = sh.halo;
+    if (STRIP && !strip_fetch<G>(X, sh.halo, sh.dead, tx, ty, pub))
+        return -1.0;
+    double Dp[TC], Yp[TC];
+#pragma unroll
+    for (int jj = 0; jj < TC; ++jj)
+        Dp[jj] = Yp[jj] = 0.0;
+    float qprev[TC], qcur[TC], qnext[TC];
+    ldrow<TC>(snq + (size_t)i0 * N + j0, qcur);
+#pragma unroll 1
+    for (int ii = 0; ii < TR; ++ii) { // rolled: one tile row at a time keeps the test's registers low
+        float grow[TC], rhp[TC], rhq[TC], prow[TC], zrow[TC], rpf[TC], rqf[TC];
+        ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, grow);
+        ldrow<TC>(snp + (size_t)(i0 + ii) * N + j0, prow);
+        if (ii < TR - 1)
+            ldrow<TC>(snq + (size_t)(i0 + ii + 1) * N + j0, qnext);
+        if (refined) {
+            ldrow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rhp);
+            ldrow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rhq);
+        }
+        if (!want_rho)
+            ldrow<TC>(wsb + (size_t)(i0 + ii) * N + j0, zrow);
+        // the neighbours beyond the tile's edges from the LDS halo
+        const float pl = has_l ? halo[G::PR + ii * NT + t - 1] : 0.f;
+        const float pr = has_r ? halo[G::PL + ii * NT + t + 1] : 0.f;
+        if (ii == 0 && has_u) {
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                qprev[jj] = halo[G::QB + jj * QLD + t];
+        }
+        if (ii == TR - 1 && has_d) {
+#pragma unroll
+            for (int jj = 0; jj < TC; ++jj)
+                qnext[jj] = halo[G::QT + jj * QLD + 2 * TX + t];
+        }
+        test_row<G>(I, gi0, i0, j0, ii, tx, a, rwire, hcoef, refined, want_rho, grow, prow, pl, has_l, pr, has_r,
+                    qprev, ii > 0 || has_u, qcur, qnext, ii < TR - 1 || has_d, rhp, rhq, zrow, Dp, Yp, rpf, rqf);
+        if (want_rho) {
+            strow<TC>(wsb + 1 * M * N + (size_t)(i0 + ii) * N + j0, rpf);
+            strow<TC>(wsb + 2 * M * N + (size_t)(i0 + ii) * N + j0, rqf);
+        }
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj) {
+            qprev[jj] = qcur[jj];
+            qcur[jj] = qnext[jj];
+        }
+    }
+    if (want_rho) {
+        __syncthreads();
+        return 0.0;
+    }
+    return test_reduce<G, STRIP>(sh, X, Dp, Yp, t, ty, j0, nbar);
 }
 
 // Estimated relative error of the zero-drop guess, from the test just run:
@@ -1392,26 +1461,21 @@ __device__ __attribute__((noinline)) void phase_coarse(Inst I, StripCtx X, float
 // max(|1-w|, |1-w(1+rho)|) -- rho/(2+rho) at w = 2/(2+rho).  Returns false
 // when the team aborted.
 // The caller's sweep state is dead across the call (it is reloaded from
-// planes 3, 4 afterwards), so the drops stay in VGPRs for all iterations
-// (wire_iteration); per iteration only the conductance is read, twice, from
-// the cache.  start: the drops begin at zero (kStartZero), at the warm start
-// (kStartWarm), or at the warm start plus the network's coarse correction
-// combined while loading (kStartWarmCoarse, whole crossbars) -- read from
-// HBM here, not from a workspace plane.  The result goes to planes 3, 4;
-// PUBLISH (whole crossbars) also publishes its halo and ends with a barrier,
-// for a caller that only tests it.
+// planes 3, 4 afterwards), so the drops and the conductance tile stay in
+// VGPRs for all iterations (wire_iteration).  start: the drops begin at zero
+// (kStartZero), at the warm start (kStartWarm), or at the warm start plus the
+// network's coarse correction combined while loading (kStartWarmCoarse, whole
+// crossbars) -- read from HBM here, not from a workspace plane.  The result
+// goes to planes 3, 4.
 constexpr int kStartZero = 0, kStartWarm = 1, kStartWarmCoarse = 2;
 
-template <class G, bool STRIP, bool PUBLISH = false>
-__device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
-                                                     int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
-                                                     int *nbar, int iters, double w, int start)
+// The starting drops of the wire-solve iterations (start as above) into the
+// register tile p, q.
+template <class G, bool STRIP>
+__device__ __forceinline__ void load_start(const Inst &I, const Shared &sh, int t, int gi0, int i0, int j0, int start,
+                                           float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC])
 {
-    RB_SH_LOCAL;
-    static_assert(!(STRIP && PUBLISH), "strips publish through the team exchange");
     constexpr int TC = G::TC, N = G::N;
-    float p[G::TR][TC], q[G::TR][TC];
-    start = uniform_i32(start);
     if (start == kStartZero) {
 #pragma unroll
         for (int ii = 0; ii < G::TR; ++ii)
@@ -1428,23 +1492,29 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
         if (!STRIP && start == kStartWarmCoarse)
             add_coarse<G>(I, sh.halo, t, gi0, i0, j0, p, q);
     }
-    // the tile's conductances (lean geometries: only the line voltages) stay
-    // in registers for all iterations: an iteration reads no memory but LDS
-    constexpr bool kHold = !G::kLean;
-    float gt[kHold ? G::TR : 1][TC], vi[G::TR], vo[TC];
+}
+
+// `iters` damped wire-solve iterations on the register tile p, q.  HOLD: the
+// tile's conductances come from gt (held in registers by the caller), else
+// every iteration reads them from gp (the cache) row by row and gt is unused.
+template <class G, bool STRIP, bool HOLD>
+__device__ __forceinline__ bool ir2s_iterate(const Inst &I, const StripCtx &X, const Shared &sh,
+                                             const float (&gt)[G::TR][G::TC], const float *gp, int gi0,
+                                             int i0, int j0, int t, int tx, int ty, double rwire, int &nbar, int iters,
+                                             double w, float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC])
+{
+    constexpr int TC = G::TC;
+    float vi[G::TR], vo[TC];
 #pragma unroll
-    for (int ii = 0; ii < G::TR; ++ii) {
-        if constexpr (kHold)
-            ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
+    for (int ii = 0; ii < G::TR; ++ii)
         vi[ii] = (float)ldg(I.vin + (gi0 + i0 + ii));
-    }
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj)
         vo[jj] = (float)ldg(I.vout + (j0 + jj));
     // s_ij = g_ij (c_ij - p_ij - q_ij) for both line solves
     auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&b)[TC]) {
         float gr[TC];
-        if constexpr (kHold) {
+        if constexpr (HOLD) {
 #pragma unroll
             for (int jj = 0; jj < TC; ++jj)
                 gr[jj] = gt[ii][jj];
@@ -1466,14 +1536,30 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj)
             asm volatile("" : "+v"(vo[jj]));
-        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, *nbar, p, q))
+        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, nbar, p, q))
             return false;
     }
+    return true;
+}
+
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                     int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
+                                                     int *nbar, int iters, double w, int start)
+{
+    RB_SH_LOCAL;
+    constexpr int TC = G::TC;
+    float p[G::TR][TC], q[G::TR][TC];
+    load_start<G, STRIP>(I, sh, t, gi0, i0, j0, uniform_i32(start), p, q);
+    // the tile's conductances stay in registers for all iterations: an
+    // iteration reads no memory but LDS
+    float gt[G::TR][TC];
+#pragma unroll
+    for (int ii = 0; ii < G::TR; ++ii)
+        ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
+    if (!ir2s_iterate<G, STRIP, true>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar, iters, w, p, q))
+        return false;
     tile_store<G>(wsb, i0, j0, p, q);
-    if constexpr (PUBLISH) {
-        publish_all<G>(p, q, sh.halo, t);
-        __syncthreads();
-    }
     return true;
 }
 
@@ -1791,101 +1877,112 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
     return true;
 }
 
-// A geometry of its own type for the guess-first kernel: the phase functions
-// instantiated for it are called only from that kernel (and compiled for its
-// register needs), and its IR2S iterations reload the conductance.
-template <class G>
-struct GuessGeo : G {
-    static constexpr bool kLean = true;
-};
-
 // The part of solve() before its first sweep, for whole crossbars whose
 // guess is tested first (FADESIM_F_TEST_FIRST, not FADESIM_F_FIXED_SWEEPS):
 // the set-up, the initial guess (warm start, coarse combine, IR2S
 // iterations), its test, and -- when a given guess is judged worse than the
 // zero guess -- the zero guess with its IR2S iterations and their test.  The
-// same phase functions in the same order as solve(), so whatever this writes
-// equals what solve() writes bit for bit.  Returns true when a test ended the
-// solve before any sweep (certified, or not finite), with every output
-// written as solve() writes it; false when the solve needs sweeps (no output
-// written; solve() then runs the crossbar from the start).
-template <class G>
-__device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *wsb, const Shared &sh)
+// same arithmetic in the same order as solve(), so whatever this writes
+// equals what solve() writes bit for bit.  The drops go from the IR2S
+// iterations' registers into fld (LDS, [2][M][N], overlaying the halo and
+// the column buffer, both idle during the test), where the test reads them
+// with their neighbours; only the adjoint crosses to the test through the
+// workspace (plane 0).  HOLD: the iterations keep the conductance tile in
+// registers, else they reread it from the cache row by row.  Returns true
+// when a test ended the solve before any sweep (certified, or not finite),
+// with every output written as solve() writes it; false when the solve needs
+// sweeps (no output written but the field, which solve() then rewrites as it
+// runs the crossbar from the start).
+template <class G, bool HOLD>
+__device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *wsb, const Shared &sh, float *fld)
 {
     constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY;
     constexpr int M = G::M, N = G::N;
-    float *halo = sh.halo;
     const StripCtx X{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const int t = threadIdx.x;
     const int tx = t % TX, ty = t / TX;
-    const int gi0 = 0;
     const float *gp = I.g;
     const int i0 = TR * ty, j0 = TC * tx;
-    const bool has_l = tx > 0, has_r = tx < TX - 1, has_u = ty > 0, has_d = ty < TY - 1;
+    const bool has_l = tx > 0, has_r = tx < TX - 1;
     const double rwire = I.rwire;
     const double a = 1.0 / rwire;
     const double hcoef = 0.5 * (double)C.omega * rwire;
     const bool dwr = (C.flags & FADESIM_F_DWR) != 0;
+    float *fp = fld, *fq = fld + M * N;
     int nbar = 0;
     if (t == 0)
         *sh.dead = 0;
     __syncthreads();
 
-    phase_setup<G, false>(I, X, wsb, gp, gi0, i0, j0, t, ty, rwire, RB_SH_ARGS(sh), &nbar);
+    setup_columns<G, false>(I, X, wsb, gp, 0, i0, j0, t, ty, rwire, sh, nbar);
 
-    // the guess into planes 3, 4 (the snapshot the test reads) and the halo;
-    // the drops themselves are not needed in registers here
-    float yp[TR][TC], yq[TR][TC];
     const bool coarse = I.coarse != nullptr;
-    if (C.ir2s_iters > 0) {
-        const int start = !I.warm ? kStartZero : coarse ? kStartWarmCoarse : kStartWarm;
-        phase_ir2s<G, false, true>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
-                                   C.ir2s_iters, C.ir2s_omega, start);
-    } else {
-        if (coarse) {
-            phase_coarse<G, false>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
-            tile_load<G>(wsb, i0, j0, yp, yq);
-        } else {
+    const int iters = uniform_i32(C.ir2s_iters);
+    // the drops of the guess into fld (and the field output)
+    auto guess = [&](int start) {
+        float p[TR][TC], q[TR][TC];
+        load_start<G, false>(I, sh, t, 0, i0, j0, start, p, q);
+        if (iters > 0) {
+            float gt[TR][TC];
+            if constexpr (HOLD) {
+#pragma unroll
+                for (int ii = 0; ii < TR; ++ii)
+                    ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
+            }
+            ir2s_iterate<G, false, HOLD>(I, X, sh, gt, gp, 0, i0, j0, t, tx, ty, rwire, nbar, iters, C.ir2s_omega, p,
+                                         q);
+        }
+        __syncthreads(); // the column buffer and the halo stage are idle
+#pragma unroll
+        for (int ii = 0; ii < TR; ++ii) {
+            strow_lds<TC>(fp + (i0 + ii) * N + j0, p[ii]);
+            strow_lds<TC>(fq + (i0 + ii) * N + j0, q[ii]);
+        }
+        if (I.field) {
 #pragma unroll
             for (int ii = 0; ii < TR; ++ii) {
-                if (I.warm) {
-                    ldrow<TC>(I.warm + (size_t)(i0 + ii) * N + j0, yp[ii]);
-                    ldrow<TC>(I.warm + (size_t)M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
-                } else {
-#pragma unroll
-                    for (int jj = 0; jj < TC; ++jj)
-                        yp[ii][jj] = yq[ii][jj] = 0.f;
-                }
+                strow<TC>(I.field + (size_t)(i0 + ii) * N + j0, p[ii]);
+                strow<TC>(I.field + (size_t)M * N + (size_t)(i0 + ii) * N + j0, q[ii]);
             }
-            tile_store<G>(wsb, i0, j0, yp, yq);
         }
-        publish_all<G>(yp, yq, halo, t);
         __syncthreads();
-    }
-    // the test's value is the same in every lane but comes back from the call
-    // in VGPRs: made a scalar, every decision below is a uniform branch (a
-    // branch the compiler takes as divergent lets it split lanes of one wave
-    // across the caller's loop and its barriers)
-    double est = uniform_f64(phase_test<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false,
-                                                  false, has_l, has_r, has_u, has_d, RB_SH_ARGS(sh), -1, &nbar));
+    };
+    // the test, one tile row at a time from fld; its value is the same in
+    // every lane, made a scalar so that every decision below is a uniform
+    // branch
+    auto test = [&]() {
+        double Dp[TC], Yp[TC];
+#pragma unroll
+        for (int jj = 0; jj < TC; ++jj)
+            Dp[jj] = Yp[jj] = 0.0;
+#pragma unroll 1
+        for (int ii = 0; ii < TR; ++ii) {
+            const int i = i0 + ii;
+            float grow[TC], prow[TC], qup[TC], qcur[TC], qdown[TC], zrow[TC], rpf[TC], rqf[TC];
+            ldrow<TC>(gp + (size_t)i * N + j0, grow);
+            ldrow<TC>(wsb + (size_t)i * N + j0, zrow);
+            ldrow_lds<TC>(fp + i * N + j0, prow);
+            ldrow_lds<TC>(fq + i * N + j0, qcur);
+            const bool up = i > 0, down = i < M - 1;
+            if (up)
+                ldrow_lds<TC>(fq + (i - 1) * N + j0, qup);
+            if (down)
+                ldrow_lds<TC>(fq + (i + 1) * N + j0, qdown);
+            const float pl = has_l ? fp[i * N + j0 - 1] : 0.f;
+            const float pr = has_r ? fp[i * N + j0 + TC] : 0.f;
+            test_row<G>(I, 0, i0, j0, ii, tx, a, rwire, hcoef, false, false, grow, prow, pl, has_l, pr, has_r, qup,
+                        up, qcur, qdown, down, zrow, zrow, zrow, Dp, Yp, rpf, rqf);
+        }
+        __syncthreads(); // fld reads done: the column sums reuse its memory
+        return uniform_f64(test_reduce<G, false>(sh, X, Dp, Yp, t, ty, j0, nbar));
+    };
+    guess(!I.warm ? kStartZero : coarse ? kStartWarmCoarse : kStartWarm);
+    double est = test();
     bool rejected = false;
     if ((I.warm != nullptr || coarse) && !(est <= zero_guess_est<G>(sh, t))) {
         rejected = true;
-        if (C.ir2s_iters > 0) {
-            phase_ir2s<G, false, true>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
-                                       C.ir2s_iters, C.ir2s_omega, kStartZero);
-        } else {
-#pragma unroll
-            for (int ii = 0; ii < TR; ++ii)
-#pragma unroll
-                for (int jj = 0; jj < TC; ++jj)
-                    yp[ii][jj] = yq[ii][jj] = 0.f;
-            tile_store<G>(wsb, i0, j0, yp, yq);
-            publish_all<G>(yp, yq, halo, t);
-            __syncthreads();
-        }
-        est = uniform_f64(phase_test<G, false>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, a, rwire, hcoef, false, false,
-                                               has_l, has_r, has_u, has_d, RB_SH_ARGS(sh), -1, &nbar));
+        guess(kStartZero);
+        est = test();
     }
     const bool nonfinite = !(est < INFINITY);
     if (!nonfinite && !(est <= (dwr ? C.tol_eff * C.dwr_scale : C.tol_eff)))
@@ -1907,15 +2004,7 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
         if (I.rejected)
             *I.rejected = rejected ? 1 : 0;
     }
-    if (I.field) {
-        tile_load<G>(wsb, i0, j0, yp, yq); // the tested field (snapshot planes)
-#pragma unroll
-        for (int ii = 0; ii < TR; ++ii) {
-            strow<TC>(I.field + (size_t)(i0 + ii) * N + j0, yp[ii]);
-            strow<TC>(I.field + (size_t)M * N + (size_t)(i0 + ii) * N + j0, yq[ii]);
-        }
-    }
-    __syncthreads(); // colI / halo reads done before the next instance reuses them
+    __syncthreads(); // colI reads done before the next instance reuses it
     return true;
 }
 

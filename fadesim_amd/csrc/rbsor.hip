@@ -107,25 +107,35 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
 
 // The guess-first part of every solve of the batch (rb::guess_solve), one
 // workgroup per crossbar as in rbsor_kernel.  It carries none of the sweep
-// loop's state, and its phases are instantiated for it alone (GuessGeo: the
-// IR2S iterations reload the conductance row by row instead of holding the
-// tile), so set-up, guess and test cost less here than inside rbsor_kernel.
-// A crossbar whose solve ended at a test before any sweep has its outputs
-// written; every other one is appended to list (list_count entries).
+// loop's state: the drops stay in registers through the IR2S iterations and
+// go to LDS for the test, so of the workspace only the adjoint plane is
+// written and read.  A crossbar whose solve ended at a test before any sweep
+// has its outputs written; every other one is appended to list (list_count
+// entries).  kGuessHoldsTile: the IR2S iterations hold the conductance tile in
+// registers instead of rereading it from the cache.
+constexpr bool kGuessHoldsTile = false;
+
 template <class G>
 __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
 {
-    using GG = rb::GuessGeo<G>;
     constexpr int M = G::M, N = G::N;
-    __shared__ __align__(16) float halo[G::HALO];
+    // the tested drops ([2][M][N]) overlay the halo (here only the stage of
+    // the coarse combine) and the column buffer, both idle during the test
+    constexpr int kHaloB = G::HALO * (int)sizeof(float), kCbufB = 2 * G::TY * N * (int)sizeof(double);
+    constexpr int kFldB = 2 * M * N * (int)sizeof(float);
+    constexpr int kBigB = kHaloB + kCbufB > kFldB ? kHaloB + kCbufB : kFldB;
+    static_assert(kHaloB % 16 == 0, "column buffer alignment");
+    __shared__ __align__(16) unsigned char big[kBigB];
     __shared__ double S1[N];
     __shared__ double Ifloor[N];
     __shared__ double colI[N];
     __shared__ double colD[N];
     __shared__ float wmax[G::NW];
     __shared__ int s_inst, s_dead;
-    __shared__ double cbuf[2 * G::TY * N];
     __shared__ double cext[3 * N];
+    float *halo = reinterpret_cast<float *>(big);
+    double *cbuf = reinterpret_cast<double *>(big + kHaloB);
+    float *fld = reinterpret_cast<float *>(big);
     rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     for (;;) {
@@ -153,7 +163,7 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
         I.cscale = A.coarse ? A.cscale[b] : 0.f;
         I.cch = A.cch;
         I.clevel = A.clevel;
-        const bool done = rb::guess_solve<GG>(I, A.ctl, wsb, sh);
+        const bool done = rb::guess_solve<G, kGuessHoldsTile>(I, A.ctl, wsb, sh, fld);
         if (!done && threadIdx.x == 0)
             A.list[atomicAdd(A.list_count, 1)] = b;
         // a barrier between this thread-0 append and the next iteration's
