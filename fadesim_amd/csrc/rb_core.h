@@ -1899,7 +1899,11 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
     constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY;
     constexpr int M = G::M, N = G::N;
     const StripCtx X{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    const int t = threadIdx.x;
+    // opaque to the optimizer: addresses derived from the thread index are
+    // computed for each crossbar, not hoisted out of the caller's loop over
+    // crossbars and held (spilled) for the kernel's lifetime
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
     const int tx = t % TX, ty = t / TX;
     const float *gp = I.g;
     const int i0 = TR * ty, j0 = TC * tx;

@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
     constexpr int kHaloB = G::HALO * (int)sizeof(float), kCbufB = 2 * G::TY * N * (int)sizeof(double);
     constexpr int kFldB = 2 * M * N * (int)sizeof(float);
     constexpr int kBigB = kHaloB + kCbufB > kFldB ? kHaloB + kCbufB : kFldB;
-    static_assert(kHaloB % 16 == 0, "column buffer alignment");
+    static_assert(kCbufB % 16 == 0, "halo alignment");
     __shared__ __align__(16) unsigned char big[kBigB];
     __shared__ double S1[N];
     __shared__ double Ifloor[N];
@@ -133,8 +133,10 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
     __shared__ float wmax[G::NW];
     __shared__ int s_inst, s_dead;
     __shared__ double cext[3 * N];
-    float *halo = reinterpret_cast<float *>(big);
-    double *cbuf = reinterpret_cast<double *>(big + kHaloB);
+    // the column buffer first: its rows are then in reach of the LDS
+    // instructions' immediate offsets from one address register
+    double *cbuf = reinterpret_cast<double *>(big);
+    float *halo = reinterpret_cast<float *>(big + kCbufB);
     float *fld = reinterpret_cast<float *>(big);
     rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
