@@ -13,8 +13,9 @@ zero drops (plain SOR), K wire-solve (IR2S) iterations inside the solve
 kernel (K from the autotuned table), and, when trained weights are present
 (fadesim_amd/weights), the learned warm start with the network's inference
 inside the timed step plus one IR2S iteration on its field (`value`, the
-metric's NN-warm-start figure; "nn_unet16" the same with the 4-level
-full-resolution network); mean sweeps of each; the roofline of the
+metric's NN-warm-start figure; "nn_fused" the same start formed inside
+the solve's guess kernel, bit for bit the same outputs; "nn_unet16" the same
+with the 4-level full-resolution network); mean sweeps of each; the roofline of the
 step's dominant kernel (the guess-first kernel when every crossbar is
 certified at its first test, else the sweeping kernel) from HIP events on
 its launch stream, with the HBM / LDS / VALU figures of the rocprofv3
@@ -234,7 +235,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-images", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=256)
-    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn,nn_unet16: run only these solve variants")
+    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn,nn_fused,nn_unet16: run only these solve variants")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -266,26 +267,36 @@ def main():
     K = ir2s_iters_128(dev)
     only = set(x for x in args.only.split(",") if x)
 
+    # name: (IR2S iterations of the solve, passes over the conductance per
+    # crossbar before the sweeps, warm field read from HBM, coarse output
+    # read from HBM, step)
     variants = {
         # zero-drop initial guess, red-black SOR sweeps
-        "plain": (0, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, timing=timing)),
+        "plain": (0, 0, False, False, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, timing=timing)),
         # K wire-solve (IR2S) iterations on the zero guess inside the solve kernel
-        "ir2s": (K, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, ir2s_iters=K, corr_iters=K,
-                                                    timing=timing)),
+        "ir2s": (K, K, False, False, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, ir2s_iters=K,
+                                                                    corr_iters=K, timing=timing)),
     }
     if net is not None:
-        # the network's field, one IR2S iteration on it, then the same test / SOR
-        # (the network output is combined with the base field while the solve loads it)
-        variants["nn"] = (1, lambda timing=False: solve_batch(tg, tr, tv, **net.warm_kwargs(tg, tr, tv),
-                                                              tol=args.tol, ir2s_iters=1, corr_iters=K,
-                                                              test_first=True, timing=timing))
+        # the network's field, one IR2S iteration on it, then the same test /
+        # SOR: features and base field, network, and the network output
+        # combined with the base field while the solve loads it
+        variants["nn"] = (1, 1, True, True, lambda timing=False: solve_batch(
+            tg, tr, tv, **net.warm_kwargs(tg, tr, tv), tol=args.tol, ir2s_iters=1, corr_iters=K, test_first=True,
+            timing=timing))
+    if net is not None and net.solves_in_kernel:
+        # the same start formed inside the solve's guess kernel (its features =
+        # two wire-solve passes from zero, the network, the combine): the same
+        # outputs bit for bit
+        variants["nn_fused"] = (1, 3, False, False, lambda timing=False: solve_batch(
+            tg, tr, tv, net=net, tol=args.tol, ir2s_iters=1, corr_iters=K, timing=timing))
     # the 4-level full-resolution encoder-decoder (~340k parameters), for comparison
     net16 = None
     if os.path.exists(os.path.join(warmstart.WEIGHTS_DIR, "warmstart_128_unet16.bin")):
         net16 = warmstart.WarmStartNet.load(dev, name="warmstart_128_unet16")
-        variants["nn_unet16"] = (1, lambda timing=False: solve_batch(tg, tr, tv, warm=net16.predict(tg, tr, tv),
-                                                                     tol=args.tol, ir2s_iters=1, corr_iters=K,
-                                                                     test_first=True, timing=timing))
+        variants["nn_unet16"] = (1, 1, True, False, lambda timing=False: solve_batch(
+            tg, tr, tv, warm=net16.predict(tg, tr, tv), tol=args.tol, ir2s_iters=1, corr_iters=K, test_first=True,
+            timing=timing))
     if only:
         variants = {k: v for k, v in variants.items() if k in only}
 
@@ -305,7 +316,9 @@ def main():
         return dt, kms, gms, torch.stack(sweeps).to(torch.float64), res
 
     meas, last = {}, {}
-    for name, (k_ir2s, step) in variants.items():
+    spec = {}
+    for name, (k_ir2s, pre_passes, warm_rd, coarse_rd, step) in variants.items():
+        spec[name] = (pre_passes, warm_rd, coarse_rd)
         dt, kms, gms, sw, res = run(step)
         last[name] = res
         if int((res.status != 0).sum()) != 0:
@@ -314,9 +327,8 @@ def main():
             raise RuntimeError(f"{name}: the warm start was rejected for {int(res.rejected.sum())} crossbars")
         kernel_ms = float(np.mean(kms))
         guess_ms = float(np.mean(gms)) if min(gms) > 0 else None
-        passes = float(sw.sum()) / args.steps + k_ir2s * B
-        ach = algorithmic_bytes(passes, B, warm=name.startswith("nn"),
-                                coarse=name == "nn") / (kernel_ms * 1e-3) / 1e9
+        passes = float(sw.sum()) / args.steps + pre_passes * B
+        ach = algorithmic_bytes(passes, B, warm=warm_rd, coarse=coarse_rd) / (kernel_ms * 1e-3) / 1e9
         meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                       "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
                       "guess_kernel_ms": guess_ms, "kernel_GBps_algorithmic": ach}
@@ -333,6 +345,7 @@ def main():
                                         "plain": "zero-drop"}[head]},
            "plain": meas.get("plain"), "ir2s_warm": meas.get("ir2s"),
            "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None,
+           "warm_start_in_kernel": meas.get("nn_fused"),
            "warm_start_unet16": dict(meas["nn_unet16"], network=net16.describe()) if "nn_unet16" in meas else None}
     if "plain" in meas and "nn" in meas:
         ps, ns = meas["plain"]["mean_sweeps"], meas["nn"]["mean_sweeps"]
@@ -351,19 +364,21 @@ def main():
     # solves end at their first test (every crossbar of the nn and ir2s
     # variants), timed alone between HIP events on its stream
     gk = hm["guess_kernel_ms"] is not None and hm["mean_sweeps"] == 0
-    kname = "guess_kernel<Geo<4,8,32,16>>" if gk else "rbsor_kernel<Geo<4,8,32,16>>"
+    gname = "guess_kernel"
+    kname = (gname + "<Geo<4,8,32,16>>") if gk else "rbsor_kernel<Geo<4,8,32,16>>"
     kms_dom = hm["guess_kernel_ms"] if gk else hm["kernel_ms"]
-    passes_dom = hm["ir2s_iters"] * B + hm["mean_sweeps"] * B
-    ach_dom = algorithmic_bytes(passes_dom, B, warm=head == "nn", coarse=head == "nn") / (kms_dom * 1e-3) / 1e9
-    ctr = load_counters(head, "guess_kernel" if gk else "rbsor_kernel")
+    pre_passes, warm_rd, coarse_rd = spec[head]
+    passes_dom = pre_passes * B + hm["mean_sweeps"] * B
+    ach_dom = algorithmic_bytes(passes_dom, B, warm=warm_rd, coarse=coarse_rd) / (kms_dom * 1e-3) / 1e9
+    ctr = load_counters(head, gname if gk else "rbsor_kernel")
     out["roofline"] = {"bound": "hbm", "achieved": ach_dom, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": ach_dom / HBM_PEAK_GBS,
                        "traffic": ctr.get("hbm_bytes_per_launch"),
                        "kernel": kname, "kernel_ms": kms_dom, "step_kernels_ms": hm["kernel_ms"],
-                       "bytes_model": "(sweeps + IR2S iterations) * 4*M*N + 4*M + 8*N per crossbar "
-                                      "(conductance streamed once per sweep or wire-solve pass), + 8*M*N "
-                                      "for the warm-start field read from HBM, + 2*16*(M/4)*(N/4) for the "
-                                      "network's coarse output combined at load",
+                       "bytes_model": "(sweeps + passes before them) * 4*M*N + 4*M + 8*N per crossbar "
+                                      "(conductance streamed once per sweep or wire-solve pass), + 8*M*N for "
+                                      "a warm-start field read from HBM, + 2*16*(M/4)*(N/4) for a coarse "
+                                      "network output read from HBM",
                        "lds": ({"achieved": ctr["lds_bytes_per_launch"] / (kms_dom * 1e-3) / 1e9,
                                 "peak": LDS_PEAK_GBS, "unit": "GB/s",
                                 "frac": ctr["lds_bytes_per_launch"] / (kms_dom * 1e-3) / 1e9 / LDS_PEAK_GBS}

@@ -108,7 +108,7 @@ EXPORTED = [
     "fadesim_nn_features", "fadesim_nn_features_pooled", "fadesim_nn_coarse_forward", "fadesim_nn_combine", "fadesim_nn_combine_up", "fadesim_nn_conv3x3_fwd", "fadesim_nn_flip_weights",
     "fadesim_nn_conv3x3_wgrad", "fadesim_nn_pool2", "fadesim_nn_pool2_bwd", "fadesim_nn_up2_add",
     "fadesim_nn_up2_bwd", "fadesim_nn_relu_mask", "fadesim_nn_add", "fadesim_nn_mse", "fadesim_nn_adam",
-    "fadesim_nn_cast",
+    "fadesim_nn_cast", "fadesim_nn_solve_batch",
 ]
 
 _LIB = None
@@ -130,6 +130,9 @@ def lib():
         if hasattr(L, "fadesim_solve_batch_noalloc"):
             L.fadesim_solve_batch_noalloc.argtypes = [ctypes.POINTER(BatchDesc), _vp]
             L.fadesim_solve_batch_noalloc.restype = ctypes.c_int
+        if hasattr(L, "fadesim_nn_solve_batch"):
+            L.fadesim_nn_solve_batch.argtypes = [ctypes.POINTER(BatchDesc), _vp, _vp, _vp]
+            L.fadesim_nn_solve_batch.restype = ctypes.c_int
         L.fadesim_last_kernel_ms.restype = ctypes.c_double
         if hasattr(L, "fadesim_last_guess_ms"):
             L.fadesim_last_guess_ms.restype = ctypes.c_double

@@ -24,12 +24,14 @@ struct DevBuf {
 std::mutex g_ws_mutex;
 std::vector<DevBuf> g_ws;     // indexed by device ordinal
 std::vector<DevBuf> g_ws_aux; // indexed by device ordinal
+std::vector<DevBuf> g_ws_warm; // indexed by device ordinal
 
 void *grow(std::vector<DevBuf> &pool, size_t bytes);
 } // namespace
 
 void *workspace(size_t bytes) { return grow(g_ws, bytes); }
 void *workspace_aux(size_t bytes) { return grow(g_ws_aux, bytes); }
+void *workspace_warm(size_t bytes) { return grow(g_ws_warm, bytes); }
 
 namespace {
 void *grow(std::vector<DevBuf> &pool, size_t bytes)

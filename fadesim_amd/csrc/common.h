@@ -19,6 +19,8 @@ void *workspace(size_t bytes);
 // A second, independent buffer with the same rules (for kernels that run
 // while the first one is in use by the same call).
 void *workspace_aux(size_t bytes);
+// A third one (the warm starts the in-kernel network start hands on).
+void *workspace_warm(size_t bytes);
 
 #define FADESIM_HIP_CHECK(expr)                                                   \
     do {                                                                          \

@@ -24,31 +24,19 @@
 #include "fadesim.h"
 #include "fadesim_nn.h"
 #include "rb_core.h"
+#include "nn_core.h"
 
 namespace fadesim {
 namespace {
 
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4;
-
-__device__ __forceinline__ bf16x8 zero8()
-{
-    bf16x8 z;
-#pragma unroll
-    for (int e = 0; e < 8; ++e)
-        z[e] = (__bf16)0.0f;
-    return z;
-}
+using nn::bf16x8;
+using nn::f4;
+using nn::zero8;
 
 // ---------------------------------------------------------------- pooled features
-// The same features for 128x128 crossbars, averaged over 4x4 cells (the
-// network's 32x32 input), with the drop fields computed the solver's way:
-// each thread holds a 4x8 register tile, (p1, q1) and (p2, q2) are two
-// undamped wire-solve iterations from zero (rb::wire_iteration: line scans
-// by lane shuffles, column sums across thread rows in FP64).  A thread's tile
-// is exactly two 4x4 pooling cells, so the pooled features come out of
-// registers; base = (p2, q2) at full resolution.
-using FeatGeo = rb::Geo<4, 8, 32, 16>;
+// nn::pooled_features for one crossbar per workgroup, with the base field
+// (p2, q2) written at full resolution and the scale S2.
+using FeatGeo = nn::FeatGeo;
 
 __global__ void __launch_bounds__(FeatGeo::NT) features_pooled_kernel(const float *__restrict__ g,
                                                                      const double *__restrict__ rw,
@@ -58,118 +46,28 @@ __global__ void __launch_bounds__(FeatGeo::NT) features_pooled_kernel(const floa
                                                                      float *__restrict__ scale)
 {
     using G = FeatGeo;
-    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY, M = G::M, N = G::N, NT = G::NT;
-    constexpr int PW = N / 4; // pooled width
-    __shared__ double cbuf[2 * TY * N];
+    constexpr int TR = G::TR, TC = G::TC, TX = G::TX, M = G::M, N = G::N, NT = G::NT;
+    __shared__ double cbuf[2 * G::TY * N];
     __shared__ double cext[3 * N];
     __shared__ int dead;
-    __shared__ float red[4][NT / 64];
+    __shared__ float red[4 * (NT / 64)];
     rb::Shared sh{};
     sh.cbuf = cbuf;
     sh.cext = cext;
     sh.dead = &dead;
-    const rb::StripCtx X{};
     const int b = blockIdx.x, t = threadIdx.x, tx = t % TX, ty = t / TX;
     const int i0 = TR * ty, j0 = TC * tx;
     const size_t MN = (size_t)M * N;
-    const float *gp = g + (size_t)b * MN;
-    const double r = rw[b];
-    float vi[TR];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii)
-        vi[ii] = (float)vin[(size_t)b * M + i0 + ii];
-    auto cur = [&](int ii, const float (&pr)[TC], const float (&qr)[TC], float (&a)[TC], float (&c)[TC]) {
-        float gr[TC];
-        rb::ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gr);
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            a[jj] = c[jj] = gr[jj] * ((vi[ii] - pr[jj]) - qr[jj]);
-    };
-    float p[TR][TC], q[TR][TC], p1[TR][TC], q1[TR][TC];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii)
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj)
-            p[ii][jj] = q[ii][jj] = 0.f;
-    int nbar = 0;
-    rb::wire_iteration<G, false>(sh, X, cur, r, 1.0, M, 0, i0, j0, tx, ty, t, nbar, p, q);
-    float s1 = 0.f, gm = 0.f, vm = 0.f;
+    float p[TR][TC], q[TR][TC];
+    const float S2 = nn::pooled_features(sh, g + (size_t)b * MN, rw[b], vin + (size_t)b * M,
+                                         feat + (size_t)b * (M / 4) * (N / 4) * nn::kFeatC, red, p, q);
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
-        vm = fmaxf(vm, fabsf(vi[ii]));
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            p1[ii][jj] = p[ii][jj];
-            q1[ii][jj] = q[ii][jj];
-            s1 = fmaxf(s1, fmaxf(fabsf(p[ii][jj]), fabsf(q[ii][jj])));
-        }
-    }
-    rb::wire_iteration<G, false>(sh, X, cur, r, 1.0, M, 0, i0, j0, tx, ty, t, nbar, p, q);
-    float s2 = 0.f;
-    float pool[2][5]; // per pooling cell: sums of p1, q1, p2 - p1, q2 - q1, g
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int e = 0; e < 5; ++e)
-            pool[h][e] = 0.f;
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii) {
-        float gr[TC];
-        rb::ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gr);
-#pragma unroll
-        for (int jj = 0; jj < TC; ++jj) {
-            const float dp = p[ii][jj] - p1[ii][jj], dq = q[ii][jj] - q1[ii][jj];
-            s2 = fmaxf(s2, fmaxf(fabsf(dp), fabsf(dq)));
-            gm = fmaxf(gm, gr[jj]);
-            float *pc = pool[jj / 4];
-            pc[0] += p1[ii][jj];
-            pc[1] += q1[ii][jj];
-            pc[2] += dp;
-            pc[3] += dq;
-            pc[4] += gr[jj];
-        }
         rb::strow<TC>(base + (size_t)b * 2 * MN + (size_t)(i0 + ii) * N + j0, p[ii]);
         rb::strow<TC>(base + (size_t)b * 2 * MN + MN + (size_t)(i0 + ii) * N + j0, q[ii]);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s1 = fmaxf(s1, __shfl_xor(s1, off, 64));
-        s2 = fmaxf(s2, __shfl_xor(s2, off, 64));
-        gm = fmaxf(gm, __shfl_xor(gm, off, 64));
-        vm = fmaxf(vm, __shfl_xor(vm, off, 64));
-    }
-    if ((t & 63) == 0) {
-        red[0][t >> 6] = s1;
-        red[1][t >> 6] = s2;
-        red[2][t >> 6] = gm;
-        red[3][t >> 6] = vm;
-    }
-    __syncthreads();
-    float S1 = 1e-30f, S2 = 1e-30f, GM = 1e-30f, VM = 1e-30f;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) {
-        S1 = fmaxf(S1, red[0][w]);
-        S2 = fmaxf(S2, red[1][w]);
-        GM = fmaxf(GM, red[2][w]);
-        VM = fmaxf(VM, red[3][w]);
-    }
     if (t == 0)
         scale[b] = S2;
-    const float vavg = 0.25f * (((vi[0] + vi[1]) + vi[2]) + vi[3]);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const float *pc = pool[h];
-        bf16x8 f;
-        f[0] = (__bf16)(pc[0] * (1.f / 16.f) / S1);
-        f[1] = (__bf16)(pc[1] * (1.f / 16.f) / S1);
-        f[2] = (__bf16)(pc[2] * (1.f / 16.f) / S2);
-        f[3] = (__bf16)(pc[3] * (1.f / 16.f) / S2);
-        f[4] = (__bf16)(pc[4] * (1.f / 16.f) / GM);
-        f[5] = (__bf16)(vavg / VM);
-        f[6] = (__bf16)((float)(r / 5.0));
-        f[7] = (__bf16)1.0f;
-        *reinterpret_cast<bf16x8 *>(feat + (((size_t)b * (M / 4) + ty) * PW + 2 * tx + h) * 8) = f;
-    }
 }
 
 // ---------------------------------------------------------------- features
@@ -559,12 +457,10 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_lds_kernel(ConvArgs A)
 
 // ---------------------------------------------------------------- fused coarse network
 // The whole coarse network (coarse32f, or coarse16f at half the width) for
-// one crossbar in one workgroup, every activation in LDS: 32x32x8 input ->
-// e2a, e2 (C1 ch, 32x32) -> 2x2 mean -> e3a, e3 (2 C1 ch), t3 (C1 ch) at
-// 16x16 -> nearest 2x upsample + e2 -> d2 (C1 ch) -> out (16 ch, to HBM).  Each layer is the same implicit GEMM as
-// conv3x3_fwd_kernel (the same A and B fragments, the same k order), so the
-// result equals the layer-by-layer kernels'.  LDS: two 32x32x32 buffers (64
-// KB each) and one 32x32x8 / 16x16x32 buffer (16 KB).
+// one crossbar per workgroup of 16 waves, every activation in LDS
+// (nn::coarse_net_body): the same implicit GEMMs as conv3x3_fwd_kernel (the
+// same A and B fragments, the same k order), so the result equals the
+// layer-by-layer kernels'.
 struct CoarseNetArgs {
     const __bf16 *x;   // [B][32][32][8]
     const __bf16 *w[7]; // e2a, e2, e3a, e3, t3, d2, out: fragment-major [Kp/32][Cout][32]
@@ -573,277 +469,23 @@ struct CoarseNetArgs {
 };
 
 constexpr int kNetThreads = 1024;
-
-constexpr int net_kpad(int cin) { return ((9 * cin + 1 + 31) / 32) * 32; }
-
-// One layer on the 16 waves of the workgroup: wave (cg, pg) computes CTW
-// output-channel tiles (16 channels each) of P pixel tiles (16 pixels each),
-// so a weight fragment serves P tiles and an input fragment CTW tiles; the
-// depth is a compile-time constant (the k loop unrolls by 2 so that the next
-// fragments load while the present ones multiply).
-template <int CIN, int COUT, int H, int W, bool RELU, int CTW, int P>
-__device__ __forceinline__ void net_conv(const __bf16 *src, const __bf16 *__restrict__ w, __bf16 *dst)
-{
-    constexpr int CT = COUT / 16, PT = H * W / 16, K9 = 9 * CIN, NW = kNetThreads / 64, KP = net_kpad(CIN);
-    constexpr int NPG = PT / P; // pixel groups
-    static_assert(W % 16 == 0 && CIN % 8 == 0 && COUT % 16 == 0, "tile shapes");
-    static_assert(CT % CTW == 0 && PT % P == 0 && (CT / CTW) * NPG == NW, "waves x tiles");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j = lane & 15, g = lane >> 4;
-    const int pg = wave % NPG, cg = wave / NPG;
-    f4 acc[P][CTW];
-#pragma unroll
-    for (int q = 0; q < P; ++q)
-#pragma unroll
-        for (int t = 0; t < CTW; ++t)
-            acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-    for (int k0 = 0; k0 < KP; k0 += 32) {
-        const int k = k0 + 8 * g;
-        bf16x8 av[CTW];
-#pragma unroll
-        for (int t = 0; t < CTW; ++t) // fragment-major weights: one contiguous 1 KB per wave
-            av[t] = *reinterpret_cast<const bf16x8 *>(w + ((k0 / 32) * COUT + 16 * (cg * CTW + t) + j) * 32 + 8 * g);
-        int tap = 0, ci = 0;
-        const bool real = k < K9;
-        if (real) {
-            tap = k / CIN;
-            ci = k - tap * CIN;
-        }
-        const int ky = tap / 3, kx = tap - 3 * ky;
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const int p0 = 16 * (pg * P + q), y = p0 / W, x0 = p0 % W;
-            bf16x8 bv = zero8();
-            if (real) {
-                const int yy = y + ky - 1, xx = x0 + j + kx - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                    bv = *reinterpret_cast<const bf16x8 *>(src + ((yy * W + xx) * CIN + ci));
-            } else if (k == K9) {
-                bv[0] = (__bf16)1.0f; // bias column
-            }
-#pragma unroll
-            for (int t = 0; t < CTW; ++t)
-                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[q][t], 0, 0, 0);
-        }
-    }
-    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const int p0 = 16 * (pg * P + q);
-#pragma unroll
-        for (int t = 0; t < CTW; ++t) {
-            bf16x4 o;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float v = acc[q][t][r];
-                if (RELU)
-                    v = fmaxf(v, 0.f);
-                o[r] = (__bf16)v;
-            }
-            *reinterpret_cast<bf16x4 *>(dst + (p0 + j) * COUT + 16 * (cg * CTW + t) + 4 * g) = o;
-        }
-    }
-}
-
-// Lane shifts within rows of 16 lanes (DPP row_shr:1 / row_shl:1): lane j
-// receives lane j-1's (j+1's) fragment, 0 where that lane is outside the row.
-__device__ __forceinline__ bf16x8 frag_from_left(bf16x8 v)
-{
-    typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i4;
-    i4 x = __builtin_bit_cast(i4, v), y;
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-        y[e] = __builtin_amdgcn_update_dpp(0, x[e], 0x111, 0xF, 0xF, true);
-    return __builtin_bit_cast(bf16x8, y);
-}
-__device__ __forceinline__ bf16x8 frag_from_right(bf16x8 v)
-{
-    typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i4;
-    i4 x = __builtin_bit_cast(i4, v), y;
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-        y[e] = __builtin_amdgcn_update_dpp(0, x[e], 0x101, 0xF, 0xF, true);
-    return __builtin_bit_cast(bf16x8, y);
-}
-
-// net_conv for CIN a multiple of 32: the k blocks of one kernel row (taps
-// kx = 0, 1, 2 of row ky, each CIN/32 blocks) are built from ONE load of the
-// centre tap: the left and right taps are the centre fragments of the
-// neighbouring lanes (DPP row shifts), only the two edge pixels of the 16-pixel
-// segment are loaded again.  Same fragments, same k order as net_conv.
-template <int CIN, int COUT, int H, int W, bool RELU, int CTW, int P>
-__device__ __forceinline__ void net_conv_rows(const __bf16 *src, const __bf16 *__restrict__ w, __bf16 *dst)
-{
-    constexpr int CT = COUT / 16, PT = H * W / 16, K9 = 9 * CIN, NW = kNetThreads / 64, KP = net_kpad(CIN);
-    constexpr int NPG = PT / P, CB = CIN / 32; // pixel groups, k blocks per tap
-    static_assert(W % 16 == 0 && CIN % 32 == 0 && COUT % 16 == 0 && KP == K9 + 32, "tile shapes");
-    static_assert(CT % CTW == 0 && PT % P == 0 && (CT / CTW) * NPG == NW, "waves x tiles");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j = lane & 15, g = lane >> 4;
-    const int pg = wave % NPG, cg = wave / NPG;
-    f4 acc[P][CTW];
-#pragma unroll
-    for (int q = 0; q < P; ++q)
-#pragma unroll
-        for (int t = 0; t < CTW; ++t)
-            acc[q][t] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-    for (int ky = 0; ky < 3; ++ky) {
-        bf16x8 av[3][CB][CTW]; // weights of the three taps of this kernel row
-#pragma unroll
-        for (int kx = 0; kx < 3; ++kx)
-#pragma unroll
-            for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-                for (int t = 0; t < CTW; ++t)
-                    av[kx][cb][t] = *reinterpret_cast<const bf16x8 *>(
-                        w + (((3 * ky + kx) * CB + cb) * COUT + 16 * (cg * CTW + t) + j) * 32 + 8 * g);
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-            const int p0 = 16 * (pg * P + q), y = p0 / W, x0 = p0 % W, yy = y + ky - 1;
-            if (yy < 0 || yy >= H)
-                continue; // the whole segment reads padding: the fragments are zero
-            const __bf16 *rowp = src + (yy * W) * CIN + 8 * g;
-            bf16x8 b[3][CB];
-#pragma unroll
-            for (int cb = 0; cb < CB; ++cb) {
-                const bf16x8 c = *reinterpret_cast<const bf16x8 *>(rowp + (x0 + j) * CIN + 32 * cb);
-                bf16x8 l = frag_from_left(c), r = frag_from_right(c);
-                if (j == 0 && x0 > 0)
-                    l = *reinterpret_cast<const bf16x8 *>(rowp + (x0 - 1) * CIN + 32 * cb);
-                if (j == 15 && x0 + 16 < W)
-                    r = *reinterpret_cast<const bf16x8 *>(rowp + (x0 + 16) * CIN + 32 * cb);
-                b[0][cb] = l;
-                b[1][cb] = c;
-                b[2][cb] = r;
-            }
-#pragma unroll
-            for (int kx = 0; kx < 3; ++kx)
-#pragma unroll
-                for (int cb = 0; cb < CB; ++cb)
-#pragma unroll
-                    for (int t = 0; t < CTW; ++t)
-                        acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[kx][cb][t], b[kx][cb], acc[q][t], 0,
-                                                                           0, 0);
-        }
-    }
-    // the bias block: k = K9 is the ones column
-    {
-        bf16x8 av[CTW];
-#pragma unroll
-        for (int t = 0; t < CTW; ++t)
-            av[t] = *reinterpret_cast<const bf16x8 *>(w + ((K9 / 32) * COUT + 16 * (cg * CTW + t) + j) * 32 + 8 * g);
-        bf16x8 bv = zero8();
-        if (g == 0)
-            bv[0] = (__bf16)1.0f;
-#pragma unroll
-        for (int q = 0; q < P; ++q)
-#pragma unroll
-            for (int t = 0; t < CTW; ++t)
-                acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[q][t], 0, 0, 0);
-    }
-    typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
-#pragma unroll
-    for (int q = 0; q < P; ++q) {
-        const int p0 = 16 * (pg * P + q);
-#pragma unroll
-        for (int t = 0; t < CTW; ++t) {
-            bf16x4 o;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float v = acc[q][t][r];
-                if (RELU)
-                    v = fmaxf(v, 0.f);
-                o[r] = (__bf16)v;
-            }
-            *reinterpret_cast<bf16x4 *>(dst + (p0 + j) * COUT + 16 * (cg * CTW + t) + 4 * g) = o;
-        }
-    }
-}
-
-// The fused network's layers for first width C1 (C1 channels at 32x32, 2 C1
-// at 16x16): the implicit-GEMM form and the split of the 16 waves over
-// output-channel tiles and pixel tiles per layer.
-template <int C1>
-struct CoarseLayers;
-template <>
-struct CoarseLayers<32> { // coarse32f
-    static __device__ __forceinline__ void e2a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<8, 32, 32, 32, true, 1, 8>(x, w, y); }
-    static __device__ __forceinline__ void e2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 32, 32, 32, true, 1, 8>(x, w, y); }
-    static __device__ __forceinline__ void e3a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 64, 16, 16, true, 1, 4>(x, w, y); }
-    static __device__ __forceinline__ void e3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<64, 64, 16, 16, true, 1, 4>(x, w, y); }
-    static __device__ __forceinline__ void t3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<64, 32, 16, 16, true, 1, 2>(x, w, y); }
-    static __device__ __forceinline__ void d2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 32, 32, 32, true, 1, 8>(x, w, y); }
-    static __device__ __forceinline__ void out(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 16, 32, 32, false, 1, 4>(x, w, y); }
-};
-template <>
-struct CoarseLayers<16> { // coarse16f
-    static __device__ __forceinline__ void e2a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<8, 16, 32, 32, true, 1, 4>(x, w, y); }
-    static __device__ __forceinline__ void e2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 16, 32, 32, true, 1, 4>(x, w, y); }
-    static __device__ __forceinline__ void e3a(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 32, 16, 16, true, 1, 2>(x, w, y); }
-    static __device__ __forceinline__ void e3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 32, 16, 16, true, 1, 2>(x, w, y); }
-    static __device__ __forceinline__ void t3(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv_rows<32, 16, 16, 16, true, 1, 1>(x, w, y); }
-    static __device__ __forceinline__ void d2(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 16, 32, 32, true, 1, 4>(x, w, y); }
-    static __device__ __forceinline__ void out(const __bf16 *x, const __bf16 *w, __bf16 *y) { net_conv<16, 16, 32, 32, false, 1, 4>(x, w, y); }
-};
+using nn::net_kpad;
 
 template <int C1>
 __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A)
 {
-    using L = CoarseLayers<C1>;
-    constexpr int C2 = 2 * C1, G1 = C1 / 8; // 8-channel groups of a C1-channel pixel
-    // bufA: e2a / d2 (32x32xC1), or e3a and e3 (16x16xC2 each); bufS: e2 and
-    // u2 (32x32xC1); bufX: the input (32x32x8), p3 and t3 (16x16xC1)
-    constexpr int kX = 32 * 32 * 8 > 16 * 16 * C1 ? 32 * 32 * 8 : 16 * 16 * C1;
-    __shared__ __align__(16) __bf16 bufA[32 * 32 * C1];
-    __shared__ __align__(16) __bf16 bufS[32 * 32 * C1];
-    __shared__ __align__(16) __bf16 bufX[kX];
+    using Lds = nn::NetLds<C1>;
+    __shared__ __align__(16) __bf16 bufA[Lds::kA];
+    __shared__ __align__(16) __bf16 bufS[Lds::kS];
+    __shared__ __align__(16) __bf16 bufX[Lds::kX];
     const long b = blockIdx.x;
     const int t = threadIdx.x;
     // input features (16 bytes per copy)
     for (int e = t; e < 32 * 32; e += kNetThreads)
         *reinterpret_cast<bf16x8 *>(bufX + 8 * e) = *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
     __syncthreads();
-    L::e2a(bufX, A.w[0], bufA);
-    __syncthreads();
-    L::e2(bufA, A.w[1], bufS); // kept for the skip
-    __syncthreads();
-    // p3: 2x2 mean of e2 -> 16x16xC1 in bufX
-    for (int e = t; e < 16 * 16 * G1; e += kNetThreads) {
-        const int c = (e % G1) * 8, po = e / G1, yo = po / 16, xo = po % 16;
-        const int i00 = ((2 * yo) * 32 + 2 * xo) * C1 + c;
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufS + i00);
-        const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + C1);
-        const bf16x8 a2 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * C1);
-        const bf16x8 a3 = *reinterpret_cast<const bf16x8 *>(bufS + i00 + 32 * C1 + C1);
-        bf16x8 o;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            o[q] = (__bf16)(0.25f * ((float)a[q] + (float)a1[q] + (float)a2[q] + (float)a3[q]));
-        *reinterpret_cast<bf16x8 *>(bufX + po * C1 + c) = o;
-    }
-    __syncthreads();
-    L::e3a(bufX, A.w[2], bufA); // -> bufA[0 : 16x16xC2)
-    __syncthreads();
-    L::e3(bufA, A.w[3], bufA + 16 * 16 * C2); // -> bufA[16x16xC2 : 2 x 16x16xC2)
-    __syncthreads();
-    L::t3(bufA + 16 * 16 * C2, A.w[4], bufX); // -> bufX
-    __syncthreads();
-    // u2 = nearest upsample of t3 + e2, in place in bufS
-    for (int e = t; e < 32 * 32 * G1; e += kNetThreads) {
-        const int c = (e % G1) * 8, p = e / G1, yy = p / 32, xx = p % 32;
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(bufX + ((yy / 2) * 16 + xx / 2) * C1 + c);
-        bf16x8 k = *reinterpret_cast<const bf16x8 *>(bufS + p * C1 + c);
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            k[q] = (__bf16)((float)a[q] + (float)k[q]);
-        *reinterpret_cast<bf16x8 *>(bufS + p * C1 + c) = k;
-    }
-    __syncthreads();
-    L::d2(bufS, A.w[5], bufA);
-    __syncthreads();
-    L::out(bufA, A.w[6], A.out + b * 1024 * 16);
+    const __bf16 *const w[7] = {A.w[0], A.w[1], A.w[2], A.w[3], A.w[4], A.w[5], A.w[6]};
+    nn::coarse_net_body<C1, kNetThreads / 64>(bufX, bufA, bufS, w, A.out + b * 1024 * nn::kOutC);
 }
 
 // wT[ci][tap'*Cout + co] = w[co][(8-tap')*Cin + ci]; bias and pad columns 0.

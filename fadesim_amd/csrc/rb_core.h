@@ -310,26 +310,17 @@ __device__ __forceinline__ void publish_all(const float (&yp)[G::TR][G::TC],
 // level) (N >> level) floats).  Sample point ((i + 1/2)/f - 1/2,
 // (j + 1/2)/f - 1/2) clamped to the coarse grid, the arithmetic of
 // fadesim_nn_combine_up term by term, so the drops equal that kernel's field
-// bit for bit.  Ends with a workgroup barrier (the stage is free again).
+// bit for bit.  add_coarse_staged: the sampling from a filled stage (level
+// lvl, scale s); add_coarse: staged from I.coarse, then sampled, ending with
+// a workgroup barrier (the stage is free again).
 template <class G>
-__device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, int gi0, int i0, int j0,
-                                           float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC])
+__device__ __forceinline__ void add_coarse_staged(const float *stage, int lvl, float s, int rows, int gi0, int i0,
+                                                  int j0, float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC])
 {
     constexpr int TR = G::TR, TC = G::TC, N = G::N;
-    const int lvl = I.clevel, f = 1 << lvl, Hc = I.rows >> lvl, Wc = N >> lvl, HW = Hc * Wc;
-    // opaque to the optimizer: the sample positions below are computed here,
-    // not hoisted out of a caller's loop over crossbars as a register tile
-    // (which would spill)
-    asm volatile("" : "+v"(i0), "+v"(j0));
-    __syncthreads();
-    const unsigned *src = reinterpret_cast<const unsigned *>(I.coarse);
-    float2 *st2 = reinterpret_cast<float2 *>(stage);
-    for (int k = t; k < HW; k += G::NT) {
-        const unsigned v = ldg(src + (size_t)k * (I.cch >> 1));
-        st2[k] = make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
-    }
-    __syncthreads();
-    const float inv = 1.0f / (float)f, s = I.cscale;
+    const int f = 1 << lvl, Hc = rows >> lvl, Wc = N >> lvl;
+    const float2 *st2 = reinterpret_cast<const float2 *>(stage);
+    const float inv = 1.0f / (float)f;
 #pragma unroll
     for (int ii = 0; ii < TR; ++ii) {
         const int i = gi0 + i0 + ii;
@@ -354,6 +345,33 @@ __device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, i
             }
         }
     }
+}
+
+// Stages channels 0 and 1 of a coarse grid of hw pixels with cch channels
+// per pixel (bf16, an even count, so the pair is one 4-byte word) as FP32
+// pairs in `stage` (bf16 -> FP32 exactly: the bits shifted up).  src is
+// global memory or LDS.
+__device__ __forceinline__ void stage_coarse_pair(float2 &dst, unsigned v)
+{
+    dst = make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
+}
+
+template <class G>
+__device__ __forceinline__ void add_coarse(const Inst &I, float *stage, int t, int gi0, int i0, int j0,
+                                           float (&yp)[G::TR][G::TC], float (&yq)[G::TR][G::TC])
+{
+    const int lvl = I.clevel, Hc = I.rows >> lvl, Wc = G::N >> lvl, HW = Hc * Wc;
+    // opaque to the optimizer: the sample positions below are computed here,
+    // not hoisted out of a caller's loop over crossbars as a register tile
+    // (which would spill)
+    asm volatile("" : "+v"(i0), "+v"(j0));
+    __syncthreads();
+    const unsigned *src = reinterpret_cast<const unsigned *>(I.coarse);
+    float2 *st2 = reinterpret_cast<float2 *>(stage);
+    for (int k = t; k < HW; k += G::NT)
+        stage_coarse_pair(st2[k], ldg(src + (size_t)k * (I.cch >> 1)));
+    __syncthreads();
+    add_coarse_staged<G>(stage, lvl, I.cscale, I.rows, gi0, i0, j0, yp, yq);
     __syncthreads();
 }
 
@@ -1468,6 +1486,9 @@ __device__ __attribute__((noinline)) void phase_coarse(Inst I, StripCtx X, float
 // crossbars) -- read from HBM here, not from a workspace plane.  The result
 // goes to planes 3, 4.
 constexpr int kStartZero = 0, kStartWarm = 1, kStartWarmCoarse = 2;
+// the learned warm start formed in-kernel (guess_solve with a START that
+// provides it)
+constexpr int kStartNet = 3;
 
 // The starting drops of the wire-solve iterations (start as above) into the
 // register tile p, q.
@@ -1892,9 +1913,16 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
 // when a test ended the solve before any sweep (certified, or not finite),
 // with every output written as solve() writes it; false when the solve needs
 // sweeps (no output written but the field, which solve() then rewrites as it
-// runs the crossbar from the start).
-template <class G, bool HOLD>
-__device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *wsb, const Shared &sh, float *fld)
+// runs the crossbar from the start).  start: the initial guess of the
+// first test (kStartZero, kStartWarm, kStartWarmCoarse as I holds them, or
+// kStartNet); a guess other than kStartZero is judged against the zero
+// guess.  START: start_fn(start, t, i0, j0, p, q) fills the register tile
+// (thread t, first row i0, first column j0) with the drops of a start mode
+// before the IR2S iterations (load_start, or a caller's that adds
+// kStartNet).
+template <class G, bool HOLD, class START>
+__device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *wsb, const Shared &sh, float *fld,
+                                            int first_start, START start_fn)
 {
     constexpr int TR = G::TR, TC = G::TC, TX = G::TX, TY = G::TY;
     constexpr int M = G::M, N = G::N;
@@ -1920,12 +1948,11 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
 
     setup_columns<G, false>(I, X, wsb, gp, 0, i0, j0, t, ty, rwire, sh, nbar);
 
-    const bool coarse = I.coarse != nullptr;
     const int iters = uniform_i32(C.ir2s_iters);
     // the drops of the guess into fld (and the field output)
     auto guess = [&](int start) {
         float p[TR][TC], q[TR][TC];
-        load_start<G, false>(I, sh, t, 0, i0, j0, start, p, q);
+        start_fn(start, t, i0, j0, p, q);
         if (iters > 0) {
             float gt[TR][TC];
             if constexpr (HOLD) {
@@ -1980,10 +2007,10 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
         __syncthreads(); // fld reads done: the column sums reuse its memory
         return uniform_f64(test_reduce<G, false>(sh, X, Dp, Yp, t, ty, j0, nbar));
     };
-    guess(!I.warm ? kStartZero : coarse ? kStartWarmCoarse : kStartWarm);
+    guess(first_start);
     double est = test();
     bool rejected = false;
-    if ((I.warm != nullptr || coarse) && !(est <= zero_guess_est<G>(sh, t))) {
+    if (first_start != kStartZero && !(est <= zero_guess_est<G>(sh, t))) {
         rejected = true;
         guess(kStartZero);
         est = test();

@@ -16,6 +16,8 @@
 
 #include "common.h"
 #include "fadesim.h"
+#include "fadesim_nn.h"
+#include "nn_core.h"
 #include "rb_core.h"
 
 namespace fadesim {
@@ -44,6 +46,11 @@ struct RbArgs {
     double *est;
     int *rejected;
     float *field;
+    // guess_net_kernel: the network's weights (fragment-major [Kp/32][Cout][32]
+    // per layer) and, per instance, the combined start of each crossbar it did
+    // not finish ([B][2][M][N], read by rbsor_kernel as a warm start)
+    const __bf16 *netw[7];
+    float *warm_out;
 };
 
 __global__ void fill_zero_f64(double *p, size_t n)
@@ -165,13 +172,130 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
         I.cscale = A.coarse ? A.cscale[b] : 0.f;
         I.cch = A.cch;
         I.clevel = A.clevel;
-        const bool done = rb::guess_solve<G, kGuessHoldsTile>(I, A.ctl, wsb, sh, fld);
+        const int start = !I.warm ? rb::kStartZero : I.coarse ? rb::kStartWarmCoarse : rb::kStartWarm;
+        auto load = [&](int st, int t, int i0, int j0, float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC]) {
+            rb::load_start<G, false>(I, sh, t, 0, i0, j0, st, p, q);
+        };
+        const bool done = rb::guess_solve<G, kGuessHoldsTile>(I, A.ctl, wsb, sh, fld, start, load);
         if (!done && threadIdx.x == 0)
             A.list[atomicAdd(A.list_count, 1)] = b;
         // a barrier between this thread-0 append and the next iteration's
         // thread-0 queue update: without one the compiler joins the two into
         // one path for thread 0 around the loop's barriers, so the other lanes
         // of its wave would read s_inst before thread 0 has updated it
+        __syncthreads();
+    }
+}
+
+// guess_kernel with the learned warm start formed in-kernel (128x128, the
+// coarse16f network): per crossbar, the pooled features and the base field
+// (two undamped wire-solve iterations from zero, nn::pooled_features), the
+// network on the workgroup's 8 waves (nn::coarse_net_body, every activation
+// in LDS), and base + S2 x bilinear(network output) -- the arithmetic of the
+// features kernel, the network kernel and the combine at load, term by term,
+// so the drops equal those of the three-kernel path bit for bit.  The
+// network's buffers overlay `big` (idle between the set-up and the test).  A
+// crossbar this kernel does not finish has its combined start recomputed and
+// written to warm_out, from which rbsor_kernel then solves it as it would from
+// the base field and the network output.
+using NetGeo = rb::Geo<4, 8, 32, 16>;
+constexpr int kNetWidth = 16;
+
+__global__ void __launch_bounds__(NetGeo::NT) guess_net_kernel(RbArgs A)
+{
+    using G = NetGeo;
+    using Lds = nn::NetLds<kNetWidth>;
+    constexpr int M = G::M, N = G::N, TR = G::TR, TC = G::TC, TX = G::TX, NT = G::NT;
+    constexpr int kHaloB = G::HALO * (int)sizeof(float), kCbufB = 2 * G::TY * N * (int)sizeof(double);
+    constexpr int kFldB = 2 * M * N * (int)sizeof(float);
+    constexpr int kBigB = kHaloB + kCbufB > kFldB ? kHaloB + kCbufB : kFldB;
+    // network buffers: bufA, bufS, bufX, then the output (32x32x16); the
+    // output's channel pairs are staged in the halo region (from byte kCbufB,
+    // over bufX, which is dead by then)
+    constexpr int kNetB = (Lds::kA + Lds::kS + Lds::kX + 32 * 32 * nn::kOutC) * (int)sizeof(__bf16);
+    static_assert(kNetB <= kBigB, "network buffers fit the solver's LDS");
+    static_assert((Lds::kA + Lds::kS) * (int)sizeof(__bf16) == kCbufB, "features land beyond the column buffer");
+    static_assert(2 * 32 * 32 * (int)sizeof(float) <= Lds::kX * (int)sizeof(__bf16), "stage within bufX");
+    __shared__ __align__(16) unsigned char big[kBigB];
+    __shared__ double S1[N];
+    __shared__ double Ifloor[N];
+    __shared__ double colI[N];
+    __shared__ double colD[N];
+    __shared__ float wmax[G::NW];
+    __shared__ int s_inst, s_dead;
+    __shared__ double cext[3 * N];
+    __shared__ float red[4 * G::NW];
+    double *cbuf = reinterpret_cast<double *>(big);
+    float *halo = reinterpret_cast<float *>(big + kCbufB);
+    float *fld = reinterpret_cast<float *>(big);
+    __bf16 *bufA = reinterpret_cast<__bf16 *>(big);
+    __bf16 *bufS = bufA + Lds::kA, *bufX = bufS + Lds::kS, *bufO = bufX + Lds::kX;
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
+    const __bf16 *const w[7] = {A.netw[0], A.netw[1], A.netw[2], A.netw[3], A.netw[4], A.netw[5], A.netw[6]};
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_inst = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(s_inst);
+        __syncthreads();
+        if (b >= A.batch)
+            break;
+        rb::Inst I;
+        I.g = A.g + (size_t)(A.gidx ? A.gidx[b] : b) * M * N;
+        I.rwire = A.rw[b];
+        I.vin = A.vin + (size_t)b * M;
+        I.vout = A.vout + (size_t)b * A.vout_ld;
+        I.warm = nullptr;
+        I.field = A.field ? A.field + (size_t)b * 2 * M * N : nullptr;
+        I.iout = A.iout + (size_t)b * N;
+        I.sweeps = A.sweeps ? A.sweeps + b : nullptr;
+        I.status = A.status ? A.status + b : nullptr;
+        I.est = A.est ? A.est + b : nullptr;
+        I.rejected = A.rejected ? A.rejected + b : nullptr;
+        I.rows = M;
+        I.coarse = nullptr;
+        I.cscale = 0.f;
+        I.cch = I.clevel = 0;
+        // the combined start: features and base, network, base + S2 x bilinear(out)
+        auto net_start = [&](int i0, int j0, float (&p)[TR][TC], float (&q)[TR][TC]) {
+            const float S2 = nn::pooled_features(sh, I.g, I.rwire, I.vin, bufX, red, p, q);
+            nn::coarse_net_body<kNetWidth, NT / 64>(bufX, bufA, bufS, w, bufO);
+            __syncthreads();
+            const unsigned *src = reinterpret_cast<const unsigned *>(bufO);
+            float2 *st2 = reinterpret_cast<float2 *>(halo);
+            for (int k = threadIdx.x; k < 32 * 32; k += NT)
+                rb::stage_coarse_pair(st2[k], src[(size_t)k * (nn::kOutC >> 1)]);
+            __syncthreads();
+            asm volatile("" : "+v"(i0), "+v"(j0));
+            rb::add_coarse_staged<G>(halo, 2, S2, M, 0, i0, j0, p, q);
+            __syncthreads();
+        };
+        auto load = [&](int st, int t, int i0, int j0, float (&p)[TR][TC], float (&q)[TR][TC]) {
+            if (st == rb::kStartNet)
+                net_start(i0, j0, p, q);
+            else
+                rb::load_start<G, false>(I, sh, t, 0, i0, j0, st, p, q);
+        };
+        const bool done = rb::guess_solve<G, kGuessHoldsTile>(I, A.ctl, wsb, sh, fld, rb::kStartNet, load);
+        if (!done) {
+            // rare: the start again, for the sweeping kernel
+            int t = threadIdx.x;
+            asm volatile("" : "+v"(t));
+            const int i0 = TR * (t / TX), j0 = TC * (t % TX);
+            float p[TR][TC], q[TR][TC];
+            net_start(i0, j0, p, q);
+            float *wo = A.warm_out + (size_t)b * 2 * M * N;
+#pragma unroll
+            for (int ii = 0; ii < TR; ++ii) {
+                rb::strow<TC>(wo + (size_t)(i0 + ii) * N + j0, p[ii]);
+                rb::strow<TC>(wo + (size_t)M * N + (size_t)(i0 + ii) * N + j0, q[ii]);
+            }
+            if (threadIdx.x == 0)
+                A.list[atomicAdd(A.list_count, 1)] = b;
+        }
+        // a barrier between this thread-0 append and the next iteration's
+        // thread-0 queue update (see guess_kernel)
         __syncthreads();
     }
 }
@@ -196,10 +320,10 @@ static __device__ __align__(256) char scratch[kScratchBytes];
 thread_local hipEvent_t t_ev[3] = {nullptr, nullptr, nullptr};
 thread_local bool t_ev_valid = false, t_guess = false;
 
-// Resident workgroups of rbsor_kernel<G> (GUESS: guess_kernel<G>) on the
-// current device: the persistent grid.
-template <class G, bool GUESS>
-int resident_slots()
+// Resident workgroups of kernel K (K_ID: its cache entry) on the current
+// device: the persistent grid.
+template <int K_ID, class KERNEL>
+int resident_slots(KERNEL kernel, int threads)
 {
     static int cached[64] = {0};
     int dev = 0;
@@ -207,11 +331,7 @@ int resident_slots()
         return 256;
     if (!cached[dev]) {
         int per_cu = 0, cus = 0;
-        hipError_t e;
-        if constexpr (GUESS)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, guess_kernel<G>, G::NT, 0);
-        else
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rbsor_kernel<G>, G::NT, 0);
+        const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0);
         if (e != hipSuccess || per_cu < 1)
             per_cu = 1;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
@@ -220,6 +340,12 @@ int resident_slots()
     }
     return cached[dev];
 }
+
+template <class G>
+int sweep_slots() { return resident_slots<0 + 3 * (G::M * 1000 + G::N), void (*)(RbArgs)>(rbsor_kernel<G>, G::NT); }
+template <class G>
+int guess_slots() { return resident_slots<1 + 3 * (G::M * 1000 + G::N), void (*)(RbArgs)>(guess_kernel<G>, G::NT); }
+int guess_net_slots() { return resident_slots<2, void (*)(RbArgs)>(guess_net_kernel, NetGeo::NT); }
 
 // Solves whose guess is tested before any sweep go through guess_kernel
 // first: they test a guess (IR2S iterations, a warm start or a coarse
@@ -235,7 +361,9 @@ bool takes_guess_path(const RbArgs &args)
 // library-owned one; the grids shrink to the slots that fit.
 // Layout: [slots][256 B: sweeping queue, guess queue, list count][guess path: list, batch ints].
 // The two kernels run one after the other on the stream and share the slots.
-template <class G>
+// NET: the guess kernel is guess_net_kernel (args.netw, args.warm_out set;
+// the crossbars it lists are solved from warm_out).
+template <class G, bool NET = false>
 int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, size_t fixed_bytes = 0)
 {
     // the coarse warm-start grid is staged in the halo planes' LDS
@@ -243,9 +371,11 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
         return set_error(FADESIM_E_UNSUPPORTED, "d_warm_coarse: coarse grid too large for this shape (raise warm_level)");
     const size_t slot_bytes = (size_t)rb::kWsPlanes * G::M * G::N * sizeof(float);
     const size_t list_bytes = ((size_t)args.batch * sizeof(int) + 255) / 256 * 256;
-    bool guess = takes_guess_path(args);
-    int grid = std::min(resident_slots<G, false>(), args.batch);
-    int ggrid = guess ? std::min(resident_slots<G, true>(), args.batch) : 0;
+    if (NET && fixed)
+        return set_error(FADESIM_E_UNSUPPORTED, "the in-kernel network start needs the library workspace");
+    bool guess = NET || takes_guess_path(args);
+    int grid = std::min(sweep_slots<G>(), args.batch);
+    int ggrid = guess ? std::min(NET ? guess_net_slots() : guess_slots<G>(), args.batch) : 0;
     char *ws;
     if (fixed) {
         if (guess && fixed_bytes < 256 + list_bytes + slot_bytes)
@@ -281,12 +411,17 @@ int launch_rb(RbArgs args, int M, int N, hipStream_t s, char *fixed = nullptr, s
         ga.queue = ctr + 1;
         ga.list = (int *)(ws + slots_bytes + 256);
         ga.list_count = ctr + 2;
-        hipLaunchKernelGGL((guess_kernel<G>), dim3(ggrid), dim3(G::NT), 0, s, ga);
+        if constexpr (NET)
+            hipLaunchKernelGGL(guess_net_kernel, dim3(ggrid), dim3(G::NT), 0, s, ga);
+        else
+            hipLaunchKernelGGL((guess_kernel<G>), dim3(ggrid), dim3(G::NT), 0, s, ga);
         FADESIM_HIP_CHECK(hipGetLastError());
         if (timing)
             FADESIM_HIP_CHECK(hipEventRecord(t_ev[2], s));
         args.list = ga.list;
         args.list_count = ga.list_count;
+        if constexpr (NET)
+            args.warm = args.warm_out;
     }
     hipLaunchKernelGGL((rbsor_kernel<G>), dim3(grid), dim3(G::NT), 0, s, args);
     FADESIM_HIP_CHECK(hipGetLastError());
@@ -417,6 +552,9 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
     args.est = d->d_est;
     args.rejected = d->d_guess_rejected;
     args.field = d->d_field;
+    for (int l = 0; l < 7; ++l)
+        args.netw[l] = nullptr;
+    args.warm_out = nullptr;
     return FADESIM_OK;
 }
 
@@ -513,4 +651,46 @@ extern "C" int fadesim_solve_batch_noalloc(const fadesim_batch_desc *d, void *st
         args.vout_ld = 0;
     }
     return launch_shape(args, d->rows, d->cols, s, base + kScratchZeroBytes, kScratchSlotsBytes);
+}
+
+extern "C" int fadesim_nn_solve_batch(const fadesim_batch_desc *d, const void *const *d_w, const int *kp, void *stream)
+{
+    RbArgs args;
+    bool empty;
+    int rc = batch_args(d, args, &empty);
+    if (rc != FADESIM_OK)
+        return rc;
+    if (!d_w || !kp)
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_solve_batch: d_w and kp are required");
+    const int cin[7] = {8, kNetWidth, kNetWidth, 2 * kNetWidth, 2 * kNetWidth, kNetWidth, kNetWidth};
+    for (int l = 0; l < 7; ++l)
+        if (!d_w[l] || kp[l] != nn::net_kpad(cin[l]))
+            return set_error(FADESIM_E_UNSUPPORTED, "fadesim_nn_solve_batch: the weights are not the coarse16f "
+                                                    "network's (layer " + std::to_string(l) + ")");
+    if (d->rows != NetGeo::M || d->cols != NetGeo::N)
+        return set_error(FADESIM_E_UNSUPPORTED, "fadesim_nn_solve_batch: 128 x 128 crossbars only");
+    if (d->d_warm || d->d_warm_coarse || d->d_vout)
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_solve_batch: d_warm, d_warm_coarse and d_vout must be NULL "
+                                            "(the start is formed in-kernel, for grounded bitlines)");
+    const unsigned f = d->flags;
+    if (!(f & FADESIM_F_TEST_FIRST) || (f & (FADESIM_F_FIXED_SWEEPS | FADESIM_F_NO_FASTPATH)))
+        return set_error(FADESIM_E_INVALID, "fadesim_nn_solve_batch: needs FADESIM_F_TEST_FIRST, without "
+                                            "FADESIM_F_FIXED_SWEEPS or FADESIM_F_NO_FASTPATH");
+    if (empty)
+        return FADESIM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    constexpr int M = NetGeo::M, N = NetGeo::N;
+    double *z = (double *)workspace((size_t)N * sizeof(double));
+    if (!z)
+        return FADESIM_E_NOMEM;
+    hipLaunchKernelGGL(fill_zero_f64, dim3(1), dim3(256), 0, s, z, (size_t)N);
+    FADESIM_HIP_CHECK(hipGetLastError());
+    args.vout = z;
+    args.vout_ld = 0;
+    for (int l = 0; l < 7; ++l)
+        args.netw[l] = (const __bf16 *)d_w[l];
+    args.warm_out = (float *)workspace_warm((size_t)args.batch * 2 * M * N * sizeof(float));
+    if (!args.warm_out)
+        return FADESIM_E_NOMEM;
+    return launch_rb<NetGeo, true>(args, M, N, s);
 }

@@ -65,7 +65,7 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
                 fixed_sweeps: Optional[int] = None, want_field: bool = False,
                 out: Optional[torch.Tensor] = None, timing: bool = False,
                 warm_coarse: Optional[torch.Tensor] = None, warm_scale: Optional[torch.Tensor] = None,
-                warm_level: int = 2, noalloc: bool = False, fastpath: bool = True) -> SolveResult:
+                warm_level: int = 2, noalloc: bool = False, fastpath: bool = True, net=None) -> SolveResult:
     """Solve B crossbars of one shape.
 
     g [B, M, N] float32 (S) -- or [S, M, N] conductance slots with ``gidx`` [B]
@@ -89,6 +89,11 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     sweeping kernel; the same results).
     ``noalloc`` solves through fadesim_solve_batch_noalloc (the library
     allocates nothing; the outputs here are still allocated by torch).
+    ``net`` (a coarse16f WarmStartNet, 128x128 crossbars, grounded bitlines)
+    forms the learned warm start inside the solve (fadesim_nn_solve_batch):
+    the same result, bit for bit, as ``**net.warm_kwargs(g, r_wire, vin)``
+    with test_first, without the features and network launches or the base
+    field's round trip through HBM.
     """
     if g.dim() != 3:
         raise ValueError("g must be [B, M, N]")
@@ -129,7 +134,13 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     status = torch.empty(B, dtype=torch.int32, device=dev)
     est = torch.empty(B, dtype=torch.float64, device=dev)
     field = torch.empty((B, 2, M, N), dtype=torch.float32, device=dev) if want_field else None
-    rejected = torch.empty(B, dtype=torch.int32, device=dev) if warm is not None else None
+    rejected = torch.empty(B, dtype=torch.int32, device=dev) if (warm is not None or net is not None) else None
+    if net is not None:
+        if warm is not None or warm_coarse is not None or vout is not None or noalloc or not fastpath:
+            raise ValueError("net forms the warm start itself: no warm, warm_coarse, vout, noalloc or fastpath=False")
+        if test_first is False or fixed_sweeps is not None:
+            raise ValueError("net needs test_first and no fixed_sweeps")
+        test_first = True
 
     flags = 0
     if fixed_sweeps is not None:
@@ -166,6 +177,14 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     if warm_coarse is not None:
         d.d_warm_coarse, d.d_warm_scale = warm_coarse.data_ptr(), warm_scale.data_ptr()
         d.warm_coarse_ch, d.warm_level = int(warm_coarse.shape[3]), int(warm_level)
+    if net is not None:
+        wptr, kp = net.fused_weights()
+        with torch.cuda.device(dev):
+            rc = nat.lib().fadesim_nn_solve_batch(ctypes.byref(d), ctypes.cast(wptr, ctypes.c_void_p),
+                                                  ctypes.cast(kp, ctypes.c_void_p),
+                                                  ctypes.c_void_p(nat.stream_handle(dev)))
+        nat.check(rc, "fadesim_nn_solve_batch")
+        return SolveResult(iout, sweeps, status, est, field, rejected)
     fn = nat.lib().fadesim_solve_batch_noalloc if noalloc else nat.lib().fadesim_solve_batch
     with torch.cuda.device(dev):
         rc = fn(ctypes.byref(d), ctypes.c_void_p(nat.stream_handle(dev)))

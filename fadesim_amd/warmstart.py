@@ -324,20 +324,33 @@ class WarmStartNet:
         (fadesim_nn_coarse_forward)."""
         return self.arch in ("coarse32f", "coarse16f") and (self.M, self.N) == (128, 128)
 
+    def fused_weights(self):
+        """The fused network's weights as the kernels read them: 7 layer
+        pointers (ctypes array) to fragment-major [Kp/32][Cout][32] bf16 copies
+        (a wave's fragment is 1 KB contiguous) and their depths Kp, refreshed
+        when the weights change."""
+        if getattr(self, "_frag_ver", None) != self._wver:
+            byname = {op.name: op for op in self.ops}
+            self._frag = [self._view(self.wb, byname[n]).view(byname[n].cout, -1, 32).transpose(0, 1).contiguous()
+                          for n in self.FUSED_LAYERS]
+            self._frag_ptrs = (ctypes.c_void_p * 7)(*[f.data_ptr() for f in self._frag])
+            self._frag_kp = (ctypes.c_int * 7)(*[kpad(byname[n].cin) for n in self.FUSED_LAYERS])
+            self._frag_ver = self._wver
+        return self._frag_ptrs, self._frag_kp
+
+    @property
+    def solves_in_kernel(self) -> bool:
+        """coarse16f at 128x128: solve_batch(..., net=self) forms the warm
+        start inside the solve (fadesim_nn_solve_batch)."""
+        return self.arch == "coarse16f" and (self.M, self.N) == (128, 128)
+
     def forward_fused(self, x, out=None):
         """x [B][32][32][8] bf16 -> out [B][32][32][16] bf16, the whole network
         in one launch (every activation of a crossbar in LDS)."""
         B = x.shape[0]
         if out is None:
             out = torch.empty((B, 32, 32, OUT_C), dtype=torch.bfloat16, device=self.dev)
-        byname = {op.name: op for op in self.ops}
-        if getattr(self, "_frag_ver", None) != self._wver:
-            # fragment-major weights [Kp/32][Cout][32]: a wave's fragment is 1 KB contiguous
-            self._frag = [self._view(self.wb, byname[n]).view(byname[n].cout, -1, 32).transpose(0, 1).contiguous()
-                          for n in self.FUSED_LAYERS]
-            self._frag_ver = self._wver
-        ptrs = (ctypes.c_void_p * 7)(*[f.data_ptr() for f in self._frag])
-        kps = (ctypes.c_int * 7)(*[kpad(byname[n].cin) for n in self.FUSED_LAYERS])
+        ptrs, kps = self.fused_weights()
         _chk(nat.lib().fadesim_nn_coarse_forward(B, ctypes.c_void_p(x.data_ptr()), ptrs, kps,
                                                  ctypes.c_void_p(out.data_ptr()), _s(self.dev)), "coarse_forward")
         return out

@@ -55,6 +55,25 @@ int fadesim_nn_combine_up(int B, int M, int N, const float *d_base, const float 
 int fadesim_nn_coarse_forward(int B, const void *d_x, const void *const *d_w, const int *kp, void *d_out,
                               void *stream);
 
+/* fadesim_solve_batch (include/fadesim.h) with the coarse16f warm start formed
+ * inside the solve: per crossbar, the pooled features and base field of
+ * fadesim_nn_features_pooled, the network of fadesim_nn_coarse_forward and the
+ * combine at load of d_warm_coarse (level 2, 16 channels), all in the
+ * guess-first kernel with the activations in LDS -- the same outputs, bit for
+ * bit, as those three steps followed by fadesim_solve_batch with d_warm = base,
+ * d_warm_coarse = out, d_warm_scale = scale, and no base field, features or
+ * network output written to HBM.  Like the reference's initial guess
+ * (crossbarHspice/V2/concrete_command.h:1171-1178), which it replaces, it feeds
+ * the solve of NodebasedGSMethod::nodeBaseGS named at fadesim_solve_batch.
+ * Needs rows = cols = 128, FADESIM_F_TEST_FIRST (not FIXED_SWEEPS or
+ * NO_FASTPATH), and d_warm, d_warm_coarse, d_vout NULL (grounded bitlines);
+ * d_w, kp: the 7 fragment-major layer weights and depths as for
+ * fadesim_nn_coarse_forward, width 16.  The library keeps a workspace of
+ * batch x 128 KB for the starts of crossbars that need sweeps. */
+struct fadesim_batch_desc;
+int fadesim_nn_solve_batch(const struct fadesim_batch_desc *desc, const void *const *d_w, const int *kp,
+                           void *stream);
+
 /* 3x3 same convolution (MFMA), optional ReLU: y [B][H][W][Cout] */
 int fadesim_nn_conv3x3_fwd(int B, int H, int W, int Cin, int Cout, int Kp, const void *d_x, const void *d_w, void *d_y,
                            int relu, void *stream);

@@ -143,6 +143,22 @@ def test_c_abi_rejects_bad_descriptors_without_a_gpu():
     assert "d_warm_coarse" in err(L.fadesim_solve_batch(ctypes.byref(d), None))
     d.warm_coarse_ch, d.warm_level = 16, 7
     assert "d_warm_coarse" in err(L.fadesim_solve_batch(ctypes.byref(d), None))
+    # the in-kernel network start: coarse16f weights, 128x128, no given
+    # start or vout, the first test before any sweep
+    n = nat.BatchDesc()
+    n.batch, n.rows, n.cols, n.tol, n.flags = 4, 128, 128, 1e-6, nat.F_TEST_FIRST
+    n.d_g = n.d_r_wire = n.d_vin = n.d_iout = 8
+    kp16 = (ctypes.c_int * 7)(96, 160, 160, 320, 320, 160, 160)
+    kp32 = (ctypes.c_int * 7)(96, 320, 320, 608, 608, 320, 320)
+    assert "coarse16f" in err(L.fadesim_nn_solve_batch(ctypes.byref(n), w, kp32, None))
+    n.rows = n.cols = 64
+    assert "128 x 128" in err(L.fadesim_nn_solve_batch(ctypes.byref(n), w, kp16, None))
+    n.rows = n.cols = 128
+    n.d_vout = 8
+    assert "d_vout" in err(L.fadesim_nn_solve_batch(ctypes.byref(n), w, kp16, None))
+    n.d_vout = None
+    n.flags = 0
+    assert "TEST_FIRST" in err(L.fadesim_nn_solve_batch(ctypes.byref(n), w, kp16, None))
 
 
 def test_c_abi_refuses_a_descriptor_of_another_size():
