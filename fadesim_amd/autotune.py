@@ -4,9 +4,11 @@ For each crossbar size (64, 128, 256, 512) it times the persistent solve of a
 sample of the uneven-size workload (fadesim_amd.workloads.uneven_batch) on the
 GPU over a grid of candidates and keeps the fastest candidate whose solves
 all reach the tolerance: first the number of wire-solve (IR2S) iterations
-applied to the initial guess and to the refinement correction (their
-contraction per iteration grows with r g n^2, so small crossbars want several
-and 512x512 few), then, with that choice, the relaxation factor
+applied to the initial guess (the same count on the refinement correction)
+and their damping -- the contraction per iteration grows with r g n^2, so
+small crossbars want few and 512x512 many, damped; then, with that guess,
+the correction's own count, which only has to take the refined error from
+the switch point to the tolerance; then the relaxation factor
 w(k) = 2 / (1 + k sin(pi / (2n + 1))) and the bounds of the interval between
 convergence tests.  The table is stored per GPU architecture under
 fadesim_amd/tuning/<arch>.json and read by RaggedPlan.
@@ -29,6 +31,7 @@ KAPPAS = (0.55, 0.7, 0.83, 1.0, 1.25)
 CHECKS = ((32, 16, 256), (64, 32, 256), (32, 32, 512))
 IR2S = (0, 2, 4, 6, 8, 10, 12, 14, 16)   # iterations on the initial guess (and the correction's)
 IR2S_OMEGA = (1.0, 0.85, 0.75, 0.7, 0.6)  # their damping: 2/(2+rho) for coupling radius rho
+CORR = (1, 2, 3, 4, 5, 6, 8, 10, 12)      # iterations on the refinement correction
 
 
 def arch_name(device) -> str:
@@ -77,20 +80,31 @@ def tune_size(n: int, count: int, tol: float, dev, log=print, reps: int = 2) -> 
             f"{ms:.3f} ms, mean sweeps {sw:.0f}{'' if ok else ' (not converged)'}")
         return (ms if ok else math.inf, kappa, entry, sw)
 
-    def make(kappa, checks, k, w):
+    def make(kappa, checks, k, w, kc=None):
         cf, cmin, cmax = checks
         return {"omega": omega_of(n, kappa), "check_first": cf, "check_min": cmin, "check_max": cmax,
-                "ir2s_iters": k, "corr_iters": max(2, k), "ir2s_omega": w}
+                "ir2s_iters": k, "corr_iters": max(2, k) if kc is None else kc, "ir2s_omega": w}
 
     stage1 = [trial(1.0, make(1.0, CHECKS[0], k, w)) for k in IR2S for w in IR2S_OMEGA if k > 0 or w == 1.0]
     e1 = min(stage1, key=lambda x: x[0])[2]
     k_best, w_best = e1["ir2s_iters"], e1["ir2s_omega"]
     results = list(stage1)
+    # the correction's count on its own, with the guess's count and one step
+    # either side
+    if k_best > 0:
+        for k in sorted({max(2, k_best - 2), k_best, k_best + 2}):
+            for kc in CORR:
+                if kc < k_best + 2 and not (k == k_best and kc == max(2, k_best)):
+                    results.append(trial(1.0, make(1.0, CHECKS[0], k, w_best, kc)))
+        e1 = min(results, key=lambda x: x[0])[2]
+        k_best, kc_best = e1["ir2s_iters"], e1["corr_iters"]
+    else:
+        kc_best = None
     for kappa in KAPPAS:
         for checks in CHECKS:
             if kappa == 1.0 and checks == CHECKS[0]:
                 continue
-            results.append(trial(kappa, make(kappa, checks, k_best, w_best)))
+            results.append(trial(kappa, make(kappa, checks, k_best, w_best, kc_best)))
     best = min(results, key=lambda x: x[0])
     out = dict(best[2])
     out.update({"kappa": best[1], "ms": best[0], "mean_sweeps": best[3], "sample": count})
@@ -114,7 +128,8 @@ def main():
         classes[str(n)] = tune_size(n, c, args.tol, dev, log)
         log(f"size {n}: best {classes[str(n)]}")
     table = {"arch": arch_name(dev), "device": torch.cuda.get_device_name(dev), "tol": args.tol,
-             "seconds": time.time() - t0, "candidates": {"kappa": KAPPAS, "checks": CHECKS, "ir2s_iters": IR2S, "ir2s_omega": IR2S_OMEGA}, "classes": classes}
+             "seconds": time.time() - t0, "candidates": {"kappa": KAPPAS, "checks": CHECKS, "ir2s_iters": IR2S, "ir2s_omega": IR2S_OMEGA,
+                            "corr_iters": CORR}, "classes": classes}
     out = args.out or table_path(dev)
     os.makedirs(os.path.dirname(out), exist_ok=True)
     with open(out, "w") as f:

@@ -145,3 +145,49 @@ def test_ir2s_damped_converged_mixed_vs_exact(cuda):
         err = np.max(np.abs(iouts[k] - iex) / np.abs(iex))
         assert err <= 1e-6, (k, n, err)
     assert res.sweeps.double().sum().item() < 0.5 * base.sweeps.double().sum().item()
+
+
+def test_strip_exchange_with_every_cu_occupied(cuda):
+    """48 crossbars of 512x512 and 64 of 256x256 among 128x128 and 64x64 ones:
+    more team tasks than the grid has teams, so every workgroup of the
+    persistent grid is busy and the teams' exchanges run side by side with
+    the small items on every CU.  Converged with the tuned schedule (IR2S
+    guess, team column sums, refinement), each sampled crossbar's outputs
+    equal bit for bit its solve alone in a plan of its own; with fixed sweeps
+    the sampled strips' drop fields equal the CPU FP32 red-black replay."""
+    import torch
+    from fadesim_amd.ragged import RaggedPlan, SIZES
+    inst = _mixed(cuda, [512] * 48 + [256] * 64 + [128] * 96 + [64] * 96, seed=41)
+    plan = RaggedPlan(inst["rows"], device=cuda)
+    assert plan.stats["big_tasks"] > plan.stats["teams"]
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    torch.cuda.synchronize()
+    assert not plan.aborted()
+    assert (res.status.cpu().numpy() == 0).all()
+    iouts = plan.unpack(res.iout, "cols")
+    rows = list(inst["rows"])
+    picks = [rows.index(512), len(rows) - 1 - rows[::-1].index(512), rows.index(256),
+             len(rows) - 1 - rows[::-1].index(256), rows.index(128), rows.index(64)]
+    for k in picks:
+        n = rows[k]
+        one = RaggedPlan([n], device=cuda)
+        g1 = torch.tensor(inst["g_list"][k].reshape(-1), device=cuda)
+        r1 = torch.tensor([inst["r_np"][k]], dtype=torch.float64, device=cuda)
+        v1 = torch.tensor(inst["vin_list"][k], dtype=torch.float64, device=cuda)
+        alone = one.solve(g1, r1, v1, tol=1e-6)
+        torch.cuda.synchronize()
+        assert np.array_equal(alone.iout.cpu().numpy(), iouts[k]), (k, n)
+        assert int(alone.sweeps[0]) == int(res.sweeps[k]), (k, n)
+    # fixed sweeps: the row exchange of every strip under full occupancy
+    plain = RaggedPlan(inst["rows"], device=cuda, tuning={})
+    sweeps = 23
+    fx = plain.solve(inst["g"], inst["r"], inst["vin"], fixed_sweeps=sweeps, want_field=True)
+    torch.cuda.synchronize()
+    assert not plain.aborted()
+    fields = plain.unpack(fx.field, "field")
+    for k in picks[:4]:
+        n = rows[k]
+        w = plain.stats["omega"][SIZES.index(n)]
+        p, q = oracle.rb_f32(inst["g_list"][k], inst["r_np"][k], inst["vin_list"][k], None, w, sweeps)
+        assert np.array_equal(fields[k][0], p), (k, n)
+        assert np.array_equal(fields[k][1], q), (k, n)
