@@ -279,9 +279,10 @@ def main():
     }
     if net is not None:
         # the network's field, one IR2S iteration on it, then the same test /
-        # SOR: features and base field, network, and the network output
-        # combined with the base field while the solve loads it
-        variants["nn"] = (1, 1, True, True, lambda timing=False: solve_batch(
+        # SOR: pooled features, network, and the network output combined
+        # while the solve loads it with the base field, which the solve forms
+        # itself (two wire-solve passes from zero)
+        variants["nn"] = (1, 3, False, True, lambda timing=False: solve_batch(
             tg, tr, tv, **net.warm_kwargs(tg, tr, tv), tol=args.tol, ir2s_iters=1, corr_iters=K, test_first=True,
             timing=timing))
     if net is not None and net.solves_in_kernel:

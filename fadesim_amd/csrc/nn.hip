@@ -61,10 +61,12 @@ __global__ void __launch_bounds__(FeatGeo::NT) features_pooled_kernel(const floa
     float p[TR][TC], q[TR][TC];
     const float S2 = nn::pooled_features(sh, g + (size_t)b * MN, rw[b], vin + (size_t)b * M,
                                          feat + (size_t)b * (M / 4) * (N / 4) * nn::kFeatC, red, p, q);
+    if (base) {
 #pragma unroll
-    for (int ii = 0; ii < TR; ++ii) {
-        rb::strow<TC>(base + (size_t)b * 2 * MN + (size_t)(i0 + ii) * N + j0, p[ii]);
-        rb::strow<TC>(base + (size_t)b * 2 * MN + MN + (size_t)(i0 + ii) * N + j0, q[ii]);
+        for (int ii = 0; ii < TR; ++ii) {
+            rb::strow<TC>(base + (size_t)b * 2 * MN + (size_t)(i0 + ii) * N + j0, p[ii]);
+            rb::strow<TC>(base + (size_t)b * 2 * MN + MN + (size_t)(i0 + ii) * N + j0, q[ii]);
+        }
     }
     if (t == 0)
         scale[b] = S2;
@@ -766,7 +768,7 @@ extern "C" int fadesim_nn_features_pooled(int B, int M, int N, int level, const 
                                           const double *d_vin, void *d_feat, float *d_base, float *d_scale,
                                           void *stream)
 {
-    if (B < 0 || !d_g || !d_r || !d_vin || !d_feat || !d_base || !d_scale)
+    if (B < 0 || !d_g || !d_r || !d_vin || !d_feat || !d_scale)
         return set_error(FADESIM_E_INVALID, "fadesim_nn_features_pooled: bad arguments");
     if (M != FeatGeo::M || N != FeatGeo::N || level != 2)
         return set_error(FADESIM_E_UNSUPPORTED, "fadesim_nn_features_pooled: 128x128 crossbars, level 2 only");

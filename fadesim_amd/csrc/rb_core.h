@@ -1441,28 +1441,6 @@ __device__ __forceinline__ void tile_store(float *wsb, int i0, int j0, const flo
     }
 }
 
-// Warm start combined at load (Inst::coarse): planes 3, 4 = base + scale x
-// bilinear(coarse) over this thread's tile, out of line so that its
-// registers do not weigh on the sweep loop.
-template <class G, bool STRIP>
-__device__ __attribute__((noinline)) void phase_coarse(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
-                                                       int i0, int j0, int t, RB_SH_PARAMS)
-{
-    RB_SH_LOCAL;
-    (void)X;
-    (void)gp;
-    constexpr int TR = G::TR, TC = G::TC, N = G::N;
-    const size_t MNall = (size_t)I.rows * N;
-    float yp[TR][TC], yq[TR][TC];
-#pragma unroll
-    for (int ii = 0; ii < TR; ++ii) {
-        ldrow<TC>(I.warm + (size_t)(gi0 + i0 + ii) * N + j0, yp[ii]);
-        ldrow<TC>(I.warm + MNall + (size_t)(gi0 + i0 + ii) * N + j0, yq[ii]);
-    }
-    add_coarse<G>(I, sh.halo, t, gi0, i0, j0, yp, yq);
-    tile_store<G>(wsb, i0, j0, yp, yq);
-}
-
 // Wire-solve (IR2S) iterations on the drops, in place on the drops held in planes 3, 4:
 //   s_ij = g_ij (c_ij - p_ij - q_ij)            (cell currents of the drops)
 //   p_ij <- r sum_j' (min(j,j')+1) s_ij'       (wordline: source segment + wires)
@@ -1484,11 +1462,17 @@ __device__ __attribute__((noinline)) void phase_coarse(Inst I, StripCtx X, float
 // (kStartZero), at the warm start (kStartWarm), or at the warm start plus the
 // network's coarse correction combined while loading (kStartWarmCoarse, whole
 // crossbars) -- read from HBM here, not from a workspace plane.  The result
-// goes to planes 3, 4.
+// goes to planes 3, 4.  kStartBaseCoarse: the warm start's base field formed
+// here -- two undamped wire-solve iterations from zero drops, the base of
+// fadesim_nn_features_pooled when the bitlines are grounded -- plus the
+// coarse correction (whole crossbars).
 constexpr int kStartZero = 0, kStartWarm = 1, kStartWarmCoarse = 2;
 // the learned warm start formed in-kernel (guess_solve with a START that
 // provides it)
 constexpr int kStartNet = 3;
+constexpr int kStartBaseCoarse = 4;
+// undamped wire-solve iterations of the base field of kStartBaseCoarse
+constexpr int kBaseIters = 2;
 
 // The starting drops of the wire-solve iterations (start as above) into the
 // register tile p, q.
@@ -1563,6 +1547,51 @@ __device__ __forceinline__ bool ir2s_iterate(const Inst &I, const StripCtx &X, c
     return true;
 }
 
+// The starting drops of mode `start` into the register tile p, q: load_start,
+// or for kStartBaseCoarse the base field's wire-solve iterations (HOLD as
+// for ir2s_iterate) and the coarse correction.  Returns false when the team
+// aborted.
+template <class G, bool STRIP, bool HOLD>
+__device__ __forceinline__ bool start_tile(const Inst &I, const StripCtx &X, const Shared &sh,
+                                           const float (&gt)[G::TR][G::TC], const float *gp, int gi0, int i0, int j0,
+                                           int t, int tx, int ty, double rwire, int &nbar, int start,
+                                           float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC])
+{
+    if (start != kStartBaseCoarse) {
+        load_start<G, STRIP>(I, sh, t, gi0, i0, j0, start, p, q);
+        return true;
+    }
+#pragma unroll
+    for (int ii = 0; ii < G::TR; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < G::TC; ++jj)
+            p[ii][jj] = q[ii][jj] = 0.f;
+    if (!ir2s_iterate<G, STRIP, HOLD>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, nbar, kBaseIters, 1.0, p, q))
+        return false;
+    if (!STRIP)
+        add_coarse<G>(I, sh.halo, t, gi0, i0, j0, p, q);
+    return true;
+}
+
+// Warm start combined at load (Inst::coarse): planes 3, 4 = base + scale x
+// bilinear(coarse) over this thread's tile, the base read from I.warm or, when
+// there is none, formed here (kStartBaseCoarse), out of line so that its
+// registers do not weigh on the sweep loop.
+template <class G, bool STRIP>
+__device__ __attribute__((noinline)) bool phase_coarse(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
+                                                       int i0, int j0, int t, int tx, int ty, double rwire,
+                                                       RB_SH_PARAMS, int *nbar)
+{
+    RB_SH_LOCAL;
+    constexpr int TR = G::TR, TC = G::TC;
+    float yp[TR][TC], yq[TR][TC], gt[TR][TC];
+    if (!start_tile<G, STRIP, false>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar,
+                                     I.warm ? kStartWarmCoarse : kStartBaseCoarse, yp, yq))
+        return false;
+    tile_store<G>(wsb, i0, j0, yp, yq);
+    return true;
+}
+
 template <class G, bool STRIP>
 __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
                                                      int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
@@ -1571,13 +1600,14 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
     RB_SH_LOCAL;
     constexpr int TC = G::TC;
     float p[G::TR][TC], q[G::TR][TC];
-    load_start<G, STRIP>(I, sh, t, gi0, i0, j0, uniform_i32(start), p, q);
     // the tile's conductances stay in registers for all iterations: an
     // iteration reads no memory but LDS
     float gt[G::TR][TC];
 #pragma unroll
     for (int ii = 0; ii < G::TR; ++ii)
         ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
+    if (!start_tile<G, STRIP, true>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar, uniform_i32(start), p, q))
+        return false;
     if (!ir2s_iterate<G, STRIP, true>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar, iters, w, p, q))
         return false;
     tile_store<G>(wsb, i0, j0, p, q);
@@ -1676,7 +1706,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
     if (C.ir2s_iters > 0) {
         // the IR2S phase reads the initial guess itself (the coarse
         // correction combined while loading)
-        const int start = !I.warm ? kStartZero : coarse ? kStartWarmCoarse : kStartWarm;
+        const int start = coarse ? (I.warm ? kStartWarmCoarse : kStartBaseCoarse) : I.warm ? kStartWarm : kStartZero;
         if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
                                   C.ir2s_omega, start))
             return false;
@@ -1684,7 +1714,8 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
         snap_valid = true;
     } else if (coarse) {
         // the warm start combined while loading, into planes 3, 4
-        phase_coarse<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, RB_SH_ARGS(sh));
+        if (!phase_coarse<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar))
+            return false;
         tile_load<G>(wsb, i0, j0, yp, yq);
         snap_valid = true;
     } else {

@@ -172,9 +172,14 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
         I.cscale = A.coarse ? A.cscale[b] : 0.f;
         I.cch = A.cch;
         I.clevel = A.clevel;
-        const int start = !I.warm ? rb::kStartZero : I.coarse ? rb::kStartWarmCoarse : rb::kStartWarm;
+        const int start = I.coarse ? (I.warm ? rb::kStartWarmCoarse : rb::kStartBaseCoarse)
+                                   : I.warm ? rb::kStartWarm : rb::kStartZero;
         auto load = [&](int st, int t, int i0, int j0, float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC]) {
-            rb::load_start<G, false>(I, sh, t, 0, i0, j0, st, p, q);
+            const rb::StripCtx X{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+            float gt[G::TR][G::TC];
+            int nbar = 0;
+            rb::start_tile<G, false, kGuessHoldsTile>(I, X, sh, gt, I.g, 0, i0, j0, t, t % G::TX, t / G::TX, I.rwire,
+                                                      nbar, st, p, q);
         };
         const bool done = rb::guess_solve<G, kGuessHoldsTile>(I, A.ctl, wsb, sh, fld, start, load);
         if (!done && threadIdx.x == 0)
@@ -513,9 +518,9 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
         return set_error(FADESIM_E_INVALID, "ir2s_omega must lie in (0, 1] (0 selects 1)");
     if (d->d_warm_coarse) {
         const int L = d->warm_level;
-        if (!d->d_warm || !d->d_warm_scale || d->warm_coarse_ch < 2 || (d->warm_coarse_ch & 1) || L < 1 || L > 6 || ((M >> L) << L) != M ||
+        if (!d->d_warm_scale || d->warm_coarse_ch < 2 || (d->warm_coarse_ch & 1) || L < 1 || L > 6 || ((M >> L) << L) != M ||
             ((N >> L) << L) != N || (M >> L) < 1 || (N >> L) < 1)
-            return set_error(FADESIM_E_INVALID, "d_warm_coarse needs d_warm, d_warm_scale, an even warm_coarse_ch "
+            return set_error(FADESIM_E_INVALID, "d_warm_coarse needs d_warm_scale, an even warm_coarse_ch "
                                                 ">= 2 and warm_level in 1..6 dividing rows and cols");
     }
     args.batch = B;

@@ -82,9 +82,11 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     (0: 2), ``ir2s_omega`` their damping.  ``test_first`` tests the initial guess before the first sweep
     (default: when ir2s_iters >= 3).
     ``warm_coarse`` [B, M >> warm_level, N >> warm_level, C] bf16 with
-    ``warm_scale`` [B]: the initial drops are warm + warm_scale x
+    ``warm_scale`` [B]: the initial drops are base + warm_scale x
     bilinear(warm_coarse[..., 0:2]), formed while loading (the arithmetic of
-    fadesim_nn_combine_up; WarmStartNet.warm_kwargs builds these).
+    fadesim_nn_combine_up; WarmStartNet.warm_kwargs builds these), base being
+    ``warm`` or, when warm is None, two undamped IR2S iterations from zero
+    drops formed in the kernel.
     ``fastpath=False`` sets FADESIM_F_NO_FASTPATH (every crossbar on the
     sweeping kernel; the same results).
     ``noalloc`` solves through fadesim_solve_batch_noalloc (the library
@@ -119,8 +121,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
         warm = warm.to(device=dev, dtype=torch.float32).reshape(B, 2, M, N).contiguous()
     if warm_coarse is not None:
         L = int(warm_level)
-        if warm is None or warm_scale is None:
-            raise ValueError("warm_coarse needs warm (the base field) and warm_scale")
+        if warm_scale is None:
+            raise ValueError("warm_coarse needs warm_scale")
         if warm_coarse.dtype != torch.bfloat16 or warm_coarse.dim() != 4 or \
                 tuple(warm_coarse.shape[:3]) != (B, M >> L, N >> L) or warm_coarse.shape[3] < 2 or \
                 warm_coarse.shape[3] % 2:
@@ -134,7 +136,8 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     status = torch.empty(B, dtype=torch.int32, device=dev)
     est = torch.empty(B, dtype=torch.float64, device=dev)
     field = torch.empty((B, 2, M, N), dtype=torch.float32, device=dev) if want_field else None
-    rejected = torch.empty(B, dtype=torch.int32, device=dev) if (warm is not None or net is not None) else None
+    rejected = torch.empty(B, dtype=torch.int32, device=dev) if (warm is not None or warm_coarse is not None
+                                                                  or net is not None) else None
     if net is not None:
         if warm is not None or warm_coarse is not None or vout is not None or noalloc or not fastpath:
             raise ValueError("net forms the warm start itself: no warm, warm_coarse, vout, noalloc or fastpath=False")

@@ -261,19 +261,22 @@ class WarmStartNet:
         return net
 
     # ------------------------------------------------------------------ features
-    def features(self, g, r, vin, base=None, scale=None):
+    def features(self, g, r, vin, base=None, scale=None, want_base=True):
         """Input features at the network's input level, the second-order base
         field [B][2][M][N] and the per-instance output scale (written into
-        ``base`` / ``scale`` when given)."""
+        ``base`` / ``scale`` when given).  want_base=False (pooled features
+        only): no base field is written and None is returned in its place."""
         B = g.shape[0]
         lv = self.in_level
         H, W = self._hw(lv)
         feat = torch.empty((B, H, W, N_FEAT), dtype=torch.bfloat16, device=self.dev)
-        if base is None:
+        if not want_base and lv == 0:
+            raise ValueError("the full-resolution features kernel always writes the base field")
+        if base is None and want_base:
             base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
         if scale is None:
             scale = torch.empty(B, dtype=torch.float32, device=self.dev)
-        ptrs = [ctypes.c_void_p(x.data_ptr()) for x in (g, r, vin, feat, base, scale)]
+        ptrs = [ctypes.c_void_p(x.data_ptr() if x is not None else None) for x in (g, r, vin, feat, base, scale)]
         if lv == 0:
             _chk(nat.lib().fadesim_nn_features(B, self.M, self.N, *ptrs, _s(self.dev)), "nn_features")
         else:
@@ -389,13 +392,15 @@ class WarmStartNet:
                      "nn_combine_up")
         return warm
 
-    def predict_parts(self, g, r, vin, chunk: int = 2048):
+    def predict_parts(self, g, r, vin, chunk: int = 2048, with_base: bool = True):
         """The warm start before its final combination: base field [B][2][M][N]
         FP32, network output [B][M >> level][N >> level][16] BF16, scale [B]
         FP32 and the output level (>= 1), for solve_batch(warm=base,
         warm_coarse=out, warm_scale=scale, warm_level=level), which forms
         base + scale x bilinear(out) while loading -- the same field predict()
-        returns, without writing it."""
+        returns, without writing it.  with_base=False (pooled features): base
+        is None and solve_batch(warm_coarse=out, ...) forms the base itself,
+        bit for bit the same."""
         lv = self.out_level
         if lv < 1:
             raise ValueError("predict_parts needs a network whose output is coarser than the crossbar")
@@ -403,26 +408,35 @@ class WarmStartNet:
         g = g.to(torch.float32).contiguous()
         r = r.to(torch.float64).reshape(-1).expand(B).contiguous()
         vin = vin.to(torch.float64).contiguous()
-        base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev)
+        with_base = with_base or self.in_level == 0
+        base = torch.empty((B, 2, self.M, self.N), dtype=torch.float32, device=self.dev) if with_base else None
         scale = torch.empty(B, dtype=torch.float32, device=self.dev)
         H, W = self._hw(lv)
         out = torch.empty((B, H, W, OUT_C), dtype=torch.bfloat16, device=self.dev)
         for b0 in range(0, B, chunk):
             b1 = min(B, b0 + chunk)
-            feat, _, _ = self.features(g[b0:b1], r[b0:b1], vin[b0:b1], base[b0:b1], scale[b0:b1])
+            feat, _, _ = self.features(g[b0:b1], r[b0:b1], vin[b0:b1], base[b0:b1] if with_base else None,
+                                       scale[b0:b1], want_base=with_base)
             if self.fused:
                 self.forward_fused(feat, out=out[b0:b1])
             else:
                 out[b0:b1] = self.forward(feat)
         return base, out, scale, lv
 
-    def warm_kwargs(self, g, r, vin):
+    def warm_kwargs(self, g, r, vin, base_in_kernel: bool = True):
         """solve_batch keyword arguments carrying this network's warm start:
         the parts (combined while loading) when the output is coarser than the
-        crossbar, else the combined field."""
+        crossbar, else the combined field.  With pooled features and
+        base_in_kernel (the default) the solve forms the base field itself
+        (two undamped IR2S iterations from zero, bit for bit the features
+        kernel's base for grounded bitlines), so it is neither written nor
+        read; base_in_kernel=False passes it from the features kernel."""
         if self.out_level >= 1:
-            base, out, scale, lv = self.predict_parts(g, r, vin)
-            return {"warm": base, "warm_coarse": out, "warm_scale": scale, "warm_level": lv}
+            base, out, scale, lv = self.predict_parts(g, r, vin, with_base=not base_in_kernel)
+            kw = {"warm_coarse": out, "warm_scale": scale, "warm_level": lv}
+            if base is not None:
+                kw["warm"] = base
+            return kw
         return {"warm": self.predict(g, r, vin)}
 
     # ------------------------------------------------------------------ backward

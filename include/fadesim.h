@@ -101,13 +101,16 @@ typedef struct fadesim_batch_desc {
                              * in (0, 1]; 0 -> 1.  With the coupling's spectral radius rho
                              * (~0.8 r gbar rows cols) the contraction is rho/(2+rho) at
                              * w = 2/(2+rho), against rho undamped */
-    /* Warm start given as d_warm + d_warm_scale[b] x bilinear(network output):
+    /* Warm start given as base + d_warm_scale[b] x bilinear(network output):
      * when d_warm_coarse is not NULL the initial drops are formed at load,
      * with the arithmetic of fadesim_nn_combine_up (channel 0 -> wordline drops,
-     * 1 -> bitline drops), so that kernel's field is never written.  Needs
-     * d_warm; warm_level 1..6 with the coarse grid small enough for the
-     * kernel's LDS staging: 2 (n >> level)(m >> level) <= 12800 floats, so every
-     * level >= 1 fits at 128x128. */
+     * 1 -> bitline drops), so that kernel's field is never written.  The base
+     * is d_warm or, when d_warm is NULL, formed in the kernel: two undamped
+     * IR2S iterations from zero drops (the base fadesim_nn_features_pooled
+     * writes, bit for bit, when the bitlines are grounded).  warm_level 1..6
+     * with the coarse grid small enough for the kernel's LDS staging:
+     * 2 (n >> level)(m >> level) <= 12800 floats, so every level >= 1 fits at
+     * 128x128. */
     const void *d_warm_coarse;  /* [B][n >> warm_level][m >> warm_level][warm_coarse_ch] BF16, or NULL */
     const float *d_warm_scale;  /* [B] */
     int warm_coarse_ch;         /* even, >= 2 */

@@ -30,8 +30,10 @@ int fadesim_nn_features(int B, int M, int N, const float *d_g, const double *d_r
                         void *d_feat, float *d_base, float *d_scale, void *stream);
 
 /* The same features averaged over 2^level x 2^level cells (level 2,
- * 128x128 crossbars): feat [B][M>>2][N>>2][8] bf16, base [B][2][M][N] FP32,
- * scale [B] -- the input of a network whose first layer works at level 2. */
+ * 128x128 crossbars): feat [B][M>>2][N>>2][8] bf16, base [B][2][M][N] FP32
+ * (or NULL: not written -- fadesim_solve_batch forms it itself when given
+ * d_warm_coarse without d_warm), scale [B] -- the input of a network whose
+ * first layer works at level 2. */
 int fadesim_nn_features_pooled(int B, int M, int N, int level, const float *d_g, const double *d_r,
                                const double *d_vin, void *d_feat, float *d_base, float *d_scale, void *stream);
 

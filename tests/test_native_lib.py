@@ -130,15 +130,16 @@ def test_c_abi_rejects_bad_descriptors_without_a_gpu():
     assert "Kp" in err(L.fadesim_nn_coarse_forward(1, ctypes.c_void_p(8), w, kp, ctypes.c_void_p(8), None))
     p = [ctypes.c_void_p(8)] * 6
     assert "128x128" in err(L.fadesim_nn_features_pooled(1, 64, 64, 2, *p, None))
-    # warm start combined at load: needs the base field, the scale, an even
-    # channel count and a level that divides the shape
+    # warm start combined at load: needs the scale, an even channel count and
+    # a level that divides the shape (the base field is optional: the solve
+    # forms it when d_warm is NULL)
     d = nat.BatchDesc()
     d.batch, d.rows, d.cols = 4, 128, 128
     d.d_g = d.d_r_wire = d.d_vin = d.d_iout = 8
     d.tol = 1e-6
-    d.d_warm_coarse, d.d_warm_scale, d.warm_coarse_ch, d.warm_level = 8, 8, 16, 2
-    assert "d_warm_coarse" in err(L.fadesim_solve_batch(ctypes.byref(d), None))  # no d_warm
-    d.d_warm = 8
+    d.d_warm_coarse, d.d_warm_scale, d.warm_coarse_ch, d.warm_level = 8, None, 16, 2
+    assert "d_warm_scale" in err(L.fadesim_solve_batch(ctypes.byref(d), None))
+    d.d_warm_scale = 8
     d.warm_coarse_ch = 3
     assert "d_warm_coarse" in err(L.fadesim_solve_batch(ctypes.byref(d), None))
     d.warm_coarse_ch, d.warm_level = 16, 7

@@ -1,4 +1,8 @@
-"""The learned warm start formed inside the solve (fadesim_nn_solve_batch):
+"""The learned warm start's base field formed by the solve itself
+(solve_batch(warm_coarse=...) without warm): bit for bit the solve from the
+features kernel's base field, certified or swept.
+
+The learned warm start formed inside the solve (fadesim_nn_solve_batch):
 bit for bit the outputs of the three-kernel path it fuses -- pooled features
 and base field, the network, the combine while the solve loads -- with every
 crossbar certified at its first test, with every crossbar swept by the
@@ -87,3 +91,25 @@ def test_in_kernel_start_rejects_a_bad_network(cuda):
     _assert_same(fused, three)
     assert int(fused.rejected.sum()) >= 16
     assert int((fused.status != 0).sum()) == 0
+
+
+@pytest.mark.parametrize("case", ["headline", "swept"])
+def test_base_formed_by_the_solve_equals_base_from_features(cuda, case):
+    """warm_kwargs(base_in_kernel=True) -- the solve forms the base field, two
+    undamped IR2S iterations from zero -- against the base written by the
+    features kernel: every output equal, in the guess kernel (headline call)
+    and in the sweeping kernel (no IR2S iteration, no refinement, 1e-5)."""
+    from fadesim_amd import warmstart
+    from fadesim_amd.solver import solve_batch
+    net = _net(cuda)
+    g, r, vin = warmstart.make_instances(96, seed=30_300_000, device=cuda)
+    kw = dict(tol=1e-6, ir2s_iters=1, corr_iters=4) if case == "headline" else dict(tol=1e-5, ir2s_iters=0,
+                                                                                      refine=False)
+    a = solve_batch(g, r, vin, **net.warm_kwargs(g, r, vin, base_in_kernel=True), test_first=True, want_field=True,
+                    **kw)
+    b = solve_batch(g, r, vin, **net.warm_kwargs(g, r, vin, base_in_kernel=False), test_first=True, want_field=True,
+                    **kw)
+    _assert_same(a, b)
+    if case == "swept":
+        assert int((a.sweeps > 0).sum()) >= 4
+    assert int((a.status != 0).sum()) == 0

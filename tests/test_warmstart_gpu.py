@@ -345,8 +345,8 @@ def test_warm_coarse_arguments_checked(cuda):
     base = torch.zeros((4, 2, 128, 128), dtype=torch.float32, device=cuda)
     out = torch.zeros((4, 32, 32, 16), dtype=torch.bfloat16, device=cuda)
     s = torch.ones(4, dtype=torch.float32, device=cuda)
-    with pytest.raises(ValueError):
-        solve_batch(g, r, vin, warm=None, warm_coarse=out, warm_scale=s, warm_level=2)
+    with pytest.raises(ValueError):  # no scale
+        solve_batch(g, r, vin, warm=base, warm_coarse=out, warm_scale=None, warm_level=2)
     with pytest.raises(ValueError):
         solve_batch(g, r, vin, warm=base, warm_coarse=out, warm_scale=s, warm_level=1)
     with pytest.raises(RuntimeError):
