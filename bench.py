@@ -94,6 +94,24 @@ SWEEP_FLOPS_PER_CELL = 16
 # (128 bytes per clock per CU, 256 CUs, 2.4 GHz).
 VALU_FP32_PEAK_TFLOPS = 157.3
 LDS_PEAK_GBS = 128 * 256 * 2.4
+# VALU issue rate: each SIMD issues one wave64 VALU instruction every 4
+# cycles (16 lanes per cycle), 4 SIMDs per CU, 256 CUs, 2.4 GHz -- in lane
+# operations per second (FP64 and transcendental instructions take longer,
+# so the reachable rate of FP64-heavy code is lower)
+VALU_LANE_OPS_PEAK = 64 * 256 * 2.4e9
+
+
+def valu_roofline(ctr, kernel_ms):
+    """Achieved VALU lane operations per second of a kernel (SQ_INSTS_VALU of
+    the counter runs x 64 lanes, over the kernel time measured here) against
+    the issue peak, or None without counters."""
+    if "sq_insts_valu" not in ctr:
+        return None
+    ach = ctr["sq_insts_valu"] * 64 / (kernel_ms * 1e-3)
+    return {"bound": "valu", "achieved": ach / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
+            "unit": "T lane-ops/s", "frac": ach / VALU_LANE_OPS_PEAK,
+            "model": "SQ_INSTS_VALU x 64 lanes per launch (rocprofv3 counters in profiles/) / kernel time; "
+                     "peak: one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs x 256 CUs x 2.4 GHz"}
 
 
 # Time of the restatement (oracle_nodegs, one core) over the reference V2 `sim`
@@ -333,7 +351,7 @@ def main():
         meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                       "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
                       "guess_kernel_ms": guess_ms, "kernel_GBps_algorithmic": ach}
-    head = "nn" if "nn" in meas else ("ir2s" if "ir2s" in meas else "plain")
+    head = next(k for k in ("nn", "ir2s", "plain", "nn_fused", "nn_unet16") if k in meas)
     hm = meas[head]
     out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU",
            "value": hm["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -343,7 +361,9 @@ def main():
                                   "r ~ U[1,5] ohm, vin in {0,0.2} V, to 1e-6 rel I_out err",
                       "batch": B, "rows": M, "cols": N, "tol": args.tol,
                       "initial_guess": {"nn": "nn-warm-start (+1 IR2S iteration)", "ir2s": f"{K} IR2S iterations",
-                                        "plain": "zero-drop"}[head]},
+                                        "plain": "zero-drop",
+                                        "nn_fused": "nn-warm-start formed in the solve (+1 IR2S iteration)",
+                                        "nn_unet16": "unet16 nn-warm-start (+1 IR2S iteration)"}[head]},
            "plain": meas.get("plain"), "ir2s_warm": meas.get("ir2s"),
            "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None,
            "warm_start_in_kernel": meas.get("nn_fused"),
@@ -365,7 +385,7 @@ def main():
     # solves end at their first test (every crossbar of the nn and ir2s
     # variants), timed alone between HIP events on its stream
     gk = hm["guess_kernel_ms"] is not None and hm["mean_sweeps"] == 0
-    gname = "guess_kernel"
+    gname = "guess_net_kernel" if head == "nn_fused" else "guess_kernel"
     kname = (gname + "<Geo<4,8,32,16>>") if gk else "rbsor_kernel<Geo<4,8,32,16>>"
     kms_dom = hm["guess_kernel_ms"] if gk else hm["kernel_ms"]
     pre_passes, warm_rd, coarse_rd = spec[head]
@@ -386,6 +406,9 @@ def main():
                                if "lds_bytes_per_launch" in ctr else None),
                        "valu_busy": ctr.get("valu_busy"), "wait_frac": ctr.get("wait_frac"),
                        "counters": "profiles/solve_128x128_b4096_counters.md" if ctr else None}
+    # the resource that binds the dominant kernel: VALU issue (the wire-solve
+    # iterations in FP32, the test in FP64)
+    out["roofline_valu"] = valu_roofline(ctr, kms_dom)
     if "plain" in meas:
         pm = meas["plain"]
         tflops = SWEEP_FLOPS_PER_CELL * M * N * pm["mean_sweeps"] * B / (pm["kernel_ms"] * 1e-3) / 1e12
@@ -397,6 +420,7 @@ def main():
                                      "flops_model": f"{SWEEP_FLOPS_PER_CELL} FP32 flops per cell per sweep",
                                      "kernel_ms": pm["kernel_ms"], "traffic": pc.get("hbm_bytes_per_launch"),
                                      "valu_busy": pc.get("valu_busy"),
+                                     "valu_issue": valu_roofline(pc, pm["kernel_ms"]),
                                      "hbm_GBps_measured": (pc["hbm_bytes_per_launch"] / (pm["kernel_ms"] * 1e-3) / 1e9
                                                            if "hbm_bytes_per_launch" in pc else None)}
     if not args.no_cpu:
