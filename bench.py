@@ -13,9 +13,10 @@ zero drops (plain SOR), K wire-solve (IR2S) iterations inside the solve
 kernel (K from the autotuned table), and, when trained weights are present
 (fadesim_amd/weights), the learned warm start with the network's inference
 inside the timed step plus one IR2S iteration on its field (`value`, the
-metric's NN-warm-start figure; "nn_fused" the same start formed inside
-the solve's guess kernel, bit for bit the same outputs; "nn_unet16" the same
-with the 4-level full-resolution network); mean sweeps of each; the roofline of the
+metric's NN-warm-start figure, the network running inside the solve's
+guess kernel; "nn_3k" the same start from separate features and network
+kernels, bit for bit the same outputs; "nn_unet16" the same with the 4-level
+full-resolution network); mean sweeps of each; the roofline of the
 step's dominant kernel (the guess-first kernel when every crossbar is
 certified at its first test, else the sweeping kernel) from HIP events on
 its launch stream, with the HBM / LDS / VALU figures of the rocprofv3
@@ -244,8 +245,8 @@ def ir2s_iters_128(dev) -> int:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
@@ -253,7 +254,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--mlp-images", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=256)
-    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn,nn_fused,nn_unet16: run only these solve variants")
+    ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn,nn_3k,nn_unet16: run only these solve variants")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -296,19 +297,20 @@ def main():
                                                                     corr_iters=K, timing=timing)),
     }
     if net is not None:
-        # the network's field, one IR2S iteration on it, then the same test /
-        # SOR: pooled features, network, and the network output combined
-        # while the solve loads it with the base field, which the solve forms
-        # itself (two wire-solve passes from zero)
-        variants["nn"] = (1, 3, False, True, lambda timing=False: solve_batch(
+        # the network's start formed inside the solve's guess kernel (pooled
+        # features and the base field = two wire-solve passes from zero, the
+        # network with its activations in LDS, the combine), one IR2S
+        # iteration on it, then the same test / SOR
+        fused = net.solves_in_kernel
+        if fused:
+            variants["nn"] = (1, 3, False, False, lambda timing=False: solve_batch(
+                tg, tr, tv, net=net, tol=args.tol, ir2s_iters=1, corr_iters=K, timing=timing))
+        # the same start from separate kernels: pooled features, network, and
+        # the network output combined while the solve loads it with the base
+        # field, which the solve forms itself (bit for bit the same outputs)
+        variants["nn_3k" if fused else "nn"] = (1, 3, False, True, lambda timing=False: solve_batch(
             tg, tr, tv, **net.warm_kwargs(tg, tr, tv), tol=args.tol, ir2s_iters=1, corr_iters=K, test_first=True,
             timing=timing))
-    if net is not None and net.solves_in_kernel:
-        # the same start formed inside the solve's guess kernel (its features =
-        # two wire-solve passes from zero, the network, the combine): the same
-        # outputs bit for bit
-        variants["nn_fused"] = (1, 3, False, False, lambda timing=False: solve_batch(
-            tg, tr, tv, net=net, tol=args.tol, ir2s_iters=1, corr_iters=K, timing=timing))
     # the 4-level full-resolution encoder-decoder (~340k parameters), for comparison
     net16 = None
     if os.path.exists(os.path.join(warmstart.WEIGHTS_DIR, "warmstart_128_unet16.bin")):
@@ -351,7 +353,7 @@ def main():
         meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
                       "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
                       "guess_kernel_ms": guess_ms, "kernel_GBps_algorithmic": ach}
-    head = next(k for k in ("nn", "ir2s", "plain", "nn_fused", "nn_unet16") if k in meas)
+    head = next(k for k in ("nn", "ir2s", "plain", "nn_3k", "nn_unet16") if k in meas)
     hm = meas[head]
     out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU",
            "value": hm["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -362,11 +364,11 @@ def main():
                       "batch": B, "rows": M, "cols": N, "tol": args.tol,
                       "initial_guess": {"nn": "nn-warm-start (+1 IR2S iteration)", "ir2s": f"{K} IR2S iterations",
                                         "plain": "zero-drop",
-                                        "nn_fused": "nn-warm-start formed in the solve (+1 IR2S iteration)",
+                                        "nn_3k": "nn-warm-start from separate kernels (+1 IR2S iteration)",
                                         "nn_unet16": "unet16 nn-warm-start (+1 IR2S iteration)"}[head]},
            "plain": meas.get("plain"), "ir2s_warm": meas.get("ir2s"),
            "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None,
-           "warm_start_in_kernel": meas.get("nn_fused"),
+           "warm_start_three_kernels": meas.get("nn_3k"),
            "warm_start_unet16": dict(meas["nn_unet16"], network=net16.describe()) if "nn_unet16" in meas else None}
     if "plain" in meas and "nn" in meas:
         ps, ns = meas["plain"]["mean_sweeps"], meas["nn"]["mean_sweeps"]
@@ -385,7 +387,7 @@ def main():
     # solves end at their first test (every crossbar of the nn and ir2s
     # variants), timed alone between HIP events on its stream
     gk = hm["guess_kernel_ms"] is not None and hm["mean_sweeps"] == 0
-    gname = "guess_net_kernel" if head == "nn_fused" else "guess_kernel"
+    gname = "guess_net_kernel" if head == "nn" and net is not None and net.solves_in_kernel else "guess_kernel"
     kname = (gname + "<Geo<4,8,32,16>>") if gk else "rbsor_kernel<Geo<4,8,32,16>>"
     kms_dom = hm["guess_kernel_ms"] if gk else hm["kernel_ms"]
     pre_passes, warm_rd, coarse_rd = spec[head]

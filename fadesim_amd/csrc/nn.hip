@@ -476,18 +476,32 @@ using nn::net_kpad;
 template <int C1>
 __global__ void __launch_bounds__(kNetThreads) coarse_net_kernel(CoarseNetArgs A)
 {
-    using Lds = nn::NetLds<C1>;
-    __shared__ __align__(16) __bf16 bufA[Lds::kA];
-    __shared__ __align__(16) __bf16 bufS[Lds::kS];
-    __shared__ __align__(16) __bf16 bufX[Lds::kX];
     const long b = blockIdx.x;
     const int t = threadIdx.x;
-    // input features (16 bytes per copy)
-    for (int e = t; e < 32 * 32; e += kNetThreads)
-        *reinterpret_cast<bf16x8 *>(bufX + 8 * e) = *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
-    __syncthreads();
     const __bf16 *const w[7] = {A.w[0], A.w[1], A.w[2], A.w[3], A.w[4], A.w[5], A.w[6]};
-    nn::coarse_net_body<C1, kNetThreads / 64>(bufX, bufA, bufS, w, A.out + b * 1024 * nn::kOutC);
+    if constexpr (C1 == 16) {
+        // coarse16f on the zero-bordered layout (nn::coarse16_body_pad)
+        using Lds = nn::Net16PadLds;
+        __shared__ __align__(16) __bf16 bufA[Lds::kA];
+        __shared__ __align__(16) __bf16 bufS[Lds::kS];
+        __shared__ __align__(16) __bf16 bufX[Lds::kX];
+        for (int e = t; e < 32 * 32; e += kNetThreads)
+            *reinterpret_cast<bf16x8 *>(bufX + nn::Padded<32, 32, 8>::at(e / 32, e % 32)) =
+                *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
+        nn::zero_border<kNetThreads, 32, 32, 8>(bufX);
+        __syncthreads();
+        nn::coarse16_body_pad<kNetThreads / 64, nn::kOutPlain>(bufX, bufA, bufS, w, A.out + b * 1024 * nn::kOutC);
+    } else {
+        using Lds = nn::NetLds<C1>;
+        __shared__ __align__(16) __bf16 bufA[Lds::kA];
+        __shared__ __align__(16) __bf16 bufS[Lds::kS];
+        __shared__ __align__(16) __bf16 bufX[Lds::kX];
+        // input features (16 bytes per copy)
+        for (int e = t; e < 32 * 32; e += kNetThreads)
+            *reinterpret_cast<bf16x8 *>(bufX + 8 * e) = *reinterpret_cast<const bf16x8 *>(A.x + (b * 1024 + e) * 8);
+        __syncthreads();
+        nn::coarse_net_body<C1, kNetThreads / 64>(bufX, bufA, bufS, w, A.out + b * 1024 * nn::kOutC);
+    }
 }
 
 // wT[ci][tap'*Cout + co] = w[co][(8-tap')*Cin + ci]; bias and pad columns 0.

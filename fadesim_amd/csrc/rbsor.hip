@@ -195,8 +195,8 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
 // guess_kernel with the learned warm start formed in-kernel (128x128, the
 // coarse16f network): per crossbar, the pooled features and the base field
 // (two undamped wire-solve iterations from zero, nn::pooled_features), the
-// network on the workgroup's 8 waves (nn::coarse_net_body, every activation
-// in LDS), and base + S2 x bilinear(network output) -- the arithmetic of the
+// network on the workgroup's 8 waves (nn::coarse16_body_pad, every
+// activation in LDS), and base + S2 x bilinear(network output) -- the arithmetic of the
 // features kernel, the network kernel and the combine at load, term by term,
 // so the drops equal those of the three-kernel path bit for bit.  The
 // network's buffers overlay `big` (idle between the set-up and the test).  A
@@ -209,18 +209,19 @@ constexpr int kNetWidth = 16;
 __global__ void __launch_bounds__(NetGeo::NT) guess_net_kernel(RbArgs A)
 {
     using G = NetGeo;
-    using Lds = nn::NetLds<kNetWidth>;
+    using Lds = nn::Net16PadLds;
     constexpr int M = G::M, N = G::N, TR = G::TR, TC = G::TC, TX = G::TX, NT = G::NT;
     constexpr int kHaloB = G::HALO * (int)sizeof(float), kCbufB = 2 * G::TY * N * (int)sizeof(double);
     constexpr int kFldB = 2 * M * N * (int)sizeof(float);
     constexpr int kBigB = kHaloB + kCbufB > kFldB ? kHaloB + kCbufB : kFldB;
-    // network buffers: bufA, bufS, bufX, then the output (32x32x16); the
-    // output's channel pairs are staged in the halo region (from byte kCbufB,
-    // over bufX, which is dead by then)
-    constexpr int kNetB = (Lds::kA + Lds::kS + Lds::kX + 32 * 32 * nn::kOutC) * (int)sizeof(__bf16);
-    static_assert(kNetB <= kBigB, "network buffers fit the solver's LDS");
-    static_assert((Lds::kA + Lds::kS) * (int)sizeof(__bf16) == kCbufB, "features land beyond the column buffer");
-    static_assert(2 * 32 * 32 * (int)sizeof(float) <= Lds::kX * (int)sizeof(__bf16), "stage within bufX");
+    // network buffers (zero-bordered layout): bufA, bufS, bufX; the last
+    // layer writes its channel pairs to the stage of the combine, the halo
+    // region (from byte kCbufB), which lies within bufS, idle by then
+    constexpr int kA = Lds::kA * (int)sizeof(__bf16), kS = Lds::kS * (int)sizeof(__bf16);
+    static_assert(kA + kS + Lds::kX * (int)sizeof(__bf16) <= kBigB, "network buffers fit the solver's LDS");
+    static_assert(kA + kS >= kCbufB, "the features land beyond the column buffer of the wire iterations");
+    static_assert(kCbufB >= kA && kCbufB + 2 * 32 * 32 * (int)sizeof(float) <= kA + kS, "stage within bufS");
+    static_assert(kA % 16 == 0 && kS % 16 == 0, "16-byte aligned buffers");
     __shared__ __align__(16) unsigned char big[kBigB];
     __shared__ double S1[N];
     __shared__ double Ifloor[N];
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(NetGeo::NT) guess_net_kernel(RbArgs A)
     float *halo = reinterpret_cast<float *>(big + kCbufB);
     float *fld = reinterpret_cast<float *>(big);
     __bf16 *bufA = reinterpret_cast<__bf16 *>(big);
-    __bf16 *bufS = bufA + Lds::kA, *bufX = bufS + Lds::kS, *bufO = bufX + Lds::kX;
+    __bf16 *bufS = bufA + Lds::kA, *bufX = bufS + Lds::kS;
     rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     const __bf16 *const w[7] = {A.netw[0], A.netw[1], A.netw[2], A.netw[3], A.netw[4], A.netw[5], A.netw[6]};
@@ -264,13 +265,10 @@ __global__ void __launch_bounds__(NetGeo::NT) guess_net_kernel(RbArgs A)
         I.cch = I.clevel = 0;
         // the combined start: features and base, network, base + S2 x bilinear(out)
         auto net_start = [&](int i0, int j0, float (&p)[TR][TC], float (&q)[TR][TC]) {
-            const float S2 = nn::pooled_features(sh, I.g, I.rwire, I.vin, bufX, red, p, q);
-            nn::coarse_net_body<kNetWidth, NT / 64>(bufX, bufA, bufS, w, bufO);
+            const float S2 = nn::pooled_features<true>(sh, I.g, I.rwire, I.vin, bufX, red, p, q);
+            nn::zero_border<NT, 32, 32, nn::kFeatC>(bufX);
             __syncthreads();
-            const unsigned *src = reinterpret_cast<const unsigned *>(bufO);
-            float2 *st2 = reinterpret_cast<float2 *>(halo);
-            for (int k = threadIdx.x; k < 32 * 32; k += NT)
-                rb::stage_coarse_pair(st2[k], src[(size_t)k * (nn::kOutC >> 1)]);
+            nn::coarse16_body_pad<NT / 64, nn::kOutStage>(bufX, bufA, bufS, w, halo);
             __syncthreads();
             asm volatile("" : "+v"(i0), "+v"(j0));
             rb::add_coarse_staged<G>(halo, 2, S2, M, 0, i0, j0, p, q);

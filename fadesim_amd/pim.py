@@ -411,15 +411,20 @@ def _solve_converged(bits, G, inputV, P):
         vin = bits[ii[sl], bb[sl], xx[sl]].to(torch.float64) * inputV
         gi = gidx[sl]
         rr = r.expand(n)
+        # a network that solves in-kernel forms its start inside the solve
+        in_kernel = net is not None and net.solves_in_kernel and (R, C) == (net.M, net.N)
         if P.nonlinear_cell_enable:
-            warm_fn = (lambda Gz, rw, v: net.predict(Gz, rw, v)) if net is not None else None
+            warm_fn = (lambda Gz, rw, v: net.predict(Gz, rw, v)) if net is not None and not in_kernel else None
             res = solve_batch_nonlinear(slots, rr, vin, params=P.nonlinear_params, gidx=gi, tol=P.tol,
-                                        warm_fn=warm_fn, **ir2s)
+                                        warm_fn=warm_fn, net=net if in_kernel else None, **ir2s)
             stats["sweeps"] += int(res.total_sweeps.sum())
             stats["picard_steps"] = max(stats["picard_steps"], res.outer)
         else:
-            warm = net.predict(slots.index_select(0, gi.long()), rr, vin) if net is not None else None
-            res = solve_batch(slots, rr, vin, warm=warm, gidx=gi, tol=P.tol, **ir2s)
+            if in_kernel:
+                res = solve_batch(slots, rr, vin, gidx=gi, tol=P.tol, net=net, **ir2s)
+            else:
+                warm = net.predict(slots.index_select(0, gi.long()), rr, vin) if net is not None else None
+                res = solve_batch(slots, rr, vin, warm=warm, gidx=gi, tol=P.tol, **ir2s)
             stats["sweeps"] += int(res.sweeps.sum())
         stats["calls"] += 1
         if int((res.status != 0).sum()) != 0:

@@ -243,7 +243,7 @@ class NonlinearResult(SolveResult):
 
 def solve_batch_nonlinear(g_lin: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
                           warm: Optional[torch.Tensor] = None, *, params, gidx: Optional[torch.Tensor] = None,
-                          tol: float = 1e-6, max_outer: int = 40, warm_fn=None, **kw) -> NonlinearResult:
+                          tol: float = 1e-6, max_outer: int = 40, warm_fn=None, net=None, **kw) -> NonlinearResult:
     """Crossbars with sinh-nonlinear cells: Picard iteration G <- G(V) around
     the linear batched solve, each solve warm-started from the previous
     field, until the bitline currents of consecutive steps agree to tol/4
@@ -253,7 +253,9 @@ def solve_batch_nonlinear(g_lin: torch.Tensor, r_wire, vin: torch.Tensor, vout: 
     the reference's lagged-conductance sweeps (split_arr_matmul.cu:241-262)
     converge to.  ``warm_fn(G, r, vin)``, if given, supplies the first solve's
     initial field from the conductances of the zero-drop state (e.g. the
-    learned warm start); ``warm`` supplies it directly.
+    learned warm start); ``warm`` supplies it directly; ``net`` (a network
+    that solves in-kernel, grounded bitlines) forms the first solve's start
+    inside that solve (solve_batch(..., net=net)).
     """
     S, M, N = g_lin.shape
     B = S if gidx is None else int(gidx.numel())
@@ -271,7 +273,10 @@ def solve_batch_nonlinear(g_lin: torch.Tensor, r_wire, vin: torch.Tensor, vout: 
     res = None
     k = 0
     for k in range(1, max_outer + 1):
-        res = solve_batch(G, r_wire, vin, vout, field, tol=inner_tol, want_field=True, **kw)
+        if k == 1 and field is None and net is not None:
+            res = solve_batch(G, r_wire, vin, vout, tol=inner_tol, want_field=True, net=net, **kw)
+        else:
+            res = solve_batch(G, r_wire, vin, vout, field, tol=inner_tol, want_field=True, **kw)
         total += res.sweeps.to(torch.int64)
         field = res.field
         if int((res.status != 0).sum()) != 0:
