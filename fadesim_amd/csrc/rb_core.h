@@ -732,18 +732,21 @@ __device__ __forceinline__ void row_green_row_f(const float (&v)[G::TC], int j0,
                                                 float (&out)[G::TC])
 {
     constexpr int TC = G::TC;
+    // the column indices j + 1 as floats: one conversion, then exact
+    // additions of small integers
+    const float j1 = (float)(j0 + 1);
     float L0 = 0.f, L1 = 0.f;
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
         L0 += v[jj];
-        L1 = fmaf((float)(j0 + jj + 1), v[jj], L1);
+        L1 = fmaf(j1 + (float)jj, v[jj], L1);
     }
     float P0, P1, R, R1;
     row_scan_f<G::TX>(L0, tx, P0, R);
     row_scan_f<G::TX>(L1, tx, P1, R1);
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj) {
-        const float jp1 = (float)(j0 + jj + 1);
+        const float jp1 = j1 + (float)jj;
         out[jj] = scale * fmaf(jp1, R - P0, P1);
         P0 += v[jj];
         P1 = fmaf(jp1, v[jj], P1);

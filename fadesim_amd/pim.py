@@ -340,9 +340,6 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         elif P.ir_mode == "converged":
             if P.useOUSize:
                 raise NotImplementedError("OU mode runs the hook's OU iterations (ir_mode 'reference' or 'ir2s')")
-            if ref:
-                raise NotImplementedError("reference columns make R x (C+2) tiles; the converged solver takes "
-                                          "square 64/128 tiles (ir_mode 'reference' or 'ir2s')")
             I = _solve_converged(bits, G, inputV, P)
         elif P.ir_mode in ("reference", "ir2s"):
             I = _solve_reference(bits, G, inputV, P, float_type)
@@ -366,13 +363,14 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
         return output
 
 
-def tile_instances(bits: Tensor, nslice: int, numY: int):
+def tile_instances(bits: Tensor, nslice: int, numY: int, ns: int = 2):
     """Instance list of a layer's crossbar solves: every (bit, image, row tile)
-    with a non-zero input vector x every (slice, sign, column tile).  Returns
-    index tensors (i, b, x, j, s, y), each [K]."""
+    with a non-zero input vector x every (slice, array, column tile), ns
+    arrays per slice (2: positive and negative, 1: a reference-column array).
+    Returns index tensors (i, b, x, j, s, y), each [K]."""
     dev = bits.device
     ii, bb, xx = torch.nonzero(bits.any(-1), as_tuple=True)
-    J, S, Y = torch.meshgrid(torch.arange(nslice, device=dev), torch.arange(2, device=dev),
+    J, S, Y = torch.meshgrid(torch.arange(nslice, device=dev), torch.arange(ns, device=dev),
                              torch.arange(numY, device=dev), indexing="ij")
     J, S, Y = J.reshape(-1), S.reshape(-1), Y.reshape(-1)
     n_in, n_t = ii.numel(), J.numel()
@@ -381,21 +379,24 @@ def tile_instances(bits: Tensor, nslice: int, numY: int):
 
 
 def _solve_converged(bits, G, inputV, P):
-    """All tile solves of the layer through the batched solver."""
-    from .solver import solve_batch, solve_batch_nonlinear
+    """All tile solves of the layer through the batched solver; tiles of a
+    shape no kernel takes (reference-column arrays: R x (C+2)) as the
+    smallest larger crossbar with the added cells off (solver.solve_padded,
+    linear cells)."""
+    from .solver import SUPPORTED_SHAPES, solve_batch, solve_batch_nonlinear, solve_padded
     nbit, bsize, numX, R = bits.shape
-    nv, nslice, _, _, numY, _, C = G.shape
+    nv, nslice, ns, _, numY, _, C = G.shape
     if not G.is_cuda:
         raise RuntimeError("the converged crossbar model runs on the GPU (G is on the CPU)")
     dev = G.device
-    # slot (((v*nslice + j)*2 + s)*numX + x)*numY + y, v the input bit when
+    # slot (((v*nslice + j)*ns + s)*numX + x)*numY + y, v the input bit when
     # every bit has its own conductances (variation), else 0
     slots = G.reshape(-1, R, C).to(torch.float32).contiguous()
-    I = torch.zeros((nbit, nslice, 2, bsize, numX, numY, C), dtype=torch.float64, device=dev)
-    ii, bb, xx, jj, ss, yy = tile_instances(bits, nslice, numY)
+    I = torch.zeros((nbit, nslice, ns, bsize, numX, numY, C), dtype=torch.float64, device=dev)
+    ii, bb, xx, jj, ss, yy = tile_instances(bits, nslice, numY, ns)
     total = ii.numel()
     vv = ii if nv > 1 else torch.zeros_like(ii)
-    gidx = ((((vv * nslice + jj) * 2 + ss) * numX + xx) * numY + yy).to(torch.int32)
+    gidx = ((((vv * nslice + jj) * ns + ss) * numX + xx) * numY + yy).to(torch.int32)
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
     net = P.warm_start
     k = int(P.ir2s_iters)
@@ -413,7 +414,13 @@ def _solve_converged(bits, G, inputV, P):
         rr = r.expand(n)
         # a network that solves in-kernel forms its start inside the solve
         in_kernel = net is not None and net.solves_in_kernel and (R, C) == (net.M, net.N)
-        if P.nonlinear_cell_enable:
+        if (R, C) not in SUPPORTED_SHAPES:
+            if P.nonlinear_cell_enable:
+                raise NotImplementedError(f"converged sinh cells on {R} x {C} tiles (the shapes of the batched "
+                                          f"solver: {sorted(SUPPORTED_SHAPES)})")
+            res = solve_padded(slots, rr, vin, gidx=gi, tol=P.tol, **ir2s)
+            stats["sweeps"] += int(res.sweeps.sum())
+        elif P.nonlinear_cell_enable:
             warm_fn = (lambda Gz, rw, v: net.predict(Gz, rw, v)) if net is not None and not in_kernel else None
             res = solve_batch_nonlinear(slots, rr, vin, params=P.nonlinear_params, gidx=gi, tol=P.tol,
                                         warm_fn=warm_fn, net=net if in_kernel else None, **ir2s)

@@ -195,6 +195,71 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     return SolveResult(iout, sweeps, status, est, field, rejected)
 
 
+def embedding_shape(rows: int, cols: int):
+    """The smallest crossbar shape a kernel takes that holds rows x cols
+    cells: ((Rq, Cq), False) for a shape of the batched solver, ((n, n), True)
+    for a square of the mixed-size solver (256 or 512)."""
+    fits = [(r, c) for (r, c) in SUPPORTED_SHAPES if r >= rows and c >= cols]
+    if fits:
+        return min(fits, key=lambda s: (s[0] * s[1], s)), False
+    for n in (256, 512):
+        if rows <= n and cols <= n:
+            return (n, n), True
+    raise ValueError(f"{rows} x {cols} crossbars are larger than any solver takes (512 x 512)")
+
+
+def solve_padded(g: torch.Tensor, r_wire, vin: torch.Tensor, *, gidx: Optional[torch.Tensor] = None,
+                 tol: float = 1e-6, chunk: int = 2048, **kw) -> SolveResult:
+    """Crossbars of R x C cells for which no kernel is compiled (reference-
+    column tiles R x (C+2), odd sizes), solved as the smallest larger
+    crossbar a kernel takes (embedding_shape): the R rows at the bottom of
+    its Rq rows, the C columns at the left of its Cq columns, every added
+    cell off (g = 0).  The solution on the original nodes is exact: no cell
+    current enters the added rows, so the bitline segments above the first
+    row carry none (the open top end), and none enters the added columns,
+    so the wordline segments right of the last column carry none (the open
+    right end); the added bitlines carry nothing into the original ones.
+    Linear cells, grounded bitlines.  g [S, R, C] (slots with ``gidx`` [B],
+    else S = B), vin [B, R]; returns iout [B, C] with sweeps and status.
+    Batched shapes go through solve_batch (``kw``: its IR2S and test
+    controls), the 256/512 squares through a mixed-size plan (its tuned
+    schedule), ``chunk`` crossbars at a time."""
+    from .ragged import RaggedPlan
+    S, R, C = g.shape
+    B = S if gidx is None else int(gidx.numel())
+    dev = g.device
+    (Rq, Cq), ragged = embedding_shape(R, C)
+    if (Rq, Cq) == (R, C):
+        return solve_batch(g, r_wire, vin, gidx=gidx, tol=tol, **kw)
+    gp = torch.zeros((S, Rq, Cq), dtype=torch.float32, device=dev)
+    gp[:, Rq - R:, :C] = g.to(torch.float32)
+    if not torch.is_tensor(r_wire):
+        r_wire = torch.full((B,), float(r_wire), dtype=torch.float64, device=dev)
+    r_wire = r_wire.to(device=dev, dtype=torch.float64).reshape(-1).expand(B).contiguous()
+    vp = torch.zeros((B, Rq), dtype=torch.float64, device=dev)
+    vp[:, Rq - R:] = vin.to(device=dev, dtype=torch.float64).reshape(B, R)
+    if not ragged:
+        res = solve_batch(gp, r_wire, vp, gidx=gidx, tol=tol, **kw)
+        return SolveResult(res.iout[:, :C].contiguous(), res.sweeps, res.status, res.est, None, res.rejected)
+    iout = torch.empty((B, C), dtype=torch.float64, device=dev)
+    sweeps = torch.empty(B, dtype=torch.int32, device=dev)
+    status = torch.empty(B, dtype=torch.int32, device=dev)
+    est = torch.empty(B, dtype=torch.float64, device=dev)
+    gi = gidx.to(device=dev, dtype=torch.long).reshape(B) if gidx is not None else None
+    plans = {}
+    for c0 in range(0, B, chunk):
+        c1 = min(B, c0 + chunk)
+        n = c1 - c0
+        if n not in plans:
+            plans[n] = RaggedPlan([Rq] * n, device=dev)
+        gs = gp[gi[c0:c1]] if gi is not None else gp[c0:c1]
+        res = plans[n].solve(gs.reshape(-1), r_wire[c0:c1].contiguous(), vp[c0:c1].reshape(-1).contiguous(),
+                             tol=tol)
+        iout[c0:c1] = res.iout.view(n, Cq)[:, :C]
+        sweeps[c0:c1], status[c0:c1], est[c0:c1] = res.sweeps, res.status, res.est
+    return SolveResult(iout, sweeps, status, est, None, None)
+
+
 def cell_conductance(g_lin: torch.Tensor, vin: torch.Tensor, vout: Optional[torch.Tensor] = None,
                      field: Optional[torch.Tensor] = None, *, params, gidx: Optional[torch.Tensor] = None,
                      out: Optional[torch.Tensor] = None) -> torch.Tensor:

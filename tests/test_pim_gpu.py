@@ -352,3 +352,49 @@ def test_ou_node_by_node_layer_runs_each_ou_block_alone(cuda):
     ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=block_currents, ou_rows=our)
     bound = _tie_bound(info, our / 1023, 1e-4)
     assert np.max(np.abs(out - ref)) <= bound + 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("shape", [(64, 66), (100, 40), (128, 130), (200, 250)])
+def test_padded_crossbars_match_exact(cuda, shape):
+    """Crossbars of shapes no kernel takes, solved as the smallest larger
+    crossbar with the added cells off (solver.solve_padded: a batched shape,
+    or a 256x256 mixed-size square), against the exact FP64 solve of the
+    original shape at 1e-6 relative in every bitline; conductance slots
+    shared through gidx."""
+    import torch
+    from fadesim_amd.solver import embedding_shape, solve_padded
+    R, C = shape
+    rng = np.random.default_rng(R * 1000 + C)
+    S, B = 3, 8
+    g = np.exp(rng.uniform(np.log(5e-7), np.log(1e-6), (S, R, C))).astype(np.float32)
+    r = rng.uniform(1.0, 5.0, B)
+    vin = 0.3 * rng.integers(0, 8, (B, R)) / 7.0
+    gidx = rng.integers(0, S, B).astype(np.int32)
+    res = solve_padded(torch.tensor(g, device=cuda), torch.tensor(r, device=cuda), torch.tensor(vin, device=cuda),
+                       gidx=torch.tensor(gidx, device=cuda), tol=1e-6, ir2s_iters=4, corr_iters=4, test_first=True)
+    torch.cuda.synchronize()
+    assert (res.status.cpu().numpy() == 0).all()
+    iex = oracle.exact_batch(g[gidx], r, vin)
+    err = np.abs(res.iout.cpu().numpy() - iex) / np.abs(iex)
+    print(f"{R}x{C} as {embedding_shape(R, C)}: max rel err {err.max():.2e}")
+    assert err.max() <= 1e-6
+
+
+def test_converged_ref_layer_matches_exact_restatement(cuda):
+    """Reference-column arrays (R x (C+2) tiles) in the converged mode, solved
+    as 64x128 crossbars with the added cells off, against the restatement
+    driven by exact FP64 solves of the 64x66 tiles."""
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(21)
+    insize, outsize, bsize = 130, 40, 2
+    x = torch.rand(bsize, insize, generator=gen) - 0.3
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.mlp_inference_params(nonlinear_cell_enable=False, arrRowSize=64, arrColSize=64)
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, pim.phyArrMode.Ref,
+                                 params=P).cpu().numpy()
+    ref, info = opim.split_matmul_ref(x.numpy(), w.numpy(), cell_bits=2, rows=64, cols=64, vmax=0.2,
+                                      ron=P.Ron, roff=P.Roff, currents=_exact_currents(P.r_wire, 66))
+    bound = _tie_bound(info, 64 * 3 / 1023 * 129, 2e-3)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-5 * np.abs(ref).max()
+    assert P.stats["solves"] > 0
