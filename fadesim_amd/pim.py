@@ -338,9 +338,7 @@ def splitArr_matmul_nn(input: Tensor, w: Tensor, inputBits: int, dacBits: int, a
             else:
                 I = torch.einsum("ibxr,ijsxyrc->ijsbxyc", vin, Gb)
         elif P.ir_mode == "converged":
-            if P.useOUSize:
-                raise NotImplementedError("OU mode runs the hook's OU iterations (ir_mode 'reference' or 'ir2s')")
-            I = _solve_converged(bits, G, inputV, P)
+            I = _solve_converged_ou(bits, G, inputV, P) if P.useOUSize else _solve_converged(bits, G, inputV, P)
         elif P.ir_mode in ("reference", "ir2s"):
             I = _solve_reference(bits, G, inputV, P, float_type)
         else:
@@ -437,6 +435,47 @@ def _solve_converged(bits, G, inputV, P):
         if int((res.status != 0).sum()) != 0:
             raise RuntimeError("crossbar solve did not converge")
         I[ii[sl], jj[sl], ss[sl], bb[sl], xx[sl], yy[sl]] = res.iout
+    if P.record is not None:
+        P.record.append(I)
+    return I
+
+
+def _solve_converged_ou(bits, G, inputV, P):
+    """OU mode in the converged model: only one OUSize[0] x OUSize[1] block of
+    a tile conducts during a read, so each block's column currents are those
+    of its whole tile with every other cell off and only the block's rows
+    driven -- the state the reference's OU iterations (ir_gs_ou,
+    split_arr_matmul.cu:1086-1327, with the wire loads of the block's
+    position) converge to.  One converged solve of the masked tiles per
+    (row block, column block); the result has one entry per OU row block,
+    [..][numX * R / OUSize[0]][numY][Cx] (x-major), as the OU iterations give
+    it.  Linear cells."""
+    if P.nonlinear_cell_enable:
+        raise NotImplementedError("converged OU mode with sinh cells (the cell model of an off cell is not defined)")
+    nbit, bsize, numX, R = bits.shape
+    nv, nslice, ns, _, numY, _, Cx = G.shape
+    our, ouc = P.OUSize
+    if R % our or Cx % ouc:
+        raise ValueError(f"{R} x {Cx} tiles do not split into {our} x {ouc} OU blocks")
+    nrb, ncb = R // our, Cx // ouc
+    I = torch.zeros((nbit, nslice, ns, bsize, numX, nrb, numY, Cx), dtype=torch.float64, device=G.device)
+    record, P.record = P.record, None
+    try:
+        for rb in range(nrb):
+            rows = slice(rb * our, (rb + 1) * our)
+            bits_rb = torch.zeros_like(bits)
+            bits_rb[..., rows] = bits[..., rows]
+            if not bool(bits_rb.any()):
+                continue
+            for cb in range(ncb):
+                cols = slice(cb * ouc, (cb + 1) * ouc)
+                Gm = torch.zeros_like(G)
+                Gm[..., rows, cols] = G[..., rows, cols]
+                Ib = _solve_converged(bits_rb, Gm, inputV, P)
+                I[:, :, :, :, :, rb, :, cols] = Ib[..., cols]
+    finally:
+        P.record = record
+    I = I.reshape(nbit, nslice, ns, bsize, numX * nrb, numY, Cx)
     if P.record is not None:
         P.record.append(I)
     return I

@@ -142,8 +142,9 @@ def test_ou_size_checks():
     with pytest.raises(Exception, match="not less than 2"):
         pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="none", useOUSize=True,
                                                                                  OUSize=(8, 1)))
-    with pytest.raises(NotImplementedError):
-        pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="converged", useOUSize=True))
+    with pytest.raises(NotImplementedError):  # converged OU blocks: linear cells only
+        pim.splitArr_matmul_nn(x, w, 8, 1, 10, 10, 8, params=pim.PhyArrParams(ir_mode="converged", useOUSize=True,
+                                                                                 nonlinear_cell_enable=True))
 
 
 # --- reference-column arrays and Conv2d_sp -------------------------------------

@@ -398,3 +398,38 @@ def test_converged_ref_layer_matches_exact_restatement(cuda):
     bound = _tie_bound(info, 64 * 3 / 1023 * 129, 2e-3)
     assert np.max(np.abs(out - ref)) <= bound + 1e-5 * np.abs(ref).max()
     assert P.stats["solves"] > 0
+
+
+@pytest.mark.parametrize("ref_cols", [False, True])
+def test_converged_ou_layer_matches_converged_ou_iterations(cuda, ref_cols):
+    """OU mode in the converged model (every OU block solved as its whole tile
+    with the other cells off, 16x8 blocks of 64x64 tiles, PN arrays or
+    reference-column arrays of 64 + 8 columns) against the restatement
+    driven by the reference's OU iteration (ir_gs_ou's CPU replay in FP64,
+    the block's own wire loads) run until no node moves by 1e-15 V."""
+    import torch
+    from fadesim_amd import pim
+    gen = torch.Generator().manual_seed(23)
+    insize, outsize, bsize, ou = 70, 20, 2, (16, 8)
+    x = torch.rand(bsize, insize, generator=gen) - 0.3
+    w = torch.randn(insize, outsize, generator=gen)
+    P = pim.PhyArrParams(ir_mode="converged", useOUSize=True, OUSize=ou)
+    mode = pim.phyArrMode.Ref if ref_cols else pim.phyArrMode.PN
+    out = pim.splitArr_matmul_nn(x.to(cuda), w.to(cuda), 8, 1, 10, 10, 8, mode, params=P).cpu().numpy()
+    Cx = 64 + (ou[1] if ref_cols else 0)
+
+    def converged_ou(vin, G):
+        bsize_, numX, R = vin.shape
+        numY = G.shape[2] // Cx
+        o, sw = oracle.hook_gs_ou(vin.astype(np.float64), G.astype(np.float64), bsize_, R, Cx, numX, numY, 100000,
+                                  1 / P.r_wire, 1.5, True, 1e-15, False, [], ou_rows=ou[0], ou_cols=ou[1])
+        return o.reshape(bsize_, -1, G.shape[2])
+    if ref_cols:
+        ref, info = opim.split_matmul_ref(x.numpy(), w.numpy(), rows=64, cols=64, currents=converged_ou,
+                                          ou_rows=ou[0], ou_cols=ou[1])
+        bound = _tie_bound(info, ou[0] / 1023 * 129, 2e-3)
+    else:
+        ref, info = opim.split_matmul(x.numpy(), w.numpy(), rows=64, cols=64, currents=converged_ou, ou_rows=ou[0])
+        bound = _tie_bound(info, ou[0] / 1023, 2e-3)
+    assert np.max(np.abs(out - ref)) <= bound + 1e-5 * np.abs(ref).max()
+    assert P.stats["solves"] > 0
