@@ -235,11 +235,12 @@ def bench_mlp(args, dev, net, k_ir2s):
     return out
 
 
-def ir2s_iters_128(dev) -> int:
-    """IR2S iterations for 128x128 from the autotuned table (4 without one)."""
+def ir2s_schedule_128(dev):
+    """IR2S iterations and their damping for 128x128 from the autotuned table
+    (4 undamped without one; a negative damping is the Chebyshev schedule)."""
     from fadesim_amd import autotune
-    t = autotune.load_table(dev) or {}
-    return int(t.get("128", {}).get("ir2s_iters", 4)) or 4
+    t = (autotune.load_table(dev) or {}).get("128", {})
+    return int(t.get("ir2s_iters", 4)) or 4, float(t.get("ir2s_omega", 1.0)) or 1.0
 
 
 def main():
@@ -283,7 +284,7 @@ def main():
     tr = torch.tensor(r, device=dev)
     tv = torch.tensor(vin, device=dev)
     net = warmstart.load_default(dev)
-    K = ir2s_iters_128(dev)
+    K, W = ir2s_schedule_128(dev)
     only = set(x for x in args.only.split(",") if x)
 
     # name: (IR2S iterations of the solve, passes over the conductance per
@@ -294,7 +295,7 @@ def main():
         "plain": (0, 0, False, False, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, timing=timing)),
         # K wire-solve (IR2S) iterations on the zero guess inside the solve kernel
         "ir2s": (K, K, False, False, lambda timing=False: solve_batch(tg, tr, tv, tol=args.tol, ir2s_iters=K,
-                                                                    corr_iters=K, timing=timing)),
+                                                                    corr_iters=K, ir2s_omega=W, timing=timing)),
     }
     if net is not None:
         # the network's start formed inside the solve's guess kernel (pooled
@@ -351,7 +352,8 @@ def main():
         passes = float(sw.sum()) / args.steps + pre_passes * B
         ach = algorithmic_bytes(passes, B, warm=warm_rd, coarse=coarse_rd) / (kernel_ms * 1e-3) / 1e9
         meas[name] = {"solves_per_s": B * args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
-                      "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s, "kernel_ms": kernel_ms,
+                      "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s,
+                      "ir2s_omega": W if name == "ir2s" else 1.0, "kernel_ms": kernel_ms,
                       "guess_kernel_ms": guess_ms, "kernel_GBps_algorithmic": ach}
     head = next(k for k in ("nn", "ir2s", "plain", "nn_3k", "nn_unet16") if k in meas)
     hm = meas[head]

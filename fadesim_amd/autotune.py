@@ -8,7 +8,9 @@ applied to the initial guess (the same count on the refinement correction)
 and their damping -- the contraction per iteration grows with r g n^2, so
 small crossbars want few and 512x512 many, damped; then, with that guess,
 the correction's own count, which only has to take the refined error from
-the switch point to the tolerance; then the relaxation factor
+the switch point to the tolerance (a negative damping is the Chebyshev
+schedule over that multiple of the estimated spectral radius, computed per
+crossbar in the kernel); then the relaxation factor
 w(k) = 2 / (1 + k sin(pi / (2n + 1))) and the bounds of the interval between
 convergence tests.  The table is stored per GPU architecture under
 fadesim_amd/tuning/<arch>.json and read by RaggedPlan.
@@ -29,8 +31,9 @@ import torch
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 KAPPAS = (0.55, 0.7, 0.83, 1.0, 1.25)
 CHECKS = ((32, 16, 256), (64, 32, 256), (32, 32, 512))
-IR2S = (0, 2, 4, 6, 8, 10, 12, 14, 16)   # iterations on the initial guess (and the correction's)
-IR2S_OMEGA = (1.0, 0.85, 0.75, 0.7, 0.6)  # their damping: 2/(2+rho) for coupling radius rho
+IR2S = tuple(range(17))                  # iterations on the initial guess (and the correction's)
+IR2S_OMEGA = (1.0, 0.85, 0.75, 0.7, 0.6,  # their damping: 2/(2+rho) for coupling radius rho
+              -1.0, -1.15, -1.3)          # negative: Chebyshev schedule over |w| x the estimated radius
 CORR = (1, 2, 3, 4, 5, 6, 8, 10, 12)      # iterations on the refinement correction
 
 

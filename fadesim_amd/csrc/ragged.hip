@@ -121,9 +121,10 @@ __global__ void __launch_bounds__(kThreads) ragged_kernel(RaggedArgs A)
     __shared__ double colD[kNmax];
     __shared__ float wmax[kThreads / 64];
     __shared__ int s_task, s_dead;
+    __shared__ double s_gsum;
     __shared__ double cbuf[kCbufMax];
     __shared__ double cext[3 * kNmax];
-    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext, &s_gsum};
     const rb::StripCtx none{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * kSlotCells;
     const int team = blockIdx.x / kTeam, rank = blockIdx.x % kTeam;
@@ -295,19 +296,22 @@ extern "C" int fadesim_ragged_plan_create(fadesim_ragged_plan **out, int count, 
         C.ir2s_iters = 0;
         C.corr_iters = 2;
         C.ir2s_omega = 1.0;
+        C.ir2s_geo = rb::ir2s_geo(64 << c, 64 << c);
         if (opts) {
             if (opts[c].ir2s_iters < 0 || opts[c].ir2s_iters > FADESIM_MAX_IR2S_ITERS ||
                 opts[c].corr_iters < 0 || opts[c].corr_iters > FADESIM_MAX_IR2S_ITERS ||
-                !(opts[c].ir2s_omega >= 0.0 && opts[c].ir2s_omega <= 1.0)) {
+                !(opts[c].ir2s_omega >= -FADESIM_MAX_IR2S_RHO_SCALE && opts[c].ir2s_omega <= 1.0)) {
                 delete P;
                 return set_error(FADESIM_E_INVALID, "ir2s_iters and corr_iters must lie in [0, " +
                                                         std::to_string(FADESIM_MAX_IR2S_ITERS) +
-                                                        "], ir2s_omega in (0, 1] (0 selects 1)");
+                                                        "], ir2s_omega in (0, 1] (0 selects 1) or [-" +
+                                                        std::to_string((int)FADESIM_MAX_IR2S_RHO_SCALE) +
+                                                        ", 0) (Chebyshev schedule)");
             }
             C.ir2s_iters = opts[c].ir2s_iters;
             if (opts[c].corr_iters > 0)
                 C.corr_iters = opts[c].corr_iters;
-            if (opts[c].ir2s_omega > 0.0)
+            if (opts[c].ir2s_omega != 0.0)
                 C.ir2s_omega = opts[c].ir2s_omega;
             if (opts[c].omega > 0) {
                 if (!(opts[c].omega < 2.0)) {

@@ -78,7 +78,8 @@ struct Ctl {
     int check_min, check_max; // bounds of the predicted test interval
     int ir2s_iters;    // wire-solve (IR2S) iterations applied to the initial guess
     int corr_iters;    // IR2S iterations of the refinement correction's guess (>= 1)
-    double ir2s_omega; // damping of the IR2S iterations, (0, 1]
+    double ir2s_omega; // damping of the IR2S iterations, (0, 1]; < 0: Chebyshev schedule (ir2s_step)
+    double ir2s_geo;   // the schedule's geometry factor, ir2s_geo(rows, cols)
 };
 
 // One crossbar.  rows is the full row count (all strips).
@@ -655,6 +656,7 @@ struct Shared {
     int *dead;
     double *cbuf; // [2][TY][N] per-thread-row column partials
     double *cext; // [3][N] column values shared by the workgroup
+    double *gsum; // [1] total conductance of the crossbar (all strips), from the set-up
 };
 
 // ---------------------------------------------------------------- tile reductions
@@ -1031,8 +1033,9 @@ __device__ __forceinline__ bool setup_tile(const Inst &I, const StripCtx &X, con
                                            int gi0, int i0, int j0, int t, int ty, double rwire, const Shared &sh,
                                            int &nbar, float (&zq)[G::TR][G::TC])
 {
-    constexpr int TR = G::TR, TC = G::TC, N = G::N;
+    constexpr int TR = G::TR, TC = G::TC, N = G::N, NW = G::NW;
     double si[TC], sf[TC];
+    float gs = 0.f; // this tile's conductance, for the wire-solve schedule's spectral estimate
 #pragma unroll
     for (int jj = 0; jj < TC; ++jj)
         si[jj] = sf[jj] = 0.0;
@@ -1044,15 +1047,27 @@ __device__ __forceinline__ bool setup_tile(const Inst &I, const StripCtx &X, con
             const double c = vi - ldg(I.vout + (j0 + jj));
             si[jj] += (double)gt[ii][jj] * c;
             sf[jj] += (double)gt[ii][jj] * fabs(c);
+            gs += gt[ii][jj];
         }
     }
-    col_reduce2<G>(sh, si, sf, t, ty, j0, sh.S1, sh.Ifloor);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        gs += __shfl_xor(gs, off, 64);
+    if ((t & 63) == 0)
+        sh.wmax[t >> 6] = gs;
+    col_reduce2<G>(sh, si, sf, t, ty, j0, sh.S1, sh.Ifloor); // its barriers publish wmax
+    double gsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k)
+        gsum += (double)sh.wmax[k];
     if (STRIP) {
         double *mine = X.setup + ((size_t)X.s * 4) * kRowsLd;
         if (t < N) {
             team_st(mine + t, sh.S1[t]);
             team_st(mine + kRowsLd + t, sh.Ifloor[t]);
         }
+        if (t == 0)
+            team_st(mine + 2 * kRowsLd, gsum);
         if (!team_barrier(X, sh.dead, t, nbar++))
             return false;
         if (t < N) {
@@ -1061,7 +1076,14 @@ __device__ __forceinline__ bool setup_tile(const Inst &I, const StripCtx &X, con
             sh.S1[t] = a;
             sh.Ifloor[t] = b;
         }
+        if (t == 0) {
+            gsum = 0.0;
+            for (int k = 0; k < X.S; ++k)
+                gsum += team_ld(X.setup + ((size_t)k * 4 + 2) * kRowsLd);
+        }
     }
+    if (t == 0)
+        *sh.gsum = gsum; // read after the barriers of the bitline adjoint below
     if (t < N)
         // a column carrying under 1e-9 of its driven current is judged absolutely
         sh.Ifloor[t] = fmax(1e-9 * sh.Ifloor[t], 1e-300);
@@ -1357,6 +1379,10 @@ __device__ __forceinline__ T *lds_at(unsigned off)
     typedef __attribute__((address_space(3))) T lds_t;
     return (T *)(lds_t *)(size_t)__builtin_amdgcn_readfirstlane(off);
 }
+__device__ __forceinline__ float uniform_f32(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
 __device__ __forceinline__ double uniform_f64(double x)
 {
     const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
@@ -1388,14 +1414,15 @@ __device__ __forceinline__ void uniform_strip(StripCtx &X)
 
 #define RB_SH_PARAMS                                                                                    \
     unsigned sh_halo, unsigned sh_S1, unsigned sh_Ifloor, unsigned sh_colI, unsigned sh_colD,                \
-        unsigned sh_wmax, unsigned sh_dead, unsigned sh_cbuf, unsigned sh_cext
+        unsigned sh_wmax, unsigned sh_dead, unsigned sh_cbuf, unsigned sh_cext, unsigned sh_gsum
 #define RB_SH_ARGS(s)                                                                                      \
     lds_off(s.halo), lds_off(s.S1), lds_off(s.Ifloor), lds_off(s.colI), lds_off(s.colD), lds_off(s.wmax),    \
-        lds_off(s.dead), lds_off(s.cbuf), lds_off(s.cext)
+        lds_off(s.dead), lds_off(s.cbuf), lds_off(s.cext), lds_off(s.gsum)
 #define RB_SH_LOCAL                                                                                     \
     Shared sh{lds_at<float>(sh_halo),  lds_at<double>(sh_S1),   lds_at<double>(sh_Ifloor),                 \
               lds_at<double>(sh_colI), lds_at<double>(sh_colD), lds_at<float>(sh_wmax),                    \
-              lds_at<int>(sh_dead),    lds_at<double>(sh_cbuf), lds_at<double>(sh_cext)};                  \
+              lds_at<int>(sh_dead),    lds_at<double>(sh_cbuf), lds_at<double>(sh_cext),                   \
+              lds_at<double>(sh_gsum)};                                                                  \
     uniform_inst(I);                                                                                         \
     uniform_strip(X);                                                                                        \
     wsb = uniform_ptr(wsb);                                                                                  \
@@ -1502,6 +1529,36 @@ __device__ __forceinline__ void load_start(const Inst &I, const Shared &sh, int 
     }
 }
 
+// Damping of wire-solve iteration k of K.  w in (0, 1]: that fixed damping.
+// w < 0: the Chebyshev schedule for the eigenvalues of M^-1 K in [0, rho],
+// rho = |w| x rho_est,
+//   rho_est = r gbar (1 / (4 sin^2(pi / (4 cols + 2))) + 1 / (4 sin^2(pi / (4 rows + 2))))
+// -- r gbar times the largest eigenvalues of the wordline and bitline Green's
+// matrices min(j,j') + 1 (cols x cols) and n - max(i,i') (rows x rows), exact
+// for uniform g, with gbar the crossbar's mean conductance: rho_est = r gsum
+// geo, geo (Ctl::ir2s_geo) = the bracket / (rows cols), formed on the host.
+// Step k is the Richardson step 1 / mu_k at the Chebyshev node
+//   mu_k = 1 + rho c_k,  c_k = (1 + cos(pi (2k + 1) / (2K))) / 2
+// of [1, 1 + rho], so after K steps the error is p(M^-1 K) e_0 with
+// p(lambda) = T_K((2 lambda - rho) / rho) / T_K(-(2 + rho) / rho): at most
+// 1 / T_K((2 + rho) / rho) on the interval, against (rho / (2 + rho))^K for
+// the best fixed damping.  Every factor |1 - (1 + lambda) / mu_k| is below
+// max(1, rho), so the order of the steps is immaterial at these K.  FP32,
+// the cosine by the hardware's v_cos_f32 (argument in turns).
+__device__ __forceinline__ double ir2s_step(double w, float rho, int k, int K)
+{
+    if (w > 0.0)
+        return w;
+    const float c = 0.5f * (1.0f + __builtin_amdgcn_cosf((float)(2 * k + 1) / (float)(4 * K)));
+    return (double)(1.0f / fmaf(rho, c, 1.0f));
+}
+// geo of ir2s_step for a rows x cols crossbar (host)
+inline double ir2s_geo(int rows, int cols)
+{
+    const double sc = std::sin(M_PI / (4.0 * cols + 2.0)), sr = std::sin(M_PI / (4.0 * rows + 2.0));
+    return (0.25 / (sc * sc) + 0.25 / (sr * sr)) / ((double)rows * (double)cols);
+}
+
 // `iters` damped wire-solve iterations on the register tile p, q.  HOLD: the
 // tile's conductances come from gt (held in registers by the caller), else
 // every iteration reads them from gp (the cache) row by row and gt is unused.
@@ -1509,7 +1566,8 @@ template <class G, bool STRIP, bool HOLD>
 __device__ __forceinline__ bool ir2s_iterate(const Inst &I, const StripCtx &X, const Shared &sh,
                                              const float (&gt)[G::TR][G::TC], const float *gp, int gi0,
                                              int i0, int j0, int t, int tx, int ty, double rwire, int &nbar, int iters,
-                                             double w, float (&p)[G::TR][G::TC], float (&q)[G::TR][G::TC])
+                                             double w, double geo, float (&p)[G::TR][G::TC],
+                                             float (&q)[G::TR][G::TC])
 {
     constexpr int TC = G::TC;
     float vi[G::TR], vo[TC];
@@ -1534,6 +1592,7 @@ __device__ __forceinline__ bool ir2s_iterate(const Inst &I, const StripCtx &X, c
             a[jj] = b[jj] = gr[jj] * (((vi[ii] - vo[jj]) - pr[jj]) - qr[jj]);
     };
     iters = uniform_i32(iters); // a loop with barriers: its trip count a scalar
+    const float rho = uniform_f32(w < 0.0 ? (float)(-w * rwire * uniform_f64(*sh.gsum) * geo) : 0.f);
 #pragma unroll 1
     for (int k = 0; k < iters; ++k) {
         // the cell voltages vi - vo are recomputed in every iteration rather
@@ -1544,7 +1603,8 @@ __device__ __forceinline__ bool ir2s_iterate(const Inst &I, const StripCtx &X, c
 #pragma unroll
         for (int jj = 0; jj < TC; ++jj)
             asm volatile("" : "+v"(vo[jj]));
-        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, nbar, p, q))
+        const double wk = uniform_f64(ir2s_step(w, rho, k, iters));
+        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, wk, I.rows, gi0, i0, j0, tx, ty, t, nbar, p, q))
             return false;
     }
     return true;
@@ -1569,7 +1629,8 @@ __device__ __forceinline__ bool start_tile(const Inst &I, const StripCtx &X, con
 #pragma unroll
         for (int jj = 0; jj < G::TC; ++jj)
             p[ii][jj] = q[ii][jj] = 0.f;
-    if (!ir2s_iterate<G, STRIP, HOLD>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, nbar, kBaseIters, 1.0, p, q))
+    if (!ir2s_iterate<G, STRIP, HOLD>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, nbar, kBaseIters, 1.0, 0.0, p,
+                                      q))
         return false;
     if (!STRIP)
         add_coarse<G>(I, sh.halo, t, gi0, i0, j0, p, q);
@@ -1598,7 +1659,7 @@ __device__ __attribute__((noinline)) bool phase_coarse(Inst I, StripCtx X, float
 template <class G, bool STRIP>
 __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
                                                      int i0, int j0, int t, int tx, int ty, double rwire, RB_SH_PARAMS,
-                                                     int *nbar, int iters, double w, int start)
+                                                     int *nbar, int iters, double w, double geo, int start)
 {
     RB_SH_LOCAL;
     constexpr int TC = G::TC;
@@ -1611,7 +1672,7 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
         ldrow<TC>(gp + (size_t)(i0 + ii) * G::N + j0, gt[ii]);
     if (!start_tile<G, STRIP, true>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar, uniform_i32(start), p, q))
         return false;
-    if (!ir2s_iterate<G, STRIP, true>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar, iters, w, p, q))
+    if (!ir2s_iterate<G, STRIP, true>(I, X, sh, gt, gp, gi0, i0, j0, t, tx, ty, rwire, *nbar, iters, w, geo, p, q))
         return false;
     tile_store<G>(wsb, i0, j0, p, q);
     return true;
@@ -1628,7 +1689,8 @@ __device__ __attribute__((noinline)) bool phase_ir2s(Inst I, StripCtx X, float *
 template <class G, bool STRIP>
 __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float *wsb, const float *gp, int gi0,
                                                       int i0, int j0, int t, int tx, int ty, double rwire,
-                                                      double hcoef, RB_SH_PARAMS, int *nbar, int iters, double w)
+                                                      double hcoef, RB_SH_PARAMS, int *nbar, int iters, double w,
+                                                      double geo)
 {
     RB_SH_LOCAL;
     constexpr int TR = G::TR, TC = G::TC, M = G::M, N = G::N;
@@ -1656,9 +1718,11 @@ __device__ __attribute__((noinline)) bool phase_green(Inst I, StripCtx X, float 
         }
     };
     iters = uniform_i32(iters); // a loop with barriers: its trip count a scalar
+    const float rho = uniform_f32(w < 0.0 ? (float)(-w * rwire * uniform_f64(*sh.gsum) * geo) : 0.f);
 #pragma unroll 1
     for (int k = 0; k < iters; ++k)
-        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, w, I.rows, gi0, i0, j0, tx, ty, t, *nbar, p, q))
+        if (!wire_iteration<G, STRIP>(sh, X, cur, rwire, uniform_f64(ir2s_step(w, rho, k, iters)), I.rows, gi0, i0,
+                                      j0, tx, ty, t, *nbar, p, q))
             return false;
     tile_store<G>(wsb, i0, j0, p, q);
     return true;
@@ -1711,7 +1775,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
         // correction combined while loading)
         const int start = coarse ? (I.warm ? kStartWarmCoarse : kStartBaseCoarse) : I.warm ? kStartWarm : kStartZero;
         if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar, C.ir2s_iters,
-                                  C.ir2s_omega, start))
+                                  C.ir2s_omega, C.ir2s_geo, start))
             return false;
         tile_load<G>(wsb, i0, j0, yp, yq);
         snap_valid = true;
@@ -1797,7 +1861,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 rejected = true;
                 if (C.ir2s_iters > 0) {
                     if (!phase_ir2s<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, RB_SH_ARGS(sh), &nbar,
-                                              C.ir2s_iters, C.ir2s_omega, kStartZero))
+                                              C.ir2s_iters, C.ir2s_omega, C.ir2s_geo, kStartZero))
                         return false;
                     tile_load<G>(wsb, i0, j0, yp, yq);
                 } else {
@@ -1853,7 +1917,7 @@ __device__ __forceinline__ bool solve(const Inst &I, const StripCtx &X, const Ct
                 snap_valid = false;
             } else {
                 if (!phase_green<G, STRIP>(I, X, wsb, gp, gi0, i0, j0, t, tx, ty, rwire, hcoef, RB_SH_ARGS(sh), &nbar,
-                                           C.corr_iters, C.ir2s_omega))
+                                           C.corr_iters, C.ir2s_omega, C.ir2s_geo))
                     return false;
                 tile_load<G>(wsb, i0, j0, yp, yq);
             }
@@ -1994,8 +2058,8 @@ __device__ __forceinline__ bool guess_solve(const Inst &I, const Ctl &C, float *
                 for (int ii = 0; ii < TR; ++ii)
                     ldrow<TC>(gp + (size_t)(i0 + ii) * N + j0, gt[ii]);
             }
-            ir2s_iterate<G, false, HOLD>(I, X, sh, gt, gp, 0, i0, j0, t, tx, ty, rwire, nbar, iters, C.ir2s_omega, p,
-                                         q);
+            ir2s_iterate<G, false, HOLD>(I, X, sh, gt, gp, 0, i0, j0, t, tx, ty, rwire, nbar, iters, C.ir2s_omega,
+                                         C.ir2s_geo, p, q);
         }
         __syncthreads(); // the column buffer and the halo stage are idle
 #pragma unroll

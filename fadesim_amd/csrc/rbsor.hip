@@ -72,9 +72,10 @@ __global__ void __launch_bounds__(G::NT) rbsor_kernel(RbArgs A)
     __shared__ double colD[N];
     __shared__ float wmax[G::NW];
     __shared__ int s_inst, s_dead;
+    __shared__ double s_gsum;
     __shared__ double cbuf[2 * G::TY * N];
     __shared__ double cext[3 * N];
-    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext, &s_gsum};
     const rb::StripCtx none{1, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     for (;;) {
@@ -139,13 +140,14 @@ __global__ void __launch_bounds__(G::NT) guess_kernel(RbArgs A)
     __shared__ double colD[N];
     __shared__ float wmax[G::NW];
     __shared__ int s_inst, s_dead;
+    __shared__ double s_gsum;
     __shared__ double cext[3 * N];
     // the column buffer first: its rows are then in reach of the LDS
     // instructions' immediate offsets from one address register
     double *cbuf = reinterpret_cast<double *>(big);
     float *halo = reinterpret_cast<float *>(big + kCbufB);
     float *fld = reinterpret_cast<float *>(big);
-    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext, &s_gsum};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     for (;;) {
         if (threadIdx.x == 0)
@@ -229,6 +231,7 @@ __global__ void __launch_bounds__(NetGeo::NT) guess_net_kernel(RbArgs A)
     __shared__ double colD[N];
     __shared__ float wmax[G::NW];
     __shared__ int s_inst, s_dead;
+    __shared__ double s_gsum;
     __shared__ double cext[3 * N];
     __shared__ float red[4 * G::NW];
     double *cbuf = reinterpret_cast<double *>(big);
@@ -236,7 +239,7 @@ __global__ void __launch_bounds__(NetGeo::NT) guess_net_kernel(RbArgs A)
     float *fld = reinterpret_cast<float *>(big);
     __bf16 *bufA = reinterpret_cast<__bf16 *>(big);
     __bf16 *bufS = bufA + Lds::kA, *bufX = bufS + Lds::kS;
-    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext};
+    rb::Shared sh{halo, S1, Ifloor, colI, colD, wmax, &s_dead, cbuf, cext, &s_gsum};
     float *wsb = A.ws + (size_t)blockIdx.x * rb::kWsPlanes * M * N;
     const __bf16 *const w[7] = {A.netw[0], A.netw[1], A.netw[2], A.netw[3], A.netw[4], A.netw[5], A.netw[6]};
     for (;;) {
@@ -512,8 +515,10 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
         d->corr_iters > FADESIM_MAX_IR2S_ITERS)
         return set_error(FADESIM_E_INVALID, "ir2s_iters and corr_iters must lie in [0, " +
                                                 std::to_string(FADESIM_MAX_IR2S_ITERS) + "]");
-    if (!(d->ir2s_omega >= 0.0 && d->ir2s_omega <= 1.0))
-        return set_error(FADESIM_E_INVALID, "ir2s_omega must lie in (0, 1] (0 selects 1)");
+    if (!(d->ir2s_omega >= -FADESIM_MAX_IR2S_RHO_SCALE && d->ir2s_omega <= 1.0))
+        return set_error(FADESIM_E_INVALID, "ir2s_omega must lie in (0, 1] (0 selects 1), or in [-" +
+                                                 std::to_string((int)FADESIM_MAX_IR2S_RHO_SCALE) +
+                                                 ", 0) for the Chebyshev schedule");
     if (d->d_warm_coarse) {
         const int L = d->warm_level;
         if (!d->d_warm_scale || d->warm_coarse_ch < 2 || (d->warm_coarse_ch & 1) || L < 1 || L > 6 || ((M >> L) << L) != M ||
@@ -548,7 +553,8 @@ int batch_args(const fadesim_batch_desc *d, RbArgs &args, bool *empty)
     args.ctl.check_max = 256;
     args.ctl.ir2s_iters = d->ir2s_iters;
     args.ctl.corr_iters = d->corr_iters > 0 ? d->corr_iters : 2;
-    args.ctl.ir2s_omega = d->ir2s_omega > 0.0 ? d->ir2s_omega : 1.0;
+    args.ctl.ir2s_omega = d->ir2s_omega != 0.0 ? d->ir2s_omega : 1.0;
+    args.ctl.ir2s_geo = rb::ir2s_geo(d->rows, d->cols);
     args.iout = d->d_iout;
     args.sweeps = d->d_sweeps;
     args.status = d->d_status;

@@ -79,7 +79,9 @@ def solve_batch(g: torch.Tensor, r_wire, vin: torch.Tensor, vout: Optional[torch
     for the cell currents of the present drops, the iteration of the
     reference's ir2s (crossbarHspice/ir_drop_fast_solver.h:24-80) -- and
     ``corr_iters`` sets the iterations of the refinement correction's guess
-    (0: 2), ``ir2s_omega`` their damping.  ``test_first`` tests the initial guess before the first sweep
+    (0: 2), ``ir2s_omega`` their damping in (0, 1], or, negative, the
+    Chebyshev schedule whose bound is |ir2s_omega| x the estimated spectral
+    radius of the coupling (fadesim_batch_desc).  ``test_first`` tests the initial guess before the first sweep
     (default: when ir2s_iters >= 3).
     ``warm_coarse`` [B, M >> warm_level, N >> warm_level, C] bf16 with
     ``warm_scale`` [B]: the initial drops are base + warm_scale x

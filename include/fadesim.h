@@ -51,6 +51,8 @@ extern "C" {
                                    * and only the crossbars that test does not certify reach the
                                    * sweeping kernel; the outputs are the same bit for bit) */
 #define FADESIM_MAX_IR2S_ITERS 64
+/* Largest |ir2s_omega| of the Chebyshev schedule (ir2s_omega < 0). */
+#define FADESIM_MAX_IR2S_RHO_SCALE 4.0
 
 const char *fadesim_last_error(void);
 int fadesim_version(void);
@@ -100,7 +102,14 @@ typedef struct fadesim_batch_desc {
     double ir2s_omega;      /* damping w of the IR2S iterations, x <- (1-w) x + w x_IR2S,
                              * in (0, 1]; 0 -> 1.  With the coupling's spectral radius rho
                              * (~0.8 r gbar rows cols) the contraction is rho/(2+rho) at
-                             * w = 2/(2+rho), against rho undamped */
+                             * w = 2/(2+rho), against rho undamped.  Negative, in
+                             * [-FADESIM_MAX_IR2S_RHO_SCALE, 0): a Chebyshev schedule, step k
+                             * of K with w_k = 1 / (1 + (rho/2)(1 + cos(pi (2k+1) / (2K)))),
+                             * rho = |ir2s_omega| r gbar (1/(4 sin^2(pi/(4 cols+2))) +
+                             * 1/(4 sin^2(pi/(4 rows+2)))), gbar the instance's mean
+                             * conductance: the K steps reduce the error by
+                             * 1/T_K((2+rho)/rho) (T_K: Chebyshev polynomial), against
+                             * (rho/(2+rho))^K.  The same schedule serves corr_iters. */
     /* Warm start given as base + d_warm_scale[b] x bilinear(network output):
      * when d_warm_coarse is not NULL the initial drops are formed at load,
      * with the arithmetic of fadesim_nn_combine_up (channel 0 -> wordline drops,
@@ -194,7 +203,8 @@ typedef struct fadesim_ragged_class_opts {
     int check_max;
     int ir2s_iters;   /* IR2S iterations on the initial guess (0 = none) */
     int corr_iters;   /* IR2S iterations of the refinement correction's guess (0 = built-in, 2) */
-    double ir2s_omega; /* damping of the IR2S iterations in (0, 1] (0 = built-in, 1) */
+    double ir2s_omega; /* damping of the IR2S iterations in (0, 1] (0 = built-in, 1), or the
+                        * Chebyshev schedule's spectral scale, negative (fadesim_batch_desc) */
 } fadesim_ragged_class_opts;
 
 typedef struct fadesim_ragged_plan_stats {

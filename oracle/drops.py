@@ -28,8 +28,28 @@ def wire_drops(s: np.ndarray, r: float):
     return p, q
 
 
+def spectral_estimate(g, r: float) -> float:
+    """r gbar (1/(4 sin^2(pi/(4m+2))) + 1/(4 sin^2(pi/(4n+2)))): the largest
+    eigenvalue of s -> g (P + Q) s for uniform g (the wordline Green's matrix
+    min(j,j')+1 and the bitline one n - max(i,i') have largest eigenvalues
+    1/(4 sin^2(pi/(4m+2))) and 1/(4 sin^2(pi/(4n+2))); with uniform g the
+    operator is r g (P x I + I x Q)), used with the crossbar's mean
+    conductance for any g -- the bound of the Chebyshev schedule."""
+    g = np.asarray(g, dtype=np.float64)
+    n, m = g.shape
+    sc, sr = np.sin(np.pi / (4 * m + 2)), np.sin(np.pi / (4 * n + 2))
+    return float(r * g.mean() * (0.25 / sc ** 2 + 0.25 / sr ** 2))
+
+
+def chebyshev_schedule(K: int, rho: float):
+    """Damping of steps k = 0..K-1: 1 / mu_k at the Chebyshev nodes
+    mu_k = 1 + rho (1 + cos(pi (2k+1) / (2K))) / 2 of [1, 1 + rho]."""
+    return [1.0 / (1.0 + rho * 0.5 * (1.0 + np.cos(np.pi * (2 * k + 1) / (2 * K)))) for k in range(int(K))]
+
+
 def ir2s_drops(g, r: float, vin, vout=None, iters: int = 1, omega: float = 1.0, p0=None, q0=None):
     """`iters` damped IR2S iterations from (p0, q0) (zero drops by default).
+    omega < 0: the Chebyshev schedule over [0, |omega| spectral_estimate].
     Returns (p, q, iout) with iout_j = sum_i g_ij (c_ij - p_ij - q_ij)."""
     g = np.asarray(g, dtype=np.float64)
     n, m = g.shape
@@ -38,10 +58,14 @@ def ir2s_drops(g, r: float, vin, vout=None, iters: int = 1, omega: float = 1.0, 
     c = vin[:, None] - vout[None, :]
     p = np.zeros((n, m)) if p0 is None else np.asarray(p0, dtype=np.float64).copy()
     q = np.zeros((n, m)) if q0 is None else np.asarray(q0, dtype=np.float64).copy()
-    for _ in range(int(iters)):
+    if omega < 0:
+        steps = chebyshev_schedule(iters, -omega * spectral_estimate(g, r))
+    else:
+        steps = [omega] * int(iters)
+    for w in steps:
         pn, qn = wire_drops(g * (c - p - q), float(r))
-        p = (1.0 - omega) * p + omega * pn
-        q = (1.0 - omega) * q + omega * qn
+        p = (1.0 - w) * p + w * pn
+        q = (1.0 - w) * q + w * qn
     return p, q, (g * (c - p - q)).sum(0)
 
 

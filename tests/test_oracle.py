@@ -499,6 +499,67 @@ def test_ir2s_contraction_scales_with_r_g_n2():
     assert 0.5 < np.mean(vals) < 1.1
 
 
+
+@pytest.mark.parametrize("n,m", [(32, 32), (24, 40)])
+def test_spectral_estimate_is_the_radius_for_uniform_g(n, m):
+    """r g (1/(4 sin^2(pi/(4m+2))) + 1/(4 sin^2(pi/(4n+2)))) is the coupling's
+    spectral radius (power iteration) for uniform g, and within a few percent
+    of it for i.i.d. conductances (mean g); a transposed size or a dropped
+    source/sink segment (pi/(4n) instead of pi/(4n+2)) misses it."""
+    from oracle.drops import coupling_radius, spectral_estimate
+    r = 3.0
+    g = np.full((n, m), 7e-7)
+    rho = coupling_radius(g, r, iters=400)
+    assert abs(spectral_estimate(g, r) / rho - 1) < 1e-6
+    sc, sr = np.sin(np.pi / (4 * m)), np.sin(np.pi / (4 * n))
+    assert abs(r * 7e-7 * (0.25 / sc ** 2 + 0.25 / sr ** 2) / rho - 1) > 1e-2
+    rng = np.random.default_rng(n + m)
+    g = np.exp(rng.uniform(np.log(5e-7), np.log(1e-6), (n, m)))
+    assert abs(spectral_estimate(g, r) / coupling_radius(g, r, iters=400) - 1) < 0.03
+
+
+@pytest.mark.parametrize("rho", [0.05, 0.6, 1.4])
+@pytest.mark.parametrize("K", [1, 2, 3, 5, 8])
+def test_chebyshev_schedule_is_the_chebyshev_polynomial(rho, K):
+    """prod_k (1 - (1 + lam) w_k) = T_K((2 lam - rho)/rho) / T_K(-(2 + rho)/rho)
+    for lam in [0, rho] (the error polynomial of the schedule), K = 1 is the
+    best fixed damping 2/(2 + rho), and its bound beats (rho/(2+rho))^K."""
+    from numpy.polynomial import chebyshev as C
+    from oracle.drops import chebyshev_schedule
+    w = chebyshev_schedule(K, rho)
+    lam = np.linspace(0, rho, 41)
+    got = np.prod([1 - (1 + lam) * wk for wk in w], axis=0)
+    TK = [0] * K + [1]
+    want = C.chebval((2 * lam - rho) / rho, TK) / C.chebval(-(2 + rho) / rho, TK)
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-15)
+    if K == 1:
+        assert abs(w[0] - 2 / (2 + rho)) < 1e-15
+    else:
+        assert np.abs(got).max() < (rho / (2 + rho)) ** K
+
+
+def test_chebyshev_iteration_reaches_its_bound():
+    """The scheduled iteration on a crossbar converges to the exact solve at
+    the rate its polynomial predicts, and faster than the best fixed damping
+    over the same steps."""
+    from numpy.polynomial import chebyshev as C
+    from oracle.drops import ir2s_drops, spectral_estimate
+    rng = np.random.default_rng(12)
+    n, r = 48, 10.0   # r g n^2 large: the undamped iteration diverges here
+    g = np.exp(rng.uniform(np.log(5e-5), np.log(1e-4), (n, n)))
+    vin = 0.3 * rng.integers(0, 8, n) / 7.0
+    u, d, Iex = oracle.exact(g, vin, np.zeros(n), *oracle.segments_uniform(n, n, r))
+    rho = 1.05 * spectral_estimate(g, r)
+    assert rho > 1.0
+    e0 = np.max(np.abs(Iex))
+    for K in (4, 8):
+        _, _, io = ir2s_drops(g, r, vin, None, K, -1.05)
+        bound = 1 / C.chebval((2 + rho) / rho, [0] * K + [1])
+        err = np.max(np.abs(io - Iex)) / e0
+        assert err <= 1.5 * bound, (K, err, bound)
+        _, _, iofix = ir2s_drops(g, r, vin, None, K, 2 / (2 + rho))
+        assert err < np.max(np.abs(iofix - Iex)) / e0
+
 def _gen_inputs(z):
     n, m = int(z["n"]), int(z["m"])
     g = 1.0 / z["cell_R"][z["states"]]

@@ -113,17 +113,20 @@ def test_plan_rejects_unsupported_sizes(cuda):
         RaggedPlan([128], cols=[64], device=cuda)
 
 
-def test_ir2s_guess_on_strips_matches_oracle(cuda):
+@pytest.mark.parametrize("iters,omega", [(3, 0.8), (5, -1.1)])
+def test_ir2s_guess_on_strips_matches_oracle(cuda, iters, omega):
+    """Damped and Chebyshev-scheduled (omega < 0: the bound from the
+    conductance summed over the strips of a team) wire-solve guesses."""
     from fadesim_amd.ragged import RaggedPlan, SIZES
     from oracle.drops import ir2s_drops
     inst = _mixed(cuda, [256, 512, 64, 128], seed=21)
-    tuning = {str(n): {"ir2s_iters": 3, "ir2s_omega": 0.8} for n in SIZES}
+    tuning = {str(n): {"ir2s_iters": iters, "ir2s_omega": omega} for n in SIZES}
     plan = RaggedPlan(inst["rows"], device=cuda, tuning=tuning)
     res = plan.solve(inst["g"], inst["r"], inst["vin"], fixed_sweeps=0, want_field=True)
     fields = plan.unpack(res.field, "field")
     for k, n in enumerate(inst["rows"]):
         p, q, _ = ir2s_drops(inst["g_list"][k].astype(np.float64), inst["r_np"][k], inst["vin_list"][k],
-                             None, 3, 0.8)
+                             None, iters, omega)
         for a, b in ((fields[k][0], p), (fields[k][1], q)):
             err = np.max(np.abs(a.astype(np.float64) - b)) / np.abs(b).max()
             assert err <= 2e-5, (k, n, err)

@@ -156,8 +156,10 @@ def _field_close(a, b, rel=2e-5):
 
 
 @pytest.mark.parametrize("n,m", [(32, 32), (64, 64), (128, 128), (64, 128)])
-@pytest.mark.parametrize("iters,omega", [(1, 1.0), (3, 1.0), (4, 0.7)])
+@pytest.mark.parametrize("iters,omega", [(1, 1.0), (3, 1.0), (4, 0.7), (3, -1.0), (6, -1.3)])
 def test_ir2s_guess_matches_oracle(cuda, n, m, iters, omega):
+    """omega < 0: the Chebyshev schedule, its bound from the crossbar's
+    summed conductance formed in the kernel (the oracle's from the mean)."""
     from oracle.drops import ir2s_drops
     B = 3
     g, r, vin = config2_instances(B, n, m, seed=7 * n + m + iters)
@@ -214,6 +216,8 @@ def test_ir2s_bad_arguments(cuda):
         _solve(cuda, g, r, vin, ir2s_iters=65)
     with pytest.raises(RuntimeError):
         _solve(cuda, g, r, vin, ir2s_iters=2, ir2s_omega=1.5)
+    with pytest.raises(RuntimeError):
+        _solve(cuda, g, r, vin, ir2s_iters=2, ir2s_omega=-4.5)
 
 
 @pytest.mark.parametrize("case", ["ir2s4", "some_sweep", "dwr", "rejected", "coarse_warm", "ir2s1_warm"])
