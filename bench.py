@@ -195,7 +195,7 @@ def bench_uneven(args, dev):
             "plan": plan.stats, "tuned": bool(plan.tuning)}
 
 
-def bench_mlp(args, dev, net, k_ir2s):
+def bench_mlp(args, dev, net, k_ir2s, w_ir2s):
     """784-256-10 crossbar MLP (8-bit, 2 bits/cell, 128x128 PN tiles, sinh
     cells a = 2.5/V): tile solves plain, IR2S-warm-started and NN-warm-started."""
     import torch
@@ -207,7 +207,8 @@ def bench_mlp(args, dev, net, k_ir2s):
     res = {}
     for name in ("plain", "ir2s", "nn") if net is not None else ("plain", "ir2s"):
         P = pim.mlp_inference_params(warm_start=net if name == "nn" else None, tol=args.tol,
-                                     ir2s_iters={"plain": 0, "ir2s": k_ir2s, "nn": 1}[name])
+                                     ir2s_iters={"plain": 0, "ir2s": k_ir2s, "nn": 1}[name],
+                                     ir2s_omega=w_ir2s if name == "ir2s" else 1.0)
         model = pim.MLP(params=P, seed=0).to(dev).eval()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -439,7 +440,7 @@ def main():
     else:
         out["cpu_baseline"] = None
     for key, flag, fn in (("uneven", args.no_uneven, lambda: bench_uneven(args, dev)),
-                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net, K))):
+                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net, K, W))):
         if flag:
             continue
         try:

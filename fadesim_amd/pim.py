@@ -160,6 +160,7 @@ class PhyArrParams:
         self.warm_start = None                # converged mode: WarmStartNet or None
         self.record = None                    # a list: every layer call appends its tile currents
         self.ir2s_iters = 0                   # converged mode: IR2S iterations on each solve's initial guess
+        self.ir2s_omega = 1.0                 # their damping in (0, 1]; negative: the Chebyshev schedule
         self.max_batch = 65536                # instances per batched solve
         self.stats = {}                       # converged mode: solves, sweeps, Picard steps (accumulated)
         for k, v in kw.items():
@@ -398,7 +399,8 @@ def _solve_converged(bits, G, inputV, P):
     r = torch.full((1,), float(P.r_wire), dtype=torch.float64, device=dev)
     net = P.warm_start
     k = int(P.ir2s_iters)
-    ir2s = {"ir2s_iters": k, "corr_iters": max(2, k), "test_first": True} if k > 0 else {}
+    ir2s = {"ir2s_iters": k, "corr_iters": max(2, k), "ir2s_omega": float(P.ir2s_omega),
+            "test_first": True} if k > 0 else {}
     stats = P.stats  # accumulates over calls until the caller resets it
     for k in ("solves", "sweeps", "picard_steps", "calls"):
         stats.setdefault(k, 0)
