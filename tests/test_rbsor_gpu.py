@@ -210,6 +210,24 @@ def test_ir2s_converged_vs_exact(cuda, n):
     assert _relerr(slow.iout.cpu().numpy(), iex) <= 1e-6
 
 
+
+def test_benchmarked_ir2s_schedule_vs_exact(cuda):
+    """The IR2S step the bench times -- the tuned count and damping for
+    128x128 (the Chebyshev schedule), tested before the first sweep -- on 64
+    config-2 crossbars: every result within 1e-6 of the exact solve, and the
+    schedule's guess alone certifies most of them (no sweep)."""
+    import torch
+    from fadesim_amd import autotune
+    t = (autotune.load_table(torch.device(cuda)) or {}).get("128", {})
+    K, W = int(t.get("ir2s_iters", 4)) or 4, float(t.get("ir2s_omega", 1.0)) or 1.0
+    B = 64
+    g, r, vin = config2_instances(B, 128, seed=2025)
+    iex = oracle.exact_batch(g, r, vin)
+    res = _solve(cuda, g, r, vin, tol=1e-6, ir2s_iters=K, corr_iters=K, ir2s_omega=W, test_first=True)
+    assert (res.status.cpu().numpy() == 0).all()
+    assert _relerr(res.iout.cpu().numpy(), iex) <= 1e-6
+    assert (res.sweeps.cpu().numpy() == 0).mean() >= 0.9, res.sweeps.cpu().numpy()
+
 def test_ir2s_bad_arguments(cuda):
     g, r, vin = config2_instances(1, 32, seed=1)
     with pytest.raises(ValueError):
