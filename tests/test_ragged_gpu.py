@@ -150,6 +150,22 @@ def test_ir2s_damped_converged_mixed_vs_exact(cuda):
     assert res.sweeps.double().sum().item() < 0.5 * base.sweeps.double().sum().item()
 
 
+
+def test_shipped_tuning_table_converged_vs_exact(cuda):
+    """The plan with the committed table for this GPU (the Chebyshev-scheduled
+    guesses: 2 / 3 / 5 / 8 iterations by size on gfx950): every crossbar of a
+    mixed batch within 1e-6 of the exact solve."""
+    from fadesim_amd.ragged import RaggedPlan
+    inst = _mixed(cuda, [512, 64, 256, 128, 512, 256, 128, 64], seed=29)
+    plan = RaggedPlan(inst["rows"], device=cuda)
+    res = plan.solve(inst["g"], inst["r"], inst["vin"], tol=1e-6)
+    assert (res.status.cpu().numpy() == 0).all()
+    iouts = plan.unpack(res.iout, "cols")
+    for k, n in enumerate(inst["rows"]):
+        _, _, iex, _ = line_solve(inst["g_list"][k], inst["r_np"][k], inst["vin_list"][k])
+        err = np.max(np.abs(iouts[k] - iex) / np.abs(iex))
+        assert err <= 1e-6, (k, n, err)
+
 def test_strip_exchange_with_every_cu_occupied(cuda):
     """48 crossbars of 512x512 and 64 of 256x256 among 128x128 and 64x64 ones:
     more team tasks than the grid has teams, so every workgroup of the
