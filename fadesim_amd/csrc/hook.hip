@@ -222,6 +222,162 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
         atomicOr(A.moved, 1u);
 }
 
+// FP32 linear whole-tile units of at most kLdsMax x kLdsMax cells: the same
+// sweeps, operation by operation, with both fields in LDS for every sweep of
+// the launch (row stride cols + 1, so the wordline walk -- one lane per row --
+// and the bitline walk -- one lane per column -- are free of bank conflicts)
+// and the conductance read from global memory a block of kLdsAhead steps
+// ahead of the walk: in the bitline walk from the tile itself (lanes on
+// consecutive columns), in the wordline walk from a transposed copy of the
+// tile made on the first launch (lanes on consecutive rows).  The fields go
+// to the workspace at the end of the launch (and come back at the start of
+// the next one in break mode).
+constexpr int kLdsMax = 128;
+constexpr int kLdsThreads = 128;
+constexpr int kLdsAhead = 16;
+
+__global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A, float *__restrict__ GTall)
+{
+    __shared__ float su[kLdsMax * (kLdsMax + 1)];
+    __shared__ float sd[kLdsMax * (kLdsMax + 1)];
+    const int xb = blockIdx.x;
+    const HookUnit<float> U = hook_unit(A, xb);
+    const int x = U.x, y = U.y;
+    const int row = U.rows, col = U.cols, ld = col + 1, t = threadIdx.x;
+    const size_t cells = (size_t)row * col;
+    float *u = A.u + (size_t)xb * cells;
+    float *d = A.d + (size_t)xb * cells;
+    const float *in = A.input + ((size_t)U.b * A.numX + x) * A.row;
+    // G(i, c) = Gt[i * gstride + c]; its transposed copy gT[c * row + i]
+    const size_t gstride = (size_t)A.numY * A.col;
+    const float *__restrict__ Gt = A.G + (size_t)x * A.row * gstride + (size_t)y * A.col;
+    float *__restrict__ gT = GTall + (size_t)xb * cells;
+    const float gl = U.gl, gd = U.gd, gw = A.gw, be = A.be, omb = A.omb, thr = A.thr;
+
+    if (A.first_sweep) {
+        for (size_t k = t; k < cells; k += kLdsThreads) {
+            const int i = (int)(k / col), c = (int)(k % col);
+            su[i * ld + c] = in[i];
+            sd[i * ld + c] = 0.f;
+        }
+        for (size_t k = t; k < cells; k += kLdsThreads) {
+            const int c = (int)(k / row), i = (int)(k % row);
+            gT[k] = Gt[(size_t)i * gstride + c];
+        }
+    } else {
+        for (size_t k = t; k < cells; k += kLdsThreads) {
+            const int i = (int)(k / col), c = (int)(k % col);
+            su[i * ld + c] = u[k];
+            sd[i * ld + c] = d[k];
+        }
+    }
+    __syncthreads(); // the fields, and this workgroup's transposed conductance
+
+    bool moved = false;
+    for (int s = 0; s < A.sweeps_this_launch; ++s) {
+        // wordlines, left -> right (split_arr_matmul.cu:75-119)
+        if (t < row) {
+            const int i = t;
+            float *sur = su + i * ld;
+            const float *sdr = sd + i * ld;
+            const float *gti = gT + i; // G(i, j) = gti[j * row]
+            float left = in[i];
+            float old = sur[0];
+            const float G0 = gti[0];
+            float sumg = gw + gl + G0;
+            float nv = be * (gl * left + gw * sur[1] + G0 * sdr[0]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > thr;
+            sur[0] = nv;
+            left = nv;
+            float gc[kLdsAhead], gn[kLdsAhead];
+#pragma unroll
+            for (int k = 0; k < kLdsAhead; ++k)
+                gc[k] = 1 + k < col - 1 ? gti[(size_t)(1 + k) * row] : 0.f;
+            for (int j0 = 1; j0 < col - 1; j0 += kLdsAhead) {
+#pragma unroll
+                for (int k = 0; k < kLdsAhead; ++k)
+                    gn[k] = j0 + kLdsAhead + k < col - 1 ? gti[(size_t)(j0 + kLdsAhead + k) * row] : 0.f;
+#pragma unroll
+                for (int k = 0; k < kLdsAhead; ++k) {
+                    const int j = j0 + k;
+                    if (j < col - 1) {
+                        old = sur[j];
+                        const float Gj = gc[k];
+                        sumg = gw * 2.f + Gj;
+                        nv = be * (gw * (left + sur[j + 1]) + Gj * sdr[j]) / sumg + omb * old;
+                        moved |= dev_abs(nv - old) > thr;
+                        sur[j] = nv;
+                        left = nv;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kLdsAhead; ++k)
+                    gc[k] = gn[k];
+            }
+            const int j = col - 1;
+            old = sur[j];
+            const float Gj = gti[(size_t)j * row];
+            sumg = gw + Gj;
+            nv = be * (gw * left + Gj * sdr[j]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > thr;
+            sur[j] = nv;
+        }
+        __syncthreads();
+        // bitlines, bottom -> top (split_arr_matmul.cu:27-73)
+        if (t < col) {
+            const int c = t;
+            const float *gcol = Gt + c; // G(i, c) = gcol[i * gstride]
+            int i = row - 1;
+            float old = sd[i * ld + c];
+            float Gi = gcol[(size_t)i * gstride];
+            float sumg = gw + gd + Gi;
+            float nv = be * (gw * sd[(i - 1) * ld + c] + gd * 0.f + Gi * su[i * ld + c]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > thr;
+            sd[i * ld + c] = nv;
+            float down = nv;
+            float gc[kLdsAhead], gn[kLdsAhead];
+#pragma unroll
+            for (int k = 0; k < kLdsAhead; ++k)
+                gc[k] = row - 2 - k > 0 ? gcol[(size_t)(row - 2 - k) * gstride] : 0.f;
+            for (int i0 = row - 2; i0 > 0; i0 -= kLdsAhead) {
+#pragma unroll
+                for (int k = 0; k < kLdsAhead; ++k)
+                    gn[k] = i0 - kLdsAhead - k > 0 ? gcol[(size_t)(i0 - kLdsAhead - k) * gstride] : 0.f;
+#pragma unroll
+                for (int k = 0; k < kLdsAhead; ++k) {
+                    i = i0 - k;
+                    if (i > 0) {
+                        old = sd[i * ld + c];
+                        Gi = gc[k];
+                        sumg = gw * 2.f + Gi;
+                        nv = be * (gw * (sd[(i - 1) * ld + c] + down) + Gi * su[i * ld + c]) / sumg + omb * old;
+                        moved |= dev_abs(nv - old) > thr;
+                        sd[i * ld + c] = nv;
+                        down = nv;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kLdsAhead; ++k)
+                    gc[k] = gn[k];
+            }
+            old = sd[c];
+            Gi = gcol[0];
+            sumg = gw + Gi;
+            nv = be * (gw * down + Gi * su[c]) / sumg + omb * old;
+            moved |= dev_abs(nv - old) > thr;
+            sd[c] = nv;
+        }
+        __syncthreads();
+    }
+    for (size_t k = t; k < cells; k += kLdsThreads) {
+        const int i = (int)(k / col), c = (int)(k % col);
+        u[k] = su[i * ld + c];
+        d[k] = sd[i * ld + c];
+    }
+    if (A.want_moved && moved)
+        atomicOr(A.moved, 1u);
+}
+
 // Readout: out[b][x][y*col + c] = sum_i (u - d) G, or the voltage differences;
 // OU mode: out[b][x][ou_x][y*col + c0 + c], the current of each OU.
 template <typename T>
@@ -263,7 +419,10 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     const int grid = D->batch_size * D->numX * D->numY * nOuX * nOuY;
     const size_t cells = ou ? (size_t)D->ou_rows * D->ou_cols : (size_t)D->row_size * D->col_size;
     const size_t nfield = (size_t)grid * cells;
-    const size_t planes = D->nonlinear ? 4 : 2;
+    // FP32 linear whole tiles that fit: hook_lds_kernel (one more plane, the
+    // transposed conductance)
+    const bool lds = sizeof(T) == 4 && !D->nonlinear && !ou && D->row_size <= kLdsMax && D->col_size <= kLdsMax;
+    const size_t planes = D->nonlinear ? 4 : lds ? 3 : 2;
     char *ws = (char *)workspace(planes * nfield * sizeof(T) + 256);
     if (!ws)
         return FADESIM_E_NOMEM;
@@ -293,12 +452,23 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     A.Gc = D->nonlinear ? A.d + nfield : nullptr;
     A.Gn = D->nonlinear ? A.Gc + nfield : nullptr;
     A.moved = (unsigned *)(ws + planes * nfield * sizeof(T));
+    T *GT = lds ? A.d + nfield : nullptr;
+    // one launch of `sweeps` sweeps
+    auto sweep_launch = [&]() {
+        if constexpr (sizeof(T) == 4) {
+            if (lds) {
+                hipLaunchKernelGGL(hook_lds_kernel, dim3(grid), dim3(kLdsThreads), 0, s, A, GT);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+    };
     int done = 0;
     if (!D->enable_break) {
         A.sweeps_this_launch = D->iter_times;
         A.first_sweep = 1;
         A.want_moved = 0;
-        hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+        sweep_launch();
         FADESIM_HIP_CHECK(hipGetLastError());
         done = D->iter_times;
     } else {
@@ -309,7 +479,7 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
         for (int it = 0; it < D->iter_times; ++it) {
             A.first_sweep = (it == 0);
             hipLaunchKernelGGL(set_u32, dim3(1), dim3(1), 0, s, A.moved, 0u);
-            hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+            sweep_launch();
             FADESIM_HIP_CHECK(hipGetLastError());
             unsigned h = 0;
             FADESIM_HIP_CHECK(hipMemcpyAsync(&h, A.moved, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -324,7 +494,7 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
         A.sweeps_this_launch = 0;
         A.first_sweep = 1;
         A.want_moved = 0;
-        hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+        sweep_launch();
         FADESIM_HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL((hook_readout<T>), dim3(grid), dim3(128), 0, s, A, (T *)D->d_out, D->ret_v);
