@@ -30,10 +30,13 @@ Extra objects in the same line (not part of `value`): "uneven" -- 1024
 crossbars each of 64x64, 128x128, 256x256 and 512x512 (shuffled, seed 33)
 solved in one persistent launch and as a captured graph replay; "mlp" -- the
 784-256-10 crossbar MLP on 1024 images (sinh cells, every tile solve
-batched), plain vs IR2S- and NN-warm-started tile solves.
+batched), plain vs IR2S- and NN-warm-started tile solves; "hook" -- the
+PimTorch hook drop-in with the reference's own iteration (256 FP32 tiles of
+128x128, 600 sweeps), beside the oracle's restatement of that iteration on
+one core (bit for bit the same results).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
-                [--no-uneven] [--no-mlp] [--mlp-images N] [--only plain,ir2s,nn]
+                [--no-uneven] [--no-mlp] [--no-hook] [--mlp-images N] [--only plain,ir2s,nn]
 """
 from __future__ import annotations
 
@@ -195,6 +198,43 @@ def bench_uneven(args, dev):
             "plan": plan.stats, "tuned": bool(plan.tuning)}
 
 
+def bench_hook(args, dev):
+    """The PimTorch hook drop-in (ir_drop_gs) with the reference's own
+    iteration: 256 FP32 tiles of 128x128 (16 inputs x 4 x 4 tiles), 600
+    sweeps at beta 1.90 (PimTorch's defaults), against the oracle's C
+    restatement of the same iteration on one core (bit for bit the same)."""
+    import torch
+    from fadesim_amd import hook
+    import oracle
+    rng = np.random.default_rng(7)
+    bsize, numX, numY, n, sweeps = 16, 4, 4, 128, 600
+    G = (1.0 / rng.choice(STATES4, (numX, n, numY * n))).astype(np.float32)
+    x = np.where(rng.random((bsize, numX, n)) < 0.5, 0.2, 0.0).astype(np.float32)
+    tG, tx = torch.tensor(G, device=dev), torch.tensor(x, device=dev)
+    call = lambda: hook.ir_drop_gs(tx, tG, bsize, n, n, numX, numY, sweeps, 1 / 2.93, 1.9, False, 0.0, False, [])
+    out = call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        out = call()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    tiles = bsize * numX * numY
+    res = {"workload": f"{tiles} FP32 tiles of 128x128 (PimTorch hook ir_drop_gs), {sweeps} sweeps, beta 1.90",
+           "ms_per_call": 1e3 * dt, "tiles_per_s": tiles / dt}
+    if not args.no_cpu:
+        # CPU baseline and checker: the oracle's restatement on one input
+        # vector's 4 x 4 tiles
+        t1 = time.perf_counter()
+        ref, _ = oracle.hook_gs(x[:1], G, 1, n, n, numX, numY, sweeps, 1 / 2.93, 1.9, False, 0.0, False, [])
+        dc = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": numX * numY / dc, "unit": "tiles/s", "cores": 1, "kind": "port",
+                               "sample": f"{numX * numY} tiles (one input vector), the same iteration"}
+        res["bitwise_equal_to_cpu"] = bool(np.array_equal(out[:1].cpu().numpy(), ref))
+    return res
+
+
 def bench_mlp(args, dev, net, k_ir2s, w_ir2s):
     """784-256-10 crossbar MLP (8-bit, 2 bits/cell, 128x128 PN tiles, sinh
     cells a = 2.5/V): tile solves plain, IR2S-warm-started and NN-warm-started."""
@@ -254,6 +294,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--no-hook", action="store_true")
     ap.add_argument("--mlp-images", type=int, default=1024)
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--only", default="", help="comma list of plain,ir2s,nn,nn_3k,nn_unet16: run only these solve variants")
@@ -440,7 +481,8 @@ def main():
     else:
         out["cpu_baseline"] = None
     for key, flag, fn in (("uneven", args.no_uneven, lambda: bench_uneven(args, dev)),
-                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net, K, W))):
+                          ("mlp", args.no_mlp, lambda: bench_mlp(args, dev, net, K, W)),
+                          ("hook", args.no_hook, lambda: bench_hook(args, dev))):
         if flag:
             continue
         try:
