@@ -289,14 +289,17 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A
             moved |= dev_abs(nv - old) > thr;
             sur[0] = nv;
             left = nv;
+            // loads of steps past the walk's end read column col - 1 and go
+            // unused: unconditional, so the compiler waits for each block's
+            // own loads only, not for the block fetched ahead
             float gc[kLdsAhead], gn[kLdsAhead];
 #pragma unroll
             for (int k = 0; k < kLdsAhead; ++k)
-                gc[k] = 1 + k < col - 1 ? gti[(size_t)(1 + k) * row] : 0.f;
+                gc[k] = gti[(size_t)min(1 + k, col - 1) * row];
             for (int j0 = 1; j0 < col - 1; j0 += kLdsAhead) {
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k)
-                    gn[k] = j0 + kLdsAhead + k < col - 1 ? gti[(size_t)(j0 + kLdsAhead + k) * row] : 0.f;
+                    gn[k] = gti[(size_t)min(j0 + kLdsAhead + k, col - 1) * row];
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k) {
                     const int j = j0 + k;
@@ -338,11 +341,11 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A
             float gc[kLdsAhead], gn[kLdsAhead];
 #pragma unroll
             for (int k = 0; k < kLdsAhead; ++k)
-                gc[k] = row - 2 - k > 0 ? gcol[(size_t)(row - 2 - k) * gstride] : 0.f;
+                gc[k] = gcol[(size_t)max(row - 2 - k, 0) * gstride];
             for (int i0 = row - 2; i0 > 0; i0 -= kLdsAhead) {
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k)
-                    gn[k] = i0 - kLdsAhead - k > 0 ? gcol[(size_t)(i0 - kLdsAhead - k) * gstride] : 0.f;
+                    gn[k] = gcol[(size_t)max(i0 - kLdsAhead - k, 0) * gstride];
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k) {
                     i = i0 - k;
