@@ -205,7 +205,6 @@ def bench_hook(args, dev):
     restatement of the same iteration on one core (bit for bit the same)."""
     import torch
     from fadesim_amd import hook
-    import oracle
     rng = np.random.default_rng(7)
     bsize, numX, numY, n, sweeps = 16, 4, 4, 128, 600
     G = (1.0 / rng.choice(STATES4, (numX, n, numY * n))).astype(np.float32)
@@ -226,6 +225,7 @@ def bench_hook(args, dev):
     if not args.no_cpu:
         # CPU baseline and checker: the oracle's restatement on one input
         # vector's 4 x 4 tiles
+        import oracle
         t1 = time.perf_counter()
         ref, _ = oracle.hook_gs(x[:1], G, 1, n, n, numX, numY, sweeps, 1 / 2.93, 1.9, False, 0.0, False, [])
         dc = time.perf_counter() - t1
