@@ -222,43 +222,56 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
         atomicOr(A.moved, 1u);
 }
 
-// FP32 linear whole-tile units of at most kLdsMax x kLdsMax cells: the same
-// sweeps, operation by operation, with both fields in LDS for every sweep of
-// the launch (row stride cols + 1, so the wordline walk -- one lane per row --
-// and the bitline walk -- one lane per column -- are free of bank conflicts)
-// and the conductance read from global memory a block of kLdsAhead steps
-// ahead of the walk: in the bitline walk from the tile itself (lanes on
-// consecutive columns), in the wordline walk from a transposed copy of the
-// tile made on the first launch (lanes on consecutive rows).  The fields go
-// to the workspace at the end of the launch (and come back at the start of
-// the next one in break mode).
+// Linear whole-tile units of at most kLdsMax x kLdsMax cells: the same
+// sweeps, operation by operation, with the wordline field in LDS for every
+// sweep of the launch and, in FP32, the bitline field too (FP64: 132 KB for
+// one field, so the bitline field stays in the workspace).  LDS rows have
+// stride cols + 1, so the wordline walk -- one lane per row -- and the
+// bitline walk -- one lane per column -- are free of bank conflicts.
+// Everything a step reads from global memory -- the conductance (in the
+// bitline walk from the tile, lanes on consecutive columns; in the wordline
+// walk from a transposed copy made on the first launch, lanes on
+// consecutive rows) and, in FP64, the bitline field's old values -- is
+// loaded a block of kLdsAhead steps ahead.  The fields go to the workspace at
+// the end of the launch (and come back at the start of the next one in break
+// mode).
 constexpr int kLdsMax = 128;
 constexpr int kLdsThreads = 128;
 constexpr int kLdsAhead = 16;
 
-__global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A, float *__restrict__ GTall)
+template <typename T>
+__global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T *__restrict__ GTall)
 {
-    __shared__ float su[kLdsMax * (kLdsMax + 1)];
-    __shared__ float sd[kLdsMax * (kLdsMax + 1)];
+    constexpr bool DL = sizeof(T) == 4; // the bitline field in LDS too
+    __shared__ T su[kLdsMax * (kLdsMax + 1)];
+    __shared__ T sd[DL ? kLdsMax * (kLdsMax + 1) : 1];
     const int xb = blockIdx.x;
-    const HookUnit<float> U = hook_unit(A, xb);
+    const HookUnit<T> U = hook_unit(A, xb);
     const int x = U.x, y = U.y;
     const int row = U.rows, col = U.cols, ld = col + 1, t = threadIdx.x;
     const size_t cells = (size_t)row * col;
-    float *u = A.u + (size_t)xb * cells;
-    float *d = A.d + (size_t)xb * cells;
-    const float *in = A.input + ((size_t)U.b * A.numX + x) * A.row;
+    T *u = A.u + (size_t)xb * cells;
+    T *d = A.d + (size_t)xb * cells;
+    const T *in = A.input + ((size_t)U.b * A.numX + x) * A.row;
     // G(i, c) = Gt[i * gstride + c]; its transposed copy gT[c * row + i]
     const size_t gstride = (size_t)A.numY * A.col;
-    const float *__restrict__ Gt = A.G + (size_t)x * A.row * gstride + (size_t)y * A.col;
-    float *__restrict__ gT = GTall + (size_t)xb * cells;
-    const float gl = U.gl, gd = U.gd, gw = A.gw, be = A.be, omb = A.omb, thr = A.thr;
+    const T *__restrict__ Gt = A.G + (size_t)x * A.row * gstride + (size_t)y * A.col;
+    T *__restrict__ gT = GTall + (size_t)xb * cells;
+    const T gl = U.gl, gd = U.gd, gw = A.gw, be = A.be, omb = A.omb, thr = A.thr;
+    // the bitline field: LDS (FP32) or the workspace (FP64)
+    auto dget = [&](int i, int c) -> T { return DL ? sd[i * ld + c] : d[(size_t)i * col + c]; };
+    auto dput = [&](int i, int c, T v) {
+        if constexpr (DL)
+            sd[i * ld + c] = v;
+        else
+            d[(size_t)i * col + c] = v;
+    };
 
     if (A.first_sweep) {
         for (size_t k = t; k < cells; k += kLdsThreads) {
             const int i = (int)(k / col), c = (int)(k % col);
             su[i * ld + c] = in[i];
-            sd[i * ld + c] = 0.f;
+            dput(i, c, (T)0);
         }
         for (size_t k = t; k < cells; k += kLdsThreads) {
             const int c = (int)(k / row), i = (int)(k % row);
@@ -268,7 +281,8 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A
         for (size_t k = t; k < cells; k += kLdsThreads) {
             const int i = (int)(k / col), c = (int)(k % col);
             su[i * ld + c] = u[k];
-            sd[i * ld + c] = d[k];
+            if constexpr (DL)
+                sd[i * ld + c] = d[k];
         }
     }
     __syncthreads(); // the fields, and this workgroup's transposed conductance
@@ -278,50 +292,61 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A
         // wordlines, left -> right (split_arr_matmul.cu:75-119)
         if (t < row) {
             const int i = t;
-            float *sur = su + i * ld;
-            const float *sdr = sd + i * ld;
-            const float *gti = gT + i; // G(i, j) = gti[j * row]
-            float left = in[i];
-            float old = sur[0];
-            const float G0 = gti[0];
-            float sumg = gw + gl + G0;
-            float nv = be * (gl * left + gw * sur[1] + G0 * sdr[0]) / sumg + omb * old;
+            T *sur = su + i * ld;
+            const T *gti = gT + i; // G(i, j) = gti[j * row]
+            T left = in[i];
+            T old = sur[0];
+            const T G0 = gti[0];
+            T sumg = gw + gl + G0;
+            T nv = be * (gl * left + gw * sur[1] + G0 * dget(i, 0)) / sumg + omb * old;
             moved |= dev_abs(nv - old) > thr;
             sur[0] = nv;
             left = nv;
-            // loads of steps past the walk's end read column col - 1 and go
+            // a block's loads past the walk's end reread node col - 1 and go
             // unused: unconditional, so the compiler waits for each block's
             // own loads only, not for the block fetched ahead
-            float gc[kLdsAhead], gn[kLdsAhead];
+            T gc[kLdsAhead], gn[kLdsAhead], dc[DL ? 1 : kLdsAhead], dn[DL ? 1 : kLdsAhead];
 #pragma unroll
-            for (int k = 0; k < kLdsAhead; ++k)
-                gc[k] = gti[(size_t)min(1 + k, col - 1) * row];
+            for (int k = 0; k < kLdsAhead; ++k) {
+                const int j = min(1 + k, col - 1);
+                gc[k] = gti[(size_t)j * row];
+                if constexpr (!DL)
+                    dc[k] = dget(i, j);
+            }
             for (int j0 = 1; j0 < col - 1; j0 += kLdsAhead) {
 #pragma unroll
-                for (int k = 0; k < kLdsAhead; ++k)
-                    gn[k] = gti[(size_t)min(j0 + kLdsAhead + k, col - 1) * row];
+                for (int k = 0; k < kLdsAhead; ++k) {
+                    const int j = min(j0 + kLdsAhead + k, col - 1);
+                    gn[k] = gti[(size_t)j * row];
+                    if constexpr (!DL)
+                        dn[k] = dget(i, j);
+                }
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k) {
                     const int j = j0 + k;
                     if (j < col - 1) {
                         old = sur[j];
-                        const float Gj = gc[k];
-                        sumg = gw * 2.f + Gj;
-                        nv = be * (gw * (left + sur[j + 1]) + Gj * sdr[j]) / sumg + omb * old;
+                        const T Gj = gc[k];
+                        const T dj = DL ? dget(i, j) : dc[DL ? 0 : k];
+                        sumg = gw * (T)2 + Gj;
+                        nv = be * (gw * (left + sur[j + 1]) + Gj * dj) / sumg + omb * old;
                         moved |= dev_abs(nv - old) > thr;
                         sur[j] = nv;
                         left = nv;
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < kLdsAhead; ++k)
+                for (int k = 0; k < kLdsAhead; ++k) {
                     gc[k] = gn[k];
+                    if constexpr (!DL)
+                        dc[k] = dn[k];
+                }
             }
             const int j = col - 1;
             old = sur[j];
-            const float Gj = gti[(size_t)j * row];
+            const T Gj = gti[(size_t)j * row];
             sumg = gw + Gj;
-            nv = be * (gw * left + Gj * sdr[j]) / sumg + omb * old;
+            nv = be * (gw * left + Gj * dget(i, j)) / sumg + omb * old;
             moved |= dev_abs(nv - old) > thr;
             sur[j] = nv;
         }
@@ -329,53 +354,68 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<float> A
         // bitlines, bottom -> top (split_arr_matmul.cu:27-73)
         if (t < col) {
             const int c = t;
-            const float *gcol = Gt + c; // G(i, c) = gcol[i * gstride]
+            const T *gcol = Gt + c; // G(i, c) = gcol[i * gstride]
             int i = row - 1;
-            float old = sd[i * ld + c];
-            float Gi = gcol[(size_t)i * gstride];
-            float sumg = gw + gd + Gi;
-            float nv = be * (gw * sd[(i - 1) * ld + c] + gd * 0.f + Gi * su[i * ld + c]) / sumg + omb * old;
+            T old = dget(i, c);
+            T up = dget(i - 1, c);
+            T Gi = gcol[(size_t)i * gstride];
+            T sumg = gw + gd + Gi;
+            T nv = be * (gw * up + gd * (T)0 + Gi * su[i * ld + c]) / sumg + omb * old;
             moved |= dev_abs(nv - old) > thr;
-            sd[i * ld + c] = nv;
-            float down = nv;
-            float gc[kLdsAhead], gn[kLdsAhead];
+            dput(i, c, nv);
+            T down = nv;
+            // old value of the next node up: the up value just read
+            T gc[kLdsAhead], gn[kLdsAhead], uc[DL ? 1 : kLdsAhead], un[DL ? 1 : kLdsAhead];
 #pragma unroll
-            for (int k = 0; k < kLdsAhead; ++k)
-                gc[k] = gcol[(size_t)max(row - 2 - k, 0) * gstride];
+            for (int k = 0; k < kLdsAhead; ++k) {
+                const int r = max(row - 2 - k, 0);
+                gc[k] = gcol[(size_t)r * gstride];
+                if constexpr (!DL)
+                    uc[k] = dget(max(r - 1, 0), c); // the old value above node r
+            }
             for (int i0 = row - 2; i0 > 0; i0 -= kLdsAhead) {
 #pragma unroll
-                for (int k = 0; k < kLdsAhead; ++k)
-                    gn[k] = gcol[(size_t)max(i0 - kLdsAhead - k, 0) * gstride];
+                for (int k = 0; k < kLdsAhead; ++k) {
+                    const int r = max(i0 - kLdsAhead - k, 0);
+                    gn[k] = gcol[(size_t)r * gstride];
+                    if constexpr (!DL)
+                        un[k] = dget(max(r - 1, 0), c);
+                }
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k) {
                     i = i0 - k;
                     if (i > 0) {
-                        old = sd[i * ld + c];
+                        old = up; // d(i, c) before this sweep wrote it
+                        up = DL ? dget(i - 1, c) : uc[DL ? 0 : k];
                         Gi = gc[k];
-                        sumg = gw * 2.f + Gi;
-                        nv = be * (gw * (sd[(i - 1) * ld + c] + down) + Gi * su[i * ld + c]) / sumg + omb * old;
+                        sumg = gw * (T)2 + Gi;
+                        nv = be * (gw * (up + down) + Gi * su[i * ld + c]) / sumg + omb * old;
                         moved |= dev_abs(nv - old) > thr;
-                        sd[i * ld + c] = nv;
+                        dput(i, c, nv);
                         down = nv;
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < kLdsAhead; ++k)
+                for (int k = 0; k < kLdsAhead; ++k) {
                     gc[k] = gn[k];
+                    if constexpr (!DL)
+                        uc[k] = un[k];
+                }
             }
-            old = sd[c];
+            old = up; // d(0, c) before this sweep
             Gi = gcol[0];
             sumg = gw + Gi;
             nv = be * (gw * down + Gi * su[c]) / sumg + omb * old;
             moved |= dev_abs(nv - old) > thr;
-            sd[c] = nv;
+            dput(0, c, nv);
         }
         __syncthreads();
     }
     for (size_t k = t; k < cells; k += kLdsThreads) {
         const int i = (int)(k / col), c = (int)(k % col);
         u[k] = su[i * ld + c];
-        d[k] = sd[i * ld + c];
+        if constexpr (DL)
+            d[k] = sd[i * ld + c];
     }
     if (A.want_moved && moved)
         atomicOr(A.moved, 1u);
@@ -422,9 +462,9 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     const int grid = D->batch_size * D->numX * D->numY * nOuX * nOuY;
     const size_t cells = ou ? (size_t)D->ou_rows * D->ou_cols : (size_t)D->row_size * D->col_size;
     const size_t nfield = (size_t)grid * cells;
-    // FP32 linear whole tiles that fit: hook_lds_kernel (one more plane, the
+    // linear whole tiles that fit: hook_lds_kernel (one more plane, the
     // transposed conductance)
-    const bool lds = sizeof(T) == 4 && !D->nonlinear && !ou && D->row_size <= kLdsMax && D->col_size <= kLdsMax;
+    const bool lds = !D->nonlinear && !ou && D->row_size <= kLdsMax && D->col_size <= kLdsMax;
     const size_t planes = D->nonlinear ? 4 : lds ? 3 : 2;
     char *ws = (char *)workspace(planes * nfield * sizeof(T) + 256);
     if (!ws)
@@ -458,13 +498,10 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     T *GT = lds ? A.d + nfield : nullptr;
     // one launch of `sweeps` sweeps
     auto sweep_launch = [&]() {
-        if constexpr (sizeof(T) == 4) {
-            if (lds) {
-                hipLaunchKernelGGL(hook_lds_kernel, dim3(grid), dim3(kLdsThreads), 0, s, A, GT);
-                return;
-            }
-        }
-        hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+        if (lds)
+            hipLaunchKernelGGL((hook_lds_kernel<T>), dim3(grid), dim3(kLdsThreads), 0, s, A, GT);
+        else
+            hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
     };
     int done = 0;
     if (!D->enable_break) {

@@ -37,18 +37,21 @@ def test_hook_linear_bitwise(cuda, dtype, iters, brk, retv):
 
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("row,col,iters,brk", [(128, 128, 60, False), (128, 96, 25, False), (100, 128, 400, True),
                                                (2, 128, 9, False), (128, 2, 9, False)])
-def test_hook_linear_bitwise_full_tiles(cuda, row, col, iters, brk):
-    """FP32 tiles up to 128x128 run with both fields in LDS (hook_lds_kernel):
-    bit for bit the CPU restatement, also at two rows or two columns."""
+def test_hook_linear_bitwise_full_tiles(cuda, dtype, row, col, iters, brk):
+    """Linear tiles up to 128x128 run with the wordline field in LDS (FP32: the
+    bitline field too; hook_lds_kernel): bit for bit the CPU restatement, also
+    at two rows or two columns."""
     import torch
     from fadesim_amd.hook import ir_drop_gs
     bsize, numX, numY = 2, 2, 1
-    inp, G = _case(np.float32, bsize, row, col, numX, numY, seed=row + col + iters)
+    inp, G = _case(dtype, bsize, row, col, numX, numY, seed=row + col + iters)
+    thr = 1e-9 if dtype == np.float64 else 1e-7
     out = ir_drop_gs(torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda), bsize, row, col, numX,
-                     numY, iters, 1 / 2.93, 1.9, brk, 1e-7, False, [])
-    ref, _ = oracle.hook_gs(inp, G, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.9, brk, 1e-7, False, [])
+                     numY, iters, 1 / 2.93, 1.9, brk, thr, False, [])
+    ref, _ = oracle.hook_gs(inp, G, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.9, brk, thr, False, [])
     assert np.array_equal(out.cpu().numpy(), ref)
 
 @pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-12), (np.float32, 2e-5)])
