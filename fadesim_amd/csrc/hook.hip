@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
         atomicOr(A.moved, 1u);
 }
 
-// Linear whole-tile units of at most kLdsMax x kLdsMax cells: the same
+// Whole-tile units of at most kLdsMax x kLdsMax cells: the same
 // sweeps, operation by operation, with the wordline field in LDS for every
 // sweep of the launch and, in FP32, the bitline field too (FP64: 132 KB for
 // one field, so the bitline field stays in the workspace).  LDS rows have
@@ -236,12 +236,16 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
 // the end of the launch (and come back at the start of the next one in break
 // mode).
 constexpr int kLdsMax = 128;
-constexpr int kLdsThreads = 128;
 constexpr int kLdsAhead = 16;
 
-template <typename T>
-__global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T *__restrict__ GTall)
+// NT threads: 128 (a lane per row or column of the walks) for linear cells;
+// 256 for sinh cells, whose per-sweep conductance update over every cell
+// then runs on four waves
+template <typename T, int NT, bool NL>
+__global__ void __launch_bounds__(NT) hook_lds_kernel(HookArgs<T> A, T *__restrict__ GTall, T *__restrict__ NTall,
+                                                      T *__restrict__ Zall)
 {
+    constexpr int kLdsThreads = NT;
     constexpr bool DL = sizeof(T) == 4; // the bitline field in LDS too
     __shared__ T su[kLdsMax * (kLdsMax + 1)];
     __shared__ T sd[DL ? kLdsMax * (kLdsMax + 1) : 1];
@@ -287,13 +291,59 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T 
     }
     __syncthreads(); // the fields, and this workgroup's transposed conductance
 
+    // Sinh cells (one launch for all sweeps): F_s, the conductance of the
+    // state at the start of sweep s, goes to buffer s & 1 (natural layout:
+    // A.Gc, A.Gn; transposed: NTall's two planes); sweep s walks with
+    // F_{max(s-1, 0)} -- the reference's two-sweeps-back conductance
+    // (split_arr_matmul.cu:241-262): its kernel reads the buffer it is about
+    // to overwrite.  z of each cell (its linear conductance's place between
+    // Ron and Roff) is computed once, by the reference's expression.
+    constexpr bool nl = NL;
+    T *Fn[2] = {nullptr, nullptr}, *FT[2] = {nullptr, nullptr}, *zc = nullptr;
+    if constexpr (nl) {
+        Fn[0] = A.Gc + (size_t)xb * cells;
+        Fn[1] = A.Gn + (size_t)xb * cells;
+        FT[0] = NTall + (size_t)xb * cells;
+        FT[1] = NTall + ((size_t)gridDim.x + xb) * cells;
+        zc = Zall + (size_t)xb * cells;
+        for (size_t k = t; k < cells; k += kLdsThreads) {
+            const int i = (int)(k / col), c = (int)(k % col);
+            const T g = Gt[(size_t)i * gstride + c];
+            T z = ((T)1 / g - (T)A.nl[0]) / (T)(A.nl[1] - A.nl[0]);
+            if (z < (T)0)
+                z = (T)0;
+            zc[k] = z;
+        }
+    }
+    // F of the present state into buffer b, both layouts
+    auto nonlinear_update = [&](int b) {
+        for (size_t k = t; k < cells; k += kLdsThreads) {
+            const int i = (int)(k / col), c = (int)(k % col);
+            const T v = nonlinear_g<T>(su[i * ld + c] - dget(i, c), zc[k], A.nl);
+            Fn[b][k] = v;
+            FT[b][(size_t)c * row + i] = v;
+        }
+    };
+
     bool moved = false;
     for (int s = 0; s < A.sweeps_this_launch; ++s) {
+        // the walks' conductance: linear, or F_{max(s-1, 0)} of sinh cells
+        const T *__restrict__ gnat = Gt;
+        const T *__restrict__ gtr = gT;
+        size_t gst = gstride;
+        if constexpr (nl) {
+            nonlinear_update(s & 1);
+            __syncthreads();
+            const int b = s > 0 ? (s - 1) & 1 : 0;
+            gnat = Fn[b];
+            gtr = FT[b];
+            gst = (size_t)col;
+        }
         // wordlines, left -> right (split_arr_matmul.cu:75-119)
         if (t < row) {
             const int i = t;
             T *sur = su + i * ld;
-            const T *gti = gT + i; // G(i, j) = gti[j * row]
+            const T *gti = gtr + i; // G(i, j) = gti[j * row]
             T left = in[i];
             T old = sur[0];
             const T G0 = gti[0];
@@ -354,11 +404,11 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T 
         // bitlines, bottom -> top (split_arr_matmul.cu:27-73)
         if (t < col) {
             const int c = t;
-            const T *gcol = Gt + c; // G(i, c) = gcol[i * gstride]
+            const T *gcol = gnat + c; // G(i, c) = gcol[i * gst]
             int i = row - 1;
             T old = dget(i, c);
             T up = dget(i - 1, c);
-            T Gi = gcol[(size_t)i * gstride];
+            T Gi = gcol[(size_t)i * gst];
             T sumg = gw + gd + Gi;
             T nv = be * (gw * up + gd * (T)0 + Gi * su[i * ld + c]) / sumg + omb * old;
             moved |= dev_abs(nv - old) > thr;
@@ -369,7 +419,7 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T 
 #pragma unroll
             for (int k = 0; k < kLdsAhead; ++k) {
                 const int r = max(row - 2 - k, 0);
-                gc[k] = gcol[(size_t)r * gstride];
+                gc[k] = gcol[(size_t)r * gst];
                 if constexpr (!DL)
                     uc[k] = dget(max(r - 1, 0), c); // the old value above node r
             }
@@ -377,7 +427,7 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T 
 #pragma unroll
                 for (int k = 0; k < kLdsAhead; ++k) {
                     const int r = max(i0 - kLdsAhead - k, 0);
-                    gn[k] = gcol[(size_t)r * gstride];
+                    gn[k] = gcol[(size_t)r * gst];
                     if constexpr (!DL)
                         un[k] = dget(max(r - 1, 0), c);
                 }
@@ -416,6 +466,15 @@ __global__ void __launch_bounds__(kLdsThreads) hook_lds_kernel(HookArgs<T> A, T 
         u[k] = su[i * ld + c];
         if constexpr (DL)
             d[k] = sd[i * ld + c];
+    }
+    if constexpr (nl) {
+        // the readout takes the conductance of the last sweep from A.Gc (a
+        // call without sweeps reads out with the linear conductance)
+        const int S = A.sweeps_this_launch;
+        if (S >= 2 && ((S - 2) & 1)) {
+            for (size_t k = t; k < cells; k += kLdsThreads)
+                Fn[0][k] = Fn[1][k];
+        }
     }
     if (A.want_moved && moved)
         atomicOr(A.moved, 1u);
@@ -462,10 +521,11 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     const int grid = D->batch_size * D->numX * D->numY * nOuX * nOuY;
     const size_t cells = ou ? (size_t)D->ou_rows * D->ou_cols : (size_t)D->row_size * D->col_size;
     const size_t nfield = (size_t)grid * cells;
-    // linear whole tiles that fit: hook_lds_kernel (one more plane, the
-    // transposed conductance)
-    const bool lds = !D->nonlinear && !ou && D->row_size <= kLdsMax && D->col_size <= kLdsMax;
-    const size_t planes = D->nonlinear ? 4 : lds ? 3 : 2;
+    // whole tiles up to kLdsMax x kLdsMax: hook_lds_kernel -- one more plane
+    // (the transposed conductance), and for sinh cells three more (z and the
+    // two transposed conductance buffers); sinh cells in one launch only
+    const bool lds = !ou && D->row_size <= kLdsMax && D->col_size <= kLdsMax && !(D->nonlinear && D->enable_break);
+    const size_t planes = (D->nonlinear ? 4 : 2) + (lds ? (D->nonlinear ? 4 : 1) : 0);
     char *ws = (char *)workspace(planes * nfield * sizeof(T) + 256);
     if (!ws)
         return FADESIM_E_NOMEM;
@@ -495,11 +555,15 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     A.Gc = D->nonlinear ? A.d + nfield : nullptr;
     A.Gn = D->nonlinear ? A.Gc + nfield : nullptr;
     A.moved = (unsigned *)(ws + planes * nfield * sizeof(T));
-    T *GT = lds ? A.d + nfield : nullptr;
+    T *GT = lds ? A.u + (D->nonlinear ? 4 : 2) * nfield : nullptr;
+    T *NTp = lds && D->nonlinear ? GT + nfield : nullptr; // two planes
+    T *Z = lds && D->nonlinear ? GT + 3 * nfield : nullptr;
     // one launch of `sweeps` sweeps
     auto sweep_launch = [&]() {
-        if (lds)
-            hipLaunchKernelGGL((hook_lds_kernel<T>), dim3(grid), dim3(kLdsThreads), 0, s, A, GT);
+        if (lds && A.nonlinear)
+            hipLaunchKernelGGL((hook_lds_kernel<T, 256, true>), dim3(grid), dim3(256), 0, s, A, GT, NTp, Z);
+        else if (lds)
+            hipLaunchKernelGGL((hook_lds_kernel<T, 128, false>), dim3(grid), dim3(128), 0, s, A, GT, NTp, Z);
         else
             hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
     };
@@ -530,7 +594,10 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
         }
     }
     if (done == 0) {
-        // no sweep: the fields still need their initial state for the readout
+        // no sweep: the fields still need their initial state for the
+        // readout, which, as the reference's, uses the linear conductance
+        // (sinh cells' conductance is first formed inside a sweep)
+        A.nonlinear = 0;
         A.sweeps_this_launch = 0;
         A.first_sweep = 1;
         A.want_moved = 0;

@@ -71,6 +71,31 @@ def test_hook_nonlinear(cuda, dtype, rtol):
     np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=rtol)
 
 
+
+@pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+@pytest.mark.parametrize("row,col,iters", [(128, 128, 40), (96, 128, 7), (128, 64, 1), (64, 64, 0)])
+def test_hook_nonlinear_full_tiles(cuda, dtype, rtol, row, col, iters):
+    """Sinh cells on whole tiles in one launch (hook_lds_kernel, the
+    conductance buffers ping-ponged by sweep): against the CPU restatement to
+    sinh rounding, and bit for bit against the workspace kernel that break
+    mode runs (never breaking here)."""
+    import torch
+    from fadesim_amd.hook import ir_drop_gs
+    bsize, numX, numY = 2, 1, 2
+    rng = np.random.default_rng(row + col + iters)
+    inp = rng.uniform(0.05, 0.3, (bsize, numX, row)).astype(dtype)
+    Ron, Roff = 1e3, 1e6
+    G = (1.0 / rng.uniform(Ron, 1e4, (numX, row, numY * col))).astype(dtype)
+    a = 2.5
+    nl = [Ron, Roff, a, 1 / (Ron * np.sinh(a)), a, 1 / (Roff * np.sinh(a))]
+    ti, tG = torch.tensor(inp, device=cuda), torch.tensor(G, device=cuda)
+    out = ir_drop_gs(ti, tG, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.5, False, 0.0, True, nl)
+    ref, _ = oracle.hook_gs(inp, G, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.5, False, 0.0, True, nl)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=rtol)
+    if iters > 0:
+        other = ir_drop_gs(ti, tG, bsize, row, col, numX, numY, iters, 1 / 2.93, 1.5, True, -1.0, True, nl)
+        assert torch.equal(out, other)
+
 def test_hook_reference_errors(cuda):
     import torch
     from fadesim_amd.hook import ir_drop_gs
