@@ -397,7 +397,12 @@ def main():
                       "mean_sweeps": float(sw.mean()), "ir2s_iters": k_ir2s,
                       "ir2s_omega": W if name == "ir2s" else 1.0, "kernel_ms": kernel_ms,
                       "guess_kernel_ms": guess_ms, "kernel_GBps_algorithmic": ach}
-    head = next(k for k in ("nn", "ir2s", "plain", "nn_3k", "nn_unet16") if k in meas)
+    # the NN-warm step reported as `value`: the faster of its two forms (the
+    # network inside the guess kernel, or separate features and network
+    # kernels; the same outputs bit for bit)
+    nn_forms = [k for k in ("nn", "nn_3k") if k in meas]
+    nn_head = max(nn_forms, key=lambda k: meas[k]["solves_per_s"]) if nn_forms else None
+    head = nn_head or next(k for k in ("ir2s", "plain", "nn_unet16") if k in meas)
     hm = meas[head]
     out = {"metric": "IR-drop solves/s at 1e-6 rel I_out err: NN-warm-start vs plain SOR, 1 GPU",
            "value": hm["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -414,17 +419,18 @@ def main():
            "warm_start": dict(meas["nn"], network=net.describe()) if "nn" in meas else None,
            "warm_start_three_kernels": meas.get("nn_3k"),
            "warm_start_unet16": dict(meas["nn_unet16"], network=net16.describe()) if "nn_unet16" in meas else None}
-    if "plain" in meas and "nn" in meas:
-        ps, ns = meas["plain"]["mean_sweeps"], meas["nn"]["mean_sweeps"]
+    if "plain" in meas and nn_head:
+        nm = meas[nn_head]
+        ps, ns = meas["plain"]["mean_sweeps"], nm["mean_sweeps"]
         # SOR sweeps per solve, plain vs NN-warm (the NN-warm solve also runs one
         # wire-solve pass); the ratio is null when the NN-warm solve needs none
         out["sweeps_per_solve"] = {"plain": ps, "nn": ns}
         out["sweep_reduction"] = ps / ns if ns > 0 else None
         # passes over the crossbar: SOR sweeps, plus the IR2S iteration the
         # NN-warm solve runs (each costs about three sweeps of kernel time)
-        out["pass_reduction"] = ps / (ns + meas["nn"]["ir2s_iters"])
-        out["kernel_time_ratio"] = meas["plain"]["kernel_ms"] / meas["nn"]["kernel_ms"]
-        out["nn_vs_plain_speedup"] = meas["nn"]["solves_per_s"] / meas["plain"]["solves_per_s"]
+        out["pass_reduction"] = ps / (ns + nm["ir2s_iters"])
+        out["kernel_time_ratio"] = meas["plain"]["kernel_ms"] / nm["kernel_ms"]
+        out["nn_vs_plain_speedup"] = nm["solves_per_s"] / meas["plain"]["solves_per_s"]
     if "plain" in meas and "ir2s" in meas:
         out["ir2s_vs_plain_speedup"] = meas["ir2s"]["solves_per_s"] / meas["plain"]["solves_per_s"]
     # the dominant kernel of the reported step: guess_kernel when the step's
