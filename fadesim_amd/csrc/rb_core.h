@@ -665,16 +665,42 @@ struct Shared {
 // lanes of a thread row, sums along a column are LDS scans over the TY thread
 // rows (completed over the team for strips).  Accumulation is FP64.
 
-// Exclusive prefix over the TX lanes of a thread row, and the row total.
+// v of the lane `CTRL` selects (a DPP control, both 32-bit halves moved
+// alike), +0 where that lane lies outside the row of 16 or the destination
+// row is masked off by ROW_MASK.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Exclusive prefix over the TX lanes of a thread row, and the row total:
+// the additions of a shift-by-1, 2, 4, ... scan, by DPP row shifts for rows
+// of 16 lanes or more (a lane with nothing to its left adds +0), by lane
+// shuffles for narrower rows.
 template <int TX>
 __device__ __forceinline__ void row_scan(double x, int tx, double &excl, double &total)
 {
     double incl = x;
+    if constexpr (TX < 16) {
 #pragma unroll
-    for (int d = 1; d < TX; d <<= 1) {
-        const double y = __shfl_up(incl, (unsigned)d, TX);
-        if (tx >= d)
-            incl += y;
+        for (int d = 1; d < TX; d <<= 1) {
+            const double y = __shfl_up(incl, (unsigned)d, TX);
+            if (tx >= d)
+                incl += y;
+        }
+    } else {
+        incl += dpp_d<0x111>(incl); // row_shr:1
+        incl += dpp_d<0x112>(incl); // row_shr:2
+        incl += dpp_d<0x114>(incl); // row_shr:4
+        incl += dpp_d<0x118>(incl); // row_shr:8
+        if constexpr (TX >= 32)
+            incl += dpp_d<0x142, 0xa>(incl); // row_bcast:15
+        if constexpr (TX == 64)
+            incl += dpp_d<0x143, 0xc>(incl); // row_bcast:31
     }
     total = __shfl(incl, TX - 1, TX);
     excl = incl - x;
