@@ -236,7 +236,6 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
 // the end of the launch (and come back at the start of the next one in break
 // mode).
 constexpr int kLdsMax = 128;
-constexpr int kLdsAhead = 16;
 
 // NT threads: 128 (a lane per row or column of the walks) for linear cells;
 // 256 for sinh cells, whose per-sweep conductance update over every cell
@@ -246,6 +245,9 @@ __global__ void __launch_bounds__(NT) hook_lds_kernel(HookArgs<T> A, T *__restri
                                                       T *__restrict__ Zall)
 {
     constexpr int kLdsThreads = NT;
+    // steps fetched ahead (measured: 16 in FP32; 8 in FP64, whose walk also
+    // fetches the bitline field and holds twice the registers per value)
+    constexpr int kLdsAhead = sizeof(T) == 4 ? 16 : 8;
     constexpr bool DL = sizeof(T) == 4; // the bitline field in LDS too
     __shared__ T su[kLdsMax * (kLdsMax + 1)];
     __shared__ T sd[DL ? kLdsMax * (kLdsMax + 1) : 1];
