@@ -22,6 +22,7 @@
 // sweep first computes the next sweep's G from the current state.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -112,6 +113,7 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
     const HookUnit<T> U = hook_unit(A, xb);
     const int b = U.b, x = U.x, y = U.y;
     const int row = U.rows, col = U.cols, t = threadIdx.x;
+    const int nth = blockDim.x; // a multiple of 64 up to kHookThreads, sized to the unit's lines
     const size_t cells = (size_t)row * col;
     T *u = A.u + (size_t)xb * cells;
     T *d = A.d + (size_t)xb * cells;
@@ -126,14 +128,14 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
     const T gw = A.gw, be = A.be, omb = A.omb;
 
     if (A.first_sweep) {
-        for (size_t k = t; k < cells; k += kHookThreads) {
+        for (size_t k = t; k < cells; k += nth) {
             u[k] = in[k / col];
             d[k] = (T)0;
         }
         __syncthreads();
         if (A.nonlinear) {
             // G of sweeps 0 and 1 both come from the initial state
-            for (size_t k = t; k < cells; k += kHookThreads) {
+            for (size_t k = t; k < cells; k += nth) {
                 const int i = (int)(k / col), c = (int)(k % col);
                 const T gl = Glin(i, c);
                 T z = ((T)1 / gl - (T)A.nl[0]) / (T)(A.nl[1] - A.nl[0]);
@@ -148,7 +150,7 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
     for (int s = 0; s < A.sweeps_this_launch; ++s) {
         if (A.nonlinear && !(A.first_sweep && s == 0)) {
             // Gc <- Gn (computed last sweep); Gn <- G of the current state
-            for (size_t k = t; k < cells; k += kHookThreads) {
+            for (size_t k = t; k < cells; k += nth) {
                 const int i = (int)(k / col), c = (int)(k % col);
                 const T gl = Glin(i, c);
                 T z = ((T)1 / gl - (T)A.nl[0]) / (T)(A.nl[1] - A.nl[0]);
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
         }
         auto Gof = [&](int i, int c) -> T { return A.nonlinear ? Gc[(size_t)i * col + c] : Glin(i, c); };
         // wordlines, left -> right (split_arr_matmul.cu:75-119)
-        for (int i = t; i < row; i += kHookThreads) {
+        for (int i = t; i < row; i += nth) {
             T *ur = u + (size_t)i * col;
             const T *dr = d + (size_t)i * col;
             T left = in[i];
@@ -191,7 +193,7 @@ __global__ void __launch_bounds__(kHookThreads) hook_kernel(HookArgs<T> A)
         }
         __syncthreads();
         // bitlines, bottom -> top (split_arr_matmul.cu:27-73)
-        for (int c = t; c < col; c += kHookThreads) {
+        for (int c = t; c < col; c += nth) {
             int i = row - 1;
             T old = d[(size_t)i * col + c];
             T Gi = Gof(i, c);
@@ -560,6 +562,10 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     T *GT = lds ? A.u + (D->nonlinear ? 4 : 2) * nfield : nullptr;
     T *NTp = lds && D->nonlinear ? GT + nfield : nullptr; // two planes
     T *Z = lds && D->nonlinear ? GT + 3 * nfield : nullptr;
+    // the workspace kernel: a thread per line of the unit, in whole waves
+    // (small OU blocks leave room for more workgroups per CU)
+    const int unit_lines = ou ? std::max(D->ou_rows, D->ou_cols) : std::max(D->row_size, D->col_size);
+    const int hook_threads = std::min(kHookThreads, std::max(64, (unit_lines + 63) / 64 * 64));
     // one launch of `sweeps` sweeps
     auto sweep_launch = [&]() {
         if (lds && A.nonlinear)
@@ -567,7 +573,7 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
         else if (lds)
             hipLaunchKernelGGL((hook_lds_kernel<T, 128, false>), dim3(grid), dim3(128), 0, s, A, GT, NTp, Z);
         else
-            hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(kHookThreads), 0, s, A);
+            hipLaunchKernelGGL((hook_kernel<T>), dim3(grid), dim3(hook_threads), 0, s, A);
     };
     int done = 0;
     if (!D->enable_break) {
