@@ -31,6 +31,7 @@
 // one's G from the present state.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -132,6 +133,7 @@ __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
     const int xb = blockIdx.x;
     const Cross<T> X = cross_of(A, xb);
     const int row = X.rows, col = X.cols, t = threadIdx.x;
+    const int nth = blockDim.x; // a multiple of 64 up to kIr2sThreads, sized to the crossbar's lines
     const size_t cells = (size_t)row * col;
     T *u = A.u + (size_t)xb * cells, *d = A.d + (size_t)xb * cells, *cur = A.cur + (size_t)xb * cells;
     T *wres = A.wres + (size_t)xb * row, *bres = A.bres + (size_t)xb * col;
@@ -146,17 +148,17 @@ __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
     __shared__ unsigned s_moved;
 
     if (A.first) {
-        for (size_t k = t; k < cells; k += kIr2sThreads) {
+        for (size_t k = t; k < cells; k += nth) {
             u[k] = X.in[k / col];
             d[k] = (T)0;
         }
-        for (int i = t; i < row; i += kIr2sThreads)
+        for (int i = t; i < row; i += nth)
             wres[i] = (T)0;
-        for (int c = t; c < col; c += kIr2sThreads)
+        for (int c = t; c < col; c += nth)
             bres[c] = (T)0;
         __syncthreads();
         if (A.nonlinear) {
-            for (size_t k = t; k < cells; k += kIr2sThreads)
+            for (size_t k = t; k < cells; k += nth)
                 Gc[k] = Gn[k] = cell_g<T>(u[k] - d[k], zof(k), A.nl);
             __syncthreads();
         }
@@ -170,18 +172,18 @@ __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
     __syncthreads();
     for (int it = 0; it < A.iters_this_launch; ++it) {
         if (A.nonlinear && !(A.first && it == 0)) {
-            for (size_t k = t; k < cells; k += kIr2sThreads) {
+            for (size_t k = t; k < cells; k += nth) {
                 Gc[k] = Gn[k];
                 Gn[k] = cell_g<T>(u[k] - d[k], zof(k), A.nl);
             }
             __syncthreads();
         }
-        for (size_t k = t; k < cells; k += kIr2sThreads)
+        for (size_t k = t; k < cells; k += nth)
             cur[k] = (u[k] - d[k]) * (A.nonlinear ? Gc[k] : Glin(k));
         __syncthreads();
         T dmax = (T)0;
         // wordlines: segment j carries the currents of columns >= j
-        for (int i = t; i < row; i += kIr2sThreads) {
+        for (int i = t; i < row; i += nth) {
             const T *ci = cur + (size_t)i * col;
             T *ui = u + (size_t)i * col;
             T s = A.carry_residue ? wres[i] : (T)0;
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(kIr2sThreads) ir2s_kernel(Ir2sArgs<T> A)
                 wres[i] = s;
         }
         // bitlines: the segment below row i carries the currents of rows <= i
-        for (int c = t; c < col; c += kIr2sThreads) {
+        for (int c = t; c < col; c += nth) {
             T s = A.carry_residue ? bres[c] : (T)0;
             for (int i = 0; i < row; ++i)
                 s += cur[(size_t)i * col + c];
@@ -290,6 +292,11 @@ int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
     const bool ou = D->ou_rows > 0;
     const int nOuX = ou ? D->row_size / D->ou_rows : 1, nOuY = ou ? D->col_size / D->ou_cols : 1;
     const int grid = D->batch_size * D->numX * D->numY * nOuX * nOuY;
+    // OU blocks: a thread per line of the block, in whole waves (more
+    // workgroups per CU); whole tiles: kIr2sThreads, whose per-cell passes
+    // gain from every thread (measured)
+    const int ulines = std::max(D->ou_rows, D->ou_cols);
+    const int nthreads = ou ? std::min(kIr2sThreads, std::max(64, (ulines + 63) / 64 * 64)) : kIr2sThreads;
     const int urows = ou ? D->ou_rows : D->row_size, ucols = ou ? D->ou_cols : D->col_size;
     const size_t cells = (size_t)urows * ucols;
     const size_t nfield = (size_t)grid * cells;
@@ -337,7 +344,7 @@ int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
     if (D->enable_break != FADESIM_IR2S_BREAK_BATCH) {
         A.iters_this_launch = D->iter_times;
         A.first = 1;
-        hipLaunchKernelGGL((ir2s_kernel<T>), dim3(grid), dim3(kIr2sThreads), 0, s, A);
+        hipLaunchKernelGGL((ir2s_kernel<T>), dim3(grid), dim3(nthreads), 0, s, A);
         FADESIM_HIP_CHECK(hipGetLastError());
         done = D->iter_times;
     } else {
@@ -347,7 +354,7 @@ int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
         for (int it = 0; it < D->iter_times; ++it) {
             A.first = (it == 0);
             hipLaunchKernelGGL(ir2s_set_u32, dim3(1), dim3(1), 0, s, A.moved, 0u);
-            hipLaunchKernelGGL((ir2s_kernel<T>), dim3(grid), dim3(kIr2sThreads), 0, s, A);
+            hipLaunchKernelGGL((ir2s_kernel<T>), dim3(grid), dim3(nthreads), 0, s, A);
             FADESIM_HIP_CHECK(hipGetLastError());
             unsigned h = 0;
             FADESIM_HIP_CHECK(hipMemcpyAsync(&h, A.moved, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -360,7 +367,7 @@ int run_ir2s(const fadesim_ir2s_desc *D, hipStream_t s)
     if (done == 0) {
         A.iters_this_launch = 0;
         A.first = 1;
-        hipLaunchKernelGGL((ir2s_kernel<T>), dim3(grid), dim3(kIr2sThreads), 0, s, A);
+        hipLaunchKernelGGL((ir2s_kernel<T>), dim3(grid), dim3(nthreads), 0, s, A);
         FADESIM_HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL((ir2s_readout<T>), dim3(grid), dim3(128), 0, s, A, (T *)D->d_out, D->ret_v, (T *)D->d_vup,
