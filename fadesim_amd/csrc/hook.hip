@@ -562,10 +562,11 @@ int run_hook(const fadesim_hook_desc *D, hipStream_t s)
     T *GT = lds ? A.u + (D->nonlinear ? 4 : 2) * nfield : nullptr;
     T *NTp = lds && D->nonlinear ? GT + nfield : nullptr; // two planes
     T *Z = lds && D->nonlinear ? GT + 3 * nfield : nullptr;
-    // the workspace kernel: a thread per line of the unit, in whole waves
-    // (small OU blocks leave room for more workgroups per CU)
-    const int unit_lines = ou ? std::max(D->ou_rows, D->ou_cols) : std::max(D->row_size, D->col_size);
-    const int hook_threads = std::min(kHookThreads, std::max(64, (unit_lines + 63) / 64 * 64));
+    // the workspace kernel: OU blocks a thread per line of the block, in
+    // whole waves (more workgroups per CU); whole tiles kHookThreads, whose
+    // per-cell passes (sinh cells) gain from every thread
+    const int unit_lines = std::max(D->ou_rows, D->ou_cols);
+    const int hook_threads = ou ? std::min(kHookThreads, std::max(64, (unit_lines + 63) / 64 * 64)) : kHookThreads;
     // one launch of `sweeps` sweeps
     auto sweep_launch = [&]() {
         if (lds && A.nonlinear)
